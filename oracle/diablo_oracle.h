@@ -1,0 +1,90 @@
+/*
+ * diablo_oracle.h -- CPU FP64 restatement of DIABLO 2.0's channel time-stepper.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This oracle is the checker that the HIP product
+ * path is compared against.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it.  The product (diablo_amd/, libdiablo_gpu.so)
+ * never links, imports or falls back to it.
+ *
+ * Layout follows the reference exactly (for/header:214-263): every field is one
+ * REAL*8 array (0:NX+1, 0:NZ+1, 0:NY+1), column-major (i fastest, then k, then
+ * j), and its Fourier view is COMPLEX*16 (0:NX/2, 0:NZ+1, 0:NY+1) over the same
+ * bytes (the EQUIVALENCE at for/header:247-257).  The configuration is the MPI
+ * code path with NPROCS = NPROCY = NPROCZ = 1 (for/mpi.f:868-1135).
+ */
+#ifndef DIABLO_ORACLE_H
+#define DIABLO_ORACLE_H
+
+#include <stdint.h>
+
+#define DOR_MAX_TH 4
+
+/* Runtime parameters: the subset of /INPUT/, /SCALAR_VARS/, /GEOPHYS_VARS/,
+ * /OSCILL_VARS/, /VARIABLE_VISC/ and /DRHODX_VARS/ (for/header:83-301) that the
+ * channel path reads. */
+typedef struct dor_params {
+    int32_t nx, ny, nz, n_th;
+    int32_t f_type;
+    int32_t u_bc_ymin, v_bc_ymin, w_bc_ymin;
+    int32_t u_bc_ymax, v_bc_ymax, w_bc_ymax;
+    int32_t th_bc_ymin[DOR_MAX_TH], th_bc_ymax[DOR_MAX_TH];
+    double nu, beta, lx, ly, lz, nu_v_scale, delta_t;
+    double i_ro, grav_x, grav_y, grav_z;
+    double px0, omega0, amp_omega0;
+    double u_bc_ymin_c1, v_bc_ymin_c1, w_bc_ymin_c1;
+    double u_bc_ymax_c1, v_bc_ymax_c1, w_bc_ymax_c1;
+    double th_bc_ymin_c1[DOR_MAX_TH], th_bc_ymax_c1[DOR_MAX_TH];
+    double ri[DOR_MAX_TH], pr[DOR_MAX_TH];
+    double drhodx[DOR_MAX_TH], drhodz[DOR_MAX_TH];
+} dor_params;
+
+/* Field identifiers for dor_field(). */
+enum {
+    DOR_U1 = 0, DOR_U2, DOR_U3, DOR_P, DOR_R1, DOR_R2, DOR_R3,
+    DOR_F1, DOR_F2, DOR_F3, DOR_S1,
+    DOR_TH0 = 16,      /* TH(:,:,:,n)   = DOR_TH0   + n (0-based n) */
+    DOR_FTH0 = 24,     /* FTH, unused by the channel path but kept   */
+    DOR_RTH0 = 32,     /* RTH(:,:,:,n)  */
+    DOR_CTH0 = 40,     /* CTH  (Fourier, not aliased with TH)        */
+    DOR_CFTH0 = 48,    /* CFTH */
+    DOR_CRTH0 = 56     /* CRTH */
+};
+
+/* Grid / wavenumber vector identifiers for dor_vec(). */
+enum {
+    DOR_GY = 0, DOR_GYF, DOR_DY, DOR_DYF, DOR_KX, DOR_KZ, DOR_KX2, DOR_KZ2,
+    DOR_GX, DOR_GZ
+};
+
+typedef struct dor_state dor_state;
+
+/* Creates the state: INIT_CHAN_MPI index ranges, grid metrics from GY(1..NY+1)
+ * (ReadGridHDF5 + CREATE_GRID_CHAN), INIT_FFT wavenumbers and the RKW3 table.
+ * gy_1 holds NY+1 values GY(1..NY+1). Returns NULL on bad sizes. */
+dor_state *dor_create(const dor_params *p, const double *gy_1);
+void dor_destroy(dor_state *s);
+
+double *dor_field(dor_state *s, int id);
+double *dor_vec(dor_state *s, int id);
+int dor_info(const dor_state *s, int what);  /* 0 jstart 1 jend 2 nkx 3 nkz 4 tnkz 5 nxp 10+n jstart_th 20+n jend_th */
+void dor_set_time(dor_state *s, double time);
+void dor_set_dt(dor_state *s, double dt);
+void dor_set_threads(int n);
+
+/* The reference call surface (for/mpi.f:1530-1674, for/channel.f:62-1453). */
+void dor_fft_xz_to_fourier(dor_state *s, double *v, int jmin, int jmax);
+void dor_fft_xz_to_physical(dor_state *s, double *vv, int jmin, int jmax);
+void dor_rk_chan_1(dor_state *s, int rk_step);
+void dor_rem_div_chan(dor_state *s);
+void dor_poisson_p_chan(dor_state *s);
+void dor_apply_bc_vel(dor_state *s);
+void dor_apply_bc_scalar(dor_state *s);
+
+/* Building blocks exposed for unit tests. */
+void dor_thomas_real(double *a, double *b, double *c, double *g, int ny, int nx);
+void dor_thomas_complex(double *a, double *b, double *c, double *g_c, int ny, int nxp);
+void dor_fft_c2c(double *data, int n, int sign);   /* unnormalised, sign -1 forward */
+void dor_r2c(const double *x, double *X, int n);    /* X: n/2+1 complex           */
+void dor_c2r(const double *X, double *x, int n);    /* FFTW-style C2R, unnormalised */
+
+#endif
