@@ -1,0 +1,170 @@
+#!/usr/bin/env python3
+"""Throughput of the channel time-stepper on one MI355X.
+
+A step is one RK3 time step (three RK_CHAN_1 substeps) of the full device-
+resident state.  The default workload is the 384 x 257 x 384 channel at
+Re = 395 (tanh grid, CS = 1.75, dt = 5e-4, seeded perturbation 0.05 plus the
+streak seed), FP64.  Inputs are resident in HBM before the timed region.
+
+Prints one JSON line:
+  value        Mpts * RK-substeps / s (NX*NY*NZ*3*steps / seconds / 1e6)
+  roofline     dominant kernel: algorithmic bytes per launch / mean HIP-event
+               duration of that kernel over the timed region, against 8 TB/s
+  cpu_baseline the oracle (C restatement of the reference path, OpenMP) on a
+               bounded sample of the same workload on the host cores
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def kernel_bytes(cfg, jlo: int, s: int = 8) -> dict:
+    """Algorithmic HBM bytes per launch of each kernel (inputs read once,
+    outputs written once), s bytes per real."""
+    nin = 3 + cfg.n_th
+    p2 = cfg.ny + 2
+    mk, mz = cfg.nkx + 1, cfg.tnkz + 1
+    ps = mk * mz
+    zrow = cfg.nz * mk
+    pl = cfg.ny - jlo + 1
+    c = 2 * s
+    return {
+        "zinv": c * (nin * p2 * ps + nin * p2 * zrow),
+        "xfused": c * (nin * (pl + 2) * zrow + 2 * nin * pl * zrow),
+        "zfwd_nl": c * (2 * nin * pl * zrow + nin * pl * ps),
+        "columns": c * ((10 + 3 * cfg.n_th) + (7 + 2 * cfg.n_th)) * p2 * ps,
+    }
+
+
+def substep_model_bytes(cfg, s: int = 8) -> float:
+    """Stage-level byte model per substep: s*[2(3+N)P + (23+7N)F]."""
+    P = cfg.nx * cfg.ny * cfg.nz
+    F = 2 * (cfg.nkx + 1) * (2 * cfg.nkz + 1) * cfg.ny
+    return s * (2 * (3 + cfg.n_th) * P + (23 + 7 * cfg.n_th) * F)
+
+
+def cpu_baseline(cfg, phys, threads: int) -> dict:
+    from oracle import oracle
+    o = oracle.Oracle(cfg, threads=threads)
+    o.initialize(*phys)
+    t0 = time.perf_counter()
+    o.step()
+    dt = time.perf_counter() - t0
+    return {"value": cfg.points * 3 / dt / 1e6, "unit": "Mpts*substeps/s", "cores": threads,
+            "kind": "port",
+            "sample": f"1 RK3 step (3 substeps) of the same {cfg.nx}x{cfg.ny}x{cfg.nz} case from the same "
+                      f"initial state, oracle (C restatement of RK_CHAN_1, physical-space CN as in the "
+                      f"reference) with {threads} OpenMP threads, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="channel384")
+    ap.add_argument("--cpu-baseline", choices=("on", "off"), default="on")
+    ap.add_argument("--no-kernel-timing", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one MI355X: this build runs the channel case on a single GPU "
+              "(no sharding, no collectives); --gpus must be 1", file=sys.stderr)
+        sys.exit(2)
+
+    from diablo_amd import case, gpu
+
+    cfg = case.WORKLOADS[args.workload]()
+    log(f"[bench] workload {args.workload}: {cfg.nx}x{cfg.ny}x{cfg.nz}, building initial state")
+    phys = case.initial_physical(cfg)
+    g = gpu.GpuChannel(cfg)
+    g.initialize(*phys)
+    log("[bench] initial state on device; warmup")
+    g.step(args.warmup)
+    g.synchronize()
+    if not args.no_kernel_timing:
+        g.set_kernel_timing(True)
+        g.kernel_times()  # reset
+    log(f"[bench] timing {args.steps} steps")
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.step(args.steps)
+    g.synchronize()
+    elapsed = time.perf_counter() - t0
+    kt = g.kernel_times() if not args.no_kernel_timing else {}
+    g.set_kernel_timing(False)
+    substeps = 3 * args.steps
+    value = cfg.points * substeps / elapsed / 1e6
+    # sanity: the state must stay finite
+    cu1 = g.get_field(gpu.CU1)
+    if not math.isfinite(float(abs(cu1).max())):
+        raise RuntimeError("state diverged")
+
+    jlo = min([2] + [1 if cfg.th_bc_ymin[n] == 1 else 2 for n in range(cfg.n_th)])
+    kb = kernel_bytes(cfg, jlo)
+    roof = None
+    if kt:
+        per = {k: (ms / n if n else float("nan"), n) for k, (ms, n) in kt.items()}
+        for k, (avg, n) in per.items():
+            log(f"[bench] kernel {k:8s} launches {n:5d} mean {avg:8.3f} ms  "
+                f"{kb[k] / (avg * 1e-3) / 1e9:8.1f} GB/s algorithmic")
+        dom = max(kt, key=lambda k: kt[k][0])
+        avg_s = per[dom][0] * 1e-3
+        achieved = kb[dom] / avg_s / 1e9
+        traffic = None
+        tfile = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
+        if os.path.exists(tfile):
+            traffic = json.load(open(tfile)).get(dom)
+        roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "bytes_per_launch": kb[dom], "ms_per_launch": round(per[dom][0], 4)}
+    model = substep_model_bytes(cfg)
+    log(f"[bench] {value:.1f} Mpts*substeps/s, {elapsed / args.steps * 1e3:.2f} ms/step; stage model "
+        f"{model * substeps / elapsed / 1e9:.0f} GB/s ({model * substeps / elapsed / 1e12 / 8 * 100:.1f}% of 8 TB/s)")
+
+    cpu = None
+    if args.cpu_baseline == "on":
+        threads = min(16, os.cpu_count() or 1)
+        log(f"[bench] cpu baseline: oracle, 1 step, {threads} threads")
+        cpu = cpu_baseline(cfg, phys, threads)
+        log(f"[bench] cpu baseline {cpu['value']:.3f} Mpts*substeps/s")
+
+    out = {
+        "metric": "Mpts·RK-substeps/s FP64 vs mixed+batched at rel-L2≤1e-5, 1 MI355X",
+        "value": round(value, 2),
+        "unit": "Mpts*substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded laminar profile + perturbations, tanh y-grid)",
+        "config": {"workload": f"{args.workload}: channel {cfg.nx}x{cfg.ny}x{cfg.nz}, Re={1 / cfg.nu:.0f}, "
+                               f"dt={cfg.delta_t}, one passive scalar, FP64 RK3/CN",
+                   "nx": cfg.nx, "ny": cfg.ny, "nz": cfg.nz, "n_th": cfg.n_th, "precision": "fp64",
+                   "substeps_per_step": 3, "stage_model_bytes_per_substep": model},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

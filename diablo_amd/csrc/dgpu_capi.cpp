@@ -1,0 +1,741 @@
+// dgpu_capi.cpp -- host side of libdiablo_gpu.so: the extern "C" entry points
+// declared in include/diablo_gpu.h, device memory, FFT plans and the
+// Crank-Nicolson solve tables.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/diablo_gpu.h"
+#include "dgpu_kernels.hpp"
+
+using dgpu::cx;
+
+namespace {
+
+constexpr int kKinds = 4;  // timed kernel kinds: zinv, xfused, zfwd_nl, columns
+
+struct TimedLaunch {
+    int kind;
+    hipEvent_t a, b;
+};
+
+}  // namespace
+
+extern "C" {
+struct dgpu_ctx {
+    dgpu_params p;
+    dgpu::Geo g;
+    int jstart, jend, jstart_th[DGPU_MAX_TH], jend_th[DGPU_MAX_TH];
+    std::vector<double> gy, gyf, dy, dyf, kx, kz, kx2, kz2;
+    double *d_vec = nullptr;  // dy, dyf, kx, kz, kx2, kz2 (device)
+    double *d_dy = nullptr, *d_dyf = nullptr, *d_kx = nullptr, *d_kz = nullptr, *d_kx2 = nullptr, *d_kz2 = nullptr;
+    dgpu::Fft<double> fx, fz;
+    cx<double> *d_tw = nullptr;
+    // Fourier state, one allocation: [slot][j][K][I]
+    cx<double> *d_state = nullptr;
+    long fs = 0;  // complex elements per field
+    int nslot = 0;
+    cx<double> *cu[3] = {nullptr, nullptr, nullptr};
+    cx<double> *cp = nullptr;
+    cx<double> *cth[DGPU_MAX_TH] = {};
+    cx<double> *cf[3] = {nullptr, nullptr, nullptr};
+    cx<double> *cfth[DGPU_MAX_TH] = {};
+    cx<double> *nl[3 + DGPU_MAX_TH] = {};
+    cx<double> *pois = nullptr;
+    double *d_piv = nullptr;
+    cx<double> **d_nlptr = nullptr;
+    // transform intermediates
+    cx<double> *d_zt = nullptr, *d_wt = nullptr;
+    long zfs = 0;  // complex elements per intermediate field: (NY+2)*NZ*mk
+    double *d_fort = nullptr;  // Fortran-layout staging for host entry points
+    // CN tables: [stage 0..2][sys][5][NY+2]
+    double *d_cn = nullptr;
+    std::vector<double> h_cn;
+    double cn_temp1[3] = {NAN, NAN, NAN};
+    int *d_codes = nullptr;
+    double *d_c1 = nullptr;
+    hipStream_t stream = nullptr;
+    std::string err;
+    bool timing = false;
+    std::vector<TimedLaunch> pending;
+    std::vector<hipEvent_t> event_pool;
+    double t_ms[kKinds] = {0, 0, 0, 0};
+    long long t_n[kKinds] = {0, 0, 0, 0};
+};
+}
+
+namespace {
+
+int fail(dgpu_ctx *c, int code, const std::string &msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(c, expr)                                                                             \
+    do {                                                                                            \
+        hipError_t e_ = (expr);                                                                     \
+        if (e_ != hipSuccess)                                                                       \
+            return fail((c), DGPU_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));        \
+    } while (0)
+
+bool make_len(int n, dgpu::FftLen &f)
+{
+    f.n = n;
+    f.nstages = 0;
+    int m = n;
+    bool has2 = false, has3 = false, has4 = false;
+    while (m % 4 == 0 && f.nstages < dgpu::kMaxStages) { f.radix[f.nstages++] = 4; m /= 4; has4 = true; }
+    while (m % 2 == 0 && f.nstages < dgpu::kMaxStages) { f.radix[f.nstages++] = 2; m /= 2; has2 = true; }
+    while (m % 3 == 0 && f.nstages < dgpu::kMaxStages) { f.radix[f.nstages++] = 3; m /= 3; has3 = true; }
+    if (m != 1 || n < 4) return false;
+    int l = has4 ? 4 : (has2 ? 2 : 1);
+    if (has3) l = (l == 4) ? 12 : (l == 2 ? 6 : 3);
+    const int cand[5] = {4, 6, 8, 12, 16};
+    int best = -1;
+    for (int e : cand) {
+        if (e % l || n % e) continue;
+        if (best < 0) best = e;
+        if (n / e <= 64) { best = e; break; }
+        best = e;
+    }
+    if (best < 0) return false;
+    f.e = best;
+    f.t = n / best;
+    return true;
+}
+
+void twiddles(int n, std::vector<cx<double>> &out)
+{
+    const long double twopi = 6.283185307179586476925286766559005768L;
+    out.resize(n);
+    for (int j = 0; j < n; ++j) {
+        long double a = -twopi * (long double)j / (long double)n;
+        out[j].re = (double)cosl(a);
+        out[j].im = (double)sinl(a);
+    }
+}
+
+// Matrices of one CN system exactly as RK_CHAN_1 + APPLY_BC_*_MPI build them
+// (for/channel.f:771-1029, 1660-2282), then THOMAS_FORWARD_REAL_MPI's
+// elimination (for/mpi.f:673-699) of the matrix part.
+void build_cn(const dgpu_ctx *c, int sys, double temp1, double *mult, double *piv, double *up, double *bc00,
+              double *isbc, int &lo, int &hi)
+{
+    const int NY = c->g.ny;
+    const double NUV = c->p.nu_v_scale;
+    const double *DY = c->dy.data(), *DYF = c->dyf.data();
+    std::vector<double> L(NY + 2, 0.0), D(NY + 2, 1.0), U(NY + 2, 0.0);
+    for (int j = 0; j <= NY + 1; ++j) { bc00[j] = 0.0; isbc[j] = 1.0; }
+    auto setrow = [&](int j, double l, double d, double u, double v) {
+        L[j] = l; D[j] = d; U[j] = u; bc00[j] = v; isbc[j] = 1.0;
+    };
+    if (sys == dgpu::SYS_U1 || sys == dgpu::SYS_U3) {
+        for (int j = c->jstart; j <= c->jend; ++j) {
+            L[j] = -temp1 * NUV / (DY[j] * DYF[j]);
+            D[j] = 1. - temp1 * NUV * (-1. / (DY[j + 1] * DYF[j]) - 1. / (DY[j] * DYF[j]));
+            U[j] = -temp1 * NUV / (DY[j + 1] * DYF[j]);
+            isbc[j] = 0.0;
+        }
+        const bool isu = (sys == dgpu::SYS_U1);
+        const int cmin = isu ? c->p.u_bc_ymin : c->p.w_bc_ymin;
+        const int cmax = isu ? c->p.u_bc_ymax : c->p.w_bc_ymax;
+        const double vmin = isu ? c->p.u_bc_ymin_c1 : c->p.w_bc_ymin_c1;
+        const double vmax = isu ? c->p.u_bc_ymax_c1 : c->p.w_bc_ymax_c1;
+        setrow(0, 0., 1., 0., 0.);
+        if (cmin == 0) setrow(1, 0., 1., 0., vmin);
+        else setrow(1, 0., -1., 1., DY[2] * vmin);
+        if (cmax == 0) {
+            setrow(NY + 1, 0., 1., 0., 0.);
+            setrow(NY, 0., 1., 0., vmax);
+        } else {
+            setrow(NY, -1., 1., 0., DY[NY] * vmax);
+            setrow(NY + 1, 0., 1., 0., 0.);
+        }
+        lo = c->jstart - 1;
+        hi = c->jend + 1;
+    } else if (sys == dgpu::SYS_U2) {
+        for (int j = 2; j <= NY; ++j) {
+            L[j] = -temp1 * NUV / (DYF[j - 1] * DY[j]);
+            D[j] = 1. + temp1 * NUV / (DYF[j] * DY[j]) + temp1 * NUV / (DYF[j - 1] * DY[j]);
+            U[j] = -temp1 * NUV / (DYF[j] * DY[j]);
+            isbc[j] = 0.0;
+        }
+        if (c->p.v_bc_ymin == 0) {
+            setrow(1, 0., 1., 0., c->p.v_bc_ymin_c1);
+            setrow(2, 0., 1., 0., c->p.v_bc_ymin_c1);
+        } else if (c->p.v_bc_ymin == 1) {
+            setrow(1, 0., -1., 1., DYF[1] * c->p.v_bc_ymin_c1);
+        }
+        setrow(0, 0., 1., 0., 0.);
+        if (c->p.v_bc_ymax == 0) {
+            setrow(NY + 1, 0., 1., 0., c->p.v_bc_ymax_c1);
+            setrow(NY, 0., 1., 0., c->p.v_bc_ymax_c1);
+        } else if (c->p.v_bc_ymax == 1) {
+            setrow(NY + 1, -1., 1., 0., DYF[NY] * c->p.v_bc_ymax_c1);
+        }
+        lo = 1;
+        hi = NY + 1;
+    } else {
+        const int n = sys - dgpu::SYS_TH0;
+        const double cc = temp1 / c->p.pr[n] * NUV;
+        for (int j = c->jstart_th[n]; j <= c->jend_th[n]; ++j) {
+            L[j] = -(cc) / (DY[j] * DYF[j]);
+            D[j] = 1. + (cc) / (DY[j + 1] * DYF[j]) + (cc) / (DY[j] * DYF[j]);
+            U[j] = -(cc) / (DY[j + 1] * DYF[j]);
+            isbc[j] = 0.0;
+        }
+        const double vmin = c->p.th_bc_ymin_c1[n], vmax = c->p.th_bc_ymax_c1[n];
+        if (c->p.th_bc_ymin[n] == 0) {
+            setrow(0, 0., 1., 0., 0.);
+            setrow(1, 0., 1., 0., vmin);
+        } else {
+            setrow(1, 0., -1., 1., DY[2] * vmin);
+            setrow(0, 0., 1., 0., 0.);
+        }
+        if (c->p.th_bc_ymax[n] == 0) {
+            setrow(NY + 1, 0., 1., 0., 0.);
+            setrow(NY, 0., 1., 0., vmax);
+        } else {
+            setrow(NY, -1., 1., 0., DY[NY] * vmax);
+            setrow(NY + 1, 0., 1., 0., 0.);
+        }
+        lo = c->jstart_th[n] - 1;
+        hi = c->jend_th[n] + 1;
+    }
+    // forward elimination of the matrix (rows 1..NY+1)
+    std::vector<double> A = L, B = D;
+    for (int j = 1; j <= NY + 1; ++j) {
+        A[j] = -A[j] / B[j - 1];
+        B[j] = B[j] + A[j] * U[j - 1];
+    }
+    for (int j = 0; j <= NY + 1; ++j) {
+        mult[j] = A[j];
+        piv[j] = B[j];
+        up[j] = U[j];
+    }
+    mult[0] = 0.0;
+}
+
+int nsys_of(const dgpu_ctx *c) { return 3 + c->g.nth; }
+
+long cn_block(const dgpu_ctx *c) { return 5L * (c->g.ny + 2); }
+
+int ensure_cn(dgpu_ctx *c, int stage, double temp1)
+{
+    if (c->cn_temp1[stage] == temp1) return 0;
+    const int nsys = nsys_of(c);
+    const long blk = cn_block(c);
+    double *base = c->h_cn.data() + (long)stage * nsys * blk;
+    for (int q = 0; q < nsys; ++q) {
+        double *t = base + q * blk;
+        const int ny2 = c->g.ny + 2;
+        int lo, hi;
+        build_cn(c, q, temp1, t, t + ny2, t + 2 * ny2, t + 3 * ny2, t + 4 * ny2, lo, hi);
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(c->d_cn + (long)stage * nsys * blk, base, sizeof(double) * nsys * blk, hipMemcpyHostToDevice));
+    c->cn_temp1[stage] = temp1;
+    return 0;
+}
+
+hipEvent_t get_event(dgpu_ctx *c)
+{
+    if (!c->event_pool.empty()) {
+        hipEvent_t e = c->event_pool.back();
+        c->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+struct TimerScope {
+    dgpu_ctx *c;
+    int kind;
+    hipEvent_t a = nullptr;
+    TimerScope(dgpu_ctx *c_, int k) : c(c_), kind(k)
+    {
+        if (c->timing) {
+            a = get_event(c);
+            if (a) hipEventRecord(a, c->stream);
+        }
+    }
+    ~TimerScope()
+    {
+        if (c->timing && a) {
+            hipEvent_t b = get_event(c);
+            if (b) {
+                hipEventRecord(b, c->stream);
+                c->pending.push_back({kind, a, b});
+            }
+        }
+    }
+};
+
+int check_launch(dgpu_ctx *c, const char *what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(c, DGPU_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    return 0;
+}
+
+int ensure_fort(dgpu_ctx *c)
+{
+    if (c->d_fort) return 0;
+    const size_t n = (size_t)(c->g.nx + 2) * (c->g.nz + 2) * (c->g.ny + 2);
+    HIPCHK(c, hipMalloc(&c->d_fort, n * sizeof(double)));
+    return 0;
+}
+
+cx<double> *field_ptr(dgpu_ctx *c, int field)
+{
+    if (field >= DGPU_CU1 && field <= DGPU_CU3) return c->cu[field - DGPU_CU1];
+    if (field == DGPU_CP) return c->cp;
+    if (field >= DGPU_CF1 && field <= DGPU_CF3) return c->cf[field - DGPU_CF1];
+    if (field >= DGPU_CTH0 && field < DGPU_CTH0 + c->g.nth) return c->cth[field - DGPU_CTH0];
+    if (field >= DGPU_CFTH0 && field < DGPU_CFTH0 + c->g.nth) return c->cfth[field - DGPU_CFTH0];
+    return nullptr;
+}
+
+__global__ void k_pack_fortran(const cx<double> *__restrict__ kept, double *__restrict__ fort, int nxp, int nz,
+                               int mk, int mz)
+{
+    const int j = blockIdx.y;
+    const long plane = (long)(nxp + 1) * (nz + 2);
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < plane; e += (long)gridDim.x * blockDim.x) {
+        const int K = (int)(e / (nxp + 1)), i = (int)(e - (long)K * (nxp + 1));
+        cx<double> v;
+        v.re = 0.0;
+        v.im = 0.0;
+        if (i < mk && K < mz) v = kept[(long)j * mk * mz + (long)K * mk + i];
+        double *p = fort + ((long)j * plane + e) * 2;
+        p[0] = v.re;
+        p[1] = v.im;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t dgpu_abi_version(void) { return DGPU_ABI_VERSION; }
+
+const char *dgpu_build_info(void)
+{
+    return "libdiablo_gpu: DIABLO channel time-stepper, HIP kernels for gfx950 (FP64 path)";
+}
+
+const char *dgpu_last_error(const dgpu_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int32_t dgpu_create(const dgpu_params *p, dgpu_ctx **out)
+{
+    if (!p || !out) return DGPU_E_ARG;
+    *out = nullptr;
+    if (p->nx < 4 || p->nz < 4 || p->ny < 4 || (p->nx & 1) || (p->nz & 1)) return DGPU_E_ARG;
+    if (p->n_th < 0 || p->n_th > 1) return DGPU_E_ARG;  // one active scalar supported by the fused kernels
+    if (p->precision != DGPU_FP64) return DGPU_E_ARG;
+    if (!p->gy) return DGPU_E_ARG;
+    for (int code : {p->u_bc_ymin, p->w_bc_ymin, p->u_bc_ymax, p->w_bc_ymax})
+        if (code != 0 && code != 1) return DGPU_E_ARG;
+    for (int code : {p->v_bc_ymin, p->v_bc_ymax})
+        if (code < 0 || code > 2) return DGPU_E_ARG;
+    for (int n = 0; n < p->n_th; ++n)
+        if ((p->th_bc_ymin[n] != 0 && p->th_bc_ymin[n] != 1) || (p->th_bc_ymax[n] != 0 && p->th_bc_ymax[n] != 1))
+            return DGPU_E_ARG;
+    dgpu_ctx *c = new (std::nothrow) dgpu_ctx();
+    if (!c) return DGPU_E_NOMEM;
+    c->p = *p;
+    c->p.gy = c->p.gyf = c->p.dy = c->p.dyf = c->p.kx = c->p.kz = c->p.kx2 = c->p.kz2 = nullptr;
+    dgpu::Geo &g = c->g;
+    g.nx = p->nx; g.ny = p->ny; g.nz = p->nz; g.nth = p->n_th;
+    g.nkx = p->nx / 3; g.nkz = p->nz / 3; g.tnkz = 2 * g.nkz;
+    g.mk = g.nkx + 1; g.mz = g.tnkz + 1;
+    g.ps = (long)g.mk * g.mz;
+    const int NY = g.ny, NXP = g.nx / 2;
+    // INIT_CHAN_MPI at NPROCY = 1 (for/mpi.f:1080-1100)
+    c->jstart = 2;
+    c->jend = NY;
+    for (int n = 0; n < g.nth; ++n) {
+        c->jend_th[n] = NY;
+        c->jstart_th[n] = (p->th_bc_ymin[n] == 1) ? 1 : 2;
+    }
+    // grid (for/hdf5.f:676-703, for/channel.f:1646-1652)
+    c->gy.assign(NY + 2, 0.0);
+    c->gyf.assign(NY + 2, 0.0);
+    c->dy.assign(NY + 2, 0.0);
+    c->dyf.assign(NY + 2, 0.0);
+    for (int j = 1; j <= NY + 1; ++j) c->gy[j] = p->gy[j - 1];
+    for (int j = 1; j <= NY; ++j) c->gyf[j] = 0.5 * (c->gy[j] + c->gy[j + 1]);
+    c->gyf[0] = 2.0 * c->gyf[1] - c->gyf[2];
+    c->gyf[NY + 1] = 2.0 * c->gyf[NY] - c->gyf[NY - 1];
+    for (int j = 1; j <= NY + 1; ++j) c->dy[j] = c->gyf[j] - c->gyf[j - 1];
+    for (int j = 1; j <= NY; ++j) c->dyf[j] = c->gy[j + 1] - c->gy[j];
+    c->dyf[NY + 1] = c->dyf[NY];
+    if (p->gyf) c->gyf.assign(p->gyf, p->gyf + NY + 2);
+    if (p->dy) c->dy.assign(p->dy, p->dy + NY + 2);
+    if (p->dyf) c->dyf.assign(p->dyf, p->dyf + NY + 2);
+    // wavenumbers (for/fft.f:72-105)
+    const double PI = 4.0 * std::atan(1.0);
+    c->kx.assign(NXP + 1, 0.0);
+    c->kz.assign(g.tnkz + 1, 0.0);
+    for (int i = 0; i <= NXP - 1; ++i) c->kx[i] = ((double)i * (2.0 * PI)) / p->lx;
+    for (int k = 0; k <= g.nkz; ++k) c->kz[k] = ((double)k * (2.0 * PI)) / p->lz;
+    for (int k = 1; k <= g.nkz; ++k) c->kz[g.tnkz + 1 - k] = ((double)(-k) * (2.0 * PI)) / p->lz;
+    if (p->kx) c->kx.assign(p->kx, p->kx + NXP + 1);
+    if (p->kz) c->kz.assign(p->kz, p->kz + g.tnkz + 1);
+    c->kx2.resize(NXP + 1);
+    c->kz2.resize(g.tnkz + 1);
+    for (int i = 0; i <= NXP; ++i) c->kx2[i] = c->kx[i] * c->kx[i];
+    for (int k = 0; k <= g.tnkz; ++k) c->kz2[k] = c->kz[k] * c->kz[k];
+    if (p->kx2) c->kx2.assign(p->kx2, p->kx2 + NXP + 1);
+    if (p->kz2) c->kz2.assign(p->kz2, p->kz2 + g.tnkz + 1);
+
+    if (!make_len(g.nx, c->fx.len) || !make_len(g.nz, c->fz.len)) {
+        delete c;
+        return DGPU_E_ARG;
+    }
+    int rc = 0;
+    auto bail = [&](int code) { rc = code; dgpu_destroy(c); return code; };
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return bail(DGPU_E_HIP);
+    // device vectors: dy, dyf (NY+2), kx, kx2 (NXP+1), kz, kz2 (TNKZ+1)
+    const long nv = 2L * (NY + 2) + 2L * (NXP + 1) + 2L * (g.tnkz + 1);
+    if (hipMalloc(&c->d_vec, sizeof(double) * nv) != hipSuccess) return bail(DGPU_E_NOMEM);
+    {
+        std::vector<double> h;
+        h.insert(h.end(), c->dy.begin(), c->dy.end());
+        h.insert(h.end(), c->dyf.begin(), c->dyf.end());
+        h.insert(h.end(), c->kx.begin(), c->kx.end());
+        h.insert(h.end(), c->kx2.begin(), c->kx2.end());
+        h.insert(h.end(), c->kz.begin(), c->kz.end());
+        h.insert(h.end(), c->kz2.begin(), c->kz2.end());
+        if (hipMemcpy(c->d_vec, h.data(), sizeof(double) * nv, hipMemcpyHostToDevice) != hipSuccess)
+            return bail(DGPU_E_HIP);
+        c->d_dy = c->d_vec;
+        c->d_dyf = c->d_dy + (NY + 2);
+        c->d_kx = c->d_dyf + (NY + 2);
+        c->d_kx2 = c->d_kx + (NXP + 1);
+        c->d_kz = c->d_kx2 + (NXP + 1);
+        c->d_kz2 = c->d_kz + (g.tnkz + 1);
+    }
+    // twiddles
+    {
+        std::vector<cx<double>> twx, twz;
+        twiddles(g.nx, twx);
+        twiddles(g.nz, twz);
+        if (hipMalloc(&c->d_tw, sizeof(cx<double>) * (g.nx + g.nz)) != hipSuccess) return bail(DGPU_E_NOMEM);
+        if (hipMemcpy(c->d_tw, twx.data(), sizeof(cx<double>) * g.nx, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(c->d_tw + g.nx, twz.data(), sizeof(cx<double>) * g.nz, hipMemcpyHostToDevice) != hipSuccess)
+            return bail(DGPU_E_HIP);
+        c->fx.tw = c->d_tw;
+        c->fz.tw = c->d_tw + g.nx;
+    }
+    // state: cu1 cu2 cu3 cth[n] | cp cf1 cf2 cf3 cfth[n] | nl[3+n] | pois
+    const int nin = 3 + g.nth;
+    c->fs = (long)(NY + 2) * g.ps;
+    c->nslot = nin + 4 + g.nth + nin + 1;
+    if (hipMalloc(&c->d_state, sizeof(cx<double>) * c->fs * c->nslot) != hipSuccess) return bail(DGPU_E_NOMEM);
+    if (hipMemset(c->d_state, 0, sizeof(cx<double>) * c->fs * c->nslot) != hipSuccess) return bail(DGPU_E_HIP);
+    {
+        int s = 0;
+        for (int f = 0; f < 3; ++f) c->cu[f] = c->d_state + (s++) * c->fs;
+        for (int n = 0; n < g.nth; ++n) c->cth[n] = c->d_state + (s++) * c->fs;
+        c->cp = c->d_state + (s++) * c->fs;
+        for (int f = 0; f < 3; ++f) c->cf[f] = c->d_state + (s++) * c->fs;
+        for (int n = 0; n < g.nth; ++n) c->cfth[n] = c->d_state + (s++) * c->fs;
+        for (int f = 0; f < nin; ++f) c->nl[f] = c->d_state + (s++) * c->fs;
+        c->pois = c->d_state + (s++) * c->fs;
+    }
+    if (hipMalloc(&c->d_piv, sizeof(double) * c->fs) != hipSuccess) return bail(DGPU_E_NOMEM);
+    if (hipMalloc(&c->d_nlptr, sizeof(cx<double> *) * (3 + DGPU_MAX_TH)) != hipSuccess) return bail(DGPU_E_NOMEM);
+    if (hipMemcpy(c->d_nlptr, c->nl, sizeof(cx<double> *) * (3 + DGPU_MAX_TH), hipMemcpyHostToDevice) != hipSuccess)
+        return bail(DGPU_E_HIP);
+    // intermediates
+    c->zfs = (long)(NY + 2) * g.nz * g.mk;
+    if (hipMalloc(&c->d_zt, sizeof(cx<double>) * c->zfs * nin) != hipSuccess) return bail(DGPU_E_NOMEM);
+    if (hipMalloc(&c->d_wt, sizeof(cx<double>) * c->zfs * 2 * nin) != hipSuccess) return bail(DGPU_E_NOMEM);
+    // CN tables
+    c->h_cn.assign(3L * nsys_of(c) * cn_block(c), 0.0);
+    if (hipMalloc(&c->d_cn, sizeof(double) * c->h_cn.size()) != hipSuccess) return bail(DGPU_E_NOMEM);
+    // BC codes for APPLY_BC_VEL
+    {
+        int codes[6] = {p->u_bc_ymin, p->v_bc_ymin, p->w_bc_ymin, p->u_bc_ymax, p->v_bc_ymax, p->w_bc_ymax};
+        double c1[6] = {p->u_bc_ymin_c1, p->v_bc_ymin_c1, p->w_bc_ymin_c1,
+                        p->u_bc_ymax_c1, p->v_bc_ymax_c1, p->w_bc_ymax_c1};
+        if (hipMalloc(&c->d_codes, sizeof(codes)) != hipSuccess || hipMalloc(&c->d_c1, sizeof(c1)) != hipSuccess)
+            return bail(DGPU_E_NOMEM);
+        if (hipMemcpy(c->d_codes, codes, sizeof(codes), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(c->d_c1, c1, sizeof(c1), hipMemcpyHostToDevice) != hipSuccess)
+            return bail(DGPU_E_HIP);
+    }
+    (void)rc;
+    *out = c;
+    return DGPU_OK;
+}
+
+int32_t dgpu_destroy(dgpu_ctx *c)
+{
+    if (!c) return DGPU_OK;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (auto &t : c->pending) { hipEventDestroy(t.a); hipEventDestroy(t.b); }
+    for (auto e : c->event_pool) hipEventDestroy(e);
+    void *bufs[] = {c->d_vec, c->d_tw, c->d_state, c->d_piv, c->d_nlptr, c->d_zt, c->d_wt,
+                    c->d_fort, c->d_cn, c->d_codes, c->d_c1};
+    for (void *b : bufs)
+        if (b) hipFree(b);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return DGPU_OK;
+}
+
+int32_t dgpu_synchronize(dgpu_ctx *c)
+{
+    if (!c) return DGPU_E_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return DGPU_OK;
+}
+
+int32_t dgpu_set_field(dgpu_ctx *c, int32_t field, const double *a)
+{
+    if (!c || !a) return DGPU_E_ARG;
+    cx<double> *dst = field_ptr(c, field);
+    if (!dst) return fail(c, DGPU_E_ARG, "unknown field id");
+    if (int rc = ensure_fort(c)) return rc;
+    const size_t n = (size_t)(c->g.nx + 2) * (c->g.nz + 2) * (c->g.ny + 2);
+    HIPCHK(c, hipMemcpyAsync(c->d_fort, a, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    dgpu::launch_unpack_fortran(c->stream, c->g, c->d_fort, dst, 0, c->g.ny + 2);
+    if (int rc = check_launch(c, "unpack")) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return DGPU_OK;
+}
+
+int32_t dgpu_get_field(dgpu_ctx *c, int32_t field, double *a)
+{
+    if (!c || !a) return DGPU_E_ARG;
+    cx<double> *src = field_ptr(c, field);
+    if (!src) return fail(c, DGPU_E_ARG, "unknown field id");
+    if (int rc = ensure_fort(c)) return rc;
+    const dgpu::Geo &g = c->g;
+    const long plane = (long)(g.nx / 2 + 1) * (g.nz + 2);
+    hipLaunchKernelGGL(k_pack_fortran, dim3((unsigned)((plane + 255) / 256), g.ny + 2), dim3(256), 0, c->stream, src,
+                       c->d_fort, g.nx / 2, g.nz, g.mk, g.mz);
+    if (int rc = check_launch(c, "pack")) return rc;
+    const size_t n = (size_t)(g.nx + 2) * (g.nz + 2) * (g.ny + 2);
+    HIPCHK(c, hipMemcpyAsync(a, c->d_fort, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return DGPU_OK;
+}
+
+static void fill_colargs(dgpu_ctx *c, dgpu::ColArgs &A, dgpu::ColPtrs<double> &P, int stage)
+{
+    const dgpu::Geo &g = c->g;
+    std::memset(&A, 0, sizeof(A));
+    A.g = g;
+    A.jstart = c->jstart;
+    A.jend = c->jend;
+    for (int n = 0; n < g.nth; ++n) {
+        A.jstart_th[n] = c->jstart_th[n];
+        A.jend_th[n] = c->jend_th[n];
+        A.ri[n] = c->p.ri[n];
+        A.pr[n] = c->p.pr[n];
+        A.drhodx[n] = c->p.drhodx[n];
+        A.drhodz[n] = c->p.drhodz[n];
+    }
+    A.nu = c->p.nu;
+    A.beta = c->p.beta;
+    A.i_ro = c->p.i_ro;
+    A.grav_x = c->p.grav_x;
+    A.grav_y = c->p.grav_y;
+    A.grav_z = c->p.grav_z;
+    A.nuv = c->p.nu_v_scale;
+    A.f_type = c->p.f_type;
+    A.dy = c->d_dy;
+    A.dyf = c->d_dyf;
+    A.kx = c->d_kx;
+    A.kz = c->d_kz;
+    A.kx2 = c->d_kx2;
+    A.kz2 = c->d_kz2;
+    const int nsys = nsys_of(c);
+    const long blk = cn_block(c);
+    const int ny2 = g.ny + 2;
+    for (int q = 0; q < nsys; ++q) {
+        const double *t = c->d_cn + ((long)stage * nsys + q) * blk;
+        dgpu::CnSys &S = A.sys[q];
+        S.mult = t;
+        S.piv = t + ny2;
+        S.up = t + 2 * ny2;
+        S.bc00 = t + 3 * ny2;
+        S.isbc = t + 4 * ny2;
+        if (q == dgpu::SYS_U1 || q == dgpu::SYS_U3) { S.lo = c->jstart - 1; S.hi = c->jend + 1; }
+        else if (q == dgpu::SYS_U2) { S.lo = 1; S.hi = g.ny + 1; }
+        else { S.lo = c->jstart_th[q - 3] - 1; S.hi = c->jend_th[q - 3] + 1; }
+    }
+    std::memset(&P, 0, sizeof(P));
+    P.cu1 = c->cu[0];
+    P.cu2 = c->cu[1];
+    P.cu3 = c->cu[2];
+    P.cp = c->cp;
+    P.cf1 = c->cf[0];
+    P.cf2 = c->cf[1];
+    P.cf3 = c->cf[2];
+    for (int n = 0; n < g.nth; ++n) {
+        P.cth[n] = c->cth[n];
+        P.cfth[n] = c->cfth[n];
+    }
+    for (int f = 0; f < 3 + g.nth; ++f) P.nl[f] = c->nl[f];
+    P.pois = c->pois;
+    P.piv = c->d_piv;
+}
+
+int32_t dgpu_rk_chan_1(dgpu_ctx *c, int32_t rk_step, double time, const double *h_bar, const double *beta_bar,
+                       const double *zeta_bar)
+{
+    if (!c || !h_bar || !beta_bar || !zeta_bar || rk_step < 1 || rk_step > 3) return DGPU_E_ARG;
+    const int k = rk_step - 1;
+    // for/channel.f:77-81
+    const double temp1 = c->p.nu * h_bar[k] / 2.0;
+    const double temp3 = zeta_bar[k] * h_bar[k];
+    const double temp4 = h_bar[k];
+    const double temp5 = beta_bar[k] * h_bar[k];
+    if (int rc = ensure_cn(c, k, temp1)) return rc;
+    const dgpu::Geo &g = c->g;
+    const int nin = 3 + g.nth;
+    int jlo = 2;
+    for (int n = 0; n < g.nth; ++n) jlo = std::min(jlo, c->jstart_th[n]);
+    {
+        TimerScope t(c, 0);
+        dgpu::launch_zinv<double>(c->stream, g, c->fz, c->cu[0], c->fs, g.ps, g.mk, g.mk, c->d_zt, c->zfs,
+                                  (long)g.nz * g.mk, g.mk, 0, g.ny + 2, nin);
+    }
+    if (int rc = check_launch(c, "zinv")) return rc;
+    {
+        TimerScope t(c, 1);
+        dgpu::launch_xfused<double>(c->stream, g, c->fx, c->d_zt, c->zfs, c->d_wt, c->zfs, c->d_dy, c->d_dyf, c->d_kx,
+                                    jlo, g.ny);
+    }
+    if (int rc = check_launch(c, "xfused")) return rc;
+    {
+        TimerScope t(c, 2);
+        dgpu::launch_zfwd_nl<double>(c->stream, g, c->fz, c->d_wt, c->zfs, c->d_nlptr, c->d_kz, jlo, g.ny, nin);
+    }
+    if (int rc = check_launch(c, "zfwd_nl")) return rc;
+    dgpu::ColArgs A;
+    dgpu::ColPtrs<double> P;
+    fill_colargs(c, A, P, k);
+    A.mode = 0;
+    A.rk_gt1 = rk_step > 1;
+    A.temp1 = temp1;
+    A.temp3 = temp3;
+    A.temp4 = temp4;
+    A.temp5 = temp5;
+    A.force00 = (c->p.f_type == 2) ? (c->p.px0 + c->p.amp_omega0 * std::cos(c->p.omega0 * time)) : c->p.px0;
+    {
+        TimerScope t(c, 3);
+        dgpu::launch_columns<double>(c->stream, A, P);
+    }
+    if (int rc = check_launch(c, "columns")) return rc;
+    return DGPU_OK;
+}
+
+int32_t dgpu_rem_div_chan(dgpu_ctx *c)
+{
+    if (!c) return DGPU_E_ARG;
+    dgpu::ColArgs A;
+    dgpu::ColPtrs<double> P;
+    fill_colargs(c, A, P, 0);
+    A.mode = 1;
+    A.temp4 = 1.0;
+    dgpu::launch_columns<double>(c->stream, A, P);
+    return check_launch(c, "rem_div");
+}
+
+int32_t dgpu_apply_bc_vel(dgpu_ctx *c)
+{
+    if (!c) return DGPU_E_ARG;
+    dgpu::launch_apply_bc_vel(c->stream, c->g, c->cu[0], c->cu[1], c->cu[2], c->d_codes, c->d_c1, c->d_dy, c->d_dyf);
+    return check_launch(c, "apply_bc_vel");
+}
+
+int32_t dgpu_fft_xz_to_fourier(dgpu_ctx *c, double *v, int32_t jmin, int32_t jmax)
+{
+    if (!c || !v) return DGPU_E_ARG;
+    const dgpu::Geo &g = c->g;
+    if (jmin < 0 || jmax > g.ny + 1) return fail(c, DGPU_E_ARG, "plane range outside 0..NY+1");
+    if (jmax < jmin) return DGPU_OK;
+    if (int rc = ensure_fort(c)) return rc;
+    const size_t plane = (size_t)(g.nx + 2) * (g.nz + 2);
+    const int np = jmax - jmin + 1;
+    HIPCHK(c, hipMemcpyAsync(c->d_fort + plane * jmin, v + plane * jmin, plane * np * sizeof(double),
+                             hipMemcpyHostToDevice, c->stream));
+    dgpu::launch_x_r2c<double>(c->stream, g, c->fx, c->d_fort, c->d_zt, jmin, np);
+    if (int rc = check_launch(c, "x_r2c")) return rc;
+    dgpu::launch_zfwd_pack<double>(c->stream, g, c->fz, c->d_zt, c->d_fort, jmin, np);
+    if (int rc = check_launch(c, "zfwd_pack")) return rc;
+    HIPCHK(c, hipMemcpyAsync(v + plane * jmin, c->d_fort + plane * jmin, plane * np * sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return DGPU_OK;
+}
+
+int32_t dgpu_fft_xz_to_physical(dgpu_ctx *c, double *vv, int32_t jmin, int32_t jmax)
+{
+    if (!c || !vv) return DGPU_E_ARG;
+    const dgpu::Geo &g = c->g;
+    if (jmin < 0 || jmax > g.ny + 1) return fail(c, DGPU_E_ARG, "plane range outside 0..NY+1");
+    if (jmax < jmin) return DGPU_OK;
+    if (int rc = ensure_fort(c)) return rc;
+    const size_t plane = (size_t)(g.nx + 2) * (g.nz + 2);
+    const int np = jmax - jmin + 1;
+    const int nxp = g.nx / 2;
+    HIPCHK(c, hipMemcpyAsync(c->d_fort + plane * jmin, vv + plane * jmin, plane * np * sizeof(double),
+                             hipMemcpyHostToDevice, c->stream));
+    const cx<double> *in = reinterpret_cast<const cx<double> *>(c->d_fort);
+    const long in_ps = (long)(nxp + 1) * (g.nz + 2);
+    dgpu::launch_zinv<double>(c->stream, g, c->fz, in, 0, in_ps, nxp + 1, nxp, c->d_wt, 0, (long)g.nz * nxp, nxp,
+                              jmin, np, 1);
+    if (int rc = check_launch(c, "zinv(host)")) return rc;
+    dgpu::launch_x_c2r<double>(c->stream, g, c->fx, c->d_wt, nxp, c->d_fort, jmin, np);
+    if (int rc = check_launch(c, "x_c2r")) return rc;
+    HIPCHK(c, hipMemcpyAsync(vv + plane * jmin, c->d_fort + plane * jmin, plane * np * sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return DGPU_OK;
+}
+
+int32_t dgpu_set_kernel_timing(dgpu_ctx *c, int32_t enable)
+{
+    if (!c) return DGPU_E_ARG;
+    c->timing = enable != 0;
+    return DGPU_OK;
+}
+
+int32_t dgpu_kernel_times(dgpu_ctx *c, double *ms, int64_t *count, int32_t n)
+{
+    if (!c) return DGPU_E_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (auto &t : c->pending) {
+        float e = 0.f;
+        if (hipEventElapsedTime(&e, t.a, t.b) == hipSuccess) {
+            c->t_ms[t.kind] += e;
+            c->t_n[t.kind] += 1;
+        }
+        c->event_pool.push_back(t.a);
+        c->event_pool.push_back(t.b);
+    }
+    c->pending.clear();
+    for (int k = 0; k < n && k < kKinds; ++k) {
+        if (ms) ms[k] = c->t_ms[k];
+        if (count) count[k] = c->t_n[k];
+    }
+    for (int k = 0; k < kKinds; ++k) { c->t_ms[k] = 0; c->t_n[k] = 0; }
+    return DGPU_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,918 @@
+// dgpu_kernels.hip -- gfx950 kernels of the channel time-stepper.
+//
+// One RK3 substep (RK_CHAN_1, for/channel.f:62-1073) is four launches:
+//
+//   zinv     CU1,CU2,CU3,CTH (kept modes)  --z-inverse-->  Zt[f][j][z][kx]
+//   xfused   per z-row, sweeping j: C2R of the four fields (pairs packed in one
+//            complex FFT), the eleven nonlinear products of
+//            for/channel.f:410-571 and 670-720 in physical space, R2C of the
+//            products, the d/dx terms applied (-i kx), written as
+//            G = -(i kx X(a) + X(b)) and H = -X(c) per momentum/scalar eq.
+//   zfwd_nl  NL_f = Z(G_f) + i kz Z(H_f)   (z-forward of both, scaled 1/(NX NZ))
+//   columns  one thread per (kx,kz) mode: explicit RHS, Crank-Nicolson solves
+//            of TH, U2, U1, U3 in Fourier space, REM_DIV_CHAN and the CP update.
+//
+// The reference's eight physical<->Fourier transforms around its physical-space
+// CN solves are not needed: with LES off the CN matrix is the same for every
+// (x,z) column, so the solve commutes with the x-z transform and is done per
+// Fourier mode, with the wall values entering only the (0,0) mode.
+#include "dgpu_kernels.hpp"
+
+namespace dgpu {
+
+// -------------------------------------------------------------------------
+// helpers
+// -------------------------------------------------------------------------
+
+__device__ __forceinline__ int kzslot_to_z(int s, int nkz, int tnkz, int nz)
+{
+    // packed slot K <= NKZ holds kz = K, slot K > NKZ holds kz = K - (TNKZ+1)
+    // (for/mpi.f:1602-1606); its index in the length-NZ transform:
+    return s <= nkz ? s : nz - (tnkz + 1 - s);
+}
+
+template <typename T>
+__device__ __forceinline__ cx<T> ld(const cx<T> *p) { return *p; }
+
+// -------------------------------------------------------------------------
+// z-inverse: kept slots -> length-NZ columns (FFT_XZ_TO_PHYSICAL z part,
+// for/mpi.f:1627-1641): unpack, zero the gap, unnormalised backward C2C.
+// grid (col-chunks, planes, fields); block W*T threads; LDS W*NZ complex.
+// -------------------------------------------------------------------------
+template <typename T, int E>
+__global__ void k_zinv(FftLen fz, const cx<T> *__restrict__ tw, const cx<T> *__restrict__ in, long in_fs,
+                       long in_ps, int in_mk, int ncols, cx<T> *__restrict__ out, long out_fs, long out_ps,
+                       int out_mk, int j0, int nkz, int tnkz, int W)
+{
+    extern __shared__ unsigned char smem_raw[];
+    cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
+    const int nz = fz.n;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int i0 = blockIdx.x * W;
+    const int j = j0 + blockIdx.y;
+    const int f = blockIdx.z;
+    const cx<T> *src = in + f * in_fs + (long)j * in_ps;
+    cx<T> *dst = out + f * out_fs + (long)j * out_ps;
+    const cx<T> zero = make_cx<T>(0, 0);
+    for (int idx = tid; idx < W * nz; idx += nthr) lds[idx] = zero;
+    __syncthreads();
+    const int mz = tnkz + 1;
+    for (int idx = tid; idx < W * mz; idx += nthr) {
+        const int s = idx / W, b = idx - s * W;
+        const int i = i0 + b;
+        if (i < ncols) lds[kzslot_to_z(s, nkz, tnkz, nz) * W + b] = src[(long)s * in_mk + i];
+    }
+    __syncthreads();
+    const int b = tid / fz.t, th = tid - b * fz.t;
+    fft_lds<T, E, +1>(lds + b, W, fz, th, b < W, tw);
+    for (int idx = tid; idx < W * nz; idx += nthr) {
+        const int z = idx / W, bb = idx - z * W;
+        const int i = i0 + bb;
+        if (i < ncols) dst[(long)z * out_mk + i] = lds[idx];
+    }
+}
+
+// -------------------------------------------------------------------------
+// xfused: per z-row, planes jlo..jhi.  LDS: window of 3 planes x NPI complex
+// rows (pairs of real fields) + NPO complex rows for the products.
+// -------------------------------------------------------------------------
+template <int NTH>
+struct XCounts {
+    static constexpr int nin = 3 + NTH;                 // u1 u2 u3 th...
+    static constexpr int npi = (nin + 1) / 2;           // C2R pairs
+    static constexpr int nprod = 8 + 3 * NTH;           // eleven products for NTH = 1
+    static constexpr int npo = (nprod + 1) / 2;         // R2C pairs
+    static constexpr int nout = 2 * nin;                // G,H per equation
+};
+
+template <typename T, int NTH>
+__device__ __forceinline__ T pick(const cx<T> *row, int fld, int x)
+{
+    const cx<T> v = row[(fld >> 1) * 0 + x];
+    return (fld & 1) ? v.im : v.re;
+}
+
+template <typename T, int E, int NTH>
+__global__ void k_xfused(FftLen fx, const cx<T> *__restrict__ tw, int mk, const cx<T> *__restrict__ zt,
+                         long zt_fs, long zt_ps, cx<T> *__restrict__ wt, long wt_fs, long wt_ps,
+                         const double *__restrict__ dy, const double *__restrict__ dyf,
+                         const double *__restrict__ kxv, int jlo, int jhi, int chunk)
+{
+    using C = XCounts<NTH>;
+    extern __shared__ unsigned char smem_raw[];
+    cx<T> *win = reinterpret_cast<cx<T> *>(smem_raw);     // [3][npi][nx]
+    const int nx = fx.n;
+    cx<T> *fb = win + 3 * C::npi * nx;                    // [npo][nx]
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int z = blockIdx.x;
+    const int ja = jlo + blockIdx.y * chunk;
+    const int jb = min(jhi, ja + chunk - 1);
+    const long zrow = (long)z * mk;
+    const cx<T> zero = make_cx<T>(0, 0);
+    const T inv_nx = T(1) / T(nx);
+
+    auto load_plane = [&](int q) {
+        cx<T> *w = win + (q % 3) * C::npi * nx;
+        for (int idx = tid; idx < C::npi * nx; idx += nthr) {
+            const int p = idx / nx, k = idx - p * nx;
+            const int fa = 2 * p, fb2 = 2 * p + 1;
+            cx<T> A = zero, B = zero;
+            bool hit = false, cj = false;
+            int kk = k;
+            if (k < mk) {
+                hit = true;
+            } else if (k > nx - mk) {
+                hit = true; cj = true; kk = nx - k;
+            }
+            if (hit) {
+                A = zt[fa * zt_fs + (long)q * zt_ps + zrow + kk];
+                if (fb2 < C::nin) B = zt[fb2 * zt_fs + (long)q * zt_ps + zrow + kk];
+                if (kk == 0) { A.im = 0; B.im = 0; }   // C2R ignores Im of the DC term
+                if (cj) { A.im = -A.im; B.im = -B.im; }
+            }
+            w[idx] = make_cx<T>(A.re - B.im, A.im + B.re);  // A + i B
+        }
+        __syncthreads();
+        const int p = tid / fx.t, th = tid - p * fx.t;
+        fft_lds<T, E, +1>(win + (q % 3) * C::npi * nx + p * nx, 1, fx, th, p < C::npi, tw);
+    };
+
+    load_plane(ja - 1);
+    load_plane(ja);
+    for (int j = ja; j <= jb; ++j) {
+        load_plane(j + 1);
+        const cx<T> *wm = win + ((j - 1) % 3) * C::npi * nx;
+        const cx<T> *w0 = win + (j % 3) * C::npi * nx;
+        const cx<T> *wp = win + ((j + 1) % 3) * C::npi * nx;
+        const T dyj = T(dy[j]), dyfj = T(dyf[j]), dyfm = T(dyf[j - 1]), dyp = T(dy[j + 1]);
+        (void)dyp;
+        for (int x = tid; x < nx; x += nthr) {
+            const cx<T> a0 = w0[x], am = wm[x], ap = wp[x];
+            const cx<T> b0 = w0[nx + x], bm = wm[nx + x], bp = wp[nx + x];
+            const T u1 = a0.re, u2 = a0.im, u3 = b0.re;
+            const T u1m = am.re, u2m = am.im, u3m = bm.re;
+            const T u1p = ap.re, u2p = ap.im, u3p = bp.re;
+            T pr[C::nprod + 1];
+            pr[0] = u3 * u1;                                                        // U1*U3
+            pr[1] = u1 * u1;                                                        // U1*U1
+            pr[2] = u3 * u3;                                                        // U3*U3
+            pr[3] = ((dyfj * u1 + dyfm * u1m) / (T(2) * dyj)) * u2;                 // U1*U2 at GY
+            pr[4] = ((dyfj * u3 + dyfm * u3m) / (T(2) * dyj)) * u2;                 // U3*U2 at GY
+            pr[5] = (u1p * u2p + u1 * u2p - u1 * u2 - u1m * u2) / (T(2) * dyfj);    // d/dy(U1 U2)
+            pr[6] = (u3p * u2p + u3 * u2p - u3 * u2 - u3m * u2) / (T(2) * dyfj);    // d/dy(U3 U2)
+            {
+                const T sp = u2 + u2p, sm = u2 + u2m;
+                pr[7] = (T(0.25) * (sp * sp) - T(0.25) * (sm * sm)) / dyj;          // d/dy(U2 U2)
+            }
+            if constexpr (NTH >= 1) {
+                const T th = b0.im, thm = bm.im, thp = bp.im;
+                pr[8] = th * u1;
+                pr[9] = th * u3;
+                pr[10] = (thp * u2p + th * u2p - th * u2 - thm * u2) / (T(2) * dyfj);
+            }
+            pr[C::nprod] = T(0);
+#pragma unroll
+            for (int p = 0; p < C::npo; ++p) fb[p * nx + x] = make_cx<T>(pr[2 * p], pr[2 * p + 1]);
+        }
+        __syncthreads();
+        {
+            const int p = tid / fx.t, th = tid - p * fx.t;
+            fft_lds<T, E, -1>(fb + p * nx, 1, fx, th, p < C::npo, tw);
+        }
+        for (int k = tid; k < mk; k += nthr) {
+            T xr[C::nprod + 1], xi[C::nprod + 1];
+#pragma unroll
+            for (int p = 0; p < C::npo; ++p) {
+                const cx<T> Z = fb[p * nx + k];
+                const cx<T> Zn = fb[p * nx + (k == 0 ? 0 : nx - k)];
+                // A = (Z + conj Zn)/2, B = (Z - conj Zn)/(2i); then 1/NX
+                const T h = T(0.5) * inv_nx;
+                xr[2 * p] = (Z.re + Zn.re) * h;
+                xi[2 * p] = (Z.im - Zn.im) * h;
+                xr[2 * p + 1] = (Z.im + Zn.im) * h;
+                xi[2 * p + 1] = -(Z.re - Zn.re) * h;
+            }
+            const T kx = T(kxv[k]);
+            // i*kx*X = (-kx*Xi, kx*Xr)
+            auto outw = [&](int o, T re, T im) {
+                wt[o * wt_fs + (long)j * wt_ps + zrow + k] = make_cx<T>(re, im);
+            };
+            // momentum x: CF1 -= i kz X(U1U3) + i kx X(U1U1) + X(dy U1U2)
+            outw(0, -(-kx * xi[1] + xr[5]), -(kx * xr[1] + xi[5]));
+            outw(1, -xr[0], -xi[0]);
+            // momentum y: CF2 -= i kx X(U1U2) + i kz X(U3U2) + X(dy U2U2)
+            outw(2, -(-kx * xi[3] + xr[7]), -(kx * xr[3] + xi[7]));
+            outw(3, -xr[4], -xi[4]);
+            // momentum z: CF3 -= i kx X(U1U3) + i kz X(U3U3) + X(dy U3U2)
+            outw(4, -(-kx * xi[0] + xr[6]), -(kx * xr[0] + xi[6]));
+            outw(5, -xr[2], -xi[2]);
+            if constexpr (NTH >= 1) {
+                // scalar: CFTH -= i kx X(TH U1) + i kz X(TH U3) + X(dy TH U2)
+                outw(6, -(-kx * xi[8] + xr[10]), -(kx * xr[8] + xi[10]));
+                outw(7, -xr[9], -xi[9]);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// -------------------------------------------------------------------------
+// zfwd_nl: NL_f(j, K, I) = [FFT_z G_f + i kz FFT_z H_f] / NZ at kept slots.
+// grid (col-chunks, planes, outputs); block W*T; LDS W*NZ complex.
+// -------------------------------------------------------------------------
+template <typename T, int E>
+__global__ void k_zfwd_nl(FftLen fz, const cx<T> *__restrict__ tw, const cx<T> *__restrict__ wt, long wt_fs,
+                          long wt_ps, int mk, cx<T> *const *__restrict__ nl, long ps, const double *__restrict__ kzv,
+                          int jlo, int nkz, int tnkz, int W)
+{
+    extern __shared__ unsigned char smem_raw[];
+    cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
+    constexpr int MAXIT = 24;
+    const int nz = fz.n;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int i0 = blockIdx.x * W;
+    const int j = jlo + blockIdx.y;
+    const int o = blockIdx.z;
+    const int mz = tnkz + 1;
+    const T inv_nz = T(1) / T(nz);
+    cx<T> keep[MAXIT];
+    const int b = tid / fz.t, th = tid - b * fz.t;
+#pragma unroll
+    for (int pass = 0; pass < 2; ++pass) {
+        const cx<T> *src = wt + (2 * o + pass) * wt_fs + (long)j * wt_ps;
+        for (int idx = tid; idx < W * nz; idx += nthr) {
+            const int z = idx / W, bb = idx - z * W;
+            const int i = i0 + bb;
+            lds[idx] = (i < mk) ? src[(long)z * mk + i] : make_cx<T>(0, 0);
+        }
+        __syncthreads();
+        fft_lds<T, E, -1>(lds + b, W, fz, th, b < W, tw);
+#pragma unroll
+        for (int it = 0; it < MAXIT; ++it) {
+            const int idx = tid + it * nthr;
+            if (idx < W * mz) {
+                const int s = idx / W, bb = idx - s * W;
+                const cx<T> v = lds[kzslot_to_z(s, nkz, tnkz, nz) * W + bb];
+                if (pass == 0) {
+                    keep[it] = scale(v, inv_nz);
+                } else {
+                    const T kz = T(kzv[s]);
+                    const cx<T> h = scale(v, inv_nz);
+                    keep[it] = make_cx<T>(keep[it].re - kz * h.im, keep[it].im + kz * h.re);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    cx<T> *dst = nl[o] + (long)j * ps;
+#pragma unroll
+    for (int it = 0; it < MAXIT; ++it) {
+        const int idx = tid + it * nthr;
+        if (idx < W * mz) {
+            const int s = idx / W, bb = idx - s * W;
+            const int i = i0 + bb;
+            if (i < mk) dst[(long)s * mk + i] = keep[it];
+        }
+    }
+}
+
+// -------------------------------------------------------------------------
+// columns: one thread per kept mode.  See ColArgs for the inputs.
+// -------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ cx<T> ikmul(T k, cx<T> a) { return make_cx<T>(-k * a.im, k * a.re); }
+
+template <typename T>
+__device__ __forceinline__ cx<T> axpy(T a, cx<T> x, cx<T> y) { return make_cx<T>(y.re + a * x.re, y.im + a * x.im); }
+
+template <typename T>
+__device__ __forceinline__ cx<T> divr(cx<T> a, T r) { return make_cx<T>(a.re / r, a.im / r); }
+
+template <typename T>
+__global__ void k_columns(ColArgs A, ColPtrs<T> P)
+{
+    const Geo &g = A.g;
+    const long MM = g.ps;
+    const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= MM) return;
+    const int s = (int)(m / g.mk), i = (int)(m - (long)s * g.mk);
+    const bool is00 = (i == 0 && s == 0);
+    const int NY = g.ny;
+    const long ps = g.ps;
+    const double *DY = A.dy, *DYF = A.dyf;
+    const T kx = T(A.kx[i]), kz = T(A.kz[s]);
+    const T kx2 = T(A.kx2[i]), kz2 = T(A.kz2[s]);
+    const cx<T> zero = make_cx<T>(0, 0);
+    const int nth = g.nth;
+
+    if (A.mode == 0) {
+        const T pw = T(pow(double(kx2 + kz2), A.beta));
+        const T temp1 = T(A.temp1), temp3 = T(A.temp3), temp4 = T(A.temp4), temp5 = T(A.temp5);
+        const T nuv = T(A.nuv);
+        const T visc = T(A.nu) * pw;
+        // --- sweep A: explicit RHS, new CF, CN forward elimination ---
+        cx<T> u1m = zero, u2m = zero, u3m = zero, cpm = zero;
+        cx<T> u1 = P.cu1[m], u2 = P.cu2[m], u3 = P.cu3[m], cp0 = P.cp[m];
+        cx<T> thm[kMaxTh], th0[kMaxTh], gth[kMaxTh];
+        for (int n = 0; n < nth; ++n) { thm[n] = zero; th0[n] = P.cth[n][m]; gth[n] = zero; }
+        cx<T> g1 = zero, g2 = zero, g3 = zero;
+        for (int j = 0; j <= NY + 1; ++j) {
+            const long o = m + (long)j * ps;
+            const bool hasp = (j <= NY);
+            const cx<T> u1p = hasp ? P.cu1[o + ps] : zero;
+            const cx<T> u2p = hasp ? P.cu2[o + ps] : zero;
+            const cx<T> u3p = hasp ? P.cu3[o + ps] : zero;
+            cx<T> thp[kMaxTh];
+            for (int n = 0; n < nth; ++n) thp[n] = hasp ? P.cth[n][o + ps] : zero;
+            const T dyj = T(DY[j]), dyfj = T(DYF[j]);
+            const T dyp = hasp ? T(DY[j + 1]) : T(1);
+            const T dyfm = j > 0 ? T(DYF[j - 1]) : T(1);
+
+            // ---- U1 and U3 (rows JSTART..JEND) ----
+            {
+                const bool inr = (j >= A.jstart && j <= A.jend);
+                cx<T> cf1n = zero, cf3n = zero;
+                if (inr) {
+                    const cx<T> nl1 = P.nl[0][o], nl3 = P.nl[2][o];
+                    cf1n = make_cx<T>(-(visc * u1.re), -(visc * u1.im));
+                    cf3n = make_cx<T>(-(visc * u3.re), -(visc * u3.im));
+                    cf1n = axpy(T(A.i_ro), u3, cf1n);
+                    cf3n = axpy(-T(A.i_ro), u1, cf3n);
+                    for (int n = 0; n < nth; ++n) {
+                        cf1n = axpy(T(A.ri[n] * A.grav_x), th0[n], cf1n);
+                        cf3n = axpy(T(A.ri[n] * A.grav_z), th0[n], cf3n);
+                    }
+                    cf1n = cf1n + nl1;
+                    cf3n = cf3n + nl3;
+                }
+                // U1
+                {
+                    const CnSys &S = A.sys[SYS_U1];
+                    cx<T> vec;
+                    if (S.isbc[j] != 0.0) {
+                        vec = is00 ? make_cx<T>(T(S.bc00[j]), 0) : zero;
+                    } else {
+                        cx<T> cr = u1;
+                        if (A.rk_gt1) cr = axpy(temp3, P.cf1[o], cr);
+                        cr = cr - scale(ikmul(kx, cp0), temp4);
+                        if (is00 && (A.f_type == 1 || A.f_type == 2)) cr.re = cr.re - temp4 * T(A.force00);
+                        cr = axpy(temp5, cf1n, cr);
+                        const cx<T> d2 = divr(divr(u1p - u1, dyp) - divr(u1 - u1m, dyj), dyfj);
+                        vec = axpy(temp1 * nuv, d2, cr);
+                    }
+                    g1 = (j == 0) ? vec : axpy(T(S.mult[j]), g1, vec);
+                }
+                // U3
+                {
+                    const CnSys &S = A.sys[SYS_U3];
+                    cx<T> vec;
+                    if (S.isbc[j] != 0.0) {
+                        vec = is00 ? make_cx<T>(T(S.bc00[j]), 0) : zero;
+                    } else {
+                        cx<T> cr = u3;
+                        if (A.rk_gt1) cr = axpy(temp3, P.cf3[o], cr);
+                        cr = cr - scale(ikmul(kz, cp0), temp4);
+                        cr = axpy(temp5, cf3n, cr);
+                        const cx<T> d2 = divr(divr(u3p - u3, dyp) - divr(u3 - u3m, dyj), dyfj);
+                        vec = axpy(temp1 * nuv, d2, cr);
+                    }
+                    g3 = (j == 0) ? vec : axpy(T(S.mult[j]), g3, vec);
+                }
+                if (inr) {
+                    P.cf1[o] = cf1n;
+                    P.cf3[o] = cf3n;
+                }
+            }
+            // ---- U2 (rows 2..NY) ----
+            {
+                const bool inr = (j >= 2 && j <= NY);
+                cx<T> cf2n = zero;
+                if (inr) {
+                    cf2n = make_cx<T>(-(visc * u2.re), -(visc * u2.im));
+                    if (!is00) {
+                        for (int n = 0; n < nth; ++n) {
+                            const cx<T> t = make_cx<T>(th0[n].re * dyfm + thm[n].re * dyfj, th0[n].im * dyfm + thm[n].im * dyfj);
+                            const T c = T(A.ri[n] * A.grav_y);
+                            cf2n = cf2n + divr(make_cx<T>(c * t.re, c * t.im), T(2) * dyj);
+                        }
+                    }
+                    cf2n = cf2n + P.nl[1][o];
+                }
+                const CnSys &S = A.sys[SYS_U2];
+                cx<T> vec;
+                if (S.isbc[j] != 0.0) {
+                    vec = is00 ? make_cx<T>(T(S.bc00[j]), 0) : zero;
+                } else {
+                    cx<T> cr = u2;
+                    if (A.rk_gt1) cr = axpy(temp3, P.cf2[o], cr);
+                    cr = cr - scale(divr(cp0 - cpm, dyj), temp4);
+                    cr = axpy(temp5, cf2n, cr);
+                    const cx<T> d2 = divr(divr(u2p - u2, dyfj) - divr(u2 - u2m, dyfm), dyj);
+                    vec = axpy(temp1 * nuv, d2, cr);
+                }
+                g2 = (j == 0) ? vec : axpy(T(S.mult[j]), g2, vec);
+                if (inr) P.cf2[o] = cf2n;
+            }
+            // ---- scalars ----
+            for (int n = 0; n < nth; ++n) {
+                const bool inr = (j >= A.jstart_th[n] && j <= A.jend_th[n]);
+                cx<T> cfn = zero;
+                if (inr) {
+                    const T vp = T(A.nu / A.pr[n]) * pw;
+                    cfn = make_cx<T>(-(vp * th0[n].re), -(vp * th0[n].im));
+                    cfn = cfn - scale(u1, T(A.drhodx[n])) - scale(u3, T(A.drhodz[n]));
+                    cfn = cfn + P.nl[3 + n][o];
+                }
+                const CnSys &S = A.sys[SYS_TH0 + n];
+                cx<T> vec;
+                if (S.isbc[j] != 0.0) {
+                    vec = is00 ? make_cx<T>(T(S.bc00[j]), 0) : zero;
+                } else {
+                    cx<T> cr = th0[n];
+                    if (A.rk_gt1) cr = axpy(temp3, P.cfth[n][o], cr);
+                    cr = axpy(temp5, cfn, cr);
+                    const cx<T> d2 = divr(divr(thp[n] - th0[n], dyp) - divr(th0[n] - thm[n], dyj), dyfj);
+                    vec = axpy((temp1 / T(A.pr[n])) * nuv, d2, cr);
+                }
+                gth[n] = (j == 0) ? vec : axpy(T(S.mult[j]), gth[n], vec);
+                if (inr) P.cfth[n][o] = cfn;
+            }
+            // forward-eliminated right-hand sides overwrite the consumed NL rows
+            P.nl[0][o] = g1;
+            P.nl[1][o] = g2;
+            P.nl[2][o] = g3;
+            for (int n = 0; n < nth; ++n) P.nl[3 + n][o] = gth[n];
+            // roll
+            u1m = u1; u2m = u2; u3m = u3; cpm = cp0;
+            u1 = u1p; u2 = u2p; u3 = u3p;
+            if (hasp) cp0 = P.cp[o + ps];
+            for (int n = 0; n < nth; ++n) { thm[n] = th0[n]; th0[n] = thp[n]; }
+        }
+        // --- sweep B: back substitution, copy-out rows only ---
+        {
+            const int nsys = 3 + nth;
+            for (int q = 0; q < nsys; ++q) {
+                const CnSys &S = A.sys[q];
+                cx<T> *fld = (q == SYS_U1) ? P.cu1 : (q == SYS_U3) ? P.cu3 : (q == SYS_U2) ? P.cu2 : P.cth[q - SYS_TH0];
+                cx<T> *gg = (q == SYS_U1) ? P.nl[0] : (q == SYS_U3) ? P.nl[2] : (q == SYS_U2) ? P.nl[1] : P.nl[q];
+                cx<T> x = zero;
+                for (int j = NY + 1; j >= 0; --j) {
+                    const long o = m + (long)j * ps;
+                    const cx<T> gj = gg[o];
+                    const T piv = T(S.piv[j]);
+                    if (j == NY + 1) {
+                        x = divr(gj, piv);
+                    } else if (piv != T(0)) {
+                        x = divr(gj - scale(x, T(S.up[j])), piv);
+                    } else {
+                        x = zero;
+                    }
+                    if (j >= S.lo && j <= S.hi) fld[o] = x;
+                }
+            }
+        }
+    }
+
+    // --- REM_DIV_CHAN (for/channel.f:1103-1214) per column ---
+    // sweep C: forward elimination j = 2..NY of the Poisson system
+    {
+        T bprev = T(1);                 // row 1: D = 1
+        cx<T> gprev = zero;             // row 1: VEC = 0
+        const T u1row = is00 ? T(0) : T(-1);
+        P.piv[m + ps] = bprev;
+        P.pois[m + ps] = gprev;
+        cx<T> c2j = P.cu2[m + 2 * ps];
+        for (int j = 2; j <= NY; ++j) {
+            const long o = m + (long)j * ps;
+            T L, D, Uprev;
+            cx<T> vec;
+            Uprev = (j - 1 == 1) ? u1row : T(1.0 / (DY[j] * DYF[j - 1]));
+            const cx<T> c2p = P.cu2[o + ps];
+            if (j == NY) {
+                L = T(1); D = T(-1); vec = zero;
+            } else {
+                L = T(1.0 / (DY[j] * DYF[j]));
+                D = -kx2 - kz2 - T(1.0 / (DY[j + 1] * DYF[j])) - T(1.0 / (DY[j] * DYF[j]));
+                const cx<T> a = ikmul(kx, P.cu1[o]);
+                const cx<T> c = ikmul(kz, P.cu3[o]);
+                vec = (a + divr(c2p - c2j, T(DYF[j]))) + c;
+            }
+            const T mult = (bprev != T(0)) ? -L / bprev : T(0);
+            const T bj = D + mult * Uprev;
+            const cx<T> gj = axpy(mult, gprev, vec);
+            P.piv[o] = bj;
+            P.pois[o] = gj;
+            bprev = bj;
+            gprev = gj;
+            c2j = c2p;
+        }
+    }
+    // sweep D: back substitution and projection (j = NY..1)
+    {
+        const T u1row = is00 ? T(0) : T(-1);
+        const T temp4 = T(A.temp4);
+        cx<T> phip = zero;
+        for (int j = NY; j >= 1; --j) {
+            const long o = m + (long)j * ps;
+            const T piv = P.piv[o];
+            const cx<T> gj = P.pois[o];
+            cx<T> phi;
+            if (j == NY) {
+                phi = (piv != T(0)) ? divr(gj, piv) : zero;
+            } else {
+                const T U = (j == 1) ? u1row : T(1.0 / (DY[j + 1] * DYF[j]));
+                phi = (piv != T(0)) ? divr(gj - scale(phip, U), piv) : zero;
+            }
+            P.cu1[o] = P.cu1[o] - ikmul(kx, phi);
+            P.cu3[o] = P.cu3[o] - ikmul(kz, phi);
+            if (j + 1 <= NY) P.cu2[o + ps] = P.cu2[o + ps] - divr(phip - phi, T(DY[j + 1]));
+            if (A.mode == 0 && j >= A.jstart && j <= A.jend) P.cp[o] = P.cp[o] + divr(phi, temp4);
+            phip = phi;
+        }
+    }
+}
+
+// -------------------------------------------------------------------------
+// Standalone transforms on Fortran-layout planes (host entry points)
+// -------------------------------------------------------------------------
+
+// R2C along x of the physical rows of Fortran-layout planes; output
+// xt[j][z][0..NKX] scaled 1/NX (for/mpi.f:1547-1561). Pairs of rows per FFT.
+template <typename T, int E>
+__global__ void k_x_r2c(FftLen fx, const cx<T> *__restrict__ tw, const double *__restrict__ phys, int nzp2,
+                        cx<T> *__restrict__ xt, int mk, int nz, int j0)
+{
+    extern __shared__ unsigned char smem_raw[];
+    cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
+    const int nx = fx.n;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int j = j0 + blockIdx.y;
+    const int npair = blockDim.x / fx.t;
+    const int z0 = blockIdx.x * 2 * npair;
+    const long rowlen = nx + 2;
+    const double *pl = phys + (long)j * nzp2 * rowlen;
+    for (int idx = tid; idx < npair * nx; idx += nthr) {
+        const int p = idx / nx, x = idx - p * nx;
+        const int za = z0 + 2 * p, zb = za + 1;
+        const T a = za < nz ? T(pl[(long)za * rowlen + x]) : T(0);
+        const T b = zb < nz ? T(pl[(long)zb * rowlen + x]) : T(0);
+        lds[idx] = make_cx<T>(a, b);
+    }
+    __syncthreads();
+    const int p = tid / fx.t, th = tid - p * fx.t;
+    fft_lds<T, E, -1>(lds + p * nx, 1, fx, th, p < npair, tw);
+    const T h = T(0.5) / T(nx);
+    for (int idx = tid; idx < npair * mk; idx += nthr) {
+        const int pp = idx / mk, k = idx - pp * mk;
+        const cx<T> Z = lds[pp * nx + k];
+        const cx<T> Zn = lds[pp * nx + (k == 0 ? 0 : nx - k)];
+        const int za = z0 + 2 * pp, zb = za + 1;
+        cx<T> *o = xt + ((long)j * nz) * mk;
+        if (za < nz) o[(long)za * mk + k] = make_cx<T>((Z.re + Zn.re) * h, (Z.im - Zn.im) * h);
+        if (zb < nz) o[(long)zb * mk + k] = make_cx<T>((Z.im + Zn.im) * h, -(Z.re - Zn.re) * h);
+    }
+}
+
+// z-forward of xt columns, scale 1/NZ, pack kept slots into Fortran layout;
+// every other slot of the plane is set to zero.
+template <typename T, int E>
+__global__ void k_zfwd_pack(FftLen fz, const cx<T> *__restrict__ tw, const cx<T> *__restrict__ xt, int mk,
+                            double *__restrict__ fort, int nxp, int nkz, int tnkz, int j0, int W)
+{
+    extern __shared__ unsigned char smem_raw[];
+    cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
+    const int nz = fz.n;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int i0 = blockIdx.x * W;
+    const int j = j0 + blockIdx.y;
+    const cx<T> *src = xt + (long)j * nz * mk;
+    for (int idx = tid; idx < W * nz; idx += nthr) {
+        const int z = idx / W, b = idx - z * W;
+        const int i = i0 + b;
+        lds[idx] = (i < mk) ? src[(long)z * mk + i] : make_cx<T>(0, 0);
+    }
+    __syncthreads();
+    const int b = tid / fz.t, th = tid - b * fz.t;
+    fft_lds<T, E, -1>(lds + b, W, fz, th, b < W, tw);
+    // Fortran complex view (0:NXP, 0:NZ+1) per plane
+    double *pl = fort + (long)j * (long)(nxp + 1) * (nz + 2) * 2;
+    const T inv = T(1) / T(nz);
+    for (int idx = tid; idx < W * (nz + 2); idx += nthr) {
+        const int K = idx / W, bb = idx - K * W;
+        const int i = i0 + bb;
+        if (i > nxp) continue;
+        cx<T> v = make_cx<T>(0, 0);
+        if (i < mk && K <= tnkz) v = scale(lds[kzslot_to_z(K, nkz, tnkz, nz) * W + bb], inv);
+        const long e = ((long)K * (nxp + 1) + i) * 2;
+        pl[e] = double(v.re);
+        pl[e + 1] = double(v.im);
+    }
+}
+
+// C2R along x: zt[j][z][0..mkin-1] (Nyquist zeroed, Im of DC ignored) into the
+// physical rows of Fortran-layout planes (for/mpi.f:1643-1672). The x pads of
+// rows 0..NZ-1 are set to zero; rows NZ, NZ+1 are not touched.
+template <typename T, int E>
+__global__ void k_x_c2r(FftLen fx, const cx<T> *__restrict__ tw, const cx<T> *__restrict__ zt, int mkin,
+                        double *__restrict__ fort, int nz, int j0)
+{
+    extern __shared__ unsigned char smem_raw[];
+    cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
+    const int nx = fx.n;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int j = j0 + blockIdx.y;
+    const int npair = blockDim.x / fx.t;
+    const int z0 = blockIdx.x * 2 * npair;
+    const cx<T> *src = zt + (long)j * nz * mkin;
+    for (int idx = tid; idx < npair * nx; idx += nthr) {
+        const int p = idx / nx, k = idx - p * nx;
+        const int za = z0 + 2 * p, zb = za + 1;
+        cx<T> A = make_cx<T>(0, 0), B = make_cx<T>(0, 0);
+        int kk = -1;
+        bool cj = false;
+        if (k < mkin && k < nx / 2) kk = k;
+        else if (k > nx / 2 && nx - k < mkin) { kk = nx - k; cj = true; }
+        if (kk >= 0) {
+            if (za < nz) A = src[(long)za * mkin + kk];
+            if (zb < nz) B = src[(long)zb * mkin + kk];
+            if (kk == 0) { A.im = 0; B.im = 0; }
+            if (cj) { A.im = -A.im; B.im = -B.im; }
+        }
+        lds[idx] = make_cx<T>(A.re - B.im, A.im + B.re);
+    }
+    __syncthreads();
+    const int p = tid / fx.t, th = tid - p * fx.t;
+    fft_lds<T, E, +1>(lds + p * nx, 1, fx, th, p < npair, tw);
+    const long rowlen = nx + 2;
+    double *pl = fort + (long)j * (nz + 2) * rowlen;
+    for (int idx = tid; idx < npair * (nx + 2); idx += nthr) {
+        const int pp = idx / (nx + 2), x = idx - pp * (nx + 2);
+        const int za = z0 + 2 * pp, zb = za + 1;
+        const cx<T> v = x < nx ? lds[pp * nx + x] : make_cx<T>(0, 0);
+        if (za < nz) pl[(long)za * rowlen + x] = double(v.re);
+        if (zb < nz) pl[(long)zb * rowlen + x] = double(v.im);
+    }
+}
+
+// Fortran complex layout -> kept-mode layout.
+__global__ void k_unpack_fortran(const double *__restrict__ fort, cx<double> *__restrict__ kept, int nxp, int nz,
+                                 int mk, int mz, int j0)
+{
+    const int j = j0 + blockIdx.y;
+    const long plane = (long)mk * mz;
+    for (long m = (long)blockIdx.x * blockDim.x + threadIdx.x; m < plane; m += (long)gridDim.x * blockDim.x) {
+        const int s = (int)(m / mk), i = (int)(m - (long)s * mk);
+        const double *p = fort + (((long)j * (nz + 2) + s) * (nxp + 1) + i) * 2;
+        kept[(long)j * plane + m] = make_cx<double>(p[0], p[1]);
+    }
+}
+
+// APPLY_BC_VEL_LOWER / UPPER (for/channel.f:2285-2492) on kept-mode fields.
+// One thread per mode (i,K) of the rows 0, 1, NY, NY+1; the (0,0) mode and
+// the V code-0 upper rule are handled by the thread of mode 0.
+__global__ void k_apply_bc_vel(cx<double> *cu1, cx<double> *cu2, cx<double> *cu3, long ps, int ny, const int *codes,
+                               const double *c1, const double *dy, const double *dyf)
+{
+    const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= ps) return;
+    const bool is00 = (m == 0);
+    auto at = [&](cx<double> *f, int j) -> cx<double> & { return f[m + (long)j * ps]; };
+    const cx<double> z0 = make_cx<double>(0, 0);
+    // codes: u_min, v_min, w_min, u_max, v_max, w_max
+    cx<double> *hf[2] = {cu1, cu3};
+    const int hmin[2] = {codes[0], codes[2]}, hmax[2] = {codes[3], codes[5]};
+    const double cmin[2] = {c1[0], c1[2]}, cmax[2] = {c1[3], c1[5]};
+    for (int c = 0; c < 2; ++c) {
+        cx<double> *f = hf[c];
+        if (hmin[c] == 0) {
+            at(f, 0) = z0; at(f, 1) = z0;
+            if (is00) { at(f, 1) = make_cx<double>(cmin[c], 0); at(f, 0) = make_cx<double>(cmin[c], 0); }
+        } else {
+            at(f, 1) = at(f, 2); at(f, 0) = at(f, 1);
+            if (is00) {
+                at(f, 1) = at(f, 2) - make_cx<double>(dy[2] * cmin[c], 0);
+                at(f, 0) = at(f, 1) - make_cx<double>(dy[1] * cmin[c], 0);
+            }
+        }
+    }
+    if (codes[1] == 0) {
+        at(cu2, 1) = make_cx<double>(2.0 * c1[1], 0) - at(cu2, 2);
+        at(cu2, 0) = at(cu2, 1);
+    } else if (codes[1] == 1) {
+        at(cu2, 1) = at(cu2, 2); at(cu2, 0) = at(cu2, 1);
+        if (is00) {
+            at(cu2, 1) = at(cu2, 2) - make_cx<double>(dyf[1] * c1[1], 0);
+            at(cu2, 0) = at(cu2, 1) - make_cx<double>(dyf[1] * c1[1], 0);
+        }
+    } else if (codes[1] == 2) {
+        if (is00) at(cu2, 1) = make_cx<double>(-at(cu2, 2).re, -at(cu2, 2).im);
+    }
+    for (int c = 0; c < 2; ++c) {
+        cx<double> *f = hf[c];
+        if (hmax[c] == 0) {
+            at(f, ny) = z0; at(f, ny + 1) = z0;
+            if (is00) { at(f, ny) = make_cx<double>(cmax[c], 0); at(f, ny + 1) = make_cx<double>(cmax[c], 0); }
+            if (c == 1 && is00) at(f, ny + 1) = z0;  // for/channel.f:2445
+        } else {
+            at(f, ny) = at(f, ny - 1); at(f, ny + 1) = at(f, ny);
+            if (is00) {
+                at(f, ny) = at(f, ny - 1) + make_cx<double>(dy[ny] * cmax[c], 0);
+                at(f, ny + 1) = at(f, ny) + make_cx<double>(dy[ny] * cmax[c], 0);
+            }
+        }
+    }
+    if (codes[4] == 0) {
+        if (is00) at(cu2, ny + 1) = make_cx<double>(2.0 * c1[4], 0) - at(cu2, ny);
+    } else if (codes[4] == 1) {
+        at(cu2, ny) = at(cu2, ny - 1); at(cu2, ny + 1) = at(cu2, ny);
+        if (is00) at(cu2, ny + 1) = at(cu2, ny) + make_cx<double>(dy[ny] * c1[4], 0);
+    } else if (codes[4] == 2) {
+        if (is00) at(cu2, ny + 1) = make_cx<double>(-at(cu2, ny).re, -at(cu2, ny).im);
+    }
+}
+
+// -------------------------------------------------------------------------
+// launchers
+// -------------------------------------------------------------------------
+
+static inline int chunk_w(int t, int threads) { int w = threads / t; return w < 1 ? 1 : w; }
+
+template <typename T, int E>
+static void zinv_e(hipStream_t s, const Geo &g, const Fft<T> &fz, const cx<T> *in, long in_fs, long in_ps,
+                   int in_mk, int ncols, cx<T> *out, long out_fs, long out_ps, int out_mk, int j0, int nplanes,
+                   int nfields)
+{
+    const int W = chunk_w(fz.len.t, 256);
+    dim3 grid((ncols + W - 1) / W, nplanes, nfields);
+    const size_t lds = sizeof(cx<T>) * W * fz.len.n;
+    hipLaunchKernelGGL((k_zinv<T, E>), grid, dim3(W * fz.len.t), lds, s, fz.len, fz.tw, in, in_fs, in_ps, in_mk,
+                       ncols, out, out_fs, out_ps, out_mk, j0, g.nkz, g.tnkz, W);
+}
+
+#define DGPU_DISPATCH_E(E_, CALL)                                  \
+    switch (E_) {                                                  \
+    case 4: { constexpr int EE = 4; CALL; } break;                 \
+    case 6: { constexpr int EE = 6; CALL; } break;                 \
+    case 8: { constexpr int EE = 8; CALL; } break;                 \
+    case 12: { constexpr int EE = 12; CALL; } break;               \
+    case 16: { constexpr int EE = 16; CALL; } break;               \
+    default: break;                                                \
+    }
+
+template <typename T>
+void launch_zinv(hipStream_t s, const Geo &g, const Fft<T> &fz, const cx<T> *in, long in_fs, long in_ps, int in_mk,
+                 int ncols, cx<T> *out, long out_fs, long out_ps, int out_mk, int j0, int nplanes, int nfields)
+{
+    DGPU_DISPATCH_E(fz.len.e, (zinv_e<T, EE>(s, g, fz, in, in_fs, in_ps, in_mk, ncols, out, out_fs, out_ps, out_mk,
+                                             j0, nplanes, nfields)));
+}
+
+template <typename T, int E, int NTH>
+static void xfused_e(hipStream_t s, const Geo &g, const Fft<T> &fx, const cx<T> *zt, long zt_fs, cx<T> *wt,
+                     long wt_fs, const double *dy, const double *dyf, const double *kx, int jlo, int jhi)
+{
+    using C = XCounts<NTH>;
+    const int chunk = 16;
+    const int nchunks = (jhi - jlo + 1 + chunk - 1) / chunk;
+    int threads = fx.len.t * C::npo;
+    threads = ((threads + 63) / 64) * 64;
+    if (threads < 64) threads = 64;
+    const size_t lds = sizeof(cx<T>) * fx.len.n * (3 * C::npi + C::npo);
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_xfused<T, E, NTH>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        attr = true;
+    }
+    const long zt_ps = (long)g.nz * g.mk;
+    hipLaunchKernelGGL((k_xfused<T, E, NTH>), dim3(g.nz, nchunks), dim3(threads), lds, s, fx.len, fx.tw, g.mk, zt,
+                       zt_fs, zt_ps, wt, wt_fs, zt_ps, dy, dyf, kx, jlo, jhi, chunk);
+}
+
+template <typename T>
+void launch_xfused(hipStream_t s, const Geo &g, const Fft<T> &fx, const cx<T> *zt, long zt_fs, cx<T> *wt,
+                   long wt_fs, const double *dy, const double *dyf, const double *kx, int jlo, int jhi)
+{
+    if (g.nth == 0) {
+        DGPU_DISPATCH_E(fx.len.e, (xfused_e<T, EE, 0>(s, g, fx, zt, zt_fs, wt, wt_fs, dy, dyf, kx, jlo, jhi)));
+    } else {
+        DGPU_DISPATCH_E(fx.len.e, (xfused_e<T, EE, 1>(s, g, fx, zt, zt_fs, wt, wt_fs, dy, dyf, kx, jlo, jhi)));
+    }
+}
+
+template <typename T, int E>
+static void zfwd_nl_e(hipStream_t s, const Geo &g, const Fft<T> &fz, const cx<T> *wt, long wt_fs,
+                      cx<T> *const *nl, const double *kz, int jlo, int jhi, int nout)
+{
+    const int W = chunk_w(fz.len.t, 256);
+    dim3 grid((g.mk + W - 1) / W, jhi - jlo + 1, nout);
+    const size_t lds = sizeof(cx<T>) * W * fz.len.n;
+    const long wt_ps = (long)g.nz * g.mk;
+    hipLaunchKernelGGL((k_zfwd_nl<T, E>), grid, dim3(W * fz.len.t), lds, s, fz.len, fz.tw, wt, wt_fs, wt_ps, g.mk,
+                       nl, g.ps, kz, jlo, g.nkz, g.tnkz, W);
+}
+
+template <typename T>
+void launch_zfwd_nl(hipStream_t s, const Geo &g, const Fft<T> &fz, const cx<T> *wt, long wt_fs, cx<T> *const *nl,
+                    const double *kz, int jlo, int jhi, int nout)
+{
+    DGPU_DISPATCH_E(fz.len.e, (zfwd_nl_e<T, EE>(s, g, fz, wt, wt_fs, nl, kz, jlo, jhi, nout)));
+}
+
+template <typename T>
+void launch_columns(hipStream_t s, const ColArgs &a, const ColPtrs<T> &p)
+{
+    const int threads = 64;
+    const long blocks = (a.g.ps + threads - 1) / threads;
+    hipLaunchKernelGGL((k_columns<T>), dim3(blocks), dim3(threads), 0, s, a, p);
+}
+
+template <typename T, int E>
+static void x_r2c_e(hipStream_t s, const Geo &g, const Fft<T> &fx, const double *phys, cx<T> *xt, int j0,
+                    int nplanes)
+{
+    int npair = 256 / fx.len.t;
+    if (npair < 1) npair = 1;
+    const int zper = 2 * npair;
+    dim3 grid((g.nz + zper - 1) / zper, nplanes);
+    hipLaunchKernelGGL((k_x_r2c<T, E>), grid, dim3(npair * fx.len.t), sizeof(cx<T>) * npair * fx.len.n, s, fx.len,
+                       fx.tw, phys, g.nz + 2, xt, g.mk, g.nz, j0);
+}
+
+template <typename T>
+void launch_x_r2c(hipStream_t s, const Geo &g, const Fft<T> &fx, const double *phys, cx<T> *xt, int j0, int nplanes)
+{
+    DGPU_DISPATCH_E(fx.len.e, (x_r2c_e<T, EE>(s, g, fx, phys, xt, j0, nplanes)));
+}
+
+template <typename T, int E>
+static void zfwd_pack_e(hipStream_t s, const Geo &g, const Fft<T> &fz, const cx<T> *xt, double *fort, int j0,
+                        int nplanes)
+{
+    const int W = chunk_w(fz.len.t, 256);
+    const int ncols = g.nx / 2 + 1;  // 0..NXP, zero beyond NKX
+    dim3 grid((ncols + W - 1) / W, nplanes);
+    hipLaunchKernelGGL((k_zfwd_pack<T, E>), grid, dim3(W * fz.len.t), sizeof(cx<T>) * W * fz.len.n, s, fz.len, fz.tw,
+                       xt, g.mk, fort, g.nx / 2, g.nkz, g.tnkz, j0, W);
+}
+
+template <typename T>
+void launch_zfwd_pack(hipStream_t s, const Geo &g, const Fft<T> &fz, const cx<T> *xt, double *fort, int j0,
+                      int nplanes)
+{
+    DGPU_DISPATCH_E(fz.len.e, (zfwd_pack_e<T, EE>(s, g, fz, xt, fort, j0, nplanes)));
+}
+
+template <typename T, int E>
+static void x_c2r_e(hipStream_t s, const Geo &g, const Fft<T> &fx, const cx<T> *zt, int mkin, double *fort, int j0,
+                    int nplanes)
+{
+    int npair = 256 / fx.len.t;
+    if (npair < 1) npair = 1;
+    const int zper = 2 * npair;
+    dim3 grid((g.nz + zper - 1) / zper, nplanes);
+    hipLaunchKernelGGL((k_x_c2r<T, E>), grid, dim3(npair * fx.len.t), sizeof(cx<T>) * npair * fx.len.n, s, fx.len,
+                       fx.tw, zt, mkin, fort, g.nz, j0);
+}
+
+template <typename T>
+void launch_x_c2r(hipStream_t s, const Geo &g, const Fft<T> &fx, const cx<T> *zt, int mkin, double *fort, int j0,
+                  int nplanes)
+{
+    DGPU_DISPATCH_E(fx.len.e, (x_c2r_e<T, EE>(s, g, fx, zt, mkin, fort, j0, nplanes)));
+}
+
+void launch_unpack_fortran(hipStream_t s, const Geo &g, const double *fort, cx<double> *kept, int j0, int nplanes)
+{
+    const long plane = g.ps;
+    const int threads = 256;
+    int bx = (int)((plane + threads - 1) / threads);
+    hipLaunchKernelGGL(k_unpack_fortran, dim3(bx, nplanes), dim3(threads), 0, s, fort, kept, g.nx / 2, g.nz, g.mk,
+                       g.mz, j0);
+}
+
+void launch_apply_bc_vel(hipStream_t s, const Geo &g, cx<double> *cu1, cx<double> *cu2, cx<double> *cu3,
+                         const int *codes, const double *c1, const double *dy, const double *dyf)
+{
+    const int threads = 256;
+    const long blocks = (g.ps + threads - 1) / threads;
+    hipLaunchKernelGGL(k_apply_bc_vel, dim3(blocks), dim3(threads), 0, s, cu1, cu2, cu3, g.ps, g.ny, codes, c1, dy,
+                       dyf);
+}
+
+// explicit instantiations
+template void launch_zinv<double>(hipStream_t, const Geo &, const Fft<double> &, const cx<double> *, long, long, int,
+                                  int, cx<double> *, long, long, int, int, int, int);
+template void launch_xfused<double>(hipStream_t, const Geo &, const Fft<double> &, const cx<double> *, long,
+                                    cx<double> *, long, const double *, const double *, const double *, int, int);
+template void launch_zfwd_nl<double>(hipStream_t, const Geo &, const Fft<double> &, const cx<double> *, long,
+                                     cx<double> *const *, const double *, int, int, int);
+template void launch_columns<double>(hipStream_t, const ColArgs &, const ColPtrs<double> &);
+template void launch_x_r2c<double>(hipStream_t, const Geo &, const Fft<double> &, const double *, cx<double> *, int,
+                                   int);
+template void launch_zfwd_pack<double>(hipStream_t, const Geo &, const Fft<double> &, const cx<double> *, double *,
+                                       int, int);
+template void launch_x_c2r<double>(hipStream_t, const Geo &, const Fft<double> &, const cx<double> *, int, double *,
+                                   int, int);
+
+}  // namespace dgpu

@@ -1,0 +1,239 @@
+"""Python host mirror of the reference call surface over libdiablo_gpu.so.
+
+``GpuChannel`` exposes the reference's subroutine names (RK_CHAN_1,
+REM_DIV_CHAN, FFT_XZ_TO_FOURIER, FFT_XZ_TO_PHYSICAL, APPLY_BC_VEL) with the
+same argument meaning, and goes through the C-ABI declared in
+include/diablo_gpu.h.  There is no CPU fallback: if the HIP library is
+missing or cannot create a context, construction raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .case import MAX_TH, ChannelConfig, grid_gy, rkw3_table
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdiablo_gpu.so")
+
+I32x = C.c_int32 * MAX_TH
+F64x = C.c_double * MAX_TH
+DP = C.POINTER(C.c_double)
+
+# field ids (include/diablo_gpu.h)
+CU1, CU2, CU3, CP, CF1, CF2, CF3 = 0, 1, 2, 3, 4, 5, 6
+CTH0, CFTH0 = 16, 24
+KERNEL_KINDS = ("zinv", "xfused", "zfwd_nl", "columns")
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("n_th", C.c_int32),
+        ("precision", C.c_int32), ("f_type", C.c_int32),
+        ("u_bc_ymin", C.c_int32), ("v_bc_ymin", C.c_int32), ("w_bc_ymin", C.c_int32),
+        ("u_bc_ymax", C.c_int32), ("v_bc_ymax", C.c_int32), ("w_bc_ymax", C.c_int32),
+        ("th_bc_ymin", I32x), ("th_bc_ymax", I32x),
+        ("nu", C.c_double), ("beta", C.c_double), ("lx", C.c_double), ("ly", C.c_double),
+        ("lz", C.c_double), ("nu_v_scale", C.c_double),
+        ("i_ro", C.c_double), ("grav_x", C.c_double), ("grav_y", C.c_double), ("grav_z", C.c_double),
+        ("px0", C.c_double), ("omega0", C.c_double), ("amp_omega0", C.c_double),
+        ("u_bc_ymin_c1", C.c_double), ("v_bc_ymin_c1", C.c_double), ("w_bc_ymin_c1", C.c_double),
+        ("u_bc_ymax_c1", C.c_double), ("v_bc_ymax_c1", C.c_double), ("w_bc_ymax_c1", C.c_double),
+        ("th_bc_ymin_c1", F64x), ("th_bc_ymax_c1", F64x),
+        ("ri", F64x), ("pr", F64x), ("drhodx", F64x), ("drhodz", F64x),
+        ("gy", DP), ("gyf", DP), ("dy", DP), ("dyf", DP),
+        ("kx", DP), ("kz", DP), ("kx2", DP), ("kz2", DP),
+    ]
+
+
+EXPORTS = (
+    "dgpu_abi_version", "dgpu_build_info", "dgpu_create", "dgpu_destroy", "dgpu_last_error",
+    "dgpu_set_field", "dgpu_get_field", "dgpu_rk_chan_1", "dgpu_rem_div_chan", "dgpu_apply_bc_vel",
+    "dgpu_fft_xz_to_fourier", "dgpu_fft_xz_to_physical", "dgpu_synchronize",
+    "dgpu_set_kernel_timing", "dgpu_kernel_times",
+)
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Loads libdiablo_gpu.so and declares its signatures (no GPU needed)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with `python -m diablo_amd.build`")
+    L = C.CDLL(path)
+    vp, i32 = C.c_void_p, C.c_int32
+    L.dgpu_abi_version.restype = i32
+    L.dgpu_build_info.restype = C.c_char_p
+    L.dgpu_create.restype = i32
+    L.dgpu_create.argtypes = [C.POINTER(Params), C.POINTER(vp)]
+    L.dgpu_destroy.restype = i32
+    L.dgpu_destroy.argtypes = [vp]
+    L.dgpu_last_error.restype = C.c_char_p
+    L.dgpu_last_error.argtypes = [vp]
+    for name in ("dgpu_set_field", "dgpu_get_field"):
+        getattr(L, name).restype = i32
+        getattr(L, name).argtypes = [vp, i32, DP]
+    L.dgpu_rk_chan_1.restype = i32
+    L.dgpu_rk_chan_1.argtypes = [vp, i32, C.c_double, DP, DP, DP]
+    for name in ("dgpu_rem_div_chan", "dgpu_apply_bc_vel", "dgpu_synchronize"):
+        getattr(L, name).restype = i32
+        getattr(L, name).argtypes = [vp]
+    for name in ("dgpu_fft_xz_to_fourier", "dgpu_fft_xz_to_physical"):
+        getattr(L, name).restype = i32
+        getattr(L, name).argtypes = [vp, DP, i32, i32]
+    L.dgpu_set_kernel_timing.restype = i32
+    L.dgpu_set_kernel_timing.argtypes = [vp, i32]
+    L.dgpu_kernel_times.restype = i32
+    L.dgpu_kernel_times.argtypes = [vp, DP, C.POINTER(C.c_int64), i32]
+    _lib = L
+    return L
+
+
+def _dp(a: np.ndarray):
+    if not (a.dtype in (np.float64, np.complex128) and a.flags.c_contiguous):
+        raise ValueError("expected a C-contiguous float64 array in Fortran field layout")
+    return a.ctypes.data_as(DP)
+
+
+def params_of(cfg: ChannelConfig, gy_1: np.ndarray) -> Params:
+    p = Params()
+    for name, _ in Params._fields_:
+        if name in ("gy", "gyf", "dy", "dyf", "kx", "kz", "kx2", "kz2", "precision"):
+            continue
+        v = getattr(cfg, name)
+        if isinstance(v, list):
+            arr = getattr(p, name)
+            for n, x in enumerate(v[:MAX_TH]):
+                arr[n] = x
+        else:
+            setattr(p, name, v)
+    p.precision = 0
+    p.gy = _dp(gy_1)
+    return p
+
+
+class DgpuError(RuntimeError):
+    pass
+
+
+class GpuChannel:
+    """The channel state on one MI355X, driven through the C-ABI."""
+
+    def __init__(self, cfg: ChannelConfig, gy_1: np.ndarray | None = None):
+        cfg.validate()
+        self.cfg = cfg
+        self.L = load_library()
+        self._gy = np.ascontiguousarray(grid_gy(cfg) if gy_1 is None else gy_1, dtype=np.float64)
+        self._p = params_of(cfg, self._gy)
+        h = C.c_void_p()
+        rc = self.L.dgpu_create(C.byref(self._p), C.byref(h))
+        if rc != 0 or not h.value:
+            raise DgpuError(f"dgpu_create failed ({rc}) for {cfg.nx}x{cfg.ny}x{cfg.nz}")
+        self.h = h
+        self.time = 0.0
+        self.h_bar, self.beta_bar, self.zeta_bar = (np.ascontiguousarray(a) for a in rkw3_table(cfg.delta_t))
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.L.dgpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc: int, what: str):
+        if rc != 0:
+            msg = self.L.dgpu_last_error(self.h)
+            raise DgpuError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    # -- state exchange (Fortran-layout host arrays) --
+    def set_field(self, fid: int, a: np.ndarray):
+        self._chk(self.L.dgpu_set_field(self.h, fid, _dp(a)), "dgpu_set_field")
+
+    def get_field(self, fid: int) -> np.ndarray:
+        a = np.zeros(self.cfg.field_shape())
+        self._chk(self.L.dgpu_get_field(self.h, fid, _dp(a)), "dgpu_get_field")
+        return a
+
+    def state(self) -> dict:
+        d = {"CU1": self.get_field(CU1), "CU2": self.get_field(CU2), "CU3": self.get_field(CU3),
+             "CP": self.get_field(CP), "CF1": self.get_field(CF1), "CF2": self.get_field(CF2),
+             "CF3": self.get_field(CF3)}
+        for n in range(self.cfg.n_th):
+            d[f"CTH{n}"] = self.get_field(CTH0 + n)
+            d[f"CFTH{n}"] = self.get_field(CFTH0 + n)
+        return d
+
+    def load_state(self, d: dict):
+        for k, f in (("CU1", CU1), ("CU2", CU2), ("CU3", CU3), ("CP", CP), ("CF1", CF1), ("CF2", CF2),
+                     ("CF3", CF3)):
+            if k in d:
+                self.set_field(f, np.ascontiguousarray(d[k]))
+        for n in range(self.cfg.n_th):
+            self.set_field(CTH0 + n, np.ascontiguousarray(d[f"CTH{n}"]))
+            if f"CFTH{n}" in d:
+                self.set_field(CFTH0 + n, np.ascontiguousarray(d[f"CFTH{n}"]))
+
+    # -- the reference call surface --
+    def rk_chan_1(self, rk_step: int, time: float | None = None):
+        t = self.time if time is None else time
+        self._chk(self.L.dgpu_rk_chan_1(self.h, rk_step, t, _dp(self.h_bar), _dp(self.beta_bar),
+                                        _dp(self.zeta_bar)), "dgpu_rk_chan_1")
+
+    def step(self, n: int = 1):
+        """n time steps: RK_STEP = 1..3 each, then TIME += DELTA_T (for/diablo.f:70-109)."""
+        for _ in range(n):
+            for k in (1, 2, 3):
+                self.rk_chan_1(k)
+            self.time += self.cfg.delta_t
+
+    def rem_div_chan(self):
+        self._chk(self.L.dgpu_rem_div_chan(self.h), "dgpu_rem_div_chan")
+
+    def apply_bc_vel(self):
+        self._chk(self.L.dgpu_apply_bc_vel(self.h), "dgpu_apply_bc_vel")
+
+    def fft_xz_to_fourier(self, a: np.ndarray, jmin: int, jmax: int):
+        self._chk(self.L.dgpu_fft_xz_to_fourier(self.h, _dp(a), jmin, jmax), "dgpu_fft_xz_to_fourier")
+
+    def fft_xz_to_physical(self, a: np.ndarray, jmin: int, jmax: int):
+        self._chk(self.L.dgpu_fft_xz_to_physical(self.h, _dp(a), jmin, jmax), "dgpu_fft_xz_to_physical")
+
+    def synchronize(self):
+        self._chk(self.L.dgpu_synchronize(self.h), "dgpu_synchronize")
+
+    def set_kernel_timing(self, on: bool):
+        self._chk(self.L.dgpu_set_kernel_timing(self.h, 1 if on else 0), "dgpu_set_kernel_timing")
+
+    def kernel_times(self) -> dict:
+        ms = (C.c_double * 4)()
+        cnt = (C.c_int64 * 4)()
+        self._chk(self.L.dgpu_kernel_times(self.h, ms, cnt, 4), "dgpu_kernel_times")
+        return {k: (ms[i], cnt[i]) for i, k in enumerate(KERNEL_KINDS)}
+
+    # -- CREATE_FLOW_CHAN / CREATE_TH_CHAN on the device --
+    def initialize(self, u1, u2, u3, ths):
+        """Physical start fields -> device Fourier state, then APPLY_BC_VEL and
+        REM_DIV_CHAN (for/set_ics.f:110-168, 251-253)."""
+        cfg = self.cfg
+        for fid, a in ((CU1, u1), (CU2, u2), (CU3, u3)):
+            w = np.ascontiguousarray(a, dtype=np.float64).copy()
+            self.fft_xz_to_fourier(w, 0, cfg.ny + 1)
+            self.set_field(fid, w)
+        self.set_field(CP, np.zeros(cfg.field_shape()))
+        for n in range(cfg.n_th):
+            w = np.ascontiguousarray(ths[n], dtype=np.float64).copy()
+            self.fft_xz_to_fourier(w, 0, cfg.ny + 1)
+            self.set_field(CTH0 + n, w)
+        self.apply_bc_vel()
+        self.rem_div_chan()
+        self.synchronize()
+        self.time = 0.0

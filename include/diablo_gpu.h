@@ -1,0 +1,153 @@
+/*
+ * diablo_gpu.h -- C-ABI of the MI355X channel time-stepper (libdiablo_gpu.so).
+ *
+ * The library replaces the arithmetic of DIABLO 2.0's channel path
+ * (diablo_2.0/for/channel.f, for/mpi.f, for/fft.f) behind the reference's own
+ * call surface.  A Fortran bridge (diablo_amd/fortran/diablo_gpu_bridge.f)
+ * keeps the reference subroutine names RK_CHAN_1, REM_DIV_CHAN,
+ * POISSON_P_CHAN, FFT_XZ_TO_FOURIER and FFT_XZ_TO_PHYSICAL and forwards to the
+ * entry points below through ISO_C_BINDING.
+ *
+ * Conventions
+ *  - Plain C types only; every array argument is a host pointer into a
+ *    Fortran-layout array: a field is REAL*8 (0:NX+1, 0:NZ+1, 0:NY+1),
+ *    column-major, aliased with COMPLEX*16 (0:NX/2, 0:NZ+1, 0:NY+1)
+ *    (for/header:214-263).
+ *  - The Fourier state (CU1..3, CP, CTH, CF1..3, CFTH) lives in device
+ *    memory between calls.  dgpu_set_field / dgpu_get_field move it to and
+ *    from the Fortran arrays; only the dealiased slots I = 0..NKX,
+ *    K = 0..TNKZ are carried (every other slot reads back as zero).
+ *  - Every function returns 0 on success and a negative DGPU_E_* code on
+ *    failure; dgpu_last_error() gives the message.  No C++ exception crosses
+ *    the boundary.  Calls on one context are not thread-safe.
+ *  - Work is enqueued on the context's own HIP stream.  Functions that take or
+ *    return host arrays synchronise before returning; dgpu_rk_chan_1,
+ *    dgpu_rem_div_chan and dgpu_apply_bc_vel only enqueue (call
+ *    dgpu_synchronize before reading results on the host).
+ */
+#ifndef DIABLO_GPU_H
+#define DIABLO_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DGPU_ABI_VERSION 1
+#define DGPU_MAX_TH 4
+
+enum dgpu_status {
+    DGPU_OK = 0,
+    DGPU_E_ARG = -1,      /* bad argument or unsupported configuration */
+    DGPU_E_HIP = -2,      /* HIP runtime error (message has the HIP error) */
+    DGPU_E_NOMEM = -3,    /* device or host allocation failed */
+    DGPU_E_STATE = -4     /* call not valid in the current state */
+};
+
+enum dgpu_precision { DGPU_FP64 = 0 };
+
+/* Field identifiers for dgpu_set_field / dgpu_get_field. */
+enum dgpu_field {
+    DGPU_CU1 = 0, DGPU_CU2 = 1, DGPU_CU3 = 2, DGPU_CP = 3,
+    DGPU_CF1 = 4, DGPU_CF2 = 5, DGPU_CF3 = 6,
+    DGPU_CTH0 = 16,   /* CTH(:,:,:,n+1)  = DGPU_CTH0  + n */
+    DGPU_CFTH0 = 24   /* CFTH(:,:,:,n+1) = DGPU_CFTH0 + n */
+};
+
+/*
+ * Run parameters: the values of /INPUT/, /SCALAR_VARS/, /GEOPHYS_VARS/,
+ * /OSCILL_VARS/, /VARIABLE_VISC/ and /DRHODX_VARS/ (for/header:83-301) that the
+ * channel path reads, plus grid_def's sizes.
+ *
+ * gy points at GY(1..NY+1) (the /grids/y dataset, for/hdf5.f:656).  The
+ * library derives GYF, DY, DYF as ReadGridHDF5 + CREATE_GRID_CHAN do
+ * (for/hdf5.f:676-703, for/channel.f:1646-1652) and KX, KZ, KX2, KZ2 as
+ * INIT_FFT does (for/fft.f:72-105).  A caller that already holds those
+ * arrays (the Fortran bridge) may pass them instead: any non-NULL pointer in
+ * gyf/dy/dyf (0:NY+1), kx/kx2 (0:NX/2) or kz/kz2 (0:TNKZ) is used verbatim.
+ */
+typedef struct dgpu_params {
+    int32_t nx, ny, nz, n_th;
+    int32_t precision;
+    int32_t f_type;
+    int32_t u_bc_ymin, v_bc_ymin, w_bc_ymin;
+    int32_t u_bc_ymax, v_bc_ymax, w_bc_ymax;
+    int32_t th_bc_ymin[DGPU_MAX_TH], th_bc_ymax[DGPU_MAX_TH];
+    double nu, beta, lx, ly, lz, nu_v_scale;
+    double i_ro, grav_x, grav_y, grav_z;
+    double px0, omega0, amp_omega0;
+    double u_bc_ymin_c1, v_bc_ymin_c1, w_bc_ymin_c1;
+    double u_bc_ymax_c1, v_bc_ymax_c1, w_bc_ymax_c1;
+    double th_bc_ymin_c1[DGPU_MAX_TH], th_bc_ymax_c1[DGPU_MAX_TH];
+    double ri[DGPU_MAX_TH], pr[DGPU_MAX_TH];
+    double drhodx[DGPU_MAX_TH], drhodz[DGPU_MAX_TH];
+    const double *gy;
+    const double *gyf, *dy, *dyf;
+    const double *kx, *kz, *kx2, *kz2;
+} dgpu_params;
+
+typedef struct dgpu_ctx dgpu_ctx;
+
+/* Library version (DGPU_ABI_VERSION) and a one-line build description. */
+int32_t dgpu_abi_version(void);
+const char *dgpu_build_info(void);
+
+/* Creates a context on the current HIP device: INIT_CHAN_MPI index ranges
+ * (for/mpi.f:1080-1100), grid metrics, wavenumbers, FFT plans and device
+ * buffers.  Replaces the set-up half of INITIALIZE for this path
+ * (for/diablo_io.f:69-124).  The state starts at zero. */
+int32_t dgpu_create(const dgpu_params *p, dgpu_ctx **out);
+int32_t dgpu_destroy(dgpu_ctx *ctx);
+const char *dgpu_last_error(const dgpu_ctx *ctx);
+
+/* Copies the kept Fourier slots of one Fortran-layout field to the device
+ * (set) or back (get).  get writes every slot of planes 0..NY+1: kept slots
+ * get the state, the rest zero. */
+int32_t dgpu_set_field(dgpu_ctx *ctx, int32_t field, const double *fortran_array);
+int32_t dgpu_get_field(dgpu_ctx *ctx, int32_t field, double *fortran_array);
+
+/* RK_CHAN_1 (for/channel.f:62-1073): one RK3 / Crank-Nicolson substep of the
+ * device-resident state, then REM_DIV_CHAN and the CP update.  rk_step is
+ * RK_STEP (1..3); time is TIME (read only when F_TYPE = 2); h_bar, beta_bar
+ * and zeta_bar are the three-entry /RKW3_PARAMS/ arrays (for/header:175). */
+int32_t dgpu_rk_chan_1(dgpu_ctx *ctx, int32_t rk_step, double time, const double *h_bar,
+                       const double *beta_bar, const double *zeta_bar);
+
+/* REM_DIV_CHAN (for/channel.f:1103-1214) on the device-resident CU1..3. */
+int32_t dgpu_rem_div_chan(dgpu_ctx *ctx);
+
+/* APPLY_BC_VEL_MPI (for/mpi.f:1371-1385 -> for/channel.f:2285-2492) on the
+ * device-resident CU1..3. */
+int32_t dgpu_apply_bc_vel(dgpu_ctx *ctx);
+
+/* FFT_XZ_TO_FOURIER(V, VV, JMIN, JMAX) (for/mpi.f:1530-1609) on a host
+ * Fortran-layout array, in place (V and VV are the same memory, as in every
+ * reference call).  Planes outside JMIN..JMAX are not touched.  Kept slots
+ * are as in the reference; I > NKX, the padding column and rows NZ..NZ+1 are
+ * zero; the slots K = TNKZ+1..NZ-1, which the reference leaves holding
+ * unscaled intermediate values that no caller reads, are zero here. */
+int32_t dgpu_fft_xz_to_fourier(dgpu_ctx *ctx, double *v, int32_t jmin, int32_t jmax);
+
+/* FFT_XZ_TO_PHYSICAL(VV, V, JMIN, JMAX) (for/mpi.f:1613-1674) on a host
+ * Fortran-layout array, in place.  Uses slots I = 0..NX/2-1, K = 0..TNKZ of
+ * each plane (x modes above NKX are transformed, not zeroed, as in the
+ * reference); writes x = 0..NX-1 of rows 0..NZ-1, zeroes the two x pads of
+ * those rows and leaves rows NZ..NZ+1 untouched. */
+int32_t dgpu_fft_xz_to_physical(dgpu_ctx *ctx, double *vv, int32_t jmin, int32_t jmax);
+
+/* Waits for all work enqueued on the context. */
+int32_t dgpu_synchronize(dgpu_ctx *ctx);
+
+/* Kernel timing: when enabled, every kernel launch is bracketed by HIP events
+ * on the context's stream.  dgpu_kernel_times returns, per kernel kind
+ * (0 zinv, 1 xfused, 2 zfwd_nl, 3 columns), the summed milliseconds and the
+ * number of launches since the last reset.  Timing syncs the stream. */
+int32_t dgpu_set_kernel_timing(dgpu_ctx *ctx, int32_t enable);
+int32_t dgpu_kernel_times(dgpu_ctx *ctx, double *ms, int64_t *count, int32_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DIABLO_GPU_H */

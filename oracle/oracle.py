@@ -156,13 +156,14 @@ class Oracle:
     def fft_array_to_physical(self, a: np.ndarray, jmin: int, jmax: int):
         self.L.dor_fft_xz_to_physical(self.h, _dp(a), jmin, jmax)
 
-    def rk_chan_1(self, rk_step: int):
+    def rk_chan_1(self, rk_step: int, time: float | None = None):
+        """RK_CHAN_1 with TIME = time (default: the current step's start time)."""
+        self.L.dor_set_time(self.h, self.time if time is None else time)
         self.L.dor_rk_chan_1(self.h, rk_step)
 
     def step(self):
         """One time step: RK_STEP = 1..3 then TIME += DELTA_T (for/diablo.f:70-109)."""
         for k in (1, 2, 3):
-            self.L.dor_set_time(self.h, self.time)
             self.rk_chan_1(k)
         self.time += self.cfg.delta_t
 

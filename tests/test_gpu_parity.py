@@ -1,0 +1,179 @@
+"""Parity of the HIP path (through the C-ABI) with the CPU oracle.
+
+Every test starts both sides from the same seeded state.  FP64 path bar:
+1e-11 relative after one substep, 1e-10 after several steps (relative to the
+field's max over the kept Fourier slots)."""
+import numpy as np
+import pytest
+
+from diablo_amd import case
+from gpu_util import FIELDS, compare_states, kitchen_sink, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def gpu_mod():
+    from diablo_amd import gpu
+    return gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def make_pair(oracle_mod, gpu_mod, cfg, nsteps_oracle=0):
+    o = oracle_mod.Oracle(cfg)
+    o.initialize(*case.initial_physical(cfg))
+    for _ in range(nsteps_oracle):
+        o.step()
+    g = gpu_mod.GpuChannel(cfg)
+    g.load_state(o.state())
+    g.time = o.time
+    return o, g
+
+
+SIZES = [(16, 9, 16), (24, 17, 18), (32, 13, 48), (48, 33, 24), (64, 33, 64), (96, 17, 96)]
+
+
+@pytest.mark.parametrize("dims", SIZES)
+def test_fft_to_fourier(oracle_mod, gpu_mod, dims):
+    cfg = case.channel_case(*dims)
+    say("fft_to_fourier", dims)
+    o = oracle_mod.Oracle(cfg)
+    g = gpu_mod.GpuChannel(cfg)
+    rng = np.random.default_rng(1)
+    a = rng.standard_normal(cfg.field_shape())
+    b = a.copy()
+    o.fft_array_to_fourier(a, 0, cfg.ny + 1)
+    g.fft_xz_to_fourier(b, 0, cfg.ny + 1)
+    vv_o, vv_g = case.fourier_view(a), case.fourier_view(b)
+    assert np.abs(case.kept(cfg, b) - case.kept(cfg, a)).max() < 1e-13 * max(1, np.abs(case.kept(cfg, a)).max())
+    assert np.all(vv_g[:, :, cfg.nkx + 1:] == 0) and np.all(vv_g[:, cfg.nz:, :] == 0)
+    del vv_o
+
+
+@pytest.mark.parametrize("dims", SIZES)
+def test_fft_to_physical(oracle_mod, gpu_mod, dims):
+    cfg = case.channel_case(*dims)
+    say("fft_to_physical", dims)
+    o = oracle_mod.Oracle(cfg)
+    g = gpu_mod.GpuChannel(cfg)
+    rng = np.random.default_rng(2)
+    a = np.zeros(cfg.field_shape())
+    vv = case.fourier_view(a)
+    nxp = cfg.nx // 2
+    shp = vv[:, :cfg.tnkz + 1, :nxp].shape
+    vv[:, :cfg.tnkz + 1, :nxp] = rng.standard_normal(shp) + 1j * rng.standard_normal(shp)
+    b = a.copy()
+    o.fft_array_to_physical(a, 0, cfg.ny + 1)
+    g.fft_xz_to_physical(b, 0, cfg.ny + 1)
+    pa, pb = a[:, :cfg.nz, :cfg.nx], b[:, :cfg.nz, :cfg.nx]
+    assert np.abs(pb - pa).max() < 1e-12 * np.abs(pa).max()
+
+
+def test_fft_plane_range(oracle_mod, gpu_mod):
+    cfg = case.channel_case(24, 9, 18)
+    g = gpu_mod.GpuChannel(cfg)
+    rng = np.random.default_rng(3)
+    a = rng.standard_normal(cfg.field_shape())
+    b = a.copy()
+    g.fft_xz_to_fourier(b, 3, 5)
+    assert np.array_equal(a[:3], b[:3]) and np.array_equal(a[6:], b[6:])
+    o = oracle_mod.Oracle(cfg)
+    o.fft_array_to_fourier(a, 3, 5)
+    assert np.abs(case.kept(cfg, a)[3:6] - case.kept(cfg, b)[3:6]).max() < 1e-13
+
+
+def test_set_get_roundtrip(gpu_mod):
+    cfg = case.channel_case(24, 9, 18)
+    g = gpu_mod.GpuChannel(cfg)
+    rng = np.random.default_rng(4)
+    a = rng.standard_normal(cfg.field_shape())
+    g.set_field(gpu_mod.CU2, a)
+    b = g.get_field(gpu_mod.CU2)
+    assert np.array_equal(case.kept(cfg, b), case.kept(cfg, a))
+    mask = np.ones(case.fourier_view(b).shape, bool)
+    mask[:, :cfg.tnkz + 1, :cfg.nkx + 1] = False
+    assert np.all(case.fourier_view(b)[mask] == 0)
+
+
+@pytest.mark.parametrize("maker", ["channel", "kh", "sink"])
+def test_initialize_and_rem_div(oracle_mod, gpu_mod, maker):
+    cfg = {"channel": lambda: case.channel_case(32, 17, 24),
+           "kh": lambda: case.kh_case(32, 33, 24, kick=0.05),
+           "sink": lambda: kitchen_sink()}[maker]()
+    say("initialize", maker)
+    o = oracle_mod.Oracle(cfg)
+    phys = case.initial_physical(cfg)
+    o.initialize(*phys)
+    g = gpu_mod.GpuChannel(cfg)
+    g.initialize(*phys)
+    errs, bad = compare_states(cfg, g.state(), o.state(), 1e-12, fields=("CU1", "CU2", "CU3", "CTH0"))
+    say(errs)
+    assert not bad, bad
+
+
+@pytest.mark.parametrize("maker", ["channel", "kh", "sink"])
+@pytest.mark.parametrize("rk", [1, 2, 3])
+def test_one_substep(oracle_mod, gpu_mod, maker, rk):
+    cfg = {"channel": lambda: case.channel_case(32, 17, 24, kick=0.05),
+           "kh": lambda: case.kh_case(32, 33, 24, kick=0.05, dt=0.01),
+           "sink": lambda: kitchen_sink()}[maker]()
+    say("one_substep", maker, rk)
+    o, g = make_pair(oracle_mod, gpu_mod, cfg, nsteps_oracle=1)
+    for k in range(1, rk):
+        o.rk_chan_1(k)
+    start = o.state()
+    g.load_state(start)
+    o.rk_chan_1(rk)
+    g.rk_chan_1(rk, time=o.time)
+    g.synchronize()
+    ref = o.state()
+    # CP = CP + phi/H_BAR amplifies round-off (phi is the small projection
+    # potential): its bar is the oracle's own spread when its input moves by
+    # 1e-16, measured here, times 10.
+    rng = np.random.default_rng(99)
+    o.load_state({k: v * (1 + 1e-16 * rng.standard_normal(v.shape)) for k, v in start.items()})
+    o.rk_chan_1(rk)
+    cp_sens = rel_err(cfg, o.state()["CP"], ref["CP"])
+    errs, bad = compare_states(cfg, g.state(), ref, 1e-11, fields=[f for f in FIELDS if f != "CP"])
+    errs["CP"] = rel_err(cfg, g.get_field(gpu_mod.CP), ref["CP"])
+    say(errs, "CP self-sensitivity", cp_sens)
+    assert not bad, bad
+    assert errs["CP"] <= max(1e-11, 10 * cp_sens), (errs["CP"], cp_sens)
+
+
+@pytest.mark.parametrize("maker,nsteps", [("channel", 10), ("kh", 10), ("sink", 6)])
+def test_multi_step(oracle_mod, gpu_mod, maker, nsteps):
+    cfg = {"channel": lambda: case.channel_case(48, 33, 48, kick=0.05),
+           "kh": lambda: case.kh_case(48, 33, 48, kick=0.05, dt=0.01),
+           "sink": lambda: kitchen_sink(48, 25, 36)}[maker]()
+    say("multi_step", maker, nsteps)
+    o, g = make_pair(oracle_mod, gpu_mod, cfg)
+    for _ in range(nsteps):
+        o.step()
+        g.step()
+    g.synchronize()
+    errs, bad = compare_states(cfg, g.state(), o.state(), 1e-10)
+    say(errs)
+    assert not bad, bad
+
+
+def test_divergence_free_after_steps(gpu_mod):
+    """Property at a larger size: the discrete divergence of REM_DIV_CHAN's
+    stencil vanishes on the interior rows after every step."""
+    cfg = case.channel_case(96, 65, 96, kick=0.05)
+    g = gpu_mod.GpuChannel(cfg)
+    g.initialize(*case.initial_physical(cfg))
+    g.step(3)
+    g.synchronize()
+    kx, kz, _, _ = case.wavenumbers(cfg)
+    _, _, _, dyf = case.grid_metrics(case.grid_gy(cfg), cfg.ny)
+    cu1, cu2, cu3 = (case.kept(cfg, g.get_field(f)) for f in (gpu_mod.CU1, gpu_mod.CU2, gpu_mod.CU3))
+    kxk = kx[:cfg.nkx + 1][None, None, :]
+    kzk = kz[None, :, None]
+    j = np.arange(2, cfg.ny)
+    div = 1j * kxk * cu1[j] + (cu2[j + 1] - cu2[j]) / dyf[j][:, None, None] + 1j * kzk * cu3[j]
+    scale = np.abs(cu2[j + 1] - cu2[j]).max() / dyf[1:cfg.ny].min() + np.abs(kxk * cu1[j]).max()
+    assert np.abs(div).max() < 1e-10 * scale
