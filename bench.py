@@ -47,7 +47,10 @@ def kernel_bytes(cfg, jlo: int, s: int = 8) -> dict:
         "zinv": c * (nin * p2 * ps + nin * p2 * zrow),
         "xfused": c * (nin * (pl + 2) * zrow + 2 * nin * pl * zrow),
         "zfwd_nl": c * (2 * nin * pl * zrow + nin * pl * ps),
-        "columns": c * ((10 + 3 * cfg.n_th) + (7 + 2 * cfg.n_th)) * p2 * ps,
+        # explicit RHS + CN: reads CU1-3, CTH, CP, CF (4), NL (4); writes CF (4), CU1-3, CTH
+        "cn": c * ((5 + 4 + 4) + (4 + 4)) * p2 * ps,
+        # REM_DIV + CP update: reads CU1-3, CP and the two Poisson factors (reals); writes CU1-3, CP
+        "remdiv": c * (4 + 1 + 4) * cfg.ny * ps,
     }
 
 

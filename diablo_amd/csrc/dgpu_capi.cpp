@@ -18,7 +18,7 @@ using dgpu::cx;
 
 namespace {
 
-constexpr int kKinds = 4;  // timed kernel kinds: zinv, xfused, zfwd_nl, columns
+constexpr int kKinds = 5;  // timed kernel kinds: zinv, xfused, zfwd_nl, cn, remdiv
 
 struct TimedLaunch {
     int kind;
@@ -47,17 +47,17 @@ struct dgpu_ctx {
     cx<double> *cf[3] = {nullptr, nullptr, nullptr};
     cx<double> *cfth[DGPU_MAX_TH] = {};
     cx<double> *nl[3 + DGPU_MAX_TH] = {};
-    cx<double> *pois = nullptr;
-    double *d_piv = nullptr;
     cx<double> **d_nlptr = nullptr;
     // transform intermediates
     cx<double> *d_zt = nullptr, *d_wt = nullptr;
     long zfs = 0;  // complex elements per intermediate field: (NY+2)*NZ*mk
     double *d_fort = nullptr;  // Fortran-layout staging for host entry points
-    // CN tables: [stage 0..2][sys][5][NY+2]
+    // CN tables: [stage 0..2][sys][NY+2][kRow]
     double *d_cn = nullptr;
     std::vector<double> h_cn;
-    double cn_temp1[3] = {NAN, NAN, NAN};
+    double cn_key[3][4] = {{NAN}, {NAN}, {NAN}};
+    // REM_DIV: per-mode Poisson factors [NY+2][ps] and per-row constants
+    double *d_pmult = nullptr, *d_pinv = nullptr, *d_rd = nullptr;
     int *d_codes = nullptr;
     double *d_c1 = nullptr;
     hipStream_t stream = nullptr;
@@ -65,8 +65,8 @@ struct dgpu_ctx {
     bool timing = false;
     std::vector<TimedLaunch> pending;
     std::vector<hipEvent_t> event_pool;
-    double t_ms[kKinds] = {0, 0, 0, 0};
-    long long t_n[kKinds] = {0, 0, 0, 0};
+    double t_ms[kKinds] = {0, 0, 0, 0, 0};
+    long long t_n[kKinds] = {0, 0, 0, 0, 0};
 };
 }
 
@@ -123,27 +123,32 @@ void twiddles(int n, std::vector<cx<double>> &out)
 }
 
 // Matrices of one CN system exactly as RK_CHAN_1 + APPLY_BC_*_MPI build them
-// (for/channel.f:771-1029, 1660-2282), then THOMAS_FORWARD_REAL_MPI's
-// elimination (for/mpi.f:673-699) of the matrix part.
-void build_cn(const dgpu_ctx *c, int sys, double temp1, double *mult, double *piv, double *up, double *bc00,
-              double *isbc, int &lo, int &hi)
+// (for/channel.f:771-1029, 1660-2282), THOMAS_FORWARD_REAL_MPI's elimination of
+// the matrix part (for/mpi.f:673-699), and the other per-row constants of the
+// Fourier-space solve (see the T_* layout in dgpu_kernels.hpp).
+void build_cn(const dgpu_ctx *c, int sys, double temp1, double temp4, double *tab)
 {
+    using namespace dgpu;
     const int NY = c->g.ny;
     const double NUV = c->p.nu_v_scale;
     const double *DY = c->dy.data(), *DYF = c->dyf.data();
-    std::vector<double> L(NY + 2, 0.0), D(NY + 2, 1.0), U(NY + 2, 0.0);
-    for (int j = 0; j <= NY + 1; ++j) { bc00[j] = 0.0; isbc[j] = 1.0; }
+    std::vector<double> L(NY + 2, 0.0), D(NY + 2, 1.0), U(NY + 2, 0.0), bc00(NY + 2, 0.0), isbc(NY + 2, 1.0);
+    std::vector<double> wp(NY + 2, 0.0), wm(NY + 2, 0.0);
     auto setrow = [&](int j, double l, double d, double u, double v) {
         L[j] = l; D[j] = d; U[j] = u; bc00[j] = v; isbc[j] = 1.0;
     };
-    if (sys == dgpu::SYS_U1 || sys == dgpu::SYS_U3) {
+    int cflo, cfhi, lo, hi;
+    if (sys == SYS_U1 || sys == SYS_U3) {
         for (int j = c->jstart; j <= c->jend; ++j) {
             L[j] = -temp1 * NUV / (DY[j] * DYF[j]);
             D[j] = 1. - temp1 * NUV * (-1. / (DY[j + 1] * DYF[j]) - 1. / (DY[j] * DYF[j]));
             U[j] = -temp1 * NUV / (DY[j + 1] * DYF[j]);
             isbc[j] = 0.0;
+            // R += TEMP1*NU_V_SCALE*((u(j+1)-u(j))/DY(j+1) - (u(j)-u(j-1))/DY(j))/DYF(j)
+            wp[j] = temp1 * NUV / (DY[j + 1] * DYF[j]);
+            wm[j] = temp1 * NUV / (DY[j] * DYF[j]);
         }
-        const bool isu = (sys == dgpu::SYS_U1);
+        const bool isu = (sys == SYS_U1);
         const int cmin = isu ? c->p.u_bc_ymin : c->p.w_bc_ymin;
         const int cmax = isu ? c->p.u_bc_ymax : c->p.w_bc_ymax;
         const double vmin = isu ? c->p.u_bc_ymin_c1 : c->p.w_bc_ymin_c1;
@@ -158,14 +163,16 @@ void build_cn(const dgpu_ctx *c, int sys, double temp1, double *mult, double *pi
             setrow(NY, -1., 1., 0., DY[NY] * vmax);
             setrow(NY + 1, 0., 1., 0., 0.);
         }
-        lo = c->jstart - 1;
-        hi = c->jend + 1;
-    } else if (sys == dgpu::SYS_U2) {
+        cflo = c->jstart; cfhi = c->jend;
+        lo = c->jstart - 1; hi = c->jend + 1;
+    } else if (sys == SYS_U2) {
         for (int j = 2; j <= NY; ++j) {
             L[j] = -temp1 * NUV / (DYF[j - 1] * DY[j]);
             D[j] = 1. + temp1 * NUV / (DYF[j] * DY[j]) + temp1 * NUV / (DYF[j - 1] * DY[j]);
             U[j] = -temp1 * NUV / (DYF[j] * DY[j]);
             isbc[j] = 0.0;
+            wp[j] = temp1 * NUV / (DYF[j] * DY[j]);
+            wm[j] = temp1 * NUV / (DYF[j - 1] * DY[j]);
         }
         if (c->p.v_bc_ymin == 0) {
             setrow(1, 0., 1., 0., c->p.v_bc_ymin_c1);
@@ -180,16 +187,18 @@ void build_cn(const dgpu_ctx *c, int sys, double temp1, double *mult, double *pi
         } else if (c->p.v_bc_ymax == 1) {
             setrow(NY + 1, -1., 1., 0., DYF[NY] * c->p.v_bc_ymax_c1);
         }
-        lo = 1;
-        hi = NY + 1;
+        cflo = 2; cfhi = NY;
+        lo = 1; hi = NY + 1;
     } else {
-        const int n = sys - dgpu::SYS_TH0;
+        const int n = sys - SYS_TH0;
         const double cc = temp1 / c->p.pr[n] * NUV;
         for (int j = c->jstart_th[n]; j <= c->jend_th[n]; ++j) {
             L[j] = -(cc) / (DY[j] * DYF[j]);
             D[j] = 1. + (cc) / (DY[j + 1] * DYF[j]) + (cc) / (DY[j] * DYF[j]);
             U[j] = -(cc) / (DY[j + 1] * DYF[j]);
             isbc[j] = 0.0;
+            wp[j] = cc / (DY[j + 1] * DYF[j]);
+            wm[j] = cc / (DY[j] * DYF[j]);
         }
         const double vmin = c->p.th_bc_ymin_c1[n], vmax = c->p.th_bc_ymax_c1[n];
         if (c->p.th_bc_ymin[n] == 0) {
@@ -206,8 +215,8 @@ void build_cn(const dgpu_ctx *c, int sys, double temp1, double *mult, double *pi
             setrow(NY, -1., 1., 0., DY[NY] * vmax);
             setrow(NY + 1, 0., 1., 0., 0.);
         }
-        lo = c->jstart_th[n] - 1;
-        hi = c->jend_th[n] + 1;
+        cflo = c->jstart_th[n]; cfhi = c->jend_th[n];
+        lo = c->jstart_th[n] - 1; hi = c->jend_th[n] + 1;
     }
     // forward elimination of the matrix (rows 1..NY+1)
     std::vector<double> A = L, B = D;
@@ -215,33 +224,47 @@ void build_cn(const dgpu_ctx *c, int sys, double temp1, double *mult, double *pi
         A[j] = -A[j] / B[j - 1];
         B[j] = B[j] + A[j] * U[j - 1];
     }
+    const double rgy = (c->g.nth > 0) ? c->p.ri[0] * c->p.grav_y : 0.0;
     for (int j = 0; j <= NY + 1; ++j) {
-        mult[j] = A[j];
-        piv[j] = B[j];
-        up[j] = U[j];
+        double *t = tab + (long)j * kRow;
+        for (int e = 0; e < kRow; ++e) t[e] = 0.0;
+        t[T_MULT] = (j == 0) ? 0.0 : A[j];
+        t[T_INVPIV] = (B[j] != 0.0) ? 1.0 / B[j] : 0.0;
+        t[T_UP] = U[j];
+        t[T_BC00] = bc00[j];
+        t[T_ISBC] = isbc[j];
+        if (isbc[j] == 0.0) {
+            t[T_WP] = wp[j];
+            t[T_WM] = wm[j];
+            t[T_W0] = -(wp[j] + wm[j]);
+        }
+        if (sys == SYS_U2 && j >= 2 && j <= NY) {
+            t[T_CPW] = temp4 / DY[j];
+            // RI*GRAV_Y*(CTH(j)*DYF(j-1) + CTH(j-1)*DYF(j))/(2*DY(j))
+            t[T_B0] = rgy * DYF[j - 1] / (2.0 * DY[j]);
+            t[T_BM] = rgy * DYF[j] / (2.0 * DY[j]);
+        }
+        t[T_CFR] = (j >= cflo && j <= cfhi) ? 1.0 : 0.0;
+        t[T_CPY] = (j >= lo && j <= hi) ? 1.0 : 0.0;
     }
-    mult[0] = 0.0;
 }
 
 int nsys_of(const dgpu_ctx *c) { return 3 + c->g.nth; }
 
-long cn_block(const dgpu_ctx *c) { return 5L * (c->g.ny + 2); }
+long cn_block(const dgpu_ctx *c) { return (long)dgpu::kRow * (c->g.ny + 2); }
 
-int ensure_cn(dgpu_ctx *c, int stage, double temp1)
+int ensure_cn(dgpu_ctx *c, int stage, double temp1, double temp4)
 {
-    if (c->cn_temp1[stage] == temp1) return 0;
+    const double key[4] = {temp1, temp4, 0, 0};
+    if (c->cn_key[stage][0] == key[0] && c->cn_key[stage][1] == key[1]) return 0;
     const int nsys = nsys_of(c);
     const long blk = cn_block(c);
     double *base = c->h_cn.data() + (long)stage * nsys * blk;
-    for (int q = 0; q < nsys; ++q) {
-        double *t = base + q * blk;
-        const int ny2 = c->g.ny + 2;
-        int lo, hi;
-        build_cn(c, q, temp1, t, t + ny2, t + 2 * ny2, t + 3 * ny2, t + 4 * ny2, lo, hi);
-    }
+    for (int q = 0; q < nsys; ++q) build_cn(c, q, temp1, temp4, base + q * blk);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipMemcpy(c->d_cn + (long)stage * nsys * blk, base, sizeof(double) * nsys * blk, hipMemcpyHostToDevice));
-    c->cn_temp1[stage] = temp1;
+    c->cn_key[stage][0] = key[0];
+    c->cn_key[stage][1] = key[1];
     return 0;
 }
 
@@ -437,10 +460,10 @@ int32_t dgpu_create(const dgpu_params *p, dgpu_ctx **out)
         c->fx.tw = c->d_tw;
         c->fz.tw = c->d_tw + g.nx;
     }
-    // state: cu1 cu2 cu3 cth[n] | cp cf1 cf2 cf3 cfth[n] | nl[3+n] | pois
+    // state: cu1 cu2 cu3 cth[n] | cp cf1 cf2 cf3 cfth[n] | nl[3+n]
     const int nin = 3 + g.nth;
     c->fs = (long)(NY + 2) * g.ps;
-    c->nslot = nin + 4 + g.nth + nin + 1;
+    c->nslot = nin + 4 + g.nth + nin;
     if (hipMalloc(&c->d_state, sizeof(cx<double>) * c->fs * c->nslot) != hipSuccess) return bail(DGPU_E_NOMEM);
     if (hipMemset(c->d_state, 0, sizeof(cx<double>) * c->fs * c->nslot) != hipSuccess) return bail(DGPU_E_HIP);
     {
@@ -451,9 +474,25 @@ int32_t dgpu_create(const dgpu_params *p, dgpu_ctx **out)
         for (int f = 0; f < 3; ++f) c->cf[f] = c->d_state + (s++) * c->fs;
         for (int n = 0; n < g.nth; ++n) c->cfth[n] = c->d_state + (s++) * c->fs;
         for (int f = 0; f < nin; ++f) c->nl[f] = c->d_state + (s++) * c->fs;
-        c->pois = c->d_state + (s++) * c->fs;
     }
-    if (hipMalloc(&c->d_piv, sizeof(double) * c->fs) != hipSuccess) return bail(DGPU_E_NOMEM);
+    // REM_DIV: per-mode Poisson factorisation (time independent) and row constants
+    if (hipMalloc(&c->d_pmult, sizeof(double) * c->fs) != hipSuccess ||
+        hipMalloc(&c->d_pinv, sizeof(double) * c->fs) != hipSuccess ||
+        hipMalloc(&c->d_rd, sizeof(double) * 4 * (NY + 2)) != hipSuccess)
+        return bail(DGPU_E_NOMEM);
+    {
+        std::vector<double> rd(4 * (NY + 2), 0.0);
+        for (int j = 1; j <= NY; ++j) {
+            if (j <= NY - 1) rd[4 * j + 0] = 1. / (c->dy[j + 1] * c->dyf[j]);
+            rd[4 * j + 1] = 1. / c->dyf[j];
+            rd[4 * j + 2] = 1. / c->dy[j];
+        }
+        rd[4 * (NY + 1) + 2] = 1. / c->dy[NY + 1];
+        if (hipMemcpy(c->d_rd, rd.data(), sizeof(double) * rd.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return bail(DGPU_E_HIP);
+        dgpu::launch_pois_factor(c->stream, g, c->d_dy, c->d_dyf, c->d_kx2, c->d_kz2, c->d_pmult, c->d_pinv);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess) return bail(DGPU_E_HIP);
+    }
     if (hipMalloc(&c->d_nlptr, sizeof(cx<double> *) * (3 + DGPU_MAX_TH)) != hipSuccess) return bail(DGPU_E_NOMEM);
     if (hipMemcpy(c->d_nlptr, c->nl, sizeof(cx<double> *) * (3 + DGPU_MAX_TH), hipMemcpyHostToDevice) != hipSuccess)
         return bail(DGPU_E_HIP);
@@ -486,7 +525,7 @@ int32_t dgpu_destroy(dgpu_ctx *c)
     if (c->stream) hipStreamSynchronize(c->stream);
     for (auto &t : c->pending) { hipEventDestroy(t.a); hipEventDestroy(t.b); }
     for (auto e : c->event_pool) hipEventDestroy(e);
-    void *bufs[] = {c->d_vec, c->d_tw, c->d_state, c->d_piv, c->d_nlptr, c->d_zt, c->d_wt,
+    void *bufs[] = {c->d_vec, c->d_tw, c->d_state, c->d_pmult, c->d_pinv, c->d_rd, c->d_nlptr, c->d_zt, c->d_wt,
                     c->d_fort, c->d_cn, c->d_codes, c->d_c1};
     for (void *b : bufs)
         if (b) hipFree(b);
@@ -533,50 +572,8 @@ int32_t dgpu_get_field(dgpu_ctx *c, int32_t field, double *a)
     return DGPU_OK;
 }
 
-static void fill_colargs(dgpu_ctx *c, dgpu::ColArgs &A, dgpu::ColPtrs<double> &P, int stage)
+static void fill_ptrs(dgpu_ctx *c, dgpu::CnPtrs<double> &P)
 {
-    const dgpu::Geo &g = c->g;
-    std::memset(&A, 0, sizeof(A));
-    A.g = g;
-    A.jstart = c->jstart;
-    A.jend = c->jend;
-    for (int n = 0; n < g.nth; ++n) {
-        A.jstart_th[n] = c->jstart_th[n];
-        A.jend_th[n] = c->jend_th[n];
-        A.ri[n] = c->p.ri[n];
-        A.pr[n] = c->p.pr[n];
-        A.drhodx[n] = c->p.drhodx[n];
-        A.drhodz[n] = c->p.drhodz[n];
-    }
-    A.nu = c->p.nu;
-    A.beta = c->p.beta;
-    A.i_ro = c->p.i_ro;
-    A.grav_x = c->p.grav_x;
-    A.grav_y = c->p.grav_y;
-    A.grav_z = c->p.grav_z;
-    A.nuv = c->p.nu_v_scale;
-    A.f_type = c->p.f_type;
-    A.dy = c->d_dy;
-    A.dyf = c->d_dyf;
-    A.kx = c->d_kx;
-    A.kz = c->d_kz;
-    A.kx2 = c->d_kx2;
-    A.kz2 = c->d_kz2;
-    const int nsys = nsys_of(c);
-    const long blk = cn_block(c);
-    const int ny2 = g.ny + 2;
-    for (int q = 0; q < nsys; ++q) {
-        const double *t = c->d_cn + ((long)stage * nsys + q) * blk;
-        dgpu::CnSys &S = A.sys[q];
-        S.mult = t;
-        S.piv = t + ny2;
-        S.up = t + 2 * ny2;
-        S.bc00 = t + 3 * ny2;
-        S.isbc = t + 4 * ny2;
-        if (q == dgpu::SYS_U1 || q == dgpu::SYS_U3) { S.lo = c->jstart - 1; S.hi = c->jend + 1; }
-        else if (q == dgpu::SYS_U2) { S.lo = 1; S.hi = g.ny + 1; }
-        else { S.lo = c->jstart_th[q - 3] - 1; S.hi = c->jend_th[q - 3] + 1; }
-    }
     std::memset(&P, 0, sizeof(P));
     P.cu1 = c->cu[0];
     P.cu2 = c->cu[1];
@@ -585,13 +582,28 @@ static void fill_colargs(dgpu_ctx *c, dgpu::ColArgs &A, dgpu::ColPtrs<double> &P
     P.cf1 = c->cf[0];
     P.cf2 = c->cf[1];
     P.cf3 = c->cf[2];
-    for (int n = 0; n < g.nth; ++n) {
-        P.cth[n] = c->cth[n];
-        P.cfth[n] = c->cfth[n];
-    }
-    for (int f = 0; f < 3 + g.nth; ++f) P.nl[f] = c->nl[f];
-    P.pois = c->pois;
-    P.piv = c->d_piv;
+    P.cth = c->g.nth ? c->cth[0] : nullptr;
+    P.cfth = c->g.nth ? c->cfth[0] : nullptr;
+    // zfwd_nl writes the x-, y-, z-momentum and scalar terms to nl[0..3]
+    P.nl[dgpu::SYS_U1] = c->nl[0];
+    P.nl[dgpu::SYS_U2] = c->nl[1];
+    P.nl[dgpu::SYS_U3] = c->nl[2];
+    P.nl[dgpu::SYS_TH0] = c->g.nth ? c->nl[3] : nullptr;
+}
+
+static void fill_rdargs(dgpu_ctx *c, dgpu::RdArgs &R, int update_cp, double temp4)
+{
+    std::memset(&R, 0, sizeof(R));
+    R.g = c->g;
+    R.update_cp = update_cp;
+    R.jstart = c->jstart;
+    R.jend = c->jend;
+    R.inv_temp4 = 1.0 / temp4;
+    R.pmult = c->d_pmult;
+    R.pinv = c->d_pinv;
+    R.rd = c->d_rd;
+    R.kx = c->d_kx;
+    R.kz = c->d_kz;
 }
 
 int32_t dgpu_rk_chan_1(dgpu_ctx *c, int32_t rk_step, double time, const double *h_bar, const double *beta_bar,
@@ -604,7 +616,7 @@ int32_t dgpu_rk_chan_1(dgpu_ctx *c, int32_t rk_step, double time, const double *
     const double temp3 = zeta_bar[k] * h_bar[k];
     const double temp4 = h_bar[k];
     const double temp5 = beta_bar[k] * h_bar[k];
-    if (int rc = ensure_cn(c, k, temp1)) return rc;
+    if (int rc = ensure_cn(c, k, temp1, temp4)) return rc;
     const dgpu::Geo &g = c->g;
     const int nin = 3 + g.nth;
     int jlo = 2;
@@ -626,33 +638,53 @@ int32_t dgpu_rk_chan_1(dgpu_ctx *c, int32_t rk_step, double time, const double *
         dgpu::launch_zfwd_nl<double>(c->stream, g, c->fz, c->d_wt, c->zfs, c->d_nlptr, c->d_kz, jlo, g.ny, nin);
     }
     if (int rc = check_launch(c, "zfwd_nl")) return rc;
-    dgpu::ColArgs A;
-    dgpu::ColPtrs<double> P;
-    fill_colargs(c, A, P, k);
-    A.mode = 0;
+    dgpu::CnPtrs<double> P;
+    fill_ptrs(c, P);
+    dgpu::CnArgs A;
+    std::memset(&A, 0, sizeof(A));
+    A.g = g;
     A.rk_gt1 = rk_step > 1;
-    A.temp1 = temp1;
+    A.f_type = c->p.f_type;
     A.temp3 = temp3;
     A.temp4 = temp4;
     A.temp5 = temp5;
+    A.nu = c->p.nu;
+    A.nupr = g.nth ? c->p.nu / c->p.pr[0] : 0.0;
+    A.beta = c->p.beta;
+    A.i_ro = c->p.i_ro;
     A.force00 = (c->p.f_type == 2) ? (c->p.px0 + c->p.amp_omega0 * std::cos(c->p.omega0 * time)) : c->p.px0;
+    A.rgx = g.nth ? c->p.ri[0] * c->p.grav_x : 0.0;
+    A.rgz = g.nth ? c->p.ri[0] * c->p.grav_z : 0.0;
+    A.drhodx = g.nth ? c->p.drhodx[0] : 0.0;
+    A.drhodz = g.nth ? c->p.drhodz[0] : 0.0;
+    A.tab = c->d_cn + (long)k * nsys_of(c) * cn_block(c);
+    A.kx = c->d_kx;
+    A.kz = c->d_kz;
+    A.kx2 = c->d_kx2;
+    A.kz2 = c->d_kz2;
     {
         TimerScope t(c, 3);
-        dgpu::launch_columns<double>(c->stream, A, P);
+        dgpu::launch_cn<double>(c->stream, A, P);
     }
-    if (int rc = check_launch(c, "columns")) return rc;
+    if (int rc = check_launch(c, "cn")) return rc;
+    dgpu::RdArgs R;
+    fill_rdargs(c, R, 1, temp4);
+    {
+        TimerScope t(c, 4);
+        dgpu::launch_remdiv<double>(c->stream, R, P);
+    }
+    if (int rc = check_launch(c, "remdiv")) return rc;
     return DGPU_OK;
 }
 
 int32_t dgpu_rem_div_chan(dgpu_ctx *c)
 {
     if (!c) return DGPU_E_ARG;
-    dgpu::ColArgs A;
-    dgpu::ColPtrs<double> P;
-    fill_colargs(c, A, P, 0);
-    A.mode = 1;
-    A.temp4 = 1.0;
-    dgpu::launch_columns<double>(c->stream, A, P);
+    dgpu::CnPtrs<double> P;
+    fill_ptrs(c, P);
+    dgpu::RdArgs R;
+    fill_rdargs(c, R, 0, 1.0);
+    dgpu::launch_remdiv<double>(c->stream, R, P);
     return check_launch(c, "rem_div");
 }
 

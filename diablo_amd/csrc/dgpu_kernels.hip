@@ -1,6 +1,6 @@
 // dgpu_kernels.hip -- gfx950 kernels of the channel time-stepper.
 //
-// One RK3 substep (RK_CHAN_1, for/channel.f:62-1073) is four launches:
+// One RK3 substep (RK_CHAN_1, for/channel.f:62-1073) is five launches:
 //
 //   zinv     CU1,CU2,CU3,CTH (kept modes)  --z-inverse-->  Zt[f][j][z][kx]
 //   xfused   per z-row, sweeping j: C2R of the four fields (pairs packed in one
@@ -9,8 +9,9 @@
 //            products, the d/dx terms applied (-i kx), written as
 //            G = -(i kx X(a) + X(b)) and H = -X(c) per momentum/scalar eq.
 //   zfwd_nl  NL_f = Z(G_f) + i kz Z(H_f)   (z-forward of both, scaled 1/(NX NZ))
-//   columns  one thread per (kx,kz) mode: explicit RHS, Crank-Nicolson solves
-//            of TH, U2, U1, U3 in Fourier space, REM_DIV_CHAN and the CP update.
+//   cn       per (kx,kz) mode: explicit RHS and the Crank-Nicolson solves of
+//            TH, U2, U1, U3 in Fourier space (dgpu_columns.hip)
+//   remdiv   per mode: REM_DIV_CHAN and the CP update (dgpu_columns.hip)
 //
 // The reference's eight physical<->Fourier transforms around its physical-space
 // CN solves are not needed: with LES off the CN matrix is the same for every
@@ -30,9 +31,6 @@ __device__ __forceinline__ int kzslot_to_z(int s, int nkz, int tnkz, int nz)
     // (for/mpi.f:1602-1606); its index in the length-NZ transform:
     return s <= nkz ? s : nz - (tnkz + 1 - s);
 }
-
-template <typename T>
-__device__ __forceinline__ cx<T> ld(const cx<T> *p) { return *p; }
 
 // -------------------------------------------------------------------------
 // z-inverse: kept slots -> length-NZ columns (FFT_XZ_TO_PHYSICAL z part,
@@ -84,13 +82,6 @@ struct XCounts {
     static constexpr int npo = (nprod + 1) / 2;         // R2C pairs
     static constexpr int nout = 2 * nin;                // G,H per equation
 };
-
-template <typename T, int NTH>
-__device__ __forceinline__ T pick(const cx<T> *row, int fld, int x)
-{
-    const cx<T> v = row[(fld >> 1) * 0 + x];
-    return (fld & 1) ? v.im : v.re;
-}
 
 template <typename T, int E, int NTH>
 __global__ void k_xfused(FftLen fx, const cx<T> *__restrict__ tw, int mk, const cx<T> *__restrict__ zt,
@@ -272,262 +263,6 @@ __global__ void k_zfwd_nl(FftLen fz, const cx<T> *__restrict__ tw, const cx<T> *
             const int s = idx / W, bb = idx - s * W;
             const int i = i0 + bb;
             if (i < mk) dst[(long)s * mk + i] = keep[it];
-        }
-    }
-}
-
-// -------------------------------------------------------------------------
-// columns: one thread per kept mode.  See ColArgs for the inputs.
-// -------------------------------------------------------------------------
-template <typename T>
-__device__ __forceinline__ cx<T> ikmul(T k, cx<T> a) { return make_cx<T>(-k * a.im, k * a.re); }
-
-template <typename T>
-__device__ __forceinline__ cx<T> axpy(T a, cx<T> x, cx<T> y) { return make_cx<T>(y.re + a * x.re, y.im + a * x.im); }
-
-template <typename T>
-__device__ __forceinline__ cx<T> divr(cx<T> a, T r) { return make_cx<T>(a.re / r, a.im / r); }
-
-template <typename T>
-__global__ void k_columns(ColArgs A, ColPtrs<T> P)
-{
-    const Geo &g = A.g;
-    const long MM = g.ps;
-    const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= MM) return;
-    const int s = (int)(m / g.mk), i = (int)(m - (long)s * g.mk);
-    const bool is00 = (i == 0 && s == 0);
-    const int NY = g.ny;
-    const long ps = g.ps;
-    const double *DY = A.dy, *DYF = A.dyf;
-    const T kx = T(A.kx[i]), kz = T(A.kz[s]);
-    const T kx2 = T(A.kx2[i]), kz2 = T(A.kz2[s]);
-    const cx<T> zero = make_cx<T>(0, 0);
-    const int nth = g.nth;
-
-    if (A.mode == 0) {
-        const T pw = T(pow(double(kx2 + kz2), A.beta));
-        const T temp1 = T(A.temp1), temp3 = T(A.temp3), temp4 = T(A.temp4), temp5 = T(A.temp5);
-        const T nuv = T(A.nuv);
-        const T visc = T(A.nu) * pw;
-        // --- sweep A: explicit RHS, new CF, CN forward elimination ---
-        cx<T> u1m = zero, u2m = zero, u3m = zero, cpm = zero;
-        cx<T> u1 = P.cu1[m], u2 = P.cu2[m], u3 = P.cu3[m], cp0 = P.cp[m];
-        cx<T> thm[kMaxTh], th0[kMaxTh], gth[kMaxTh];
-        for (int n = 0; n < nth; ++n) { thm[n] = zero; th0[n] = P.cth[n][m]; gth[n] = zero; }
-        cx<T> g1 = zero, g2 = zero, g3 = zero;
-        for (int j = 0; j <= NY + 1; ++j) {
-            const long o = m + (long)j * ps;
-            const bool hasp = (j <= NY);
-            const cx<T> u1p = hasp ? P.cu1[o + ps] : zero;
-            const cx<T> u2p = hasp ? P.cu2[o + ps] : zero;
-            const cx<T> u3p = hasp ? P.cu3[o + ps] : zero;
-            cx<T> thp[kMaxTh];
-            for (int n = 0; n < nth; ++n) thp[n] = hasp ? P.cth[n][o + ps] : zero;
-            const T dyj = T(DY[j]), dyfj = T(DYF[j]);
-            const T dyp = hasp ? T(DY[j + 1]) : T(1);
-            const T dyfm = j > 0 ? T(DYF[j - 1]) : T(1);
-
-            // ---- U1 and U3 (rows JSTART..JEND) ----
-            {
-                const bool inr = (j >= A.jstart && j <= A.jend);
-                cx<T> cf1n = zero, cf3n = zero;
-                if (inr) {
-                    const cx<T> nl1 = P.nl[0][o], nl3 = P.nl[2][o];
-                    cf1n = make_cx<T>(-(visc * u1.re), -(visc * u1.im));
-                    cf3n = make_cx<T>(-(visc * u3.re), -(visc * u3.im));
-                    cf1n = axpy(T(A.i_ro), u3, cf1n);
-                    cf3n = axpy(-T(A.i_ro), u1, cf3n);
-                    for (int n = 0; n < nth; ++n) {
-                        cf1n = axpy(T(A.ri[n] * A.grav_x), th0[n], cf1n);
-                        cf3n = axpy(T(A.ri[n] * A.grav_z), th0[n], cf3n);
-                    }
-                    cf1n = cf1n + nl1;
-                    cf3n = cf3n + nl3;
-                }
-                // U1
-                {
-                    const CnSys &S = A.sys[SYS_U1];
-                    cx<T> vec;
-                    if (S.isbc[j] != 0.0) {
-                        vec = is00 ? make_cx<T>(T(S.bc00[j]), 0) : zero;
-                    } else {
-                        cx<T> cr = u1;
-                        if (A.rk_gt1) cr = axpy(temp3, P.cf1[o], cr);
-                        cr = cr - scale(ikmul(kx, cp0), temp4);
-                        if (is00 && (A.f_type == 1 || A.f_type == 2)) cr.re = cr.re - temp4 * T(A.force00);
-                        cr = axpy(temp5, cf1n, cr);
-                        const cx<T> d2 = divr(divr(u1p - u1, dyp) - divr(u1 - u1m, dyj), dyfj);
-                        vec = axpy(temp1 * nuv, d2, cr);
-                    }
-                    g1 = (j == 0) ? vec : axpy(T(S.mult[j]), g1, vec);
-                }
-                // U3
-                {
-                    const CnSys &S = A.sys[SYS_U3];
-                    cx<T> vec;
-                    if (S.isbc[j] != 0.0) {
-                        vec = is00 ? make_cx<T>(T(S.bc00[j]), 0) : zero;
-                    } else {
-                        cx<T> cr = u3;
-                        if (A.rk_gt1) cr = axpy(temp3, P.cf3[o], cr);
-                        cr = cr - scale(ikmul(kz, cp0), temp4);
-                        cr = axpy(temp5, cf3n, cr);
-                        const cx<T> d2 = divr(divr(u3p - u3, dyp) - divr(u3 - u3m, dyj), dyfj);
-                        vec = axpy(temp1 * nuv, d2, cr);
-                    }
-                    g3 = (j == 0) ? vec : axpy(T(S.mult[j]), g3, vec);
-                }
-                if (inr) {
-                    P.cf1[o] = cf1n;
-                    P.cf3[o] = cf3n;
-                }
-            }
-            // ---- U2 (rows 2..NY) ----
-            {
-                const bool inr = (j >= 2 && j <= NY);
-                cx<T> cf2n = zero;
-                if (inr) {
-                    cf2n = make_cx<T>(-(visc * u2.re), -(visc * u2.im));
-                    if (!is00) {
-                        for (int n = 0; n < nth; ++n) {
-                            const cx<T> t = make_cx<T>(th0[n].re * dyfm + thm[n].re * dyfj, th0[n].im * dyfm + thm[n].im * dyfj);
-                            const T c = T(A.ri[n] * A.grav_y);
-                            cf2n = cf2n + divr(make_cx<T>(c * t.re, c * t.im), T(2) * dyj);
-                        }
-                    }
-                    cf2n = cf2n + P.nl[1][o];
-                }
-                const CnSys &S = A.sys[SYS_U2];
-                cx<T> vec;
-                if (S.isbc[j] != 0.0) {
-                    vec = is00 ? make_cx<T>(T(S.bc00[j]), 0) : zero;
-                } else {
-                    cx<T> cr = u2;
-                    if (A.rk_gt1) cr = axpy(temp3, P.cf2[o], cr);
-                    cr = cr - scale(divr(cp0 - cpm, dyj), temp4);
-                    cr = axpy(temp5, cf2n, cr);
-                    const cx<T> d2 = divr(divr(u2p - u2, dyfj) - divr(u2 - u2m, dyfm), dyj);
-                    vec = axpy(temp1 * nuv, d2, cr);
-                }
-                g2 = (j == 0) ? vec : axpy(T(S.mult[j]), g2, vec);
-                if (inr) P.cf2[o] = cf2n;
-            }
-            // ---- scalars ----
-            for (int n = 0; n < nth; ++n) {
-                const bool inr = (j >= A.jstart_th[n] && j <= A.jend_th[n]);
-                cx<T> cfn = zero;
-                if (inr) {
-                    const T vp = T(A.nu / A.pr[n]) * pw;
-                    cfn = make_cx<T>(-(vp * th0[n].re), -(vp * th0[n].im));
-                    cfn = cfn - scale(u1, T(A.drhodx[n])) - scale(u3, T(A.drhodz[n]));
-                    cfn = cfn + P.nl[3 + n][o];
-                }
-                const CnSys &S = A.sys[SYS_TH0 + n];
-                cx<T> vec;
-                if (S.isbc[j] != 0.0) {
-                    vec = is00 ? make_cx<T>(T(S.bc00[j]), 0) : zero;
-                } else {
-                    cx<T> cr = th0[n];
-                    if (A.rk_gt1) cr = axpy(temp3, P.cfth[n][o], cr);
-                    cr = axpy(temp5, cfn, cr);
-                    const cx<T> d2 = divr(divr(thp[n] - th0[n], dyp) - divr(th0[n] - thm[n], dyj), dyfj);
-                    vec = axpy((temp1 / T(A.pr[n])) * nuv, d2, cr);
-                }
-                gth[n] = (j == 0) ? vec : axpy(T(S.mult[j]), gth[n], vec);
-                if (inr) P.cfth[n][o] = cfn;
-            }
-            // forward-eliminated right-hand sides overwrite the consumed NL rows
-            P.nl[0][o] = g1;
-            P.nl[1][o] = g2;
-            P.nl[2][o] = g3;
-            for (int n = 0; n < nth; ++n) P.nl[3 + n][o] = gth[n];
-            // roll
-            u1m = u1; u2m = u2; u3m = u3; cpm = cp0;
-            u1 = u1p; u2 = u2p; u3 = u3p;
-            if (hasp) cp0 = P.cp[o + ps];
-            for (int n = 0; n < nth; ++n) { thm[n] = th0[n]; th0[n] = thp[n]; }
-        }
-        // --- sweep B: back substitution, copy-out rows only ---
-        {
-            const int nsys = 3 + nth;
-            for (int q = 0; q < nsys; ++q) {
-                const CnSys &S = A.sys[q];
-                cx<T> *fld = (q == SYS_U1) ? P.cu1 : (q == SYS_U3) ? P.cu3 : (q == SYS_U2) ? P.cu2 : P.cth[q - SYS_TH0];
-                cx<T> *gg = (q == SYS_U1) ? P.nl[0] : (q == SYS_U3) ? P.nl[2] : (q == SYS_U2) ? P.nl[1] : P.nl[q];
-                cx<T> x = zero;
-                for (int j = NY + 1; j >= 0; --j) {
-                    const long o = m + (long)j * ps;
-                    const cx<T> gj = gg[o];
-                    const T piv = T(S.piv[j]);
-                    if (j == NY + 1) {
-                        x = divr(gj, piv);
-                    } else if (piv != T(0)) {
-                        x = divr(gj - scale(x, T(S.up[j])), piv);
-                    } else {
-                        x = zero;
-                    }
-                    if (j >= S.lo && j <= S.hi) fld[o] = x;
-                }
-            }
-        }
-    }
-
-    // --- REM_DIV_CHAN (for/channel.f:1103-1214) per column ---
-    // sweep C: forward elimination j = 2..NY of the Poisson system
-    {
-        T bprev = T(1);                 // row 1: D = 1
-        cx<T> gprev = zero;             // row 1: VEC = 0
-        const T u1row = is00 ? T(0) : T(-1);
-        P.piv[m + ps] = bprev;
-        P.pois[m + ps] = gprev;
-        cx<T> c2j = P.cu2[m + 2 * ps];
-        for (int j = 2; j <= NY; ++j) {
-            const long o = m + (long)j * ps;
-            T L, D, Uprev;
-            cx<T> vec;
-            Uprev = (j - 1 == 1) ? u1row : T(1.0 / (DY[j] * DYF[j - 1]));
-            const cx<T> c2p = P.cu2[o + ps];
-            if (j == NY) {
-                L = T(1); D = T(-1); vec = zero;
-            } else {
-                L = T(1.0 / (DY[j] * DYF[j]));
-                D = -kx2 - kz2 - T(1.0 / (DY[j + 1] * DYF[j])) - T(1.0 / (DY[j] * DYF[j]));
-                const cx<T> a = ikmul(kx, P.cu1[o]);
-                const cx<T> c = ikmul(kz, P.cu3[o]);
-                vec = (a + divr(c2p - c2j, T(DYF[j]))) + c;
-            }
-            const T mult = (bprev != T(0)) ? -L / bprev : T(0);
-            const T bj = D + mult * Uprev;
-            const cx<T> gj = axpy(mult, gprev, vec);
-            P.piv[o] = bj;
-            P.pois[o] = gj;
-            bprev = bj;
-            gprev = gj;
-            c2j = c2p;
-        }
-    }
-    // sweep D: back substitution and projection (j = NY..1)
-    {
-        const T u1row = is00 ? T(0) : T(-1);
-        const T temp4 = T(A.temp4);
-        cx<T> phip = zero;
-        for (int j = NY; j >= 1; --j) {
-            const long o = m + (long)j * ps;
-            const T piv = P.piv[o];
-            const cx<T> gj = P.pois[o];
-            cx<T> phi;
-            if (j == NY) {
-                phi = (piv != T(0)) ? divr(gj, piv) : zero;
-            } else {
-                const T U = (j == 1) ? u1row : T(1.0 / (DY[j + 1] * DYF[j]));
-                phi = (piv != T(0)) ? divr(gj - scale(phip, U), piv) : zero;
-            }
-            P.cu1[o] = P.cu1[o] - ikmul(kx, phi);
-            P.cu3[o] = P.cu3[o] - ikmul(kz, phi);
-            if (j + 1 <= NY) P.cu2[o + ps] = P.cu2[o + ps] - divr(phip - phi, T(DY[j + 1]));
-            if (A.mode == 0 && j >= A.jstart && j <= A.jend) P.cp[o] = P.cp[o] + divr(phi, temp4);
-            phip = phi;
         }
     }
 }
@@ -819,14 +554,6 @@ void launch_zfwd_nl(hipStream_t s, const Geo &g, const Fft<T> &fz, const cx<T> *
     DGPU_DISPATCH_E(fz.len.e, (zfwd_nl_e<T, EE>(s, g, fz, wt, wt_fs, nl, kz, jlo, jhi, nout)));
 }
 
-template <typename T>
-void launch_columns(hipStream_t s, const ColArgs &a, const ColPtrs<T> &p)
-{
-    const int threads = 64;
-    const long blocks = (a.g.ps + threads - 1) / threads;
-    hipLaunchKernelGGL((k_columns<T>), dim3(blocks), dim3(threads), 0, s, a, p);
-}
-
 template <typename T, int E>
 static void x_r2c_e(hipStream_t s, const Geo &g, const Fft<T> &fx, const double *phys, cx<T> *xt, int j0,
                     int nplanes)
@@ -907,7 +634,6 @@ template void launch_xfused<double>(hipStream_t, const Geo &, const Fft<double> 
                                     cx<double> *, long, const double *, const double *, const double *, int, int);
 template void launch_zfwd_nl<double>(hipStream_t, const Geo &, const Fft<double> &, const cx<double> *, long,
                                      cx<double> *const *, const double *, int, int, int);
-template void launch_columns<double>(hipStream_t, const ColArgs &, const ColPtrs<double> &);
 template void launch_x_r2c<double>(hipStream_t, const Geo &, const Fft<double> &, const double *, cx<double> *, int,
                                    int);
 template void launch_zfwd_pack<double>(hipStream_t, const Geo &, const Fft<double> &, const cx<double> *, double *,

@@ -27,40 +27,41 @@ struct Fft {
     const cx<T> *tw;
 };
 
-// Solve tables for one tridiagonal CN system (rows 0..NY+1): the reference's
-// matrix after THOMAS_FORWARD_REAL_MPI (multiplier A', pivot B', upper C),
-// which rows are boundary rows and their right-hand side for the (0,0) mode.
-struct CnSys {
-    const double *mult, *piv, *up, *bc00, *isbc;
-    int lo, hi;  // rows copied back into the field
-};
-
 enum { SYS_U1 = 0, SYS_U3 = 1, SYS_U2 = 2, SYS_TH0 = 3 };
 
-struct ColArgs {
+// Per-row constants of one CN system for one RK stage (doubles, kRow per row):
+// the matrix after THOMAS_FORWARD_REAL_MPI's elimination (multiplier,
+// reciprocal pivot, upper diagonal), whether the row is a boundary row and its
+// right-hand side for the (0,0) mode, the explicit y-diffusion stencil weights
+// (TEMP1*NU_V_SCALE folded in), the pressure-gradient weight TEMP4/DY (U2),
+// the buoyancy interpolation weights (U2), and whether the row is in the
+// explicit-term range (CF written) and in the copy-back range.
+enum { T_MULT = 0, T_INVPIV, T_UP, T_BC00, T_ISBC, T_WP, T_W0, T_WM, T_CPW, T_B0, T_BM, T_CFR, T_CPY, kRow = 16 };
+
+struct CnArgs {
     Geo g;
-    int mode;  // 0: full RK substep, 1: REM_DIV_CHAN only
-    int rk_gt1;
-    int jstart, jend;
-    int jstart_th[kMaxTh], jend_th[kMaxTh];
-    double temp1, temp3, temp4, temp5;
-    double nu, beta, i_ro, grav_x, grav_y, grav_z, nuv;
-    double force00;  // PX0 (F_TYPE 1) or PX0 + AMP*cos(OMEGA0*TIME) (F_TYPE 2)
-    int f_type;
-    double ri[kMaxTh], pr[kMaxTh], drhodx[kMaxTh], drhodz[kMaxTh];
-    const double *dy, *dyf, *kx, *kz, *kx2, *kz2;
-    CnSys sys[3 + kMaxTh];
+    int rk_gt1, f_type;
+    double temp3, temp4, temp5;
+    double nu, nupr, beta, i_ro, force00;
+    double rgx, rgz, drhodx, drhodz;  // RI*GRAV_X, RI*GRAV_Z, background gradients
+    const double *tab;                // [sys][NY+2][kRow] for this stage
+    const double *kx, *kz, *kx2, *kz2;
 };
 
 template <typename T>
-struct ColPtrs {
-    cx<T> *cu1, *cu2, *cu3, *cp;
-    cx<T> *cth[kMaxTh];
-    cx<T> *cf1, *cf2, *cf3;
-    cx<T> *cfth[kMaxTh];
-    cx<T> *nl[3 + kMaxTh];  // nonlinear accumulators, reused as solve scratch
-    cx<T> *pois;            // Poisson elimination scratch (complex)
-    T *piv;                 // Poisson pivots
+struct CnPtrs {
+    cx<T> *cu1, *cu2, *cu3, *cp, *cth;
+    cx<T> *cf1, *cf2, *cf3, *cfth;
+    cx<T> *nl[4];  // nonlinear term per system (index SYS_*), reused as solve scratch
+};
+
+// REM_DIV_CHAN: per-mode Poisson factors and per-row constants
+// rd[j*4 + {0,1,2}] = {1/(DY(j+1)*DYF(j)), 1/DYF(j), 1/DY(j)}.
+struct RdArgs {
+    Geo g;
+    int update_cp, jstart, jend;
+    double inv_temp4;
+    const double *pmult, *pinv, *rd, *kx, *kz;
 };
 
 // ---- launchers (dgpu_kernels.hip) ----
@@ -78,7 +79,11 @@ void launch_zfwd_nl(hipStream_t s, const Geo &g, const Fft<T> &fz, const cx<T> *
                     const double *kz, int jlo, int jhi, int nout);
 
 template <typename T>
-void launch_columns(hipStream_t s, const ColArgs &a, const ColPtrs<T> &p);
+void launch_cn(hipStream_t s, const CnArgs &a, const CnPtrs<T> &p);
+template <typename T>
+void launch_remdiv(hipStream_t s, const RdArgs &a, const CnPtrs<T> &p);
+void launch_pois_factor(hipStream_t s, const Geo &g, const double *dy, const double *dyf, const double *kx2,
+                        const double *kz2, double *pmult, double *pinv);
 
 // Standalone FFT_XZ_TO_FOURIER / FFT_XZ_TO_PHYSICAL on Fortran-layout planes.
 template <typename T>

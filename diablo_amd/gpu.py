@@ -25,7 +25,7 @@ DP = C.POINTER(C.c_double)
 # field ids (include/diablo_gpu.h)
 CU1, CU2, CU3, CP, CF1, CF2, CF3 = 0, 1, 2, 3, 4, 5, 6
 CTH0, CFTH0 = 16, 24
-KERNEL_KINDS = ("zinv", "xfused", "zfwd_nl", "columns")
+KERNEL_KINDS = ("zinv", "xfused", "zfwd_nl", "cn", "remdiv")
 
 
 class Params(C.Structure):
@@ -214,9 +214,9 @@ class GpuChannel:
         self._chk(self.L.dgpu_set_kernel_timing(self.h, 1 if on else 0), "dgpu_set_kernel_timing")
 
     def kernel_times(self) -> dict:
-        ms = (C.c_double * 4)()
-        cnt = (C.c_int64 * 4)()
-        self._chk(self.L.dgpu_kernel_times(self.h, ms, cnt, 4), "dgpu_kernel_times")
+        ms = (C.c_double * 5)()
+        cnt = (C.c_int64 * 5)()
+        self._chk(self.L.dgpu_kernel_times(self.h, ms, cnt, 5), "dgpu_kernel_times")
         return {k: (ms[i], cnt[i]) for i, k in enumerate(KERNEL_KINDS)}
 
     # -- CREATE_FLOW_CHAN / CREATE_TH_CHAN on the device --

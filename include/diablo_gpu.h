@@ -141,7 +141,7 @@ int32_t dgpu_synchronize(dgpu_ctx *ctx);
 
 /* Kernel timing: when enabled, every kernel launch is bracketed by HIP events
  * on the context's stream.  dgpu_kernel_times returns, per kernel kind
- * (0 zinv, 1 xfused, 2 zfwd_nl, 3 columns), the summed milliseconds and the
+ * (0 zinv, 1 xfused, 2 zfwd_nl, 3 cn, 4 remdiv), the summed milliseconds and the
  * number of launches since the last reset.  Timing syncs the stream. */
 int32_t dgpu_set_kernel_timing(dgpu_ctx *ctx, int32_t enable);
 int32_t dgpu_kernel_times(dgpu_ctx *ctx, double *ms, int64_t *count, int32_t n);
