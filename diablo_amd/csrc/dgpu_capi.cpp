@@ -52,6 +52,8 @@ struct dgpu_ctx {
     cx<double> *d_zt = nullptr, *d_wt = nullptr;
     long zfs = 0;  // complex elements per intermediate field: (NY+2)*NZ*mk
     double *d_fort = nullptr;  // Fortran-layout staging for host entry points
+    double *d_fa = nullptr, *d_fb = nullptr;  // POISSON_P_CHAN work planes
+    double *d_gyf = nullptr;
     // CN tables: [stage 0..2][sys][NY+2][kRow]
     double *d_cn = nullptr;
     std::vector<double> h_cn;
@@ -526,6 +528,7 @@ int32_t dgpu_destroy(dgpu_ctx *c)
     for (auto &t : c->pending) { hipEventDestroy(t.a); hipEventDestroy(t.b); }
     for (auto e : c->event_pool) hipEventDestroy(e);
     void *bufs[] = {c->d_vec, c->d_tw, c->d_state, c->d_pmult, c->d_pinv, c->d_rd, c->d_nlptr, c->d_zt, c->d_wt,
+                    c->d_fa, c->d_fb, c->d_gyf,
                     c->d_fort, c->d_cn, c->d_codes, c->d_c1};
     for (void *b : bufs)
         if (b) hipFree(b);
@@ -737,6 +740,74 @@ int32_t dgpu_fft_xz_to_physical(dgpu_ctx *c, double *vv, int32_t jmin, int32_t j
     if (int rc = check_launch(c, "x_c2r")) return rc;
     HIPCHK(c, hipMemcpyAsync(vv + plane * jmin, c->d_fort + plane * jmin, plane * np * sizeof(double),
                              hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return DGPU_OK;
+}
+
+// Device-side Fortran-layout plane transforms used by POISSON_P_CHAN.
+static int fort_to_physical(dgpu_ctx *c, double *buf, int j0, int j1)
+{
+    const dgpu::Geo &g = c->g;
+    const int nxp = g.nx / 2, np = j1 - j0 + 1;
+    const cx<double> *in = reinterpret_cast<const cx<double> *>(buf);
+    dgpu::launch_zinv<double>(c->stream, g, c->fz, in, 0, (long)(nxp + 1) * (g.nz + 2), nxp + 1, nxp, c->d_wt, 0,
+                              (long)g.nz * nxp, nxp, j0, np, 1);
+    if (int rc = check_launch(c, "zinv(planes)")) return rc;
+    dgpu::launch_x_c2r<double>(c->stream, g, c->fx, c->d_wt, nxp, buf, j0, np);
+    return check_launch(c, "x_c2r(planes)");
+}
+
+static int fort_to_fourier(dgpu_ctx *c, double *buf, int j0, int j1)
+{
+    const dgpu::Geo &g = c->g;
+    const int np = j1 - j0 + 1;
+    dgpu::launch_x_r2c<double>(c->stream, g, c->fx, buf, c->d_zt, j0, np);
+    if (int rc = check_launch(c, "x_r2c(planes)")) return rc;
+    dgpu::launch_zfwd_pack<double>(c->stream, g, c->fz, c->d_zt, buf, j0, np);
+    return check_launch(c, "zfwd_pack(planes)");
+}
+
+int32_t dgpu_poisson_p_chan(dgpu_ctx *c)
+{
+    if (!c) return DGPU_E_ARG;
+    const dgpu::Geo &g = c->g;
+    if (int rc = ensure_fort(c)) return rc;
+    const size_t n = (size_t)(g.nx + 2) * (g.nz + 2) * (g.ny + 2);
+    if (!c->d_fa) HIPCHK(c, hipMalloc(&c->d_fa, n * sizeof(double)));
+    if (!c->d_fb) HIPCHK(c, hipMalloc(&c->d_fb, n * sizeof(double)));
+    if (!c->d_gyf) {
+        HIPCHK(c, hipMalloc(&c->d_gyf, sizeof(double) * (g.ny + 2)));
+        HIPCHK(c, hipMemcpy(c->d_gyf, c->gyf.data(), sizeof(double) * (g.ny + 2), hipMemcpyHostToDevice));
+    }
+    double *fa = c->d_fa, *fb = c->d_fb, *fc = c->d_fort;
+    cx<double> *cs = c->nl[1], *scr = c->nl[0];
+    const int j0 = 2, j1 = g.ny;
+    for (int group = 0; group < 4; ++group) {
+        dgpu::launch_pp_spec(c->stream, group, g, c->cu[0], c->cu[1], c->cu[2], c->d_dyf, c->d_kx, c->d_kz, fa, fb,
+                             group == 0 ? fc : nullptr);
+        if (int rc = check_launch(c, "pp_spec")) return rc;
+        if (int rc = fort_to_physical(c, fa, j0, j1)) return rc;
+        if (int rc = fort_to_physical(c, fb, j0, j1)) return rc;
+        if (group == 0) {
+            if (int rc = fort_to_physical(c, fc, j0, j1)) return rc;
+            dgpu::launch_pp_phys(c->stream, 0, g, fa, fb, fb, fc);
+        } else {
+            dgpu::launch_pp_phys(c->stream, 1, g, fa, fb, fb, nullptr);
+        }
+        if (int rc = check_launch(c, "pp_phys")) return rc;
+        if (int rc = fort_to_fourier(c, fa, j0, j1)) return rc;
+        if (group == 0) {
+            if (int rc = fort_to_fourier(c, fb, j0, j1)) return rc;
+            if (int rc = fort_to_fourier(c, fc, j0, j1)) return rc;
+            dgpu::launch_pp_accum(c->stream, g, cs, fa, fb, fc, 1);
+        } else {
+            dgpu::launch_pp_accum(c->stream, g, cs, fa, nullptr, nullptr, 0);
+        }
+        if (int rc = check_launch(c, "pp_accum")) return rc;
+    }
+    dgpu::launch_pp_solve(c->stream, g, cs, g.nth ? c->cth[0] : nullptr, g.nth ? c->p.ri[0] : 0.0, c->d_gyf,
+                          c->d_pmult, c->d_pinv, c->d_rd, scr, c->cp);
+    if (int rc = check_launch(c, "pp_solve")) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return DGPU_OK;
 }

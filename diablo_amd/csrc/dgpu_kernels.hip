@@ -61,8 +61,9 @@ __global__ void k_zinv(FftLen fz, const cx<T> *__restrict__ tw, const cx<T> *__r
         if (i < ncols) lds[kzslot_to_z(s, nkz, tnkz, nz) * W + b] = src[(long)s * in_mk + i];
     }
     __syncthreads();
-    const int b = tid / fz.t, th = tid - b * fz.t;
-    fft_lds<T, E, +1>(lds + b, W, fz, th, b < W, tw);
+    // batch index fastest across lanes: LDS accesses of a wavefront are contiguous
+    const int b = tid % W, th = tid / W;
+    fft_lds<T, E, +1>(lds + b, W, fz, th, th < fz.t, tw);
     for (int idx = tid; idx < W * nz; idx += nthr) {
         const int z = idx / W, bb = idx - z * W;
         const int i = i0 + bb;
@@ -227,7 +228,7 @@ __global__ void k_zfwd_nl(FftLen fz, const cx<T> *__restrict__ tw, const cx<T> *
     const int mz = tnkz + 1;
     const T inv_nz = T(1) / T(nz);
     cx<T> keep[MAXIT];
-    const int b = tid / fz.t, th = tid - b * fz.t;
+    const int b = tid % W, th = tid / W;
 #pragma unroll
     for (int pass = 0; pass < 2; ++pass) {
         const cx<T> *src = wt + (2 * o + pass) * wt_fs + (long)j * wt_ps;
@@ -237,7 +238,7 @@ __global__ void k_zfwd_nl(FftLen fz, const cx<T> *__restrict__ tw, const cx<T> *
             lds[idx] = (i < mk) ? src[(long)z * mk + i] : make_cx<T>(0, 0);
         }
         __syncthreads();
-        fft_lds<T, E, -1>(lds + b, W, fz, th, b < W, tw);
+        fft_lds<T, E, -1>(lds + b, W, fz, th, th < fz.t, tw);
 #pragma unroll
         for (int it = 0; it < MAXIT; ++it) {
             const int idx = tid + it * nthr;
@@ -327,8 +328,8 @@ __global__ void k_zfwd_pack(FftLen fz, const cx<T> *__restrict__ tw, const cx<T>
         lds[idx] = (i < mk) ? src[(long)z * mk + i] : make_cx<T>(0, 0);
     }
     __syncthreads();
-    const int b = tid / fz.t, th = tid - b * fz.t;
-    fft_lds<T, E, -1>(lds + b, W, fz, th, b < W, tw);
+    const int b = tid % W, th = tid / W;
+    fft_lds<T, E, -1>(lds + b, W, fz, th, th < fz.t, tw);
     // Fortran complex view (0:NXP, 0:NZ+1) per plane
     double *pl = fort + (long)j * (long)(nxp + 1) * (nz + 2) * 2;
     const T inv = T(1) / T(nz);

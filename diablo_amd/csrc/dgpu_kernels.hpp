@@ -95,6 +95,16 @@ void launch_zfwd_pack(hipStream_t s, const Geo &g, const Fft<T> &fz, const cx<T>
 template <typename T>
 void launch_x_c2r(hipStream_t s, const Geo &g, const Fft<T> &fx, const cx<T> *zt, int mkin, double *fort, int j0,
                   int nplanes);
+// POISSON_P_CHAN pieces (dgpu_poisson.hip)
+void launch_pp_spec(hipStream_t s, int group, const Geo &g, const cx<double> *cu1, const cx<double> *cu2,
+                    const cx<double> *cu3, const double *dyf, const double *kx, const double *kz, double *fa,
+                    double *fb, double *fc);
+void launch_pp_phys(hipStream_t s, int op, const Geo &g, double *fa, const double *fb, double *fbw, double *fc);
+void launch_pp_accum(hipStream_t s, const Geo &g, cx<double> *cs, const double *fa, const double *fb,
+                     const double *fc, int first);
+void launch_pp_solve(hipStream_t s, const Geo &g, const cx<double> *cs, const cx<double> *cth, double ri,
+                     const double *gyf, const double *pmult, const double *pinv, const double *rd, cx<double> *scr,
+                     cx<double> *cp);
 void launch_unpack_fortran(hipStream_t s, const Geo &g, const double *fort, cx<double> *kept, int j0, int nplanes);
 void launch_apply_bc_vel(hipStream_t s, const Geo &g, cx<double> *cu1, cx<double> *cu2, cx<double> *cu3,
                          const int *codes, const double *c1, const double *dy, const double *dyf);

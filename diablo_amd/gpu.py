@@ -50,7 +50,8 @@ class Params(C.Structure):
 
 EXPORTS = (
     "dgpu_abi_version", "dgpu_build_info", "dgpu_create", "dgpu_destroy", "dgpu_last_error",
-    "dgpu_set_field", "dgpu_get_field", "dgpu_rk_chan_1", "dgpu_rem_div_chan", "dgpu_apply_bc_vel",
+    "dgpu_set_field", "dgpu_get_field", "dgpu_rk_chan_1", "dgpu_rem_div_chan", "dgpu_poisson_p_chan",
+    "dgpu_apply_bc_vel",
     "dgpu_fft_xz_to_fourier", "dgpu_fft_xz_to_physical", "dgpu_synchronize",
     "dgpu_set_kernel_timing", "dgpu_kernel_times",
 )
@@ -80,7 +81,7 @@ def load_library(path: str = LIB_PATH):
         getattr(L, name).argtypes = [vp, i32, DP]
     L.dgpu_rk_chan_1.restype = i32
     L.dgpu_rk_chan_1.argtypes = [vp, i32, C.c_double, DP, DP, DP]
-    for name in ("dgpu_rem_div_chan", "dgpu_apply_bc_vel", "dgpu_synchronize"):
+    for name in ("dgpu_rem_div_chan", "dgpu_poisson_p_chan", "dgpu_apply_bc_vel", "dgpu_synchronize"):
         getattr(L, name).restype = i32
         getattr(L, name).argtypes = [vp]
     for name in ("dgpu_fft_xz_to_fourier", "dgpu_fft_xz_to_physical"):
@@ -197,6 +198,9 @@ class GpuChannel:
 
     def rem_div_chan(self):
         self._chk(self.L.dgpu_rem_div_chan(self.h), "dgpu_rem_div_chan")
+
+    def poisson_p_chan(self):
+        self._chk(self.L.dgpu_poisson_p_chan(self.h), "dgpu_poisson_p_chan")
 
     def apply_bc_vel(self):
         self._chk(self.L.dgpu_apply_bc_vel(self.h), "dgpu_apply_bc_vel")

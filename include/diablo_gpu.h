@@ -117,6 +117,12 @@ int32_t dgpu_rk_chan_1(dgpu_ctx *ctx, int32_t rk_step, double time, const double
 /* REM_DIV_CHAN (for/channel.f:1103-1214) on the device-resident CU1..3. */
 int32_t dgpu_rem_div_chan(dgpu_ctx *ctx);
 
+/* POISSON_P_CHAN (for/channel.f:1217-1453): CP from the device-resident
+ * CU1..3 and CTH.  Rows 1..NY of CP are written (as in the reference); the
+ * reference's use of CF1..3 as scratch is not reproduced (device scratch is
+ * used instead, CF is left unchanged).  Synchronises before returning. */
+int32_t dgpu_poisson_p_chan(dgpu_ctx *ctx);
+
 /* APPLY_BC_VEL_MPI (for/mpi.f:1371-1385 -> for/channel.f:2285-2492) on the
  * device-resident CU1..3. */
 int32_t dgpu_apply_bc_vel(dgpu_ctx *ctx);

@@ -177,3 +177,18 @@ def test_divergence_free_after_steps(gpu_mod):
     div = 1j * kxk * cu1[j] + (cu2[j + 1] - cu2[j]) / dyf[j][:, None, None] + 1j * kzk * cu3[j]
     scale = np.abs(cu2[j + 1] - cu2[j]).max() / dyf[1:cfg.ny].min() + np.abs(kxk * cu1[j]).max()
     assert np.abs(div).max() < 1e-10 * scale
+
+
+@pytest.mark.parametrize("maker", ["channel", "kh"])
+def test_poisson_p_chan(oracle_mod, gpu_mod, maker):
+    cfg = {"channel": lambda: case.channel_case(32, 17, 24, kick=0.3),
+           "kh": lambda: case.kh_case(32, 33, 24, kick=0.3).replace(ri=[0.7])}[maker]()
+    say("poisson_p_chan", maker)
+    o, g = make_pair(oracle_mod, gpu_mod, cfg, nsteps_oracle=1)
+    o.poisson_p_chan()
+    g.poisson_p_chan()
+    a = case.kept(cfg, g.get_field(gpu_mod.CP))[1:cfg.ny + 1]
+    b = case.kept(cfg, o.field(oracle_mod.P))[1:cfg.ny + 1]
+    err = np.abs(a - b).max() / np.abs(b).max()
+    say("CP rel err", err)
+    assert err < 1e-11
