@@ -110,6 +110,11 @@ bool make_len(int n, dgpu::FftLen &f)
     if (best < 0) return false;
     f.e = best;
     f.t = n / best;
+    unsigned long long ns = 1;
+    for (int s = 0; s < f.nstages; ++s) {
+        f.magic[s] = (ns == 1) ? 0u : (unsigned)(((1ULL << 32) + ns - 1) / ns);
+        ns *= (unsigned long long)f.radix[s];
+    }
     return true;
 }
 

@@ -49,9 +49,18 @@ struct FftLen {
     int n;
     int nstages;
     int radix[kMaxStages];
+    unsigned magic[kMaxStages];  // ceil(2^32 / ns) of each stage: t / ns = umulhi(t, magic)
     int e;      // values per thread (multiple of every radix)
     int t;      // threads per transform = n / e
 };
+
+// Quotient by a stage span with a precomputed reciprocal; exact for
+// t, ns < 2^16 (transform lengths here are far below that).  magic == 0
+// stands for a span of 1, whose reciprocal 2^32 does not fit.
+__device__ __forceinline__ int div_magic(int t, unsigned magic)
+{
+    return magic ? (int)__umulhi((unsigned)t, magic) : t;
+}
 
 template <typename T, int R, int SIGN>
 __device__ __forceinline__ void butterfly(cx<T> *v)
@@ -103,35 +112,42 @@ __device__ __forceinline__ void butterfly(cx<T> *v)
 // One Stockham stage of radix R for the transform whose element e sits at
 // lds[e * stride].  th is this thread's index inside the transform.
 template <typename T, int E, int R, int SIGN>
-__device__ __forceinline__ void stage(cx<T> *lds, int stride, int n, int ns, int th, int nthr,
+__device__ __forceinline__ void stage(cx<T> *lds, int stride, int n, int ns, unsigned magic, int th, int nthr,
                                       const cx<T> *__restrict__ tw, bool active)
 {
     constexpr int NB = E / R;
     const int L = n / R;
+    const int tws = n / (ns * R);
     cx<T> v[E];
+    cx<T> w[E];
     if (active) {
 #pragma unroll
         for (int u = 0; u < NB; ++u) {
             const int t = th + nthr * u;
+            const int q = div_magic(t, magic);
+            const int k = t - q * ns;
 #pragma unroll
             for (int r = 0; r < R; ++r) v[u * R + r] = lds[(t + r * L) * stride];
+            // twiddle loads depend only on t: issue them before the barrier
+#pragma unroll
+            for (int r = 1; r < R; ++r) w[u * R + r] = tw[r * k * tws];
         }
     }
     __syncthreads();
     if (active) {
-        const int tws = n / (ns * R);
 #pragma unroll
         for (int u = 0; u < NB; ++u) {
             const int t = th + nthr * u;
-            const int k = t % ns;
+            const int q = div_magic(t, magic);
+            const int k = t - q * ns;
 #pragma unroll
             for (int r = 1; r < R; ++r) {
-                cx<T> w = tw[r * k * tws];
-                if (SIGN > 0) w.im = -w.im;
-                v[u * R + r] = cmul(v[u * R + r], w);
+                cx<T> ww = w[u * R + r];
+                if (SIGN > 0) ww.im = -ww.im;
+                v[u * R + r] = cmul(v[u * R + r], ww);
             }
             butterfly<T, R, SIGN>(v + u * R);
-            const int d = (t / ns) * ns * R + k;
+            const int d = q * ns * R + k;
 #pragma unroll
             for (int r = 0; r < R; ++r) lds[(d + r * ns) * stride] = v[u * R + r];
         }
@@ -148,17 +164,18 @@ __device__ void fft_lds(cx<T> *lds, int stride, const FftLen &f, int th, bool ac
     int ns = 1;
     for (int s = 0; s < f.nstages; ++s) {
         const int R = f.radix[s];
+        const unsigned mg = f.magic[s];
         if constexpr (E % 4 == 0) {
-            if (R == 4) { stage<T, E, 4, SIGN>(lds, stride, f.n, ns, th, f.t, tw, active); ns *= 4; continue; }
+            if (R == 4) { stage<T, E, 4, SIGN>(lds, stride, f.n, ns, mg, th, f.t, tw, active); ns *= 4; continue; }
         }
         if constexpr (E % 2 == 0) {
-            if (R == 2) { stage<T, E, 2, SIGN>(lds, stride, f.n, ns, th, f.t, tw, active); ns *= 2; continue; }
+            if (R == 2) { stage<T, E, 2, SIGN>(lds, stride, f.n, ns, mg, th, f.t, tw, active); ns *= 2; continue; }
         }
         if constexpr (E % 3 == 0) {
-            if (R == 3) { stage<T, E, 3, SIGN>(lds, stride, f.n, ns, th, f.t, tw, active); ns *= 3; continue; }
+            if (R == 3) { stage<T, E, 3, SIGN>(lds, stride, f.n, ns, mg, th, f.t, tw, active); ns *= 3; continue; }
         }
         if constexpr (E % 5 == 0) {
-            if (R == 5) { stage<T, E, 5, SIGN>(lds, stride, f.n, ns, th, f.t, tw, active); ns *= 5; continue; }
+            if (R == 5) { stage<T, E, 5, SIGN>(lds, stride, f.n, ns, mg, th, f.t, tw, active); ns *= 5; continue; }
         }
     }
 }

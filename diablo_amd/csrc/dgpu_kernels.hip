@@ -40,10 +40,11 @@ __device__ __forceinline__ int kzslot_to_z(int s, int nkz, int tnkz, int nz)
 template <typename T, int E>
 __global__ void k_zinv(FftLen fz, const cx<T> *__restrict__ tw, const cx<T> *__restrict__ in, long in_fs,
                        long in_ps, int in_mk, int ncols, cx<T> *__restrict__ out, long out_fs, long out_ps,
-                       int out_mk, int j0, int nkz, int tnkz, int W)
+                       int out_mk, int j0, int nkz, int tnkz, int lw)
 {
     extern __shared__ unsigned char smem_raw[];
     cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
+    const int W = 1 << lw;
     const int nz = fz.n;
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int i0 = blockIdx.x * W;
@@ -56,16 +57,16 @@ __global__ void k_zinv(FftLen fz, const cx<T> *__restrict__ tw, const cx<T> *__r
     __syncthreads();
     const int mz = tnkz + 1;
     for (int idx = tid; idx < W * mz; idx += nthr) {
-        const int s = idx / W, b = idx - s * W;
+        const int s = idx >> lw, b = idx & (W - 1);
         const int i = i0 + b;
         if (i < ncols) lds[kzslot_to_z(s, nkz, tnkz, nz) * W + b] = src[(long)s * in_mk + i];
     }
     __syncthreads();
     // batch index fastest across lanes: LDS accesses of a wavefront are contiguous
-    const int b = tid % W, th = tid / W;
+    const int b = tid & (W - 1), th = tid >> lw;
     fft_lds<T, E, +1>(lds + b, W, fz, th, th < fz.t, tw);
     for (int idx = tid; idx < W * nz; idx += nthr) {
-        const int z = idx / W, bb = idx - z * W;
+        const int z = idx >> lw, bb = idx & (W - 1);
         const int i = i0 + bb;
         if (i < ncols) dst[(long)z * out_mk + i] = lds[idx];
     }
@@ -105,8 +106,9 @@ __global__ void k_xfused(FftLen fx, const cx<T> *__restrict__ tw, int mk, const 
 
     auto load_plane = [&](int q) {
         cx<T> *w = win + (q % 3) * C::npi * nx;
-        for (int idx = tid; idx < C::npi * nx; idx += nthr) {
-            const int p = idx / nx, k = idx - p * nx;
+        for (int p = 0; p < C::npi; ++p)
+        for (int k = tid; k < nx; k += nthr) {
+            const int idx = p * nx + k;
             const int fa = 2 * p, fb2 = 2 * p + 1;
             cx<T> A = zero, B = zero;
             bool hit = false, cj = false;
@@ -215,10 +217,11 @@ __global__ void k_xfused(FftLen fx, const cx<T> *__restrict__ tw, int mk, const 
 template <typename T, int E>
 __global__ void k_zfwd_nl(FftLen fz, const cx<T> *__restrict__ tw, const cx<T> *__restrict__ wt, long wt_fs,
                           long wt_ps, int mk, cx<T> *const *__restrict__ nl, long ps, const double *__restrict__ kzv,
-                          int jlo, int nkz, int tnkz, int W)
+                          int jlo, int nkz, int tnkz, int lw)
 {
     extern __shared__ unsigned char smem_raw[];
     cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
+    const int W = 1 << lw;
     constexpr int MAXIT = 24;
     const int nz = fz.n;
     const int tid = threadIdx.x, nthr = blockDim.x;
@@ -228,12 +231,12 @@ __global__ void k_zfwd_nl(FftLen fz, const cx<T> *__restrict__ tw, const cx<T> *
     const int mz = tnkz + 1;
     const T inv_nz = T(1) / T(nz);
     cx<T> keep[MAXIT];
-    const int b = tid % W, th = tid / W;
+    const int b = tid & (W - 1), th = tid >> lw;
 #pragma unroll
     for (int pass = 0; pass < 2; ++pass) {
         const cx<T> *src = wt + (2 * o + pass) * wt_fs + (long)j * wt_ps;
         for (int idx = tid; idx < W * nz; idx += nthr) {
-            const int z = idx / W, bb = idx - z * W;
+            const int z = idx >> lw, bb = idx & (W - 1);
             const int i = i0 + bb;
             lds[idx] = (i < mk) ? src[(long)z * mk + i] : make_cx<T>(0, 0);
         }
@@ -243,7 +246,7 @@ __global__ void k_zfwd_nl(FftLen fz, const cx<T> *__restrict__ tw, const cx<T> *
         for (int it = 0; it < MAXIT; ++it) {
             const int idx = tid + it * nthr;
             if (idx < W * mz) {
-                const int s = idx / W, bb = idx - s * W;
+                const int s = idx >> lw, bb = idx & (W - 1);
                 const cx<T> v = lds[kzslot_to_z(s, nkz, tnkz, nz) * W + bb];
                 if (pass == 0) {
                     keep[it] = scale(v, inv_nz);
@@ -261,7 +264,7 @@ __global__ void k_zfwd_nl(FftLen fz, const cx<T> *__restrict__ tw, const cx<T> *
     for (int it = 0; it < MAXIT; ++it) {
         const int idx = tid + it * nthr;
         if (idx < W * mz) {
-            const int s = idx / W, bb = idx - s * W;
+            const int s = idx >> lw, bb = idx & (W - 1);
             const int i = i0 + bb;
             if (i < mk) dst[(long)s * mk + i] = keep[it];
         }
@@ -313,28 +316,29 @@ __global__ void k_x_r2c(FftLen fx, const cx<T> *__restrict__ tw, const double *_
 // every other slot of the plane is set to zero.
 template <typename T, int E>
 __global__ void k_zfwd_pack(FftLen fz, const cx<T> *__restrict__ tw, const cx<T> *__restrict__ xt, int mk,
-                            double *__restrict__ fort, int nxp, int nkz, int tnkz, int j0, int W)
+                            double *__restrict__ fort, int nxp, int nkz, int tnkz, int j0, int lw)
 {
     extern __shared__ unsigned char smem_raw[];
     cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
+    const int W = 1 << lw;
     const int nz = fz.n;
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int i0 = blockIdx.x * W;
     const int j = j0 + blockIdx.y;
     const cx<T> *src = xt + (long)j * nz * mk;
     for (int idx = tid; idx < W * nz; idx += nthr) {
-        const int z = idx / W, b = idx - z * W;
+        const int z = idx >> lw, b = idx & (W - 1);
         const int i = i0 + b;
         lds[idx] = (i < mk) ? src[(long)z * mk + i] : make_cx<T>(0, 0);
     }
     __syncthreads();
-    const int b = tid % W, th = tid / W;
+    const int b = tid & (W - 1), th = tid >> lw;
     fft_lds<T, E, -1>(lds + b, W, fz, th, th < fz.t, tw);
     // Fortran complex view (0:NXP, 0:NZ+1) per plane
     double *pl = fort + (long)j * (long)(nxp + 1) * (nz + 2) * 2;
     const T inv = T(1) / T(nz);
     for (int idx = tid; idx < W * (nz + 2); idx += nthr) {
-        const int K = idx / W, bb = idx - K * W;
+        const int K = idx >> lw, bb = idx & (W - 1);
         const int i = i0 + bb;
         if (i > nxp) continue;
         cx<T> v = make_cx<T>(0, 0);
@@ -471,18 +475,25 @@ __global__ void k_apply_bc_vel(cx<double> *cu1, cx<double> *cu2, cx<double> *cu3
 // launchers
 // -------------------------------------------------------------------------
 
-static inline int chunk_w(int t, int threads) { int w = threads / t; return w < 1 ? 1 : w; }
+// Columns per block for the z transforms: the largest power of two with
+// W * T <= threads (T threads per transform); returns log2(W).
+static inline int chunk_lw(int t, int threads)
+{
+    int lw = 0;
+    while ((2 << lw) * t <= threads) ++lw;
+    return lw;
+}
 
 template <typename T, int E>
 static void zinv_e(hipStream_t s, const Geo &g, const Fft<T> &fz, const cx<T> *in, long in_fs, long in_ps,
                    int in_mk, int ncols, cx<T> *out, long out_fs, long out_ps, int out_mk, int j0, int nplanes,
                    int nfields)
 {
-    const int W = chunk_w(fz.len.t, 256);
+    const int lw = chunk_lw(fz.len.t, 256), W = 1 << lw;
     dim3 grid((ncols + W - 1) / W, nplanes, nfields);
     const size_t lds = sizeof(cx<T>) * W * fz.len.n;
     hipLaunchKernelGGL((k_zinv<T, E>), grid, dim3(W * fz.len.t), lds, s, fz.len, fz.tw, in, in_fs, in_ps, in_mk,
-                       ncols, out, out_fs, out_ps, out_mk, j0, g.nkz, g.tnkz, W);
+                       ncols, out, out_fs, out_ps, out_mk, j0, g.nkz, g.tnkz, lw);
 }
 
 #define DGPU_DISPATCH_E(E_, CALL)                                  \
@@ -540,12 +551,12 @@ template <typename T, int E>
 static void zfwd_nl_e(hipStream_t s, const Geo &g, const Fft<T> &fz, const cx<T> *wt, long wt_fs,
                       cx<T> *const *nl, const double *kz, int jlo, int jhi, int nout)
 {
-    const int W = chunk_w(fz.len.t, 256);
+    const int lw = chunk_lw(fz.len.t, 256), W = 1 << lw;
     dim3 grid((g.mk + W - 1) / W, jhi - jlo + 1, nout);
     const size_t lds = sizeof(cx<T>) * W * fz.len.n;
     const long wt_ps = (long)g.nz * g.mk;
     hipLaunchKernelGGL((k_zfwd_nl<T, E>), grid, dim3(W * fz.len.t), lds, s, fz.len, fz.tw, wt, wt_fs, wt_ps, g.mk,
-                       nl, g.ps, kz, jlo, g.nkz, g.tnkz, W);
+                       nl, g.ps, kz, jlo, g.nkz, g.tnkz, lw);
 }
 
 template <typename T>
@@ -577,11 +588,11 @@ template <typename T, int E>
 static void zfwd_pack_e(hipStream_t s, const Geo &g, const Fft<T> &fz, const cx<T> *xt, double *fort, int j0,
                         int nplanes)
 {
-    const int W = chunk_w(fz.len.t, 256);
+    const int lw = chunk_lw(fz.len.t, 256), W = 1 << lw;
     const int ncols = g.nx / 2 + 1;  // 0..NXP, zero beyond NKX
     dim3 grid((ncols + W - 1) / W, nplanes);
     hipLaunchKernelGGL((k_zfwd_pack<T, E>), grid, dim3(W * fz.len.t), sizeof(cx<T>) * W * fz.len.n, s, fz.len, fz.tw,
-                       xt, g.mk, fort, g.nx / 2, g.nkz, g.tnkz, j0, W);
+                       xt, g.mk, fort, g.nx / 2, g.nkz, g.tnkz, j0, lw);
 }
 
 template <typename T>
