@@ -37,6 +37,13 @@ struct dgpu_ctx {
     double *d_dy = nullptr, *d_dyf = nullptr, *d_kx = nullptr, *d_kz = nullptr, *d_kx2 = nullptr, *d_kz2 = nullptr;
     dgpu::Fft<double> fx, fz;
     cx<double> *d_tw = nullptr;
+    // mixed path: FP32 transforms, intermediates and nonlinear terms
+    int mixed = 0;
+    dgpu::Fft<float> fxf, fzf;
+    cx<float> *d_twf = nullptr;
+    cx<float> *d_ztf = nullptr, *d_wtf = nullptr, *d_nlf = nullptr;
+    cx<float> *nlf[3 + DGPU_MAX_TH] = {};
+    cx<float> **d_nlptrf = nullptr;
     // Fourier state, one allocation: [slot][j][K][I]
     cx<double> *d_state = nullptr;
     long fs = 0;  // complex elements per field
@@ -371,7 +378,7 @@ int32_t dgpu_create(const dgpu_params *p, dgpu_ctx **out)
     *out = nullptr;
     if (p->nx < 4 || p->nz < 4 || p->ny < 4 || (p->nx & 1) || (p->nz & 1)) return DGPU_E_ARG;
     if (p->n_th < 0 || p->n_th > 1) return DGPU_E_ARG;  // one active scalar supported by the fused kernels
-    if (p->precision != DGPU_FP64) return DGPU_E_ARG;
+    if (p->precision != DGPU_FP64 && p->precision != DGPU_MIXED) return DGPU_E_ARG;
     if (!p->gy) return DGPU_E_ARG;
     for (int code : {p->u_bc_ymin, p->w_bc_ymin, p->u_bc_ymax, p->w_bc_ymax})
         if (code != 0 && code != 1) return DGPU_E_ARG;
@@ -384,6 +391,7 @@ int32_t dgpu_create(const dgpu_params *p, dgpu_ctx **out)
     if (!c) return DGPU_E_NOMEM;
     c->p = *p;
     c->p.gy = c->p.gyf = c->p.dy = c->p.dyf = c->p.kx = c->p.kz = c->p.kx2 = c->p.kz2 = nullptr;
+    c->mixed = (p->precision == DGPU_MIXED);
     dgpu::Geo &g = c->g;
     g.nx = p->nx; g.ny = p->ny; g.nz = p->nz; g.nth = p->n_th;
     g.nkx = p->nx / 3; g.nkz = p->nz / 3; g.tnkz = 2 * g.nkz;
@@ -432,6 +440,8 @@ int32_t dgpu_create(const dgpu_params *p, dgpu_ctx **out)
         delete c;
         return DGPU_E_ARG;
     }
+    c->fxf.len = c->fx.len;
+    c->fzf.len = c->fz.len;
     int rc = 0;
     auto bail = [&](int code) { rc = code; dgpu_destroy(c); return code; };
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return bail(DGPU_E_HIP);
@@ -466,6 +476,16 @@ int32_t dgpu_create(const dgpu_params *p, dgpu_ctx **out)
             return bail(DGPU_E_HIP);
         c->fx.tw = c->d_tw;
         c->fz.tw = c->d_tw + g.nx;
+        if (c->mixed) {
+            std::vector<cx<float>> tf(g.nx + g.nz);
+            for (int j = 0; j < g.nx; ++j) { tf[j].re = (float)twx[j].re; tf[j].im = (float)twx[j].im; }
+            for (int j = 0; j < g.nz; ++j) { tf[g.nx + j].re = (float)twz[j].re; tf[g.nx + j].im = (float)twz[j].im; }
+            if (hipMalloc(&c->d_twf, sizeof(cx<float>) * tf.size()) != hipSuccess) return bail(DGPU_E_NOMEM);
+            if (hipMemcpy(c->d_twf, tf.data(), sizeof(cx<float>) * tf.size(), hipMemcpyHostToDevice) != hipSuccess)
+                return bail(DGPU_E_HIP);
+            c->fxf.tw = c->d_twf;
+            c->fzf.tw = c->d_twf + g.nx;
+        }
     }
     // state: cu1 cu2 cu3 cth[n] | cp cf1 cf2 cf3 cfth[n] | nl[3+n]
     const int nin = 3 + g.nth;
@@ -507,6 +527,18 @@ int32_t dgpu_create(const dgpu_params *p, dgpu_ctx **out)
     c->zfs = (long)(NY + 2) * g.nz * g.mk;
     if (hipMalloc(&c->d_zt, sizeof(cx<double>) * c->zfs * nin) != hipSuccess) return bail(DGPU_E_NOMEM);
     if (hipMalloc(&c->d_wt, sizeof(cx<double>) * c->zfs * 2 * nin) != hipSuccess) return bail(DGPU_E_NOMEM);
+    if (c->mixed) {
+        if (hipMalloc(&c->d_ztf, sizeof(cx<float>) * c->zfs * nin) != hipSuccess ||
+            hipMalloc(&c->d_wtf, sizeof(cx<float>) * c->zfs * 2 * nin) != hipSuccess ||
+            hipMalloc(&c->d_nlf, sizeof(cx<float>) * c->fs * nin) != hipSuccess ||
+            hipMalloc(&c->d_nlptrf, sizeof(cx<float> *) * (3 + DGPU_MAX_TH)) != hipSuccess)
+            return bail(DGPU_E_NOMEM);
+        if (hipMemset(c->d_nlf, 0, sizeof(cx<float>) * c->fs * nin) != hipSuccess) return bail(DGPU_E_HIP);
+        for (int f = 0; f < nin; ++f) c->nlf[f] = c->d_nlf + f * c->fs;
+        if (hipMemcpy(c->d_nlptrf, c->nlf, sizeof(cx<float> *) * (3 + DGPU_MAX_TH), hipMemcpyHostToDevice) !=
+            hipSuccess)
+            return bail(DGPU_E_HIP);
+    }
     // CN tables
     c->h_cn.assign(3L * nsys_of(c) * cn_block(c), 0.0);
     if (hipMalloc(&c->d_cn, sizeof(double) * c->h_cn.size()) != hipSuccess) return bail(DGPU_E_NOMEM);
@@ -533,7 +565,7 @@ int32_t dgpu_destroy(dgpu_ctx *c)
     for (auto &t : c->pending) { hipEventDestroy(t.a); hipEventDestroy(t.b); }
     for (auto e : c->event_pool) hipEventDestroy(e);
     void *bufs[] = {c->d_vec, c->d_tw, c->d_state, c->d_pmult, c->d_pinv, c->d_rd, c->d_nlptr, c->d_zt, c->d_wt,
-                    c->d_fa, c->d_fb, c->d_gyf,
+                    c->d_fa, c->d_fb, c->d_gyf, c->d_twf, c->d_ztf, c->d_wtf, c->d_nlf, c->d_nlptrf,
                     c->d_fort, c->d_cn, c->d_codes, c->d_c1};
     for (void *b : bufs)
         if (b) hipFree(b);
@@ -580,7 +612,9 @@ int32_t dgpu_get_field(dgpu_ctx *c, int32_t field, double *a)
     return DGPU_OK;
 }
 
-static void fill_ptrs(dgpu_ctx *c, dgpu::CnPtrs<double> &P)
+extern "C++" {
+template <typename NLT>
+static void fill_ptrs(dgpu_ctx *c, dgpu::CnPtrs<double, NLT> &P, cx<NLT> *const *nlsrc)
 {
     std::memset(&P, 0, sizeof(P));
     P.cu1 = c->cu[0];
@@ -592,12 +626,15 @@ static void fill_ptrs(dgpu_ctx *c, dgpu::CnPtrs<double> &P)
     P.cf3 = c->cf[2];
     P.cth = c->g.nth ? c->cth[0] : nullptr;
     P.cfth = c->g.nth ? c->cfth[0] : nullptr;
-    // zfwd_nl writes the x-, y-, z-momentum and scalar terms to nl[0..3]
-    P.nl[dgpu::SYS_U1] = c->nl[0];
-    P.nl[dgpu::SYS_U2] = c->nl[1];
-    P.nl[dgpu::SYS_U3] = c->nl[2];
-    P.nl[dgpu::SYS_TH0] = c->g.nth ? c->nl[3] : nullptr;
+    // zfwd_nl writes the x-, y-, z-momentum and scalar terms to slots 0..3;
+    // the FP64 slots c->nl also serve as the solves' scratch.
+    const int map[4] = {0, 2, 1, 3};  // SYS_U1, SYS_U3, SYS_U2, SYS_TH0 -> slot
+    for (int q = 0; q < 3 + c->g.nth; ++q) {
+        P.nl[q] = nlsrc[map[q]];
+        P.gs[q] = c->nl[map[q]];
+    }
 }
+}  // extern "C++"
 
 static void fill_rdargs(dgpu_ctx *c, dgpu::RdArgs &R, int update_cp, double temp4)
 {
@@ -629,25 +666,45 @@ int32_t dgpu_rk_chan_1(dgpu_ctx *c, int32_t rk_step, double time, const double *
     const int nin = 3 + g.nth;
     int jlo = 2;
     for (int n = 0; n < g.nth; ++n) jlo = std::min(jlo, c->jstart_th[n]);
-    {
-        TimerScope t(c, 0);
-        dgpu::launch_zinv<double>(c->stream, g, c->fz, c->cu[0], c->fs, g.ps, g.mk, g.mk, c->d_zt, c->zfs,
-                                  (long)g.nz * g.mk, g.mk, 0, g.ny + 2, nin);
+    if (c->mixed) {
+        {
+            TimerScope t(c, 0);
+            dgpu::launch_zinv<float, double>(c->stream, g, c->fzf, c->cu[0], c->fs, g.ps, g.mk, g.mk, c->d_ztf,
+                                             c->zfs, (long)g.nz * g.mk, g.mk, 0, g.ny + 2, nin);
+        }
+        if (int rc = check_launch(c, "zinv")) return rc;
+        {
+            TimerScope t(c, 1);
+            dgpu::launch_xfused<float>(c->stream, g, c->fxf, c->d_ztf, c->zfs, c->d_wtf, c->zfs, c->d_dy, c->d_dyf,
+                                       c->d_kx, jlo, g.ny);
+        }
+        if (int rc = check_launch(c, "xfused")) return rc;
+        {
+            TimerScope t(c, 2);
+            dgpu::launch_zfwd_nl<float, float>(c->stream, g, c->fzf, c->d_wtf, c->zfs, c->d_nlptrf, c->d_kz, jlo,
+                                               g.ny, nin);
+        }
+        if (int rc = check_launch(c, "zfwd_nl")) return rc;
+    } else {
+        {
+            TimerScope t(c, 0);
+            dgpu::launch_zinv<double, double>(c->stream, g, c->fz, c->cu[0], c->fs, g.ps, g.mk, g.mk, c->d_zt,
+                                              c->zfs, (long)g.nz * g.mk, g.mk, 0, g.ny + 2, nin);
+        }
+        if (int rc = check_launch(c, "zinv")) return rc;
+        {
+            TimerScope t(c, 1);
+            dgpu::launch_xfused<double>(c->stream, g, c->fx, c->d_zt, c->zfs, c->d_wt, c->zfs, c->d_dy, c->d_dyf,
+                                        c->d_kx, jlo, g.ny);
+        }
+        if (int rc = check_launch(c, "xfused")) return rc;
+        {
+            TimerScope t(c, 2);
+            dgpu::launch_zfwd_nl<double, double>(c->stream, g, c->fz, c->d_wt, c->zfs, c->d_nlptr, c->d_kz, jlo,
+                                                 g.ny, nin);
+        }
+        if (int rc = check_launch(c, "zfwd_nl")) return rc;
     }
-    if (int rc = check_launch(c, "zinv")) return rc;
-    {
-        TimerScope t(c, 1);
-        dgpu::launch_xfused<double>(c->stream, g, c->fx, c->d_zt, c->zfs, c->d_wt, c->zfs, c->d_dy, c->d_dyf, c->d_kx,
-                                    jlo, g.ny);
-    }
-    if (int rc = check_launch(c, "xfused")) return rc;
-    {
-        TimerScope t(c, 2);
-        dgpu::launch_zfwd_nl<double>(c->stream, g, c->fz, c->d_wt, c->zfs, c->d_nlptr, c->d_kz, jlo, g.ny, nin);
-    }
-    if (int rc = check_launch(c, "zfwd_nl")) return rc;
-    dgpu::CnPtrs<double> P;
-    fill_ptrs(c, P);
     dgpu::CnArgs A;
     std::memset(&A, 0, sizeof(A));
     A.g = g;
@@ -670,16 +727,32 @@ int32_t dgpu_rk_chan_1(dgpu_ctx *c, int32_t rk_step, double time, const double *
     A.kz = c->d_kz;
     A.kx2 = c->d_kx2;
     A.kz2 = c->d_kz2;
-    {
-        TimerScope t(c, 3);
-        dgpu::launch_cn<double>(c->stream, A, P);
-    }
-    if (int rc = check_launch(c, "cn")) return rc;
     dgpu::RdArgs R;
     fill_rdargs(c, R, 1, temp4);
-    {
-        TimerScope t(c, 4);
-        dgpu::launch_remdiv<double>(c->stream, R, P);
+    if (c->mixed) {
+        dgpu::CnPtrs<double, float> P;
+        fill_ptrs<float>(c, P, c->nlf);
+        {
+            TimerScope t(c, 3);
+            dgpu::launch_cn<double, float>(c->stream, A, P);
+        }
+        if (int rc = check_launch(c, "cn")) return rc;
+        {
+            TimerScope t(c, 4);
+            dgpu::launch_remdiv<double, float>(c->stream, R, P);
+        }
+    } else {
+        dgpu::CnPtrs<double, double> P;
+        fill_ptrs<double>(c, P, c->nl);
+        {
+            TimerScope t(c, 3);
+            dgpu::launch_cn<double, double>(c->stream, A, P);
+        }
+        if (int rc = check_launch(c, "cn")) return rc;
+        {
+            TimerScope t(c, 4);
+            dgpu::launch_remdiv<double, double>(c->stream, R, P);
+        }
     }
     if (int rc = check_launch(c, "remdiv")) return rc;
     return DGPU_OK;
@@ -688,11 +761,11 @@ int32_t dgpu_rk_chan_1(dgpu_ctx *c, int32_t rk_step, double time, const double *
 int32_t dgpu_rem_div_chan(dgpu_ctx *c)
 {
     if (!c) return DGPU_E_ARG;
-    dgpu::CnPtrs<double> P;
-    fill_ptrs(c, P);
+    dgpu::CnPtrs<double, double> P;
+    fill_ptrs<double>(c, P, c->nl);
     dgpu::RdArgs R;
     fill_rdargs(c, R, 0, 1.0);
-    dgpu::launch_remdiv<double>(c->stream, R, P);
+    dgpu::launch_remdiv<double, double>(c->stream, R, P);
     return check_launch(c, "rem_div");
 }
 
@@ -738,8 +811,8 @@ int32_t dgpu_fft_xz_to_physical(dgpu_ctx *c, double *vv, int32_t jmin, int32_t j
                              hipMemcpyHostToDevice, c->stream));
     const cx<double> *in = reinterpret_cast<const cx<double> *>(c->d_fort);
     const long in_ps = (long)(nxp + 1) * (g.nz + 2);
-    dgpu::launch_zinv<double>(c->stream, g, c->fz, in, 0, in_ps, nxp + 1, nxp, c->d_wt, 0, (long)g.nz * nxp, nxp,
-                              jmin, np, 1);
+    dgpu::launch_zinv<double, double>(c->stream, g, c->fz, in, 0, in_ps, nxp + 1, nxp, c->d_wt, 0, (long)g.nz * nxp,
+                                      nxp, jmin, np, 1);
     if (int rc = check_launch(c, "zinv(host)")) return rc;
     dgpu::launch_x_c2r<double>(c->stream, g, c->fx, c->d_wt, nxp, c->d_fort, jmin, np);
     if (int rc = check_launch(c, "x_c2r")) return rc;
@@ -755,8 +828,8 @@ static int fort_to_physical(dgpu_ctx *c, double *buf, int j0, int j1)
     const dgpu::Geo &g = c->g;
     const int nxp = g.nx / 2, np = j1 - j0 + 1;
     const cx<double> *in = reinterpret_cast<const cx<double> *>(buf);
-    dgpu::launch_zinv<double>(c->stream, g, c->fz, in, 0, (long)(nxp + 1) * (g.nz + 2), nxp + 1, nxp, c->d_wt, 0,
-                              (long)g.nz * nxp, nxp, j0, np, 1);
+    dgpu::launch_zinv<double, double>(c->stream, g, c->fz, in, 0, (long)(nxp + 1) * (g.nz + 2), nxp + 1, nxp,
+                                      c->d_wt, 0, (long)g.nz * nxp, nxp, j0, np, 1);
     if (int rc = check_launch(c, "zinv(planes)")) return rc;
     dgpu::launch_x_c2r<double>(c->stream, g, c->fx, c->d_wt, nxp, buf, j0, np);
     return check_launch(c, "x_c2r(planes)");

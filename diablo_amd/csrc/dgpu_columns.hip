@@ -28,8 +28,8 @@ __device__ __forceinline__ int wave_uniform(int v) { return __builtin_amdgcn_rea
 // ---------------------------------------------------------------------------
 // k_cn
 // ---------------------------------------------------------------------------
-template <typename T, int NTH>
-__global__ void __launch_bounds__(256) k_cn(CnArgs A, CnPtrs<T> P)
+template <typename T, int NTH, typename NLT>
+__global__ void __launch_bounds__(256) k_cn(CnArgs A, CnPtrs<T, NLT> P)
 {
     const Geo &g = A.g;
     const int q = wave_uniform(threadIdx.x >> 6);  // system of this wavefront
@@ -50,7 +50,8 @@ __global__ void __launch_bounds__(256) k_cn(CnArgs A, CnPtrs<T> P)
     // pointers of this system: the field it solves and its accumulator
     cx<T> *fld = (q == SYS_U1) ? P.cu1 : (q == SYS_U3) ? P.cu3 : (q == SYS_U2) ? P.cu2 : P.cth;
     cx<T> *cf = (q == SYS_U1) ? P.cf1 : (q == SYS_U3) ? P.cf3 : (q == SYS_U2) ? P.cf2 : P.cfth;
-    cx<T> *nl = P.nl[q];
+    const cx<NLT> *nl = P.nl[q];
+    cx<T> *gs = P.gs[q];
     const T visc = (q == SYS_TH0) ? T(A.nupr) * pw : T(A.nu) * pw;
 
     // ---------------- sweep A ----------------
@@ -68,7 +69,7 @@ __global__ void __launch_bounds__(256) k_cn(CnArgs A, CnPtrs<T> P)
             fp = (j <= NY) ? fld[o + ps] : zero;
             cx<T> nlv = zero, cfo = zero, cp0 = zero, aux1 = zero, aux2 = zero;
             if (cfr || !isbc) {
-                nlv = nl[o];
+                nlv = make_cx<T>(T(nl[o].re), T(nl[o].im));
                 if (A.rk_gt1) cfo = cf[o];
             }
             // explicit-term inputs
@@ -121,7 +122,7 @@ __global__ void __launch_bounds__(256) k_cn(CnArgs A, CnPtrs<T> P)
             }
             const cx<T> gj = (j == 0) ? vec : fma_c(mult, gprev, vec);
             if (cfr) cf[o] = cfn;
-            nl[o] = gj;
+            gs[o] = gj;
             gprev = gj;
             fm = f0;
             f0 = fp;
@@ -136,7 +137,7 @@ __global__ void __launch_bounds__(256) k_cn(CnArgs A, CnPtrs<T> P)
         for (int j = NY + 1; j >= 0; --j) {
             const long o = m + (long)j * ps;
             const double *t = tab + (long)j * kRow;
-            const cx<T> gj = nl[o];
+            const cx<T> gj = gs[o];
             const T ip = T(t[T_INVPIV]);
             if (j == NY + 1) {
                 x = make_cx<T>(gj.re * ip, gj.im * ip);
@@ -189,8 +190,8 @@ __global__ void k_pois_factor(Geo g, const double *__restrict__ dy, const double
 // ---------------------------------------------------------------------------
 // k_remdiv
 // ---------------------------------------------------------------------------
-template <typename T>
-__global__ void __launch_bounds__(64) k_remdiv(RdArgs A, CnPtrs<T> P)
+template <typename T, typename NLT>
+__global__ void __launch_bounds__(64) k_remdiv(RdArgs A, CnPtrs<T, NLT> P)
 {
     const Geo &g = A.g;
     const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -201,7 +202,7 @@ __global__ void __launch_bounds__(64) k_remdiv(RdArgs A, CnPtrs<T> P)
     const long ps = g.ps;
     const T kx = T(A.kx[i]), kz = T(A.kz[s]);
     const cx<T> zero = make_cx<T>(0, 0);
-    cx<T> *scr = P.nl[0];  // forward-eliminated right-hand side
+    cx<T> *scr = P.gs[0];  // forward-eliminated right-hand side
     // sweep C: rows 2..NY (row 1 is phi(1) = 0 / phi(1) - phi(2) = 0, RHS 0)
     {
         cx<T> gprev = zero;
@@ -261,21 +262,21 @@ __global__ void __launch_bounds__(64) k_remdiv(RdArgs A, CnPtrs<T> P)
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-template <typename T>
-void launch_cn(hipStream_t s, const CnArgs &a, const CnPtrs<T> &p)
+template <typename T, typename NLT>
+void launch_cn(hipStream_t s, const CnArgs &a, const CnPtrs<T, NLT> &p)
 {
     const long blocks = (a.g.ps + 63) / 64;
     if (a.g.nth == 0)
-        hipLaunchKernelGGL((k_cn<T, 0>), dim3(blocks), dim3(256), 0, s, a, p);
+        hipLaunchKernelGGL((k_cn<T, 0, NLT>), dim3(blocks), dim3(256), 0, s, a, p);
     else
-        hipLaunchKernelGGL((k_cn<T, 1>), dim3(blocks), dim3(256), 0, s, a, p);
+        hipLaunchKernelGGL((k_cn<T, 1, NLT>), dim3(blocks), dim3(256), 0, s, a, p);
 }
 
-template <typename T>
-void launch_remdiv(hipStream_t s, const RdArgs &a, const CnPtrs<T> &p)
+template <typename T, typename NLT>
+void launch_remdiv(hipStream_t s, const RdArgs &a, const CnPtrs<T, NLT> &p)
 {
     const long blocks = (a.g.ps + 63) / 64;
-    hipLaunchKernelGGL((k_remdiv<T>), dim3(blocks), dim3(64), 0, s, a, p);
+    hipLaunchKernelGGL((k_remdiv<T, NLT>), dim3(blocks), dim3(64), 0, s, a, p);
 }
 
 void launch_pois_factor(hipStream_t s, const Geo &g, const double *dy, const double *dyf, const double *kx2,
@@ -285,7 +286,9 @@ void launch_pois_factor(hipStream_t s, const Geo &g, const double *dy, const dou
     hipLaunchKernelGGL(k_pois_factor, dim3(blocks), dim3(256), 0, s, g, dy, dyf, kx2, kz2, pmult, pinv);
 }
 
-template void launch_cn<double>(hipStream_t, const CnArgs &, const CnPtrs<double> &);
-template void launch_remdiv<double>(hipStream_t, const RdArgs &, const CnPtrs<double> &);
+template void launch_cn<double, double>(hipStream_t, const CnArgs &, const CnPtrs<double, double> &);
+template void launch_cn<double, float>(hipStream_t, const CnArgs &, const CnPtrs<double, float> &);
+template void launch_remdiv<double, double>(hipStream_t, const RdArgs &, const CnPtrs<double, double> &);
+template void launch_remdiv<double, float>(hipStream_t, const RdArgs &, const CnPtrs<double, float> &);
 
 }  // namespace dgpu

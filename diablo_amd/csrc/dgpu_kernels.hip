@@ -32,13 +32,16 @@ __device__ __forceinline__ int kzslot_to_z(int s, int nkz, int tnkz, int nz)
     return s <= nkz ? s : nz - (tnkz + 1 - s);
 }
 
+template <typename TO, typename FROM>
+__device__ __forceinline__ cx<TO> cvt(cx<FROM> a) { return make_cx<TO>(TO(a.re), TO(a.im)); }
+
 // -------------------------------------------------------------------------
 // z-inverse: kept slots -> length-NZ columns (FFT_XZ_TO_PHYSICAL z part,
 // for/mpi.f:1627-1641): unpack, zero the gap, unnormalised backward C2C.
 // grid (col-chunks, planes, fields); block W*T threads; LDS W*NZ complex.
 // -------------------------------------------------------------------------
-template <typename T, int E>
-__global__ void k_zinv(FftLen fz, const cx<T> *__restrict__ tw, const cx<T> *__restrict__ in, long in_fs,
+template <typename T, int E, typename TIN>
+__global__ void k_zinv(FftLen fz, const cx<T> *__restrict__ tw, const cx<TIN> *__restrict__ in, long in_fs,
                        long in_ps, int in_mk, int ncols, cx<T> *__restrict__ out, long out_fs, long out_ps,
                        int out_mk, int j0, int nkz, int tnkz, int lw)
 {
@@ -50,7 +53,7 @@ __global__ void k_zinv(FftLen fz, const cx<T> *__restrict__ tw, const cx<T> *__r
     const int i0 = blockIdx.x * W;
     const int j = j0 + blockIdx.y;
     const int f = blockIdx.z;
-    const cx<T> *src = in + f * in_fs + (long)j * in_ps;
+    const cx<TIN> *src = in + f * in_fs + (long)j * in_ps;
     cx<T> *dst = out + f * out_fs + (long)j * out_ps;
     const cx<T> zero = make_cx<T>(0, 0);
     for (int idx = tid; idx < W * nz; idx += nthr) lds[idx] = zero;
@@ -59,7 +62,7 @@ __global__ void k_zinv(FftLen fz, const cx<T> *__restrict__ tw, const cx<T> *__r
     for (int idx = tid; idx < W * mz; idx += nthr) {
         const int s = idx >> lw, b = idx & (W - 1);
         const int i = i0 + b;
-        if (i < ncols) lds[kzslot_to_z(s, nkz, tnkz, nz) * W + b] = src[(long)s * in_mk + i];
+        if (i < ncols) lds[kzslot_to_z(s, nkz, tnkz, nz) * W + b] = cvt<T>(src[(long)s * in_mk + i]);
     }
     __syncthreads();
     // batch index fastest across lanes: LDS accesses of a wavefront are contiguous
@@ -214,9 +217,9 @@ __global__ void k_xfused(FftLen fx, const cx<T> *__restrict__ tw, int mk, const 
 // zfwd_nl: NL_f(j, K, I) = [FFT_z G_f + i kz FFT_z H_f] / NZ at kept slots.
 // grid (col-chunks, planes, outputs); block W*T; LDS W*NZ complex.
 // -------------------------------------------------------------------------
-template <typename T, int E>
+template <typename T, int E, typename TOUT>
 __global__ void k_zfwd_nl(FftLen fz, const cx<T> *__restrict__ tw, const cx<T> *__restrict__ wt, long wt_fs,
-                          long wt_ps, int mk, cx<T> *const *__restrict__ nl, long ps, const double *__restrict__ kzv,
+                          long wt_ps, int mk, cx<TOUT> *const *__restrict__ nl, long ps, const double *__restrict__ kzv,
                           int jlo, int nkz, int tnkz, int lw)
 {
     extern __shared__ unsigned char smem_raw[];
@@ -259,14 +262,14 @@ __global__ void k_zfwd_nl(FftLen fz, const cx<T> *__restrict__ tw, const cx<T> *
         }
         __syncthreads();
     }
-    cx<T> *dst = nl[o] + (long)j * ps;
+    cx<TOUT> *dst = nl[o] + (long)j * ps;
 #pragma unroll
     for (int it = 0; it < MAXIT; ++it) {
         const int idx = tid + it * nthr;
         if (idx < W * mz) {
             const int s = idx >> lw, bb = idx & (W - 1);
             const int i = i0 + bb;
-            if (i < mk) dst[(long)s * mk + i] = keep[it];
+            if (i < mk) dst[(long)s * mk + i] = cvt<TOUT>(keep[it]);
         }
     }
 }
@@ -484,15 +487,15 @@ static inline int chunk_lw(int t, int threads)
     return lw;
 }
 
-template <typename T, int E>
-static void zinv_e(hipStream_t s, const Geo &g, const Fft<T> &fz, const cx<T> *in, long in_fs, long in_ps,
+template <typename T, int E, typename TIN>
+static void zinv_e(hipStream_t s, const Geo &g, const Fft<T> &fz, const cx<TIN> *in, long in_fs, long in_ps,
                    int in_mk, int ncols, cx<T> *out, long out_fs, long out_ps, int out_mk, int j0, int nplanes,
                    int nfields)
 {
     const int lw = chunk_lw(fz.len.t, 256), W = 1 << lw;
     dim3 grid((ncols + W - 1) / W, nplanes, nfields);
     const size_t lds = sizeof(cx<T>) * W * fz.len.n;
-    hipLaunchKernelGGL((k_zinv<T, E>), grid, dim3(W * fz.len.t), lds, s, fz.len, fz.tw, in, in_fs, in_ps, in_mk,
+    hipLaunchKernelGGL((k_zinv<T, E, TIN>), grid, dim3(W * fz.len.t), lds, s, fz.len, fz.tw, in, in_fs, in_ps, in_mk,
                        ncols, out, out_fs, out_ps, out_mk, j0, g.nkz, g.tnkz, lw);
 }
 
@@ -506,11 +509,11 @@ static void zinv_e(hipStream_t s, const Geo &g, const Fft<T> &fz, const cx<T> *i
     default: break;                                                \
     }
 
-template <typename T>
-void launch_zinv(hipStream_t s, const Geo &g, const Fft<T> &fz, const cx<T> *in, long in_fs, long in_ps, int in_mk,
+template <typename T, typename TIN>
+void launch_zinv(hipStream_t s, const Geo &g, const Fft<T> &fz, const cx<TIN> *in, long in_fs, long in_ps, int in_mk,
                  int ncols, cx<T> *out, long out_fs, long out_ps, int out_mk, int j0, int nplanes, int nfields)
 {
-    DGPU_DISPATCH_E(fz.len.e, (zinv_e<T, EE>(s, g, fz, in, in_fs, in_ps, in_mk, ncols, out, out_fs, out_ps, out_mk,
+    DGPU_DISPATCH_E(fz.len.e, (zinv_e<T, EE, TIN>(s, g, fz, in, in_fs, in_ps, in_mk, ncols, out, out_fs, out_ps, out_mk,
                                              j0, nplanes, nfields)));
 }
 
@@ -547,23 +550,23 @@ void launch_xfused(hipStream_t s, const Geo &g, const Fft<T> &fx, const cx<T> *z
     }
 }
 
-template <typename T, int E>
+template <typename T, int E, typename TOUT>
 static void zfwd_nl_e(hipStream_t s, const Geo &g, const Fft<T> &fz, const cx<T> *wt, long wt_fs,
-                      cx<T> *const *nl, const double *kz, int jlo, int jhi, int nout)
+                      cx<TOUT> *const *nl, const double *kz, int jlo, int jhi, int nout)
 {
     const int lw = chunk_lw(fz.len.t, 256), W = 1 << lw;
     dim3 grid((g.mk + W - 1) / W, jhi - jlo + 1, nout);
     const size_t lds = sizeof(cx<T>) * W * fz.len.n;
     const long wt_ps = (long)g.nz * g.mk;
-    hipLaunchKernelGGL((k_zfwd_nl<T, E>), grid, dim3(W * fz.len.t), lds, s, fz.len, fz.tw, wt, wt_fs, wt_ps, g.mk,
+    hipLaunchKernelGGL((k_zfwd_nl<T, E, TOUT>), grid, dim3(W * fz.len.t), lds, s, fz.len, fz.tw, wt, wt_fs, wt_ps, g.mk,
                        nl, g.ps, kz, jlo, g.nkz, g.tnkz, lw);
 }
 
-template <typename T>
-void launch_zfwd_nl(hipStream_t s, const Geo &g, const Fft<T> &fz, const cx<T> *wt, long wt_fs, cx<T> *const *nl,
+template <typename T, typename TOUT>
+void launch_zfwd_nl(hipStream_t s, const Geo &g, const Fft<T> &fz, const cx<T> *wt, long wt_fs, cx<TOUT> *const *nl,
                     const double *kz, int jlo, int jhi, int nout)
 {
-    DGPU_DISPATCH_E(fz.len.e, (zfwd_nl_e<T, EE>(s, g, fz, wt, wt_fs, nl, kz, jlo, jhi, nout)));
+    DGPU_DISPATCH_E(fz.len.e, (zfwd_nl_e<T, EE, TOUT>(s, g, fz, wt, wt_fs, nl, kz, jlo, jhi, nout)));
 }
 
 template <typename T, int E>
@@ -639,13 +642,19 @@ void launch_apply_bc_vel(hipStream_t s, const Geo &g, cx<double> *cu1, cx<double
                        dyf);
 }
 
-// explicit instantiations
-template void launch_zinv<double>(hipStream_t, const Geo &, const Fft<double> &, const cx<double> *, long, long, int,
-                                  int, cx<double> *, long, long, int, int, int, int);
+// explicit instantiations: FP64 path and the mixed path (FP32 transforms)
+template void launch_zinv<double, double>(hipStream_t, const Geo &, const Fft<double> &, const cx<double> *, long,
+                                          long, int, int, cx<double> *, long, long, int, int, int, int);
+template void launch_zinv<float, double>(hipStream_t, const Geo &, const Fft<float> &, const cx<double> *, long,
+                                         long, int, int, cx<float> *, long, long, int, int, int, int);
 template void launch_xfused<double>(hipStream_t, const Geo &, const Fft<double> &, const cx<double> *, long,
                                     cx<double> *, long, const double *, const double *, const double *, int, int);
-template void launch_zfwd_nl<double>(hipStream_t, const Geo &, const Fft<double> &, const cx<double> *, long,
-                                     cx<double> *const *, const double *, int, int, int);
+template void launch_xfused<float>(hipStream_t, const Geo &, const Fft<float> &, const cx<float> *, long,
+                                   cx<float> *, long, const double *, const double *, const double *, int, int);
+template void launch_zfwd_nl<double, double>(hipStream_t, const Geo &, const Fft<double> &, const cx<double> *, long,
+                                             cx<double> *const *, const double *, int, int, int);
+template void launch_zfwd_nl<float, float>(hipStream_t, const Geo &, const Fft<float> &, const cx<float> *, long,
+                                           cx<float> *const *, const double *, int, int, int);
 template void launch_x_r2c<double>(hipStream_t, const Geo &, const Fft<double> &, const double *, cx<double> *, int,
                                    int);
 template void launch_zfwd_pack<double>(hipStream_t, const Geo &, const Fft<double> &, const cx<double> *, double *,

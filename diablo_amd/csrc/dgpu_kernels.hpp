@@ -48,11 +48,12 @@ struct CnArgs {
     const double *kx, *kz, *kx2, *kz2;
 };
 
-template <typename T>
+template <typename T, typename NLT = T>
 struct CnPtrs {
     cx<T> *cu1, *cu2, *cu3, *cp, *cth;
     cx<T> *cf1, *cf2, *cf3, *cfth;
-    cx<T> *nl[4];  // nonlinear term per system (index SYS_*), reused as solve scratch
+    const cx<NLT> *nl[4];  // nonlinear term per system (index SYS_*)
+    cx<T> *gs[4];          // solve scratch per system (may alias nl when NLT == T)
 };
 
 // REM_DIV_CHAN: per-mode Poisson factors and per-row constants
@@ -65,8 +66,8 @@ struct RdArgs {
 };
 
 // ---- launchers (dgpu_kernels.hip) ----
-template <typename T>
-void launch_zinv(hipStream_t s, const Geo &g, const Fft<T> &fz, const cx<T> *in, long in_fs, long in_ps,
+template <typename T, typename TIN>
+void launch_zinv(hipStream_t s, const Geo &g, const Fft<T> &fz, const cx<TIN> *in, long in_fs, long in_ps,
                  int in_mk, int ncols, cx<T> *out, long out_fs, long out_ps, int out_mk, int j0, int nplanes,
                  int nfields);
 
@@ -74,14 +75,14 @@ template <typename T>
 void launch_xfused(hipStream_t s, const Geo &g, const Fft<T> &fx, const cx<T> *zt, long zt_fs, cx<T> *wt,
                    long wt_fs, const double *dy, const double *dyf, const double *kx, int jlo, int jhi);
 
-template <typename T>
-void launch_zfwd_nl(hipStream_t s, const Geo &g, const Fft<T> &fz, const cx<T> *wt, long wt_fs, cx<T> *const *nl,
+template <typename T, typename TOUT>
+void launch_zfwd_nl(hipStream_t s, const Geo &g, const Fft<T> &fz, const cx<T> *wt, long wt_fs, cx<TOUT> *const *nl,
                     const double *kz, int jlo, int jhi, int nout);
 
-template <typename T>
-void launch_cn(hipStream_t s, const CnArgs &a, const CnPtrs<T> &p);
-template <typename T>
-void launch_remdiv(hipStream_t s, const RdArgs &a, const CnPtrs<T> &p);
+template <typename T, typename NLT>
+void launch_cn(hipStream_t s, const CnArgs &a, const CnPtrs<T, NLT> &p);
+template <typename T, typename NLT>
+void launch_remdiv(hipStream_t s, const RdArgs &a, const CnPtrs<T, NLT> &p);
 void launch_pois_factor(hipStream_t s, const Geo &g, const double *dy, const double *dyf, const double *kx2,
                         const double *kz2, double *pmult, double *pinv);
 

@@ -101,7 +101,10 @@ def _dp(a: np.ndarray):
     return a.ctypes.data_as(DP)
 
 
-def params_of(cfg: ChannelConfig, gy_1: np.ndarray) -> Params:
+PRECISIONS = {"fp64": 0, "mixed": 1}
+
+
+def params_of(cfg: ChannelConfig, gy_1: np.ndarray, precision: str = "fp64") -> Params:
     p = Params()
     for name, _ in Params._fields_:
         if name in ("gy", "gyf", "dy", "dyf", "kx", "kz", "kx2", "kz2", "precision"):
@@ -113,7 +116,7 @@ def params_of(cfg: ChannelConfig, gy_1: np.ndarray) -> Params:
                 arr[n] = x
         else:
             setattr(p, name, v)
-    p.precision = 0
+    p.precision = PRECISIONS[precision]
     p.gy = _dp(gy_1)
     return p
 
@@ -125,12 +128,15 @@ class DgpuError(RuntimeError):
 class GpuChannel:
     """The channel state on one MI355X, driven through the C-ABI."""
 
-    def __init__(self, cfg: ChannelConfig, gy_1: np.ndarray | None = None):
+    def __init__(self, cfg: ChannelConfig, gy_1: np.ndarray | None = None, precision: str = "fp64"):
+        """precision: "fp64" (every kernel FP64) or "mixed" (FP32 transforms,
+        products and nonlinear terms; FP64 state and column solves)."""
         cfg.validate()
         self.cfg = cfg
+        self.precision = precision
         self.L = load_library()
         self._gy = np.ascontiguousarray(grid_gy(cfg) if gy_1 is None else gy_1, dtype=np.float64)
-        self._p = params_of(cfg, self._gy)
+        self._p = params_of(cfg, self._gy, precision)
         h = C.c_void_p()
         rc = self.L.dgpu_create(C.byref(self._p), C.byref(h))
         if rc != 0 or not h.value:

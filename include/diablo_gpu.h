@@ -45,7 +45,12 @@ enum dgpu_status {
     DGPU_E_STATE = -4     /* call not valid in the current state */
 };
 
-enum dgpu_precision { DGPU_FP64 = 0 };
+/* DGPU_FP64: every kernel in FP64.
+ * DGPU_MIXED: the state (CU, CP, CTH, CF) and the per-mode Crank-Nicolson,
+ * projection and CP solves stay FP64 (so the (0,0) mean mode, the pressure
+ * gauge and the Poisson solve are FP64); the x-z transforms, the nonlinear
+ * products and the nonlinear-term accumulators run in FP32. */
+enum dgpu_precision { DGPU_FP64 = 0, DGPU_MIXED = 1 };
 
 /* Field identifiers for dgpu_set_field / dgpu_get_field. */
 enum dgpu_field {

@@ -192,3 +192,27 @@ def test_poisson_p_chan(oracle_mod, gpu_mod, maker):
     err = np.abs(a - b).max() / np.abs(b).max()
     say("CP rel err", err)
     assert err < 1e-11
+
+
+@pytest.mark.parametrize("maker,nsteps", [("channel", 100), ("kh", 100)])
+def test_mixed_precision_error(gpu_mod, maker, nsteps):
+    """Mixed path (FP32 transforms/products/NL, FP64 state and solves) against
+    the FP64 path from the same initial state: rel-L2 <= 1e-5, KE drift and
+    max|div u| reported."""
+    from diablo_amd import metrics
+    cfg = {"channel": lambda: case.channel_case(48, 33, 48, kick=0.05),
+           "kh": lambda: case.kh_case(48, 33, 48, kick=0.05, dt=0.01)}[maker]()
+    phys = case.initial_physical(cfg)
+    g64 = gpu_mod.GpuChannel(cfg, precision="fp64")
+    gmx = gpu_mod.GpuChannel(cfg, precision="mixed")
+    g64.initialize(*phys)
+    gmx.initialize(*phys)
+    g64.step(nsteps)
+    gmx.step(nsteps)
+    g64.synchronize()
+    gmx.synchronize()
+    r = metrics.compare(g64, g64.state(), gmx, gmx.state())
+    say(maker, nsteps, r)
+    assert r["rel_l2"] <= 1e-5
+    assert r["ke_drift"] <= 1e-5
+    assert r["max_div"] <= 1e-10 * max(1.0, r["max_div_ref"] * 1e10)
