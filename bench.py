@@ -61,6 +61,44 @@ def substep_model_bytes(cfg, s: int = 8) -> float:
     return s * (2 * (3 + cfg.n_th) * P + (23 + 7 * cfg.n_th) * F)
 
 
+def time_path(cfg, phys, precision: str, steps: int, warmup: int, kernel_timing: bool):
+    """Initial state -> warm-up -> timed steps; returns (seconds, kernel times, channel)."""
+    from diablo_amd import gpu
+    g = gpu.GpuChannel(cfg, precision=precision)
+    g.initialize(*phys)
+    g.step(warmup)
+    g.synchronize()
+    if kernel_timing:
+        g.set_kernel_timing(True)
+        g.kernel_times()  # reset
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.step(steps)
+    g.synchronize()
+    elapsed = time.perf_counter() - t0
+    kt = g.kernel_times() if kernel_timing else {}
+    g.set_kernel_timing(False)
+    return elapsed, kt, g
+
+
+def accuracy(cfg, phys, steps: int) -> dict:
+    """Mixed path against the FP64 path after `steps` steps from the same state."""
+    from diablo_amd import gpu, metrics
+    g64 = gpu.GpuChannel(cfg, precision="fp64")
+    gmx = gpu.GpuChannel(cfg, precision="mixed")
+    g64.initialize(*phys)
+    gmx.initialize(*phys)
+    g64.step(steps)
+    gmx.step(steps)
+    g64.synchronize()
+    gmx.synchronize()
+    r = metrics.compare(g64, g64.state(), gmx, gmx.state())
+    g64.close()
+    gmx.close()
+    r["steps"] = steps
+    return r
+
+
 def cpu_baseline(cfg, phys, threads: int) -> dict:
     from oracle import oracle
     o = oracle.Oracle(cfg, threads=threads)
@@ -83,6 +121,8 @@ def main():
     ap.add_argument("--workload", default="channel384")
     ap.add_argument("--cpu-baseline", choices=("on", "off"), default="on")
     ap.add_argument("--no-kernel-timing", action="store_true")
+    ap.add_argument("--mixed", choices=("on", "off"), default="on")
+    ap.add_argument("--accuracy-steps", type=int, default=100)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: this build runs the channel case on a single GPU "
@@ -94,28 +134,15 @@ def main():
     cfg = case.WORKLOADS[args.workload]()
     log(f"[bench] workload {args.workload}: {cfg.nx}x{cfg.ny}x{cfg.nz}, building initial state")
     phys = case.initial_physical(cfg)
-    g = gpu.GpuChannel(cfg)
-    g.initialize(*phys)
-    log("[bench] initial state on device; warmup")
-    g.step(args.warmup)
-    g.synchronize()
-    if not args.no_kernel_timing:
-        g.set_kernel_timing(True)
-        g.kernel_times()  # reset
-    log(f"[bench] timing {args.steps} steps")
-    g.synchronize()
-    t0 = time.perf_counter()
-    g.step(args.steps)
-    g.synchronize()
-    elapsed = time.perf_counter() - t0
-    kt = g.kernel_times() if not args.no_kernel_timing else {}
-    g.set_kernel_timing(False)
+    log(f"[bench] FP64: warmup {args.warmup}, timing {args.steps} steps")
+    elapsed, kt, g = time_path(cfg, phys, "fp64", args.steps, args.warmup, not args.no_kernel_timing)
     substeps = 3 * args.steps
     value = cfg.points * substeps / elapsed / 1e6
     # sanity: the state must stay finite
     cu1 = g.get_field(gpu.CU1)
     if not math.isfinite(float(abs(cu1).max())):
         raise RuntimeError("state diverged")
+    g.close()
 
     jlo = min([2] + [1 if cfg.th_bc_ymin[n] == 1 else 2 for n in range(cfg.n_th)])
     kb = kernel_bytes(cfg, jlo)
@@ -138,6 +165,24 @@ def main():
     model = substep_model_bytes(cfg)
     log(f"[bench] {value:.1f} Mpts*substeps/s, {elapsed / args.steps * 1e3:.2f} ms/step; stage model "
         f"{model * substeps / elapsed / 1e9:.0f} GB/s ({model * substeps / elapsed / 1e12 / 8 * 100:.1f}% of 8 TB/s)")
+
+    mixed = None
+    if args.mixed == "on":
+        log(f"[bench] mixed: warmup {args.warmup}, timing {args.steps} steps")
+        el_m, kt_m, gm = time_path(cfg, phys, "mixed", args.steps, args.warmup, not args.no_kernel_timing)
+        gm.close()
+        vm = cfg.points * substeps / el_m / 1e6
+        for k, (ms, n) in kt_m.items():
+            if n:
+                log(f"[bench] mixed kernel {k:8s} mean {ms / n:8.3f} ms")
+        log(f"[bench] mixed {vm:.1f} Mpts*substeps/s ({vm / value:.2f}x FP64); accuracy over "
+            f"{args.accuracy_steps} steps")
+        acc = accuracy(cfg, phys, args.accuracy_steps)
+        log(f"[bench] mixed vs FP64: {acc}")
+        mixed = {"value": round(vm, 2), "unit": "Mpts*substeps/s", "ms_per_step": round(el_m / args.steps * 1e3, 3),
+                 "speedup_vs_fp64": round(vm / value, 3), "dtype": "f32 transforms/products, f64 state and solves",
+                 "rel_l2_vs_fp64": acc["rel_l2"], "ke_drift_vs_fp64": acc["ke_drift"],
+                 "max_div": acc["max_div"], "accuracy_steps": acc["steps"]}
 
     cpu = None
     if args.cpu_baseline == "on":
@@ -165,6 +210,7 @@ def main():
                    "substeps_per_step": 3, "stage_model_bytes_per_substep": model},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "mixed": mixed,
     }
     print(json.dumps(out), flush=True)
 
