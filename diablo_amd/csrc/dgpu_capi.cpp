@@ -70,6 +70,10 @@ struct dgpu_ctx {
     int *d_codes = nullptr;
     double *d_c1 = nullptr;
     hipStream_t stream = nullptr;
+    bool own_stream = true;
+    bool use_graphs = true;
+    hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
+    double gkey[3][4] = {};
     std::string err;
     bool timing = false;
     std::vector<TimedLaunch> pending;
@@ -563,13 +567,15 @@ int32_t dgpu_destroy(dgpu_ctx *c)
     if (!c) return DGPU_OK;
     if (c->stream) hipStreamSynchronize(c->stream);
     for (auto &t : c->pending) { hipEventDestroy(t.a); hipEventDestroy(t.b); }
+    for (auto &ge : c->gexec)
+        if (ge) hipGraphExecDestroy(ge);
     for (auto e : c->event_pool) hipEventDestroy(e);
     void *bufs[] = {c->d_vec, c->d_tw, c->d_state, c->d_pmult, c->d_pinv, c->d_rd, c->d_nlptr, c->d_zt, c->d_wt,
                     c->d_fa, c->d_fb, c->d_gyf, c->d_twf, c->d_ztf, c->d_wtf, c->d_nlf, c->d_nlptrf,
                     c->d_fort, c->d_cn, c->d_codes, c->d_c1};
     for (void *b : bufs)
         if (b) hipFree(b);
-    if (c->stream) hipStreamDestroy(c->stream);
+    if (c->stream && c->own_stream) hipStreamDestroy(c->stream);
     delete c;
     return DGPU_OK;
 }
@@ -651,17 +657,11 @@ static void fill_rdargs(dgpu_ctx *c, dgpu::RdArgs &R, int update_cp, double temp
     R.kz = c->d_kz;
 }
 
-int32_t dgpu_rk_chan_1(dgpu_ctx *c, int32_t rk_step, double time, const double *h_bar, const double *beta_bar,
-                       const double *zeta_bar)
+// Enqueues the five kernels of one substep on the context's stream.
+static int rk_enqueue(dgpu_ctx *c, int rk_step, double time, double temp1, double temp3, double temp4, double temp5)
 {
-    if (!c || !h_bar || !beta_bar || !zeta_bar || rk_step < 1 || rk_step > 3) return DGPU_E_ARG;
     const int k = rk_step - 1;
-    // for/channel.f:77-81
-    const double temp1 = c->p.nu * h_bar[k] / 2.0;
-    const double temp3 = zeta_bar[k] * h_bar[k];
-    const double temp4 = h_bar[k];
-    const double temp5 = beta_bar[k] * h_bar[k];
-    if (int rc = ensure_cn(c, k, temp1, temp4)) return rc;
+    (void)temp1;
     const dgpu::Geo &g = c->g;
     const int nin = 3 + g.nth;
     int jlo = 2;
@@ -755,6 +755,52 @@ int32_t dgpu_rk_chan_1(dgpu_ctx *c, int32_t rk_step, double time, const double *
         }
     }
     if (int rc = check_launch(c, "remdiv")) return rc;
+    return DGPU_OK;
+}
+
+int32_t dgpu_rk_chan_1(dgpu_ctx *c, int32_t rk_step, double time, const double *h_bar, const double *beta_bar,
+                       const double *zeta_bar)
+{
+    if (!c || !h_bar || !beta_bar || !zeta_bar || rk_step < 1 || rk_step > 3) return DGPU_E_ARG;
+    const int k = rk_step - 1;
+    // for/channel.f:77-81
+    const double temp1 = c->p.nu * h_bar[k] / 2.0;
+    const double temp3 = zeta_bar[k] * h_bar[k];
+    const double temp4 = h_bar[k];
+    const double temp5 = beta_bar[k] * h_bar[k];
+    if (int rc = ensure_cn(c, k, temp1, temp4)) return rc;
+    // One captured graph per RK stage.  Not used while kernels are timed one
+    // by one, or when the forcing depends on TIME (F_TYPE = 2).
+    const bool graphable = c->use_graphs && !c->timing && c->p.f_type != 2;
+    if (!graphable) return rk_enqueue(c, rk_step, time, temp1, temp3, temp4, temp5);
+    const double key[4] = {temp1, temp3, temp4, temp5};
+    if (!(c->gexec[k] && std::memcmp(key, c->gkey[k], sizeof(key)) == 0)) {
+        if (c->gexec[k]) {
+            HIPCHK(c, hipGraphExecDestroy(c->gexec[k]));
+            c->gexec[k] = nullptr;
+        }
+        hipGraph_t graph = nullptr;
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = rk_enqueue(c, rk_step, time, temp1, temp3, temp4, temp5);
+        const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
+        if (rc) {
+            if (graph) hipGraphDestroy(graph);
+            return rc;
+        }
+        HIPCHK(c, ec);
+        const hipError_t ei = hipGraphInstantiate(&c->gexec[k], graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        HIPCHK(c, ei);
+        std::memcpy(c->gkey[k], key, sizeof(key));
+    }
+    HIPCHK(c, hipGraphLaunch(c->gexec[k], c->stream));
+    return DGPU_OK;
+}
+
+int32_t dgpu_set_graphs(dgpu_ctx *c, int32_t enable)
+{
+    if (!c) return DGPU_E_ARG;
+    c->use_graphs = enable != 0;
     return DGPU_OK;
 }
 

@@ -53,7 +53,7 @@ EXPORTS = (
     "dgpu_set_field", "dgpu_get_field", "dgpu_rk_chan_1", "dgpu_rem_div_chan", "dgpu_poisson_p_chan",
     "dgpu_apply_bc_vel",
     "dgpu_fft_xz_to_fourier", "dgpu_fft_xz_to_physical", "dgpu_synchronize",
-    "dgpu_set_kernel_timing", "dgpu_kernel_times",
+    "dgpu_set_kernel_timing", "dgpu_kernel_times", "dgpu_set_graphs",
 )
 
 _lib = None
@@ -87,6 +87,8 @@ def load_library(path: str = LIB_PATH):
     for name in ("dgpu_fft_xz_to_fourier", "dgpu_fft_xz_to_physical"):
         getattr(L, name).restype = i32
         getattr(L, name).argtypes = [vp, DP, i32, i32]
+    L.dgpu_set_graphs.restype = i32
+    L.dgpu_set_graphs.argtypes = [vp, i32]
     L.dgpu_set_kernel_timing.restype = i32
     L.dgpu_set_kernel_timing.argtypes = [vp, i32]
     L.dgpu_kernel_times.restype = i32
@@ -219,6 +221,9 @@ class GpuChannel:
 
     def synchronize(self):
         self._chk(self.L.dgpu_synchronize(self.h), "dgpu_synchronize")
+
+    def set_graphs(self, on: bool):
+        self._chk(self.L.dgpu_set_graphs(self.h, 1 if on else 0), "dgpu_set_graphs")
 
     def set_kernel_timing(self, on: bool):
         self._chk(self.L.dgpu_set_kernel_timing(self.h, 1 if on else 0), "dgpu_set_kernel_timing")

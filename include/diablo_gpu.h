@@ -119,6 +119,12 @@ int32_t dgpu_get_field(dgpu_ctx *ctx, int32_t field, double *fortran_array);
 int32_t dgpu_rk_chan_1(dgpu_ctx *ctx, int32_t rk_step, double time, const double *h_bar,
                        const double *beta_bar, const double *zeta_bar);
 
+/* Substep graphs: by default dgpu_rk_chan_1 records its kernels once per RK
+ * stage into a HIP graph and replays it (the stage constants are part of the
+ * key; F_TYPE = 2 and kernel timing run without graphs).  enable = 0 switches
+ * graphs off for this context. */
+int32_t dgpu_set_graphs(dgpu_ctx *ctx, int32_t enable);
+
 /* REM_DIV_CHAN (for/channel.f:1103-1214) on the device-resident CU1..3. */
 int32_t dgpu_rem_div_chan(dgpu_ctx *ctx);
 
