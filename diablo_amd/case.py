@@ -177,6 +177,21 @@ def kh_case(nx=256, ny=193, nz=256, re=1000.0, ri=0.1, dt=2.0e-3, kick=1.0e-3,
         grid_type=2, grid_cs=1.5, ic="kh", kick=kick, seed=seed, n_steps=n_steps)
 
 
+def ensemble_cases(n: int = 64, seed: int = 2024, dt: float = 2.0e-3, kick: float = 1.0e-3):
+    """Kelvin-Helmholtz members on uneven grids: NX = NZ in {64, 96, 128, 192}
+    and NY in {33, 65, 97, 129} (16 shapes, used in turn), Ri uniform in
+    [0.05, 0.25], Re in {500, 1000} and member seed = seed + m."""
+    rng = np.random.default_rng(seed)
+    shapes = [(nx, ny) for nx in (64, 96, 128, 192) for ny in (33, 65, 97, 129)]
+    out = []
+    for m in range(n):
+        nx, ny = shapes[m % len(shapes)]
+        ri = float(rng.uniform(0.05, 0.25))
+        re = float(rng.choice([500.0, 1000.0]))
+        out.append(kh_case(nx, ny, nx, re=re, ri=ri, dt=dt, kick=kick, seed=seed + m, n_steps=500))
+    return out
+
+
 WORKLOADS = {
     # CPU-runnable channel, FP64, 200 steps
     "channel64": lambda: channel_case(64, 65, 64, re=180.0, dt=1.0e-3, kick=0.01, n_steps=200),

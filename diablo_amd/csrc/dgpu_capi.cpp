@@ -81,6 +81,17 @@ struct dgpu_ctx {
     double t_ms[kKinds] = {0, 0, 0, 0, 0};
     long long t_n[kKinds] = {0, 0, 0, 0, 0};
 };
+
+struct dgpu_ensemble {
+    std::vector<dgpu_ctx *> m;
+    hipStream_t main = nullptr;
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join;
+    hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
+    std::vector<double> gkey[3];
+    bool use_graphs = true;
+    std::string err;
+};
 }
 
 namespace {
@@ -933,6 +944,148 @@ int32_t dgpu_poisson_p_chan(dgpu_ctx *c)
                           c->d_pmult, c->d_pinv, c->d_rd, scr, c->cp);
     if (int rc = check_launch(c, "pp_solve")) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return DGPU_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Ensemble: independent runs (their own grids and parameters) advanced
+// together.  One graph per RK stage holds every member's substep as a
+// parallel branch: fork event on the ensemble stream, each member's kernels on
+// its own stream, join events back.
+// ---------------------------------------------------------------------------
+int32_t dgpu_ensemble_destroy(dgpu_ensemble *e)
+{
+    if (!e) return DGPU_OK;
+    if (e->main) hipStreamSynchronize(e->main);
+    for (auto &ge : e->gexec)
+        if (ge) hipGraphExecDestroy(ge);
+    for (auto *c : e->m) dgpu_destroy(c);
+    for (auto ev : e->join)
+        if (ev) hipEventDestroy(ev);
+    if (e->fork) hipEventDestroy(e->fork);
+    if (e->main) hipStreamDestroy(e->main);
+    delete e;
+    return DGPU_OK;
+}
+
+int32_t dgpu_ensemble_create(const dgpu_params *members, int32_t n, dgpu_ensemble **out)
+{
+    if (!members || n < 1 || !out) return DGPU_E_ARG;
+    *out = nullptr;
+    dgpu_ensemble *e = new (std::nothrow) dgpu_ensemble();
+    if (!e) return DGPU_E_NOMEM;
+    for (int i = 0; i < n; ++i) {
+        dgpu_ctx *c = nullptr;
+        const int rc = dgpu_create(&members[i], &c);
+        if (rc) {
+            dgpu_ensemble_destroy(e);
+            return rc;
+        }
+        e->m.push_back(c);
+    }
+    if (hipStreamCreateWithFlags(&e->main, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&e->fork, hipEventDisableTiming) != hipSuccess) {
+        dgpu_ensemble_destroy(e);
+        return DGPU_E_HIP;
+    }
+    e->join.assign(n, nullptr);
+    for (int i = 0; i < n; ++i)
+        if (hipEventCreateWithFlags(&e->join[i], hipEventDisableTiming) != hipSuccess) {
+            dgpu_ensemble_destroy(e);
+            return DGPU_E_HIP;
+        }
+    *out = e;
+    return DGPU_OK;
+}
+
+int32_t dgpu_ensemble_size(const dgpu_ensemble *e) { return e ? (int32_t)e->m.size() : 0; }
+
+dgpu_ctx *dgpu_ensemble_member(dgpu_ensemble *e, int32_t i)
+{
+    if (!e || i < 0 || i >= (int32_t)e->m.size()) return nullptr;
+    return e->m[i];
+}
+
+const char *dgpu_ensemble_last_error(const dgpu_ensemble *e) { return e ? e->err.c_str() : "null ensemble"; }
+
+int32_t dgpu_ensemble_set_graphs(dgpu_ensemble *e, int32_t enable)
+{
+    if (!e) return DGPU_E_ARG;
+    e->use_graphs = enable != 0;
+    return DGPU_OK;
+}
+
+int32_t dgpu_ensemble_rk_chan_1(dgpu_ensemble *e, int32_t rk_step, const double *h_bar, const double *beta_bar,
+                                const double *zeta_bar)
+{
+    if (!e || !h_bar || !beta_bar || !zeta_bar || rk_step < 1 || rk_step > 3) return DGPU_E_ARG;
+    const int k = rk_step - 1;
+    const int n = (int)e->m.size();
+    std::vector<double> key;
+    bool graphable = e->use_graphs;
+    for (int i = 0; i < n; ++i) {
+        dgpu_ctx *c = e->m[i];
+        const double h = h_bar[3 * i + k];
+        const double temp1 = c->p.nu * h / 2.0, temp3 = zeta_bar[3 * i + k] * h, temp4 = h;
+        const double temp5 = beta_bar[3 * i + k] * h;
+        if (int rc = ensure_cn(c, k, temp1, temp4)) {
+            e->err = c->err;
+            return rc;
+        }
+        key.insert(key.end(), {temp1, temp3, temp4, temp5});
+        if (c->p.f_type == 2 || c->timing) graphable = false;
+    }
+    auto enqueue_all = [&](bool joined) -> int {
+        for (int i = 0; i < n; ++i) {
+            dgpu_ctx *c = e->m[i];
+            if (joined && hipStreamWaitEvent(c->stream, e->fork, 0) != hipSuccess) return DGPU_E_HIP;
+            const double *q = &key[4 * i];
+            if (int rc = rk_enqueue(c, rk_step, 0.0, q[0], q[1], q[2], q[3])) {
+                e->err = c->err;
+                return rc;
+            }
+            if (joined) {
+                if (hipEventRecord(e->join[i], c->stream) != hipSuccess) return DGPU_E_HIP;
+                if (hipStreamWaitEvent(e->main, e->join[i], 0) != hipSuccess) return DGPU_E_HIP;
+            }
+        }
+        return DGPU_OK;
+    };
+    if (!graphable) return enqueue_all(false);
+    if (!(e->gexec[k] && e->gkey[k] == key)) {
+        if (e->gexec[k]) {
+            hipGraphExecDestroy(e->gexec[k]);
+            e->gexec[k] = nullptr;
+        }
+        if (hipStreamBeginCapture(e->main, hipStreamCaptureModeThreadLocal) != hipSuccess) return DGPU_E_HIP;
+        hipGraph_t graph = nullptr;
+        int rc = DGPU_OK;
+        if (hipEventRecord(e->fork, e->main) != hipSuccess) rc = DGPU_E_HIP;
+        if (!rc) rc = enqueue_all(true);
+        const hipError_t ec = hipStreamEndCapture(e->main, &graph);
+        if (rc || ec != hipSuccess) {
+            if (graph) hipGraphDestroy(graph);
+            if (!rc) e->err = std::string("graph capture: ") + hipGetErrorString(ec);
+            return rc ? rc : DGPU_E_HIP;
+        }
+        const hipError_t ei = hipGraphInstantiate(&e->gexec[k], graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        if (ei != hipSuccess) {
+            e->err = std::string("graph instantiate: ") + hipGetErrorString(ei);
+            return DGPU_E_HIP;
+        }
+        e->gkey[k] = key;
+    }
+    if (hipGraphLaunch(e->gexec[k], e->main) != hipSuccess) return DGPU_E_HIP;
+    return DGPU_OK;
+}
+
+int32_t dgpu_ensemble_synchronize(dgpu_ensemble *e)
+{
+    if (!e) return DGPU_E_ARG;
+    if (hipStreamSynchronize(e->main) != hipSuccess) return DGPU_E_HIP;
+    for (auto *c : e->m)
+        if (hipStreamSynchronize(c->stream) != hipSuccess) return DGPU_E_HIP;
     return DGPU_OK;
 }
 

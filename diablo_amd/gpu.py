@@ -54,6 +54,9 @@ EXPORTS = (
     "dgpu_apply_bc_vel",
     "dgpu_fft_xz_to_fourier", "dgpu_fft_xz_to_physical", "dgpu_synchronize",
     "dgpu_set_kernel_timing", "dgpu_kernel_times", "dgpu_set_graphs",
+    "dgpu_ensemble_create", "dgpu_ensemble_destroy", "dgpu_ensemble_size", "dgpu_ensemble_member",
+    "dgpu_ensemble_last_error", "dgpu_ensemble_set_graphs", "dgpu_ensemble_rk_chan_1",
+    "dgpu_ensemble_synchronize",
 )
 
 _lib = None
@@ -87,6 +90,22 @@ def load_library(path: str = LIB_PATH):
     for name in ("dgpu_fft_xz_to_fourier", "dgpu_fft_xz_to_physical"):
         getattr(L, name).restype = i32
         getattr(L, name).argtypes = [vp, DP, i32, i32]
+    L.dgpu_ensemble_create.restype = i32
+    L.dgpu_ensemble_create.argtypes = [C.POINTER(Params), i32, C.POINTER(vp)]
+    L.dgpu_ensemble_destroy.restype = i32
+    L.dgpu_ensemble_destroy.argtypes = [vp]
+    L.dgpu_ensemble_size.restype = i32
+    L.dgpu_ensemble_size.argtypes = [vp]
+    L.dgpu_ensemble_member.restype = vp
+    L.dgpu_ensemble_member.argtypes = [vp, i32]
+    L.dgpu_ensemble_last_error.restype = C.c_char_p
+    L.dgpu_ensemble_last_error.argtypes = [vp]
+    L.dgpu_ensemble_set_graphs.restype = i32
+    L.dgpu_ensemble_set_graphs.argtypes = [vp, i32]
+    L.dgpu_ensemble_rk_chan_1.restype = i32
+    L.dgpu_ensemble_rk_chan_1.argtypes = [vp, i32, DP, DP, DP]
+    L.dgpu_ensemble_synchronize.restype = i32
+    L.dgpu_ensemble_synchronize.argtypes = [vp]
     L.dgpu_set_graphs.restype = i32
     L.dgpu_set_graphs.argtypes = [vp, i32]
     L.dgpu_set_kernel_timing.restype = i32
@@ -147,10 +166,24 @@ class GpuChannel:
         self.time = 0.0
         self.h_bar, self.beta_bar, self.zeta_bar = (np.ascontiguousarray(a) for a in rkw3_table(cfg.delta_t))
 
+    @classmethod
+    def attach(cls, cfg: ChannelConfig, handle: int, precision: str, gy_1: np.ndarray) -> "GpuChannel":
+        """Wraps a context owned by someone else (an ensemble member)."""
+        self = cls.__new__(cls)
+        self.cfg = cfg
+        self.precision = precision
+        self.L = load_library()
+        self._gy = gy_1
+        self.h = C.c_void_p(handle)
+        self._owned = False
+        self.time = 0.0
+        self.h_bar, self.beta_bar, self.zeta_bar = (np.ascontiguousarray(a) for a in rkw3_table(cfg.delta_t))
+        return self
+
     def close(self):
-        if getattr(self, "h", None) is not None and self.h.value:
+        if getattr(self, "h", None) is not None and self.h.value and getattr(self, "_owned", True):
             self.L.dgpu_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:
@@ -252,3 +285,69 @@ class GpuChannel:
         self.rem_div_chan()
         self.synchronize()
         self.time = 0.0
+
+
+class GpuEnsemble:
+    """Independent channel runs (their own sizes, parameters and state)
+    advanced together through dgpu_ensemble_rk_chan_1."""
+
+    def __init__(self, cfgs, precision: str = "mixed"):
+        self.cfgs = list(cfgs)
+        self.L = load_library()
+        n = len(self.cfgs)
+        self._gys = [np.ascontiguousarray(grid_gy(c), dtype=np.float64) for c in self.cfgs]
+        arr = (Params * n)()
+        for i, c in enumerate(self.cfgs):
+            c.validate()
+            arr[i] = params_of(c, self._gys[i], precision)
+        self._params = arr
+        h = C.c_void_p()
+        rc = self.L.dgpu_ensemble_create(arr, n, C.byref(h))
+        if rc != 0 or not h.value:
+            raise DgpuError(f"dgpu_ensemble_create failed ({rc}) for {n} members")
+        self.h = h
+        self.members = [GpuChannel.attach(c, self.L.dgpu_ensemble_member(h, i), precision, self._gys[i])
+                        for i, c in enumerate(self.cfgs)]
+        tabs = [rkw3_table(c.delta_t) for c in self.cfgs]
+        self.h_bar = np.ascontiguousarray(np.concatenate([t[0] for t in tabs]))
+        self.beta_bar = np.ascontiguousarray(np.concatenate([t[1] for t in tabs]))
+        self.zeta_bar = np.ascontiguousarray(np.concatenate([t[2] for t in tabs]))
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            msg = self.L.dgpu_ensemble_last_error(self.h)
+            raise DgpuError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    def set_graphs(self, on: bool):
+        self._chk(self.L.dgpu_ensemble_set_graphs(self.h, 1 if on else 0), "dgpu_ensemble_set_graphs")
+
+    def rk_chan_1(self, rk_step: int):
+        self._chk(self.L.dgpu_ensemble_rk_chan_1(self.h, rk_step, _dp(self.h_bar), _dp(self.beta_bar),
+                                                 _dp(self.zeta_bar)), "dgpu_ensemble_rk_chan_1")
+
+    def step(self, n: int = 1):
+        for _ in range(n):
+            for k in (1, 2, 3):
+                self.rk_chan_1(k)
+            for m in self.members:
+                m.time += m.cfg.delta_t
+
+    def synchronize(self):
+        self._chk(self.L.dgpu_ensemble_synchronize(self.h), "dgpu_ensemble_synchronize")
+
+    @property
+    def points(self) -> int:
+        return sum(c.points for c in self.cfgs)
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            for m in self.members:
+                m.h = None
+            self.L.dgpu_ensemble_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

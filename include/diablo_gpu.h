@@ -163,6 +163,26 @@ int32_t dgpu_synchronize(dgpu_ctx *ctx);
 int32_t dgpu_set_kernel_timing(dgpu_ctx *ctx, int32_t enable);
 int32_t dgpu_kernel_times(dgpu_ctx *ctx, double *ms, int64_t *count, int32_t n);
 
+/*
+ * Ensembles (beyond the reference surface: its sizes are compile-time
+ * PARAMETERs, for/header:8-13).  n independent runs, each with its own grid
+ * sizes, parameters and state, advanced together: dgpu_ensemble_rk_chan_1
+ * runs RK_CHAN_1 on every member, as one HIP graph per RK stage whose
+ * branches are the members.  h_bar, beta_bar and zeta_bar hold 3 values per
+ * member (member-major).  Members are ordinary contexts for every other call
+ * (dgpu_ensemble_member): set/get fields, initial transforms, REM_DIV.
+ */
+typedef struct dgpu_ensemble dgpu_ensemble;
+int32_t dgpu_ensemble_create(const dgpu_params *members, int32_t n, dgpu_ensemble **out);
+int32_t dgpu_ensemble_destroy(dgpu_ensemble *ens);
+int32_t dgpu_ensemble_size(const dgpu_ensemble *ens);
+dgpu_ctx *dgpu_ensemble_member(dgpu_ensemble *ens, int32_t i);
+const char *dgpu_ensemble_last_error(const dgpu_ensemble *ens);
+int32_t dgpu_ensemble_set_graphs(dgpu_ensemble *ens, int32_t enable);
+int32_t dgpu_ensemble_rk_chan_1(dgpu_ensemble *ens, int32_t rk_step, const double *h_bar,
+                                const double *beta_bar, const double *zeta_bar);
+int32_t dgpu_ensemble_synchronize(dgpu_ensemble *ens);
+
 #ifdef __cplusplus
 }
 #endif
