@@ -99,6 +99,28 @@ def accuracy(cfg, phys, steps: int) -> dict:
     return r
 
 
+def ensemble_run(n: int, steps: int, warmup: int, precision: str = "mixed") -> dict:
+    """Aggregate throughput of n independent KH runs on uneven grids advanced
+    together (one graph per RK stage, one branch per member)."""
+    from diablo_amd import case, gpu
+    cfgs = case.ensemble_cases(n)
+    ens = gpu.GpuEnsemble(cfgs, precision=precision)
+    for cfg, m in zip(cfgs, ens.members):
+        m.initialize(*case.initial_physical(cfg))
+    ens.step(warmup)
+    ens.synchronize()
+    t0 = time.perf_counter()
+    ens.step(steps)
+    ens.synchronize()
+    el = time.perf_counter() - t0
+    pts = ens.points
+    ens.close()
+    return {"value": round(pts * 3 * steps / el / 1e6, 2), "unit": "Mpts*substeps/s", "members": n,
+            "steps": steps, "precision": precision, "ms_per_step": round(el / steps * 1e3, 3),
+            "points_total": pts,
+            "grids": "NX=NZ in {64,96,128,192} x NY in {33,65,97,129}, Ri in [0.05,0.25], Re in {500,1000}"}
+
+
 def cpu_baseline(cfg, phys, threads: int) -> dict:
     from oracle import oracle
     o = oracle.Oracle(cfg, threads=threads)
@@ -123,6 +145,8 @@ def main():
     ap.add_argument("--no-kernel-timing", action="store_true")
     ap.add_argument("--mixed", choices=("on", "off"), default="on")
     ap.add_argument("--accuracy-steps", type=int, default=100)
+    ap.add_argument("--ensemble", type=int, default=64, help="members of the batched run (0: skip)")
+    ap.add_argument("--ensemble-steps", type=int, default=10)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: this build runs the channel case on a single GPU "
@@ -184,6 +208,13 @@ def main():
                  "rel_l2_vs_fp64": acc["rel_l2"], "ke_drift_vs_fp64": acc["ke_drift"],
                  "max_div": acc["max_div"], "accuracy_steps": acc["steps"]}
 
+    batched = None
+    if args.ensemble > 0:
+        log(f"[bench] batched: {args.ensemble} KH members, mixed, {args.ensemble_steps} steps")
+        batched = ensemble_run(args.ensemble, args.ensemble_steps, 1)
+        log(f"[bench] batched aggregate {batched['value']:.1f} Mpts*substeps/s "
+            f"({batched['value'] / value:.2f}x FP64 384)")
+
     cpu = None
     if args.cpu_baseline == "on":
         threads = min(16, os.cpu_count() or 1)
@@ -211,6 +242,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "mixed": mixed,
+        "batched": batched,
     }
     print(json.dumps(out), flush=True)
 
