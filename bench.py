@@ -135,6 +135,19 @@ def cpu_baseline(cfg, phys, threads: int) -> dict:
                       f"reference) with {threads} OpenMP threads, {dt:.1f} s"}
 
 
+def copy_calibration(mib: int) -> None:
+    """A plain device-to-device copy of known size: what FETCH_SIZE / WRITE_SIZE
+    read for it scales the same counters read for the solver kernels."""
+    import torch
+    n = mib * (1 << 20) // 8
+    src = torch.arange(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "repeats": 5}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -147,11 +160,17 @@ def main():
     ap.add_argument("--accuracy-steps", type=int, default=100)
     ap.add_argument("--ensemble", type=int, default=64, help="members of the batched run (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=10)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB five times (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: this build runs the channel case on a single GPU "
               "(no sharding, no collectives); --gpus must be 1", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_calibration > 0:
+        copy_calibration(args.copy_calibration)
+        return
 
     from diablo_amd import case, gpu
 
