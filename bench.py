@@ -161,7 +161,7 @@ def main():
     ap.add_argument("--ensemble", type=int, default=64, help="members of the batched run (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=10)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
-                    help="only run a plain device copy of MIB MiB five times (HBM counter calibration)")
+                    help="first run a plain device copy of MIB MiB five times (reference point for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: this build runs the channel case on a single GPU "
@@ -170,7 +170,6 @@ def main():
 
     if args.copy_calibration > 0:
         copy_calibration(args.copy_calibration)
-        return
 
     from diablo_amd import case, gpu
 
