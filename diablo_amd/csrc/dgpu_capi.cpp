@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -13,6 +14,7 @@
 
 #include "../../include/diablo_gpu.h"
 #include "dgpu_kernels.hpp"
+#include "dgpu_plan.hpp"
 
 using dgpu::cx;
 
@@ -37,10 +39,15 @@ struct dgpu_ctx {
     double *d_dy = nullptr, *d_dyf = nullptr, *d_kx = nullptr, *d_kz = nullptr, *d_kx2 = nullptr, *d_kz2 = nullptr;
     dgpu::Fft<double> fx, fz;
     cx<double> *d_tw = nullptr;
+    // in-place plans of the substep's transforms and their twiddle tables
+    // (x table, then z table; FP32 copies for the mixed path)
+    dgpu::IpLen ipx, ipz;
+    cx<double> *d_iptw = nullptr;
+    cx<float> *d_iptwf = nullptr;
+    int ld = 0;      // row length of the intermediates (>= mk, multiple of 8)
+    int xchunk = 16; // planes per block of the fused x kernel
     // mixed path: FP32 transforms, intermediates and nonlinear terms
     int mixed = 0;
-    dgpu::Fft<float> fxf, fzf;
-    cx<float> *d_twf = nullptr;
     cx<float> *d_ztf = nullptr, *d_wtf = nullptr, *d_nlf = nullptr;
     cx<float> *nlf[3 + DGPU_MAX_TH] = {};
     cx<float> **d_nlptrf = nullptr;
@@ -57,7 +64,7 @@ struct dgpu_ctx {
     cx<double> **d_nlptr = nullptr;
     // transform intermediates
     cx<double> *d_zt = nullptr, *d_wt = nullptr;
-    long zfs = 0;  // complex elements per intermediate field: (NY+2)*NZ*mk
+    long zfs = 0;  // complex elements per intermediate field: (NY+2)*NZ*ld
     double *d_fort = nullptr;  // Fortran-layout staging for host entry points
     double *d_fa = nullptr, *d_fb = nullptr;  // POISSON_P_CHAN work planes
     double *d_gyf = nullptr;
@@ -455,8 +462,21 @@ int32_t dgpu_create(const dgpu_params *p, dgpu_ctx **out)
         delete c;
         return DGPU_E_ARG;
     }
-    c->fxf.len = c->fx.len;
-    c->fzf.len = c->fz.len;
+    {
+        // in-place plans; DGPU_FFT_EX / DGPU_FFT_EZ ask for a number of values
+        // per thread, DGPU_XCHUNK for the planes per block of the x kernel
+        const char *ex = std::getenv("DGPU_FFT_EX"), *ez = std::getenv("DGPU_FFT_EZ");
+        const char *xc = std::getenv("DGPU_XCHUNK");
+        // FP64 kernels spill above 12 values per thread
+        const int emax = c->mixed ? 24 : 12;
+        if (!dgpu::ip_plan_x(g.nx, ex ? std::atoi(ex) : 0, c->mixed ? 16 : 12, 64, c->ipx) ||
+            !dgpu::ip_plan_z(g.nz, ez ? std::atoi(ez) : 0, emax, c->ipz)) {
+            delete c;
+            return DGPU_E_ARG;
+        }
+        if (xc && std::atoi(xc) > 0) c->xchunk = std::atoi(xc);
+        c->ld = (g.mk + 7) / 8 * 8;
+    }
     int rc = 0;
     auto bail = [&](int code) { rc = code; dgpu_destroy(c); return code; };
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return bail(DGPU_E_HIP);
@@ -491,15 +511,19 @@ int32_t dgpu_create(const dgpu_params *p, dgpu_ctx **out)
             return bail(DGPU_E_HIP);
         c->fx.tw = c->d_tw;
         c->fz.tw = c->d_tw + g.nx;
+        std::vector<cx<double>> ipx, ipz;
+        dgpu::ip_twiddles(c->ipx, ipx);
+        dgpu::ip_twiddles(c->ipz, ipz);
+        ipx.insert(ipx.end(), ipz.begin(), ipz.end());
+        if (hipMalloc(&c->d_iptw, sizeof(cx<double>) * ipx.size()) != hipSuccess) return bail(DGPU_E_NOMEM);
+        if (hipMemcpy(c->d_iptw, ipx.data(), sizeof(cx<double>) * ipx.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return bail(DGPU_E_HIP);
         if (c->mixed) {
-            std::vector<cx<float>> tf(g.nx + g.nz);
-            for (int j = 0; j < g.nx; ++j) { tf[j].re = (float)twx[j].re; tf[j].im = (float)twx[j].im; }
-            for (int j = 0; j < g.nz; ++j) { tf[g.nx + j].re = (float)twz[j].re; tf[g.nx + j].im = (float)twz[j].im; }
-            if (hipMalloc(&c->d_twf, sizeof(cx<float>) * tf.size()) != hipSuccess) return bail(DGPU_E_NOMEM);
-            if (hipMemcpy(c->d_twf, tf.data(), sizeof(cx<float>) * tf.size(), hipMemcpyHostToDevice) != hipSuccess)
+            std::vector<cx<float>> tf(ipx.size());
+            for (size_t j = 0; j < ipx.size(); ++j) { tf[j].re = (float)ipx[j].re; tf[j].im = (float)ipx[j].im; }
+            if (hipMalloc(&c->d_iptwf, sizeof(cx<float>) * tf.size()) != hipSuccess) return bail(DGPU_E_NOMEM);
+            if (hipMemcpy(c->d_iptwf, tf.data(), sizeof(cx<float>) * tf.size(), hipMemcpyHostToDevice) != hipSuccess)
                 return bail(DGPU_E_HIP);
-            c->fxf.tw = c->d_twf;
-            c->fzf.tw = c->d_twf + g.nx;
         }
     }
     // state: cu1 cu2 cu3 cth[n] | cp cf1 cf2 cf3 cfth[n] | nl[3+n]
@@ -539,7 +563,7 @@ int32_t dgpu_create(const dgpu_params *p, dgpu_ctx **out)
     if (hipMemcpy(c->d_nlptr, c->nl, sizeof(cx<double> *) * (3 + DGPU_MAX_TH), hipMemcpyHostToDevice) != hipSuccess)
         return bail(DGPU_E_HIP);
     // intermediates
-    c->zfs = (long)(NY + 2) * g.nz * g.mk;
+    c->zfs = (long)(NY + 2) * g.nz * c->ld;
     if (hipMalloc(&c->d_zt, sizeof(cx<double>) * c->zfs * nin) != hipSuccess) return bail(DGPU_E_NOMEM);
     if (hipMalloc(&c->d_wt, sizeof(cx<double>) * c->zfs * 2 * nin) != hipSuccess) return bail(DGPU_E_NOMEM);
     if (c->mixed) {
@@ -582,7 +606,7 @@ int32_t dgpu_destroy(dgpu_ctx *c)
         if (ge) hipGraphExecDestroy(ge);
     for (auto e : c->event_pool) hipEventDestroy(e);
     void *bufs[] = {c->d_vec, c->d_tw, c->d_state, c->d_pmult, c->d_pinv, c->d_rd, c->d_nlptr, c->d_zt, c->d_wt,
-                    c->d_fa, c->d_fb, c->d_gyf, c->d_twf, c->d_ztf, c->d_wtf, c->d_nlf, c->d_nlptrf,
+                    c->d_fa, c->d_fb, c->d_gyf, c->d_iptw, c->d_iptwf, c->d_ztf, c->d_wtf, c->d_nlf, c->d_nlptrf,
                     c->d_fort, c->d_cn, c->d_codes, c->d_c1};
     for (void *b : bufs)
         if (b) hipFree(b);
@@ -677,42 +701,45 @@ static int rk_enqueue(dgpu_ctx *c, int rk_step, double time, double temp1, doubl
     const int nin = 3 + g.nth;
     int jlo = 2;
     for (int n = 0; n < g.nth; ++n) jlo = std::min(jlo, c->jstart_th[n]);
+    const long zps = (long)g.nz * c->ld;
     if (c->mixed) {
+        const cx<float> *twx = c->d_iptwf, *twz = c->d_iptwf + c->ipx.twn;
         {
             TimerScope t(c, 0);
-            dgpu::launch_zinv<float, double>(c->stream, g, c->fzf, c->cu[0], c->fs, g.ps, g.mk, g.mk, c->d_ztf,
-                                             c->zfs, (long)g.nz * g.mk, g.mk, 0, g.ny + 2, nin);
+            dgpu::launch_zinv_ip<float, double>(c->stream, g, c->ipz, twz, c->cu[0], c->fs, g.ps, c->d_ztf, c->zfs,
+                                                zps, c->ld, 0, g.ny + 2, nin);
         }
         if (int rc = check_launch(c, "zinv")) return rc;
         {
             TimerScope t(c, 1);
-            dgpu::launch_xfused<float>(c->stream, g, c->fxf, c->d_ztf, c->zfs, c->d_wtf, c->zfs, c->d_dy, c->d_dyf,
-                                       c->d_kx, jlo, g.ny);
+            dgpu::launch_xfused_ip<float>(c->stream, g, c->ipx, twx, c->ld, c->d_ztf, c->zfs, c->d_wtf, c->zfs,
+                                          c->d_dy, c->d_dyf, c->d_kx, jlo, g.ny, c->xchunk);
         }
         if (int rc = check_launch(c, "xfused")) return rc;
         {
             TimerScope t(c, 2);
-            dgpu::launch_zfwd_nl<float, float>(c->stream, g, c->fzf, c->d_wtf, c->zfs, c->d_nlptrf, c->d_kz, jlo,
-                                               g.ny, nin);
+            dgpu::launch_zfwd_nl_ip<float, float>(c->stream, g, c->ipz, twz, c->d_wtf, c->zfs, c->ld, c->d_nlptrf,
+                                                  c->d_kz, jlo, g.ny, nin);
         }
         if (int rc = check_launch(c, "zfwd_nl")) return rc;
     } else {
+        const cx<double> *twx = c->d_iptw, *twz = c->d_iptw + c->ipx.twn;
         {
             TimerScope t(c, 0);
-            dgpu::launch_zinv<double, double>(c->stream, g, c->fz, c->cu[0], c->fs, g.ps, g.mk, g.mk, c->d_zt,
-                                              c->zfs, (long)g.nz * g.mk, g.mk, 0, g.ny + 2, nin);
+            dgpu::launch_zinv_ip<double, double>(c->stream, g, c->ipz, twz, c->cu[0], c->fs, g.ps, c->d_zt, c->zfs,
+                                                 zps, c->ld, 0, g.ny + 2, nin);
         }
         if (int rc = check_launch(c, "zinv")) return rc;
         {
             TimerScope t(c, 1);
-            dgpu::launch_xfused<double>(c->stream, g, c->fx, c->d_zt, c->zfs, c->d_wt, c->zfs, c->d_dy, c->d_dyf,
-                                        c->d_kx, jlo, g.ny);
+            dgpu::launch_xfused_ip<double>(c->stream, g, c->ipx, twx, c->ld, c->d_zt, c->zfs, c->d_wt, c->zfs,
+                                           c->d_dy, c->d_dyf, c->d_kx, jlo, g.ny, c->xchunk);
         }
         if (int rc = check_launch(c, "xfused")) return rc;
         {
             TimerScope t(c, 2);
-            dgpu::launch_zfwd_nl<double, double>(c->stream, g, c->fz, c->d_wt, c->zfs, c->d_nlptr, c->d_kz, jlo,
-                                                 g.ny, nin);
+            dgpu::launch_zfwd_nl_ip<double, double>(c->stream, g, c->ipz, twz, c->d_wt, c->zfs, c->ld, c->d_nlptr,
+                                                    c->d_kz, jlo, g.ny, nin);
         }
         if (int rc = check_launch(c, "zfwd_nl")) return rc;
     }

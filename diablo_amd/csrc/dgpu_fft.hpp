@@ -1,4 +1,22 @@
-// dgpu_fft.hpp -- LDS-staged Stockham FFTs for gfx950 (device side).
+// dgpu_fft.hpp -- LDS-staged FFTs for gfx950 (device side).
+//
+// Two families share the butterflies below.
+//
+// In-place mixed-radix transforms (IpLen, ip_*) run the time-stepper's hot
+// kernels.  A length n = R_0 R_1 ... R_{S-1} transform has S stages; stage i
+// works on blocks of m_i = R_i R_{i+1} ... elements (m_0 = n) and its groups
+// (block b, offset k < L_i = m_i / R_i) touch the R_i positions
+// b m_i + k + q L_i.  Decimation in frequency (DIF: butterfly, then twiddle
+// w_{m_i}^{kq}) takes natural-order input to output in mixed-radix
+// digit-reversed positions; decimation in time (DIT: twiddle, then butterfly,
+// stages in reverse order) is its transpose and takes that permuted order back
+// to natural order.  A thread reads and writes the same positions, so a stage
+// needs one barrier, not two, and the physical-space side of a transform pair
+// stays in permuted order (the products there are pointwise).
+//
+// Stockham transforms (FftLen, fft_lds) serve the standalone
+// FFT_XZ_TO_FOURIER / FFT_XZ_TO_PHYSICAL entry points, whose physical-space
+// side is in natural order.
 //
 // A batch of transforms of one length n lives in LDS.  T = n / E threads work
 // on each transform; every thread holds E complex values in registers during
@@ -22,24 +40,26 @@ struct cx {
     T re, im;
 };
 
+#define DGPU_HD __host__ __device__ __forceinline__
+
 template <typename T>
-__device__ __forceinline__ cx<T> make_cx(T r, T i) { cx<T> c; c.re = r; c.im = i; return c; }
+DGPU_HD cx<T> make_cx(T r, T i) { cx<T> c; c.re = r; c.im = i; return c; }
 template <typename T>
-__device__ __forceinline__ cx<T> operator+(cx<T> a, cx<T> b) { return make_cx(a.re + b.re, a.im + b.im); }
+DGPU_HD cx<T> operator+(cx<T> a, cx<T> b) { return make_cx(a.re + b.re, a.im + b.im); }
 template <typename T>
-__device__ __forceinline__ cx<T> operator-(cx<T> a, cx<T> b) { return make_cx(a.re - b.re, a.im - b.im); }
+DGPU_HD cx<T> operator-(cx<T> a, cx<T> b) { return make_cx(a.re - b.re, a.im - b.im); }
 template <typename T>
-__device__ __forceinline__ cx<T> cmul(cx<T> a, cx<T> b)
+DGPU_HD cx<T> cmul(cx<T> a, cx<T> b)
 {
     return make_cx(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
 }
 template <typename T>
-__device__ __forceinline__ cx<T> scale(cx<T> a, T s) { return make_cx(a.re * s, a.im * s); }
+DGPU_HD cx<T> scale(cx<T> a, T s) { return make_cx(a.re * s, a.im * s); }
 template <typename T>
-__device__ __forceinline__ cx<T> conj_(cx<T> a) { return make_cx(a.re, -a.im); }
+DGPU_HD cx<T> conj_(cx<T> a) { return make_cx(a.re, -a.im); }
 // multiply by i
 template <typename T>
-__device__ __forceinline__ cx<T> mul_i(cx<T> a) { return make_cx(-a.im, a.re); }
+DGPU_HD cx<T> mul_i(cx<T> a) { return make_cx(-a.im, a.re); }
 
 // Maximum number of radix stages of a supported length.
 constexpr int kMaxStages = 16;
@@ -57,13 +77,13 @@ struct FftLen {
 // Quotient by a stage span with a precomputed reciprocal; exact for
 // t, ns < 2^16 (transform lengths here are far below that).  magic == 0
 // stands for a span of 1, whose reciprocal 2^32 does not fit.
-__device__ __forceinline__ int div_magic(int t, unsigned magic)
+DGPU_HD int div_magic(int t, unsigned magic)
 {
-    return magic ? (int)__umulhi((unsigned)t, magic) : t;
+    return magic ? (int)(((unsigned long long)(unsigned)t * magic) >> 32) : t;
 }
 
 template <typename T, int R, int SIGN>
-__device__ __forceinline__ void butterfly(cx<T> *v)
+DGPU_HD void butterfly(cx<T> *v)
 {
     if constexpr (R == 2) {
         cx<T> a = v[0], b = v[1];
@@ -106,7 +126,153 @@ __device__ __forceinline__ void butterfly(cx<T> *v)
         v[4] = t1 - u1;
         v[2] = t2 + u2;
         v[3] = t2 - u2;
+    } else if constexpr (R == 6) {
+        // 6 = 2 x 3: DFT3 of the even and odd samples, then one radix-2 step
+        // with the twiddles w6^k, k = 0..2 (w6 = exp(SIGN 2 pi i / 6)).
+        cx<T> e[3] = {v[0], v[2], v[4]}, o[3] = {v[1], v[3], v[5]};
+        butterfly<T, 3, SIGN>(e);
+        butterfly<T, 3, SIGN>(o);
+        const T h = T(0.5);
+        const T s = T(SIGN) * T(0.86602540378443864676372317075294);
+        const cx<T> o1 = make_cx(h * o[1].re - s * o[1].im, h * o[1].im + s * o[1].re);     // o1 * w6
+        const cx<T> o2 = make_cx(-h * o[2].re - s * o[2].im, -h * o[2].im + s * o[2].re);   // o2 * w6^2
+        v[0] = e[0] + o[0];
+        v[3] = e[0] - o[0];
+        v[1] = e[1] + o1;
+        v[4] = e[1] - o1;
+        v[2] = e[2] + o2;
+        v[5] = e[2] - o2;
+    } else if constexpr (R == 8) {
+        // 8 = 2 x 4: DFT4 of the even and odd samples, then one radix-2 step
+        // with w8^k, k = 0..3 (w8 = exp(SIGN 2 pi i / 8)).
+        cx<T> e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+        butterfly<T, 4, SIGN>(e);
+        butterfly<T, 4, SIGN>(o);
+        const T r = T(0.70710678118654752440084436210485);
+        const T sg = T(SIGN);
+        // o1 * (r + i sg r), o2 * (i sg), o3 * (-r + i sg r)
+        const cx<T> o1 = make_cx(r * (o[1].re - sg * o[1].im), r * (o[1].im + sg * o[1].re));
+        const cx<T> o2 = make_cx(-sg * o[2].im, sg * o[2].re);
+        const cx<T> o3 = make_cx(-r * (o[3].re + sg * o[3].im), r * (sg * o[3].re - o[3].im));
+        v[0] = e[0] + o[0];
+        v[4] = e[0] - o[0];
+        v[1] = e[1] + o1;
+        v[5] = e[1] - o1;
+        v[2] = e[2] + o2;
+        v[6] = e[2] - o2;
+        v[3] = e[3] + o3;
+        v[7] = e[3] - o3;
     }
+}
+
+// ---------------------------------------------------------------------------
+// In-place mixed-radix transforms
+// ---------------------------------------------------------------------------
+
+struct IpLen {
+    int n;
+    int nstages;
+    int radix[kMaxStages];      // R_i, stage 0 works on the whole transform
+    int span[kMaxStages];       // L_i = m_i / R_i
+    unsigned magic[kMaxStages]; // t / L_i = div_magic(t, magic[i])
+    int twoff[kMaxStages];      // stage i's twiddles: tw[twoff[i] + k*(R_i-1) + q-1] = w_{m_i}^{kq}
+    int twn;                    // length of the whole twiddle table
+    int e;                      // values per thread (multiple of every R_i)
+    int t;                      // threads per transform = n / e
+};
+
+// Twiddle + butterfly of one group (values v[0..R-1] at positions
+// base + q L_i); tw points at the stage's table, k is the group offset.
+template <typename T, int R, int SIGN, bool DIF>
+DGPU_HD void ip_group(cx<T> *v, const cx<T> *__restrict__ tw, int k)
+{
+    cx<T> w[R];
+#pragma unroll
+    for (int q = 1; q < R; ++q) {
+        w[q] = tw[k * (R - 1) + q - 1];
+        if (SIGN > 0) w[q].im = -w[q].im;
+    }
+    if (!DIF) {
+#pragma unroll
+        for (int q = 1; q < R; ++q) v[q] = cmul(v[q], w[q]);
+    }
+    butterfly<T, R, SIGN>(v);
+    if (DIF) {
+#pragma unroll
+        for (int q = 1; q < R; ++q) v[q] = cmul(v[q], w[q]);
+    }
+}
+
+// Groups of stage i handled by thread th: t = th + nthr*u, u < E/R.  fn(u, base, k)
+template <int E, int R, class F>
+DGPU_HD void ip_groups(const IpLen &f, int i, int th, F &&fn)
+{
+    constexpr int NB = E / R;
+    const int L = f.span[i];
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+        const int t = th + f.t * u;
+        const int b = div_magic(t, f.magic[i]);
+        const int k = t - b * L;
+        fn(u, b * R * L + k, k);
+    }
+}
+
+// One stage with arbitrary sources and sinks: ld(idx, pos) gives the value at
+// position pos (idx = u*R + q is its slot in this thread's E values),
+// st(idx, pos, v) takes the result.  All loads come before any store.
+template <typename T, int E, int R, int SIGN, bool DIF, class Ld, class St>
+DGPU_HD void ip_stage(const IpLen &f, int i, const cx<T> *__restrict__ tw, int th, Ld &&ld, St &&st)
+{
+    constexpr int NB = E / R;
+    const int L = f.span[i];
+    cx<T> v[E];
+    int base[NB], kk[NB];
+    ip_groups<E, R>(f, i, th, [&](int u, int b0, int k) {
+        base[u] = b0;
+        kk[u] = k;
+#pragma unroll
+        for (int q = 0; q < R; ++q) v[u * R + q] = ld(u * R + q, b0 + q * L);
+    });
+#pragma unroll
+    for (int u = 0; u < NB; ++u) ip_group<T, R, SIGN, DIF>(v + u * R, tw + f.twoff[i], kk[u]);
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+#pragma unroll
+        for (int q = 0; q < R; ++q) st(u * R + q, base[u] + q * L, v[u * R + q]);
+    }
+}
+
+// Calls fn(std::integral_constant<int, R>) for the runtime radix R (those
+// that divide E; the plan only uses such radices).
+template <int R>
+struct RadixC {
+    static constexpr int value = R;
+};
+template <int E, class F>
+DGPU_HD void with_radix(int R, F &&fn)
+{
+    switch (R) {
+    case 2: if constexpr (E % 2 == 0) fn(RadixC<2>{}); break;
+    case 3: if constexpr (E % 3 == 0) fn(RadixC<3>{}); break;
+    case 4: if constexpr (E % 4 == 0) fn(RadixC<4>{}); break;
+    case 5: if constexpr (E % 5 == 0) fn(RadixC<5>{}); break;
+    case 6: if constexpr (E % 6 == 0) fn(RadixC<6>{}); break;
+    case 8: if constexpr (E % 8 == 0) fn(RadixC<8>{}); break;
+    default: break;
+    }
+}
+
+// In-place stage i on x[pos * stride] (LDS).
+template <typename T, int E, int SIGN, bool DIF>
+DGPU_HD void ip_stage_mem(cx<T> *x, int stride, const IpLen &f, int i, const cx<T> *__restrict__ tw, int th)
+{
+    with_radix<E>(f.radix[i], [&](auto rc) {
+        constexpr int R = decltype(rc)::value;
+        ip_stage<T, E, R, SIGN, DIF>(
+            f, i, tw, th, [&](int, int pos) { return x[pos * stride]; },
+            [&](int, int pos, cx<T> v) { x[pos * stride] = v; });
+    });
 }
 
 // One Stockham stage of radix R for the transform whose element e sits at

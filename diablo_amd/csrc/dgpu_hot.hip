@@ -1,0 +1,508 @@
+// dgpu_hot.hip -- the transform kernels of one RK substep (RK_CHAN_1,
+// for/channel.f:62-1073), built on the in-place transforms of dgpu_fft.hpp.
+//
+//   zinv     CU1,CU2,CU3,CTH (kept modes) --z backward, DIF--> Zt[f][j][zp][kx]
+//   xfused   per z row zp, sweeping j with a window of three planes: x
+//            backward (DIF) of the fields packed in pairs, the eleven products
+//            of for/channel.f:410-571 and 670-720, x forward (DIT) of the
+//            products packed in pairs, written as G = -(i kx X(a) + X(b)) and
+//            H = -X(c) per momentum/scalar equation into Wt[o][j][zp][kx]
+//   zfwd_nl  NL_f = [Z(G_f) + i kz Z(H_f)] / (NX NZ) at the kept slots (DIT)
+//
+// zp and the x index inside xfused are positions of the in-place transforms
+// (digit-reversed), not z and x: every product is pointwise, and the forward
+// transforms take the same permuted order back, so the order never matters.
+//
+// Intermediate rows (Zt, Wt) are ld >= NKX+1 complex values long, ld a
+// multiple of 8, so a chunk of 8 columns is one 128-byte line.
+#include "dgpu_kernels.hpp"
+
+namespace dgpu {
+
+namespace {
+
+template <typename TO, typename FROM>
+__device__ __forceinline__ cx<TO> cvt_(cx<FROM> a) { return make_cx<TO>(TO(a.re), TO(a.im)); }
+
+// Kept slot of z frequency index f (0..NZ-1), or -1 in the dealiasing gap
+// (slot K <= NKZ holds kz = K, slot K > NKZ holds kz = K - (TNKZ+1),
+// for/mpi.f:1602-1606).
+__device__ __forceinline__ int slot_of_freq(int f, int nz, int nkz, int tnkz)
+{
+    if (f <= nkz) return f;
+    if (f >= nz - nkz) return f - nz + tnkz + 1;
+    return -1;
+}
+
+template <int NTH>
+struct XCount {
+    static constexpr int nin = 3 + NTH;         // u1 u2 u3 th...
+    static constexpr int npi = (nin + 1) / 2;   // backward pairs
+    static constexpr int npo = (8 + 3 * NTH + 1) / 2;  // forward pairs of the products
+};
+
+}  // namespace
+
+// -------------------------------------------------------------------------
+// zinv: W columns per block, T = fz.t threads per column.  Stage 0 reads the
+// kept slots straight from the state, the last stage writes Zt rows.
+// grid (column chunks, planes, fields), block W*T, LDS W*NZ values.
+// -------------------------------------------------------------------------
+template <typename T, int E, typename TIN>
+__global__ void __launch_bounds__(256) k_zinv_ip(IpLen fz, const cx<T> *__restrict__ tw,
+                                                  const cx<TIN> *__restrict__ in, long in_fs, long in_ps, int mk,
+                                                  cx<T> *__restrict__ out, long out_fs, long out_ps, int ld, int j0,
+                                                  int nkz, int tnkz, int lw)
+{
+    extern __shared__ unsigned char smem_raw[];
+    cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
+    const int W = 1 << lw;
+    const int tid = threadIdx.x;
+    const int b = tid & (W - 1), th = tid >> lw;
+    const int i = blockIdx.x * W + b;
+    const bool col = i < mk;
+    const int j = j0 + blockIdx.y;
+    const cx<TIN> *src = in + blockIdx.z * in_fs + (long)j * in_ps;
+    cx<T> *dst = out + blockIdx.z * out_fs + (long)j * out_ps;
+    const int nz = fz.n, S = fz.nstages;
+    cx<T> *x = lds + b;
+    with_radix<E>(fz.radix[0], [&](auto rc) {
+        constexpr int R = decltype(rc)::value;
+        ip_stage<T, E, R, +1, true>(
+            fz, 0, tw, th,
+            [&](int, int f) {
+                const int s = slot_of_freq(f, nz, nkz, tnkz);
+                return (col && s >= 0) ? cvt_<T>(src[(long)s * mk + i]) : make_cx<T>(0, 0);
+            },
+            [&](int, int pos, cx<T> v) { x[pos * W] = v; });
+    });
+    __syncthreads();
+    for (int s = 1; s < S - 1; ++s) {
+        ip_stage_mem<T, E, +1, true>(x, W, fz, s, tw, th);
+        __syncthreads();
+    }
+    with_radix<E>(fz.radix[S - 1], [&](auto rc) {
+        constexpr int R = decltype(rc)::value;
+        ip_stage<T, E, R, +1, true>(
+            fz, S - 1, tw, th, [&](int, int pos) { return x[pos * W]; },
+            [&](int, int pos, cx<T> v) {
+                if (col) dst[(long)pos * ld + i] = v;
+            });
+    });
+}
+
+// -------------------------------------------------------------------------
+// xfused: one z row per block, planes ja..jb (a chunk of the sweep).
+//
+// Rows of the transform passes (T = fx.t threads each): rows 0..npi-1 run the
+// x backward of the next plane (pairs (u1,u2), (u3,th)), rows npi..npi+npo-1
+// the x forward of the previous plane's products, in one pass of S stages.
+// The backward's stage 0 takes its input from registers (fetched one plane
+// ahead); the forward's stage 0 (the last) writes the output, which needs the
+// products paired so that each G/H comes from one pair row:
+//   row 0 (U1U1, d/dy U1U2) -> G1           row 3 (U3U2@GY, U3U3) -> H2, H3
+//   row 1 (U1U2@GY, d/dy U2U2) -> G2        row 4 (TH U1, d/dy TH U2) -> G4
+//   row 2 (U1U3, d/dy U3U2) -> G3, H1       row 5 (TH U3, 0) -> H4
+// LDS: window of three physical planes [3][npi][NX], products [npo][NX],
+// twiddles.
+// -------------------------------------------------------------------------
+template <typename T, int E, int NTH>
+__global__ void __launch_bounds__(512) k_xfused_ip(IpLen fx, const cx<T> *__restrict__ twg, int mk, int ld,
+                                                    const cx<T> *__restrict__ zt, long zt_fs, long zt_ps,
+                                                    cx<T> *__restrict__ wt, long wt_fs, long wt_ps,
+                                                    const double *__restrict__ dy, const double *__restrict__ dyf,
+                                                    const double *__restrict__ kxv, int jlo, int jhi, int chunk)
+{
+    using C = XCount<NTH>;
+    constexpr int npi = C::npi, npo = C::npo;
+    constexpr int R0 = E / 2;
+    extern __shared__ unsigned char smem_raw[];
+    const int nx = fx.n;
+    cx<T> *win = reinterpret_cast<cx<T> *>(smem_raw);
+    cx<T> *fb = win + 3 * npi * nx;
+    cx<T> *tw = fb + npo * nx;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    for (int e = tid; e < fx.twn; e += nthr) tw[e] = twg[e];
+    const int TT = fx.t, S = fx.nstages, L0 = fx.span[0];
+    const int p = tid / TT;
+    int th = tid - p * TT;  // re-laundered every plane (see the loop)
+    const long zrow = (long)blockIdx.x * ld;
+    const int ja = jlo + blockIdx.y * chunk;
+    const int jb = min(jhi, ja + chunk - 1);
+    const cx<T> zero = make_cx<T>(0, 0);
+
+    // Spectral input of stage 0's two groups (k = th, th + T; frequencies
+    // k + q L0) of pair `pair` of plane `plane`, kept in registers.
+    cx<T> pa[E], pb[E];
+    auto fetch = [&](int plane, int pair) {
+        const cx<T> *fa = zt + (2 * pair) * zt_fs + (long)plane * zt_ps + zrow;
+        const cx<T> *fbb = fa + zt_fs;
+        const bool hasb = 2 * pair + 1 < C::nin;
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+#pragma unroll
+            for (int q = 0; q < R0; ++q) {
+                const int f = th + TT * u + q * L0;
+                const int kk = f < mk ? f : (f > nx - mk ? nx - f : -1);
+                pa[u * R0 + q] = kk >= 0 ? fa[kk] : zero;
+                pb[u * R0 + q] = (kk >= 0 && hasb) ? fbb[kk] : zero;
+            }
+        }
+    };
+    // Backward stage 0 from the registers: spectrum of A + iB with the
+    // Hermitian halves of both rows (C2R ignores Im of the DC term).
+    auto dif0 = [&](cx<T> *row) {
+        ip_stage<T, E, R0, +1, true>(
+            fx, 0, tw, th,
+            [&](int idx, int f) {
+                cx<T> A = pa[idx], B = pb[idx];
+                if (f == 0) {
+                    A.im = T(0);
+                    B.im = T(0);
+                }
+                if (f > nx - mk) {
+                    A.im = -A.im;
+                    B.im = -B.im;
+                }
+                return make_cx<T>(A.re - B.im, A.im + B.re);
+            },
+            [&](int, int pos, cx<T> v) { row[pos] = v; });
+    };
+    // Forward stage 0 of product pair row r (plane j): groups th and L0 - th
+    // (0 and L0/2 for th == 0) hold Z(k') and Z(NX - k') of their frequencies.
+    auto dit_out = [&](const cx<T> *row, int r, int j) {
+        const int ga = th, gb = th == 0 ? TT : L0 - th;
+        cx<T> va[R0], vb[R0];
+#pragma unroll
+        for (int q = 0; q < R0; ++q) {
+            va[q] = row[ga + q * L0];
+            vb[q] = row[gb + q * L0];
+        }
+        ip_group<T, R0, -1, false>(va, tw + fx.twoff[0], ga);
+        ip_group<T, R0, -1, false>(vb, tw + fx.twoff[0], gb);
+        const T h = T(0.5) / T(nx);
+        cx<T> *o0 = wt + (long)j * wt_ps + zrow;
+        auto emit = [&](int kp, cx<T> Zk, cx<T> Zn) {
+            const cx<T> Xa = make_cx<T>((Zk.re + Zn.re) * h, (Zk.im - Zn.im) * h);
+            const cx<T> Xb = make_cx<T>((Zk.im + Zn.im) * h, -(Zk.re - Zn.re) * h);
+            const T kx = T(kxv[kp]);
+            // -(i kx Xa + Xb)
+            const cx<T> G = make_cx<T>(kx * Xa.im - Xb.re, -kx * Xa.re - Xb.im);
+            const cx<T> nXa = make_cx<T>(-Xa.re, -Xa.im), nXb = make_cx<T>(-Xb.re, -Xb.im);
+            switch (r) {
+            case 0: o0[0 * wt_fs + kp] = G; break;
+            case 1: o0[2 * wt_fs + kp] = G; break;
+            case 2: o0[4 * wt_fs + kp] = G; o0[1 * wt_fs + kp] = nXa; break;
+            case 3: o0[3 * wt_fs + kp] = nXa; o0[5 * wt_fs + kp] = nXb; break;
+            case 4: o0[6 * wt_fs + kp] = G; break;
+            default: o0[7 * wt_fs + kp] = nXa; break;
+            }
+        };
+#pragma unroll
+        for (int q = 0; q < R0; ++q) {
+            const int ka = ga + q * L0;
+            if (ka < mk) emit(ka, va[q], ga == 0 ? va[(R0 - q) % R0] : vb[R0 - 1 - q]);
+            const int kb = gb + q * L0;
+            if (kb < mk) emit(kb, vb[q], th == 0 ? vb[R0 - 1 - q] : va[R0 - 1 - q]);
+        }
+    };
+    auto slot_row = [&](int plane, int pair) { return win + (plane % 3) * npi * nx + pair * nx; };
+    // The eleven products of plane j at every position, from planes j-1, j, j+1.
+    auto products = [&](int j) {
+        const cx<T> *wm = slot_row(j - 1, 0), *w0 = slot_row(j, 0), *wp = slot_row(j + 1, 0);
+        const T dyj = T(dy[j]), dyfj = T(dyf[j]), dyfm = T(dyf[j - 1]);
+        for (int x = tid; x < nx; x += nthr) {
+            const cx<T> a0 = w0[x], am = wm[x], ap = wp[x];
+            const cx<T> b0 = w0[nx + x], bm = wm[nx + x], bp = wp[nx + x];
+            const T u1 = a0.re, u2 = a0.im, u3 = b0.re;
+            const T u1m = am.re, u2m = am.im, u3m = bm.re;
+            const T u1p = ap.re, u2p = ap.im, u3p = bp.re;
+            const T p0 = u3 * u1;                                                      // U1*U3
+            const T p1 = u1 * u1;                                                      // U1*U1
+            const T p2 = u3 * u3;                                                      // U3*U3
+            const T p3 = ((dyfj * u1 + dyfm * u1m) / (T(2) * dyj)) * u2;               // U1*U2 at GY
+            const T p4 = ((dyfj * u3 + dyfm * u3m) / (T(2) * dyj)) * u2;               // U3*U2 at GY
+            const T p5 = (u1p * u2p + u1 * u2p - u1 * u2 - u1m * u2) / (T(2) * dyfj);  // d/dy(U1 U2)
+            const T p6 = (u3p * u2p + u3 * u2p - u3 * u2 - u3m * u2) / (T(2) * dyfj);  // d/dy(U3 U2)
+            const T sp = u2 + u2p, sm = u2 + u2m;
+            const T p7 = (T(0.25) * (sp * sp) - T(0.25) * (sm * sm)) / dyj;            // d/dy(U2 U2)
+            fb[0 * nx + x] = make_cx<T>(p1, p5);
+            fb[1 * nx + x] = make_cx<T>(p3, p7);
+            fb[2 * nx + x] = make_cx<T>(p0, p6);
+            fb[3 * nx + x] = make_cx<T>(p4, p2);
+            if constexpr (NTH >= 1) {
+                const T t0 = b0.im, tm = bm.im, tp = bp.im;
+                const T p8 = t0 * u1;
+                const T p9 = t0 * u3;
+                const T p10 = (tp * u2p + t0 * u2p - t0 * u2 - tm * u2) / (T(2) * dyfj);
+                fb[4 * nx + x] = make_cx<T>(p8, p10);
+                fb[5 * nx + x] = make_cx<T>(p9, T(0));
+            }
+        }
+    };
+
+    const bool dif_row = p < npi;
+    const bool dit_row = p >= npi && p < npi + npo;
+    // Prologue: planes ja-1, ja (rows 0..2 npi-1) through every backward
+    // stage, plane ja+1 (rows 0..npi-1) through stage 0.
+    const bool pro = p < 2 * npi;
+    const int pro_plane = ja - 1 + (p >= npi ? 1 : 0), pro_pair = p < npi ? p : p - npi;
+    if (pro) fetch(pro_plane, pro_pair);
+    __syncthreads();  // twiddles
+    if (pro) dif0(slot_row(pro_plane, pro_pair));
+    __syncthreads();
+    for (int s = 1; s < S; ++s) {
+        if (pro) ip_stage_mem<T, E, +1, true>(slot_row(pro_plane, pro_pair), 1, fx, s, tw, th);
+        __syncthreads();
+    }
+    if (dif_row) {
+        fetch(ja + 1, p);
+        dif0(slot_row(ja + 1, p));
+    }
+    __syncthreads();
+    // Iteration j: stages 1..S-1 of plane j+1's backward and the forward of
+    // plane j-1's products (S phases), the products of plane j, then stage 0
+    // of plane j+2's backward from registers fetched during the products.
+    // Slot (j+2)%3 held plane j-1, free once the products of plane j are done.
+    for (int j = ja; j <= jb + 1; ++j) {
+        // Keeps the compiler from hoisting the per-thread index arithmetic of
+        // every stage out of the sweep, which would hold it in registers
+        // across the whole loop: recomputing it is cheaper than the spills.
+        asm volatile("" : "+v"(th));
+        const bool inv = j <= jb && dif_row;  // backward of plane j+1
+        const bool fwd = j > ja && dit_row;   // forward of the products of plane j-1
+        cx<T> *drow = inv ? slot_row(j + 1, p) : win;
+        cx<T> *frow = fb + (dit_row ? p - npi : 0) * nx;
+        for (int s = 0; s < S - 1; ++s) {
+            if (inv) ip_stage_mem<T, E, +1, true>(drow, 1, fx, s + 1, tw, th);
+            if (fwd) ip_stage_mem<T, E, -1, false>(frow, 1, fx, S - 1 - s, tw, th);
+            __syncthreads();
+        }
+        if (fwd) dit_out(frow, p - npi, j - 1);
+        __syncthreads();
+        if (j <= jb) {
+            const bool nxt = dif_row && j + 2 <= jb + 1;
+            if (nxt) fetch(j + 2, p);
+            products(j);
+            __syncthreads();
+            if (nxt) dif0(slot_row(j + 2, p));
+            __syncthreads();
+        }
+    }
+}
+
+// -------------------------------------------------------------------------
+// zfwd_nl: W columns per block; both transforms of output o (G then H) in the
+// same LDS; G's result stays in registers, H's last stage combines and writes
+// the kept slots.  grid (column chunks, planes, outputs), block W*T.
+// -------------------------------------------------------------------------
+template <typename T, int E, typename TOUT>
+__global__ void __launch_bounds__(256) k_zfwd_nl_ip(IpLen fz, const cx<T> *__restrict__ tw,
+                                                     const cx<T> *__restrict__ wt, long wt_fs, long wt_ps, int ld,
+                                                     int mk, cx<TOUT> *const *__restrict__ nl, long ps,
+                                                     const double *__restrict__ kzv, int jlo, int nkz, int tnkz, int lw)
+{
+    extern __shared__ unsigned char smem_raw[];
+    cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
+    const int W = 1 << lw;
+    const int tid = threadIdx.x;
+    const int b = tid & (W - 1), th = tid >> lw;
+    const int i = blockIdx.x * W + b;
+    const bool col = i < mk;
+    const int j = jlo + blockIdx.y;
+    const int o = blockIdx.z;
+    const int nz = fz.n, S = fz.nstages;
+    const cx<T> *sg = wt + (2 * o) * wt_fs + (long)j * wt_ps + i;
+    const cx<T> *sh = sg + wt_fs;
+    cx<T> *x = lds + b;
+    const int rl = fz.radix[S - 1];
+    cx<T> gin[E], hin[E], gv[E];
+    // inputs of the first forward stage (stage S-1: groups of R consecutive positions)
+    auto load_first = [&](const cx<T> *src, cx<T> *regs) {
+        with_radix<E>(rl, [&](auto rc) {
+            constexpr int R = decltype(rc)::value;
+            ip_groups<E, R>(fz, S - 1, th, [&](int u, int base, int) {
+#pragma unroll
+                for (int q = 0; q < R; ++q)
+                    regs[u * R + q] = col ? src[(long)(base + q * fz.span[S - 1]) * ld] : make_cx<T>(0, 0);
+            });
+        });
+    };
+    auto first = [&](cx<T> *regs) {
+        with_radix<E>(rl, [&](auto rc) {
+            constexpr int R = decltype(rc)::value;
+            ip_stage<T, E, R, -1, false>(
+                fz, S - 1, tw, th, [&](int idx, int) { return regs[idx]; },
+                [&](int, int pos, cx<T> v) { x[pos * W] = v; });
+        });
+    };
+    load_first(sg, gin);
+    load_first(sh, hin);
+    first(gin);
+    __syncthreads();
+    for (int s = 1; s < S - 1; ++s) {
+        ip_stage_mem<T, E, -1, false>(x, W, fz, S - 1 - s, tw, th);
+        __syncthreads();
+    }
+    with_radix<E>(fz.radix[0], [&](auto rc) {
+        constexpr int R = decltype(rc)::value;
+        ip_stage<T, E, R, -1, false>(
+            fz, 0, tw, th, [&](int, int pos) { return x[pos * W]; }, [&](int idx, int, cx<T> v) { gv[idx] = v; });
+    });
+    __syncthreads();
+    first(hin);
+    __syncthreads();
+    for (int s = 1; s < S - 1; ++s) {
+        ip_stage_mem<T, E, -1, false>(x, W, fz, S - 1 - s, tw, th);
+        __syncthreads();
+    }
+    const T inv_nz = T(1) / T(nz);
+    cx<TOUT> *dst = nl[o] + (long)j * ps + i;
+    with_radix<E>(fz.radix[0], [&](auto rc) {
+        constexpr int R = decltype(rc)::value;
+        ip_stage<T, E, R, -1, false>(
+            fz, 0, tw, th, [&](int, int pos) { return x[pos * W]; },
+            [&](int idx, int f, cx<T> hv) {
+                const int s = slot_of_freq(f, nz, nkz, tnkz);
+                if (col && s >= 0) {
+                    const cx<T> g = scale(gv[idx], inv_nz), hs = scale(hv, inv_nz);
+                    const T kz = T(kzv[s]);
+                    dst[(long)s * mk] = cvt_<TOUT>(make_cx<T>(g.re - kz * hs.im, g.im + kz * hs.re));
+                }
+            });
+    });
+}
+
+// -------------------------------------------------------------------------
+// launchers
+// -------------------------------------------------------------------------
+
+#define DGPU_DISPATCH_IPE(E_, CALL)                                \
+    switch (E_) {                                                  \
+    case 4: { constexpr int EE = 4; CALL; } break;                 \
+    case 6: { constexpr int EE = 6; CALL; } break;                 \
+    case 8: { constexpr int EE = 8; CALL; } break;                 \
+    case 12: { constexpr int EE = 12; CALL; } break;               \
+    case 16: { constexpr int EE = 16; CALL; } break;               \
+    case 24: { constexpr int EE = 24; CALL; } break;               \
+    default: break;                                                \
+    }
+
+static inline int z_lw(int t)
+{
+    int lw = 0;
+    while ((2 << lw) * t <= 256) ++lw;
+    return lw;
+}
+
+size_t zip_lds_bytes(const IpLen &fz, size_t elem) { return elem * ((size_t)1 << z_lw(fz.t)) * fz.n; }
+
+template <typename T, int E, typename TIN>
+static void zinv_ip_e(hipStream_t s, const Geo &g, const IpLen &fz, const cx<T> *tw, const cx<TIN> *in, long in_fs,
+                      long in_ps, cx<T> *out, long out_fs, long out_ps, int ld, int j0, int nplanes, int nfields)
+{
+    const int lw = z_lw(fz.t), W = 1 << lw;
+    dim3 grid((g.mk + W - 1) / W, nplanes, nfields);
+    const size_t lds = sizeof(cx<T>) * W * fz.n;
+    hipLaunchKernelGGL((k_zinv_ip<T, E, TIN>), grid, dim3(W * fz.t), lds, s, fz, tw, in, in_fs, in_ps, g.mk, out,
+                       out_fs, out_ps, ld, j0, g.nkz, g.tnkz, lw);
+}
+
+template <typename T, typename TIN>
+void launch_zinv_ip(hipStream_t s, const Geo &g, const IpLen &fz, const cx<T> *tw, const cx<TIN> *in, long in_fs,
+                    long in_ps, cx<T> *out, long out_fs, long out_ps, int ld, int j0, int nplanes, int nfields)
+{
+    DGPU_DISPATCH_IPE(fz.e, (zinv_ip_e<T, EE, TIN>(s, g, fz, tw, in, in_fs, in_ps, out, out_fs, out_ps, ld, j0,
+                                                  nplanes, nfields)));
+}
+
+template <typename T, int E, int NTH>
+static void xfused_ip_e(hipStream_t s, const Geo &g, const IpLen &fx, const cx<T> *tw, int ld, const cx<T> *zt,
+                        long zt_fs, cx<T> *wt, long wt_fs, const double *dy, const double *dyf, const double *kx,
+                        int jlo, int jhi, int chunk)
+{
+    using C = XCount<NTH>;
+    const int nchunks = (jhi - jlo + 1 + chunk - 1) / chunk;
+    const int threads = fx.t * (C::npi + C::npo);
+    const size_t lds = sizeof(cx<T>) * ((size_t)fx.n * (3 * C::npi + C::npo) + fx.twn);
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_xfused_ip<T, E, NTH>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        attr = true;
+    }
+    const long ps = (long)g.nz * ld;
+    hipLaunchKernelGGL((k_xfused_ip<T, E, NTH>), dim3(g.nz, nchunks), dim3(threads), lds, s, fx, tw, g.mk, ld, zt,
+                       zt_fs, ps, wt, wt_fs, ps, dy, dyf, kx, jlo, jhi, chunk);
+}
+
+size_t xfused_ip_lds_bytes(const IpLen &fx, int nth, size_t elem)
+{
+    const int npi = (3 + nth + 1) / 2, npo = (8 + 3 * nth + 1) / 2;
+    return elem * ((size_t)fx.n * (3 * npi + npo) + fx.twn);
+}
+
+int xfused_ip_threads(const IpLen &fx, int nth) { return fx.t * ((3 + nth + 1) / 2 + (8 + 3 * nth + 1) / 2); }
+
+template <typename T>
+void launch_xfused_ip(hipStream_t s, const Geo &g, const IpLen &fx, const cx<T> *tw, int ld, const cx<T> *zt,
+                      long zt_fs, cx<T> *wt, long wt_fs, const double *dy, const double *dyf, const double *kx,
+                      int jlo, int jhi, int chunk)
+{
+#define DGPU_XF(EE_)                                                                                            \
+    do {                                                                                                        \
+        if (g.nth == 0)                                                                                         \
+            xfused_ip_e<T, EE_, 0>(s, g, fx, tw, ld, zt, zt_fs, wt, wt_fs, dy, dyf, kx, jlo, jhi, chunk);       \
+        else                                                                                                    \
+            xfused_ip_e<T, EE_, 1>(s, g, fx, tw, ld, zt, zt_fs, wt, wt_fs, dy, dyf, kx, jlo, jhi, chunk);       \
+    } while (0)
+    switch (fx.e) {
+    case 4: DGPU_XF(4); break;
+    case 6: DGPU_XF(6); break;
+    case 8: DGPU_XF(8); break;
+    case 12: DGPU_XF(12); break;
+    case 16: DGPU_XF(16); break;
+    default: break;
+    }
+#undef DGPU_XF
+}
+
+template <typename T, int E, typename TOUT>
+static void zfwd_nl_ip_e(hipStream_t s, const Geo &g, const IpLen &fz, const cx<T> *tw, const cx<T> *wt, long wt_fs,
+                         int ld, cx<TOUT> *const *nl, const double *kz, int jlo, int jhi, int nout)
+{
+    const int lw = z_lw(fz.t), W = 1 << lw;
+    dim3 grid((g.mk + W - 1) / W, jhi - jlo + 1, nout);
+    const size_t lds = sizeof(cx<T>) * W * fz.n;
+    const long wt_ps = (long)g.nz * ld;
+    hipLaunchKernelGGL((k_zfwd_nl_ip<T, E, TOUT>), grid, dim3(W * fz.t), lds, s, fz, tw, wt, wt_fs, wt_ps, ld, g.mk,
+                       nl, g.ps, kz, jlo, g.nkz, g.tnkz, lw);
+}
+
+template <typename T, typename TOUT>
+void launch_zfwd_nl_ip(hipStream_t s, const Geo &g, const IpLen &fz, const cx<T> *tw, const cx<T> *wt, long wt_fs,
+                       int ld, cx<TOUT> *const *nl, const double *kz, int jlo, int jhi, int nout)
+{
+    DGPU_DISPATCH_IPE(fz.e, (zfwd_nl_ip_e<T, EE, TOUT>(s, g, fz, tw, wt, wt_fs, ld, nl, kz, jlo, jhi, nout)));
+}
+
+template void launch_zinv_ip<double, double>(hipStream_t, const Geo &, const IpLen &, const cx<double> *,
+                                             const cx<double> *, long, long, cx<double> *, long, long, int, int, int,
+                                             int);
+template void launch_zinv_ip<float, double>(hipStream_t, const Geo &, const IpLen &, const cx<float> *,
+                                            const cx<double> *, long, long, cx<float> *, long, long, int, int, int,
+                                            int);
+template void launch_xfused_ip<double>(hipStream_t, const Geo &, const IpLen &, const cx<double> *, int,
+                                       const cx<double> *, long, cx<double> *, long, const double *, const double *,
+                                       const double *, int, int, int);
+template void launch_xfused_ip<float>(hipStream_t, const Geo &, const IpLen &, const cx<float> *, int,
+                                      const cx<float> *, long, cx<float> *, long, const double *, const double *,
+                                      const double *, int, int, int);
+template void launch_zfwd_nl_ip<double, double>(hipStream_t, const Geo &, const IpLen &, const cx<double> *,
+                                                const cx<double> *, long, int, cx<double> *const *, const double *,
+                                                int, int, int);
+template void launch_zfwd_nl_ip<float, float>(hipStream_t, const Geo &, const IpLen &, const cx<float> *,
+                                              const cx<float> *, long, int, cx<float> *const *, const double *, int,
+                                              int, int);
+
+}  // namespace dgpu

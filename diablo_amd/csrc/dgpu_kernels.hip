@@ -1,22 +1,8 @@
-// dgpu_kernels.hip -- gfx950 kernels of the channel time-stepper.
-//
-// One RK3 substep (RK_CHAN_1, for/channel.f:62-1073) is five launches:
-//
-//   zinv     CU1,CU2,CU3,CTH (kept modes)  --z-inverse-->  Zt[f][j][z][kx]
-//   xfused   per z-row, sweeping j: C2R of the four fields (pairs packed in one
-//            complex FFT), the eleven nonlinear products of
-//            for/channel.f:410-571 and 670-720 in physical space, R2C of the
-//            products, the d/dx terms applied (-i kx), written as
-//            G = -(i kx X(a) + X(b)) and H = -X(c) per momentum/scalar eq.
-//   zfwd_nl  NL_f = Z(G_f) + i kz Z(H_f)   (z-forward of both, scaled 1/(NX NZ))
-//   cn       per (kx,kz) mode: explicit RHS and the Crank-Nicolson solves of
-//            TH, U2, U1, U3 in Fourier space (dgpu_columns.hip)
-//   remdiv   per mode: REM_DIV_CHAN and the CP update (dgpu_columns.hip)
-//
-// The reference's eight physical<->Fourier transforms around its physical-space
-// CN solves are not needed: with LES off the CN matrix is the same for every
-// (x,z) column, so the solve commutes with the x-z transform and is done per
-// Fourier mode, with the wall values entering only the (0,0) mode.
+// dgpu_kernels.hip -- Stockham-transform kernels of the standalone entry points
+// FFT_XZ_TO_FOURIER / FFT_XZ_TO_PHYSICAL (for/mpi.f:1530-1674) and of
+// POISSON_P_CHAN, plus APPLY_BC_VEL on kept-mode fields.  The physical-space
+// side of these transforms is in natural order (Fortran-layout planes).  The
+// substep's own transforms are in dgpu_hot.hip.
 #include "dgpu_kernels.hpp"
 
 namespace dgpu {
@@ -72,205 +58,6 @@ __global__ void k_zinv(FftLen fz, const cx<T> *__restrict__ tw, const cx<TIN> *_
         const int z = idx >> lw, bb = idx & (W - 1);
         const int i = i0 + bb;
         if (i < ncols) dst[(long)z * out_mk + i] = lds[idx];
-    }
-}
-
-// -------------------------------------------------------------------------
-// xfused: per z-row, planes jlo..jhi.  LDS: window of 3 planes x NPI complex
-// rows (pairs of real fields) + NPO complex rows for the products.
-// -------------------------------------------------------------------------
-template <int NTH>
-struct XCounts {
-    static constexpr int nin = 3 + NTH;                 // u1 u2 u3 th...
-    static constexpr int npi = (nin + 1) / 2;           // C2R pairs
-    static constexpr int nprod = 8 + 3 * NTH;           // eleven products for NTH = 1
-    static constexpr int npo = (nprod + 1) / 2;         // R2C pairs
-    static constexpr int nout = 2 * nin;                // G,H per equation
-};
-
-template <typename T, int E, int NTH>
-__global__ void k_xfused(FftLen fx, const cx<T> *__restrict__ tw, int mk, const cx<T> *__restrict__ zt,
-                         long zt_fs, long zt_ps, cx<T> *__restrict__ wt, long wt_fs, long wt_ps,
-                         const double *__restrict__ dy, const double *__restrict__ dyf,
-                         const double *__restrict__ kxv, int jlo, int jhi, int chunk)
-{
-    using C = XCounts<NTH>;
-    extern __shared__ unsigned char smem_raw[];
-    cx<T> *win = reinterpret_cast<cx<T> *>(smem_raw);     // [3][npi][nx]
-    const int nx = fx.n;
-    cx<T> *fb = win + 3 * C::npi * nx;                    // [npo][nx]
-    const int tid = threadIdx.x, nthr = blockDim.x;
-    const int z = blockIdx.x;
-    const int ja = jlo + blockIdx.y * chunk;
-    const int jb = min(jhi, ja + chunk - 1);
-    const long zrow = (long)z * mk;
-    const cx<T> zero = make_cx<T>(0, 0);
-    const T inv_nx = T(1) / T(nx);
-
-    auto load_plane = [&](int q) {
-        cx<T> *w = win + (q % 3) * C::npi * nx;
-        for (int p = 0; p < C::npi; ++p)
-        for (int k = tid; k < nx; k += nthr) {
-            const int idx = p * nx + k;
-            const int fa = 2 * p, fb2 = 2 * p + 1;
-            cx<T> A = zero, B = zero;
-            bool hit = false, cj = false;
-            int kk = k;
-            if (k < mk) {
-                hit = true;
-            } else if (k > nx - mk) {
-                hit = true; cj = true; kk = nx - k;
-            }
-            if (hit) {
-                A = zt[fa * zt_fs + (long)q * zt_ps + zrow + kk];
-                if (fb2 < C::nin) B = zt[fb2 * zt_fs + (long)q * zt_ps + zrow + kk];
-                if (kk == 0) { A.im = 0; B.im = 0; }   // C2R ignores Im of the DC term
-                if (cj) { A.im = -A.im; B.im = -B.im; }
-            }
-            w[idx] = make_cx<T>(A.re - B.im, A.im + B.re);  // A + i B
-        }
-        __syncthreads();
-        const int p = tid / fx.t, th = tid - p * fx.t;
-        fft_lds<T, E, +1>(win + (q % 3) * C::npi * nx + p * nx, 1, fx, th, p < C::npi, tw);
-    };
-
-    load_plane(ja - 1);
-    load_plane(ja);
-    for (int j = ja; j <= jb; ++j) {
-        load_plane(j + 1);
-        const cx<T> *wm = win + ((j - 1) % 3) * C::npi * nx;
-        const cx<T> *w0 = win + (j % 3) * C::npi * nx;
-        const cx<T> *wp = win + ((j + 1) % 3) * C::npi * nx;
-        const T dyj = T(dy[j]), dyfj = T(dyf[j]), dyfm = T(dyf[j - 1]), dyp = T(dy[j + 1]);
-        (void)dyp;
-        for (int x = tid; x < nx; x += nthr) {
-            const cx<T> a0 = w0[x], am = wm[x], ap = wp[x];
-            const cx<T> b0 = w0[nx + x], bm = wm[nx + x], bp = wp[nx + x];
-            const T u1 = a0.re, u2 = a0.im, u3 = b0.re;
-            const T u1m = am.re, u2m = am.im, u3m = bm.re;
-            const T u1p = ap.re, u2p = ap.im, u3p = bp.re;
-            T pr[C::nprod + 1];
-            pr[0] = u3 * u1;                                                        // U1*U3
-            pr[1] = u1 * u1;                                                        // U1*U1
-            pr[2] = u3 * u3;                                                        // U3*U3
-            pr[3] = ((dyfj * u1 + dyfm * u1m) / (T(2) * dyj)) * u2;                 // U1*U2 at GY
-            pr[4] = ((dyfj * u3 + dyfm * u3m) / (T(2) * dyj)) * u2;                 // U3*U2 at GY
-            pr[5] = (u1p * u2p + u1 * u2p - u1 * u2 - u1m * u2) / (T(2) * dyfj);    // d/dy(U1 U2)
-            pr[6] = (u3p * u2p + u3 * u2p - u3 * u2 - u3m * u2) / (T(2) * dyfj);    // d/dy(U3 U2)
-            {
-                const T sp = u2 + u2p, sm = u2 + u2m;
-                pr[7] = (T(0.25) * (sp * sp) - T(0.25) * (sm * sm)) / dyj;          // d/dy(U2 U2)
-            }
-            if constexpr (NTH >= 1) {
-                const T th = b0.im, thm = bm.im, thp = bp.im;
-                pr[8] = th * u1;
-                pr[9] = th * u3;
-                pr[10] = (thp * u2p + th * u2p - th * u2 - thm * u2) / (T(2) * dyfj);
-            }
-            pr[C::nprod] = T(0);
-#pragma unroll
-            for (int p = 0; p < C::npo; ++p) fb[p * nx + x] = make_cx<T>(pr[2 * p], pr[2 * p + 1]);
-        }
-        __syncthreads();
-        {
-            const int p = tid / fx.t, th = tid - p * fx.t;
-            fft_lds<T, E, -1>(fb + p * nx, 1, fx, th, p < C::npo, tw);
-        }
-        for (int k = tid; k < mk; k += nthr) {
-            T xr[C::nprod + 1], xi[C::nprod + 1];
-#pragma unroll
-            for (int p = 0; p < C::npo; ++p) {
-                const cx<T> Z = fb[p * nx + k];
-                const cx<T> Zn = fb[p * nx + (k == 0 ? 0 : nx - k)];
-                // A = (Z + conj Zn)/2, B = (Z - conj Zn)/(2i); then 1/NX
-                const T h = T(0.5) * inv_nx;
-                xr[2 * p] = (Z.re + Zn.re) * h;
-                xi[2 * p] = (Z.im - Zn.im) * h;
-                xr[2 * p + 1] = (Z.im + Zn.im) * h;
-                xi[2 * p + 1] = -(Z.re - Zn.re) * h;
-            }
-            const T kx = T(kxv[k]);
-            // i*kx*X = (-kx*Xi, kx*Xr)
-            auto outw = [&](int o, T re, T im) {
-                wt[o * wt_fs + (long)j * wt_ps + zrow + k] = make_cx<T>(re, im);
-            };
-            // momentum x: CF1 -= i kz X(U1U3) + i kx X(U1U1) + X(dy U1U2)
-            outw(0, -(-kx * xi[1] + xr[5]), -(kx * xr[1] + xi[5]));
-            outw(1, -xr[0], -xi[0]);
-            // momentum y: CF2 -= i kx X(U1U2) + i kz X(U3U2) + X(dy U2U2)
-            outw(2, -(-kx * xi[3] + xr[7]), -(kx * xr[3] + xi[7]));
-            outw(3, -xr[4], -xi[4]);
-            // momentum z: CF3 -= i kx X(U1U3) + i kz X(U3U3) + X(dy U3U2)
-            outw(4, -(-kx * xi[0] + xr[6]), -(kx * xr[0] + xi[6]));
-            outw(5, -xr[2], -xi[2]);
-            if constexpr (NTH >= 1) {
-                // scalar: CFTH -= i kx X(TH U1) + i kz X(TH U3) + X(dy TH U2)
-                outw(6, -(-kx * xi[8] + xr[10]), -(kx * xr[8] + xi[10]));
-                outw(7, -xr[9], -xi[9]);
-            }
-        }
-        __syncthreads();
-    }
-}
-
-// -------------------------------------------------------------------------
-// zfwd_nl: NL_f(j, K, I) = [FFT_z G_f + i kz FFT_z H_f] / NZ at kept slots.
-// grid (col-chunks, planes, outputs); block W*T; LDS W*NZ complex.
-// -------------------------------------------------------------------------
-template <typename T, int E, typename TOUT>
-__global__ void k_zfwd_nl(FftLen fz, const cx<T> *__restrict__ tw, const cx<T> *__restrict__ wt, long wt_fs,
-                          long wt_ps, int mk, cx<TOUT> *const *__restrict__ nl, long ps, const double *__restrict__ kzv,
-                          int jlo, int nkz, int tnkz, int lw)
-{
-    extern __shared__ unsigned char smem_raw[];
-    cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
-    const int W = 1 << lw;
-    constexpr int MAXIT = 24;
-    const int nz = fz.n;
-    const int tid = threadIdx.x, nthr = blockDim.x;
-    const int i0 = blockIdx.x * W;
-    const int j = jlo + blockIdx.y;
-    const int o = blockIdx.z;
-    const int mz = tnkz + 1;
-    const T inv_nz = T(1) / T(nz);
-    cx<T> keep[MAXIT];
-    const int b = tid & (W - 1), th = tid >> lw;
-#pragma unroll
-    for (int pass = 0; pass < 2; ++pass) {
-        const cx<T> *src = wt + (2 * o + pass) * wt_fs + (long)j * wt_ps;
-        for (int idx = tid; idx < W * nz; idx += nthr) {
-            const int z = idx >> lw, bb = idx & (W - 1);
-            const int i = i0 + bb;
-            lds[idx] = (i < mk) ? src[(long)z * mk + i] : make_cx<T>(0, 0);
-        }
-        __syncthreads();
-        fft_lds<T, E, -1>(lds + b, W, fz, th, th < fz.t, tw);
-#pragma unroll
-        for (int it = 0; it < MAXIT; ++it) {
-            const int idx = tid + it * nthr;
-            if (idx < W * mz) {
-                const int s = idx >> lw, bb = idx & (W - 1);
-                const cx<T> v = lds[kzslot_to_z(s, nkz, tnkz, nz) * W + bb];
-                if (pass == 0) {
-                    keep[it] = scale(v, inv_nz);
-                } else {
-                    const T kz = T(kzv[s]);
-                    const cx<T> h = scale(v, inv_nz);
-                    keep[it] = make_cx<T>(keep[it].re - kz * h.im, keep[it].im + kz * h.re);
-                }
-            }
-        }
-        __syncthreads();
-    }
-    cx<TOUT> *dst = nl[o] + (long)j * ps;
-#pragma unroll
-    for (int it = 0; it < MAXIT; ++it) {
-        const int idx = tid + it * nthr;
-        if (idx < W * mz) {
-            const int s = idx >> lw, bb = idx & (W - 1);
-            const int i = i0 + bb;
-            if (i < mk) dst[(long)s * mk + i] = cvt<TOUT>(keep[it]);
-        }
     }
 }
 
@@ -517,58 +304,6 @@ void launch_zinv(hipStream_t s, const Geo &g, const Fft<T> &fz, const cx<TIN> *i
                                              j0, nplanes, nfields)));
 }
 
-template <typename T, int E, int NTH>
-static void xfused_e(hipStream_t s, const Geo &g, const Fft<T> &fx, const cx<T> *zt, long zt_fs, cx<T> *wt,
-                     long wt_fs, const double *dy, const double *dyf, const double *kx, int jlo, int jhi)
-{
-    using C = XCounts<NTH>;
-    const int chunk = 16;
-    const int nchunks = (jhi - jlo + 1 + chunk - 1) / chunk;
-    int threads = fx.len.t * C::npo;
-    threads = ((threads + 63) / 64) * 64;
-    if (threads < 64) threads = 64;
-    const size_t lds = sizeof(cx<T>) * fx.len.n * (3 * C::npi + C::npo);
-    static bool attr = false;
-    if (!attr) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_xfused<T, E, NTH>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        attr = true;
-    }
-    const long zt_ps = (long)g.nz * g.mk;
-    hipLaunchKernelGGL((k_xfused<T, E, NTH>), dim3(g.nz, nchunks), dim3(threads), lds, s, fx.len, fx.tw, g.mk, zt,
-                       zt_fs, zt_ps, wt, wt_fs, zt_ps, dy, dyf, kx, jlo, jhi, chunk);
-}
-
-template <typename T>
-void launch_xfused(hipStream_t s, const Geo &g, const Fft<T> &fx, const cx<T> *zt, long zt_fs, cx<T> *wt,
-                   long wt_fs, const double *dy, const double *dyf, const double *kx, int jlo, int jhi)
-{
-    if (g.nth == 0) {
-        DGPU_DISPATCH_E(fx.len.e, (xfused_e<T, EE, 0>(s, g, fx, zt, zt_fs, wt, wt_fs, dy, dyf, kx, jlo, jhi)));
-    } else {
-        DGPU_DISPATCH_E(fx.len.e, (xfused_e<T, EE, 1>(s, g, fx, zt, zt_fs, wt, wt_fs, dy, dyf, kx, jlo, jhi)));
-    }
-}
-
-template <typename T, int E, typename TOUT>
-static void zfwd_nl_e(hipStream_t s, const Geo &g, const Fft<T> &fz, const cx<T> *wt, long wt_fs,
-                      cx<TOUT> *const *nl, const double *kz, int jlo, int jhi, int nout)
-{
-    const int lw = chunk_lw(fz.len.t, 256), W = 1 << lw;
-    dim3 grid((g.mk + W - 1) / W, jhi - jlo + 1, nout);
-    const size_t lds = sizeof(cx<T>) * W * fz.len.n;
-    const long wt_ps = (long)g.nz * g.mk;
-    hipLaunchKernelGGL((k_zfwd_nl<T, E, TOUT>), grid, dim3(W * fz.len.t), lds, s, fz.len, fz.tw, wt, wt_fs, wt_ps, g.mk,
-                       nl, g.ps, kz, jlo, g.nkz, g.tnkz, lw);
-}
-
-template <typename T, typename TOUT>
-void launch_zfwd_nl(hipStream_t s, const Geo &g, const Fft<T> &fz, const cx<T> *wt, long wt_fs, cx<TOUT> *const *nl,
-                    const double *kz, int jlo, int jhi, int nout)
-{
-    DGPU_DISPATCH_E(fz.len.e, (zfwd_nl_e<T, EE, TOUT>(s, g, fz, wt, wt_fs, nl, kz, jlo, jhi, nout)));
-}
-
 template <typename T, int E>
 static void x_r2c_e(hipStream_t s, const Geo &g, const Fft<T> &fx, const double *phys, cx<T> *xt, int j0,
                     int nplanes)
@@ -645,16 +380,6 @@ void launch_apply_bc_vel(hipStream_t s, const Geo &g, cx<double> *cu1, cx<double
 // explicit instantiations: FP64 path and the mixed path (FP32 transforms)
 template void launch_zinv<double, double>(hipStream_t, const Geo &, const Fft<double> &, const cx<double> *, long,
                                           long, int, int, cx<double> *, long, long, int, int, int, int);
-template void launch_zinv<float, double>(hipStream_t, const Geo &, const Fft<float> &, const cx<double> *, long,
-                                         long, int, int, cx<float> *, long, long, int, int, int, int);
-template void launch_xfused<double>(hipStream_t, const Geo &, const Fft<double> &, const cx<double> *, long,
-                                    cx<double> *, long, const double *, const double *, const double *, int, int);
-template void launch_xfused<float>(hipStream_t, const Geo &, const Fft<float> &, const cx<float> *, long,
-                                   cx<float> *, long, const double *, const double *, const double *, int, int);
-template void launch_zfwd_nl<double, double>(hipStream_t, const Geo &, const Fft<double> &, const cx<double> *, long,
-                                             cx<double> *const *, const double *, int, int, int);
-template void launch_zfwd_nl<float, float>(hipStream_t, const Geo &, const Fft<float> &, const cx<float> *, long,
-                                           cx<float> *const *, const double *, int, int, int);
 template void launch_x_r2c<double>(hipStream_t, const Geo &, const Fft<double> &, const double *, cx<double> *, int,
                                    int);
 template void launch_zfwd_pack<double>(hipStream_t, const Geo &, const Fft<double> &, const cx<double> *, double *,

@@ -71,13 +71,22 @@ void launch_zinv(hipStream_t s, const Geo &g, const Fft<T> &fz, const cx<TIN> *i
                  int in_mk, int ncols, cx<T> *out, long out_fs, long out_ps, int out_mk, int j0, int nplanes,
                  int nfields);
 
-template <typename T>
-void launch_xfused(hipStream_t s, const Geo &g, const Fft<T> &fx, const cx<T> *zt, long zt_fs, cx<T> *wt,
-                   long wt_fs, const double *dy, const double *dyf, const double *kx, int jlo, int jhi);
 
+// ---- substep transforms on the in-place plans (dgpu_hot.hip) ----
+// Zt/Wt rows are ld complex values long; tw points at the plan's twiddles.
+template <typename T, typename TIN>
+void launch_zinv_ip(hipStream_t s, const Geo &g, const IpLen &fz, const cx<T> *tw, const cx<TIN> *in, long in_fs,
+                    long in_ps, cx<T> *out, long out_fs, long out_ps, int ld, int j0, int nplanes, int nfields);
+template <typename T>
+void launch_xfused_ip(hipStream_t s, const Geo &g, const IpLen &fx, const cx<T> *tw, int ld, const cx<T> *zt,
+                      long zt_fs, cx<T> *wt, long wt_fs, const double *dy, const double *dyf, const double *kx,
+                      int jlo, int jhi, int chunk);
 template <typename T, typename TOUT>
-void launch_zfwd_nl(hipStream_t s, const Geo &g, const Fft<T> &fz, const cx<T> *wt, long wt_fs, cx<TOUT> *const *nl,
-                    const double *kz, int jlo, int jhi, int nout);
+void launch_zfwd_nl_ip(hipStream_t s, const Geo &g, const IpLen &fz, const cx<T> *tw, const cx<T> *wt, long wt_fs,
+                       int ld, cx<TOUT> *const *nl, const double *kz, int jlo, int jhi, int nout);
+size_t zip_lds_bytes(const IpLen &fz, size_t elem);
+size_t xfused_ip_lds_bytes(const IpLen &fx, int nth, size_t elem);
+int xfused_ip_threads(const IpLen &fx, int nth);
 
 template <typename T, typename NLT>
 void launch_cn(hipStream_t s, const CnArgs &a, const CnPtrs<T, NLT> &p);

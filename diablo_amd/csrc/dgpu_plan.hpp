@@ -1,0 +1,133 @@
+// dgpu_plan.hpp -- host-side plans for the in-place transforms (IpLen).
+//
+// A plan fixes the radices of a length, the values per thread E and the
+// per-stage twiddle tables.  The x-direction plan of the fused nonlinear
+// kernel has a constraint of its own: its outermost radix R_0 is E/2, so that
+// in the last forward stage every thread holds two whole groups, k and
+// L_0 - k, i.e. both Z(k') and Z(n - k') of every frequency k' it owns, and
+// can split the transform of a pair of real rows without another pass
+// through LDS.
+#pragma once
+
+#include <cmath>
+#include <cstdlib>
+#include <vector>
+
+#include "dgpu_fft.hpp"
+
+namespace dgpu {
+
+// Radices of m that divide e, fewest stages first (greedy on 8, 6, 5, 4, 3, 2).
+inline bool ip_factor(int m, int e, std::vector<int> &out)
+{
+    static const int cand[] = {8, 6, 5, 4, 3, 2};
+    while (m > 1) {
+        bool ok = false;
+        for (int r : cand) {
+            if (e % r == 0 && m % r == 0) {
+                out.push_back(r);
+                m /= r;
+                ok = true;
+                break;
+            }
+        }
+        if (!ok) return false;
+    }
+    return true;
+}
+
+inline bool ip_fill(int n, int e, const std::vector<int> &radix, IpLen &f)
+{
+    if ((int)radix.size() > kMaxStages || radix.empty() || n % e) return false;
+    f = IpLen{};
+    f.n = n;
+    f.nstages = (int)radix.size();
+    f.e = e;
+    f.t = n / e;
+    int m = n, off = 0;
+    for (int i = 0; i < f.nstages; ++i) {
+        const int R = radix[i];
+        if (e % R || m % R) return false;
+        f.radix[i] = R;
+        f.span[i] = m / R;
+        f.magic[i] = f.span[i] == 1 ? 0u : (unsigned)(((1ULL << 32) + f.span[i] - 1) / f.span[i]);
+        f.twoff[i] = off;
+        off += f.span[i] * (R - 1);
+        m /= R;
+    }
+    if (m != 1) return false;
+    f.twn = off;
+    return true;
+}
+
+// Twiddle tables of a plan: stage i, entry k*(R_i-1) + q-1 = exp(-2 pi i k q / m_i).
+inline void ip_twiddles(const IpLen &f, std::vector<cx<double>> &tw)
+{
+    const long double twopi = 6.283185307179586476925286766559005768L;
+    tw.assign(f.twn, cx<double>{0.0, 0.0});
+    for (int i = 0; i < f.nstages; ++i) {
+        const int R = f.radix[i], L = f.span[i], m = R * L;
+        for (int k = 0; k < L; ++k)
+            for (int q = 1; q < R; ++q) {
+                const long double a = -twopi * (long double)(k * q) / (long double)m;
+                cx<double> &w = tw[f.twoff[i] + k * (R - 1) + q - 1];
+                w.re = (double)cosl(a);
+                w.im = (double)sinl(a);
+            }
+    }
+}
+
+// Plan of a z-direction (column batch) transform: fewest stages with at most
+// e_max values per thread (e_req > 0 asks for that E).  At least two threads
+// per transform, hence at least two stages.
+inline bool ip_plan_z(int n, int e_req, int e_max, IpLen &f)
+{
+    static const int cand[] = {12, 8, 16, 24, 6, 4};
+    int best_s = 1 << 30;
+    bool found = false;
+    for (int e : cand) {
+        if (e_req > 0 ? e != e_req : e > e_max) continue;
+        if (n % e || n / e > 64 || n / e < 2) continue;
+        std::vector<int> r;
+        if (!ip_factor(n, e, r)) continue;
+        // fewer stages first; on a tie the earlier candidate (moderate E) wins
+        if ((int)r.size() < best_s) {
+            IpLen g;
+            if (!ip_fill(n, e, r, g)) continue;
+            f = g;
+            best_s = (int)r.size();
+            found = true;
+        }
+    }
+    return found;
+}
+
+// Plan of the x-direction transforms of the fused nonlinear kernel:
+// R_0 = E/2 (see the file comment); at most max_t threads per transform and
+// e_max values per thread unless e_req > 0 asks for an E.
+inline bool ip_plan_x(int n, int e_req, int e_max, int max_t, IpLen &f)
+{
+    static const int r0s[] = {6, 4, 8, 3, 2};
+    int best_s = 1 << 30, best_d = 1 << 30;
+    bool found = false;
+    for (int r0 : r0s) {
+        const int e = 2 * r0;
+        if (e_req > 0 ? e != e_req : e > e_max) continue;
+        if (n % e || n / e > max_t || n / e < 2) continue;
+        std::vector<int> r{r0};
+        if (!ip_factor(n / r0, e, r)) continue;
+        const int t = n / e;
+        const int d = std::abs(t - 32);
+        if ((int)r.size() < best_s || ((int)r.size() == best_s && d < best_d)) {
+            IpLen g;
+            if (!ip_fill(n, e, r, g)) continue;
+            f = g;
+            best_s = (int)r.size();
+            best_d = d;
+            found = true;
+        }
+    }
+    return found;
+}
+
+}  // namespace dgpu
