@@ -46,8 +46,21 @@ struct dgpu_ctx {
     cx<float> *d_iptwf = nullptr;
     int ld = 0;      // row length of the intermediates (>= mk, multiple of 8)
     int xchunk = 16; // planes per block of the fused x kernel
-    // mixed path: FP32 transforms, intermediates and nonlinear terms
+    // mixed paths: FP32 transforms, intermediates and nonlinear terms
     int mixed = 0;
+    // DGPU_MIXED: FP32 working state [slot][j][K][I] (same slots as d_state's
+    // first nstate), FP64 side columns of the low modes [slot][j][l] (state
+    // slots, then the solves' scratch), the low-mode map and FP32 Poisson
+    // factors.  work_dirty: the working copy is newer than d_state;
+    // master_dirty: d_state is newer than the working copy.
+    bool f32state = false;
+    int nstate = 0;
+    cx<float> *d_state32 = nullptr;
+    cx<double> *d_side = nullptr;
+    int *d_map = nullptr;
+    int nside = 0;
+    float *d_pmultf = nullptr, *d_pinvf = nullptr;
+    bool work_dirty = false, master_dirty = true;
     cx<float> *d_ztf = nullptr, *d_wtf = nullptr, *d_nlf = nullptr;
     cx<float> *nlf[3 + DGPU_MAX_TH] = {};
     cx<float> **d_nlptrf = nullptr;
@@ -400,7 +413,8 @@ int32_t dgpu_create(const dgpu_params *p, dgpu_ctx **out)
     *out = nullptr;
     if (p->nx < 4 || p->nz < 4 || p->ny < 4 || (p->nx & 1) || (p->nz & 1)) return DGPU_E_ARG;
     if (p->n_th < 0 || p->n_th > 1) return DGPU_E_ARG;  // one active scalar supported by the fused kernels
-    if (p->precision != DGPU_FP64 && p->precision != DGPU_MIXED) return DGPU_E_ARG;
+    if (p->precision != DGPU_FP64 && p->precision != DGPU_MIXED && p->precision != DGPU_MIXED_F64STATE)
+        return DGPU_E_ARG;
     if (!p->gy) return DGPU_E_ARG;
     for (int code : {p->u_bc_ymin, p->w_bc_ymin, p->u_bc_ymax, p->w_bc_ymax})
         if (code != 0 && code != 1) return DGPU_E_ARG;
@@ -413,7 +427,8 @@ int32_t dgpu_create(const dgpu_params *p, dgpu_ctx **out)
     if (!c) return DGPU_E_NOMEM;
     c->p = *p;
     c->p.gy = c->p.gyf = c->p.dy = c->p.dyf = c->p.kx = c->p.kz = c->p.kx2 = c->p.kz2 = nullptr;
-    c->mixed = (p->precision == DGPU_MIXED);
+    c->mixed = (p->precision != DGPU_FP64);
+    c->f32state = (p->precision == DGPU_MIXED);
     dgpu::Geo &g = c->g;
     g.nx = p->nx; g.ny = p->ny; g.nz = p->nz; g.nth = p->n_th;
     g.nkx = p->nx / 3; g.nkz = p->nz / 3; g.tnkz = 2 * g.nkz;
@@ -475,6 +490,13 @@ int32_t dgpu_create(const dgpu_params *p, dgpu_ctx **out)
             return DGPU_E_ARG;
         }
         if (xc && std::atoi(xc) > 0) c->xchunk = std::atoi(xc);
+        const char *sw = std::getenv("DGPU_XSWZ");  // 0 turns the row swizzle off
+        c->ipx.swz = (sw && std::atoi(sw) == 0) ? 0u : dgpu::ip_pick_swizzle(c->ipx);
+        // the z kernels' column batches: W = the largest power of two with W*T <= 256
+        int zw = 1;
+        while (2 * zw * c->ipz.t <= 256) zw *= 2;
+        const char *zs = std::getenv("DGPU_ZSWZ");  // 0 turns the column-batch swizzle off
+        c->ipz.cswz = (zs && std::atoi(zs) == 0) ? 0 : dgpu::ip_pick_cswz(c->ipz, zw);
         c->ld = (g.mk + 7) / 8 * 8;
     }
     int rc = 0;
@@ -578,6 +600,31 @@ int32_t dgpu_create(const dgpu_params *p, dgpu_ctx **out)
             hipSuccess)
             return bail(DGPU_E_HIP);
     }
+    if (c->f32state) {
+        // FP32 working state, FP64 side columns of the low modes, FP32 factors
+        c->nstate = 2 * nin + 1;
+        std::vector<int> map;
+        const int kl = 2;
+        for (int K = 0; K <= g.tnkz; ++K) {
+            const int kz = K <= g.nkz ? K : K - (g.tnkz + 1);
+            if (std::abs(kz) > kl) continue;
+            for (int I = 0; I <= std::min(kl, g.nkx); ++I) map.push_back(K * g.mk + I);
+        }
+        c->nside = (int)map.size();
+        const long side_fs = (long)c->nside * (NY + 2);
+        if (hipMalloc(&c->d_state32, sizeof(cx<float>) * c->fs * c->nstate) != hipSuccess ||
+            hipMalloc(&c->d_side, sizeof(cx<double>) * side_fs * (c->nstate + nin)) != hipSuccess ||
+            hipMalloc(&c->d_map, sizeof(int) * map.size()) != hipSuccess ||
+            hipMalloc(&c->d_pmultf, sizeof(float) * c->fs) != hipSuccess ||
+            hipMalloc(&c->d_pinvf, sizeof(float) * c->fs) != hipSuccess)
+            return bail(DGPU_E_NOMEM);
+        if (hipMemcpy(c->d_map, map.data(), sizeof(int) * map.size(), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemset(c->d_side, 0, sizeof(cx<double>) * side_fs * (c->nstate + nin)) != hipSuccess)
+            return bail(DGPU_E_HIP);
+        dgpu::launch_convert_real(c->stream, c->d_pmult, c->d_pmultf, c->fs);
+        dgpu::launch_convert_real(c->stream, c->d_pinv, c->d_pinvf, c->fs);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess) return bail(DGPU_E_HIP);
+    }
     // CN tables
     c->h_cn.assign(3L * nsys_of(c) * cn_block(c), 0.0);
     if (hipMalloc(&c->d_cn, sizeof(double) * c->h_cn.size()) != hipSuccess) return bail(DGPU_E_NOMEM);
@@ -597,6 +644,32 @@ int32_t dgpu_create(const dgpu_params *p, dgpu_ctx **out)
     return DGPU_OK;
 }
 
+// DGPU_MIXED: bring the FP32 working copy (and the FP64 low-mode columns)
+// up to date with d_state, or d_state up to date with them.
+static int to_work(dgpu_ctx *c)
+{
+    if (!c->f32state || !c->master_dirty) return DGPU_OK;
+    const dgpu::Geo &g = c->g;
+    dgpu::launch_convert<float, double>(c->stream, c->d_state, c->d_state32, c->fs * c->nstate);
+    dgpu::launch_side_gather<double, double>(c->stream, c->d_state, c->fs, g.ps, c->d_side,
+                                             (long)c->nside * (g.ny + 2), c->d_map, c->nside, g.ny + 2, c->nstate);
+    if (int rc = check_launch(c, "to_work")) return rc;
+    c->master_dirty = false;
+    return DGPU_OK;
+}
+
+static int to_master(dgpu_ctx *c)
+{
+    if (!c->f32state || !c->work_dirty) return DGPU_OK;
+    const dgpu::Geo &g = c->g;
+    dgpu::launch_convert<double, float>(c->stream, c->d_state32, c->d_state, c->fs * c->nstate);
+    dgpu::launch_side_scatter<double, double>(c->stream, c->d_side, (long)c->nside * (g.ny + 2), c->d_state, c->fs,
+                                              g.ps, c->d_map, c->nside, g.ny + 2, c->nstate);
+    if (int rc = check_launch(c, "to_master")) return rc;
+    c->work_dirty = false;
+    return DGPU_OK;
+}
+
 int32_t dgpu_destroy(dgpu_ctx *c)
 {
     if (!c) return DGPU_OK;
@@ -607,7 +680,8 @@ int32_t dgpu_destroy(dgpu_ctx *c)
     for (auto e : c->event_pool) hipEventDestroy(e);
     void *bufs[] = {c->d_vec, c->d_tw, c->d_state, c->d_pmult, c->d_pinv, c->d_rd, c->d_nlptr, c->d_zt, c->d_wt,
                     c->d_fa, c->d_fb, c->d_gyf, c->d_iptw, c->d_iptwf, c->d_ztf, c->d_wtf, c->d_nlf, c->d_nlptrf,
-                    c->d_fort, c->d_cn, c->d_codes, c->d_c1};
+                    c->d_fort, c->d_cn, c->d_codes, c->d_c1, c->d_state32, c->d_side, c->d_map,
+                    c->d_pmultf, c->d_pinvf};
     for (void *b : bufs)
         if (b) hipFree(b);
     if (c->stream && c->own_stream) hipStreamDestroy(c->stream);
@@ -628,6 +702,8 @@ int32_t dgpu_set_field(dgpu_ctx *c, int32_t field, const double *a)
     cx<double> *dst = field_ptr(c, field);
     if (!dst) return fail(c, DGPU_E_ARG, "unknown field id");
     if (int rc = ensure_fort(c)) return rc;
+    if (int rc = to_master(c)) return rc;
+    c->master_dirty = true;
     const size_t n = (size_t)(c->g.nx + 2) * (c->g.nz + 2) * (c->g.ny + 2);
     HIPCHK(c, hipMemcpyAsync(c->d_fort, a, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
     dgpu::launch_unpack_fortran(c->stream, c->g, c->d_fort, dst, 0, c->g.ny + 2);
@@ -642,6 +718,7 @@ int32_t dgpu_get_field(dgpu_ctx *c, int32_t field, double *a)
     cx<double> *src = field_ptr(c, field);
     if (!src) return fail(c, DGPU_E_ARG, "unknown field id");
     if (int rc = ensure_fort(c)) return rc;
+    if (int rc = to_master(c)) return rc;
     const dgpu::Geo &g = c->g;
     const long plane = (long)(g.nx / 2 + 1) * (g.nz + 2);
     hipLaunchKernelGGL(k_pack_fortran, dim3((unsigned)((plane + 255) / 256), g.ny + 2), dim3(256), 0, c->stream, src,
@@ -675,6 +752,54 @@ static void fill_ptrs(dgpu_ctx *c, dgpu::CnPtrs<double, NLT> &P, cx<NLT> *const 
         P.gs[q] = c->nl[map[q]];
     }
 }
+
+// DGPU_MIXED: the FP32 working state; the nonlinear terms (FP32) double as
+// the solves' scratch.
+static void fill_ptrs32(dgpu_ctx *c, dgpu::CnPtrs<float, float> &P)
+{
+    std::memset(&P, 0, sizeof(P));
+    cx<float> *w = c->d_state32;
+    const long fs = c->fs;
+    const int nin = 3 + c->g.nth;
+    P.cu1 = w;
+    P.cu2 = w + fs;
+    P.cu3 = w + 2 * fs;
+    P.cth = c->g.nth ? w + 3 * fs : nullptr;
+    P.cp = w + nin * fs;
+    P.cf1 = w + (nin + 1) * fs;
+    P.cf2 = w + (nin + 2) * fs;
+    P.cf3 = w + (nin + 3) * fs;
+    P.cfth = c->g.nth ? w + (nin + 4) * fs : nullptr;
+    const int map[4] = {0, 2, 1, 3};
+    for (int q = 0; q < nin; ++q) {
+        P.nl[q] = c->nlf[map[q]];
+        P.gs[q] = c->nlf[map[q]];
+    }
+}
+
+// DGPU_MIXED: the FP64 side columns of the low modes (same slot order), the
+// FP32 nonlinear terms read at the modes' own indices.
+static void fill_ptrs_side(dgpu_ctx *c, dgpu::CnPtrs<double, float> &P)
+{
+    std::memset(&P, 0, sizeof(P));
+    cx<double> *w = c->d_side;
+    const long fs = (long)c->nside * (c->g.ny + 2);
+    const int nin = 3 + c->g.nth;
+    P.cu1 = w;
+    P.cu2 = w + fs;
+    P.cu3 = w + 2 * fs;
+    P.cth = c->g.nth ? w + 3 * fs : nullptr;
+    P.cp = w + nin * fs;
+    P.cf1 = w + (nin + 1) * fs;
+    P.cf2 = w + (nin + 2) * fs;
+    P.cf3 = w + (nin + 3) * fs;
+    P.cfth = c->g.nth ? w + (nin + 4) * fs : nullptr;
+    const int map[4] = {0, 2, 1, 3};
+    for (int q = 0; q < nin; ++q) {
+        P.nl[q] = c->nlf[map[q]];
+        P.gs[q] = w + (c->nstate + q) * fs;
+    }
+}
 }  // extern "C++"
 
 static void fill_rdargs(dgpu_ctx *c, dgpu::RdArgs &R, int update_cp, double temp4)
@@ -690,6 +815,10 @@ static void fill_rdargs(dgpu_ctx *c, dgpu::RdArgs &R, int update_cp, double temp
     R.rd = c->d_rd;
     R.kx = c->d_kx;
     R.kz = c->d_kz;
+    R.pmultf = c->d_pmultf;
+    R.pinvf = c->d_pinvf;
+    R.map = nullptr;
+    R.fac_ps = c->g.ps;
 }
 
 // Enqueues the five kernels of one substep on the context's stream.
@@ -706,8 +835,12 @@ static int rk_enqueue(dgpu_ctx *c, int rk_step, double time, double temp1, doubl
         const cx<float> *twx = c->d_iptwf, *twz = c->d_iptwf + c->ipx.twn;
         {
             TimerScope t(c, 0);
-            dgpu::launch_zinv_ip<float, double>(c->stream, g, c->ipz, twz, c->cu[0], c->fs, g.ps, c->d_ztf, c->zfs,
-                                                zps, c->ld, 0, g.ny + 2, nin);
+            if (c->f32state)
+                dgpu::launch_zinv_ip<float, float>(c->stream, g, c->ipz, twz, c->d_state32, c->fs, g.ps, c->d_ztf,
+                                                   c->zfs, zps, c->ld, 0, g.ny + 2, nin);
+            else
+                dgpu::launch_zinv_ip<float, double>(c->stream, g, c->ipz, twz, c->cu[0], c->fs, g.ps, c->d_ztf,
+                                                    c->zfs, zps, c->ld, 0, g.ny + 2, nin);
         }
         if (int rc = check_launch(c, "zinv")) return rc;
         {
@@ -765,9 +898,44 @@ static int rk_enqueue(dgpu_ctx *c, int rk_step, double time, double temp1, doubl
     A.kz = c->d_kz;
     A.kx2 = c->d_kx2;
     A.kz2 = c->d_kz2;
+    A.map = nullptr;
+    A.nl_ps = g.ps;
     dgpu::RdArgs R;
     fill_rdargs(c, R, 1, temp4);
-    if (c->mixed) {
+    if (c->f32state) {
+        // FP32 solves on every mode, then the low modes again in FP64 on their
+        // side columns, copied into the working state for the transforms.
+        dgpu::CnPtrs<float, float> P;
+        fill_ptrs32(c, P);
+        dgpu::CnPtrs<double, float> Ps;
+        fill_ptrs_side(c, Ps);
+        dgpu::CnArgs As = A;
+        As.g.ps = c->nside;
+        As.map = c->d_map;
+        dgpu::RdArgs Rs = R;
+        Rs.g.ps = c->nside;
+        Rs.map = c->d_map;
+        const long side_fs = (long)c->nside * (g.ny + 2);
+        {
+            TimerScope t(c, 3);
+            dgpu::launch_cn<float, float>(c->stream, A, P);
+            dgpu::launch_cn<double, float>(c->stream, As, Ps);
+            dgpu::launch_side_scatter<double, float>(c->stream, c->d_side, side_fs, c->d_state32, c->fs, g.ps,
+                                                     c->d_map, c->nside, g.ny + 2, nin);
+        }
+        if (int rc = check_launch(c, "cn")) return rc;
+        {
+            TimerScope t(c, 4);
+            dgpu::launch_remdiv<float, float>(c->stream, R, P);
+            dgpu::launch_remdiv<double, float>(c->stream, Rs, Ps);
+            // CU1-3 and CP of the low modes
+            dgpu::launch_side_scatter<double, float>(c->stream, c->d_side, side_fs, c->d_state32, c->fs, g.ps,
+                                                     c->d_map, c->nside, g.ny + 2, 3);
+            dgpu::launch_side_scatter<double, float>(c->stream, c->d_side + nin * side_fs, side_fs,
+                                                     c->d_state32 + nin * c->fs, c->fs, g.ps, c->d_map, c->nside,
+                                                     g.ny + 2, 1);
+        }
+    } else if (c->mixed) {
         dgpu::CnPtrs<double, float> P;
         fill_ptrs<float>(c, P, c->nlf);
         {
@@ -807,6 +975,8 @@ int32_t dgpu_rk_chan_1(dgpu_ctx *c, int32_t rk_step, double time, const double *
     const double temp4 = h_bar[k];
     const double temp5 = beta_bar[k] * h_bar[k];
     if (int rc = ensure_cn(c, k, temp1, temp4)) return rc;
+    if (int rc = to_work(c)) return rc;
+    c->work_dirty = c->f32state;
     // One captured graph per RK stage.  Not used while kernels are timed one
     // by one, or when the forcing depends on TIME (F_TYPE = 2).
     const bool graphable = c->use_graphs && !c->timing && c->p.f_type != 2;
@@ -845,6 +1015,8 @@ int32_t dgpu_set_graphs(dgpu_ctx *c, int32_t enable)
 int32_t dgpu_rem_div_chan(dgpu_ctx *c)
 {
     if (!c) return DGPU_E_ARG;
+    if (int rc = to_master(c)) return rc;
+    c->master_dirty = true;
     dgpu::CnPtrs<double, double> P;
     fill_ptrs<double>(c, P, c->nl);
     dgpu::RdArgs R;
@@ -856,6 +1028,8 @@ int32_t dgpu_rem_div_chan(dgpu_ctx *c)
 int32_t dgpu_apply_bc_vel(dgpu_ctx *c)
 {
     if (!c) return DGPU_E_ARG;
+    if (int rc = to_master(c)) return rc;
+    c->master_dirty = true;
     dgpu::launch_apply_bc_vel(c->stream, c->g, c->cu[0], c->cu[1], c->cu[2], c->d_codes, c->d_c1, c->d_dy, c->d_dyf);
     return check_launch(c, "apply_bc_vel");
 }
@@ -932,6 +1106,8 @@ static int fort_to_fourier(dgpu_ctx *c, double *buf, int j0, int j1)
 int32_t dgpu_poisson_p_chan(dgpu_ctx *c)
 {
     if (!c) return DGPU_E_ARG;
+    if (int rc = to_master(c)) return rc;
+    c->master_dirty = true;
     const dgpu::Geo &g = c->g;
     if (int rc = ensure_fort(c)) return rc;
     const size_t n = (size_t)(g.nx + 2) * (g.nz + 2) * (g.ny + 2);
@@ -1061,6 +1237,15 @@ int32_t dgpu_ensemble_rk_chan_1(dgpu_ensemble *e, int32_t rk_step, const double 
         }
         key.insert(key.end(), {temp1, temp3, temp4, temp5});
         if (c->p.f_type == 2 || c->timing) graphable = false;
+        if (c->f32state && c->master_dirty) {
+            // the graph runs on the ensemble stream: finish the copy first
+            if (int rc = to_work(c)) {
+                e->err = c->err;
+                return rc;
+            }
+            if (hipStreamSynchronize(c->stream) != hipSuccess) return DGPU_E_HIP;
+        }
+        c->work_dirty = c->f32state;
     }
     auto enqueue_all = [&](bool joined) -> int {
         for (int i = 0; i < n; ++i) {

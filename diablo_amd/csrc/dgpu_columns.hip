@@ -13,6 +13,8 @@
 // k_remdiv: REM_DIV_CHAN (for/channel.f:1103-1214) per mode with the Poisson
 //   factorisation precomputed once (it depends only on kx^2+kz^2 and the grid),
 //   then the CP update (for/channel.f:1059-1065).
+#include <algorithm>
+
 #include "dgpu_kernels.hpp"
 
 namespace dgpu {
@@ -24,6 +26,32 @@ template <typename T>
 __device__ __forceinline__ cx<T> neg_scale(T a, cx<T> x) { return make_cx<T>(-(a * x.re), -(a * x.im)); }
 
 __device__ __forceinline__ int wave_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// A sweep over planes j = j0, j0+dir, ... (n planes) whose per-plane inputs
+// do not depend on the recurrence: the inputs of plane j are loaded D planes
+// ahead into a ring of registers (ld(j) must be safe for any j in range, the
+// caller clamps), so D planes of loads are in flight while the recurrence
+// runs.  body(j, in) consumes them.  The loop body is unrolled D times so the
+// ring is indexed statically.
+template <int D, class Ld, class Body>
+__device__ __forceinline__ void piped_sweep(int j0, int dir, int n, Ld &&ld, Body &&body)
+{
+    using In = decltype(ld(0));
+    In ring[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) ring[d] = ld(j0 + dir * min(d, n - 1));
+    for (int jb = 0; jb < n; jb += D) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const int jj = jb + d;
+            if (jj < n) {
+                const In cur = ring[d];
+                ring[d] = ld(j0 + dir * min(jj + D, n - 1));
+                body(j0 + dir * jj, cur);
+            }
+        }
+    }
+}
 
 // ---------------------------------------------------------------------------
 // k_cn
@@ -40,8 +68,10 @@ __global__ void __launch_bounds__(256) k_cn(CnArgs A, CnPtrs<T, NLT> P)
     const cx<T> zero = make_cx<T>(0, 0);
     const double *tab = A.tab + (long)q * (NY + 2) * kRow;
     int s = 0, i = 0;
-    if (live) { s = (int)(m / g.mk); i = (int)(m - (long)s * g.mk); }
+    const long gm = (live && A.map) ? (long)A.map[m] : m;  // mode of this lane
+    if (live) { s = (int)(gm / g.mk); i = (int)(gm - (long)s * g.mk); }
     const bool is00 = live && (i == 0 && s == 0);
+    const long nps = A.nl_ps;
     const T kx = live ? T(A.kx[i]) : T(0), kz = live ? T(A.kz[s]) : T(0);
     const T k2 = live ? T(A.kx2[i]) + T(A.kz2[s]) : T(0);
     const T pw = (A.beta == 1.0) ? k2 : T(pow(double(k2), A.beta));
@@ -55,98 +85,109 @@ __global__ void __launch_bounds__(256) k_cn(CnArgs A, CnPtrs<T, NLT> P)
     const T visc = (q == SYS_TH0) ? T(A.nupr) * pw : T(A.nu) * pw;
 
     // ---------------- sweep A ----------------
+    // Inputs of plane j are loaded four planes ahead (piped_sweep); every
+    // load is unconditional at a valid plane and selected afterwards.
     if (live) {
-        cx<T> fm = zero, f0 = fld[m], fp;          // this system's field at j-1, j, j+1
-        cx<T> thm = zero;                            // CTH at j-1 (U2 buoyancy)
-        cx<T> cpm = zero;                            // CP at j-1 (U2 pressure)
+        struct InA {
+            cx<T> fp, nl, cf, aux1, aux2, cp;
+        };
+        cx<T> fm = zero, f0 = fld[m];  // this system's field at j-1, j (j+1 comes in)
+        cx<T> thm = zero;              // CTH at j-1 (U2 buoyancy)
+        cx<T> cpm = zero;              // CP at j-1 (U2 pressure)
         cx<T> gprev = zero;
-        for (int j = 0; j <= NY + 1; ++j) {
-            const long o = m + (long)j * ps;
-            const double *t = tab + (long)j * kRow;
-            const T mult = T(t[T_MULT]);
-            const bool isbc = t[T_ISBC] != 0.0;
-            const bool cfr = t[T_CFR] != 0.0;
-            fp = (j <= NY) ? fld[o + ps] : zero;
-            cx<T> nlv = zero, cfo = zero, cp0 = zero, aux1 = zero, aux2 = zero;
-            if (cfr || !isbc) {
-                nlv = make_cx<T>(T(nl[o].re), T(nl[o].im));
-                if (A.rk_gt1) cfo = cf[o];
-            }
-            // explicit-term inputs
-            if (q == SYS_U1 || q == SYS_U3) {
-                aux1 = (q == SYS_U1) ? P.cu3[o] : P.cu1[o];   // Coriolis partner
-                if (NTH) aux2 = P.cth[o];
-                cp0 = P.cp[o];
-            } else if (q == SYS_U2) {
-                if (NTH) aux2 = P.cth[o];
-                cp0 = P.cp[o];
-            } else {
-                aux1 = P.cu1[o];
-                aux2 = P.cu3[o];
-            }
-            // new explicit term CF (for/channel.f:175-290 + nonlinear terms)
-            cx<T> cfn = neg_scale(visc, f0);
-            if (q == SYS_U1) {
-                cfn = fma_c(T(A.i_ro), aux1, cfn);
-                if (NTH) cfn = fma_c(T(A.rgx), aux2, cfn);
-            } else if (q == SYS_U3) {
-                cfn = fma_c(-T(A.i_ro), aux1, cfn);
-                if (NTH) cfn = fma_c(T(A.rgz), aux2, cfn);
-            } else if (q == SYS_U2) {
-                if (NTH && !is00) cfn = cfn + make_cx<T>(T(t[T_B0]) * aux2.re + T(t[T_BM]) * thm.re,
-                                                          T(t[T_B0]) * aux2.im + T(t[T_BM]) * thm.im);
-            } else {
-                cfn = cfn - make_cx<T>(aux1.re * T(A.drhodx) + aux2.re * T(A.drhodz),
-                                       aux1.im * T(A.drhodx) + aux2.im * T(A.drhodz));
-            }
-            cfn = cfn + nlv;
-            cx<T> vec;
-            if (isbc) {
-                vec = is00 ? make_cx<T>(T(t[T_BC00]), 0) : zero;
-            } else {
-                cx<T> cr = f0;
-                if (A.rk_gt1) cr = fma_c(temp3, cfo, cr);
+        const cx<T> *a1p = (q == SYS_U1) ? P.cu3 : P.cu1;   // Coriolis partner / TH advection
+        const cx<T> *a2p = (q == SYS_TH0) ? P.cu3 : P.cth;  // buoyancy / TH advection
+        const bool has1 = (q != SYS_U2);
+        const bool has2 = (q == SYS_TH0) || NTH;
+        const bool hascp = (q != SYS_TH0);
+        piped_sweep<4>(
+            0, +1, NY + 2,
+            [&](int j) {
+                const long o = m + (long)j * ps;
+                InA in;
+                in.fp = fld[o + (j <= NY ? ps : 0)];
+                const cx<NLT> nv = nl[gm + (long)j * nps];
+                in.nl = make_cx<T>(T(nv.re), T(nv.im));
+                in.cf = A.rk_gt1 ? cf[o] : zero;
+                in.aux1 = has1 ? a1p[o] : zero;
+                in.aux2 = has2 ? a2p[o] : zero;
+                in.cp = hascp ? P.cp[o] : zero;
+                return in;
+            },
+            [&](int j, const InA &in) {
+                const long o = m + (long)j * ps;
+                const double *t = tab + (long)j * kRow;
+                const T mult = T(t[T_MULT]);
+                const bool isbc = t[T_ISBC] != 0.0;
+                const bool cfr = t[T_CFR] != 0.0;
+                const cx<T> fp = (j <= NY) ? in.fp : zero;
+                const bool use = cfr || !isbc;
+                const cx<T> nlv = use ? in.nl : zero;
+                const cx<T> cfo = use ? in.cf : zero;
+                const cx<T> aux1 = in.aux1, aux2 = in.aux2, cp0 = in.cp;
+                // new explicit term CF (for/channel.f:175-290 + nonlinear terms)
+                cx<T> cfn = neg_scale(visc, f0);
                 if (q == SYS_U1) {
-                    cr = cr - make_cx<T>(-temp4 * (kx * cp0.im), temp4 * (kx * cp0.re));
-                    if (is00 && (A.f_type == 1 || A.f_type == 2)) cr.re = cr.re - temp4 * T(A.force00);
+                    cfn = fma_c(T(A.i_ro), aux1, cfn);
+                    if (NTH) cfn = fma_c(T(A.rgx), aux2, cfn);
                 } else if (q == SYS_U3) {
-                    cr = cr - make_cx<T>(-temp4 * (kz * cp0.im), temp4 * (kz * cp0.re));
+                    cfn = fma_c(-T(A.i_ro), aux1, cfn);
+                    if (NTH) cfn = fma_c(T(A.rgz), aux2, cfn);
                 } else if (q == SYS_U2) {
-                    const T w = T(t[T_CPW]);
-                    cr = cr - make_cx<T>(w * (cp0.re - cpm.re), w * (cp0.im - cpm.im));
+                    if (NTH && !is00) cfn = cfn + make_cx<T>(T(t[T_B0]) * aux2.re + T(t[T_BM]) * thm.re,
+                                                              T(t[T_B0]) * aux2.im + T(t[T_BM]) * thm.im);
+                } else {
+                    cfn = cfn - make_cx<T>(aux1.re * T(A.drhodx) + aux2.re * T(A.drhodz),
+                                           aux1.im * T(A.drhodx) + aux2.im * T(A.drhodz));
                 }
-                cr = fma_c(temp5, cfn, cr);
-                const T wp = T(t[T_WP]), w0 = T(t[T_W0]), wm = T(t[T_WM]);
-                vec = make_cx<T>(cr.re + (wp * fp.re + w0 * f0.re + wm * fm.re),
-                                 cr.im + (wp * fp.im + w0 * f0.im + wm * fm.im));
-            }
-            const cx<T> gj = (j == 0) ? vec : fma_c(mult, gprev, vec);
-            if (cfr) cf[o] = cfn;
-            gs[o] = gj;
-            gprev = gj;
-            fm = f0;
-            f0 = fp;
-            thm = aux2;
-            cpm = cp0;
-        }
+                cfn = cfn + nlv;
+                cx<T> vec;
+                if (isbc) {
+                    vec = is00 ? make_cx<T>(T(t[T_BC00]), 0) : zero;
+                } else {
+                    cx<T> cr = f0;
+                    if (A.rk_gt1) cr = fma_c(temp3, cfo, cr);
+                    if (q == SYS_U1) {
+                        cr = cr - make_cx<T>(-temp4 * (kx * cp0.im), temp4 * (kx * cp0.re));
+                        if (is00 && (A.f_type == 1 || A.f_type == 2)) cr.re = cr.re - temp4 * T(A.force00);
+                    } else if (q == SYS_U3) {
+                        cr = cr - make_cx<T>(-temp4 * (kz * cp0.im), temp4 * (kz * cp0.re));
+                    } else if (q == SYS_U2) {
+                        const T w = T(t[T_CPW]);
+                        cr = cr - make_cx<T>(w * (cp0.re - cpm.re), w * (cp0.im - cpm.im));
+                    }
+                    cr = fma_c(temp5, cfn, cr);
+                    const T wp = T(t[T_WP]), w0 = T(t[T_W0]), wm = T(t[T_WM]);
+                    vec = make_cx<T>(cr.re + (wp * fp.re + w0 * f0.re + wm * fm.re),
+                                     cr.im + (wp * fp.im + w0 * f0.im + wm * fm.im));
+                }
+                const cx<T> gj = (j == 0) ? vec : fma_c(mult, gprev, vec);
+                if (cfr) cf[o] = cfn;
+                gs[o] = gj;
+                gprev = gj;
+                fm = f0;
+                f0 = fp;
+                thm = aux2;
+                cpm = cp0;
+            });
     }
     __syncthreads();
     // ---------------- sweep B ----------------
     if (live) {
         cx<T> x = zero;
-        for (int j = NY + 1; j >= 0; --j) {
-            const long o = m + (long)j * ps;
-            const double *t = tab + (long)j * kRow;
-            const cx<T> gj = gs[o];
-            const T ip = T(t[T_INVPIV]);
-            if (j == NY + 1) {
-                x = make_cx<T>(gj.re * ip, gj.im * ip);
-            } else {
-                const T u = T(t[T_UP]);
-                x = make_cx<T>((gj.re - u * x.re) * ip, (gj.im - u * x.im) * ip);
-            }
-            if (t[T_CPY] != 0.0) fld[o] = x;
-        }
+        piped_sweep<8>(
+            NY + 1, -1, NY + 2, [&](int j) { return gs[m + (long)j * ps]; },
+            [&](int j, const cx<T> &gj) {
+                const double *t = tab + (long)j * kRow;
+                const T ip = T(t[T_INVPIV]);
+                if (j == NY + 1) {
+                    x = make_cx<T>(gj.re * ip, gj.im * ip);
+                } else {
+                    const T u = T(t[T_UP]);
+                    x = make_cx<T>((gj.re - u * x.re) * ip, (gj.im - u * x.im) * ip);
+                }
+                if (t[T_CPY] != 0.0) fld[m + (long)j * ps] = x;
+            });
     }
 }
 
@@ -196,66 +237,104 @@ __global__ void __launch_bounds__(64) k_remdiv(RdArgs A, CnPtrs<T, NLT> P)
     const Geo &g = A.g;
     const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= g.ps) return;
-    const int s = (int)(m / g.mk), i = (int)(m - (long)s * g.mk);
+    const long gm = A.map ? (long)A.map[m] : m;  // mode of this lane
+    const int s = (int)(gm / g.mk), i = (int)(gm - (long)s * g.mk);
     const bool is00 = (i == 0 && s == 0);
     const int NY = g.ny;
-    const long ps = g.ps;
+    const long ps = g.ps, fps = A.fac_ps;
     const T kx = T(A.kx[i]), kz = T(A.kz[s]);
+    // Poisson factors of this mode at plane j, at the state's precision
+    auto pm = [&](int j) -> T {
+        if constexpr (sizeof(T) == 4) return A.pmultf[gm + (long)j * fps];
+        else return T(A.pmult[gm + (long)j * fps]);
+    };
+    auto pv = [&](int j) -> T {
+        if constexpr (sizeof(T) == 4) return A.pinvf[gm + (long)j * fps];
+        else return T(A.pinv[gm + (long)j * fps]);
+    };
     const cx<T> zero = make_cx<T>(0, 0);
     cx<T> *scr = P.gs[0];  // forward-eliminated right-hand side
     // sweep C: rows 2..NY (row 1 is phi(1) = 0 / phi(1) - phi(2) = 0, RHS 0)
     {
+        struct InC {
+            cx<T> u1, u3, c2p;
+            T mu, idf;
+        };
         cx<T> gprev = zero;
         cx<T> c2 = P.cu2[m + 2 * ps];
         scr[m + ps] = zero;
-        for (int j = 2; j <= NY; ++j) {
-            const long o = m + (long)j * ps;
-            const cx<T> c2p = P.cu2[o + ps];
-            cx<T> vec = zero;
-            if (j < NY) {
-                const cx<T> u1 = P.cu1[o], u3 = P.cu3[o];
-                const T idf = T(A.rd[(long)j * 4 + 1]);
-                // (ikx CU1 + (CU2(j+1)-CU2(j))/DYF(j)) + ikz CU3
-                vec = make_cx<T>((-kx * u1.im + (c2p.re - c2.re) * idf) + (-kz * u3.im),
-                                 (kx * u1.re + (c2p.im - c2.im) * idf) + (kz * u3.re));
-            }
-            const T mu = T(A.pmult[o]);
-            const cx<T> gj = fma_c(mu, gprev, vec);
-            scr[o] = gj;
-            gprev = gj;
-            c2 = c2p;
-        }
+        piped_sweep<4>(
+            2, +1, NY - 1,
+            [&](int j) {
+                const long o = m + (long)j * ps;
+                InC in;
+                in.u1 = P.cu1[o];
+                in.u3 = P.cu3[o];
+                in.c2p = P.cu2[o + ps];
+                in.mu = pm(j);
+                in.idf = T(A.rd[(long)j * 4 + 1]);
+                return in;
+            },
+            [&](int j, const InC &in) {
+                const long o = m + (long)j * ps;
+                cx<T> vec = zero;
+                if (j < NY) {
+                    // (ikx CU1 + (CU2(j+1)-CU2(j))/DYF(j)) + ikz CU3
+                    vec = make_cx<T>((-kx * in.u1.im + (in.c2p.re - c2.re) * in.idf) + (-kz * in.u3.im),
+                                     (kx * in.u1.re + (in.c2p.im - c2.im) * in.idf) + (kz * in.u3.re));
+                }
+                const cx<T> gj = fma_c(in.mu, gprev, vec);
+                scr[o] = gj;
+                gprev = gj;
+                c2 = in.c2p;
+            });
     }
-    // sweep D: back substitution (j = NY..1), projection, CP update
+    // sweep D: back substitution (j = NY..1), projection, CP update.  Plane j
+    // rewrites CU1(j), CU3(j), CU2(j+1) and CP(j); each is read (ahead) only
+    // for that same plane, before its rewrite.
     {
+        struct InD {
+            cx<T> g, u1, u3, c2, cp;
+            T ip, U, idy;
+        };
         const T u1row = is00 ? T(0) : T(-1);
         const T itemp4 = T(A.inv_temp4);
+        const bool upd = A.update_cp;
         cx<T> phip = zero;
-        for (int j = NY; j >= 1; --j) {
-            const long o = m + (long)j * ps;
-            const cx<T> gj = scr[o];
-            const T ip = T(A.pinv[o]);
-            cx<T> phi;
-            if (j == NY) {
-                phi = make_cx<T>(gj.re * ip, gj.im * ip);
-            } else {
-                const T U = (j == 1) ? u1row : T(A.rd[(long)j * 4 + 0]);
-                phi = make_cx<T>((gj.re - U * phip.re) * ip, (gj.im - U * phip.im) * ip);
-            }
-            const cx<T> a = P.cu1[o], b = P.cu3[o];
-            P.cu1[o] = make_cx<T>(a.re + kx * phi.im, a.im - kx * phi.re);
-            P.cu3[o] = make_cx<T>(b.re + kz * phi.im, b.im - kz * phi.re);
-            if (j + 1 <= NY) {
-                const T idy = T(A.rd[(long)(j + 1) * 4 + 2]);
-                const cx<T> c = P.cu2[o + ps];
-                P.cu2[o + ps] = make_cx<T>(c.re - (phip.re - phi.re) * idy, c.im - (phip.im - phi.im) * idy);
-            }
-            if (A.update_cp && j >= A.jstart && j <= A.jend) {
-                const cx<T> p = P.cp[o];
-                P.cp[o] = make_cx<T>(p.re + phi.re * itemp4, p.im + phi.im * itemp4);
-            }
-            phip = phi;
-        }
+        piped_sweep<4>(
+            NY, -1, NY,
+            [&](int j) {
+                const long o = m + (long)j * ps;
+                InD in;
+                in.g = scr[o];
+                in.ip = pv(j);
+                in.U = (j == 1) ? u1row : T(A.rd[(long)j * 4 + 0]);
+                in.u1 = P.cu1[o];
+                in.u3 = P.cu3[o];
+                in.c2 = P.cu2[o + ps];  // plane NY+1 exists (unused for j = NY)
+                in.idy = T(A.rd[(long)(j + 1) * 4 + 2]);
+                in.cp = upd ? P.cp[o] : zero;
+                return in;
+            },
+            [&](int j, const InD &in) {
+                const long o = m + (long)j * ps;
+                cx<T> phi;
+                if (j == NY) {
+                    phi = make_cx<T>(in.g.re * in.ip, in.g.im * in.ip);
+                } else {
+                    phi = make_cx<T>((in.g.re - in.U * phip.re) * in.ip, (in.g.im - in.U * phip.im) * in.ip);
+                }
+                P.cu1[o] = make_cx<T>(in.u1.re + kx * phi.im, in.u1.im - kx * phi.re);
+                P.cu3[o] = make_cx<T>(in.u3.re + kz * phi.im, in.u3.im - kz * phi.re);
+                if (j + 1 <= NY) {
+                    P.cu2[o + ps] = make_cx<T>(in.c2.re - (phip.re - phi.re) * in.idy,
+                                               in.c2.im - (phip.im - phi.im) * in.idy);
+                }
+                if (upd && j >= A.jstart && j <= A.jend) {
+                    P.cp[o] = make_cx<T>(in.cp.re + phi.re * itemp4, in.cp.im + phi.im * itemp4);
+                }
+                phip = phi;
+            });
     }
 }
 
@@ -286,7 +365,85 @@ void launch_pois_factor(hipStream_t s, const Geo &g, const double *dy, const dou
     hipLaunchKernelGGL(k_pois_factor, dim3(blocks), dim3(256), 0, s, g, dy, dyf, kx2, kz2, pmult, pinv);
 }
 
+// ---------------------------------------------------------------------------
+// state copies of the mixed path
+// ---------------------------------------------------------------------------
+template <typename TO, typename FROM>
+__global__ void k_convert(const cx<FROM> *__restrict__ src, cx<TO> *__restrict__ dst, long n)
+{
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x)
+        dst[k] = make_cx<TO>(TO(src[k].re), TO(src[k].im));
+}
+
+__global__ void k_convert_real(const double *__restrict__ src, float *__restrict__ dst, long n)
+{
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x)
+        dst[k] = float(src[k]);
+}
+
+template <typename TS, typename TF, bool GATHER>
+__global__ void k_side(const cx<TF> *full_in, cx<TF> *full_out, long full_fs, long full_ps, const cx<TS> *side_in,
+                       cx<TS> *side_out, long side_fs, const int *__restrict__ map, int nside, int planes)
+{
+    const int f = blockIdx.y;
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < (long)nside * planes;
+         k += (long)gridDim.x * blockDim.x) {
+        const int j = (int)(k / nside), l = (int)(k - (long)j * nside);
+        const long fo = f * full_fs + (long)j * full_ps + map[l];
+        const long so = f * side_fs + (long)j * nside + l;
+        if (GATHER) {
+            const cx<TF> v = full_in[fo];
+            side_out[so] = make_cx<TS>(TS(v.re), TS(v.im));
+        } else {
+            const cx<TS> v = side_in[so];
+            full_out[fo] = make_cx<TF>(TF(v.re), TF(v.im));
+        }
+    }
+}
+
+static inline unsigned grid1(long n) { return (unsigned)std::min<long>((n + 255) / 256, 65536); }
+
+template <typename TO, typename FROM>
+void launch_convert(hipStream_t s, const cx<FROM> *src, cx<TO> *dst, long n)
+{
+    hipLaunchKernelGGL((k_convert<TO, FROM>), dim3(grid1(n)), dim3(256), 0, s, src, dst, n);
+}
+
+void launch_convert_real(hipStream_t s, const double *src, float *dst, long n)
+{
+    hipLaunchKernelGGL(k_convert_real, dim3(grid1(n)), dim3(256), 0, s, src, dst, n);
+}
+
+template <typename TS, typename TF>
+void launch_side_gather(hipStream_t s, const cx<TF> *full, long full_fs, long full_ps, cx<TS> *side, long side_fs,
+                        const int *map, int nside, int planes, int nf)
+{
+    const long n = (long)nside * planes;
+    hipLaunchKernelGGL((k_side<TS, TF, true>), dim3(grid1(n), nf), dim3(256), 0, s, full, nullptr, full_fs, full_ps,
+                       nullptr, side, side_fs, map, nside, planes);
+}
+
+template <typename TS, typename TF>
+void launch_side_scatter(hipStream_t s, const cx<TS> *side, long side_fs, cx<TF> *full, long full_fs, long full_ps,
+                         const int *map, int nside, int planes, int nf)
+{
+    const long n = (long)nside * planes;
+    hipLaunchKernelGGL((k_side<TS, TF, false>), dim3(grid1(n), nf), dim3(256), 0, s, nullptr, full, full_fs, full_ps,
+                       side, nullptr, side_fs, map, nside, planes);
+}
+
+template void launch_convert<float, double>(hipStream_t, const cx<double> *, cx<float> *, long);
+template void launch_convert<double, float>(hipStream_t, const cx<float> *, cx<double> *, long);
+template void launch_side_gather<double, double>(hipStream_t, const cx<double> *, long, long, cx<double> *, long,
+                                                 const int *, int, int, int);
+template void launch_side_scatter<double, double>(hipStream_t, const cx<double> *, long, cx<double> *, long, long,
+                                                  const int *, int, int, int);
+template void launch_side_scatter<double, float>(hipStream_t, const cx<double> *, long, cx<float> *, long, long,
+                                                 const int *, int, int, int);
+
 template void launch_cn<double, double>(hipStream_t, const CnArgs &, const CnPtrs<double, double> &);
+template void launch_cn<float, float>(hipStream_t, const CnArgs &, const CnPtrs<float, float> &);
+template void launch_remdiv<float, float>(hipStream_t, const RdArgs &, const CnPtrs<float, float> &);
 template void launch_cn<double, float>(hipStream_t, const CnArgs &, const CnPtrs<double, float> &);
 template void launch_remdiv<double, double>(hipStream_t, const RdArgs &, const CnPtrs<double, double> &);
 template void launch_remdiv<double, float>(hipStream_t, const RdArgs &, const CnPtrs<double, float> &);

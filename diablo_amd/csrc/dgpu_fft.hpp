@@ -35,8 +35,10 @@
 
 namespace dgpu {
 
+// Aligned to its size, so a value moves in one 16-byte (double) or 8-byte
+// (float) access to LDS and global memory rather than two halves.
 template <typename T>
-struct cx {
+struct alignas(2 * sizeof(T)) cx {
     T re, im;
 };
 
@@ -179,7 +181,25 @@ struct IpLen {
     int twn;                    // length of the whole twiddle table
     int e;                      // values per thread (multiple of every R_i)
     int t;                      // threads per transform = n / e
+    unsigned swz;               // LDS swizzle: 4-bit XOR mask of bits 0..3, per value of bits 3..5
+    int cswz;                   // column-batch swizzle: bit 0 of a position XORed with bit cswz (0: none)
 };
+
+// Slot of position pos in a column batch (lds[slot * W + column]): swapping
+// rows pairwise by a higher bit puts the two positions a 16-lane group reads
+// in different halves of the banks.
+DGPU_HD int ip_cswz(const IpLen &f, int pos)
+{
+    return f.cswz ? (pos ^ ((pos >> f.cswz) & 1)) : pos;
+}
+
+// LDS slot of position pos of a row: the low four bits are XORed with a mask
+// chosen by bits 3..5, which spreads the strided accesses of the late stages
+// over the banks (the host checks that the map is a bijection of 0..n-1).
+DGPU_HD int ip_swz(const IpLen &f, int pos)
+{
+    return pos ^ (int)((f.swz >> (((pos >> 3) & 7) << 2)) & 15u);
+}
 
 // Twiddle + butterfly of one group (values v[0..R-1] at positions
 // base + q L_i); tw points at the stage's table, k is the group offset.
@@ -263,15 +283,28 @@ DGPU_HD void with_radix(int R, F &&fn)
     }
 }
 
-// In-place stage i on x[pos * stride] (LDS).
+// In-place stage i on x[ip_cswz(pos) * stride] (LDS, the z kernels' column
+// batches).
 template <typename T, int E, int SIGN, bool DIF>
 DGPU_HD void ip_stage_mem(cx<T> *x, int stride, const IpLen &f, int i, const cx<T> *__restrict__ tw, int th)
 {
     with_radix<E>(f.radix[i], [&](auto rc) {
         constexpr int R = decltype(rc)::value;
         ip_stage<T, E, R, SIGN, DIF>(
-            f, i, tw, th, [&](int, int pos) { return x[pos * stride]; },
-            [&](int, int pos, cx<T> v) { x[pos * stride] = v; });
+            f, i, tw, th, [&](int, int pos) { return x[ip_cswz(f, pos) * stride]; },
+            [&](int, int pos, cx<T> v) { x[ip_cswz(f, pos) * stride] = v; });
+    });
+}
+
+// In-place stage i on one swizzled row x[ip_swz(pos)] (LDS).
+template <typename T, int E, int SIGN, bool DIF>
+DGPU_HD void ip_stage_row(cx<T> *x, const IpLen &f, int i, const cx<T> *__restrict__ tw, int th)
+{
+    with_radix<E>(f.radix[i], [&](auto rc) {
+        constexpr int R = decltype(rc)::value;
+        ip_stage<T, E, R, SIGN, DIF>(
+            f, i, tw, th, [&](int, int pos) { return x[ip_swz(f, pos)]; },
+            [&](int, int pos, cx<T> v) { x[ip_swz(f, pos)] = v; });
     });
 }
 

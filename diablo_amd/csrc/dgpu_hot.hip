@@ -71,10 +71,13 @@ __global__ void __launch_bounds__(256) k_zinv_ip(IpLen fz, const cx<T> *__restri
         ip_stage<T, E, R, +1, true>(
             fz, 0, tw, th,
             [&](int, int f) {
+                // unconditional load at a valid index, then select: keeps it
+                // one 16-byte access
                 const int s = slot_of_freq(f, nz, nkz, tnkz);
-                return (col && s >= 0) ? cvt_<T>(src[(long)s * mk + i]) : make_cx<T>(0, 0);
+                const cx<TIN> v = src[(long)max(s, 0) * mk + (col ? i : mk - 1)];
+                return (col && s >= 0) ? cvt_<T>(v) : make_cx<T>(0, 0);
             },
-            [&](int, int pos, cx<T> v) { x[pos * W] = v; });
+            [&](int, int pos, cx<T> v) { x[ip_cswz(fz, pos) * W] = v; });
     });
     __syncthreads();
     for (int s = 1; s < S - 1; ++s) {
@@ -84,7 +87,7 @@ __global__ void __launch_bounds__(256) k_zinv_ip(IpLen fz, const cx<T> *__restri
     with_radix<E>(fz.radix[S - 1], [&](auto rc) {
         constexpr int R = decltype(rc)::value;
         ip_stage<T, E, R, +1, true>(
-            fz, S - 1, tw, th, [&](int, int pos) { return x[pos * W]; },
+            fz, S - 1, tw, th, [&](int, int pos) { return x[ip_cswz(fz, pos) * W]; },
             [&](int, int pos, cx<T> v) {
                 if (col) dst[(long)pos * ld + i] = v;
             });
@@ -104,7 +107,9 @@ __global__ void __launch_bounds__(256) k_zinv_ip(IpLen fz, const cx<T> *__restri
 //   row 1 (U1U2@GY, d/dy U2U2) -> G2        row 4 (TH U1, d/dy TH U2) -> G4
 //   row 2 (U1U3, d/dy U3U2) -> G3, H1       row 5 (TH U3, 0) -> H4
 // LDS: window of three physical planes [3][npi][NX], products [npo][NX],
-// twiddles.
+// twiddles.  Transform positions sit at swizzled slots (ip_swz) of their rows;
+// the products read and write slot by slot, which is position by position in
+// the same permutation for every row.
 // -------------------------------------------------------------------------
 template <typename T, int E, int NTH>
 __global__ void __launch_bounds__(512) k_xfused_ip(IpLen fx, const cx<T> *__restrict__ twg, int mk, int ld,
@@ -129,7 +134,6 @@ __global__ void __launch_bounds__(512) k_xfused_ip(IpLen fx, const cx<T> *__rest
     const long zrow = (long)blockIdx.x * ld;
     const int ja = jlo + blockIdx.y * chunk;
     const int jb = min(jhi, ja + chunk - 1);
-    const cx<T> zero = make_cx<T>(0, 0);
 
     // Spectral input of stage 0's two groups (k = th, th + T; frequencies
     // k + q L0) of pair `pair` of plane `plane`, kept in registers.
@@ -144,8 +148,10 @@ __global__ void __launch_bounds__(512) k_xfused_ip(IpLen fx, const cx<T> *__rest
             for (int q = 0; q < R0; ++q) {
                 const int f = th + TT * u + q * L0;
                 const int kk = f < mk ? f : (f > nx - mk ? nx - f : -1);
-                pa[u * R0 + q] = kk >= 0 ? fa[kk] : zero;
-                pb[u * R0 + q] = (kk >= 0 && hasb) ? fbb[kk] : zero;
+                const cx<T> A = fa[max(kk, 0)];  // unconditional 16-byte loads, then select
+                const cx<T> B = hasb ? fbb[max(kk, 0)] : make_cx<T>(0, 0);
+                pa[u * R0 + q] = kk >= 0 ? A : make_cx<T>(0, 0);
+                pb[u * R0 + q] = kk >= 0 ? B : make_cx<T>(0, 0);
             }
         }
     };
@@ -166,7 +172,7 @@ __global__ void __launch_bounds__(512) k_xfused_ip(IpLen fx, const cx<T> *__rest
                 }
                 return make_cx<T>(A.re - B.im, A.im + B.re);
             },
-            [&](int, int pos, cx<T> v) { row[pos] = v; });
+            [&](int, int pos, cx<T> v) { row[ip_swz(fx, pos)] = v; });
     };
     // Forward stage 0 of product pair row r (plane j): groups th and L0 - th
     // (0 and L0/2 for th == 0) hold Z(k') and Z(NX - k') of their frequencies.
@@ -175,8 +181,8 @@ __global__ void __launch_bounds__(512) k_xfused_ip(IpLen fx, const cx<T> *__rest
         cx<T> va[R0], vb[R0];
 #pragma unroll
         for (int q = 0; q < R0; ++q) {
-            va[q] = row[ga + q * L0];
-            vb[q] = row[gb + q * L0];
+            va[q] = row[ip_swz(fx, ga + q * L0)];
+            vb[q] = row[ip_swz(fx, gb + q * L0)];
         }
         ip_group<T, R0, -1, false>(va, tw + fx.twoff[0], ga);
         ip_group<T, R0, -1, false>(vb, tw + fx.twoff[0], gb);
@@ -252,7 +258,7 @@ __global__ void __launch_bounds__(512) k_xfused_ip(IpLen fx, const cx<T> *__rest
     if (pro) dif0(slot_row(pro_plane, pro_pair));
     __syncthreads();
     for (int s = 1; s < S; ++s) {
-        if (pro) ip_stage_mem<T, E, +1, true>(slot_row(pro_plane, pro_pair), 1, fx, s, tw, th);
+        if (pro) ip_stage_row<T, E, +1, true>(slot_row(pro_plane, pro_pair), fx, s, tw, th);
         __syncthreads();
     }
     if (dif_row) {
@@ -274,8 +280,8 @@ __global__ void __launch_bounds__(512) k_xfused_ip(IpLen fx, const cx<T> *__rest
         cx<T> *drow = inv ? slot_row(j + 1, p) : win;
         cx<T> *frow = fb + (dit_row ? p - npi : 0) * nx;
         for (int s = 0; s < S - 1; ++s) {
-            if (inv) ip_stage_mem<T, E, +1, true>(drow, 1, fx, s + 1, tw, th);
-            if (fwd) ip_stage_mem<T, E, -1, false>(frow, 1, fx, S - 1 - s, tw, th);
+            if (inv) ip_stage_row<T, E, +1, true>(drow, fx, s + 1, tw, th);
+            if (fwd) ip_stage_row<T, E, -1, false>(frow, fx, S - 1 - s, tw, th);
             __syncthreads();
         }
         if (fwd) dit_out(frow, p - npi, j - 1);
@@ -312,7 +318,7 @@ __global__ void __launch_bounds__(256) k_zfwd_nl_ip(IpLen fz, const cx<T> *__res
     const int j = jlo + blockIdx.y;
     const int o = blockIdx.z;
     const int nz = fz.n, S = fz.nstages;
-    const cx<T> *sg = wt + (2 * o) * wt_fs + (long)j * wt_ps + i;
+    const cx<T> *sg = wt + (2 * o) * wt_fs + (long)j * wt_ps + (col ? i : mk - 1);
     const cx<T> *sh = sg + wt_fs;
     cx<T> *x = lds + b;
     const int rl = fz.radix[S - 1];
@@ -323,8 +329,10 @@ __global__ void __launch_bounds__(256) k_zfwd_nl_ip(IpLen fz, const cx<T> *__res
             constexpr int R = decltype(rc)::value;
             ip_groups<E, R>(fz, S - 1, th, [&](int u, int base, int) {
 #pragma unroll
-                for (int q = 0; q < R; ++q)
-                    regs[u * R + q] = col ? src[(long)(base + q * fz.span[S - 1]) * ld] : make_cx<T>(0, 0);
+                for (int q = 0; q < R; ++q) {
+                    const cx<T> v = src[(long)(base + q * fz.span[S - 1]) * ld];
+                    regs[u * R + q] = col ? v : make_cx<T>(0, 0);
+                }
             });
         });
     };
@@ -333,7 +341,7 @@ __global__ void __launch_bounds__(256) k_zfwd_nl_ip(IpLen fz, const cx<T> *__res
             constexpr int R = decltype(rc)::value;
             ip_stage<T, E, R, -1, false>(
                 fz, S - 1, tw, th, [&](int idx, int) { return regs[idx]; },
-                [&](int, int pos, cx<T> v) { x[pos * W] = v; });
+                [&](int, int pos, cx<T> v) { x[ip_cswz(fz, pos) * W] = v; });
         });
     };
     load_first(sg, gin);
@@ -347,7 +355,7 @@ __global__ void __launch_bounds__(256) k_zfwd_nl_ip(IpLen fz, const cx<T> *__res
     with_radix<E>(fz.radix[0], [&](auto rc) {
         constexpr int R = decltype(rc)::value;
         ip_stage<T, E, R, -1, false>(
-            fz, 0, tw, th, [&](int, int pos) { return x[pos * W]; }, [&](int idx, int, cx<T> v) { gv[idx] = v; });
+            fz, 0, tw, th, [&](int, int pos) { return x[ip_cswz(fz, pos) * W]; }, [&](int idx, int, cx<T> v) { gv[idx] = v; });
     });
     __syncthreads();
     first(hin);
@@ -361,7 +369,7 @@ __global__ void __launch_bounds__(256) k_zfwd_nl_ip(IpLen fz, const cx<T> *__res
     with_radix<E>(fz.radix[0], [&](auto rc) {
         constexpr int R = decltype(rc)::value;
         ip_stage<T, E, R, -1, false>(
-            fz, 0, tw, th, [&](int, int pos) { return x[pos * W]; },
+            fz, 0, tw, th, [&](int, int pos) { return x[ip_cswz(fz, pos) * W]; },
             [&](int idx, int f, cx<T> hv) {
                 const int s = slot_of_freq(f, nz, nkz, tnkz);
                 if (col && s >= 0) {
@@ -461,7 +469,9 @@ void launch_xfused_ip(hipStream_t s, const Geo &g, const IpLen &fx, const cx<T> 
     case 6: DGPU_XF(6); break;
     case 8: DGPU_XF(8); break;
     case 12: DGPU_XF(12); break;
-    case 16: DGPU_XF(16); break;
+    case 16:
+        if constexpr (sizeof(T) == 4) DGPU_XF(16);  // FP64 plans stop at 12 (16 spills)
+        break;
     default: break;
     }
 #undef DGPU_XF
@@ -489,6 +499,9 @@ void launch_zfwd_nl_ip(hipStream_t s, const Geo &g, const IpLen &fz, const cx<T>
 template void launch_zinv_ip<double, double>(hipStream_t, const Geo &, const IpLen &, const cx<double> *,
                                              const cx<double> *, long, long, cx<double> *, long, long, int, int, int,
                                              int);
+template void launch_zinv_ip<float, float>(hipStream_t, const Geo &, const IpLen &, const cx<float> *,
+                                           const cx<float> *, long, long, cx<float> *, long, long, int, int, int,
+                                           int);
 template void launch_zinv_ip<float, double>(hipStream_t, const Geo &, const IpLen &, const cx<float> *,
                                             const cx<double> *, long, long, cx<float> *, long, long, int, int, int,
                                             int);

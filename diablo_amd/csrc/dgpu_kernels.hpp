@@ -46,6 +46,11 @@ struct CnArgs {
     double rgx, rgz, drhodx, drhodz;  // RI*GRAV_X, RI*GRAV_Z, background gradients
     const double *tab;                // [sys][NY+2][kRow] for this stage
     const double *kx, *kz, *kx2, *kz2;
+    // Lane m solves mode map[m] (null: mode m) of state arrays whose plane
+    // stride is g.ps; the nonlinear terms are indexed by that mode with plane
+    // stride nl_ps.  The mixed path runs its FP64 low-|k| columns this way.
+    const int *map;
+    long nl_ps;
 };
 
 template <typename T, typename NLT = T>
@@ -63,6 +68,9 @@ struct RdArgs {
     int update_cp, jstart, jend;
     double inv_temp4;
     const double *pmult, *pinv, *rd, *kx, *kz;
+    const float *pmultf, *pinvf;  // FP32 copies of the factors (FP32 state)
+    const int *map;               // as CnArgs::map; factors have plane stride fac_ps
+    long fac_ps;
 };
 
 // ---- launchers (dgpu_kernels.hip) ----
@@ -116,6 +124,17 @@ void launch_pp_solve(hipStream_t s, const Geo &g, const cx<double> *cs, const cx
                      const double *gyf, const double *pmult, const double *pinv, const double *rd, cx<double> *scr,
                      cx<double> *cp);
 void launch_unpack_fortran(hipStream_t s, const Geo &g, const double *fort, cx<double> *kept, int j0, int nplanes);
+// Mixed path state copies (dgpu_columns.hip): dst[k] = src[k], k < n.
+template <typename TO, typename FROM>
+void launch_convert(hipStream_t s, const cx<FROM> *src, cx<TO> *dst, long n);
+void launch_convert_real(hipStream_t s, const double *src, float *dst, long n);
+// side[f][j][l] <-> full[f*full_fs + j*full_ps + map[l]], f < nf, j < planes
+template <typename TS, typename TF>
+void launch_side_gather(hipStream_t s, const cx<TF> *full, long full_fs, long full_ps, cx<TS> *side, long side_fs,
+                        const int *map, int nside, int planes, int nf);
+template <typename TS, typename TF>
+void launch_side_scatter(hipStream_t s, const cx<TS> *side, long side_fs, cx<TF> *full, long full_fs, long full_ps,
+                         const int *map, int nside, int planes, int nf);
 void launch_apply_bc_vel(hipStream_t s, const Geo &g, cx<double> *cu1, cx<double> *cu2, cx<double> *cu3,
                          const int *codes, const double *c1, const double *dy, const double *dyf);
 

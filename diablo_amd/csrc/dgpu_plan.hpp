@@ -9,6 +9,7 @@
 // through LDS.
 #pragma once
 
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <vector>
@@ -75,6 +76,129 @@ inline void ip_twiddles(const IpLen &f, std::vector<cx<double>> &tw)
                 w.im = (double)sinl(a);
             }
     }
+}
+
+// LDS cost of a plan's stage accesses for one row layout (positions of a row
+// at x[swz(pos)], rows of n 16-byte values back to back, 64 lanes a
+// wavefront, T lanes a row): the sum over the two bank models that bound
+// what the hardware does with 16-byte accesses (32 four-byte banks serving 8
+// lanes a cycle, 64 banks serving 16), relative to conflict-free.
+inline double ip_bank_cost(const IpLen &f, unsigned swz)
+{
+    IpLen g = f;
+    g.swz = swz;
+    const int T = f.t;
+    double tot = 0, ideal = 0;
+    const int models[2][2] = {{32, 8}, {64, 16}};
+    for (int i = 0; i < f.nstages; ++i) {
+        const int R = f.radix[i], L = f.span[i];
+        for (int u = 0; u < f.e / R; ++u)
+            for (int q = 0; q < R; ++q)
+                for (const auto &md : models) {
+                    const int banks = md[0], lpc = md[1];
+                    for (int g0 = 0; g0 < 64; g0 += lpc) {
+                        int cnt[64] = {0};
+                        int mx = 0;
+                        for (int lane = g0; lane < g0 + lpc; ++lane) {
+                            const int row = lane / T, th = lane % T;
+                            const int t = th + T * u, b = t / L, k = t - b * L;
+                            const long a = ((long)row * f.n + ip_swz(g, b * R * L + k + q * L)) * 4;  // 4-byte words
+                            for (int w = 0; w < 4; ++w) mx = std::max(mx, ++cnt[(a + w) % banks]);
+                        }
+                        const int per = lpc * 4 / banks;
+                        tot += (double)std::max(1, (mx + per - 1) / per);
+                        ideal += 1.0;
+                    }
+                }
+    }
+    return ideal > 0 ? tot / ideal : 1.0;
+}
+
+// Picks the cheapest bijective swizzle for the rows of an x-direction plan
+// from a few masks (identity first, so it wins ties).
+inline unsigned ip_pick_swizzle(const IpLen &f)
+{
+    auto table = [](int kind) {
+        unsigned w = 0;
+        for (int idx = 0; idx < 8; ++idx) {
+            const int b3 = idx & 1, b4 = (idx >> 1) & 1, b5 = (idx >> 2) & 1;
+            unsigned m = 0;
+            if (kind == 1) m = b4 | (b3 << 1) | (b4 << 2) | (b5 << 3);  // per-stage radix-4 spread
+            if (kind == 2) m = (unsigned)idx;                            // bits 3..5 into bits 0..2
+            w |= m << (4 * idx);
+        }
+        return w;
+    };
+    unsigned best = 0;
+    double best_c = ip_bank_cost(f, 0);
+    if (f.n % 16) return 0;
+    for (int kind = 1; kind <= 2; ++kind) {
+        const unsigned w = table(kind);
+        IpLen g = f;
+        g.swz = w;
+        std::vector<char> seen(f.n, 0);
+        bool ok = true;
+        for (int p = 0; p < f.n && ok; ++p) {
+            const int q = ip_swz(g, p);
+            if (q < 0 || q >= f.n || seen[q]) ok = false;
+            else seen[q] = 1;
+        }
+        if (!ok) continue;
+        const double c = ip_bank_cost(f, w);
+        if (c < best_c - 1e-9) {
+            best_c = c;
+            best = w;
+        }
+    }
+    return best;
+}
+
+// Same model for a column batch (lds[slot * W + b], lanes b = lane % W
+// fastest): picks the cheapest row swap ip_cswz (bit 0 ^= bit s), 0 = none.
+inline double ip_bank_cost_cols(const IpLen &f, int W, int cswz)
+{
+    IpLen g = f;
+    g.cswz = cswz;
+    double tot = 0, ideal = 0;
+    const int models[2][2] = {{32, 8}, {64, 16}};
+    for (int i = 0; i < f.nstages; ++i) {
+        const int R = f.radix[i], L = f.span[i];
+        for (int u = 0; u < f.e / R; ++u)
+            for (int q = 0; q < R; ++q)
+                for (const auto &md : models) {
+                    const int banks = md[0], lpc = md[1];
+                    for (int g0 = 0; g0 < 64; g0 += lpc) {
+                        int cnt[64] = {0};
+                        int mx = 0;
+                        for (int lane = g0; lane < g0 + lpc; ++lane) {
+                            const int b = lane % W, th = lane / W;
+                            if (th >= f.t) continue;
+                            const int t = th + f.t * u, bb = t / L, k = t - bb * L;
+                            const long a = ((long)ip_cswz(g, bb * R * L + k + q * L) * W + b) * 4;
+                            for (int w = 0; w < 4; ++w) mx = std::max(mx, ++cnt[(a + w) % banks]);
+                        }
+                        const int per = lpc * 4 / banks;
+                        tot += (double)std::max(1, (mx + per - 1) / per);
+                        ideal += 1.0;
+                    }
+                }
+    }
+    return ideal > 0 ? tot / ideal : 1.0;
+}
+
+inline int ip_pick_cswz(const IpLen &f, int W)
+{
+    int best = 0;
+    double best_c = ip_bank_cost_cols(f, W, 0);
+    for (int sbit = 1; sbit <= 5; ++sbit) {
+        if ((1 << sbit) >= f.n) break;
+        const double c = ip_bank_cost_cols(f, W, sbit);
+        if (c < best_c - 1e-9) {
+            best_c = c;
+            best = sbit;
+        }
+    }
+    return best;
 }
 
 // Plan of a z-direction (column batch) transform: fewest stages with at most

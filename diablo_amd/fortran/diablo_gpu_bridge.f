@@ -157,7 +157,15 @@
       PRM%NY = NY
       PRM%NZ = NZ
       PRM%N_TH = N_TH
+! DIABLO_GPU_PRECISION: 0 FP64 (default), 1 mixed (FP32 transforms and
+! solves, FP64 low modes), 2 mixed with the state and solves in FP64
       PRM%PRECISION = 0
+      CALL GET_ENVIRONMENT_VARIABLE('DIABLO_GPU_PRECISION', ENVV,
+     &     ENVLEN, ENVSTAT)
+      IF (ENVSTAT .EQ. 0 .AND. ENVLEN .GE. 1) THEN
+        IF (ENVV(1:1) .EQ. '1') PRM%PRECISION = 1
+        IF (ENVV(1:1) .EQ. '2') PRM%PRECISION = 2
+      END IF
       PRM%F_TYPE = F_TYPE
       PRM%U_BC_YMIN = U_BC_YMIN
       PRM%V_BC_YMIN = V_BC_YMIN

@@ -122,7 +122,7 @@ def _dp(a: np.ndarray):
     return a.ctypes.data_as(DP)
 
 
-PRECISIONS = {"fp64": 0, "mixed": 1}
+PRECISIONS = {"fp64": 0, "mixed": 1, "mixed_f64state": 2}
 
 
 def params_of(cfg: ChannelConfig, gy_1: np.ndarray, precision: str = "fp64") -> Params:
@@ -150,8 +150,12 @@ class GpuChannel:
     """The channel state on one MI355X, driven through the C-ABI."""
 
     def __init__(self, cfg: ChannelConfig, gy_1: np.ndarray | None = None, precision: str = "fp64"):
-        """precision: "fp64" (every kernel FP64) or "mixed" (FP32 transforms,
-        products and nonlinear terms; FP64 state and column solves)."""
+        """precision: "fp64" (every kernel FP64), "mixed" (FP32 transforms,
+        products, nonlinear terms and column solves on an FP32 working copy of
+        the state; the low modes I <= 2, |kz index| <= 2, among them the (0,0)
+        mean mode and the pressure gauge, solved in FP64) or "mixed_f64state"
+        (FP32 transforms, products and nonlinear terms; FP64 state and column
+        solves)."""
         cfg.validate()
         self.cfg = cfg
         self.precision = precision

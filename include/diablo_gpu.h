@@ -46,11 +46,16 @@ enum dgpu_status {
 };
 
 /* DGPU_FP64: every kernel in FP64.
- * DGPU_MIXED: the state (CU, CP, CTH, CF) and the per-mode Crank-Nicolson,
- * projection and CP solves stay FP64 (so the (0,0) mean mode, the pressure
- * gauge and the Poisson solve are FP64); the x-z transforms, the nonlinear
- * products and the nonlinear-term accumulators run in FP32. */
-enum dgpu_precision { DGPU_FP64 = 0, DGPU_MIXED = 1 };
+ * DGPU_MIXED: the x-z transforms, the nonlinear products and terms, and the
+ * per-mode Crank-Nicolson, projection and CP solves run in FP32 on an FP32
+ * working copy of the state; the low modes I <= 2, |kz index| <= 2 -- the
+ * (0,0) mean-flow mode and pressure gauge among them, and the low-wavenumber
+ * Poisson modes -- are solved in FP64 on FP64 columns.  The FP64 state that
+ * every other entry point reads and writes is refreshed from the working
+ * copy on demand.
+ * DGPU_MIXED_F64STATE: FP32 transforms, products and nonlinear terms; the
+ * state and every solve stay FP64. */
+enum dgpu_precision { DGPU_FP64 = 0, DGPU_MIXED = 1, DGPU_MIXED_F64STATE = 2 };
 
 /* Field identifiers for dgpu_set_field / dgpu_get_field. */
 enum dgpu_field {

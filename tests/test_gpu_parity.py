@@ -194,17 +194,19 @@ def test_poisson_p_chan(oracle_mod, gpu_mod, maker):
     assert err < 1e-11
 
 
-@pytest.mark.parametrize("maker,nsteps", [("channel", 100), ("kh", 100)])
-def test_mixed_precision_error(gpu_mod, maker, nsteps):
-    """Mixed path (FP32 transforms/products/NL, FP64 state and solves) against
-    the FP64 path from the same initial state: rel-L2 <= 1e-5, KE drift and
-    max|div u| reported."""
+@pytest.mark.parametrize("maker,nsteps,precision", [("channel", 100, "mixed"), ("kh", 100, "mixed"),
+                                                   ("channel", 100, "mixed_f64state"),
+                                                   ("kh", 100, "mixed_f64state")])
+def test_mixed_precision_error(gpu_mod, maker, nsteps, precision):
+    """Mixed paths against the FP64 path from the same initial state: rel-L2
+    <= 1e-5 and KE drift <= 1e-5; max|div u| at the FP64 level for the FP64
+    state, at FP32 round-off (<= 1e-4) for the FP32 working state."""
     from diablo_amd import metrics
     cfg = {"channel": lambda: case.channel_case(48, 33, 48, kick=0.05),
            "kh": lambda: case.kh_case(48, 33, 48, kick=0.05, dt=0.01)}[maker]()
     phys = case.initial_physical(cfg)
     g64 = gpu_mod.GpuChannel(cfg, precision="fp64")
-    gmx = gpu_mod.GpuChannel(cfg, precision="mixed")
+    gmx = gpu_mod.GpuChannel(cfg, precision=precision)
     g64.initialize(*phys)
     gmx.initialize(*phys)
     g64.step(nsteps)
@@ -212,10 +214,36 @@ def test_mixed_precision_error(gpu_mod, maker, nsteps):
     g64.synchronize()
     gmx.synchronize()
     r = metrics.compare(g64, g64.state(), gmx, gmx.state())
-    say(maker, nsteps, r)
+    say(maker, nsteps, precision, r)
     assert r["rel_l2"] <= 1e-5
     assert r["ke_drift"] <= 1e-5
-    assert r["max_div"] <= 1e-10 * max(1.0, r["max_div_ref"] * 1e10)
+    if precision == "mixed":
+        assert r["max_div"] <= 1e-4
+    else:
+        assert r["max_div"] <= 1e-10 * max(1.0, r["max_div_ref"] * 1e10)
+
+
+def test_mixed_state_sync(gpu_mod):
+    """The FP32 working state and the FP64 state the entry points see stay
+    coherent: reading the state between steps, or writing it back, does not
+    change the trajectory."""
+    cfg = case.kh_case(32, 33, 24, kick=0.05, dt=0.01)
+    phys = case.initial_physical(cfg)
+    runs = [gpu_mod.GpuChannel(cfg, precision="mixed") for _ in range(3)]
+    for g in runs:
+        g.initialize(*phys)
+    runs[0].step(4)
+    runs[1].step(2)
+    runs[1].state()                       # FP64 state refreshed, working copy untouched
+    runs[1].step(2)
+    runs[2].step(2)
+    runs[2].load_state(runs[2].state())   # written back: working copy refreshed from it
+    runs[2].step(2)
+    ref = runs[0].state()
+    for g in runs[1:]:
+        st = g.state()
+        for k in ("CU1", "CU2", "CU3", "CP", "CTH0", "CF1", "CFTH0"):
+            assert np.array_equal(st[k], ref[k]), k
 
 
 def test_graph_replays_match_eager(gpu_mod):
