@@ -60,6 +60,9 @@ struct dgpu_ctx {
     int *d_map = nullptr;
     int nside = 0;
     float *d_pmultf = nullptr, *d_pinvf = nullptr;
+    cx<float> *d_gs32 = nullptr;                     // FP32 solves' scratch (nin fields)
+    hipStream_t side_stream = nullptr;               // the low-mode columns run beside the FP32 solves
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     bool work_dirty = false, master_dirty = true;
     cx<float> *d_ztf = nullptr, *d_wtf = nullptr, *d_nlf = nullptr;
     cx<float> *nlf[3 + DGPU_MAX_TH] = {};
@@ -616,8 +619,13 @@ int32_t dgpu_create(const dgpu_params *p, dgpu_ctx **out)
             hipMalloc(&c->d_side, sizeof(cx<double>) * side_fs * (c->nstate + nin)) != hipSuccess ||
             hipMalloc(&c->d_map, sizeof(int) * map.size()) != hipSuccess ||
             hipMalloc(&c->d_pmultf, sizeof(float) * c->fs) != hipSuccess ||
-            hipMalloc(&c->d_pinvf, sizeof(float) * c->fs) != hipSuccess)
+            hipMalloc(&c->d_pinvf, sizeof(float) * c->fs) != hipSuccess ||
+            hipMalloc(&c->d_gs32, sizeof(cx<float>) * c->fs * nin) != hipSuccess)
             return bail(DGPU_E_NOMEM);
+        if (hipStreamCreateWithFlags(&c->side_stream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess)
+            return bail(DGPU_E_HIP);
         if (hipMemcpy(c->d_map, map.data(), sizeof(int) * map.size(), hipMemcpyHostToDevice) != hipSuccess ||
             hipMemset(c->d_side, 0, sizeof(cx<double>) * side_fs * (c->nstate + nin)) != hipSuccess)
             return bail(DGPU_E_HIP);
@@ -681,9 +689,15 @@ int32_t dgpu_destroy(dgpu_ctx *c)
     void *bufs[] = {c->d_vec, c->d_tw, c->d_state, c->d_pmult, c->d_pinv, c->d_rd, c->d_nlptr, c->d_zt, c->d_wt,
                     c->d_fa, c->d_fb, c->d_gyf, c->d_iptw, c->d_iptwf, c->d_ztf, c->d_wtf, c->d_nlf, c->d_nlptrf,
                     c->d_fort, c->d_cn, c->d_codes, c->d_c1, c->d_state32, c->d_side, c->d_map,
-                    c->d_pmultf, c->d_pinvf};
+                    c->d_pmultf, c->d_pinvf, c->d_gs32};
     for (void *b : bufs)
         if (b) hipFree(b);
+    if (c->side_stream) {
+        hipStreamSynchronize(c->side_stream);
+        hipStreamDestroy(c->side_stream);
+    }
+    if (c->ev_fork) hipEventDestroy(c->ev_fork);
+    if (c->ev_join) hipEventDestroy(c->ev_join);
     if (c->stream && c->own_stream) hipStreamDestroy(c->stream);
     delete c;
     return DGPU_OK;
@@ -753,8 +767,8 @@ static void fill_ptrs(dgpu_ctx *c, dgpu::CnPtrs<double, NLT> &P, cx<NLT> *const 
     }
 }
 
-// DGPU_MIXED: the FP32 working state; the nonlinear terms (FP32) double as
-// the solves' scratch.
+// DGPU_MIXED: the FP32 working state and the FP32 solves' own scratch (the
+// low-mode columns read the nonlinear terms at the same time).
 static void fill_ptrs32(dgpu_ctx *c, dgpu::CnPtrs<float, float> &P)
 {
     std::memset(&P, 0, sizeof(P));
@@ -773,7 +787,7 @@ static void fill_ptrs32(dgpu_ctx *c, dgpu::CnPtrs<float, float> &P)
     const int map[4] = {0, 2, 1, 3};
     for (int q = 0; q < nin; ++q) {
         P.nl[q] = c->nlf[map[q]];
-        P.gs[q] = c->nlf[map[q]];
+        P.gs[q] = c->d_gs32 + q * fs;
     }
 }
 
@@ -916,25 +930,29 @@ static int rk_enqueue(dgpu_ctx *c, int rk_step, double time, double temp1, doubl
         Rs.g.ps = c->nside;
         Rs.map = c->d_map;
         const long side_fs = (long)c->nside * (g.ny + 2);
+        // The low-mode columns (a few dozen lanes whose sweeps are latency
+        // bound) run on the side stream while the FP32 solves run; their
+        // results then replace the FP32 ones in the working state.
+        HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
+        HIPCHK(c, hipStreamWaitEvent(c->side_stream, c->ev_fork, 0));
+        dgpu::launch_cn<double, float>(c->side_stream, As, Ps);
+        dgpu::launch_remdiv<double, float>(c->side_stream, Rs, Ps);
+        if (int rc = check_launch(c, "low-mode columns")) return rc;
+        HIPCHK(c, hipEventRecord(c->ev_join, c->side_stream));
         {
             TimerScope t(c, 3);
             dgpu::launch_cn<float, float>(c->stream, A, P);
-            dgpu::launch_cn<double, float>(c->stream, As, Ps);
-            dgpu::launch_side_scatter<double, float>(c->stream, c->d_side, side_fs, c->d_state32, c->fs, g.ps,
-                                                     c->d_map, c->nside, g.ny + 2, nin);
         }
         if (int rc = check_launch(c, "cn")) return rc;
         {
             TimerScope t(c, 4);
             dgpu::launch_remdiv<float, float>(c->stream, R, P);
-            dgpu::launch_remdiv<double, float>(c->stream, Rs, Ps);
-            // CU1-3 and CP of the low modes
-            dgpu::launch_side_scatter<double, float>(c->stream, c->d_side, side_fs, c->d_state32, c->fs, g.ps,
-                                                     c->d_map, c->nside, g.ny + 2, 3);
-            dgpu::launch_side_scatter<double, float>(c->stream, c->d_side + nin * side_fs, side_fs,
-                                                     c->d_state32 + nin * c->fs, c->fs, g.ps, c->d_map, c->nside,
-                                                     g.ny + 2, 1);
         }
+        if (int rc = check_launch(c, "remdiv")) return rc;
+        HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
+        // CU1-3, CTH and CP of the low modes
+        dgpu::launch_side_scatter<double, float>(c->stream, c->d_side, side_fs, c->d_state32, c->fs, g.ps, c->d_map,
+                                                 c->nside, g.ny + 2, nin + 1);
     } else if (c->mixed) {
         dgpu::CnPtrs<double, float> P;
         fill_ptrs<float>(c, P, c->nlf);

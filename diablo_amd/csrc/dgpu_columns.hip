@@ -56,8 +56,11 @@ __device__ __forceinline__ void piped_sweep(int j0, int dir, int n, Ld &&ld, Bod
 // ---------------------------------------------------------------------------
 // k_cn
 // ---------------------------------------------------------------------------
+// 519 blocks of four wavefronts at 384x257x384: three waves per SIMD (<= 168
+// VGPRs) keep them all resident at once instead of leaving a second round
+// for a few CUs.
 template <typename T, int NTH, typename NLT>
-__global__ void __launch_bounds__(256) k_cn(CnArgs A, CnPtrs<T, NLT> P)
+__global__ void __launch_bounds__(256, 3) k_cn(CnArgs A, CnPtrs<T, NLT> P)
 {
     const Geo &g = A.g;
     const int q = wave_uniform(threadIdx.x >> 6);  // system of this wavefront
@@ -100,7 +103,8 @@ __global__ void __launch_bounds__(256) k_cn(CnArgs A, CnPtrs<T, NLT> P)
         const bool has1 = (q != SYS_U2);
         const bool has2 = (q == SYS_TH0) || NTH;
         const bool hascp = (q != SYS_TH0);
-        piped_sweep<4>(
+        constexpr int DA = sizeof(T) == 4 ? 6 : 3;  // planes in flight (register budget)
+        piped_sweep<DA>(
             0, +1, NY + 2,
             [&](int j) {
                 const long o = m + (long)j * ps;
@@ -175,7 +179,7 @@ __global__ void __launch_bounds__(256) k_cn(CnArgs A, CnPtrs<T, NLT> P)
     // ---------------- sweep B ----------------
     if (live) {
         cx<T> x = zero;
-        piped_sweep<8>(
+        piped_sweep<sizeof(T) == 4 ? 16 : 8>(
             NY + 1, -1, NY + 2, [&](int j) { return gs[m + (long)j * ps]; },
             [&](int j, const cx<T> &gj) {
                 const double *t = tab + (long)j * kRow;
@@ -263,7 +267,7 @@ __global__ void __launch_bounds__(64) k_remdiv(RdArgs A, CnPtrs<T, NLT> P)
         cx<T> gprev = zero;
         cx<T> c2 = P.cu2[m + 2 * ps];
         scr[m + ps] = zero;
-        piped_sweep<4>(
+        piped_sweep<8>(
             2, +1, NY - 1,
             [&](int j) {
                 const long o = m + (long)j * ps;
@@ -301,7 +305,7 @@ __global__ void __launch_bounds__(64) k_remdiv(RdArgs A, CnPtrs<T, NLT> P)
         const T itemp4 = T(A.inv_temp4);
         const bool upd = A.update_cp;
         cx<T> phip = zero;
-        piped_sweep<4>(
+        piped_sweep<sizeof(T) == 4 ? 8 : 6>(
             NY, -1, NY,
             [&](int j) {
                 const long o = m + (long)j * ps;
