@@ -267,7 +267,7 @@ __global__ void __launch_bounds__(64) k_remdiv(RdArgs A, CnPtrs<T, NLT> P)
         cx<T> gprev = zero;
         cx<T> c2 = P.cu2[m + 2 * ps];
         scr[m + ps] = zero;
-        piped_sweep<8>(
+        piped_sweep<sizeof(T) == 4 ? 16 : 8>(
             2, +1, NY - 1,
             [&](int j) {
                 const long o = m + (long)j * ps;
@@ -305,7 +305,7 @@ __global__ void __launch_bounds__(64) k_remdiv(RdArgs A, CnPtrs<T, NLT> P)
         const T itemp4 = T(A.inv_temp4);
         const bool upd = A.update_cp;
         cx<T> phip = zero;
-        piped_sweep<sizeof(T) == 4 ? 8 : 6>(
+        piped_sweep<sizeof(T) == 4 ? 12 : 6>(
             NY, -1, NY,
             [&](int j) {
                 const long o = m + (long)j * ps;
