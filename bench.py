@@ -160,6 +160,7 @@ def main():
     ap.add_argument("--accuracy-steps", type=int, default=100)
     ap.add_argument("--ensemble", type=int, default=64, help="members of the batched run (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=10)
+    ap.add_argument("--ensemble-precision", default="mixed", choices=("mixed", "mixed_f64state", "fp64"))
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB five times (reference point for HBM counters)")
     args = ap.parse_args()
@@ -228,8 +229,8 @@ def main():
 
     batched = None
     if args.ensemble > 0:
-        log(f"[bench] batched: {args.ensemble} KH members, mixed, {args.ensemble_steps} steps")
-        batched = ensemble_run(args.ensemble, args.ensemble_steps, 1)
+        log(f"[bench] batched: {args.ensemble} KH members, {args.ensemble_precision}, {args.ensemble_steps} steps")
+        batched = ensemble_run(args.ensemble, args.ensemble_steps, 1, args.ensemble_precision)
         log(f"[bench] batched aggregate {batched['value']:.1f} Mpts*substeps/s "
             f"({batched['value'] / value:.2f}x FP64 384)")
 

@@ -105,6 +105,17 @@ struct dgpu_ctx {
     long long t_n[kKinds] = {0, 0, 0, 0, 0};
 };
 
+// Members sharing NX, NZ and N_TH (hence transform plans, block shapes and
+// the mode count): one launch per kernel kind covers all of them.
+struct dgpu_ens_group {
+    std::vector<int> idx;
+    int max_planes = 0, max_span = 0, max_zf = 0;  // tallest member: NY+2, x-sweep planes, z-forward planes
+    void *d_zinv = nullptr, *d_xf = nullptr, *d_zf = nullptr, *d_scat = nullptr;
+    void *d_col[3] = {nullptr, nullptr, nullptr};   // column descriptors per RK stage
+    void *d_cols[3] = {nullptr, nullptr, nullptr};  // low-mode columns (FP32 state)
+    std::vector<char> col_host[3], cols_host[3];
+};
+
 struct dgpu_ensemble {
     std::vector<dgpu_ctx *> m;
     hipStream_t main = nullptr;
@@ -113,6 +124,11 @@ struct dgpu_ensemble {
     hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
     std::vector<double> gkey[3];
     bool use_graphs = true;
+    // batched launches (all members at one precision)
+    bool batched = false;
+    std::vector<dgpu_ens_group> groups;
+    hipStream_t side = nullptr;
+    hipEvent_t ev_f = nullptr, ev_j = nullptr;
     std::string err;
 };
 }
@@ -835,6 +851,37 @@ static void fill_rdargs(dgpu_ctx *c, dgpu::RdArgs &R, int update_cp, double temp
     R.fac_ps = c->g.ps;
 }
 
+// Per-stage constants of the CN kernel (for/channel.f:77-81 and the forcing).
+static void make_cn_args(dgpu_ctx *c, int rk_step, double time, double temp3, double temp4, double temp5,
+                         dgpu::CnArgs &A)
+{
+    const dgpu::Geo &g = c->g;
+    const int k = rk_step - 1;
+    std::memset(&A, 0, sizeof(A));
+    A.g = g;
+    A.rk_gt1 = rk_step > 1;
+    A.f_type = c->p.f_type;
+    A.temp3 = temp3;
+    A.temp4 = temp4;
+    A.temp5 = temp5;
+    A.nu = c->p.nu;
+    A.nupr = g.nth ? c->p.nu / c->p.pr[0] : 0.0;
+    A.beta = c->p.beta;
+    A.i_ro = c->p.i_ro;
+    A.force00 = (c->p.f_type == 2) ? (c->p.px0 + c->p.amp_omega0 * std::cos(c->p.omega0 * time)) : c->p.px0;
+    A.rgx = g.nth ? c->p.ri[0] * c->p.grav_x : 0.0;
+    A.rgz = g.nth ? c->p.ri[0] * c->p.grav_z : 0.0;
+    A.drhodx = g.nth ? c->p.drhodx[0] : 0.0;
+    A.drhodz = g.nth ? c->p.drhodz[0] : 0.0;
+    A.tab = c->d_cn + (long)k * nsys_of(c) * cn_block(c);
+    A.kx = c->d_kx;
+    A.kz = c->d_kz;
+    A.kx2 = c->d_kx2;
+    A.kz2 = c->d_kz2;
+    A.map = nullptr;
+    A.nl_ps = g.ps;
+}
+
 // Enqueues the five kernels of one substep on the context's stream.
 static int rk_enqueue(dgpu_ctx *c, int rk_step, double time, double temp1, double temp3, double temp4, double temp5)
 {
@@ -891,29 +938,7 @@ static int rk_enqueue(dgpu_ctx *c, int rk_step, double time, double temp1, doubl
         if (int rc = check_launch(c, "zfwd_nl")) return rc;
     }
     dgpu::CnArgs A;
-    std::memset(&A, 0, sizeof(A));
-    A.g = g;
-    A.rk_gt1 = rk_step > 1;
-    A.f_type = c->p.f_type;
-    A.temp3 = temp3;
-    A.temp4 = temp4;
-    A.temp5 = temp5;
-    A.nu = c->p.nu;
-    A.nupr = g.nth ? c->p.nu / c->p.pr[0] : 0.0;
-    A.beta = c->p.beta;
-    A.i_ro = c->p.i_ro;
-    A.force00 = (c->p.f_type == 2) ? (c->p.px0 + c->p.amp_omega0 * std::cos(c->p.omega0 * time)) : c->p.px0;
-    A.rgx = g.nth ? c->p.ri[0] * c->p.grav_x : 0.0;
-    A.rgz = g.nth ? c->p.ri[0] * c->p.grav_z : 0.0;
-    A.drhodx = g.nth ? c->p.drhodx[0] : 0.0;
-    A.drhodz = g.nth ? c->p.drhodz[0] : 0.0;
-    A.tab = c->d_cn + (long)k * nsys_of(c) * cn_block(c);
-    A.kx = c->d_kx;
-    A.kz = c->d_kz;
-    A.kx2 = c->d_kx2;
-    A.kz2 = c->d_kz2;
-    A.map = nullptr;
-    A.nl_ps = g.ps;
+    make_cn_args(c, rk_step, time, temp3, temp4, temp5, A);
     dgpu::RdArgs R;
     fill_rdargs(c, R, 1, temp4);
     if (c->f32state) {
@@ -1174,10 +1199,272 @@ int32_t dgpu_poisson_p_chan(dgpu_ctx *c)
 // parallel branch: fork event on the ensemble stream, each member's kernels on
 // its own stream, join events back.
 // ---------------------------------------------------------------------------
+extern "C++" {
+static int member_jlo(const dgpu_ctx *c)
+{
+    int jlo = 2;
+    for (int n = 0; n < c->g.nth; ++n) jlo = std::min(jlo, c->jstart_th[n]);
+    return jlo;
+}
+
+template <typename D>
+static int upload(void **dptr, const std::vector<D> &h)
+{
+    if (!*dptr && hipMalloc(dptr, sizeof(D) * h.size()) != hipSuccess) return DGPU_E_NOMEM;
+    if (hipMemcpy(*dptr, h.data(), sizeof(D) * h.size(), hipMemcpyHostToDevice) != hipSuccess) return DGPU_E_HIP;
+    return DGPU_OK;
+}
+
+// Groups the members by (NX, NZ, N_TH) and writes the transform descriptors
+// (they do not change between steps).  Members of different precisions keep
+// the per-member path.
+static int ens_build_groups(dgpu_ensemble *e)
+{
+    const int n = (int)e->m.size();
+    for (int i = 1; i < n; ++i)
+        if (e->m[i]->p.precision != e->m[0]->p.precision) return DGPU_OK;
+    for (int i = 0; i < n; ++i) {
+        const dgpu_ctx *c = e->m[i];
+        bool placed = false;
+        for (auto &gr : e->groups) {
+            const dgpu_ctx *c0 = e->m[gr.idx[0]];
+            if (c0->g.nx == c->g.nx && c0->g.nz == c->g.nz && c0->g.nth == c->g.nth) {
+                gr.idx.push_back(i);
+                placed = true;
+                break;
+            }
+        }
+        if (!placed) {
+            e->groups.emplace_back();
+            e->groups.back().idx.push_back(i);
+        }
+    }
+    const bool mixed = e->m[0]->mixed, f32 = e->m[0]->f32state;
+    for (auto &gr : e->groups) {
+        std::vector<dgpu::XfDesc<double>> xf64;
+        std::vector<dgpu::XfDesc<float>> xf32;
+        std::vector<dgpu::ZinvDesc<double, double>> zi64;
+        std::vector<dgpu::ZinvDesc<float, double>> zi32d;
+        std::vector<dgpu::ZinvDesc<float, float>> zi32;
+        std::vector<dgpu::ZfDesc<double, double>> zf64;
+        std::vector<dgpu::ZfDesc<float, float>> zf32;
+        std::vector<dgpu::ScatDesc<double, float>> sc;
+        for (int i : gr.idx) {
+            dgpu_ctx *c = e->m[i];
+            const int jlo = member_jlo(c), ny = c->g.ny;
+            gr.max_planes = std::max(gr.max_planes, ny + 2);
+            gr.max_span = std::max(gr.max_span, ny - jlo + 1);
+            gr.max_zf = std::max(gr.max_zf, ny - jlo + 1);
+            if (!mixed) {
+                zi64.push_back({c->cu[0], c->d_zt, c->fs, c->zfs, ny + 2});
+                xf64.push_back({c->d_zt, c->d_wt, c->zfs, c->zfs, c->d_dy, c->d_dyf, c->d_kx, jlo, ny});
+                zf64.push_back({c->d_wt, c->zfs, c->d_nlptr, c->d_kz, jlo, ny - jlo + 1});
+            } else {
+                if (f32) zi32.push_back({c->d_state32, c->d_ztf, c->fs, c->zfs, ny + 2});
+                else zi32d.push_back({c->cu[0], c->d_ztf, c->fs, c->zfs, ny + 2});
+                xf32.push_back({c->d_ztf, c->d_wtf, c->zfs, c->zfs, c->d_dy, c->d_dyf, c->d_kx, jlo, ny});
+                zf32.push_back({c->d_wtf, c->zfs, c->d_nlptrf, c->d_kz, jlo, ny - jlo + 1});
+                if (f32)
+                    sc.push_back({c->d_side, c->d_state32, (long)c->nside * (ny + 2), c->fs, c->d_map, ny + 2});
+            }
+        }
+        int rc = DGPU_OK;
+        if (!mixed) {
+            if (!rc) rc = upload(&gr.d_zinv, zi64);
+            if (!rc) rc = upload(&gr.d_xf, xf64);
+            if (!rc) rc = upload(&gr.d_zf, zf64);
+        } else {
+            if (!rc) rc = f32 ? upload(&gr.d_zinv, zi32) : upload(&gr.d_zinv, zi32d);
+            if (!rc) rc = upload(&gr.d_xf, xf32);
+            if (!rc) rc = upload(&gr.d_zf, zf32);
+            if (!rc && f32) rc = upload(&gr.d_scat, sc);
+        }
+        if (rc) return rc;
+    }
+    if (f32) {
+        if (hipStreamCreateWithFlags(&e->side, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&e->ev_f, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&e->ev_j, hipEventDisableTiming) != hipSuccess)
+            return DGPU_E_HIP;
+    }
+    e->batched = true;
+    return DGPU_OK;
+}
+
+// Writes a group's column descriptors of one RK stage if they changed (the
+// ensemble stream is idle first: a launched graph may be reading them).
+template <typename D>
+static int refresh(dgpu_ensemble *e, void **dptr, std::vector<char> &last, const std::vector<D> &h)
+{
+    const char *b = reinterpret_cast<const char *>(h.data());
+    if (*dptr && last.size() == sizeof(D) * h.size() && std::memcmp(last.data(), b, last.size()) == 0)
+        return DGPU_OK;
+    if (hipStreamSynchronize(e->main) != hipSuccess) return DGPU_E_HIP;
+    if (int rc = upload(dptr, h)) return rc;
+    last.assign(b, b + sizeof(D) * h.size());
+    return DGPU_OK;
+}
+
+static int ens_refresh_columns(dgpu_ensemble *e, int rk_step, const std::vector<double> &key)
+{
+    const int k = rk_step - 1;
+    const bool mixed = e->m[0]->mixed, f32 = e->m[0]->f32state;
+    for (auto &gr : e->groups) {
+        std::vector<dgpu::ColDesc<double, double>> c64;
+        std::vector<dgpu::ColDesc<double, float>> c64f, side;
+        std::vector<dgpu::ColDesc<float, float>> c32;
+        for (int i : gr.idx) {
+            dgpu_ctx *c = e->m[i];
+            const double *q = &key[4 * i];
+            dgpu::CnArgs A;
+            make_cn_args(c, rk_step, 0.0, q[1], q[2], q[3], A);
+            dgpu::RdArgs R;
+            fill_rdargs(c, R, 1, q[2]);
+            if (!mixed) {
+                dgpu::ColDesc<double, double> d;
+                std::memset(&d, 0, sizeof(d));
+                d.A = A;
+                d.R = R;
+                fill_ptrs<double>(c, d.P, c->nl);
+                c64.push_back(d);
+            } else if (!f32) {
+                dgpu::ColDesc<double, float> d;
+                std::memset(&d, 0, sizeof(d));
+                d.A = A;
+                d.R = R;
+                fill_ptrs<float>(c, d.P, c->nlf);
+                c64f.push_back(d);
+            } else {
+                dgpu::ColDesc<float, float> d;
+                std::memset(&d, 0, sizeof(d));
+                d.A = A;
+                d.R = R;
+                fill_ptrs32(c, d.P);
+                c32.push_back(d);
+                dgpu::ColDesc<double, float> ds;
+                std::memset(&ds, 0, sizeof(ds));
+                ds.A = A;
+                ds.A.g.ps = c->nside;
+                ds.A.map = c->d_map;
+                ds.R = R;
+                ds.R.g.ps = c->nside;
+                ds.R.map = c->d_map;
+                fill_ptrs_side(c, ds.P);
+                side.push_back(ds);
+            }
+        }
+        int rc = DGPU_OK;
+        if (!mixed) rc = refresh(e, &gr.d_col[k], gr.col_host[k], c64);
+        else if (!f32) rc = refresh(e, &gr.d_col[k], gr.col_host[k], c64f);
+        else {
+            rc = refresh(e, &gr.d_col[k], gr.col_host[k], c32);
+            if (!rc) rc = refresh(e, &gr.d_cols[k], gr.cols_host[k], side);
+        }
+        if (rc) return rc;
+    }
+    return DGPU_OK;
+}
+
+// One substep of every member: per group, the transform kernels and the
+// column kernels each as one launch over the group's members.
+static int ens_enqueue_batched(dgpu_ensemble *e, int rk_step)
+{
+    const int k = rk_step - 1;
+    const bool mixed = e->m[0]->mixed, f32 = e->m[0]->f32state;
+    hipStream_t s = e->main;
+    for (auto &gr : e->groups) {
+        dgpu_ctx *c0 = e->m[gr.idx[0]];
+        const dgpu::Geo &g = c0->g;
+        const int nm = (int)gr.idx.size(), nin = 3 + g.nth;
+        if (!mixed) {
+            const cx<double> *twx = c0->d_iptw, *twz = c0->d_iptw + c0->ipx.twn;
+            dgpu::launch_zinv_ip_batch<double, double>(s, g, c0->ipz, twz,
+                                                       (const dgpu::ZinvDesc<double, double> *)gr.d_zinv, nm,
+                                                       gr.max_planes, c0->ld, nin);
+            dgpu::launch_xfused_ip_batch<double>(s, g, c0->ipx, twx, (const dgpu::XfDesc<double> *)gr.d_xf, nm,
+                                                 gr.max_span, c0->ld, c0->xchunk);
+            dgpu::launch_zfwd_nl_ip_batch<double, double>(s, g, c0->ipz, twz,
+                                                          (const dgpu::ZfDesc<double, double> *)gr.d_zf, nm,
+                                                          gr.max_zf, c0->ld, nin);
+        } else {
+            const cx<float> *twx = c0->d_iptwf, *twz = c0->d_iptwf + c0->ipx.twn;
+            if (f32)
+                dgpu::launch_zinv_ip_batch<float, float>(s, g, c0->ipz, twz,
+                                                         (const dgpu::ZinvDesc<float, float> *)gr.d_zinv, nm,
+                                                         gr.max_planes, c0->ld, nin);
+            else
+                dgpu::launch_zinv_ip_batch<float, double>(s, g, c0->ipz, twz,
+                                                          (const dgpu::ZinvDesc<float, double> *)gr.d_zinv, nm,
+                                                          gr.max_planes, c0->ld, nin);
+            dgpu::launch_xfused_ip_batch<float>(s, g, c0->ipx, twx, (const dgpu::XfDesc<float> *)gr.d_xf, nm,
+                                                gr.max_span, c0->ld, c0->xchunk);
+            dgpu::launch_zfwd_nl_ip_batch<float, float>(s, g, c0->ipz, twz,
+                                                        (const dgpu::ZfDesc<float, float> *)gr.d_zf, nm, gr.max_zf,
+                                                        c0->ld, nin);
+        }
+        if (hipGetLastError() != hipSuccess) return DGPU_E_HIP;
+    }
+    if (f32) {
+        // the side stream waits for every group's nonlinear terms
+        if (hipEventRecord(e->ev_f, s) != hipSuccess || hipStreamWaitEvent(e->side, e->ev_f, 0) != hipSuccess)
+            return DGPU_E_HIP;
+        for (auto &gr : e->groups) {
+            dgpu::Geo gs = e->m[gr.idx[0]]->g;
+            gs.ps = e->m[gr.idx[0]]->nside;
+            const int nm = (int)gr.idx.size();
+            dgpu::launch_cn_batch<double, float>(e->side, gs, (const dgpu::ColDesc<double, float> *)gr.d_cols[k], nm);
+            dgpu::launch_remdiv_batch<double, float>(e->side, gs, (const dgpu::ColDesc<double, float> *)gr.d_cols[k],
+                                                     nm);
+        }
+        if (hipGetLastError() != hipSuccess) return DGPU_E_HIP;
+        if (hipEventRecord(e->ev_j, e->side) != hipSuccess) return DGPU_E_HIP;
+    }
+    for (auto &gr : e->groups) {
+        const dgpu::Geo &g = e->m[gr.idx[0]]->g;
+        const int nm = (int)gr.idx.size();
+        if (!mixed) {
+            dgpu::launch_cn_batch<double, double>(s, g, (const dgpu::ColDesc<double, double> *)gr.d_col[k], nm);
+            dgpu::launch_remdiv_batch<double, double>(s, g, (const dgpu::ColDesc<double, double> *)gr.d_col[k], nm);
+        } else if (!f32) {
+            dgpu::launch_cn_batch<double, float>(s, g, (const dgpu::ColDesc<double, float> *)gr.d_col[k], nm);
+            dgpu::launch_remdiv_batch<double, float>(s, g, (const dgpu::ColDesc<double, float> *)gr.d_col[k], nm);
+        } else {
+            dgpu::launch_cn_batch<float, float>(s, g, (const dgpu::ColDesc<float, float> *)gr.d_col[k], nm);
+            dgpu::launch_remdiv_batch<float, float>(s, g, (const dgpu::ColDesc<float, float> *)gr.d_col[k], nm);
+        }
+        if (hipGetLastError() != hipSuccess) return DGPU_E_HIP;
+    }
+    if (f32) {
+        if (hipStreamWaitEvent(s, e->ev_j, 0) != hipSuccess) return DGPU_E_HIP;
+        for (auto &gr : e->groups) {
+            const dgpu_ctx *c0 = e->m[gr.idx[0]];
+            dgpu::launch_side_scatter_batch<double, float>(s, (const dgpu::ScatDesc<double, float> *)gr.d_scat,
+                                                           (int)gr.idx.size(), c0->nside, c0->g.ps, gr.max_planes,
+                                                           4 + c0->g.nth);
+        }
+        if (hipGetLastError() != hipSuccess) return DGPU_E_HIP;
+    }
+    return DGPU_OK;
+}
+
+}  // extern "C++"
+
 int32_t dgpu_ensemble_destroy(dgpu_ensemble *e)
 {
     if (!e) return DGPU_OK;
     if (e->main) hipStreamSynchronize(e->main);
+    if (e->side) {
+        hipStreamSynchronize(e->side);
+        hipStreamDestroy(e->side);
+    }
+    if (e->ev_f) hipEventDestroy(e->ev_f);
+    if (e->ev_j) hipEventDestroy(e->ev_j);
+    for (auto &gr : e->groups) {
+        void *bufs[] = {gr.d_zinv, gr.d_xf, gr.d_zf, gr.d_scat, gr.d_col[0], gr.d_col[1], gr.d_col[2],
+                        gr.d_cols[0], gr.d_cols[1], gr.d_cols[2]};
+        for (void *b : bufs)
+            if (b) hipFree(b);
+    }
     for (auto &ge : e->gexec)
         if (ge) hipGraphExecDestroy(ge);
     for (auto *c : e->m) dgpu_destroy(c);
@@ -1215,6 +1502,10 @@ int32_t dgpu_ensemble_create(const dgpu_params *members, int32_t n, dgpu_ensembl
             dgpu_ensemble_destroy(e);
             return DGPU_E_HIP;
         }
+    if (int rc = ens_build_groups(e)) {
+        dgpu_ensemble_destroy(e);
+        return rc;
+    }
     *out = e;
     return DGPU_OK;
 }
@@ -1265,7 +1556,24 @@ int32_t dgpu_ensemble_rk_chan_1(dgpu_ensemble *e, int32_t rk_step, const double 
         }
         c->work_dirty = c->f32state;
     }
+    if (e->batched) {
+        if (int rc = ens_refresh_columns(e, rk_step, key)) {
+            e->err = "batched column descriptors";
+            return rc;
+        }
+    }
     auto enqueue_all = [&](bool joined) -> int {
+        if (e->batched) {
+            // one launch per kernel kind and member group on the ensemble stream
+            if (!joined) {
+                for (int i = 0; i < n; ++i)
+                    if (hipStreamSynchronize(e->m[i]->stream) != hipSuccess) return DGPU_E_HIP;
+            }
+            const int rc = ens_enqueue_batched(e, rk_step);
+            if (rc) e->err = "batched launch";
+            if (!joined && !rc && hipStreamSynchronize(e->main) != hipSuccess) return DGPU_E_HIP;
+            return rc;
+        }
         for (int i = 0; i < n; ++i) {
             dgpu_ctx *c = e->m[i];
             if (joined && hipStreamWaitEvent(c->stream, e->fork, 0) != hipSuccess) return DGPU_E_HIP;

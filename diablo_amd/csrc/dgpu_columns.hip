@@ -60,11 +60,11 @@ __device__ __forceinline__ void piped_sweep(int j0, int dir, int n, Ld &&ld, Bod
 // VGPRs) keep them all resident at once instead of leaving a second round
 // for a few CUs.
 template <typename T, int NTH, typename NLT>
-__global__ void __launch_bounds__(256, 3) k_cn(CnArgs A, CnPtrs<T, NLT> P)
+__device__ __forceinline__ void cn_body(const CnArgs &A, const CnPtrs<T, NLT> &P, int bx)
 {
     const Geo &g = A.g;
     const int q = wave_uniform(threadIdx.x >> 6);  // system of this wavefront
-    const long m = (long)blockIdx.x * 64 + (threadIdx.x & 63);
+    const long m = (long)bx * 64 + (threadIdx.x & 63);
     const bool live = m < g.ps && (q < 3 + NTH);
     const int NY = g.ny;
     const long ps = g.ps;
@@ -195,6 +195,22 @@ __global__ void __launch_bounds__(256, 3) k_cn(CnArgs A, CnPtrs<T, NLT> P)
     }
 }
 
+template <typename T, int NTH, typename NLT>
+__global__ void __launch_bounds__(256, 3) k_cn(CnArgs A, CnPtrs<T, NLT> P)
+{
+    cn_body<T, NTH, NLT>(A, P, blockIdx.x);
+}
+
+// Ensemble members sharing NX, NZ (so ps): grid (blocks, members).
+template <typename T, int NTH, typename NLT>
+__global__ void __launch_bounds__(256, 3) k_cn_batch(const ColDesc<T, NLT> *__restrict__ d)
+{
+    // references into the descriptor (uniform loads), not local copies that
+    // would live in scratch
+    const ColDesc<T, NLT> &D = d[blockIdx.y];
+    cn_body<T, NTH, NLT>(D.A, D.P, blockIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // Poisson factorisation for REM_DIV_CHAN, once per context: the reference's
 // THOMAS_FORWARD_COMPLEX_MPI elimination of the matrix (for/channel.f:1126-1134,
@@ -236,10 +252,10 @@ __global__ void k_pois_factor(Geo g, const double *__restrict__ dy, const double
 // k_remdiv
 // ---------------------------------------------------------------------------
 template <typename T, typename NLT>
-__global__ void __launch_bounds__(64) k_remdiv(RdArgs A, CnPtrs<T, NLT> P)
+__device__ __forceinline__ void remdiv_body(const RdArgs &A, const CnPtrs<T, NLT> &P, int bx)
 {
     const Geo &g = A.g;
-    const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long m = (long)bx * blockDim.x + threadIdx.x;
     if (m >= g.ps) return;
     const long gm = A.map ? (long)A.map[m] : m;  // mode of this lane
     const int s = (int)(gm / g.mk), i = (int)(gm - (long)s * g.mk);
@@ -342,9 +358,46 @@ __global__ void __launch_bounds__(64) k_remdiv(RdArgs A, CnPtrs<T, NLT> P)
     }
 }
 
+template <typename T, typename NLT>
+__global__ void __launch_bounds__(64) k_remdiv(RdArgs A, CnPtrs<T, NLT> P)
+{
+    remdiv_body<T, NLT>(A, P, blockIdx.x);
+}
+
+template <typename T, typename NLT>
+__global__ void __launch_bounds__(64) k_remdiv_batch(const ColDesc<T, NLT> *__restrict__ d)
+{
+    const ColDesc<T, NLT> &D = d[blockIdx.y];
+    remdiv_body<T, NLT>(D.R, D.P, blockIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+template <typename T, typename NLT>
+void launch_cn_batch(hipStream_t s, const Geo &g, const ColDesc<T, NLT> *d, int nmem)
+{
+    const long blocks = (g.ps + 63) / 64;
+    if (g.nth == 0)
+        hipLaunchKernelGGL((k_cn_batch<T, 0, NLT>), dim3(blocks, nmem), dim3(256), 0, s, d);
+    else
+        hipLaunchKernelGGL((k_cn_batch<T, 1, NLT>), dim3(blocks, nmem), dim3(256), 0, s, d);
+}
+
+template <typename T, typename NLT>
+void launch_remdiv_batch(hipStream_t s, const Geo &g, const ColDesc<T, NLT> *d, int nmem)
+{
+    const long blocks = (g.ps + 63) / 64;
+    hipLaunchKernelGGL((k_remdiv_batch<T, NLT>), dim3(blocks, nmem), dim3(64), 0, s, d);
+}
+
+template void launch_cn_batch<double, double>(hipStream_t, const Geo &, const ColDesc<double, double> *, int);
+template void launch_cn_batch<double, float>(hipStream_t, const Geo &, const ColDesc<double, float> *, int);
+template void launch_cn_batch<float, float>(hipStream_t, const Geo &, const ColDesc<float, float> *, int);
+template void launch_remdiv_batch<double, double>(hipStream_t, const Geo &, const ColDesc<double, double> *, int);
+template void launch_remdiv_batch<double, float>(hipStream_t, const Geo &, const ColDesc<double, float> *, int);
+template void launch_remdiv_batch<float, float>(hipStream_t, const Geo &, const ColDesc<float, float> *, int);
+
 template <typename T, typename NLT>
 void launch_cn(hipStream_t s, const CnArgs &a, const CnPtrs<T, NLT> &p)
 {
@@ -435,6 +488,30 @@ void launch_side_scatter(hipStream_t s, const cx<TS> *side, long side_fs, cx<TF>
     hipLaunchKernelGGL((k_side<TS, TF, false>), dim3(grid1(n), nf), dim3(256), 0, s, nullptr, full, full_fs, full_ps,
                        side, nullptr, side_fs, map, nside, planes);
 }
+
+template <typename TS, typename TF>
+__global__ void k_side_scatter_batch(const ScatDesc<TS, TF> *__restrict__ d, int nside, long ps)
+{
+    const ScatDesc<TS, TF> &D = d[blockIdx.z];
+    const int f = blockIdx.y;
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < (long)nside * D.planes;
+         k += (long)gridDim.x * blockDim.x) {
+        const int j = (int)(k / nside), l = (int)(k - (long)j * nside);
+        const cx<TS> v = D.side[f * D.side_fs + (long)j * nside + l];
+        D.full[f * D.full_fs + (long)j * ps + D.map[l]] = make_cx<TF>(TF(v.re), TF(v.im));
+    }
+}
+
+template <typename TS, typename TF>
+void launch_side_scatter_batch(hipStream_t s, const ScatDesc<TS, TF> *d, int nmem, int nside, long ps,
+                               int max_planes, int nf)
+{
+    const long n = (long)nside * max_planes;
+    hipLaunchKernelGGL((k_side_scatter_batch<TS, TF>), dim3(grid1(n), nf, nmem), dim3(256), 0, s, d, nside, ps);
+}
+
+template void launch_side_scatter_batch<double, float>(hipStream_t, const ScatDesc<double, float> *, int, int, long,
+                                                       int, int);
 
 template void launch_convert<float, double>(hipStream_t, const cx<double> *, cx<float> *, long);
 template void launch_convert<double, float>(hipStream_t, const cx<float> *, cx<double> *, long);

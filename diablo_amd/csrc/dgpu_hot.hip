@@ -49,21 +49,21 @@ struct XCount {
 // grid (column chunks, planes, fields), block W*T, LDS W*NZ values.
 // -------------------------------------------------------------------------
 template <typename T, int E, typename TIN>
-__global__ void __launch_bounds__(256) k_zinv_ip(IpLen fz, const cx<T> *__restrict__ tw,
-                                                  const cx<TIN> *__restrict__ in, long in_fs, long in_ps, int mk,
-                                                  cx<T> *__restrict__ out, long out_fs, long out_ps, int ld, int j0,
-                                                  int nkz, int tnkz, int lw)
+__device__ __forceinline__ void zinv_body(const IpLen &fz, const cx<T> *__restrict__ tw,
+                                          const cx<TIN> *__restrict__ in, long in_fs, long in_ps, int mk,
+                                          cx<T> *__restrict__ out, long out_fs, long out_ps, int ld, int j0, int nkz,
+                                          int tnkz, int lw, int bx, int by, int bz)
 {
     extern __shared__ unsigned char smem_raw[];
     cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
     const int W = 1 << lw;
     const int tid = threadIdx.x;
     const int b = tid & (W - 1), th = tid >> lw;
-    const int i = blockIdx.x * W + b;
+    const int i = bx * W + b;
     const bool col = i < mk;
-    const int j = j0 + blockIdx.y;
-    const cx<TIN> *src = in + blockIdx.z * in_fs + (long)j * in_ps;
-    cx<T> *dst = out + blockIdx.z * out_fs + (long)j * out_ps;
+    const int j = j0 + by;
+    const cx<TIN> *src = in + bz * in_fs + (long)j * in_ps;
+    cx<T> *dst = out + bz * out_fs + (long)j * out_ps;
     const int nz = fz.n, S = fz.nstages;
     cx<T> *x = lds + b;
     with_radix<E>(fz.radix[0], [&](auto rc) {
@@ -94,6 +94,30 @@ __global__ void __launch_bounds__(256) k_zinv_ip(IpLen fz, const cx<T> *__restri
     });
 }
 
+template <typename T, int E, typename TIN>
+__global__ void __launch_bounds__(256) k_zinv_ip(IpLen fz, const cx<T> *__restrict__ tw,
+                                                  const cx<TIN> *__restrict__ in, long in_fs, long in_ps, int mk,
+                                                  cx<T> *__restrict__ out, long out_fs, long out_ps, int ld, int j0,
+                                                  int nkz, int tnkz, int lw)
+{
+    zinv_body<T, E, TIN>(fz, tw, in, in_fs, in_ps, mk, out, out_fs, out_ps, ld, j0, nkz, tnkz, lw, blockIdx.x,
+                         blockIdx.y, blockIdx.z);
+}
+
+// Members of one transform-length group: grid.z = member * nfields + field,
+// planes beyond a member's own NY+2 exit at once.
+template <typename T, int E, typename TIN>
+__global__ void __launch_bounds__(256) k_zinv_ip_batch(const ZinvDesc<T, TIN> *__restrict__ d, IpLen fz,
+                                                        const cx<T> *__restrict__ tw, long in_ps, int mk,
+                                                        long out_ps, int ld, int nkz, int tnkz, int lw, int nfields)
+{
+    const int mem = blockIdx.z / nfields, f = blockIdx.z - mem * nfields;
+    const ZinvDesc<T, TIN> D = d[mem];
+    if ((int)blockIdx.y >= D.nplanes) return;
+    zinv_body<T, E, TIN>(fz, tw, D.in, D.in_fs, in_ps, mk, D.out, D.out_fs, out_ps, ld, 0, nkz, tnkz, lw,
+                         blockIdx.x, blockIdx.y, f);
+}
+
 // -------------------------------------------------------------------------
 // xfused: one z row per block, planes ja..jb (a chunk of the sweep).
 //
@@ -112,11 +136,12 @@ __global__ void __launch_bounds__(256) k_zinv_ip(IpLen fz, const cx<T> *__restri
 // the same permutation for every row.
 // -------------------------------------------------------------------------
 template <typename T, int E, int NTH>
-__global__ void __launch_bounds__(512) k_xfused_ip(IpLen fx, const cx<T> *__restrict__ twg, int mk, int ld,
-                                                    const cx<T> *__restrict__ zt, long zt_fs, long zt_ps,
-                                                    cx<T> *__restrict__ wt, long wt_fs, long wt_ps,
-                                                    const double *__restrict__ dy, const double *__restrict__ dyf,
-                                                    const double *__restrict__ kxv, int jlo, int jhi, int chunk)
+__device__ __forceinline__ void xfused_body(const IpLen &fx, const cx<T> *__restrict__ twg, int mk, int ld,
+                                            const cx<T> *__restrict__ zt, long zt_fs, long zt_ps,
+                                            cx<T> *__restrict__ wt, long wt_fs, long wt_ps,
+                                            const double *__restrict__ dy, const double *__restrict__ dyf,
+                                            const double *__restrict__ kxv, int jlo, int jhi, int chunk, int bx,
+                                            int by)
 {
     using C = XCount<NTH>;
     constexpr int npi = C::npi, npo = C::npo;
@@ -131,8 +156,8 @@ __global__ void __launch_bounds__(512) k_xfused_ip(IpLen fx, const cx<T> *__rest
     const int TT = fx.t, S = fx.nstages, L0 = fx.span[0];
     const int p = tid / TT;
     int th = tid - p * TT;  // re-laundered every plane (see the loop)
-    const long zrow = (long)blockIdx.x * ld;
-    const int ja = jlo + blockIdx.y * chunk;
+    const long zrow = (long)bx * ld;
+    const int ja = jlo + by * chunk;
     const int jb = min(jhi, ja + chunk - 1);
 
     // Spectral input of stage 0's two groups (k = th, th + T; frequencies
@@ -297,26 +322,49 @@ __global__ void __launch_bounds__(512) k_xfused_ip(IpLen fx, const cx<T> *__rest
     }
 }
 
+template <typename T, int E, int NTH>
+__global__ void __launch_bounds__(512) k_xfused_ip(IpLen fx, const cx<T> *__restrict__ twg, int mk, int ld,
+                                                    const cx<T> *__restrict__ zt, long zt_fs, long zt_ps,
+                                                    cx<T> *__restrict__ wt, long wt_fs, long wt_ps,
+                                                    const double *__restrict__ dy, const double *__restrict__ dyf,
+                                                    const double *__restrict__ kxv, int jlo, int jhi, int chunk)
+{
+    xfused_body<T, E, NTH>(fx, twg, mk, ld, zt, zt_fs, zt_ps, wt, wt_fs, wt_ps, dy, dyf, kxv, jlo, jhi, chunk,
+                           blockIdx.x, blockIdx.y);
+}
+
+// grid (NZ, chunks of the tallest member, members)
+template <typename T, int E, int NTH>
+__global__ void __launch_bounds__(512) k_xfused_ip_batch(const XfDesc<T> *__restrict__ d, IpLen fx,
+                                                          const cx<T> *__restrict__ twg, int mk, int ld, long ps,
+                                                          int chunk)
+{
+    const XfDesc<T> D = d[blockIdx.z];
+    if ((int)blockIdx.y * chunk > D.jhi - D.jlo) return;
+    xfused_body<T, E, NTH>(fx, twg, mk, ld, D.zt, D.zt_fs, ps, D.wt, D.wt_fs, ps, D.dy, D.dyf, D.kx, D.jlo, D.jhi,
+                           chunk, blockIdx.x, blockIdx.y);
+}
+
 // -------------------------------------------------------------------------
 // zfwd_nl: W columns per block; both transforms of output o (G then H) in the
 // same LDS; G's result stays in registers, H's last stage combines and writes
 // the kept slots.  grid (column chunks, planes, outputs), block W*T.
 // -------------------------------------------------------------------------
 template <typename T, int E, typename TOUT>
-__global__ void __launch_bounds__(256) k_zfwd_nl_ip(IpLen fz, const cx<T> *__restrict__ tw,
-                                                     const cx<T> *__restrict__ wt, long wt_fs, long wt_ps, int ld,
-                                                     int mk, cx<TOUT> *const *__restrict__ nl, long ps,
-                                                     const double *__restrict__ kzv, int jlo, int nkz, int tnkz, int lw)
+__device__ __forceinline__ void zfwd_body(const IpLen &fz, const cx<T> *__restrict__ tw,
+                                          const cx<T> *__restrict__ wt, long wt_fs, long wt_ps, int ld, int mk,
+                                          cx<TOUT> *const *__restrict__ nl, long ps, const double *__restrict__ kzv,
+                                          int jlo, int nkz, int tnkz, int lw, int bx, int by, int bz)
 {
     extern __shared__ unsigned char smem_raw[];
     cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
     const int W = 1 << lw;
     const int tid = threadIdx.x;
     const int b = tid & (W - 1), th = tid >> lw;
-    const int i = blockIdx.x * W + b;
+    const int i = bx * W + b;
     const bool col = i < mk;
-    const int j = jlo + blockIdx.y;
-    const int o = blockIdx.z;
+    const int j = jlo + by;
+    const int o = bz;
     const int nz = fz.n, S = fz.nstages;
     const cx<T> *sg = wt + (2 * o) * wt_fs + (long)j * wt_ps + (col ? i : mk - 1);
     const cx<T> *sh = sg + wt_fs;
@@ -379,6 +427,29 @@ __global__ void __launch_bounds__(256) k_zfwd_nl_ip(IpLen fz, const cx<T> *__res
                 }
             });
     });
+}
+
+template <typename T, int E, typename TOUT>
+__global__ void __launch_bounds__(256) k_zfwd_nl_ip(IpLen fz, const cx<T> *__restrict__ tw,
+                                                     const cx<T> *__restrict__ wt, long wt_fs, long wt_ps, int ld,
+                                                     int mk, cx<TOUT> *const *__restrict__ nl, long ps,
+                                                     const double *__restrict__ kzv, int jlo, int nkz, int tnkz, int lw)
+{
+    zfwd_body<T, E, TOUT>(fz, tw, wt, wt_fs, wt_ps, ld, mk, nl, ps, kzv, jlo, nkz, tnkz, lw, blockIdx.x, blockIdx.y,
+                          blockIdx.z);
+}
+
+// grid (column chunks, planes of the tallest member, members * nout)
+template <typename T, int E, typename TOUT>
+__global__ void __launch_bounds__(256) k_zfwd_nl_ip_batch(const ZfDesc<T, TOUT> *__restrict__ d, IpLen fz,
+                                                           const cx<T> *__restrict__ tw, long wt_ps, int ld, int mk,
+                                                           long ps, int nkz, int tnkz, int lw, int nout)
+{
+    const int mem = blockIdx.z / nout, o = blockIdx.z - mem * nout;
+    const ZfDesc<T, TOUT> D = d[mem];
+    if ((int)blockIdx.y >= D.nplanes) return;
+    zfwd_body<T, E, TOUT>(fz, tw, D.wt, D.wt_fs, wt_ps, ld, mk, D.nl, ps, D.kz, D.jlo, nkz, tnkz, lw, blockIdx.x,
+                          blockIdx.y, o);
 }
 
 // -------------------------------------------------------------------------
@@ -495,6 +566,100 @@ void launch_zfwd_nl_ip(hipStream_t s, const Geo &g, const IpLen &fz, const cx<T>
 {
     DGPU_DISPATCH_IPE(fz.e, (zfwd_nl_ip_e<T, EE, TOUT>(s, g, fz, tw, wt, wt_fs, ld, nl, kz, jlo, jhi, nout)));
 }
+
+// ---- batched (ensemble) launchers ----
+template <typename T, int E, typename TIN>
+static void zinv_ip_batch_e(hipStream_t s, const Geo &g, const IpLen &fz, const cx<T> *tw,
+                            const ZinvDesc<T, TIN> *d, int nmem, int max_planes, int ld, int nfields)
+{
+    const int lw = z_lw(fz.t), W = 1 << lw;
+    dim3 grid((g.mk + W - 1) / W, max_planes, nfields * nmem);
+    const size_t lds = sizeof(cx<T>) * W * fz.n;
+    hipLaunchKernelGGL((k_zinv_ip_batch<T, E, TIN>), grid, dim3(W * fz.t), lds, s, d, fz, tw, g.ps, g.mk,
+                       (long)g.nz * ld, ld, g.nkz, g.tnkz, lw, nfields);
+}
+
+template <typename T, typename TIN>
+void launch_zinv_ip_batch(hipStream_t s, const Geo &g, const IpLen &fz, const cx<T> *tw, const ZinvDesc<T, TIN> *d,
+                          int nmem, int max_planes, int ld, int nfields)
+{
+    DGPU_DISPATCH_IPE(fz.e, (zinv_ip_batch_e<T, EE, TIN>(s, g, fz, tw, d, nmem, max_planes, ld, nfields)));
+}
+
+template <typename T, int E, int NTH>
+static void xfused_ip_batch_e(hipStream_t s, const Geo &g, const IpLen &fx, const cx<T> *tw, const XfDesc<T> *d,
+                              int nmem, int max_span, int ld, int chunk)
+{
+    using C = XCount<NTH>;
+    const int threads = fx.t * (C::npi + C::npo);
+    const size_t lds = sizeof(cx<T>) * ((size_t)fx.n * (3 * C::npi + C::npo) + fx.twn);
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_xfused_ip_batch<T, E, NTH>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        attr = true;
+    }
+    dim3 grid(g.nz, (max_span + chunk - 1) / chunk, nmem);
+    hipLaunchKernelGGL((k_xfused_ip_batch<T, E, NTH>), grid, dim3(threads), lds, s, d, fx, tw, g.mk, ld,
+                       (long)g.nz * ld, chunk);
+}
+
+template <typename T>
+void launch_xfused_ip_batch(hipStream_t s, const Geo &g, const IpLen &fx, const cx<T> *tw, const XfDesc<T> *d,
+                            int nmem, int max_span, int ld, int chunk)
+{
+#define DGPU_XFB(EE_)                                                                               \
+    do {                                                                                            \
+        if (g.nth == 0)                                                                             \
+            xfused_ip_batch_e<T, EE_, 0>(s, g, fx, tw, d, nmem, max_span, ld, chunk);               \
+        else                                                                                        \
+            xfused_ip_batch_e<T, EE_, 1>(s, g, fx, tw, d, nmem, max_span, ld, chunk);               \
+    } while (0)
+    switch (fx.e) {
+    case 4: DGPU_XFB(4); break;
+    case 6: DGPU_XFB(6); break;
+    case 8: DGPU_XFB(8); break;
+    case 12: DGPU_XFB(12); break;
+    case 16:
+        if constexpr (sizeof(T) == 4) DGPU_XFB(16);
+        break;
+    default: break;
+    }
+#undef DGPU_XFB
+}
+
+template <typename T, int E, typename TOUT>
+static void zfwd_nl_ip_batch_e(hipStream_t s, const Geo &g, const IpLen &fz, const cx<T> *tw,
+                               const ZfDesc<T, TOUT> *d, int nmem, int max_planes, int ld, int nout)
+{
+    const int lw = z_lw(fz.t), W = 1 << lw;
+    dim3 grid((g.mk + W - 1) / W, max_planes, nout * nmem);
+    const size_t lds = sizeof(cx<T>) * W * fz.n;
+    hipLaunchKernelGGL((k_zfwd_nl_ip_batch<T, E, TOUT>), grid, dim3(W * fz.t), lds, s, d, fz, tw,
+                       (long)g.nz * ld, ld, g.mk, g.ps, g.nkz, g.tnkz, lw, nout);
+}
+
+template <typename T, typename TOUT>
+void launch_zfwd_nl_ip_batch(hipStream_t s, const Geo &g, const IpLen &fz, const cx<T> *tw, const ZfDesc<T, TOUT> *d,
+                             int nmem, int max_planes, int ld, int nout)
+{
+    DGPU_DISPATCH_IPE(fz.e, (zfwd_nl_ip_batch_e<T, EE, TOUT>(s, g, fz, tw, d, nmem, max_planes, ld, nout)));
+}
+
+template void launch_zinv_ip_batch<double, double>(hipStream_t, const Geo &, const IpLen &, const cx<double> *,
+                                                   const ZinvDesc<double, double> *, int, int, int, int);
+template void launch_zinv_ip_batch<float, double>(hipStream_t, const Geo &, const IpLen &, const cx<float> *,
+                                                  const ZinvDesc<float, double> *, int, int, int, int);
+template void launch_zinv_ip_batch<float, float>(hipStream_t, const Geo &, const IpLen &, const cx<float> *,
+                                                 const ZinvDesc<float, float> *, int, int, int, int);
+template void launch_xfused_ip_batch<double>(hipStream_t, const Geo &, const IpLen &, const cx<double> *,
+                                             const XfDesc<double> *, int, int, int, int);
+template void launch_xfused_ip_batch<float>(hipStream_t, const Geo &, const IpLen &, const cx<float> *,
+                                            const XfDesc<float> *, int, int, int, int);
+template void launch_zfwd_nl_ip_batch<double, double>(hipStream_t, const Geo &, const IpLen &, const cx<double> *,
+                                                      const ZfDesc<double, double> *, int, int, int, int);
+template void launch_zfwd_nl_ip_batch<float, float>(hipStream_t, const Geo &, const IpLen &, const cx<float> *,
+                                                    const ZfDesc<float, float> *, int, int, int, int);
 
 template void launch_zinv_ip<double, double>(hipStream_t, const Geo &, const IpLen &, const cx<double> *,
                                              const cx<double> *, long, long, cx<double> *, long, long, int, int, int,

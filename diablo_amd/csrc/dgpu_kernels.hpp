@@ -73,12 +73,64 @@ struct RdArgs {
     long fac_ps;
 };
 
+// Column kernels of one ensemble member (batched launches: g.ps and the
+// block shape are shared by the members of a launch; A.g.ps is the lane count).
+template <typename T, typename NLT>
+struct ColDesc {
+    CnArgs A;
+    RdArgs R;
+    CnPtrs<T, NLT> P;
+};
+// Low-mode columns -> full state, per member (batched scatter).
+template <typename TS, typename TF>
+struct ScatDesc {
+    const cx<TS> *side;
+    cx<TF> *full;
+    long side_fs, full_fs;
+    const int *map;
+    int planes;
+};
+template <typename TS, typename TF>
+void launch_side_scatter_batch(hipStream_t s, const ScatDesc<TS, TF> *d, int nmem, int nside, long ps,
+                               int max_planes, int nf);
+template <typename T, typename NLT>
+void launch_cn_batch(hipStream_t s, const Geo &g, const ColDesc<T, NLT> *d, int nmem);
+template <typename T, typename NLT>
+void launch_remdiv_batch(hipStream_t s, const Geo &g, const ColDesc<T, NLT> *d, int nmem);
+
 // ---- launchers (dgpu_kernels.hip) ----
 template <typename T, typename TIN>
 void launch_zinv(hipStream_t s, const Geo &g, const Fft<T> &fz, const cx<TIN> *in, long in_fs, long in_ps,
                  int in_mk, int ncols, cx<T> *out, long out_fs, long out_ps, int out_mk, int j0, int nplanes,
                  int nfields);
 
+
+// ---- per-member descriptors of the batched (ensemble) launches ----
+// Members of one launch share the transform lengths (NX, NZ); NY, the
+// parameters and every array differ.
+template <typename T, typename TIN>
+struct ZinvDesc {
+    const cx<TIN> *in;   // state (fields at in_fs)
+    cx<T> *out;          // Zt (fields at out_fs)
+    long in_fs, out_fs;
+    int nplanes;         // NY+2
+};
+template <typename T>
+struct XfDesc {
+    const cx<T> *zt;
+    cx<T> *wt;
+    long zt_fs, wt_fs;
+    const double *dy, *dyf, *kx;
+    int jlo, jhi;
+};
+template <typename T, typename TOUT>
+struct ZfDesc {
+    const cx<T> *wt;
+    long wt_fs;
+    cx<TOUT> *const *nl;  // device array of the member's nonlinear-term slots
+    const double *kz;
+    int jlo, nplanes;
+};
 
 // ---- substep transforms on the in-place plans (dgpu_hot.hip) ----
 // Zt/Wt rows are ld complex values long; tw points at the plan's twiddles.
@@ -92,6 +144,15 @@ void launch_xfused_ip(hipStream_t s, const Geo &g, const IpLen &fx, const cx<T> 
 template <typename T, typename TOUT>
 void launch_zfwd_nl_ip(hipStream_t s, const Geo &g, const IpLen &fz, const cx<T> *tw, const cx<T> *wt, long wt_fs,
                        int ld, cx<TOUT> *const *nl, const double *kz, int jlo, int jhi, int nout);
+template <typename T, typename TIN>
+void launch_zinv_ip_batch(hipStream_t s, const Geo &g, const IpLen &fz, const cx<T> *tw, const ZinvDesc<T, TIN> *d,
+                          int nmem, int max_planes, int ld, int nfields);
+template <typename T>
+void launch_xfused_ip_batch(hipStream_t s, const Geo &g, const IpLen &fx, const cx<T> *tw, const XfDesc<T> *d,
+                            int nmem, int max_span, int ld, int chunk);
+template <typename T, typename TOUT>
+void launch_zfwd_nl_ip_batch(hipStream_t s, const Geo &g, const IpLen &fz, const cx<T> *tw, const ZfDesc<T, TOUT> *d,
+                             int nmem, int max_planes, int ld, int nout);
 size_t zip_lds_bytes(const IpLen &fz, size_t elem);
 size_t xfused_ip_lds_bytes(const IpLen &fx, int nth, size_t elem);
 int xfused_ip_threads(const IpLen &fx, int nth);

@@ -270,18 +270,22 @@ def test_graph_replays_match_eager(gpu_mod):
         gg.fft_xz_to_physical(w, 0, cfg.ny + 1)
 
 
-def test_ensemble_members_match_single_runs(gpu_mod):
-    """Each ensemble member (uneven grids, own parameters) advances exactly as
-    the same run alone."""
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "mixed_f64state"])
+def test_ensemble_members_match_single_runs(gpu_mod, precision):
+    """Each ensemble member (uneven grids, own parameters; members with the
+    same NX, NZ share batched launches) advances exactly as the same run
+    alone."""
     cfgs = [case.kh_case(24, 17, 24, kick=0.05, dt=0.01, ri=0.1, seed=1),
             case.kh_case(48, 33, 36, kick=0.05, dt=0.01, ri=0.2, re=500.0, seed=2),
-            case.channel_case(32, 17, 24, kick=0.05, seed=3)]
-    ens = gpu_mod.GpuEnsemble(cfgs, precision="fp64")
+            case.channel_case(32, 17, 24, kick=0.05, seed=3),
+            case.kh_case(24, 33, 24, kick=0.05, dt=0.01, ri=0.15, seed=4),
+            case.kh_case(48, 17, 36, kick=0.05, dt=0.005, ri=0.05, seed=5)]
+    ens = gpu_mod.GpuEnsemble(cfgs, precision=precision)
     singles = []
     for cfg, m in zip(cfgs, ens.members):
         phys = case.initial_physical(cfg)
         m.initialize(*phys)
-        g = gpu_mod.GpuChannel(cfg, precision="fp64")
+        g = gpu_mod.GpuChannel(cfg, precision=precision)
         g.set_graphs(False)
         g.initialize(*phys)
         singles.append(g)
