@@ -99,14 +99,17 @@ def accuracy(cfg, phys, steps: int) -> dict:
     return r
 
 
-def ensemble_run(n: int, steps: int, warmup: int, precision: str = "mixed") -> dict:
+def ensemble_run(n: int, steps: int, warmup: int, precision: str = "mixed", acc_steps: int = 0) -> dict:
     """Aggregate throughput of n independent KH runs on uneven grids advanced
-    together (one graph per RK stage, one branch per member)."""
-    from diablo_amd import case, gpu
+    together (members with the same NX, NZ share one launch per kernel kind,
+    one graph per RK stage); with acc_steps > 0 also the worst member's error
+    against an FP64 ensemble run from the same states."""
+    from diablo_amd import case, gpu, metrics
     cfgs = case.ensemble_cases(n)
     ens = gpu.GpuEnsemble(cfgs, precision=precision)
-    for cfg, m in zip(cfgs, ens.members):
-        m.initialize(*case.initial_physical(cfg))
+    phys = [case.initial_physical(cfg) for cfg in cfgs]
+    for p, m in zip(phys, ens.members):
+        m.initialize(*p)
     ens.step(warmup)
     ens.synchronize()
     t0 = time.perf_counter()
@@ -114,11 +117,32 @@ def ensemble_run(n: int, steps: int, warmup: int, precision: str = "mixed") -> d
     ens.synchronize()
     el = time.perf_counter() - t0
     pts = ens.points
+    per = [cfg.points * 3 * steps / el / 1e6 for cfg in cfgs]
+    out = {"value": round(pts * 3 * steps / el / 1e6, 2), "unit": "Mpts*substeps/s", "members": n,
+           "steps": steps, "precision": precision, "ms_per_step": round(el / steps * 1e3, 3),
+           "points_total": pts, "per_member_min": round(min(per), 2), "per_member_max": round(max(per), 2),
+           "grids": "NX=NZ in {64,96,128,192} x NY in {33,65,97,129}, Ri in [0.05,0.25], Re in {500,1000}"}
     ens.close()
-    return {"value": round(pts * 3 * steps / el / 1e6, 2), "unit": "Mpts*substeps/s", "members": n,
-            "steps": steps, "precision": precision, "ms_per_step": round(el / steps * 1e3, 3),
-            "points_total": pts,
-            "grids": "NX=NZ in {64,96,128,192} x NY in {33,65,97,129}, Ri in [0.05,0.25], Re in {500,1000}"}
+    if acc_steps > 0 and precision != "fp64":
+        e64 = gpu.GpuEnsemble(cfgs, precision="fp64")
+        emx = gpu.GpuEnsemble(cfgs, precision=precision)
+        for p, a, b in zip(phys, e64.members, emx.members):
+            a.initialize(*p)
+            b.initialize(*p)
+        e64.step(acc_steps)
+        emx.step(acc_steps)
+        e64.synchronize()
+        emx.synchronize()
+        worst = {"rel_l2": 0.0, "ke_drift": 0.0, "max_div": 0.0}
+        for a, b in zip(e64.members, emx.members):
+            r = metrics.compare(a, a.state(), b, b.state())
+            for k in worst:
+                worst[k] = max(worst[k], r[k])
+        e64.close()
+        emx.close()
+        out.update({"rel_l2_vs_fp64_worst_member": worst["rel_l2"], "ke_drift_vs_fp64_worst_member": worst["ke_drift"],
+                    "max_div_worst_member": worst["max_div"], "accuracy_steps": acc_steps})
+    return out
 
 
 def cpu_baseline(cfg, phys, threads: int) -> dict:
@@ -161,6 +185,7 @@ def main():
     ap.add_argument("--ensemble", type=int, default=64, help="members of the batched run (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=10)
     ap.add_argument("--ensemble-precision", default="mixed", choices=("mixed", "mixed_f64state", "fp64"))
+    ap.add_argument("--ensemble-accuracy-steps", type=int, default=100)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB five times (reference point for HBM counters)")
     args = ap.parse_args()
@@ -230,9 +255,10 @@ def main():
     batched = None
     if args.ensemble > 0:
         log(f"[bench] batched: {args.ensemble} KH members, {args.ensemble_precision}, {args.ensemble_steps} steps")
-        batched = ensemble_run(args.ensemble, args.ensemble_steps, 1, args.ensemble_precision)
+        batched = ensemble_run(args.ensemble, args.ensemble_steps, 1, args.ensemble_precision,
+                               args.ensemble_accuracy_steps)
         log(f"[bench] batched aggregate {batched['value']:.1f} Mpts*substeps/s "
-            f"({batched['value'] / value:.2f}x FP64 384)")
+            f"({batched['value'] / value:.2f}x FP64 384); {batched}")
 
     cpu = None
     if args.cpu_baseline == "on":

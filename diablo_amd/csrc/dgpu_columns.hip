@@ -59,8 +59,11 @@ __device__ __forceinline__ void piped_sweep(int j0, int dir, int n, Ld &&ld, Bod
 // 519 blocks of four wavefronts at 384x257x384: three waves per SIMD (<= 168
 // VGPRs) keep them all resident at once instead of leaving a second round
 // for a few CUs.
-template <typename T, int NTH, typename NLT>
-__device__ __forceinline__ void cn_body(const CnArgs &A, const CnPtrs<T, NLT> &P, int bx)
+// AT / PT: the argument structs by value (a kernel's own arguments) or as
+// references into a descriptor in global memory (batched launches); either
+// way they stay out of private memory.
+template <typename T, int NTH, typename NLT, typename AT, typename PT>
+__device__ __forceinline__ void cn_body(AT A, PT P, int bx)
 {
     const Geo &g = A.g;
     const int q = wave_uniform(threadIdx.x >> 6);  // system of this wavefront
@@ -86,6 +89,16 @@ __device__ __forceinline__ void cn_body(const CnArgs &A, const CnPtrs<T, NLT> &P
     const cx<NLT> *nl = P.nl[q];
     cx<T> *gs = P.gs[q];
     const T visc = (q == SYS_TH0) ? T(A.nupr) * pw : T(A.nu) * pw;
+    // Everything the sweeps' lambdas use, as plain locals: capturing the
+    // argument structs themselves would give them an address (a private copy).
+    const bool rk_gt1 = A.rk_gt1 != 0;
+    const int f_type = A.f_type;
+    const T i_ro = T(A.i_ro), rgx = T(A.rgx), rgz = T(A.rgz), drhodx = T(A.drhodx), drhodz = T(A.drhodz);
+    const T force00 = T(A.force00);
+    const cx<T> *const p_cp = P.cp;
+    const cx<T> *const p_cu1 = P.cu1;
+    const cx<T> *const p_cu3 = P.cu3;
+    const cx<T> *const p_cth = P.cth;
 
     // ---------------- sweep A ----------------
     // Inputs of plane j are loaded four planes ahead (piped_sweep); every
@@ -98,8 +111,8 @@ __device__ __forceinline__ void cn_body(const CnArgs &A, const CnPtrs<T, NLT> &P
         cx<T> thm = zero;              // CTH at j-1 (U2 buoyancy)
         cx<T> cpm = zero;              // CP at j-1 (U2 pressure)
         cx<T> gprev = zero;
-        const cx<T> *a1p = (q == SYS_U1) ? P.cu3 : P.cu1;   // Coriolis partner / TH advection
-        const cx<T> *a2p = (q == SYS_TH0) ? P.cu3 : P.cth;  // buoyancy / TH advection
+        const cx<T> *a1p = (q == SYS_U1) ? p_cu3 : p_cu1;   // Coriolis partner / TH advection
+        const cx<T> *a2p = (q == SYS_TH0) ? p_cu3 : p_cth;  // buoyancy / TH advection
         const bool has1 = (q != SYS_U2);
         const bool has2 = (q == SYS_TH0) || NTH;
         const bool hascp = (q != SYS_TH0);
@@ -112,10 +125,10 @@ __device__ __forceinline__ void cn_body(const CnArgs &A, const CnPtrs<T, NLT> &P
                 in.fp = fld[o + (j <= NY ? ps : 0)];
                 const cx<NLT> nv = nl[gm + (long)j * nps];
                 in.nl = make_cx<T>(T(nv.re), T(nv.im));
-                in.cf = A.rk_gt1 ? cf[o] : zero;
+                in.cf = rk_gt1 ? cf[o] : zero;
                 in.aux1 = has1 ? a1p[o] : zero;
                 in.aux2 = has2 ? a2p[o] : zero;
-                in.cp = hascp ? P.cp[o] : zero;
+                in.cp = hascp ? p_cp[o] : zero;
                 return in;
             },
             [&](int j, const InA &in) {
@@ -132,17 +145,16 @@ __device__ __forceinline__ void cn_body(const CnArgs &A, const CnPtrs<T, NLT> &P
                 // new explicit term CF (for/channel.f:175-290 + nonlinear terms)
                 cx<T> cfn = neg_scale(visc, f0);
                 if (q == SYS_U1) {
-                    cfn = fma_c(T(A.i_ro), aux1, cfn);
-                    if (NTH) cfn = fma_c(T(A.rgx), aux2, cfn);
+                    cfn = fma_c(i_ro, aux1, cfn);
+                    if (NTH) cfn = fma_c(rgx, aux2, cfn);
                 } else if (q == SYS_U3) {
-                    cfn = fma_c(-T(A.i_ro), aux1, cfn);
-                    if (NTH) cfn = fma_c(T(A.rgz), aux2, cfn);
+                    cfn = fma_c(-i_ro, aux1, cfn);
+                    if (NTH) cfn = fma_c(rgz, aux2, cfn);
                 } else if (q == SYS_U2) {
                     if (NTH && !is00) cfn = cfn + make_cx<T>(T(t[T_B0]) * aux2.re + T(t[T_BM]) * thm.re,
                                                               T(t[T_B0]) * aux2.im + T(t[T_BM]) * thm.im);
                 } else {
-                    cfn = cfn - make_cx<T>(aux1.re * T(A.drhodx) + aux2.re * T(A.drhodz),
-                                           aux1.im * T(A.drhodx) + aux2.im * T(A.drhodz));
+                    cfn = cfn - make_cx<T>(aux1.re * drhodx + aux2.re * drhodz, aux1.im * drhodx + aux2.im * drhodz);
                 }
                 cfn = cfn + nlv;
                 cx<T> vec;
@@ -150,10 +162,10 @@ __device__ __forceinline__ void cn_body(const CnArgs &A, const CnPtrs<T, NLT> &P
                     vec = is00 ? make_cx<T>(T(t[T_BC00]), 0) : zero;
                 } else {
                     cx<T> cr = f0;
-                    if (A.rk_gt1) cr = fma_c(temp3, cfo, cr);
+                    if (rk_gt1) cr = fma_c(temp3, cfo, cr);
                     if (q == SYS_U1) {
                         cr = cr - make_cx<T>(-temp4 * (kx * cp0.im), temp4 * (kx * cp0.re));
-                        if (is00 && (A.f_type == 1 || A.f_type == 2)) cr.re = cr.re - temp4 * T(A.force00);
+                        if (is00 && (f_type == 1 || f_type == 2)) cr.re = cr.re - temp4 * force00;
                     } else if (q == SYS_U3) {
                         cr = cr - make_cx<T>(-temp4 * (kz * cp0.im), temp4 * (kz * cp0.re));
                     } else if (q == SYS_U2) {
@@ -198,7 +210,7 @@ __device__ __forceinline__ void cn_body(const CnArgs &A, const CnPtrs<T, NLT> &P
 template <typename T, int NTH, typename NLT>
 __global__ void __launch_bounds__(256, 3) k_cn(CnArgs A, CnPtrs<T, NLT> P)
 {
-    cn_body<T, NTH, NLT>(A, P, blockIdx.x);
+    cn_body<T, NTH, NLT, const CnArgs, const CnPtrs<T, NLT>>(A, P, blockIdx.x);
 }
 
 // Ensemble members sharing NX, NZ (so ps): grid (blocks, members).
@@ -208,7 +220,7 @@ __global__ void __launch_bounds__(256, 3) k_cn_batch(const ColDesc<T, NLT> *__re
     // references into the descriptor (uniform loads), not local copies that
     // would live in scratch
     const ColDesc<T, NLT> &D = d[blockIdx.y];
-    cn_body<T, NTH, NLT>(D.A, D.P, blockIdx.x);
+    cn_body<T, NTH, NLT, const CnArgs &, const CnPtrs<T, NLT> &>(D.A, D.P, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
