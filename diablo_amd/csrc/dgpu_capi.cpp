@@ -84,6 +84,7 @@ struct dgpu_ctx {
     double *d_fort = nullptr;  // Fortran-layout staging for host entry points
     double *d_fa = nullptr, *d_fb = nullptr;  // POISSON_P_CHAN work planes
     double *d_gyf = nullptr;
+    double *d_stats = nullptr;  // dgpu_stats_chan output
     // CN tables: [stage 0..2][sys][NY+2][kRow]
     double *d_cn = nullptr;
     std::vector<double> h_cn;
@@ -705,7 +706,7 @@ int32_t dgpu_destroy(dgpu_ctx *c)
     void *bufs[] = {c->d_vec, c->d_tw, c->d_state, c->d_pmult, c->d_pinv, c->d_rd, c->d_nlptr, c->d_zt, c->d_wt,
                     c->d_fa, c->d_fb, c->d_gyf, c->d_iptw, c->d_iptwf, c->d_ztf, c->d_wtf, c->d_nlf, c->d_nlptrf,
                     c->d_fort, c->d_cn, c->d_codes, c->d_c1, c->d_state32, c->d_side, c->d_map,
-                    c->d_pmultf, c->d_pinvf, c->d_gs32};
+                    c->d_pmultf, c->d_pinvf, c->d_gs32, c->d_stats};
     for (void *b : bufs)
         if (b) hipFree(b);
     if (c->side_stream) {
@@ -716,6 +717,30 @@ int32_t dgpu_destroy(dgpu_ctx *c)
     if (c->ev_join) hipEventDestroy(c->ev_join);
     if (c->stream && c->own_stream) hipStreamDestroy(c->stream);
     delete c;
+    return DGPU_OK;
+}
+
+int32_t dgpu_stats_chan(dgpu_ctx *c, double *profiles, double *ubulk)
+{
+    if (!c || !profiles || !ubulk) return DGPU_E_ARG;
+    if (int rc = to_master(c)) return rc;
+    const dgpu::Geo &g = c->g;
+    if (!c->d_gyf) {
+        HIPCHK(c, hipMalloc(&c->d_gyf, sizeof(double) * (g.ny + 2)));
+        HIPCHK(c, hipMemcpy(c->d_gyf, c->gyf.data(), sizeof(double) * (g.ny + 2), hipMemcpyHostToDevice));
+    }
+    if (!c->d_stats) HIPCHK(c, hipMalloc(&c->d_stats, sizeof(double) * DGPU_NSTAT * g.ny));
+    dgpu::launch_stats(c->stream, g, c->cu[0], c->cu[1], c->cu[2], c->cp, g.nth ? c->cth[0] : nullptr, c->d_dy,
+                       c->d_dyf, c->d_gyf, c->d_kx, c->d_kz, c->d_stats);
+    if (int rc = check_launch(c, "stats")) return rc;
+    HIPCHK(c, hipMemcpyAsync(profiles, c->d_stats, sizeof(double) * DGPU_NSTAT * g.ny, hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    // INTEGRATE_Y_VAR (for/mpi.f:1676-1698) of ume
+    const double *ume = profiles + (long)DGPU_ST_UME * g.ny;  // ume(J) at index J-1
+    double res = 0.;
+    for (int j = 2; j <= g.ny; j++) res = res + 0.5 * (ume[j - 1] + ume[j - 2]) * c->dy[j];
+    *ubulk = res / c->p.ly;
     return DGPU_OK;
 }
 

@@ -184,6 +184,10 @@ void launch_pp_accum(hipStream_t s, const Geo &g, cx<double> *cs, const double *
 void launch_pp_solve(hipStream_t s, const Geo &g, const cx<double> *cs, const cx<double> *cth, double ri,
                      const double *gyf, const double *pmult, const double *pinv, const double *rd, cx<double> *scr,
                      cx<double> *cp);
+// SAVE_STATS_CHAN statistics (dgpu_stats.hip): out[q * NY + (j - 1)]
+void launch_stats(hipStream_t s, const Geo &g, const cx<double> *cu1, const cx<double> *cu2, const cx<double> *cu3,
+                  const cx<double> *cp, const cx<double> *cth, const double *dy, const double *dyf,
+                  const double *gyf, const double *kx, const double *kz, double *out);
 void launch_unpack_fortran(hipStream_t s, const Geo &g, const double *fort, cx<double> *kept, int j0, int nplanes);
 // Mixed path state copies (dgpu_columns.hip): dst[k] = src[k], k < n.
 template <typename TO, typename FROM>

@@ -56,8 +56,12 @@ EXPORTS = (
     "dgpu_set_kernel_timing", "dgpu_kernel_times", "dgpu_set_graphs",
     "dgpu_ensemble_create", "dgpu_ensemble_destroy", "dgpu_ensemble_size", "dgpu_ensemble_member",
     "dgpu_ensemble_last_error", "dgpu_ensemble_set_graphs", "dgpu_ensemble_rk_chan_1",
-    "dgpu_ensemble_synchronize",
+    "dgpu_ensemble_synchronize", "dgpu_stats_chan",
 )
+
+# dgpu_stats_chan profiles, in the order of the header's DGPU_ST_*
+STATS = ("ume", "vme", "wme", "urms", "vrms", "wrms", "uv", "uw", "wv", "dudy", "dwdy", "cp", "shear",
+         "omega_x", "omega_y", "omega_z", "thme", "thrms", "thv", "dthdy", "pe_diss")
 
 _lib = None
 
@@ -112,6 +116,8 @@ def load_library(path: str = LIB_PATH):
     L.dgpu_set_kernel_timing.argtypes = [vp, i32]
     L.dgpu_kernel_times.restype = i32
     L.dgpu_kernel_times.argtypes = [vp, DP, C.POINTER(C.c_int64), i32]
+    L.dgpu_stats_chan.restype = i32
+    L.dgpu_stats_chan.argtypes = [vp, DP, DP]
     _lib = L
     return L
 
@@ -249,6 +255,15 @@ class GpuChannel:
 
     def apply_bc_vel(self):
         self._chk(self.L.dgpu_apply_bc_vel(self.h), "dgpu_apply_bc_vel")
+
+    def stats_chan(self) -> tuple[dict, float]:
+        """SAVE_STATS_CHAN's statistics from the device state:
+        {name: profile at J = 1..NY}, UBULK."""
+        ny = self.cfg.ny
+        out = np.zeros(len(STATS) * ny)
+        ub = np.zeros(1)
+        self._chk(self.L.dgpu_stats_chan(self.h, _dp(out), _dp(ub)), "dgpu_stats_chan")
+        return {k: out[q * ny:(q + 1) * ny].copy() for q, k in enumerate(STATS)}, float(ub[0])
 
     def fft_xz_to_fourier(self, a: np.ndarray, jmin: int, jmax: int):
         self._chk(self.L.dgpu_fft_xz_to_fourier(self.h, _dp(a), jmin, jmax), "dgpu_fft_xz_to_fourier")

@@ -57,6 +57,17 @@ enum dgpu_status {
  * state and every solve stay FP64. */
 enum dgpu_precision { DGPU_FP64 = 0, DGPU_MIXED = 1, DGPU_MIXED_F64STATE = 2 };
 
+/* Profiles of dgpu_stats_chan, in this order (the quantities SAVE_STATS_CHAN
+ * writes to mean.h5 / mean.txt, for/save_stats.f:350-430, and its scalar
+ * part, for/save_stats.f:420-522, for scalar 1). */
+enum dgpu_stat {
+    DGPU_ST_UME = 0, DGPU_ST_VME, DGPU_ST_WME, DGPU_ST_URMS, DGPU_ST_VRMS, DGPU_ST_WRMS,
+    DGPU_ST_UV, DGPU_ST_UW, DGPU_ST_WV, DGPU_ST_DUDY, DGPU_ST_DWDY, DGPU_ST_CP, DGPU_ST_SHEAR,
+    DGPU_ST_OMEGA_X, DGPU_ST_OMEGA_Y, DGPU_ST_OMEGA_Z,
+    DGPU_ST_THME, DGPU_ST_THRMS, DGPU_ST_THV, DGPU_ST_DTHDY, DGPU_ST_PE_DISS,
+    DGPU_NSTAT
+};
+
 /* Field identifiers for dgpu_set_field / dgpu_get_field. */
 enum dgpu_field {
     DGPU_CU1 = 0, DGPU_CU2 = 1, DGPU_CU3 = 2, DGPU_CP = 3,
@@ -160,6 +171,16 @@ int32_t dgpu_fft_xz_to_physical(dgpu_ctx *ctx, double *vv, int32_t jmin, int32_t
 
 /* Waits for all work enqueued on the context. */
 int32_t dgpu_synchronize(dgpu_ctx *ctx);
+
+/* The statistics of SAVE_STATS_CHAN (for/save_stats.f:106-347, scalar part
+ * for/save_stats.f:420-522; the running means, movies and file output stay
+ * with the Fortran driver) from the device state, in FP64.  profiles:
+ * DGPU_NSTAT * NY doubles, profiles[q * NY + (J - 1)] for J = 1..NY,
+ * q = DGPU_ST_*; *ubulk: INTEGRATE_Y_VAR of ume (for/mpi.f:1676-1698).  The
+ * scalar profiles are for scalar 1 (zero when N_TH = 0).  omega_z is the
+ * z vorticity (the reference's mean.h5 branch writes omega_y under that
+ * name, for/save_stats.f:404-406). */
+int32_t dgpu_stats_chan(dgpu_ctx *ctx, double *profiles, double *ubulk);
 
 /* Kernel timing: when enabled, every kernel launch is bracketed by HIP events
  * on the context's stream.  dgpu_kernel_times returns, per kernel kind

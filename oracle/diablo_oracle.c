@@ -1271,3 +1271,146 @@ void dor_apply_bc_scalar(dor_state *s)
     }
 #undef X
 }
+
+/* ------------------------------------------------------------------------ */
+/* SAVE_STATS_CHAN statistics (for/save_stats.f:106-347, scalar part        */
+/* for/save_stats.f:420-522), non-FINAL branch at NPROCY = 1.  Physical     */
+/* fields come from FFT_XZ_TO_PHYSICAL of copies, as the reference converts */
+/* CU1-3, CS1, CTH and the CR gradients; sums run over k = 0..NZP-1,        */
+/* i = 0..NXM in the reference's order.                                     */
+/* ------------------------------------------------------------------------ */
+void dor_stats_chan(dor_state *s, double *out, double *ubulk)
+{
+    const int NX = s->NX, NY = s->NY, NZ = s->NZ, NXP = s->NXP, TNKZ = s->TNKZ, NXM = s->NXM, NZP = s->NZP;
+    const double *DY = s->DY, *DYF = s->DYF, *GYF = s->GYF, *KX = s->KX, *KZ = s->KZ;
+    const size_t fs = s->fsize;
+    double *u1 = zalloc(fs), *u2 = zalloc(fs), *u3 = zalloc(fs), *th = zalloc(fs), *w = zalloc(fs);
+    double *w2 = zalloc(fs), *w3 = zalloc(fs);
+    const cplx *CU1 = (const cplx *)s->fld[DOR_U1], *CU2 = (const cplx *)s->fld[DOR_U2];
+    const cplx *CU3 = (const cplx *)s->fld[DOR_U3], *CP = (const cplx *)s->fld[DOR_P];
+    const cplx *CTH = s->NTH > 0 ? (const cplx *)s->fld[DOR_CTH0] : NULL;
+    double *ume = zalloc(NY + 2), *vme = zalloc(NY + 2), *wme = zalloc(NY + 2), *thme = zalloc(NY + 2);
+    double *acc = zalloc((size_t)DOR_NSTAT * (NY + 2));
+#define ST(q, j) acc[(size_t)(q) * (NY + 2) + (j)]
+#define CI(i) (I * KX[i])
+#define CK(k) (I * KZ[k])
+    for (int j = 0; j <= NY + 1; j++) {   /* for/save_stats.f:146-151 */
+        ume[j] = creal(CU1[CIDX(0, 0, j)]);
+        vme[j] = creal(CU2[CIDX(0, 0, j)]);
+        wme[j] = creal(CU3[CIDX(0, 0, j)]);
+        if (CTH) thme[j] = creal(CTH[CIDX(0, 0, j)]);
+    }
+    /* INTEGRATE_Y_VAR (for/mpi.f:1676-1698) */
+    double res = 0.;
+    for (int j = 2; j <= NY; j++) res = res + 0.5 * (ume[j] + ume[j - 1]) * DY[j];
+    *ubulk = res / s->p.ly;
+    /* for/save_stats.f:163-165 */
+    memcpy(u1, s->fld[DOR_U1], fs * sizeof(double));
+    memcpy(u2, s->fld[DOR_U2], fs * sizeof(double));
+    memcpy(u3, s->fld[DOR_U3], fs * sizeof(double));
+    dor_fft_xz_to_physical(s, u1, 0, NY + 1);
+    dor_fft_xz_to_physical(s, u2, 0, NY + 1);
+    dor_fft_xz_to_physical(s, u3, 0, NY + 1);
+    const double nn = (double)NZ * (double)NX;
+    /* for/save_stats.f:172-193 */
+    for (int j = 1; j <= NY; j++) {
+        double ur = 0., vr = 0., wr = 0., uv = 0., wv = 0., uw = 0., sh = 0.;
+        for (int k = 0; k <= NZP - 1; k++)
+            for (int i = 0; i <= NXM; i++) {
+                const double du = u1[RIDX(i, k, j)] - ume[j], dw = u3[RIDX(i, k, j)] - wme[j];
+                ur = ur + pow(du, 2.);
+                vr = vr + 0.5 * (pow(u2[RIDX(i, k, j)] - vme[j], 2.) + pow(u2[RIDX(i, k, j + 1)] - vme[j + 1], 2.));
+                wr = wr + pow(dw, 2.);
+                /* for/save_stats.f:205-218 */
+                uv = uv + (DYF[j - 1] * du + DYF[j] * (u1[RIDX(i, k, j - 1)] - ume[j - 1])) / (2.0 * DY[j]) *
+                              (u2[RIDX(i, k, j)] - vme[j]);
+                wv = wv + (DYF[j - 1] * dw + DYF[j] * (u3[RIDX(i, k, j - 1)] - wme[j - 1])) / (2.0 * DY[j]) *
+                              (u2[RIDX(i, k, j)] - vme[j]);
+                uw = uw + du * dw;
+                /* for/save_stats.f:242-250 */
+                sh = sh + pow((u1[RIDX(i, k, j + 1)] - u1[RIDX(i, k, j - 1)]) / (2.0 * DYF[j]), 2.0) +
+                     pow((u3[RIDX(i, k, j + 1)] - u3[RIDX(i, k, j - 1)]) / (2.0 * DYF[j]), 2.0);
+            }
+        ST(DOR_ST_URMS, j) = sqrt(ur / nn);
+        ST(DOR_ST_VRMS, j) = sqrt(vr / nn);
+        ST(DOR_ST_WRMS, j) = sqrt(wr / nn);
+        ST(DOR_ST_UV, j) = uv / nn;
+        ST(DOR_ST_WV, j) = wv / nn;
+        ST(DOR_ST_UW, j) = uw / nn;
+        ST(DOR_ST_SHEAR, j) = sh / (double)(NX * NZ);
+        /* for/save_stats.f:235-238 */
+        ST(DOR_ST_DUDY, j) = (ume[j] - ume[j - 1]) / (GYF[j] - GYF[j - 1]);
+        ST(DOR_ST_DWDY, j) = (wme[j] - wme[j - 1]) / (GYF[j] - GYF[j - 1]);
+        ST(DOR_ST_UME, j) = ume[j];
+        ST(DOR_ST_VME, j) = vme[j];
+        ST(DOR_ST_WME, j) = wme[j];
+        ST(DOR_ST_CP, j) = creal(CP[CIDX(0, 0, j)]);
+    }
+    /* vorticity (for/save_stats.f:268-347): CS1 from the saved CR = CU */
+    for (int comp = 0; comp < 3; comp++) {
+        memset(w, 0, fs * sizeof(double));
+        cplx *CS1 = (cplx *)w;
+        for (int j = 1; j <= NY; j++)
+            for (int K = 0; K <= TNKZ; K++)
+                for (int i = 0; i <= NXP - 1; i++) {
+                    cplx v;
+                    if (comp == 0)
+                        v = (CU3[CIDX(i, K, j + 1)] - CU3[CIDX(i, K, j - 1)]) / (2.0 * DYF[j]) -
+                            CK(K) * 0.5 * (CU2[CIDX(i, K, j + 1)] + CU2[CIDX(i, K, j)]);
+                    else if (comp == 1)
+                        v = CK(K) * CU1[CIDX(i, K, j)] - CI(i) * CU3[CIDX(i, K, j)];
+                    else
+                        v = CI(i) * 0.5 * (CU2[CIDX(i, K, j + 1)] + CU2[CIDX(i, K, j)]) -
+                            (CU1[CIDX(i, K, j + 1)] - CU1[CIDX(i, K, j - 1)]) / (2.0 * DYF[j]);
+                    CS1[CIDX(i, K, j)] = v;
+                }
+        dor_fft_xz_to_physical(s, w, 0, NY + 1);
+        for (int j = 1; j <= NY; j++) {
+            double o = 0.;
+            for (int k = 0; k <= NZP - 1; k++)
+                for (int i = 0; i <= NXM; i++) o = o + pow(w[RIDX(i, k, j)], 2.0);
+            ST(DOR_ST_OMEGA_X + comp, j) = sqrt(o / ((double)NX * (double)NZ));
+        }
+    }
+    if (CTH) {
+        /* scalar 1: TH, its rms, flux with U2, mean gradient, grad TH squared */
+        memcpy(th, s->fld[DOR_CTH0], fs * sizeof(double));
+        dor_fft_xz_to_physical(s, th, 0, NY + 1);
+        memset(w, 0, fs * sizeof(double));
+        memset(w2, 0, fs * sizeof(double));
+        memset(w3, 0, fs * sizeof(double));
+        cplx *R1c = (cplx *)w, *R2c = (cplx *)w2, *R3c = (cplx *)w3;
+        for (int j = 1; j <= NY; j++)    /* for/save_stats.f:438-448 */
+            for (int K = 0; K <= TNKZ; K++)
+                for (int i = 0; i <= NXP - 1; i++) {
+                    R1c[CIDX(i, K, j)] = CI(i) * CTH[CIDX(i, K, j)];
+                    R2c[CIDX(i, K, j)] = (CTH[CIDX(i, K, j + 1)] - CTH[CIDX(i, K, j - 1)]) / (GYF[j + 1] - GYF[j - 1]);
+                    R3c[CIDX(i, K, j)] = CK(K) * CTH[CIDX(i, K, j)];
+                }
+        dor_fft_xz_to_physical(s, w, 0, NY + 1);
+        dor_fft_xz_to_physical(s, w2, 0, NY + 1);
+        dor_fft_xz_to_physical(s, w3, 0, NY + 1);
+        for (int j = 1; j <= NY; j++) {
+            double tr = 0., tv = 0., pe = 0.;
+            for (int k = 0; k <= NZP - 1; k++)
+                for (int i = 0; i <= NXM; i++) {
+                    tr = tr + pow(fabs(th[RIDX(i, k, j)] - thme[j]), 2.);
+                    tv = tv + (th[RIDX(i, k, j)] - thme[j]) *
+                                  (0.5 * (u2[RIDX(i, k, j)] + u2[RIDX(i, k, j + 1)]) - 0.5 * (vme[j] + vme[j + 1]));
+                    pe = pe + pow(w[RIDX(i, k, j)], 2.0) + pow(w2[RIDX(i, k, j)], 2.0) + pow(w3[RIDX(i, k, j)], 2.0);
+                }
+            ST(DOR_ST_THRMS, j) = sqrt(tr / nn);
+            ST(DOR_ST_THV, j) = tv / nn;
+            ST(DOR_ST_PE_DISS, j) = pe / (double)(NX * NZ);
+            ST(DOR_ST_DTHDY, j) = (thme[j] - thme[j - 1]) / (GYF[j] - GYF[j - 1]);
+            ST(DOR_ST_THME, j) = thme[j];
+        }
+    }
+    for (int q = 0; q < DOR_NSTAT; q++)
+        for (int j = 1; j <= NY; j++) out[(size_t)q * NY + (j - 1)] = ST(q, j);
+#undef ST
+#undef CI
+#undef CK
+    free(u1); free(u2); free(u3); free(th); free(w); free(w2); free(w3);
+    free(ume); free(vme); free(wme); free(thme); free(acc);
+}

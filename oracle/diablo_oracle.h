@@ -81,6 +81,19 @@ void dor_apply_bc_vel(dor_state *s);
 void dor_apply_bc_scalar(dor_state *s);
 
 /* Building blocks exposed for unit tests. */
+/* Statistics of SAVE_STATS_CHAN (for/save_stats.f:106-347 and the scalar
+ * part for/save_stats.f:420-522, non-FINAL branch, NPROCY = 1), without the
+ * running means, movies and file output: out[q * NY + (j - 1)], j = 1..NY,
+ * q = DOR_ST_*, for scalar 1; *ubulk = INTEGRATE_Y_VAR of ume
+ * (for/mpi.f:1676-1698).  The state is not modified. */
+enum {
+    DOR_ST_UME = 0, DOR_ST_VME, DOR_ST_WME, DOR_ST_URMS, DOR_ST_VRMS, DOR_ST_WRMS,
+    DOR_ST_UV, DOR_ST_UW, DOR_ST_WV, DOR_ST_DUDY, DOR_ST_DWDY, DOR_ST_CP, DOR_ST_SHEAR,
+    DOR_ST_OMEGA_X, DOR_ST_OMEGA_Y, DOR_ST_OMEGA_Z,
+    DOR_ST_THME, DOR_ST_THRMS, DOR_ST_THV, DOR_ST_DTHDY, DOR_ST_PE_DISS, DOR_NSTAT
+};
+void dor_stats_chan(dor_state *s, double *out, double *ubulk);
+
 void dor_thomas_real(double *a, double *b, double *c, double *g, int ny, int nx);
 void dor_thomas_complex(double *a, double *b, double *c, double *g_c, int ny, int nxp);
 void dor_fft_c2c(double *data, int n, int sign);   /* unnormalised, sign -1 forward */

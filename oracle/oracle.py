@@ -80,6 +80,7 @@ def lib():
         L.dor_fft_c2c.argtypes = [dp, i, i]
         L.dor_r2c.argtypes = [dp, dp, i]
         L.dor_c2r.argtypes = [dp, dp, i]
+        L.dor_stats_chan.argtypes = [vp, dp, dp]
         _lib = L
     return _lib
 
@@ -101,6 +102,10 @@ def params_of(cfg: ChannelConfig) -> _Params:
             setattr(p, name, v)
     return p
 
+
+# SAVE_STATS_CHAN statistics, in the order of diablo_oracle.h's DOR_ST_*
+STATS = ("ume", "vme", "wme", "urms", "vrms", "wrms", "uv", "uw", "wv", "dudy", "dwdy", "cp", "shear",
+         "omega_x", "omega_y", "omega_z", "thme", "thrms", "thv", "dthdy", "pe_diss")
 
 # field ids (diablo_oracle.h)
 U1, U2, U3, P, R1, R2, R3, F1, F2, F3, S1 = range(11)
@@ -178,6 +183,14 @@ class Oracle:
 
     def apply_bc_scalar(self):
         self.L.dor_apply_bc_scalar(self.h)
+
+    def stats_chan(self) -> tuple[dict, float]:
+        """SAVE_STATS_CHAN's statistics: {name: profile at J = 1..NY}, UBULK."""
+        ny = self.cfg.ny
+        out = np.zeros(len(STATS) * ny)
+        ub = np.zeros(1)
+        self.L.dor_stats_chan(self.h, _dp(out), _dp(ub))
+        return {k: out[q * ny:(q + 1) * ny].copy() for q, k in enumerate(STATS)}, float(ub[0])
 
     # -- initial state: the CREATE_FLOW_CHAN / CREATE_TH_CHAN sequence --
     def initialize(self, u1, u2, u3, ths):

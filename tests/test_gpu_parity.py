@@ -246,6 +246,34 @@ def test_mixed_state_sync(gpu_mod):
             assert np.array_equal(st[k], ref[k]), k
 
 
+@pytest.mark.parametrize("maker", ["channel", "kh"])
+def test_stats_chan(oracle_mod, gpu_mod, maker):
+    """SAVE_STATS_CHAN's statistics: device (Fourier-space sums) against the
+    oracle (the reference's physical-space sums) on the same state; the
+    mixed path reports them from its refreshed FP64 state."""
+    cfg = {"channel": lambda: case.channel_case(32, 17, 24, kick=0.3, streak=0.1),
+           "kh": lambda: case.kh_case(32, 33, 24, kick=0.05)}[maker]()
+    say("stats_chan", maker)
+    o, g = make_pair(oracle_mod, gpu_mod, cfg, nsteps_oracle=2)
+    so, ubo = o.stats_chan()
+    sg, ubg = g.stats_chan()
+    for k in so:
+        scale = max(np.abs(so[k]).max(), 1e-300)
+        err = np.abs(sg[k] - so[k]).max() / scale
+        say(k, err)
+        assert err < 1e-11, (k, err)
+    assert ubg == pytest.approx(ubo, rel=1e-13, abs=1e-300)
+    gm = gpu_mod.GpuChannel(cfg, precision="mixed")
+    gm.load_state(o.state())
+    gm.step(1)
+    g.step(1)
+    sm, _ = gm.stats_chan()
+    s64, _ = g.stats_chan()
+    for k in ("urms", "vrms", "wrms", "uv", "shear", "thrms"):
+        err = np.abs(sm[k] - s64[k]).max() / max(np.abs(s64[k]).max(), 1e-300)
+        assert err < 1e-5, (k, err)
+
+
 def test_graph_replays_match_eager(gpu_mod):
     """Graph replays (several, with host synchronisations and eager work in
     between) give bit-identical states to eager launches."""

@@ -1,0 +1,132 @@
+"""The oracle's SAVE_STATS_CHAN statistics (physical-space sums, as the
+reference computes them) pinned by closed forms and by an independent
+spectral (Parseval) evaluation in numpy."""
+import numpy as np
+import pytest
+
+from diablo_amd import case
+
+
+def _cfg(nx=24, ny=9, nz=18):
+    return case.channel_case(nx=nx, ny=ny, nz=nz, n_steps=1)
+
+
+def _fresh(oracle_mod, cfg):
+    o = oracle_mod.Oracle(cfg)
+    for f in (oracle_mod.U1, oracle_mod.U2, oracle_mod.U3, oracle_mod.P):
+        o.field(f)[...] = 0.0
+    for n in range(cfg.n_th):
+        o.field(oracle_mod.CTH0 + n)[...] = 0.0
+    return o
+
+
+def test_single_mode_closed_forms(oracle_mod):
+    """Mean profiles plus one (kx, 0) mode: rms, covariances, vorticity and the
+    mean gradients have closed forms."""
+    cfg = _cfg()
+    o = _fresh(oracle_mod, cfg)
+    gyf = o.vec(oracle_mod.GYF)
+    dy = o.vec(oracle_mod.DY)
+    kx = o.vec(oracle_mod.KX)
+    c1 = case.fourier_view(o.field(oracle_mod.U1))
+    c2 = case.fourier_view(o.field(oracle_mod.U2))
+    c3 = case.fourier_view(o.field(oracle_mod.U3))
+    ct = case.fourier_view(o.field(oracle_mod.CTH0))
+    cp = case.fourier_view(o.field(oracle_mod.P))
+    ny = cfg.ny
+    j = np.arange(ny + 2)
+    U = 1.0 + 0.1 * j          # mean profiles (mode (0,0), real)
+    W = 0.5 - 0.02 * j * j
+    T = np.sin(0.3 * j)
+    c1[:, 0, 0] = U
+    c3[:, 0, 0] = W
+    ct[:, 0, 0] = T
+    cp[:, 0, 0] = 2.0 + j
+    a = 0.3 + 0.2j             # fluctuation amplitude of u at (I=1, K=0), y-independent
+    b = -0.1 + 0.4j            # of w
+    c1[:, 0, 1] = a
+    c3[:, 0, 1] = b
+    st, ub = o.stats_chan()
+    # u = U + 2 Re(a e^{i kx x}): rms^2 = 2|a|^2; <u'w'> = 2 Re(a conj b)
+    np.testing.assert_allclose(st["urms"], np.sqrt(2.0) * abs(a), rtol=1e-13)
+    np.testing.assert_allclose(st["wrms"], np.sqrt(2.0) * abs(b), rtol=1e-13)
+    np.testing.assert_allclose(st["vrms"], 0.0, atol=1e-15)
+    np.testing.assert_allclose(st["uw"], 2.0 * (a * np.conj(b)).real, rtol=1e-13)
+    np.testing.assert_allclose(st["uv"], 0.0, atol=1e-15)
+    np.testing.assert_allclose(st["ume"], U[1:ny + 1], rtol=0, atol=0)
+    np.testing.assert_allclose(st["cp"], 2.0 + j[1:ny + 1], rtol=0, atol=0)
+    np.testing.assert_allclose(st["dudy"], (U[1:ny + 1] - U[0:ny]) / (gyf[1:ny + 1] - gyf[0:ny]), rtol=1e-14)
+    # omega_y = dU/dz - dW/dx = -d/dx of 2 Re(b e^{i kx x}): rms sqrt(2) kx |b|
+    np.testing.assert_allclose(st["omega_y"], np.sqrt(2.0) * kx[1] * abs(b), rtol=1e-13)
+    # y-independent fluctuations: omega_x = dW/dy (mean), omega_z = -dU/dy (mean)
+    dyf = o.vec(oracle_mod.DYF)
+    dW = (W[2:ny + 2] - W[0:ny]) / (2 * dyf[1:ny + 1])
+    dU = (U[2:ny + 2] - U[0:ny]) / (2 * dyf[1:ny + 1])
+    np.testing.assert_allclose(st["omega_x"], np.abs(dW), rtol=1e-12)
+    np.testing.assert_allclose(st["omega_z"], np.abs(dU), rtol=1e-12)
+    np.testing.assert_allclose(st["shear"], dU ** 2 + dW ** 2, rtol=1e-12)
+    # scalar: mean only -> no fluctuation, grad^2 = (dT/dy centred)^2
+    np.testing.assert_allclose(st["thrms"], 0.0, atol=1e-15)
+    dT = (T[2:ny + 2] - T[0:ny]) / (gyf[2:ny + 2] - gyf[0:ny])
+    np.testing.assert_allclose(st["pe_diss"], dT ** 2, rtol=1e-12)
+    # UBULK: trapezoid rule of U over GYF(1..NY) divided by LY
+    ref = sum(0.5 * (U[jj] + U[jj - 1]) * dy[jj] for jj in range(2, ny + 1)) / cfg.ly
+    assert ub == pytest.approx(ref, rel=1e-14)
+
+
+def _spectral_stats(cfg, o, oracle_mod):
+    """The same statistics as plane sums over the kept modes (Parseval)."""
+    ny = cfg.ny
+    kx = o.vec(oracle_mod.KX)[:cfg.nkx + 1]
+    kz = o.vec(oracle_mod.KZ)[:cfg.tnkz + 1]
+    dy, dyf, gyf = o.vec(oracle_mod.DY), o.vec(oracle_mod.DYF), o.vec(oracle_mod.GYF)
+    K = lambda f: case.kept(cfg, o.field(f)).copy()
+    u1, u2, u3, th = K(oracle_mod.U1), K(oracle_mod.U2), K(oracle_mod.U3), K(oracle_mod.CTH0)
+    for f in (u1, u2, u3, th):
+        f[:, 0, 0] = f[:, 0, 0].real       # C2R keeps only Re of the (0,0) mode
+    w = np.where(np.arange(cfg.nkx + 1) == 0, 1.0, 2.0)[None, :]
+    fl = np.ones((cfg.tnkz + 1, cfg.nkx + 1))
+    fl[0, 0] = 0.0                          # fluctuations leave out the mean
+    KX, KZ = kx[None, :], kz[:, None]
+    out = {k: np.zeros(ny) for k in ("urms", "vrms", "wrms", "uv", "uw", "wv", "shear", "omega_x", "omega_y",
+                                      "omega_z", "thrms", "thv", "pe_diss")}
+    for j in range(1, ny + 1):
+        s = lambda x: float(np.sum(w * x))
+        a2 = lambda z: np.abs(z) ** 2
+        out["urms"][j - 1] = np.sqrt(s(fl * a2(u1[j])))
+        out["vrms"][j - 1] = np.sqrt(s(fl * 0.5 * (a2(u2[j]) + a2(u2[j + 1]))))
+        out["wrms"][j - 1] = np.sqrt(s(fl * a2(u3[j])))
+        gy1 = (dyf[j - 1] * u1[j] + dyf[j] * u1[j - 1]) / (2 * dy[j])
+        gy3 = (dyf[j - 1] * u3[j] + dyf[j] * u3[j - 1]) / (2 * dy[j])
+        out["uv"][j - 1] = s(fl * (gy1 * np.conj(u2[j])).real)
+        out["wv"][j - 1] = s(fl * (gy3 * np.conj(u2[j])).real)
+        out["uw"][j - 1] = s(fl * (u1[j] * np.conj(u3[j])).real)
+        du1 = (u1[j + 1] - u1[j - 1]) / (2 * dyf[j])
+        du3 = (u3[j + 1] - u3[j - 1]) / (2 * dyf[j])
+        v2 = 0.5 * (u2[j + 1] + u2[j])
+        out["shear"][j - 1] = s(a2(du1) + a2(du3))
+        out["omega_x"][j - 1] = np.sqrt(s(a2(du3 - 1j * KZ * v2)))
+        out["omega_y"][j - 1] = np.sqrt(s(a2(1j * KZ * u1[j] - 1j * KX * u3[j])))
+        out["omega_z"][j - 1] = np.sqrt(s(a2(1j * KX * v2 - du1)))
+        out["thrms"][j - 1] = np.sqrt(s(fl * a2(th[j])))
+        out["thv"][j - 1] = s(fl * (th[j] * np.conj(v2)).real)
+        dth = (th[j + 1] - th[j - 1]) / (gyf[j + 1] - gyf[j - 1])
+        out["pe_diss"][j - 1] = s(a2(1j * KX * th[j]) + a2(dth) + a2(1j * KZ * th[j]))
+    return out
+
+
+@pytest.mark.parametrize("dims", [(24, 9, 18), (32, 17, 24)])
+def test_physical_sums_match_parseval(oracle_mod, dims):
+    """Random band-limited state: the oracle's physical-space sums equal the
+    spectral evaluation to round-off."""
+    cfg = _cfg(*dims)
+    o = oracle_mod.Oracle(cfg)
+    phys = case.initial_physical(cfg.replace(kick=0.3, streak=0.1))
+    o.initialize(*phys)
+    o.step()
+    st, _ = o.stats_chan()
+    ref = _spectral_stats(cfg, o, oracle_mod)
+    for k, v in ref.items():
+        scale = max(np.abs(v).max(), 1e-300)
+        err = np.abs(st[k] - v).max() / scale
+        assert err < 1e-12, (k, err)
