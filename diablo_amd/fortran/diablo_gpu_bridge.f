@@ -117,6 +117,13 @@
           IMPORT :: C_PTR, C_INT32_T
           TYPE(C_PTR), VALUE :: CTX
         END FUNCTION
+        INTEGER(C_INT32_T) FUNCTION dgpu_stats_chan(CTX, P, U)
+     &      BIND(C, NAME='dgpu_stats_chan')
+          IMPORT :: C_PTR, C_INT32_T, C_DOUBLE
+          TYPE(C_PTR), VALUE :: CTX
+          REAL(C_DOUBLE), INTENT(OUT) :: P(*)
+          REAL(C_DOUBLE), INTENT(OUT) :: U
+        END FUNCTION
       END INTERFACE
 
       TYPE(C_PTR), SAVE :: DGPU_CTX = C_NULL_PTR
@@ -379,5 +386,23 @@
       CALL DGPU_CHECK(dgpu_fft_xz_to_physical(DGPU_CTX, VV,
      &     INT(JMIN, C_INT32_T), INT(JMAX, C_INT32_T)),
      &     'dgpu_fft_xz_to_physical')
+      RETURN
+      END
+
+!----*|--.---------.---------.---------.---------.---------.---------.-|-------|
+      SUBROUTINE DGPU_SAVE_STATS(PROF, UB)
+!----*|--.---------.---------.---------.---------.---------.---------.-|-------|
+! The statistics block of SAVE_STATS_CHAN (for/save_stats.f:126-347 and its
+! scalar part, 420-522) computed on the GPU from its own state in FP64:
+! PROF(J, Q) for J = 1..NY, Q = 1..21 in the order of DGPU_ST_* of
+! include/diablo_gpu.h, and UB = UBULK.  The host state is uploaded first
+! unless the GPU copy is current.
+      USE DGPU_BRIDGE
+      INCLUDE 'header'
+      REAL*8 PROF(NY, 21), UB
+      CALL DGPU_ENSURE_CONTEXT
+      IF (.NOT. DGPU_ON_DEVICE) CALL DGPU_SYNC_TO_DEVICE
+      CALL DGPU_CHECK(dgpu_stats_chan(DGPU_CTX, PROF, UB),
+     &     'dgpu_stats_chan')
       RETURN
       END
