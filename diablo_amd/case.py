@@ -49,6 +49,10 @@ class ChannelConfig:
     lz: float = math.pi
     nu_v_scale: float = 1.0
     delta_t: float = 1.0e-3
+    # VARIABLE_DT, CFL, UPDATE_DT (input.dat time-march line, for/diablo_io.f:33-34)
+    variable_dt: bool = False
+    cfl: float = 0.5
+    update_dt: int = 1
     i_ro: float = 0.0
     grav_x: float = 0.0
     grav_y: float = 0.0
@@ -416,7 +420,8 @@ def write_decks(cfg: ChannelConfig, directory: str, save_flow_int: int = 100000,
         "C Method:     NUM_PER_DIR, CREATE_NEW_FLOW",
         "   2   .TRUE.",
         "C Time March:  N_TIME_STEPS, TIME_LIMIT, DELTA_T, RESET_TIME, VARIABLE_DT, CFL, UPDATE_DT",
-        f"   {cfg.n_steps}  1000000.0  {_fl(cfg.delta_t)}  .FALSE.  .FALSE.  0.5  1",
+        f"   {cfg.n_steps}  1000000.0  {_fl(cfg.delta_t)}  .FALSE.  {'.TRUE.' if cfg.variable_dt else '.FALSE.'}"
+        f"  {_fl(cfg.cfl)}  {cfg.update_dt}",
         "C I/O:         VERBOSITY, SAVE_FLOW_INT, SAVE_STATS_INT,  MOVIE",
         f"   1  {save_flow_int}  {save_stats_int}  .FALSE.",
         "C Here include 6*N_TH lines, see below for format",
@@ -522,9 +527,7 @@ def read_decks(directory: str, nx: int, ny: int, nz: int, n_th: int) -> ChannelC
     if int(num_per_dir) != 2:
         raise ValueError("only NUM_PER_DIR = 2 (channel) is supported")
     r.skip()
-    nsteps, _tlim, dt, _reset, vdt, _cfl, _upd = r.read(7)
-    if _logical(vdt):
-        raise ValueError("VARIABLE_DT is not supported (fixed dt only)")
+    nsteps, _tlim, dt, _reset, vdt, cfl, upd = r.read(7)
     r.skip()
     r.read(4)
     r.skip()
@@ -574,7 +577,8 @@ def read_decks(directory: str, nx: int, ny: int, nz: int, n_th: int) -> ChannelC
         drx.append(a); drz.append(b)
     return ChannelConfig(
         nx=nx, ny=ny, nz=nz, n_th=n_th, nu=nu, beta=beta, lx=lx, ly=ly, lz=lz,
-        nu_v_scale=nu_v_scale, delta_t=float(dt), i_ro=i_ro, grav_x=gx, grav_y=gyv, grav_z=gz,
+        nu_v_scale=nu_v_scale, delta_t=float(dt), variable_dt=_logical(vdt), cfl=float(cfl),
+        update_dt=int(upd), i_ro=i_ro, grav_x=gx, grav_y=gyv, grav_z=gz,
         f_type=int(f_type), ubulk0=float(ubulk0), px0=float(px0), omega0=float(omega0),
         amp_omega0=float(amp), th_bc_ymin=thmin, th_bc_ymin_c1=thmin1, th_bc_ymax=thmax,
         th_bc_ymax_c1=thmax1, ri=ri, pr=pr, drhodx=drx, drhodz=drz,

@@ -85,6 +85,9 @@ struct dgpu_ctx {
     double *d_fa = nullptr, *d_fb = nullptr;  // POISSON_P_CHAN work planes
     double *d_gyf = nullptr;
     double *d_stats = nullptr;  // dgpu_stats_chan output
+    // COURANT: 0 idle, 1 requested for the next RK_STEP 3, 2 maxima recorded
+    int courant_state = 0;
+    double *d_vmax_part = nullptr, *d_vmax = nullptr;  // per-block, per-plane velocity maxima
     // CN tables: [stage 0..2][sys][NY+2][kRow]
     double *d_cn = nullptr;
     std::vector<double> h_cn;
@@ -706,7 +709,7 @@ int32_t dgpu_destroy(dgpu_ctx *c)
     void *bufs[] = {c->d_vec, c->d_tw, c->d_state, c->d_pmult, c->d_pinv, c->d_rd, c->d_nlptr, c->d_zt, c->d_wt,
                     c->d_fa, c->d_fb, c->d_gyf, c->d_iptw, c->d_iptwf, c->d_ztf, c->d_wtf, c->d_nlf, c->d_nlptrf,
                     c->d_fort, c->d_cn, c->d_codes, c->d_c1, c->d_state32, c->d_side, c->d_map,
-                    c->d_pmultf, c->d_pinvf, c->d_gs32, c->d_stats};
+                    c->d_pmultf, c->d_pinvf, c->d_gs32, c->d_stats, c->d_vmax_part, c->d_vmax};
     for (void *b : bufs)
         if (b) hipFree(b);
     if (c->side_stream) {
@@ -741,6 +744,59 @@ int32_t dgpu_stats_chan(dgpu_ctx *c, double *profiles, double *ubulk)
     double res = 0.;
     for (int j = 2; j <= g.ny; j++) res = res + 0.5 * (ume[j - 1] + ume[j - 2]) * c->dy[j];
     *ubulk = res / c->p.ly;
+    return DGPU_OK;
+}
+
+int32_t dgpu_courant_request(dgpu_ctx *c)
+{
+    if (!c) return DGPU_E_ARG;
+    if (dgpu::vmax_rows_per_block(c->ipx) < 1) return fail(c, DGPU_E_ARG, "x transform too long for COURANT");
+    if (!c->d_vmax) {
+        HIPCHK(c, hipMalloc(&c->d_vmax_part, sizeof(double) * dgpu::vmax_part_count(c->g, c->ipx)));
+        HIPCHK(c, hipMalloc(&c->d_vmax, sizeof(double) * 3 * c->g.ny));
+    }
+    c->courant_state = 1;
+    return DGPU_OK;
+}
+
+// COURANT (for/courant.f:14-62) from the recorded maxima.  For each plane,
+// cfl * DY(J) / max|U2| is the smallest of the reference's per-point
+// quotients (division rounds monotonically), and likewise for U1 and U3 with
+// DX = LX/NX and DZ = LZ/NZ (for/channel.f:1574-1583); a zero maximum gives
+// an infinite quotient, which the minimum ignores, as the reference's does.
+int32_t dgpu_courant(dgpu_ctx *c, double cfl, double *delta_t, double *h_bar)
+{
+    if (!c || !delta_t || !h_bar) return DGPU_E_ARG;
+    if (c->courant_state != 2) return fail(c, DGPU_E_STATE, "no velocity maxima recorded since dgpu_courant_request");
+    const int NY = c->g.ny;
+    std::vector<double> m(3 * (size_t)NY);
+    HIPCHK(c, hipMemcpyAsync(m.data(), c->d_vmax, sizeof(double) * m.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->courant_state = 0;
+    const dgpu_params &p = c->p;
+    const double NU = p.nu, PI = 4.0 * std::atan(1.0);
+    const double dx = p.lx / c->g.nx, dz = p.lz / c->g.nz;
+    double dt = 999.0;
+    dt = std::fmin(dt, 0.5 * std::pow(std::fmin(dx, c->dy[1]), 2.0) / NU);
+    for (int n = 0; n < c->g.nth; ++n) dt = std::fmin(dt, dt * NU / (NU / p.pr[n]));
+    if (p.i_ro != 0.0) dt = std::fmin(dt, 2.0 * PI / std::fabs(p.i_ro) / 20.0);
+    for (int n = 0; n < c->g.nth; ++n) {
+        if (p.ri[n] != 0.0) {
+            const double Nmax = std::sqrt(std::fabs(p.ri[n] * p.th_bc_ymin_c1[n]));
+            dt = std::fmin(dt, (double)0.1f * 2.0 * PI / Nmax);  // the literal 0.1 is a default REAL
+        }
+    }
+    for (int j = 1; j <= NY; ++j) {
+        const double dt_x = cfl * dx / m[j - 1];
+        const double dt_y = cfl * c->dy[j] / m[NY + j - 1];
+        const double dt_z = cfl * dz / m[2 * NY + j - 1];
+        dt = std::fmin(dt, std::fmin(dt_x, std::fmin(dt_y, dt_z)));
+    }
+    const double DELTA_T = dt <= 0.0 ? 0.0001 : (dt >= 999.0 ? 999.0 : dt);
+    *delta_t = DELTA_T;
+    h_bar[0] = DELTA_T * (8.0 / 15.0);
+    h_bar[1] = DELTA_T * (2.0 / 15.0);
+    h_bar[2] = DELTA_T * (5.0 / 15.0);
     return DGPU_OK;
 }
 
@@ -908,11 +964,39 @@ static void make_cn_args(dgpu_ctx *c, int rk_step, double time, double temp3, do
 }
 
 // Enqueues the five kernels of one substep on the context's stream.
+// COURANT's velocity maxima (for/channel.f:1037-1040): the velocity after the
+// implicit solves and before the projection, taken to physical space in the
+// precision of the run's transforms; planes 1..NY of CU1..CU3.
+static int enqueue_vmax(dgpu_ctx *c)
+{
+    const dgpu::Geo &g = c->g;
+    const long zps = (long)g.nz * c->ld;
+    if (c->mixed) {
+        const cx<float> *twx = c->d_iptwf, *twz = c->d_iptwf + c->ipx.twn;
+        if (c->f32state)
+            dgpu::launch_zinv_ip<float, float>(c->stream, g, c->ipz, twz, c->d_state32, c->fs, g.ps, c->d_ztf,
+                                               c->zfs, zps, c->ld, 1, g.ny, 3);
+        else
+            dgpu::launch_zinv_ip<float, double>(c->stream, g, c->ipz, twz, c->cu[0], c->fs, g.ps, c->d_ztf, c->zfs,
+                                                zps, c->ld, 1, g.ny, 3);
+        dgpu::launch_vmax_ip<float>(c->stream, g, c->ipx, twx, c->ld, c->d_ztf, c->zfs, c->d_vmax_part, c->d_vmax);
+    } else {
+        const cx<double> *twx = c->d_iptw, *twz = c->d_iptw + c->ipx.twn;
+        dgpu::launch_zinv_ip<double, double>(c->stream, g, c->ipz, twz, c->cu[0], c->fs, g.ps, c->d_zt, c->zfs, zps,
+                                             c->ld, 1, g.ny, 3);
+        dgpu::launch_vmax_ip<double>(c->stream, g, c->ipx, twx, c->ld, c->d_zt, c->zfs, c->d_vmax_part, c->d_vmax);
+    }
+    if (int rc = check_launch(c, "courant maxima")) return rc;
+    c->courant_state = 2;
+    return DGPU_OK;
+}
+
 static int rk_enqueue(dgpu_ctx *c, int rk_step, double time, double temp1, double temp3, double temp4, double temp5)
 {
     const int k = rk_step - 1;
     (void)temp1;
     const dgpu::Geo &g = c->g;
+    const bool courant = rk_step == 3 && c->courant_state == 1;
     const int nin = 3 + g.nth;
     int jlo = 2;
     for (int n = 0; n < g.nth; ++n) jlo = std::min(jlo, c->jstart_th[n]);
@@ -994,6 +1078,9 @@ static int rk_enqueue(dgpu_ctx *c, int rk_step, double time, double temp1, doubl
             dgpu::launch_cn<float, float>(c->stream, A, P);
         }
         if (int rc = check_launch(c, "cn")) return rc;
+        if (courant) {
+            if (int rc = enqueue_vmax(c)) return rc;
+        }
         {
             TimerScope t(c, 4);
             dgpu::launch_remdiv<float, float>(c->stream, R, P);
@@ -1011,6 +1098,9 @@ static int rk_enqueue(dgpu_ctx *c, int rk_step, double time, double temp1, doubl
             dgpu::launch_cn<double, float>(c->stream, A, P);
         }
         if (int rc = check_launch(c, "cn")) return rc;
+        if (courant) {
+            if (int rc = enqueue_vmax(c)) return rc;
+        }
         {
             TimerScope t(c, 4);
             dgpu::launch_remdiv<double, float>(c->stream, R, P);
@@ -1023,6 +1113,9 @@ static int rk_enqueue(dgpu_ctx *c, int rk_step, double time, double temp1, doubl
             dgpu::launch_cn<double, double>(c->stream, A, P);
         }
         if (int rc = check_launch(c, "cn")) return rc;
+        if (courant) {
+            if (int rc = enqueue_vmax(c)) return rc;
+        }
         {
             TimerScope t(c, 4);
             dgpu::launch_remdiv<double, double>(c->stream, R, P);
@@ -1047,7 +1140,8 @@ int32_t dgpu_rk_chan_1(dgpu_ctx *c, int32_t rk_step, double time, const double *
     c->work_dirty = c->f32state;
     // One captured graph per RK stage.  Not used while kernels are timed one
     // by one, or when the forcing depends on TIME (F_TYPE = 2).
-    const bool graphable = c->use_graphs && !c->timing && c->p.f_type != 2;
+    const bool graphable = c->use_graphs && !c->timing && c->p.f_type != 2 &&
+                           !(rk_step == 3 && c->courant_state == 1);
     if (!graphable) return rk_enqueue(c, rk_step, time, temp1, temp3, temp4, temp5);
     const double key[4] = {temp1, temp3, temp4, temp5};
     if (!(c->gexec[k] && std::memcmp(key, c->gkey[k], sizeof(key)) == 0)) {

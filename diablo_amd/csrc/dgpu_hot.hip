@@ -346,6 +346,130 @@ __global__ void __launch_bounds__(512) k_xfused_ip_batch(const XfDesc<T> *__rest
 }
 
 // -------------------------------------------------------------------------
+// vmax: COURANT's velocity maxima (for/courant.f:35-44).  x backward (DIF) of
+// the z-backward rows of CU1, CU2, CU3, packed as the pairs (u1, u2) and
+// (u3, 0), for planes 1..NY; block (bx, j) takes rb z rows of plane j, thread
+// row p = 2 * row + pair, and writes max |u1|, |u2|, |u3| over its points to
+// part[((j - 1) * gridDim.x + bx) * 3 + q].  A maximum is exact and does not
+// depend on the order of the comparisons.  The last stage never goes back to
+// LDS: its outputs go straight into the maxima.
+// -------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T absmax_(T m, T v)
+{
+    const T a = v < T(0) ? -v : v;
+    return a > m ? a : m;
+}
+
+template <typename T, int E>
+__global__ void __launch_bounds__(256) k_vmax_ip(IpLen fx, const cx<T> *__restrict__ twg, int mk, int ld,
+                                                  const cx<T> *__restrict__ zt, long zt_fs, long zt_ps, int nz,
+                                                  int rb, double *__restrict__ part)
+{
+    extern __shared__ unsigned char smem_raw[];
+    __shared__ T red[3][256];
+    const int nx = fx.n;
+    cx<T> *rows = reinterpret_cast<cx<T> *>(smem_raw);  // [2 rb][nx]
+    cx<T> *tw = rows + 2 * rb * nx;
+    const int tid = threadIdx.x;
+    for (int e = tid; e < fx.twn; e += 256) tw[e] = twg[e];
+    const int TT = fx.t, S = fx.nstages;
+    const int p = tid / TT, th = tid - p * TT;
+    const int pair = p & 1, zp = blockIdx.x * rb + (p >> 1);
+    const bool act = p < 2 * rb && zp < nz;
+    const int j = 1 + blockIdx.y;
+    cx<T> *row = rows + (act ? p : 0) * nx;
+    __syncthreads();  // twiddles
+    if (act) {
+        const cx<T> *fa = zt + (2 * pair) * zt_fs + (long)j * zt_ps + (long)zp * ld;
+        const cx<T> *fb = zt + zt_fs + (long)j * zt_ps + (long)zp * ld;
+        with_radix<E>(fx.radix[0], [&](auto rc) {
+            constexpr int R = decltype(rc)::value;
+            ip_stage<T, E, R, +1, true>(
+                fx, 0, tw, th,
+                [&](int, int f) {
+                    // spectrum of A + iB with the Hermitian halves of both
+                    // rows; C2R ignores Im of the DC term
+                    const int kk = f < mk ? f : (f > nx - mk ? nx - f : -1);
+                    cx<T> A = fa[max(kk, 0)];
+                    cx<T> B = fb[max(kk, 0)];
+                    if (pair) B = make_cx<T>(0, 0);
+                    if (kk < 0) {
+                        A = make_cx<T>(0, 0);
+                        B = make_cx<T>(0, 0);
+                    }
+                    if (f == 0) {
+                        A.im = T(0);
+                        B.im = T(0);
+                    }
+                    if (f > nx - mk) {
+                        A.im = -A.im;
+                        B.im = -B.im;
+                    }
+                    return make_cx<T>(A.re - B.im, A.im + B.re);
+                },
+                [&](int, int pos, cx<T> v) { row[ip_swz(fx, pos)] = v; });
+        });
+    }
+    __syncthreads();
+    for (int s = 1; s < S - 1; ++s) {
+        if (act) ip_stage_row<T, E, +1, true>(row, fx, s, tw, th);
+        __syncthreads();
+    }
+    T ma = T(0), mb = T(0);
+    if (act) {
+        with_radix<E>(fx.radix[S - 1], [&](auto rc) {
+            constexpr int R = decltype(rc)::value;
+            ip_stage<T, E, R, +1, true>(
+                fx, S - 1, tw, th, [&](int, int pos) { return row[ip_swz(fx, pos)]; },
+                [&](int, int, cx<T> v) {
+                    ma = absmax_(ma, v.re);
+                    mb = absmax_(mb, v.im);
+                });
+        });
+    }
+    red[0][tid] = (act && pair == 0) ? ma : T(0);
+    red[1][tid] = (act && pair == 0) ? mb : T(0);
+    red[2][tid] = (act && pair == 1) ? ma : T(0);
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (tid < h) {
+#pragma unroll
+            for (int q = 0; q < 3; ++q) red[q][tid] = red[q][tid + h] > red[q][tid] ? red[q][tid + h] : red[q][tid];
+        }
+        __syncthreads();
+    }
+    if (tid < 3) part[((long)(j - 1) * gridDim.x + blockIdx.x) * 3 + tid] = (double)red[tid][0];
+}
+
+// Per plane j = 1..NY: out[q * NY + j - 1] = max over the plane's partials.
+__global__ void __launch_bounds__(256) k_vmax_planes(const double *__restrict__ part, int nparts, int ny,
+                                                      double *__restrict__ out)
+{
+    __shared__ double red[3][256];
+    const int tid = threadIdx.x, j = blockIdx.x;
+    double m[3] = {0.0, 0.0, 0.0};
+    for (int b = tid; b < nparts; b += 256) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const double v = part[((long)j * nparts + b) * 3 + q];
+            m[q] = v > m[q] ? v : m[q];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) red[q][tid] = m[q];
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (tid < h) {
+#pragma unroll
+            for (int q = 0; q < 3; ++q) red[q][tid] = red[q][tid + h] > red[q][tid] ? red[q][tid + h] : red[q][tid];
+        }
+        __syncthreads();
+    }
+    if (tid < 3) out[(long)tid * ny + j] = red[tid][0];
+}
+
+// -------------------------------------------------------------------------
 // zfwd_nl: W columns per block; both transforms of output o (G then H) in the
 // same LDS; G's result stays in registers, H's last stage combines and writes
 // the kept slots.  grid (column chunks, planes, outputs), block W*T.
@@ -646,6 +770,44 @@ void launch_zfwd_nl_ip_batch(hipStream_t s, const Geo &g, const IpLen &fz, const
     DGPU_DISPATCH_IPE(fz.e, (zfwd_nl_ip_batch_e<T, EE, TOUT>(s, g, fz, tw, d, nmem, max_planes, ld, nout)));
 }
 
+int vmax_rows_per_block(const IpLen &fx) { return 256 / (2 * fx.t); }
+
+size_t vmax_part_count(const Geo &g, const IpLen &fx)
+{
+    const int rb = vmax_rows_per_block(fx);
+    return rb > 0 ? (size_t)g.ny * ((g.nz + rb - 1) / rb) * 3 : 0;
+}
+
+template <typename T, int E>
+static void vmax_ip_e(hipStream_t s, const Geo &g, const IpLen &fx, const cx<T> *tw, int ld, const cx<T> *zt,
+                      long zt_fs, double *part, double *out)
+{
+    const int rb = vmax_rows_per_block(fx);
+    const int nb = (g.nz + rb - 1) / rb;
+    const size_t lds = sizeof(cx<T>) * ((size_t)2 * rb * fx.n + fx.twn);
+    // above 64 KiB of dynamic LDS the kernel must be allowed it (its static
+    // reduction array counts against the same 160 KiB); a refusal is left as
+    // the launch error
+    if (lds > 64 * 1024)
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_vmax_ip<T, E>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    const long ps = (long)g.nz * ld;
+    hipLaunchKernelGGL((k_vmax_ip<T, E>), dim3(nb, g.ny), dim3(256), lds, s, fx, tw, g.mk, ld, zt, zt_fs, ps, g.nz,
+                       rb, part);
+    hipLaunchKernelGGL(k_vmax_planes, dim3(g.ny), dim3(256), 0, s, part, nb, g.ny, out);
+}
+
+template <typename T>
+void launch_vmax_ip(hipStream_t s, const Geo &g, const IpLen &fx, const cx<T> *tw, int ld, const cx<T> *zt,
+                    long zt_fs, double *part, double *out)
+{
+    DGPU_DISPATCH_IPE(fx.e, (vmax_ip_e<T, EE>(s, g, fx, tw, ld, zt, zt_fs, part, out)));
+}
+
+template void launch_vmax_ip<double>(hipStream_t, const Geo &, const IpLen &, const cx<double> *, int,
+                                     const cx<double> *, long, double *, double *);
+template void launch_vmax_ip<float>(hipStream_t, const Geo &, const IpLen &, const cx<float> *, int,
+                                    const cx<float> *, long, double *, double *);
 template void launch_zinv_ip_batch<double, double>(hipStream_t, const Geo &, const IpLen &, const cx<double> *,
                                                    const ZinvDesc<double, double> *, int, int, int, int);
 template void launch_zinv_ip_batch<float, double>(hipStream_t, const Geo &, const IpLen &, const cx<float> *,

@@ -153,6 +153,14 @@ void launch_xfused_ip_batch(hipStream_t s, const Geo &g, const IpLen &fx, const 
 template <typename T, typename TOUT>
 void launch_zfwd_nl_ip_batch(hipStream_t s, const Geo &g, const IpLen &fz, const cx<T> *tw, const ZfDesc<T, TOUT> *d,
                              int nmem, int max_planes, int ld, int nout);
+// COURANT's velocity maxima from the z-backward rows of CU1..CU3 (zt, planes
+// 1..NY): out[q * NY + j - 1] = max |u_{q+1}| over plane j.  part holds
+// vmax_part_count doubles.
+template <typename T>
+void launch_vmax_ip(hipStream_t s, const Geo &g, const IpLen &fx, const cx<T> *tw, int ld, const cx<T> *zt,
+                    long zt_fs, double *part, double *out);
+int vmax_rows_per_block(const IpLen &fx);
+size_t vmax_part_count(const Geo &g, const IpLen &fx);
 size_t zip_lds_bytes(const IpLen &fz, size_t elem);
 size_t xfused_ip_lds_bytes(const IpLen &fx, int nth, size_t elem);
 int xfused_ip_threads(const IpLen &fx, int nth);

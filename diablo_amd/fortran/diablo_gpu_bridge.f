@@ -124,6 +124,19 @@
           REAL(C_DOUBLE), INTENT(OUT) :: P(*)
           REAL(C_DOUBLE), INTENT(OUT) :: U
         END FUNCTION
+        INTEGER(C_INT32_T) FUNCTION dgpu_courant_request(CTX)
+     &      BIND(C, NAME='dgpu_courant_request')
+          IMPORT :: C_PTR, C_INT32_T
+          TYPE(C_PTR), VALUE :: CTX
+        END FUNCTION
+        INTEGER(C_INT32_T) FUNCTION dgpu_courant(CTX, CFL, DT, H)
+     &      BIND(C, NAME='dgpu_courant')
+          IMPORT :: C_PTR, C_INT32_T, C_DOUBLE
+          TYPE(C_PTR), VALUE :: CTX
+          REAL(C_DOUBLE), VALUE :: CFL
+          REAL(C_DOUBLE), INTENT(OUT) :: DT
+          REAL(C_DOUBLE), INTENT(OUT) :: H(3)
+        END FUNCTION
       END INTERFACE
 
       TYPE(C_PTR), SAVE :: DGPU_CTX = C_NULL_PTR
@@ -297,14 +310,23 @@
 !----*|--.---------.---------.---------.---------.---------.---------.-|-------|
       SUBROUTINE RK_CHAN_1
 !----*|--.---------.---------.---------.---------.---------.---------.-|-------|
-! One RK3/CN substep on the GPU (for/channel.f:62-1073).
+! One RK3/CN substep on the GPU (for/channel.f:62-1073).  Under VARIABLE_DT
+! the device takes COURANT's velocity maxima inside RK_STEP 3 and DELTA_T and
+! H_BAR are updated after it, for the next step (for/channel.f:1037-1040).
       USE DGPU_BRIDGE
       INCLUDE 'header'
+      LOGICAL DO_CFL
       CALL DGPU_ENSURE_CONTEXT
       IF (.NOT. DGPU_ON_DEVICE) CALL DGPU_SYNC_TO_DEVICE
+      DO_CFL = VARIABLE_DT .AND. (RK_STEP .EQ. 3) .AND.
+     &     (MOD(TIME_STEP, UPDATE_DT) .EQ. 0)
+      IF (DO_CFL) CALL DGPU_CHECK(dgpu_courant_request(DGPU_CTX),
+     &     'dgpu_courant_request')
       CALL DGPU_CHECK(dgpu_rk_chan_1(DGPU_CTX, INT(RK_STEP, C_INT32_T),
      &     REAL(TIME, C_DOUBLE), H_BAR, BETA_BAR, ZETA_BAR),
      &     'dgpu_rk_chan_1')
+      IF (DO_CFL) CALL DGPU_CHECK(dgpu_courant(DGPU_CTX,
+     &     REAL(CFL, C_DOUBLE), DELTA_T, H_BAR), 'dgpu_courant')
       IF (RK_STEP .EQ. 3 .AND. .NOT. DGPU_RESIDENT) THEN
         CALL DGPU_SYNC_TO_HOST
         DGPU_ON_DEVICE = .FALSE.

@@ -56,7 +56,7 @@ EXPORTS = (
     "dgpu_set_kernel_timing", "dgpu_kernel_times", "dgpu_set_graphs",
     "dgpu_ensemble_create", "dgpu_ensemble_destroy", "dgpu_ensemble_size", "dgpu_ensemble_member",
     "dgpu_ensemble_last_error", "dgpu_ensemble_set_graphs", "dgpu_ensemble_rk_chan_1",
-    "dgpu_ensemble_synchronize", "dgpu_stats_chan",
+    "dgpu_ensemble_synchronize", "dgpu_stats_chan", "dgpu_courant_request", "dgpu_courant",
 )
 
 # dgpu_stats_chan profiles, in the order of the header's DGPU_ST_*
@@ -118,6 +118,10 @@ def load_library(path: str = LIB_PATH):
     L.dgpu_kernel_times.argtypes = [vp, DP, C.POINTER(C.c_int64), i32]
     L.dgpu_stats_chan.restype = i32
     L.dgpu_stats_chan.argtypes = [vp, DP, DP]
+    L.dgpu_courant_request.restype = i32
+    L.dgpu_courant_request.argtypes = [vp]
+    L.dgpu_courant.restype = i32
+    L.dgpu_courant.argtypes = [vp, C.c_double, C.POINTER(C.c_double), DP]
     _lib = L
     return L
 
@@ -240,12 +244,36 @@ class GpuChannel:
         self._chk(self.L.dgpu_rk_chan_1(self.h, rk_step, t, _dp(self.h_bar), _dp(self.beta_bar),
                                         _dp(self.zeta_bar)), "dgpu_rk_chan_1")
 
-    def step(self, n: int = 1):
-        """n time steps: RK_STEP = 1..3 each, then TIME += DELTA_T (for/diablo.f:70-109)."""
+    def step(self, n: int = 1, courant_cfl: float | None = None):
+        """n time steps: RK_STEP = 1..3 each, then TIME += DELTA_T (for/diablo.f:70-109).
+        courant_cfl: VARIABLE_DT with UPDATE_DT = 1 -- COURANT inside every RK_STEP 3
+        with that CFL; the new DELTA_T applies from the next step, and TIME advances
+        by it, as the reference's does."""
         for _ in range(n):
             for k in (1, 2, 3):
+                if k == 3 and courant_cfl is not None:
+                    self._chk(self.L.dgpu_courant_request(self.h), "dgpu_courant_request")
                 self.rk_chan_1(k)
-            self.time += self.cfg.delta_t
+            if courant_cfl is not None:
+                self.courant(courant_cfl)
+            self.time += self.delta_t
+
+    @property
+    def delta_t(self) -> float:
+        return getattr(self, "_dt", self.cfg.delta_t)
+
+    def set_delta_t(self, dt: float):
+        self._dt = float(dt)
+        self.h_bar = np.ascontiguousarray(rkw3_table(dt)[0])
+
+    def courant(self, cfl: float) -> float:
+        """DELTA_T from the maxima the last RK_STEP 3 recorded; updates H_BAR."""
+        dt = C.c_double()
+        hb = np.zeros(3)
+        self._chk(self.L.dgpu_courant(self.h, cfl, C.byref(dt), _dp(hb)), "dgpu_courant")
+        self._dt = dt.value
+        self.h_bar = hb
+        return dt.value
 
     def rem_div_chan(self):
         self._chk(self.L.dgpu_rem_div_chan(self.h), "dgpu_rem_div_chan")

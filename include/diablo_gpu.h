@@ -169,6 +169,21 @@ int32_t dgpu_fft_xz_to_fourier(dgpu_ctx *ctx, double *v, int32_t jmin, int32_t j
  * those rows and leaves rows NZ..NZ+1 untouched. */
 int32_t dgpu_fft_xz_to_physical(dgpu_ctx *ctx, double *vv, int32_t jmin, int32_t jmax);
 
+/* COURANT (for/courant.f:1-65), for VARIABLE_DT runs.  The reference calls it
+ * inside RK_CHAN_1 at RK_STEP = 3 when MOD(TIME_STEP, UPDATE_DT) = 0
+ * (for/channel.f:1037-1040), on the physical velocity after the implicit
+ * solves and before the projection; that step still uses the old DELTA_T.
+ * dgpu_courant_request makes the next dgpu_rk_chan_1 call with rk_step = 3
+ * record the per-plane maxima of |U1|, |U2|, |U3| of that velocity (in the
+ * precision of the run's transforms; the call runs without a graph).
+ * dgpu_courant then applies COURANT's limits -- viscous/diffusive, rotation,
+ * buoyancy, cfl * grid spacing / |velocity|, the 0.0001 and 999 caps -- and
+ * returns DELTA_T and H_BAR(1..3) (h_bar: 3 doubles) for the caller's next
+ * steps; it waits for the step-3 call to finish.  DGPU_E_STATE when no
+ * maxima were recorded since the request. */
+int32_t dgpu_courant_request(dgpu_ctx *ctx);
+int32_t dgpu_courant(dgpu_ctx *ctx, double cfl, double *delta_t, double *h_bar);
+
 /* Waits for all work enqueued on the context. */
 int32_t dgpu_synchronize(dgpu_ctx *ctx);
 

@@ -164,6 +164,8 @@ struct dor_state {
     int JSTART, JEND, JSTART_TH[DOR_MAX_TH], JEND_TH[DOR_MAX_TH];
     double TIME;
     double H_BAR[4], BETA_BAR[4], ZETA_BAR[4]; /* 1-based */
+    int courant_pending;                       /* COURANT at the next RK_STEP 3 */
+    double courant_cfl;
     double *GX, *GZ, *GY, *GYF, *DY, *DYF;     /* 0:N+1 */
     double *KX, *KZ, *KX2, *KZ2;               /* KX 0:NXP, KZ 0:TNKZ */
     double *fld[64];
@@ -307,6 +309,56 @@ void dor_set_dt(dor_state *s, double dt)
     s->H_BAR[1] = dt * (8.0 / 15.0);
     s->H_BAR[2] = dt * (2.0 / 15.0);
     s->H_BAR[3] = dt * (5.0 / 15.0);
+}
+
+double dor_get_dt(const dor_state *s) { return s->p.delta_t; }
+
+void dor_courant_request(dor_state *s, double cfl)
+{
+    s->courant_pending = 1;
+    s->courant_cfl = cfl;
+}
+
+/* COURANT (for/courant.f:1-65) on the physical U1, U2, U3 arrays as they
+ * stand: the smallest of the viscous/diffusive, rotation and buoyancy limits
+ * and cfl * (grid spacing) / |velocity| over j = 1..NY, k = 0..NZP-1,
+ * i = 0..NXM; sets DELTA_T (0.0001 if dt <= 0, 999 if dt >= 999) and H_BAR.
+ * A zero velocity gives an infinite quotient, which the minimum ignores. */
+double dor_courant(dor_state *s, double cfl)
+{
+    const int NY = s->NY, NZP = s->NZP, NXM = s->NXM, NTH = s->NTH;
+    const double NU = s->p.nu;
+    const double PI = 4.0 * atan(1.0);
+    const double dx = s->p.lx / s->NX, dz = s->p.lz / s->NZ;   /* DX(I), DZ(K) (for/channel.f:1574-1583) */
+    const double *DY = s->DY;
+    const double *U1 = s->fld[DOR_U1], *U2 = s->fld[DOR_U2], *U3 = s->fld[DOR_U3];
+    double dt = 999.0;
+    const double h = dx < DY[1] ? dx : DY[1];
+    dt = fmin(dt, 0.5 * pow(h, 2.0) / NU);
+    for (int n = 0; n < NTH; n++) dt = fmin(dt, dt * NU / (NU / s->p.pr[n]));
+    if (s->p.i_ro != 0.0) dt = fmin(dt, 2.0 * PI / fabs(s->p.i_ro) / 20.0);
+    for (int n = 0; n < NTH; n++) {
+        if (s->p.ri[n] != 0.0) {
+            const double Nmax = sqrt(fabs(s->p.ri[n] * s->p.th_bc_ymin_c1[n]));
+            /* the literal 0.1 is default (single precision) REAL */
+            dt = fmin(dt, (double)0.1f * 2.0 * PI / Nmax);
+        }
+    }
+    for (int j = 1; j <= NY; j++)
+        for (int k = 0; k <= NZP - 1; k++)
+            for (int i = 0; i <= NXM; i++) {
+                const size_t x = RIDX(i, k, j);
+                const double dt_x = cfl * dx / fabs(U1[x]);
+                const double dt_y = cfl * DY[j] / fabs(U2[x]);
+                const double dt_z = cfl * dz / fabs(U3[x]);
+                dt = fmin(dt, fmin(dt_x, fmin(dt_y, dt_z)));
+            }
+    double DELTA_T;
+    if (dt <= 0.0) DELTA_T = 0.0001;
+    else if (dt >= 999.0) DELTA_T = 999.0;
+    else DELTA_T = dt;
+    dor_set_dt(s, DELTA_T);
+    return DELTA_T;
 }
 
 void dor_set_threads(int n)
@@ -979,7 +1031,13 @@ void dor_rk_chan_1(dor_state *s, int RK_STEP)
     cn_solve_phys(s, SYS_U2, 0, TEMP1);
     cn_solve_phys(s, SYS_U1, 0, TEMP1);
     cn_solve_phys(s, SYS_U3, 0, TEMP1);
-    /* COURANT only if VARIABLE_DT (fixed dt here). */
+    /* VARIABLE_DT: COURANT on the physical velocity after the implicit solves,
+     * before the projection (for/channel.f:1037-1040); the caller asks for it
+     * when MOD(TIME_STEP, UPDATE_DT) = 0.  TEMP1..TEMP5 keep this step's dt. */
+    if (RK_STEP == 3 && s->courant_pending) {
+        dor_courant(s, s->courant_cfl);
+        s->courant_pending = 0;
+    }
 
     /* Back to Fourier space (for/channel.f:1043-1050). */
     dor_fft_xz_to_fourier(s, U1, 0, NY + 1);

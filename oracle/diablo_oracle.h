@@ -70,6 +70,14 @@ int dor_info(const dor_state *s, int what);  /* 0 jstart 1 jend 2 nkx 3 nkz 4 tn
 void dor_set_time(dor_state *s, double time);
 void dor_set_dt(dor_state *s, double dt);
 void dor_set_threads(int n);
+double dor_get_dt(const dor_state *s);
+
+/* COURANT (for/courant.f:1-65) on the current physical U1..U3 arrays: sets
+ * and returns DELTA_T and sets H_BAR.  dor_courant_request makes the next
+ * dor_rk_chan_1(s, 3) call COURANT where RK_CHAN_1 does under VARIABLE_DT
+ * (for/channel.f:1037-1040), on the velocity before the projection. */
+double dor_courant(dor_state *s, double cfl);
+void dor_courant_request(dor_state *s, double cfl);
 
 /* The reference call surface (for/mpi.f:1530-1674, for/channel.f:62-1453). */
 void dor_fft_xz_to_fourier(dor_state *s, double *v, int jmin, int jmax);

@@ -69,6 +69,11 @@ def lib():
         L.dor_info.argtypes = [vp, i]
         L.dor_set_time.argtypes = [vp, d]
         L.dor_set_dt.argtypes = [vp, d]
+        L.dor_get_dt.restype = d
+        L.dor_get_dt.argtypes = [vp]
+        L.dor_courant.restype = d
+        L.dor_courant.argtypes = [vp, d]
+        L.dor_courant_request.argtypes = [vp, d]
         L.dor_set_threads.argtypes = [i]
         for name in ("dor_fft_xz_to_fourier", "dor_fft_xz_to_physical"):
             getattr(L, name).argtypes = [vp, dp, i, i]
@@ -166,11 +171,27 @@ class Oracle:
         self.L.dor_set_time(self.h, self.time if time is None else time)
         self.L.dor_rk_chan_1(self.h, rk_step)
 
-    def step(self):
-        """One time step: RK_STEP = 1..3 then TIME += DELTA_T (for/diablo.f:70-109)."""
+    def step(self, courant_cfl: float | None = None):
+        """One time step: RK_STEP = 1..3 then TIME += DELTA_T (for/diablo.f:70-109).
+        courant_cfl: run COURANT inside RK_STEP 3 with that CFL (VARIABLE_DT on a
+        step with MOD(TIME_STEP, UPDATE_DT) = 0); TIME then advances by the new
+        DELTA_T, as the reference's does."""
         for k in (1, 2, 3):
+            if k == 3 and courant_cfl is not None:
+                self.L.dor_courant_request(self.h, courant_cfl)
             self.rk_chan_1(k)
-        self.time += self.cfg.delta_t
+        self.time += self.delta_t
+
+    @property
+    def delta_t(self) -> float:
+        return float(self.L.dor_get_dt(self.h))
+
+    def set_delta_t(self, dt: float):
+        self.L.dor_set_dt(self.h, dt)
+
+    def courant(self, cfl: float) -> float:
+        """COURANT on the physical U1..U3 arrays as they stand; returns DELTA_T."""
+        return float(self.L.dor_courant(self.h, cfl))
 
     def rem_div_chan(self):
         self.L.dor_rem_div_chan(self.h)

@@ -274,6 +274,65 @@ def test_stats_chan(oracle_mod, gpu_mod, maker):
         assert err < 1e-5, (k, err)
 
 
+@pytest.mark.parametrize("maker", ["channel", "kh", "sink"])
+def test_courant_variable_dt(oracle_mod, gpu_mod, maker):
+    """VARIABLE_DT, UPDATE_DT = 1: each step's new DELTA_T from the device's
+    maxima of the pre-projection velocity against the oracle's COURANT, and
+    the states after steps of changing length."""
+    cfg = {"channel": lambda: case.channel_case(48, 33, 48, kick=0.3, streak=0.1),
+           "kh": lambda: case.kh_case(48, 33, 48, kick=0.05, dt=0.01),
+           "sink": lambda: kitchen_sink(48, 25, 36)}[maker]()
+    say("courant", maker)
+    o, g = make_pair(oracle_mod, gpu_mod, cfg)
+    cfl = 0.05
+    for step in range(6):
+        o.step(courant_cfl=cfl)
+        g.step(courant_cfl=cfl)
+        say(step, o.delta_t, g.delta_t)
+        assert g.delta_t == pytest.approx(o.delta_t, rel=1e-12)
+        assert np.allclose(g.h_bar, o.delta_t * np.array([8.0, 2.0, 5.0]) / 15.0, rtol=1e-12, atol=0)
+    assert o.delta_t != cfg.delta_t
+    # the velocity limit is the one that binds (not the diffusive one)
+    probe = oracle_mod.Oracle(cfg)
+    for f in (oracle_mod.U1, oracle_mod.U2, oracle_mod.U3):
+        probe.field(f)[...] = 0.0
+    assert o.delta_t < probe.courant(cfl)
+    assert g.time == pytest.approx(o.time, rel=1e-13)
+    g.synchronize()
+    errs, bad = compare_states(cfg, g.state(), o.state(), 1e-10)
+    say(errs)
+    assert not bad, bad
+
+
+def test_courant_mixed_and_state_errors(gpu_mod):
+    """The mixed path's DELTA_T (maxima of its FP32 velocity) within 1e-5 of
+    the FP64 path's; dgpu_courant without maxima recorded is a state error,
+    and a request is consumed by the RK_STEP 3 call alone."""
+    cfg = case.channel_case(48, 33, 48, kick=0.3, streak=0.1)
+    phys = case.initial_physical(cfg)
+    g64 = gpu_mod.GpuChannel(cfg)
+    gm = gpu_mod.GpuChannel(cfg, precision="mixed")
+    g64.initialize(*phys)
+    gm.initialize(*phys)
+    with pytest.raises(gpu_mod.DgpuError):
+        g64.courant(0.05)
+    g64._chk(g64.L.dgpu_courant_request(g64.h), "request")
+    g64.rk_chan_1(1)
+    with pytest.raises(gpu_mod.DgpuError):
+        g64.courant(0.05)
+    g64.rk_chan_1(2)
+    g64.rk_chan_1(3)
+    dt = g64.courant(0.05)
+    with pytest.raises(gpu_mod.DgpuError):
+        g64.courant(0.05)
+    gm.step(1, courant_cfl=0.05)
+    assert gm.delta_t == pytest.approx(dt, rel=1e-5)
+    for _ in range(3):
+        g64.step(1, courant_cfl=0.05)
+        gm.step(1, courant_cfl=0.05)
+        assert gm.delta_t == pytest.approx(g64.delta_t, rel=1e-5)
+
+
 def test_graph_replays_match_eager(gpu_mod):
     """Graph replays (several, with host synchronisations and eager work in
     between) give bit-identical states to eager launches."""
