@@ -88,6 +88,7 @@ struct dgpu_ctx {
     // COURANT: 0 idle, 1 requested for the next RK_STEP 3, 2 maxima recorded
     int courant_state = 0;
     double *d_vmax_part = nullptr, *d_vmax = nullptr;  // per-block, per-plane velocity maxima
+    double *d_file = nullptr;  // one restart-file array, NX*NY*NZ
     // CN tables: [stage 0..2][sys][NY+2][kRow]
     double *d_cn = nullptr;
     std::vector<double> h_cn;
@@ -709,7 +710,7 @@ int32_t dgpu_destroy(dgpu_ctx *c)
     void *bufs[] = {c->d_vec, c->d_tw, c->d_state, c->d_pmult, c->d_pinv, c->d_rd, c->d_nlptr, c->d_zt, c->d_wt,
                     c->d_fa, c->d_fb, c->d_gyf, c->d_iptw, c->d_iptwf, c->d_ztf, c->d_wtf, c->d_nlf, c->d_nlptrf,
                     c->d_fort, c->d_cn, c->d_codes, c->d_c1, c->d_state32, c->d_side, c->d_map,
-                    c->d_pmultf, c->d_pinvf, c->d_gs32, c->d_stats, c->d_vmax_part, c->d_vmax};
+                    c->d_pmultf, c->d_pinvf, c->d_gs32, c->d_stats, c->d_vmax_part, c->d_vmax, c->d_file};
     for (void *b : bufs)
         if (b) hipFree(b);
     if (c->side_stream) {
@@ -744,6 +745,63 @@ int32_t dgpu_stats_chan(dgpu_ctx *c, double *profiles, double *ubulk)
     double res = 0.;
     for (int j = 2; j <= g.ny; j++) res = res + 0.5 * (ume[j - 1] + ume[j - 2]) * c->dy[j];
     *ubulk = res / c->p.ly;
+    return DGPU_OK;
+}
+
+static int fort_to_physical(dgpu_ctx *c, double *buf, int j0, int j1);
+static int fort_to_fourier(dgpu_ctx *c, double *buf, int j0, int j1);
+
+// WriteHDF5 / ReadHDF5's per-field conversion (for/hdf5.f:172-268, 470-534).
+static cx<double> *flow_field(dgpu_ctx *c, int32_t field)
+{
+    if (field >= DGPU_CU1 && field <= DGPU_CU3) return c->cu[field - DGPU_CU1];
+    if (field == DGPU_CP) return c->cp;
+    if (field >= DGPU_CTH0 && field < DGPU_CTH0 + c->g.nth) return c->cth[field - DGPU_CTH0];
+    return nullptr;
+}
+
+static int ensure_file(dgpu_ctx *c)
+{
+    if (!c->d_file) HIPCHK(c, hipMalloc(&c->d_file, sizeof(double) * c->g.nx * c->g.ny * c->g.nz));
+    return ensure_fort(c);
+}
+
+int32_t dgpu_get_flow_physical(dgpu_ctx *c, int32_t field, double *out)
+{
+    if (!c || !out) return DGPU_E_ARG;
+    cx<double> *src = flow_field(c, field);
+    if (!src) return fail(c, DGPU_E_ARG, "field must be CU1..CU3, CP or CTH");
+    if (int rc = ensure_file(c)) return rc;
+    if (int rc = to_master(c)) return rc;
+    const dgpu::Geo &g = c->g;
+    const long plane = (long)(g.nx / 2 + 1) * (g.nz + 2);
+    hipLaunchKernelGGL(k_pack_fortran, dim3((unsigned)((plane + 255) / 256), g.ny + 2), dim3(256), 0, c->stream, src,
+                       c->d_fort, g.nx / 2, g.nz, g.mk, g.mz);
+    if (int rc = check_launch(c, "pack")) return rc;
+    if (int rc = fort_to_physical(c, c->d_fort, 0, g.ny + 1)) return rc;
+    dgpu::launch_to_file(c->stream, g, c->d_fort, c->d_file, field == DGPU_CU2);
+    if (int rc = check_launch(c, "to_file")) return rc;
+    HIPCHK(c, hipMemcpyAsync(out, c->d_file, sizeof(double) * g.nx * g.ny * g.nz, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return DGPU_OK;
+}
+
+int32_t dgpu_set_flow_physical(dgpu_ctx *c, int32_t field, const double *in)
+{
+    if (!c || !in) return DGPU_E_ARG;
+    cx<double> *dst = flow_field(c, field);
+    if (!dst) return fail(c, DGPU_E_ARG, "field must be CU1..CU3, CP or CTH");
+    if (int rc = ensure_file(c)) return rc;
+    if (int rc = to_master(c)) return rc;
+    c->master_dirty = true;
+    const dgpu::Geo &g = c->g;
+    HIPCHK(c, hipMemcpyAsync(c->d_file, in, sizeof(double) * g.nx * g.ny * g.nz, hipMemcpyHostToDevice, c->stream));
+    dgpu::launch_from_file(c->stream, g, c->d_file, c->d_fort, field == DGPU_CU2);
+    if (int rc = check_launch(c, "from_file")) return rc;
+    if (int rc = fort_to_fourier(c, c->d_fort, 0, g.ny + 1)) return rc;
+    dgpu::launch_unpack_fortran(c->stream, g, c->d_fort, dst, 0, g.ny + 2);
+    if (int rc = check_launch(c, "unpack")) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return DGPU_OK;
 }
 

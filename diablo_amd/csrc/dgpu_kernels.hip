@@ -3,6 +3,8 @@
 // POISSON_P_CHAN, plus APPLY_BC_VEL on kept-mode fields.  The physical-space
 // side of these transforms is in natural order (Fortran-layout planes).  The
 // substep's own transforms are in dgpu_hot.hip.
+#include <algorithm>
+
 #include "dgpu_kernels.hpp"
 
 namespace dgpu {
@@ -375,6 +377,102 @@ void launch_apply_bc_vel(hipStream_t s, const Geo &g, cx<double> *cu1, cx<double
     const long blocks = (g.ps + threads - 1) / threads;
     hipLaunchKernelGGL(k_apply_bc_vel, dim3(blocks), dim3(threads), 0, s, cu1, cu2, cu3, g.ps, g.ny, codes, c1, dy,
                        dyf);
+}
+
+// ---- restart-file layout (WriteHDF5 / ReadHDF5, for/hdf5.f:172-268, 470-534) ----
+namespace {
+
+__device__ __forceinline__ long fidx(int x, int z, int y, int nx, int nz)
+{
+    return (long)x + (long)(nx + 2) * ((long)z + (long)(nz + 2) * (long)y);
+}
+
+// SWAPZY (for/hdf5.f:1243-1258) of the physical field, planes 1..NY, into
+// out(1:NX, 1:NY, 1:NZ); with g2gf, G2GF first (for/hdf5.f:1176-1192): plane
+// y is 0.5 (v'(y) + v'(y+1)) with v'(1) = v(2), v'(NY+1) = v(NY).
+__global__ void k_to_file(const double *__restrict__ fort, double *__restrict__ out, int nx, int ny, int nz,
+                          int g2gf)
+{
+    const long n = (long)nx * ny * nz;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+        const int x = (int)(e % nx);
+        const long r = e / nx;
+        const int y = 1 + (int)(r % ny), z = (int)(r / ny);
+        double v;
+        if (g2gf) {
+            const double a = fort[fidx(x, z, y == 1 ? 2 : y, nx, nz)];
+            const double b = fort[fidx(x, z, y == ny ? ny : y + 1, nx, nz)];
+            v = 0.5 * (a + b);
+        } else {
+            v = fort[fidx(x, z, y, nx, nz)];
+        }
+        out[e] = v;
+    }
+}
+
+// SWAPYZ (for/hdf5.f:1262-1278): every element of the Fortran-layout field,
+// zero outside x < NX, z < NZ, 1 <= y <= NY.
+__global__ void k_from_file(const double *__restrict__ in, double *__restrict__ fort, int nx, int ny, int nz)
+{
+    const long n = (long)(nx + 2) * (nz + 2) * (ny + 2);
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+        const int x = (int)(e % (nx + 2));
+        const long r = e / (nx + 2);
+        const int z = (int)(r % (nz + 2)), y = (int)(r / (nz + 2));
+        double v = 0.0;
+        if (x < nx && z < nz && y >= 1 && y <= ny) v = in[(long)x + (long)nx * ((long)(y - 1) + (long)ny * z)];
+        fort[e] = v;
+    }
+}
+
+// GF2G (for/hdf5.f:1196-1238), one (x, z) column per thread: v(NY+1) = v(NY),
+// then v(y) = 2 v(y) - v(y+1) for y = NY..1.
+__global__ void k_gf2g_cols(double *__restrict__ fort, int nx, int ny, int nz)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nx * nz) return;
+    const int x = e % nx, z = e / nx;
+    double up = fort[fidx(x, z, ny, nx, nz)];
+    fort[fidx(x, z, ny + 1, nx, nz)] = up;
+    for (int y = ny; y >= 1; --y) {
+        const long q = fidx(x, z, y, nx, nz);
+        const double v = 2.0 * fort[q] - up;
+        fort[q] = v;
+        up = v;
+    }
+}
+
+// ... then the zero-mass-flux ghost planes over whole planes, pads included:
+// v(:,:,1) = -v(:,:,2), v(:,:,NY+1) = -v(:,:,NY).
+__global__ void k_gf2g_walls(double *__restrict__ fort, int nx, int ny, int nz)
+{
+    const long plane = (long)(nx + 2) * (nz + 2);
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < plane; e += (long)gridDim.x * blockDim.x) {
+        fort[plane + e] = -fort[2 * plane + e];
+        fort[(long)(ny + 1) * plane + e] = -fort[(long)ny * plane + e];
+    }
+}
+
+}  // namespace
+
+void launch_to_file(hipStream_t s, const Geo &g, const double *fort, double *out, bool g2gf)
+{
+    const long n = (long)g.nx * g.ny * g.nz;
+    const int blocks = (int)std::min<long>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_to_file, dim3(blocks), dim3(256), 0, s, fort, out, g.nx, g.ny, g.nz, g2gf ? 1 : 0);
+}
+
+void launch_from_file(hipStream_t s, const Geo &g, const double *in, double *fort, bool gf2g)
+{
+    const long n = (long)(g.nx + 2) * (g.nz + 2) * (g.ny + 2);
+    const int blocks = (int)std::min<long>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_from_file, dim3(blocks), dim3(256), 0, s, in, fort, g.nx, g.ny, g.nz);
+    if (gf2g) {
+        hipLaunchKernelGGL(k_gf2g_cols, dim3((g.nx * g.nz + 255) / 256), dim3(256), 0, s, fort, g.nx, g.ny, g.nz);
+        const long plane = (long)(g.nx + 2) * (g.nz + 2);
+        hipLaunchKernelGGL(k_gf2g_walls, dim3((unsigned)((plane + 255) / 256)), dim3(256), 0, s, fort, g.nx, g.ny,
+                           g.nz);
+    }
 }
 
 // explicit instantiations: FP64 path and the mixed path (FP32 transforms)

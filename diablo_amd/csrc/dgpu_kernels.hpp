@@ -197,6 +197,10 @@ void launch_stats(hipStream_t s, const Geo &g, const cx<double> *cu1, const cx<d
                   const cx<double> *cp, const cx<double> *cth, const double *dy, const double *dyf,
                   const double *gyf, const double *kx, const double *kz, double *out);
 void launch_unpack_fortran(hipStream_t s, const Geo &g, const double *fort, cx<double> *kept, int j0, int nplanes);
+// Restart-file layout: out(1:NX, 1:NY, 1:NZ) from the physical Fortran-layout
+// field (G2GF first for V), and back (GF2G for V).
+void launch_to_file(hipStream_t s, const Geo &g, const double *fort, double *out, bool g2gf);
+void launch_from_file(hipStream_t s, const Geo &g, const double *in, double *fort, bool gf2g);
 // Mixed path state copies (dgpu_columns.hip): dst[k] = src[k], k < n.
 template <typename TO, typename FROM>
 void launch_convert(hipStream_t s, const cx<FROM> *src, cx<TO> *dst, long n);

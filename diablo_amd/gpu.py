@@ -57,6 +57,7 @@ EXPORTS = (
     "dgpu_ensemble_create", "dgpu_ensemble_destroy", "dgpu_ensemble_size", "dgpu_ensemble_member",
     "dgpu_ensemble_last_error", "dgpu_ensemble_set_graphs", "dgpu_ensemble_rk_chan_1",
     "dgpu_ensemble_synchronize", "dgpu_stats_chan", "dgpu_courant_request", "dgpu_courant",
+    "dgpu_get_flow_physical", "dgpu_set_flow_physical",
 )
 
 # dgpu_stats_chan profiles, in the order of the header's DGPU_ST_*
@@ -118,6 +119,10 @@ def load_library(path: str = LIB_PATH):
     L.dgpu_kernel_times.argtypes = [vp, DP, C.POINTER(C.c_int64), i32]
     L.dgpu_stats_chan.restype = i32
     L.dgpu_stats_chan.argtypes = [vp, DP, DP]
+    L.dgpu_get_flow_physical.restype = i32
+    L.dgpu_get_flow_physical.argtypes = [vp, i32, DP]
+    L.dgpu_set_flow_physical.restype = i32
+    L.dgpu_set_flow_physical.argtypes = [vp, i32, DP]
     L.dgpu_courant_request.restype = i32
     L.dgpu_courant_request.argtypes = [vp]
     L.dgpu_courant.restype = i32
@@ -283,6 +288,20 @@ class GpuChannel:
 
     def apply_bc_vel(self):
         self._chk(self.L.dgpu_apply_bc_vel(self.h), "dgpu_apply_bc_vel")
+
+    def get_flow_physical(self, fid: int) -> np.ndarray:
+        """WriteHDF5's array for field fid (CU1..CU3, CP, CTH0+n): (NZ, NY, NX)
+        C order = tmp(1:NX, 1:NY, 1:NZ) in Fortran order."""
+        out = np.zeros((self.cfg.nz, self.cfg.ny, self.cfg.nx))
+        self._chk(self.L.dgpu_get_flow_physical(self.h, fid, _dp(out)), "dgpu_get_flow_physical")
+        return out
+
+    def set_flow_physical(self, fid: int, a: np.ndarray):
+        """ReadHDF5's conversion of a file array into field fid."""
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if a.shape != (self.cfg.nz, self.cfg.ny, self.cfg.nx):
+            raise ValueError(f"file array must be (NZ, NY, NX), got {a.shape}")
+        self._chk(self.L.dgpu_set_flow_physical(self.h, fid, _dp(a)), "dgpu_set_flow_physical")
 
     def stats_chan(self) -> tuple[dict, float]:
         """SAVE_STATS_CHAN's statistics from the device state:

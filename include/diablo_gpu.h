@@ -169,6 +169,19 @@ int32_t dgpu_fft_xz_to_fourier(dgpu_ctx *ctx, double *v, int32_t jmin, int32_t j
  * those rows and leaves rows NZ..NZ+1 untouched. */
 int32_t dgpu_fft_xz_to_physical(dgpu_ctx *ctx, double *vv, int32_t jmin, int32_t jmax);
 
+/* Restart files: the per-field conversion of WriteHDF5 and ReadHDF5
+ * (for/hdf5.f:172-268, 470-534; the HDF5 calls, the Resolution and Time
+ * attributes and the file itself stay with the caller).  field is DGPU_CU1
+ * (dataset U), DGPU_CU2 (V), DGPU_CU3 (W), DGPU_CP (P) or DGPU_CTH0 + n
+ * (TH<n+1>); the file array is tmp(1:NX, 1:NY, 1:NZ), x fastest, NX*NY*NZ
+ * doubles.  get: FFT_XZ_TO_PHYSICAL of the device field, G2GF for V, SWAPZY;
+ * the device state is not changed (the reference's round trip through
+ * physical space changes it by round-off only).  set: SWAPYZ, GF2G for V
+ * (ghost planes V(1) = -V(2), V(NY+1) = -V(NY)), FFT_XZ_TO_FOURIER into the
+ * device field's kept slots, planes 0..NY+1. */
+int32_t dgpu_get_flow_physical(dgpu_ctx *ctx, int32_t field, double *out);
+int32_t dgpu_set_flow_physical(dgpu_ctx *ctx, int32_t field, const double *in);
+
 /* COURANT (for/courant.f:1-65), for VARIABLE_DT runs.  The reference calls it
  * inside RK_CHAN_1 at RK_STEP = 3 when MOD(TIME_STEP, UPDATE_DT) = 0
  * (for/channel.f:1037-1040), on the physical velocity after the implicit

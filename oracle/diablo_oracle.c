@@ -1472,3 +1472,67 @@ void dor_stats_chan(dor_state *s, double *out, double *ubulk)
     free(u1); free(u2); free(u3); free(th); free(w); free(w2); free(w3);
     free(ume); free(vme); free(wme); free(thme); free(acc);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Restart-file layout of WriteHDF5 / ReadHDF5 (for/hdf5.f:172-268,          */
+/* 470-534) at NPROCY = NPROCZ = 1: the physical field, SWAPZY'd to          */
+/* tmp(1:NX, 1:NY, 1:NZ) (x fastest, then y, then z), V interpolated to the  */
+/* fractional grid by G2GF first (for/hdf5.f:1176-1192); reading SWAPYZ's    */
+/* the file array back (zeroing the pads and planes 0 and NY+1), applies     */
+/* GF2G to V (for/hdf5.f:1196-1238) and transforms to Fourier space.         */
+/* The field id selects the Fourier array: U1/U2/U3/P or CTH0 + n.           */
+/* ------------------------------------------------------------------------ */
+static void g2gf(dor_state *s, double *v)
+{
+    const int NX = s->NX, NY = s->NY, NZP = s->NZP;
+    for (int x = 0; x <= NX - 1; x++)
+        for (int z = 0; z <= NZP - 1; z++) {
+            v[RIDX(x, z, 1)] = v[RIDX(x, z, 2)];
+            v[RIDX(x, z, NY + 1)] = v[RIDX(x, z, NY)];
+            for (int y = 1; y <= NY; y++) v[RIDX(x, z, y)] = 0.5 * (v[RIDX(x, z, y)] + v[RIDX(x, z, y + 1)]);
+        }
+}
+
+static void gf2g(dor_state *s, double *v)
+{
+    const int NX = s->NX, NY = s->NY, NZP = s->NZP, NZ = s->NZ;
+    for (int x = 0; x <= NX - 1; x++)
+        for (int z = 0; z <= NZP - 1; z++) v[RIDX(x, z, NY + 1)] = v[RIDX(x, z, NY)];
+    for (int x = 0; x <= NX - 1; x++)
+        for (int z = 0; z <= NZP - 1; z++)
+            for (int y = NY; y >= 1; y--) v[RIDX(x, z, y)] = 2.0 * v[RIDX(x, z, y)] - v[RIDX(x, z, y + 1)];
+    /* var(:,:,1) = -var(:,:,2) over the whole plane, pads included */
+    for (int z = 0; z <= NZ + 1; z++)
+        for (int x = 0; x <= NX + 1; x++) {
+            v[RIDX(x, z, 1)] = -v[RIDX(x, z, 2)];
+            v[RIDX(x, z, NY + 1)] = -v[RIDX(x, z, NY)];
+        }
+}
+
+void dor_flow_to_file(dor_state *s, int id, double *out)
+{
+    const int NX = s->NX, NY = s->NY, NZP = s->NZP;
+    double *w = zalloc(s->fsize);
+    memcpy(w, s->fld[id], sizeof(double) * s->fsize);
+    dor_fft_xz_to_physical(s, w, 0, NY + 1);
+    if (id == DOR_U2) g2gf(s, w);
+    for (int x = 0; x <= NX - 1; x++)        /* SWAPZY */
+        for (int y = 1; y <= NY; y++)
+            for (int z = 0; z <= NZP - 1; z++)
+                out[(size_t)x + (size_t)NX * ((size_t)(y - 1) + (size_t)NY * (size_t)z)] = w[RIDX(x, z, y)];
+    free(w);
+}
+
+void dor_flow_from_file(dor_state *s, int id, const double *in)
+{
+    const int NX = s->NX, NY = s->NY, NZP = s->NZP;
+    double *w = zalloc(s->fsize);           /* SWAPYZ: out = 0 first */
+    for (int x = 0; x <= NX - 1; x++)
+        for (int y = 1; y <= NY; y++)
+            for (int z = 0; z <= NZP - 1; z++)
+                w[RIDX(x, z, y)] = in[(size_t)x + (size_t)NX * ((size_t)(y - 1) + (size_t)NY * (size_t)z)];
+    if (id == DOR_U2) gf2g(s, w);
+    dor_fft_xz_to_fourier(s, w, 0, NY + 1);
+    memcpy(s->fld[id], w, sizeof(double) * s->fsize);
+    free(w);
+}

@@ -102,6 +102,13 @@ enum {
 };
 void dor_stats_chan(dor_state *s, double *out, double *ubulk);
 
+/* WriteHDF5 / ReadHDF5's field layout (for/hdf5.f:172-268, 470-534): id is
+ * DOR_U1, DOR_U2, DOR_U3, DOR_P or DOR_CTH0 + n; the file array is
+ * tmp(1:NX, 1:NY, 1:NZ), x fastest; V is on the fractional grid (G2GF on
+ * writing, GF2G on reading).  Writing leaves the state unchanged. */
+void dor_flow_to_file(dor_state *s, int id, double *out);
+void dor_flow_from_file(dor_state *s, int id, const double *in);
+
 void dor_thomas_real(double *a, double *b, double *c, double *g, int ny, int nx);
 void dor_thomas_complex(double *a, double *b, double *c, double *g_c, int ny, int nxp);
 void dor_fft_c2c(double *data, int n, int sign);   /* unnormalised, sign -1 forward */

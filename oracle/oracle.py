@@ -74,6 +74,8 @@ def lib():
         L.dor_courant.restype = d
         L.dor_courant.argtypes = [vp, d]
         L.dor_courant_request.argtypes = [vp, d]
+        L.dor_flow_to_file.argtypes = [vp, i, dp]
+        L.dor_flow_from_file.argtypes = [vp, i, dp]
         L.dor_set_threads.argtypes = [i]
         for name in ("dor_fft_xz_to_fourier", "dor_fft_xz_to_physical"):
             getattr(L, name).argtypes = [vp, dp, i, i]
@@ -212,6 +214,20 @@ class Oracle:
         ub = np.zeros(1)
         self.L.dor_stats_chan(self.h, _dp(out), _dp(ub))
         return {k: out[q * ny:(q + 1) * ny].copy() for q, k in enumerate(STATS)}, float(ub[0])
+
+    # -- restart-file layout (WriteHDF5 / ReadHDF5) --
+    def flow_to_file(self, fid: int) -> np.ndarray:
+        """The array WriteHDF5 writes for field fid, as (NZ, NY, NX) C order
+        (= tmp(1:NX, 1:NY, 1:NZ) in Fortran order)."""
+        out = np.zeros((self.cfg.nz, self.cfg.ny, self.cfg.nx))
+        self.L.dor_flow_to_file(self.h, fid, _dp(out))
+        return out
+
+    def flow_from_file(self, fid: int, a: np.ndarray):
+        """ReadHDF5's conversion of a file array into field fid."""
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        assert a.shape == (self.cfg.nz, self.cfg.ny, self.cfg.nx)
+        self.L.dor_flow_from_file(self.h, fid, _dp(a))
 
     # -- initial state: the CREATE_FLOW_CHAN / CREATE_TH_CHAN sequence --
     def initialize(self, u1, u2, u3, ths):
