@@ -254,13 +254,101 @@ DGPU_HD void ip_stage(const IpLen &f, int i, const cx<T> *__restrict__ tw, int t
 #pragma unroll
         for (int q = 0; q < R; ++q) v[u * R + q] = ld(u * R + q, b0 + q * L);
     });
+    if (L == 1) {
+        // the last stage of a DIF (first of a DIT) transform: every group has
+        // k = 0, whose twiddles are all 1
 #pragma unroll
-    for (int u = 0; u < NB; ++u) ip_group<T, R, SIGN, DIF>(v + u * R, tw + f.twoff[i], kk[u]);
+        for (int u = 0; u < NB; ++u) butterfly<T, R, SIGN>(v + u * R);
+    } else {
+#pragma unroll
+        for (int u = 0; u < NB; ++u) ip_group<T, R, SIGN, DIF>(v + u * R, tw + f.twoff[i], kk[u]);
+    }
 #pragma unroll
     for (int u = 0; u < NB; ++u) {
 #pragma unroll
         for (int q = 0; q < R; ++q) st(u * R + q, base[u] + q * L, v[u * R + q]);
     }
+}
+
+// Runs fn(s) for s = s0..s1-1; fully unrolled when UNROLL (a plan known at
+// compile time, whose stage count then is a constant).
+template <bool UNROLL, class F>
+DGPU_HD void stage_loop(int s0, int s1, F &&fn)
+{
+    if constexpr (UNROLL) {
+#pragma unroll
+        for (int s = s0; s < s1; ++s) fn(s);
+    } else {
+        for (int s = s0; s < s1; ++s) fn(s);
+    }
+}
+
+// Plans known at compile time.  const_plan(id) builds the same IpLen as the
+// host planner (ip_fill after a greedy factorisation, dgpu_plan.hpp) for the
+// lengths the kernels are specialised for; with such a plan every span,
+// radix and twiddle offset is a constant, divisions by power-of-two spans
+// become shifts and the stage loops unroll.  Id 0 is "runtime plan".
+struct ConstPlanSpec {
+    int n, e, r0;  // r0 > 0: outermost radix fixed (x plans), 0: greedy (z plans)
+    unsigned swz;
+    int cswz;
+};
+constexpr ConstPlanSpec kConstPlanSpecs[] = {
+    {0, 0, 0, 0u, 0},
+    // x plans (rows of the fused nonlinear kernel)
+    {384, 12, 6, 0xfda87520u, 0},  // 1: (6,4,4,4)
+    {192, 12, 6, 0xfda87520u, 0},  // 2: (6,4,4,2)
+    {96, 12, 6, 0xfda87520u, 0},   // 3: (6,4,4)
+    {128, 8, 4, 0x76543210u, 0},   // 4: (4,8,4)
+    {64, 16, 8, 0xfda87520u, 0},   // 5: (8,8), FP32
+    // z plans (column batches)
+    {384, 12, 0, 0u, 2},           // 6: (6,4,4,4), FP64
+    {384, 24, 0, 0u, 0},           // 7: (8,8,6), FP32
+    {64, 8, 0, 0u, 0},             // 8: (8,8)
+    {96, 12, 0, 0u, 0},            // 9: (6,4,4)
+    {128, 8, 0, 0u, 0},            // 10: (8,8,2)
+    {192, 24, 0, 0u, 0},           // 11: (8,8,3), FP32
+};
+constexpr int kNumConstPlans = (int)(sizeof(kConstPlanSpecs) / sizeof(kConstPlanSpecs[0]));
+
+constexpr IpLen const_plan(int id)
+{
+    IpLen f{};
+    if (id <= 0 || id >= kNumConstPlans) return f;
+    const ConstPlanSpec c = kConstPlanSpecs[id];
+    const int cand[6] = {8, 6, 5, 4, 3, 2};
+    int radix[kMaxStages] = {};
+    int ns = 0, m = c.n;
+    if (c.r0 > 0) {
+        radix[ns++] = c.r0;
+        m /= c.r0;
+    }
+    while (m > 1 && ns < kMaxStages) {
+        int r = 0;
+        for (int q = 0; q < 6 && !r; ++q)
+            if (c.e % cand[q] == 0 && m % cand[q] == 0) r = cand[q];
+        if (!r) return IpLen{};
+        radix[ns++] = r;
+        m /= r;
+    }
+    f.n = c.n;
+    f.nstages = ns;
+    f.e = c.e;
+    f.t = c.n / c.e;
+    int mm = c.n, off = 0;
+    for (int i = 0; i < ns; ++i) {
+        const int R = radix[i];
+        f.radix[i] = R;
+        f.span[i] = mm / R;
+        f.magic[i] = f.span[i] == 1 ? 0u : (unsigned)(((1ULL << 32) + f.span[i] - 1) / f.span[i]);
+        f.twoff[i] = off;
+        off += f.span[i] * (R - 1);
+        mm /= R;
+    }
+    f.twn = off;
+    f.swz = c.swz;
+    f.cswz = c.cswz;
+    return f;
 }
 
 // Calls fn(std::integral_constant<int, R>) for the runtime radix R (those

@@ -1,122 +1,9 @@
-// dgpu_hot.hip -- the transform kernels of one RK substep (RK_CHAN_1,
-// for/channel.f:62-1073), built on the in-place transforms of dgpu_fft.hpp.
-//
-//   zinv     CU1,CU2,CU3,CTH (kept modes) --z backward, DIF--> Zt[f][j][zp][kx]
-//   xfused   per z row zp, sweeping j with a window of three planes: x
-//            backward (DIF) of the fields packed in pairs, the eleven products
-//            of for/channel.f:410-571 and 670-720, x forward (DIT) of the
-//            products packed in pairs, written as G = -(i kx X(a) + X(b)) and
-//            H = -X(c) per momentum/scalar equation into Wt[o][j][zp][kx]
-//   zfwd_nl  NL_f = [Z(G_f) + i kz Z(H_f)] / (NX NZ) at the kept slots (DIT)
-//
-// zp and the x index inside xfused are positions of the in-place transforms
-// (digit-reversed), not z and x: every product is pointwise, and the forward
-// transforms take the same permuted order back, so the order never matters.
-//
-// Intermediate rows (Zt, Wt) are ld >= NKX+1 complex values long, ld a
-// multiple of 8, so a chunk of 8 columns is one 128-byte line.
-#include "dgpu_kernels.hpp"
+// dgpu_hot_x.hip -- the fused x kernel of one RK substep (the transforms and
+// products of RK_CHAN_1, for/channel.f:62-1073) and COURANT's velocity maxima
+// (for/courant.f:35-44), on the in-place transforms of dgpu_fft.hpp.
+#include "dgpu_hot.hpp"
 
 namespace dgpu {
-
-namespace {
-
-template <typename TO, typename FROM>
-__device__ __forceinline__ cx<TO> cvt_(cx<FROM> a) { return make_cx<TO>(TO(a.re), TO(a.im)); }
-
-// Kept slot of z frequency index f (0..NZ-1), or -1 in the dealiasing gap
-// (slot K <= NKZ holds kz = K, slot K > NKZ holds kz = K - (TNKZ+1),
-// for/mpi.f:1602-1606).
-__device__ __forceinline__ int slot_of_freq(int f, int nz, int nkz, int tnkz)
-{
-    if (f <= nkz) return f;
-    if (f >= nz - nkz) return f - nz + tnkz + 1;
-    return -1;
-}
-
-template <int NTH>
-struct XCount {
-    static constexpr int nin = 3 + NTH;         // u1 u2 u3 th...
-    static constexpr int npi = (nin + 1) / 2;   // backward pairs
-    static constexpr int npo = (8 + 3 * NTH + 1) / 2;  // forward pairs of the products
-};
-
-}  // namespace
-
-// -------------------------------------------------------------------------
-// zinv: W columns per block, T = fz.t threads per column.  Stage 0 reads the
-// kept slots straight from the state, the last stage writes Zt rows.
-// grid (column chunks, planes, fields), block W*T, LDS W*NZ values.
-// -------------------------------------------------------------------------
-template <typename T, int E, typename TIN>
-__device__ __forceinline__ void zinv_body(const IpLen &fz, const cx<T> *__restrict__ tw,
-                                          const cx<TIN> *__restrict__ in, long in_fs, long in_ps, int mk,
-                                          cx<T> *__restrict__ out, long out_fs, long out_ps, int ld, int j0, int nkz,
-                                          int tnkz, int lw, int bx, int by, int bz)
-{
-    extern __shared__ unsigned char smem_raw[];
-    cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
-    const int W = 1 << lw;
-    const int tid = threadIdx.x;
-    const int b = tid & (W - 1), th = tid >> lw;
-    const int i = bx * W + b;
-    const bool col = i < mk;
-    const int j = j0 + by;
-    const cx<TIN> *src = in + bz * in_fs + (long)j * in_ps;
-    cx<T> *dst = out + bz * out_fs + (long)j * out_ps;
-    const int nz = fz.n, S = fz.nstages;
-    cx<T> *x = lds + b;
-    with_radix<E>(fz.radix[0], [&](auto rc) {
-        constexpr int R = decltype(rc)::value;
-        ip_stage<T, E, R, +1, true>(
-            fz, 0, tw, th,
-            [&](int, int f) {
-                // unconditional load at a valid index, then select: keeps it
-                // one 16-byte access
-                const int s = slot_of_freq(f, nz, nkz, tnkz);
-                const cx<TIN> v = src[(long)max(s, 0) * mk + (col ? i : mk - 1)];
-                return (col && s >= 0) ? cvt_<T>(v) : make_cx<T>(0, 0);
-            },
-            [&](int, int pos, cx<T> v) { x[ip_cswz(fz, pos) * W] = v; });
-    });
-    __syncthreads();
-    for (int s = 1; s < S - 1; ++s) {
-        ip_stage_mem<T, E, +1, true>(x, W, fz, s, tw, th);
-        __syncthreads();
-    }
-    with_radix<E>(fz.radix[S - 1], [&](auto rc) {
-        constexpr int R = decltype(rc)::value;
-        ip_stage<T, E, R, +1, true>(
-            fz, S - 1, tw, th, [&](int, int pos) { return x[ip_cswz(fz, pos) * W]; },
-            [&](int, int pos, cx<T> v) {
-                if (col) dst[(long)pos * ld + i] = v;
-            });
-    });
-}
-
-template <typename T, int E, typename TIN>
-__global__ void __launch_bounds__(256) k_zinv_ip(IpLen fz, const cx<T> *__restrict__ tw,
-                                                  const cx<TIN> *__restrict__ in, long in_fs, long in_ps, int mk,
-                                                  cx<T> *__restrict__ out, long out_fs, long out_ps, int ld, int j0,
-                                                  int nkz, int tnkz, int lw)
-{
-    zinv_body<T, E, TIN>(fz, tw, in, in_fs, in_ps, mk, out, out_fs, out_ps, ld, j0, nkz, tnkz, lw, blockIdx.x,
-                         blockIdx.y, blockIdx.z);
-}
-
-// Members of one transform-length group: grid.z = member * nfields + field,
-// planes beyond a member's own NY+2 exit at once.
-template <typename T, int E, typename TIN>
-__global__ void __launch_bounds__(256) k_zinv_ip_batch(const ZinvDesc<T, TIN> *__restrict__ d, IpLen fz,
-                                                        const cx<T> *__restrict__ tw, long in_ps, int mk,
-                                                        long out_ps, int ld, int nkz, int tnkz, int lw, int nfields)
-{
-    const int mem = blockIdx.z / nfields, f = blockIdx.z - mem * nfields;
-    const ZinvDesc<T, TIN> D = d[mem];
-    if ((int)blockIdx.y >= D.nplanes) return;
-    zinv_body<T, E, TIN>(fz, tw, D.in, D.in_fs, in_ps, mk, D.out, D.out_fs, out_ps, ld, 0, nkz, tnkz, lw,
-                         blockIdx.x, blockIdx.y, f);
-}
 
 // -------------------------------------------------------------------------
 // xfused: one z row per block, planes ja..jb (a chunk of the sweep).
@@ -135,7 +22,7 @@ __global__ void __launch_bounds__(256) k_zinv_ip_batch(const ZinvDesc<T, TIN> *_
 // the products read and write slot by slot, which is position by position in
 // the same permutation for every row.
 // -------------------------------------------------------------------------
-template <typename T, int E, int NTH>
+template <typename T, int E, int NTH, bool CP>
 __device__ __forceinline__ void xfused_body(const IpLen &fx, const cx<T> *__restrict__ twg, int mk, int ld,
                                             const cx<T> *__restrict__ zt, long zt_fs, long zt_ps,
                                             cx<T> *__restrict__ wt, long wt_fs, long wt_ps,
@@ -282,10 +169,10 @@ __device__ __forceinline__ void xfused_body(const IpLen &fx, const cx<T> *__rest
     __syncthreads();  // twiddles
     if (pro) dif0(slot_row(pro_plane, pro_pair));
     __syncthreads();
-    for (int s = 1; s < S; ++s) {
+    stage_loop<CP>(1, S, [&](int s) {
         if (pro) ip_stage_row<T, E, +1, true>(slot_row(pro_plane, pro_pair), fx, s, tw, th);
         __syncthreads();
-    }
+    });
     if (dif_row) {
         fetch(ja + 1, p);
         dif0(slot_row(ja + 1, p));
@@ -304,11 +191,11 @@ __device__ __forceinline__ void xfused_body(const IpLen &fx, const cx<T> *__rest
         const bool fwd = j > ja && dit_row;   // forward of the products of plane j-1
         cx<T> *drow = inv ? slot_row(j + 1, p) : win;
         cx<T> *frow = fb + (dit_row ? p - npi : 0) * nx;
-        for (int s = 0; s < S - 1; ++s) {
+        stage_loop<CP>(0, S - 1, [&](int s) {
             if (inv) ip_stage_row<T, E, +1, true>(drow, fx, s + 1, tw, th);
             if (fwd) ip_stage_row<T, E, -1, false>(frow, fx, S - 1 - s, tw, th);
             __syncthreads();
-        }
+        });
         if (fwd) dit_out(frow, p - npi, j - 1);
         __syncthreads();
         if (j <= jb) {
@@ -322,27 +209,27 @@ __device__ __forceinline__ void xfused_body(const IpLen &fx, const cx<T> *__rest
     }
 }
 
-template <typename T, int E, int NTH>
+template <typename T, int E, int NTH, int PID>
 __global__ void __launch_bounds__(512) k_xfused_ip(IpLen fx, const cx<T> *__restrict__ twg, int mk, int ld,
                                                     const cx<T> *__restrict__ zt, long zt_fs, long zt_ps,
                                                     cx<T> *__restrict__ wt, long wt_fs, long wt_ps,
                                                     const double *__restrict__ dy, const double *__restrict__ dyf,
                                                     const double *__restrict__ kxv, int jlo, int jhi, int chunk)
 {
-    xfused_body<T, E, NTH>(fx, twg, mk, ld, zt, zt_fs, zt_ps, wt, wt_fs, wt_ps, dy, dyf, kxv, jlo, jhi, chunk,
-                           blockIdx.x, blockIdx.y);
+    xfused_body<T, E, NTH, PID != 0>(plan_ref<PID>(fx), twg, mk, ld, zt, zt_fs, zt_ps, wt, wt_fs, wt_ps, dy, dyf,
+                                      kxv, jlo, jhi, chunk, blockIdx.x, blockIdx.y);
 }
 
 // grid (NZ, chunks of the tallest member, members)
-template <typename T, int E, int NTH>
+template <typename T, int E, int NTH, int PID>
 __global__ void __launch_bounds__(512) k_xfused_ip_batch(const XfDesc<T> *__restrict__ d, IpLen fx,
                                                           const cx<T> *__restrict__ twg, int mk, int ld, long ps,
                                                           int chunk)
 {
     const XfDesc<T> D = d[blockIdx.z];
     if ((int)blockIdx.y * chunk > D.jhi - D.jlo) return;
-    xfused_body<T, E, NTH>(fx, twg, mk, ld, D.zt, D.zt_fs, ps, D.wt, D.wt_fs, ps, D.dy, D.dyf, D.kx, D.jlo, D.jhi,
-                           chunk, blockIdx.x, blockIdx.y);
+    xfused_body<T, E, NTH, PID != 0>(plan_ref<PID>(fx), twg, mk, ld, D.zt, D.zt_fs, ps, D.wt, D.wt_fs, ps, D.dy,
+                                     D.dyf, D.kx, D.jlo, D.jhi, chunk, blockIdx.x, blockIdx.y);
 }
 
 // -------------------------------------------------------------------------
@@ -470,156 +357,10 @@ __global__ void __launch_bounds__(256) k_vmax_planes(const double *__restrict__ 
 }
 
 // -------------------------------------------------------------------------
-// zfwd_nl: W columns per block; both transforms of output o (G then H) in the
-// same LDS; G's result stays in registers, H's last stage combines and writes
-// the kept slots.  grid (column chunks, planes, outputs), block W*T.
-// -------------------------------------------------------------------------
-template <typename T, int E, typename TOUT>
-__device__ __forceinline__ void zfwd_body(const IpLen &fz, const cx<T> *__restrict__ tw,
-                                          const cx<T> *__restrict__ wt, long wt_fs, long wt_ps, int ld, int mk,
-                                          cx<TOUT> *const *__restrict__ nl, long ps, const double *__restrict__ kzv,
-                                          int jlo, int nkz, int tnkz, int lw, int bx, int by, int bz)
-{
-    extern __shared__ unsigned char smem_raw[];
-    cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
-    const int W = 1 << lw;
-    const int tid = threadIdx.x;
-    const int b = tid & (W - 1), th = tid >> lw;
-    const int i = bx * W + b;
-    const bool col = i < mk;
-    const int j = jlo + by;
-    const int o = bz;
-    const int nz = fz.n, S = fz.nstages;
-    const cx<T> *sg = wt + (2 * o) * wt_fs + (long)j * wt_ps + (col ? i : mk - 1);
-    const cx<T> *sh = sg + wt_fs;
-    cx<T> *x = lds + b;
-    const int rl = fz.radix[S - 1];
-    cx<T> gin[E], hin[E], gv[E];
-    // inputs of the first forward stage (stage S-1: groups of R consecutive positions)
-    auto load_first = [&](const cx<T> *src, cx<T> *regs) {
-        with_radix<E>(rl, [&](auto rc) {
-            constexpr int R = decltype(rc)::value;
-            ip_groups<E, R>(fz, S - 1, th, [&](int u, int base, int) {
-#pragma unroll
-                for (int q = 0; q < R; ++q) {
-                    const cx<T> v = src[(long)(base + q * fz.span[S - 1]) * ld];
-                    regs[u * R + q] = col ? v : make_cx<T>(0, 0);
-                }
-            });
-        });
-    };
-    auto first = [&](cx<T> *regs) {
-        with_radix<E>(rl, [&](auto rc) {
-            constexpr int R = decltype(rc)::value;
-            ip_stage<T, E, R, -1, false>(
-                fz, S - 1, tw, th, [&](int idx, int) { return regs[idx]; },
-                [&](int, int pos, cx<T> v) { x[ip_cswz(fz, pos) * W] = v; });
-        });
-    };
-    load_first(sg, gin);
-    load_first(sh, hin);
-    first(gin);
-    __syncthreads();
-    for (int s = 1; s < S - 1; ++s) {
-        ip_stage_mem<T, E, -1, false>(x, W, fz, S - 1 - s, tw, th);
-        __syncthreads();
-    }
-    with_radix<E>(fz.radix[0], [&](auto rc) {
-        constexpr int R = decltype(rc)::value;
-        ip_stage<T, E, R, -1, false>(
-            fz, 0, tw, th, [&](int, int pos) { return x[ip_cswz(fz, pos) * W]; }, [&](int idx, int, cx<T> v) { gv[idx] = v; });
-    });
-    __syncthreads();
-    first(hin);
-    __syncthreads();
-    for (int s = 1; s < S - 1; ++s) {
-        ip_stage_mem<T, E, -1, false>(x, W, fz, S - 1 - s, tw, th);
-        __syncthreads();
-    }
-    const T inv_nz = T(1) / T(nz);
-    cx<TOUT> *dst = nl[o] + (long)j * ps + i;
-    with_radix<E>(fz.radix[0], [&](auto rc) {
-        constexpr int R = decltype(rc)::value;
-        ip_stage<T, E, R, -1, false>(
-            fz, 0, tw, th, [&](int, int pos) { return x[ip_cswz(fz, pos) * W]; },
-            [&](int idx, int f, cx<T> hv) {
-                const int s = slot_of_freq(f, nz, nkz, tnkz);
-                if (col && s >= 0) {
-                    const cx<T> g = scale(gv[idx], inv_nz), hs = scale(hv, inv_nz);
-                    const T kz = T(kzv[s]);
-                    dst[(long)s * mk] = cvt_<TOUT>(make_cx<T>(g.re - kz * hs.im, g.im + kz * hs.re));
-                }
-            });
-    });
-}
-
-template <typename T, int E, typename TOUT>
-__global__ void __launch_bounds__(256) k_zfwd_nl_ip(IpLen fz, const cx<T> *__restrict__ tw,
-                                                     const cx<T> *__restrict__ wt, long wt_fs, long wt_ps, int ld,
-                                                     int mk, cx<TOUT> *const *__restrict__ nl, long ps,
-                                                     const double *__restrict__ kzv, int jlo, int nkz, int tnkz, int lw)
-{
-    zfwd_body<T, E, TOUT>(fz, tw, wt, wt_fs, wt_ps, ld, mk, nl, ps, kzv, jlo, nkz, tnkz, lw, blockIdx.x, blockIdx.y,
-                          blockIdx.z);
-}
-
-// grid (column chunks, planes of the tallest member, members * nout)
-template <typename T, int E, typename TOUT>
-__global__ void __launch_bounds__(256) k_zfwd_nl_ip_batch(const ZfDesc<T, TOUT> *__restrict__ d, IpLen fz,
-                                                           const cx<T> *__restrict__ tw, long wt_ps, int ld, int mk,
-                                                           long ps, int nkz, int tnkz, int lw, int nout)
-{
-    const int mem = blockIdx.z / nout, o = blockIdx.z - mem * nout;
-    const ZfDesc<T, TOUT> D = d[mem];
-    if ((int)blockIdx.y >= D.nplanes) return;
-    zfwd_body<T, E, TOUT>(fz, tw, D.wt, D.wt_fs, wt_ps, ld, mk, D.nl, ps, D.kz, D.jlo, nkz, tnkz, lw, blockIdx.x,
-                          blockIdx.y, o);
-}
-
-// -------------------------------------------------------------------------
 // launchers
 // -------------------------------------------------------------------------
 
-#define DGPU_DISPATCH_IPE(E_, CALL)                                \
-    switch (E_) {                                                  \
-    case 4: { constexpr int EE = 4; CALL; } break;                 \
-    case 6: { constexpr int EE = 6; CALL; } break;                 \
-    case 8: { constexpr int EE = 8; CALL; } break;                 \
-    case 12: { constexpr int EE = 12; CALL; } break;               \
-    case 16: { constexpr int EE = 16; CALL; } break;               \
-    case 24: { constexpr int EE = 24; CALL; } break;               \
-    default: break;                                                \
-    }
-
-static inline int z_lw(int t)
-{
-    int lw = 0;
-    while ((2 << lw) * t <= 256) ++lw;
-    return lw;
-}
-
-size_t zip_lds_bytes(const IpLen &fz, size_t elem) { return elem * ((size_t)1 << z_lw(fz.t)) * fz.n; }
-
-template <typename T, int E, typename TIN>
-static void zinv_ip_e(hipStream_t s, const Geo &g, const IpLen &fz, const cx<T> *tw, const cx<TIN> *in, long in_fs,
-                      long in_ps, cx<T> *out, long out_fs, long out_ps, int ld, int j0, int nplanes, int nfields)
-{
-    const int lw = z_lw(fz.t), W = 1 << lw;
-    dim3 grid((g.mk + W - 1) / W, nplanes, nfields);
-    const size_t lds = sizeof(cx<T>) * W * fz.n;
-    hipLaunchKernelGGL((k_zinv_ip<T, E, TIN>), grid, dim3(W * fz.t), lds, s, fz, tw, in, in_fs, in_ps, g.mk, out,
-                       out_fs, out_ps, ld, j0, g.nkz, g.tnkz, lw);
-}
-
-template <typename T, typename TIN>
-void launch_zinv_ip(hipStream_t s, const Geo &g, const IpLen &fz, const cx<T> *tw, const cx<TIN> *in, long in_fs,
-                    long in_ps, cx<T> *out, long out_fs, long out_ps, int ld, int j0, int nplanes, int nfields)
-{
-    DGPU_DISPATCH_IPE(fz.e, (zinv_ip_e<T, EE, TIN>(s, g, fz, tw, in, in_fs, in_ps, out, out_fs, out_ps, ld, j0,
-                                                  nplanes, nfields)));
-}
-
-template <typename T, int E, int NTH>
+template <typename T, int E, int NTH, int PID>
 static void xfused_ip_e(hipStream_t s, const Geo &g, const IpLen &fx, const cx<T> *tw, int ld, const cx<T> *zt,
                         long zt_fs, cx<T> *wt, long wt_fs, const double *dy, const double *dyf, const double *kx,
                         int jlo, int jhi, int chunk)
@@ -630,13 +371,13 @@ static void xfused_ip_e(hipStream_t s, const Geo &g, const IpLen &fx, const cx<T
     const size_t lds = sizeof(cx<T>) * ((size_t)fx.n * (3 * C::npi + C::npo) + fx.twn);
     static bool attr = false;
     if (!attr) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_xfused_ip<T, E, NTH>),
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_xfused_ip<T, E, NTH, PID>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         attr = true;
     }
     const long ps = (long)g.nz * ld;
-    hipLaunchKernelGGL((k_xfused_ip<T, E, NTH>), dim3(g.nz, nchunks), dim3(threads), lds, s, fx, tw, g.mk, ld, zt,
-                       zt_fs, ps, wt, wt_fs, ps, dy, dyf, kx, jlo, jhi, chunk);
+    hipLaunchKernelGGL((k_xfused_ip<T, E, NTH, PID>), dim3(g.nz, nchunks), dim3(threads), lds, s, fx, tw, g.mk, ld,
+                       zt, zt_fs, ps, wt, wt_fs, ps, dy, dyf, kx, jlo, jhi, chunk);
 }
 
 size_t xfused_ip_lds_bytes(const IpLen &fx, int nth, size_t elem)
@@ -647,17 +388,23 @@ size_t xfused_ip_lds_bytes(const IpLen &fx, int nth, size_t elem)
 
 int xfused_ip_threads(const IpLen &fx, int nth) { return fx.t * ((3 + nth + 1) / 2 + (8 + 3 * nth + 1) / 2); }
 
+// The single-run x kernel is built for the 384 plan only (the ensembles use
+// the batched kernel); one scalar.
 template <typename T>
 void launch_xfused_ip(hipStream_t s, const Geo &g, const IpLen &fx, const cx<T> *tw, int ld, const cx<T> *zt,
                       long zt_fs, cx<T> *wt, long wt_fs, const double *dy, const double *dyf, const double *kx,
                       int jlo, int jhi, int chunk)
 {
-#define DGPU_XF(EE_)                                                                                            \
-    do {                                                                                                        \
-        if (g.nth == 0)                                                                                         \
-            xfused_ip_e<T, EE_, 0>(s, g, fx, tw, ld, zt, zt_fs, wt, wt_fs, dy, dyf, kx, jlo, jhi, chunk);       \
-        else                                                                                                    \
-            xfused_ip_e<T, EE_, 1>(s, g, fx, tw, ld, zt, zt_fs, wt, wt_fs, dy, dyf, kx, jlo, jhi, chunk);       \
+    if (g.nth == 1 && const_plan_id(fx) == 1) {
+        xfused_ip_e<T, 12, 1, 1>(s, g, fx, tw, ld, zt, zt_fs, wt, wt_fs, dy, dyf, kx, jlo, jhi, chunk);
+        return;
+    }
+#define DGPU_XF(EE_)                                                                                             \
+    do {                                                                                                         \
+        if (g.nth == 0)                                                                                          \
+            xfused_ip_e<T, EE_, 0, 0>(s, g, fx, tw, ld, zt, zt_fs, wt, wt_fs, dy, dyf, kx, jlo, jhi, chunk);     \
+        else                                                                                                     \
+            xfused_ip_e<T, EE_, 1, 0>(s, g, fx, tw, ld, zt, zt_fs, wt, wt_fs, dy, dyf, kx, jlo, jhi, chunk);     \
     } while (0)
     switch (fx.e) {
     case 4: DGPU_XF(4); break;
@@ -672,45 +419,7 @@ void launch_xfused_ip(hipStream_t s, const Geo &g, const IpLen &fx, const cx<T> 
 #undef DGPU_XF
 }
 
-template <typename T, int E, typename TOUT>
-static void zfwd_nl_ip_e(hipStream_t s, const Geo &g, const IpLen &fz, const cx<T> *tw, const cx<T> *wt, long wt_fs,
-                         int ld, cx<TOUT> *const *nl, const double *kz, int jlo, int jhi, int nout)
-{
-    const int lw = z_lw(fz.t), W = 1 << lw;
-    dim3 grid((g.mk + W - 1) / W, jhi - jlo + 1, nout);
-    const size_t lds = sizeof(cx<T>) * W * fz.n;
-    const long wt_ps = (long)g.nz * ld;
-    hipLaunchKernelGGL((k_zfwd_nl_ip<T, E, TOUT>), grid, dim3(W * fz.t), lds, s, fz, tw, wt, wt_fs, wt_ps, ld, g.mk,
-                       nl, g.ps, kz, jlo, g.nkz, g.tnkz, lw);
-}
-
-template <typename T, typename TOUT>
-void launch_zfwd_nl_ip(hipStream_t s, const Geo &g, const IpLen &fz, const cx<T> *tw, const cx<T> *wt, long wt_fs,
-                       int ld, cx<TOUT> *const *nl, const double *kz, int jlo, int jhi, int nout)
-{
-    DGPU_DISPATCH_IPE(fz.e, (zfwd_nl_ip_e<T, EE, TOUT>(s, g, fz, tw, wt, wt_fs, ld, nl, kz, jlo, jhi, nout)));
-}
-
-// ---- batched (ensemble) launchers ----
-template <typename T, int E, typename TIN>
-static void zinv_ip_batch_e(hipStream_t s, const Geo &g, const IpLen &fz, const cx<T> *tw,
-                            const ZinvDesc<T, TIN> *d, int nmem, int max_planes, int ld, int nfields)
-{
-    const int lw = z_lw(fz.t), W = 1 << lw;
-    dim3 grid((g.mk + W - 1) / W, max_planes, nfields * nmem);
-    const size_t lds = sizeof(cx<T>) * W * fz.n;
-    hipLaunchKernelGGL((k_zinv_ip_batch<T, E, TIN>), grid, dim3(W * fz.t), lds, s, d, fz, tw, g.ps, g.mk,
-                       (long)g.nz * ld, ld, g.nkz, g.tnkz, lw, nfields);
-}
-
-template <typename T, typename TIN>
-void launch_zinv_ip_batch(hipStream_t s, const Geo &g, const IpLen &fz, const cx<T> *tw, const ZinvDesc<T, TIN> *d,
-                          int nmem, int max_planes, int ld, int nfields)
-{
-    DGPU_DISPATCH_IPE(fz.e, (zinv_ip_batch_e<T, EE, TIN>(s, g, fz, tw, d, nmem, max_planes, ld, nfields)));
-}
-
-template <typename T, int E, int NTH>
+template <typename T, int E, int NTH, int PID>
 static void xfused_ip_batch_e(hipStream_t s, const Geo &g, const IpLen &fx, const cx<T> *tw, const XfDesc<T> *d,
                               int nmem, int max_span, int ld, int chunk)
 {
@@ -719,12 +428,12 @@ static void xfused_ip_batch_e(hipStream_t s, const Geo &g, const IpLen &fx, cons
     const size_t lds = sizeof(cx<T>) * ((size_t)fx.n * (3 * C::npi + C::npo) + fx.twn);
     static bool attr = false;
     if (!attr) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_xfused_ip_batch<T, E, NTH>),
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_xfused_ip_batch<T, E, NTH, PID>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         attr = true;
     }
     dim3 grid(g.nz, (max_span + chunk - 1) / chunk, nmem);
-    hipLaunchKernelGGL((k_xfused_ip_batch<T, E, NTH>), grid, dim3(threads), lds, s, d, fx, tw, g.mk, ld,
+    hipLaunchKernelGGL((k_xfused_ip_batch<T, E, NTH, PID>), grid, dim3(threads), lds, s, d, fx, tw, g.mk, ld,
                        (long)g.nz * ld, chunk);
 }
 
@@ -732,12 +441,17 @@ template <typename T>
 void launch_xfused_ip_batch(hipStream_t s, const Geo &g, const IpLen &fx, const cx<T> *tw, const XfDesc<T> *d,
                             int nmem, int max_span, int ld, int chunk)
 {
+    if (g.nth == 1 && with_pid(const_plan_id(fx), XPids<T>{}, [&](auto pc) {
+            constexpr int P = decltype(pc)::value;
+            xfused_ip_batch_e<T, kConstPlanSpecs[P].e, 1, P>(s, g, fx, tw, d, nmem, max_span, ld, chunk);
+        }))
+        return;
 #define DGPU_XFB(EE_)                                                                               \
     do {                                                                                            \
         if (g.nth == 0)                                                                             \
-            xfused_ip_batch_e<T, EE_, 0>(s, g, fx, tw, d, nmem, max_span, ld, chunk);               \
+            xfused_ip_batch_e<T, EE_, 0, 0>(s, g, fx, tw, d, nmem, max_span, ld, chunk);            \
         else                                                                                        \
-            xfused_ip_batch_e<T, EE_, 1>(s, g, fx, tw, d, nmem, max_span, ld, chunk);               \
+            xfused_ip_batch_e<T, EE_, 1, 0>(s, g, fx, tw, d, nmem, max_span, ld, chunk);            \
     } while (0)
     switch (fx.e) {
     case 4: DGPU_XFB(4); break;
@@ -750,24 +464,6 @@ void launch_xfused_ip_batch(hipStream_t s, const Geo &g, const IpLen &fx, const 
     default: break;
     }
 #undef DGPU_XFB
-}
-
-template <typename T, int E, typename TOUT>
-static void zfwd_nl_ip_batch_e(hipStream_t s, const Geo &g, const IpLen &fz, const cx<T> *tw,
-                               const ZfDesc<T, TOUT> *d, int nmem, int max_planes, int ld, int nout)
-{
-    const int lw = z_lw(fz.t), W = 1 << lw;
-    dim3 grid((g.mk + W - 1) / W, max_planes, nout * nmem);
-    const size_t lds = sizeof(cx<T>) * W * fz.n;
-    hipLaunchKernelGGL((k_zfwd_nl_ip_batch<T, E, TOUT>), grid, dim3(W * fz.t), lds, s, d, fz, tw,
-                       (long)g.nz * ld, ld, g.mk, g.ps, g.nkz, g.tnkz, lw, nout);
-}
-
-template <typename T, typename TOUT>
-void launch_zfwd_nl_ip_batch(hipStream_t s, const Geo &g, const IpLen &fz, const cx<T> *tw, const ZfDesc<T, TOUT> *d,
-                             int nmem, int max_planes, int ld, int nout)
-{
-    DGPU_DISPATCH_IPE(fz.e, (zfwd_nl_ip_batch_e<T, EE, TOUT>(s, g, fz, tw, d, nmem, max_planes, ld, nout)));
 }
 
 int vmax_rows_per_block(const IpLen &fx) { return 256 / (2 * fx.t); }
@@ -808,41 +504,14 @@ template void launch_vmax_ip<double>(hipStream_t, const Geo &, const IpLen &, co
                                      const cx<double> *, long, double *, double *);
 template void launch_vmax_ip<float>(hipStream_t, const Geo &, const IpLen &, const cx<float> *, int,
                                     const cx<float> *, long, double *, double *);
-template void launch_zinv_ip_batch<double, double>(hipStream_t, const Geo &, const IpLen &, const cx<double> *,
-                                                   const ZinvDesc<double, double> *, int, int, int, int);
-template void launch_zinv_ip_batch<float, double>(hipStream_t, const Geo &, const IpLen &, const cx<float> *,
-                                                  const ZinvDesc<float, double> *, int, int, int, int);
-template void launch_zinv_ip_batch<float, float>(hipStream_t, const Geo &, const IpLen &, const cx<float> *,
-                                                 const ZinvDesc<float, float> *, int, int, int, int);
 template void launch_xfused_ip_batch<double>(hipStream_t, const Geo &, const IpLen &, const cx<double> *,
                                              const XfDesc<double> *, int, int, int, int);
 template void launch_xfused_ip_batch<float>(hipStream_t, const Geo &, const IpLen &, const cx<float> *,
                                             const XfDesc<float> *, int, int, int, int);
-template void launch_zfwd_nl_ip_batch<double, double>(hipStream_t, const Geo &, const IpLen &, const cx<double> *,
-                                                      const ZfDesc<double, double> *, int, int, int, int);
-template void launch_zfwd_nl_ip_batch<float, float>(hipStream_t, const Geo &, const IpLen &, const cx<float> *,
-                                                    const ZfDesc<float, float> *, int, int, int, int);
-
-template void launch_zinv_ip<double, double>(hipStream_t, const Geo &, const IpLen &, const cx<double> *,
-                                             const cx<double> *, long, long, cx<double> *, long, long, int, int, int,
-                                             int);
-template void launch_zinv_ip<float, float>(hipStream_t, const Geo &, const IpLen &, const cx<float> *,
-                                           const cx<float> *, long, long, cx<float> *, long, long, int, int, int,
-                                           int);
-template void launch_zinv_ip<float, double>(hipStream_t, const Geo &, const IpLen &, const cx<float> *,
-                                            const cx<double> *, long, long, cx<float> *, long, long, int, int, int,
-                                            int);
 template void launch_xfused_ip<double>(hipStream_t, const Geo &, const IpLen &, const cx<double> *, int,
                                        const cx<double> *, long, cx<double> *, long, const double *, const double *,
                                        const double *, int, int, int);
 template void launch_xfused_ip<float>(hipStream_t, const Geo &, const IpLen &, const cx<float> *, int,
                                       const cx<float> *, long, cx<float> *, long, const double *, const double *,
                                       const double *, int, int, int);
-template void launch_zfwd_nl_ip<double, double>(hipStream_t, const Geo &, const IpLen &, const cx<double> *,
-                                                const cx<double> *, long, int, cx<double> *const *, const double *,
-                                                int, int, int);
-template void launch_zfwd_nl_ip<float, float>(hipStream_t, const Geo &, const IpLen &, const cx<float> *,
-                                              const cx<float> *, long, int, cx<float> *const *, const double *, int,
-                                              int, int);
-
 }  // namespace dgpu

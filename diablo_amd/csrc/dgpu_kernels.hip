@@ -2,7 +2,7 @@
 // FFT_XZ_TO_FOURIER / FFT_XZ_TO_PHYSICAL (for/mpi.f:1530-1674) and of
 // POISSON_P_CHAN, plus APPLY_BC_VEL on kept-mode fields.  The physical-space
 // side of these transforms is in natural order (Fortran-layout planes).  The
-// substep's own transforms are in dgpu_hot.hip.
+// substep's own transforms are in dgpu_hot_z.hip and dgpu_hot_x.hip.
 #include <algorithm>
 
 #include "dgpu_kernels.hpp"

@@ -132,7 +132,7 @@ struct ZfDesc {
     int jlo, nplanes;
 };
 
-// ---- substep transforms on the in-place plans (dgpu_hot.hip) ----
+// ---- substep transforms on the in-place plans (dgpu_hot_z.hip, dgpu_hot_x.hip) ----
 // Zt/Wt rows are ld complex values long; tw points at the plan's twiddles.
 template <typename T, typename TIN>
 void launch_zinv_ip(hipStream_t s, const Geo &g, const IpLen &fz, const cx<T> *tw, const cx<TIN> *in, long in_fs,

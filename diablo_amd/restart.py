@@ -31,9 +31,12 @@ def save_flow(path: str, chan: GpuChannel, save_pressure: bool = True) -> None:
 
 
 def load_flow(path: str, chan: GpuChannel, create_new_th: tuple[bool, ...] = ()) -> None:
-    """ReadHDF5: the Resolution check, Time, the fields (a scalar n is read
-    unless create_new_th[n] is set, as CREATE_NEW_TH), and POISSON_P_CHAN when
-    the file holds no P (for/hdf5.f:380-544)."""
+    """READ_FLOW with ReadHDF5 (for/diablo_io.f:201-301, for/hdf5.f:380-544):
+    the Resolution check, Time, the fields (a scalar n is read unless
+    create_new_th[n] is set, as CREATE_NEW_TH), POISSON_P_CHAN when the file
+    holds no P, then the velocity BCs (APPLY_BC_VEL), which set the ghost
+    planes the file does not hold.  As in the reference, the scalars' ghost
+    planes stay zero until the first scalar solve."""
     cfg = chan.cfg
     with np.load(path, allow_pickle=False) as f:
         res = tuple(int(v) for v in f["Resolution"])
@@ -51,3 +54,4 @@ def load_flow(path: str, chan: GpuChannel, create_new_th: tuple[bool, ...] = ())
             chan.set_flow_physical(CP, f["P"])
     if not has_p:
         chan.poisson_p_chan()
+    chan.apply_bc_vel()
