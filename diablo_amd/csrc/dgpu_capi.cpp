@@ -858,6 +858,41 @@ int32_t dgpu_courant(dgpu_ctx *c, double cfl, double *delta_t, double *h_bar)
     return DGPU_OK;
 }
 
+int32_t dgpu_tke_chan(dgpu_ctx *c, double *epsilon)
+{
+    if (!c || !epsilon) return DGPU_E_ARG;
+    if (int rc = to_master(c)) return rc;
+    const dgpu::Geo &g = c->g;
+    const int NY = g.ny;
+    if (!c->d_stats) HIPCHK(c, hipMalloc(&c->d_stats, sizeof(double) * DGPU_NSTAT * NY));
+    static_assert(DGPU_NSTAT >= 10, "the statistics buffer holds the 10 epsilon sums");
+    dgpu::launch_tke(c->stream, g, c->cu[0], c->cu[1], c->cu[2], c->d_kx, c->d_kz, c->d_stats);
+    if (int rc = check_launch(c, "tke")) return rc;
+    std::vector<double> o(10 * (size_t)NY);
+    HIPCHK(c, hipMemcpyAsync(o.data(), c->d_stats, sizeof(double) * o.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    auto P = [&](int q, int j) { return (j >= 1 && j <= NY) ? o[(size_t)q * NY + (j - 1)] : 0.0; };
+    const std::vector<double> &DY = c->dy, &DYF = c->dyf, &GY = c->gy;
+    for (int j = 1; j <= NY; ++j) {
+        // x and z derivatives of U1, U3 on GY: the DYF-weighted average of
+        // planes j and j-1 (plane 0's term is left out: the reference reads
+        // an unset scratch plane there)
+        auto gyw = [&](int q) { return (DYF[j - 1] * P(q, j) + DYF[j] * (j > 1 ? P(q, j - 1) : 0.0)) / (2.0 * DY[j]); };
+        double dvdy;
+        if (j >= 2 && j <= NY - 1) {
+            const double d = GY[j + 1] - GY[j - 1];
+            const double mean = (P(9, j + 1) - P(9, j)) / d;  // vme(j), vme(j-1) beside U2(j+1), U2(j-1)
+            dvdy = P(8, j) / (d * d) + mean * mean;
+        } else {
+            dvdy = P(3, j);  // the reference's scratch still holds du/dz there
+        }
+        const double e = gyw(0) + P(1, j) + P(6, j) / (DY[j] * DY[j]) + gyw(2) + gyw(3) + dvdy +
+                         P(7, j) / (DY[j] * DY[j]) + P(4, j) + gyw(5);
+        epsilon[j - 1] = c->p.nu * e;
+    }
+    return DGPU_OK;
+}
+
 int32_t dgpu_synchronize(dgpu_ctx *c)
 {
     if (!c) return DGPU_E_ARG;

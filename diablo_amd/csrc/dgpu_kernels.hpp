@@ -193,6 +193,9 @@ void launch_pp_solve(hipStream_t s, const Geo &g, const cx<double> *cs, const cx
                      const double *gyf, const double *pmult, const double *pinv, const double *rd, cx<double> *scr,
                      cx<double> *cp);
 // SAVE_STATS_CHAN statistics (dgpu_stats.hip): out[q * NY + (j - 1)]
+// tkebudget_chan's epsilon as per-plane Parseval sums (10 * NY doubles)
+void launch_tke(hipStream_t s, const Geo &g, const cx<double> *cu1, const cx<double> *cu2, const cx<double> *cu3,
+                const double *kx, const double *kz, double *out);
 void launch_stats(hipStream_t s, const Geo &g, const cx<double> *cu1, const cx<double> *cu2, const cx<double> *cu3,
                   const cx<double> *cp, const cx<double> *cth, const double *dy, const double *dyf,
                   const double *gyf, const double *kx, const double *kz, double *out);

@@ -123,6 +123,74 @@ __global__ void __launch_bounds__(256) k_stats(Geo g, const cx<double> *__restri
     }
 }
 
+// tkebudget_chan's epsilon (for/save_stats.f:965-1159), per plane j = 1..NY,
+// as Parseval sums the host combines: out[q * NY + j - 1] for
+//   q = 0..5  sum w kx^2 |CU1|^2, kx^2 |CU2|^2, kx^2 |CU3|^2, kz^2 |CU1|^2,
+//             kz^2 |CU2|^2, kz^2 |CU3|^2 of plane j (the x and z derivative
+//             blocks; the host forms the DYF-weighted averages onto GY)
+//   q = 6, 7  sum over fluctuating modes of w |CU1(j) - CU1(j-1)|^2 and of
+//             CU3 (du/dy, dw/dy before the 1/DY(j)^2)
+//   q = 8     sum over fluctuating modes of w |CU2(j+1) - CU2(j-1)|^2,
+//             j = 2..NY-1 (dv/dy), 0 elsewhere
+//   q = 9     Re CU2(0,0,j) (vme)
+__global__ void __launch_bounds__(256) k_tke(Geo g, const cx<double> *__restrict__ cu1,
+                                             const cx<double> *__restrict__ cu2, const cx<double> *__restrict__ cu3,
+                                             const double *__restrict__ kxv, const double *__restrict__ kzv,
+                                             double *__restrict__ out)
+{
+    constexpr int kT = 9;
+    __shared__ double red[kT][256];
+    const int j = 1 + blockIdx.x;
+    const int NY = g.ny;
+    const long ps = g.ps;
+    double acc[kT];
+#pragma unroll
+    for (int q = 0; q < kT; ++q) acc[q] = 0.0;
+    const bool dv = (j >= 2 && j <= NY - 1);
+    for (long m = threadIdx.x; m < ps; m += blockDim.x) {
+        const int K = (int)(m / g.mk), i = (int)(m - (long)K * g.mk);
+        const bool mean = (m == 0);
+        const double w = (i == 0) ? 1.0 : 2.0;
+        const double kx2 = kxv[i] * kxv[i], kz2 = kzv[K] * kzv[K];
+        const cx<double> a = cu1[m + (long)j * ps], am = cu1[m + (long)(j - 1) * ps];
+        const cx<double> v = cu2[m + (long)j * ps];
+        const cx<double> c = cu3[m + (long)j * ps], cm = cu3[m + (long)(j - 1) * ps];
+        const double a2 = abs2(a), v2 = abs2(v), c2 = abs2(c);
+        acc[0] += w * kx2 * a2;
+        acc[1] += w * kx2 * v2;
+        acc[2] += w * kx2 * c2;
+        acc[3] += w * kz2 * a2;
+        acc[4] += w * kz2 * v2;
+        acc[5] += w * kz2 * c2;
+        if (!mean) {
+            acc[6] += w * abs2(a - am);
+            acc[7] += w * abs2(c - cm);
+            if (dv) acc[8] += w * abs2(cu2[m + (long)(j + 1) * ps] - cu2[m + (long)(j - 1) * ps]);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < kT; ++q) red[q][threadIdx.x] = acc[q];
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) {
+#pragma unroll
+            for (int q = 0; q < kT; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < kT; ++q) out[(long)q * NY + (j - 1)] = red[q][0];
+        out[(long)kT * NY + (j - 1)] = cu2[(long)j * ps].re;
+    }
+}
+
+void launch_tke(hipStream_t s, const Geo &g, const cx<double> *cu1, const cx<double> *cu2, const cx<double> *cu3,
+                const double *kx, const double *kz, double *out)
+{
+    hipLaunchKernelGGL(k_tke, dim3(g.ny), dim3(256), 0, s, g, cu1, cu2, cu3, kx, kz, out);
+}
+
 void launch_stats(hipStream_t s, const Geo &g, const cx<double> *cu1, const cx<double> *cu2, const cx<double> *cu3,
                   const cx<double> *cp, const cx<double> *cth, const double *dy, const double *dyf,
                   const double *gyf, const double *kx, const double *kz, double *out)

@@ -124,6 +124,12 @@
           REAL(C_DOUBLE), INTENT(OUT) :: P(*)
           REAL(C_DOUBLE), INTENT(OUT) :: U
         END FUNCTION
+        INTEGER(C_INT32_T) FUNCTION dgpu_tke_chan(CTX, E)
+     &      BIND(C, NAME='dgpu_tke_chan')
+          IMPORT :: C_PTR, C_INT32_T, C_DOUBLE
+          TYPE(C_PTR), VALUE :: CTX
+          REAL(C_DOUBLE), INTENT(OUT) :: E(*)
+        END FUNCTION
         INTEGER(C_INT32_T) FUNCTION dgpu_courant_request(CTX)
      &      BIND(C, NAME='dgpu_courant_request')
           IMPORT :: C_PTR, C_INT32_T
@@ -426,5 +432,21 @@
       IF (.NOT. DGPU_ON_DEVICE) CALL DGPU_SYNC_TO_DEVICE
       CALL DGPU_CHECK(dgpu_stats_chan(DGPU_CTX, PROF, UB),
      &     'dgpu_stats_chan')
+      RETURN
+      END
+
+!----*|--.---------.---------.---------.---------.---------.---------.-|-------|
+      SUBROUTINE DGPU_TKE_EPSILON(TKEEPS)
+!----*|--.---------.---------.---------.---------.---------.---------.-|-------|
+! tkebudget_chan's epsilon (for/save_stats.f:965-1159) on the GPU in FP64:
+! TKEEPS(J), J = 1..NY, as tke.h5 / tke.txt hold it.  The host state is
+! uploaded first unless the GPU copy is current.
+      USE DGPU_BRIDGE
+      INCLUDE 'header'
+      REAL*8 TKEEPS(NY)
+      CALL DGPU_ENSURE_CONTEXT
+      IF (.NOT. DGPU_ON_DEVICE) CALL DGPU_SYNC_TO_DEVICE
+      CALL DGPU_CHECK(dgpu_tke_chan(DGPU_CTX, TKEEPS),
+     &     'dgpu_tke_chan')
       RETURN
       END

@@ -57,7 +57,7 @@ EXPORTS = (
     "dgpu_ensemble_create", "dgpu_ensemble_destroy", "dgpu_ensemble_size", "dgpu_ensemble_member",
     "dgpu_ensemble_last_error", "dgpu_ensemble_set_graphs", "dgpu_ensemble_rk_chan_1",
     "dgpu_ensemble_synchronize", "dgpu_stats_chan", "dgpu_courant_request", "dgpu_courant",
-    "dgpu_get_flow_physical", "dgpu_set_flow_physical",
+    "dgpu_get_flow_physical", "dgpu_set_flow_physical", "dgpu_tke_chan",
 )
 
 # dgpu_stats_chan profiles, in the order of the header's DGPU_ST_*
@@ -119,6 +119,8 @@ def load_library(path: str = LIB_PATH):
     L.dgpu_kernel_times.argtypes = [vp, DP, C.POINTER(C.c_int64), i32]
     L.dgpu_stats_chan.restype = i32
     L.dgpu_stats_chan.argtypes = [vp, DP, DP]
+    L.dgpu_tke_chan.restype = i32
+    L.dgpu_tke_chan.argtypes = [vp, DP]
     L.dgpu_get_flow_physical.restype = i32
     L.dgpu_get_flow_physical.argtypes = [vp, i32, DP]
     L.dgpu_set_flow_physical.restype = i32
@@ -288,6 +290,12 @@ class GpuChannel:
 
     def apply_bc_vel(self):
         self._chk(self.L.dgpu_apply_bc_vel(self.h), "dgpu_apply_bc_vel")
+
+    def tke_chan(self) -> np.ndarray:
+        """tkebudget_chan's epsilon at J = 1..NY from the device state."""
+        out = np.zeros(self.cfg.ny)
+        self._chk(self.L.dgpu_tke_chan(self.h, _dp(out)), "dgpu_tke_chan")
+        return out
 
     def get_flow_physical(self, fid: int) -> np.ndarray:
         """WriteHDF5's array for field fid (CU1..CU3, CP, CTH0+n): (NZ, NY, NX)

@@ -210,6 +210,14 @@ int32_t dgpu_synchronize(dgpu_ctx *ctx);
  * name, for/save_stats.f:404-406). */
 int32_t dgpu_stats_chan(dgpu_ctx *ctx, double *profiles, double *ubulk);
 
+/* tkebudget_chan's pseudo-dissipation epsilon = NU <du_i/dx_j du_i/dx_j>
+ * (for/save_stats.f:965-1159), as SAVE_STATS_CHAN calls it, from the device
+ * state in FP64: epsilon[J - 1], J = 1..NY (tke.h5's 'epsilon').  Planes
+ * 2..NY follow the reference, including its dv/dy mean removal and the
+ * du/dz values its scratch still holds on plane NY; on plane 1 the reference
+ * also adds terms from a scratch plane it never sets, which are left out. */
+int32_t dgpu_tke_chan(dgpu_ctx *ctx, double *epsilon);
+
 /* Kernel timing: when enabled, every kernel launch is bracketed by HIP events
  * on the context's stream.  dgpu_kernel_times returns, per kernel kind
  * (0 zinv, 1 xfused, 2 zfwd_nl, 3 cn, 4 remdiv), the summed milliseconds and the

@@ -1536,3 +1536,83 @@ void dor_flow_from_file(dor_state *s, int id, const double *in)
     memcpy(s->fld[id], w, sizeof(double) * s->fsize);
     free(w);
 }
+
+/* ------------------------------------------------------------------------ */
+/* tkebudget_chan's dissipation rate (for/save_stats.f:965-1159), as         */
+/* SAVE_STATS_CHAN calls it (for/save_stats.f:134-169): CR1-3 hold copies    */
+/* of CU1-3, U1-3 their physical fields, ume/vme/wme the (0,0) modes.  The  */
+/* scratch CS1/S1 is the state's own S1 array with whatever it holds (the   */
+/* reference's plane 0 of it is never set here, so epsilon(1) depends on     */
+/* it), and each gradient block overwrites planes 1..NY only (dv/dy: 2..NY-1, */
+/* the others keep du/dz's values).  eps[j-1], j = 1..NY.                    */
+/* ------------------------------------------------------------------------ */
+void dor_tke_chan(dor_state *s, double *eps)
+{
+    const int NX = s->NX, NY = s->NY, NZ = s->NZ, NXP = s->NXP, TNKZ = s->TNKZ, NXM = s->NXM, NZP = s->NZP;
+    const double *DY = s->DY, *DYF = s->DYF, *GY = s->GY, *KX = s->KX, *KZ = s->KZ;
+    const size_t fs = s->fsize;
+    const cplx *CR1 = (const cplx *)s->fld[DOR_U1], *CR2 = (const cplx *)s->fld[DOR_U2];
+    const cplx *CR3 = (const cplx *)s->fld[DOR_U3];
+    double *U1 = zalloc(fs), *U2 = zalloc(fs), *U3 = zalloc(fs);
+    memcpy(U1, s->fld[DOR_U1], sizeof(double) * fs);
+    memcpy(U2, s->fld[DOR_U2], sizeof(double) * fs);
+    memcpy(U3, s->fld[DOR_U3], sizeof(double) * fs);
+    double *ume = zalloc(NY + 2), *vme = zalloc(NY + 2), *wme = zalloc(NY + 2), *e = zalloc(NY + 2);
+    for (int j = 0; j <= NY + 1; j++) {
+        ume[j] = creal(CR1[CIDX(0, 0, j)]);
+        vme[j] = creal(CR2[CIDX(0, 0, j)]);
+        wme[j] = creal(CR3[CIDX(0, 0, j)]);
+    }
+    dor_fft_xz_to_physical(s, U1, 0, NY + 1);
+    dor_fft_xz_to_physical(s, U2, 0, NY + 1);
+    dor_fft_xz_to_physical(s, U3, 0, NY + 1);
+    double *S1 = s->fld[DOR_S1];
+    cplx *CS1 = (cplx *)S1;
+    /* CS1 = (i kx or i kz) * CR for j = 1..NY, then to physical space */
+    #define SPEC(CR, KV, KIDX)                                                        \
+        do {                                                                            \
+            for (int j = 1; j <= NY; j++)                                               \
+                for (int k = 0; k <= TNKZ; k++)                                         \
+                    for (int i = 0; i <= NXP - 1; i++)                                  \
+                        CS1[CIDX(i, k, j)] = (I * KV[KIDX]) * CR[CIDX(i, k, j)];        \
+            dor_fft_xz_to_physical(s, S1, 0, NY + 1);                                   \
+        } while (0)
+    /* interpolated to GY: (DYF(j-1) S1(j)^2 + DYF(j) S1(j-1)^2) / (2 DY(j)) */
+    #define ACC_GY()                                                                    \
+        for (int j = 1; j <= NY; j++)                                                   \
+            for (int k = 0; k <= NZP - 1; k++)                                          \
+                for (int i = 0; i <= NXM; i++)                                          \
+                    e[j] = e[j] + (DYF[j - 1] * pow(S1[RIDX(i, k, j)], 2.0) +          \
+                                   DYF[j] * pow(S1[RIDX(i, k, j - 1)], 2.0)) / (2.0 * DY[j])
+    #define ACC_SQ()                                                                    \
+        for (int j = 1; j <= NY; j++)                                                   \
+            for (int k = 0; k <= NZP - 1; k++)                                          \
+                for (int i = 0; i <= NXM; i++) e[j] = e[j] + pow(S1[RIDX(i, k, j)], 2.0)
+    SPEC(CR1, KX, i); ACC_GY();                                   /* du/dx */
+    SPEC(CR2, KX, i); ACC_SQ();                                   /* dv/dx */
+    for (int j = 1; j <= NY; j++)                                 /* du/dy */
+        for (int k = 0; k <= NZP - 1; k++)
+            for (int i = 0; i <= NXM; i++)
+                S1[RIDX(i, k, j)] = ((U1[RIDX(i, k, j)] - ume[j]) - (U1[RIDX(i, k, j - 1)] - ume[j - 1])) / DY[j];
+    ACC_SQ();
+    SPEC(CR3, KX, i); ACC_GY();                                   /* dw/dx */
+    SPEC(CR1, KZ, k); ACC_GY();                                   /* du/dz */
+    for (int j = 2; j <= NY - 1; j++)                             /* dv/dy */
+        for (int k = 0; k <= NZP - 1; k++)
+            for (int i = 0; i <= NXM; i++)
+                S1[RIDX(i, k, j)] = ((U2[RIDX(i, k, j + 1)] - vme[j]) - (U2[RIDX(i, k, j - 1)] - vme[j - 1]))
+                                    / (GY[j + 1] - GY[j - 1]);
+    ACC_SQ();
+    for (int j = 1; j <= NY; j++)                                 /* dw/dy */
+        for (int k = 0; k <= NZP - 1; k++)
+            for (int i = 0; i <= NXM; i++)
+                S1[RIDX(i, k, j)] = ((U3[RIDX(i, k, j)] - wme[j]) - (U3[RIDX(i, k, j - 1)] - wme[j - 1])) / DY[j];
+    ACC_SQ();
+    SPEC(CR2, KZ, k); ACC_SQ();                                   /* dv/dz */
+    SPEC(CR3, KZ, k); ACC_GY();                                   /* dw/dz */
+    #undef SPEC
+    #undef ACC_GY
+    #undef ACC_SQ
+    for (int j = 1; j <= NY; j++) eps[j - 1] = s->p.nu * e[j] / (double)(NX * NZ);
+    free(U1); free(U2); free(U3); free(ume); free(vme); free(wme); free(e);
+}

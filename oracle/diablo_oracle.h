@@ -101,6 +101,11 @@ enum {
     DOR_ST_THME, DOR_ST_THRMS, DOR_ST_THV, DOR_ST_DTHDY, DOR_ST_PE_DISS, DOR_NSTAT
 };
 void dor_stats_chan(dor_state *s, double *out, double *ubulk);
+/* tkebudget_chan's epsilon (for/save_stats.f:965-1159) as SAVE_STATS_CHAN
+ * calls it: eps[j - 1], j = 1..NY.  Uses the state's S1 array as the
+ * reference uses CS1/S1 (its plane 0 is read unset, so eps[0] depends on
+ * what S1 held); the velocity state is not modified. */
+void dor_tke_chan(dor_state *s, double *eps);
 
 /* WriteHDF5 / ReadHDF5's field layout (for/hdf5.f:172-268, 470-534): id is
  * DOR_U1, DOR_U2, DOR_U3, DOR_P or DOR_CTH0 + n; the file array is

@@ -88,6 +88,7 @@ def lib():
         L.dor_r2c.argtypes = [dp, dp, i]
         L.dor_c2r.argtypes = [dp, dp, i]
         L.dor_stats_chan.argtypes = [vp, dp, dp]
+        L.dor_tke_chan.argtypes = [vp, dp]
         _lib = L
     return _lib
 
@@ -214,6 +215,13 @@ class Oracle:
         ub = np.zeros(1)
         self.L.dor_stats_chan(self.h, _dp(out), _dp(ub))
         return {k: out[q * ny:(q + 1) * ny].copy() for q, k in enumerate(STATS)}, float(ub[0])
+
+    def tke_chan(self) -> np.ndarray:
+        """tkebudget_chan's epsilon at J = 1..NY (entry 0 depends on the S1
+        scratch's previous contents, as in the reference)."""
+        out = np.zeros(self.cfg.ny)
+        self.L.dor_tke_chan(self.h, _dp(out))
+        return out
 
     # -- restart-file layout (WriteHDF5 / ReadHDF5) --
     def flow_to_file(self, fid: int) -> np.ndarray:

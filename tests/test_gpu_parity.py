@@ -274,6 +274,29 @@ def test_stats_chan(oracle_mod, gpu_mod, maker):
         assert err < 1e-5, (k, err)
 
 
+@pytest.mark.parametrize("maker", ["channel", "kh"])
+def test_tke_chan(oracle_mod, gpu_mod, maker):
+    """tkebudget_chan's epsilon: device Parseval sums against the oracle's
+    physical-space sums on planes 2..NY (plane 1 of the reference reads a
+    scratch plane it never sets); mixed path within 1e-5."""
+    cfg = {"channel": lambda: case.channel_case(32, 17, 24, kick=0.3, streak=0.1),
+           "kh": lambda: case.kh_case(32, 33, 24, kick=0.05)}[maker]()
+    say("tke_chan", maker)
+    o, g = make_pair(oracle_mod, gpu_mod, cfg, nsteps_oracle=2)
+    eo = o.tke_chan()
+    eg = g.tke_chan()
+    err = np.abs(eg[1:] - eo[1:]).max() / np.abs(eo[1:]).max()
+    say("epsilon", err)
+    assert err < 1e-11, err
+    assert np.all(np.isfinite(eg)) and eg[0] >= 0.0
+    gm = gpu_mod.GpuChannel(cfg, precision="mixed")
+    gm.load_state(o.state())
+    gm.step(1)
+    g.step(1)
+    em, e64 = gm.tke_chan(), g.tke_chan()
+    assert np.abs(em - e64).max() / np.abs(e64).max() < 1e-5
+
+
 @pytest.mark.parametrize("maker", ["channel", "kh", "sink"])
 def test_courant_variable_dt(oracle_mod, gpu_mod, maker):
     """VARIABLE_DT, UPDATE_DT = 1: each step's new DELTA_T from the device's

@@ -130,3 +130,77 @@ def test_physical_sums_match_parseval(oracle_mod, dims):
         scale = max(np.abs(v).max(), 1e-300)
         err = np.abs(st[k] - v).max() / scale
         assert err < 1e-12, (k, err)
+
+
+def test_tke_epsilon_closed_forms(oracle_mod):
+    """tkebudget_chan's epsilon for a state whose gradients have closed forms:
+    u' = 2 Re(a(y) e^{i kx x}) with a linear in the plane index (du/dx on GY
+    by the DYF-weighted average, du/dy from the plane difference), w' =
+    2 Re(b e^{i kz z}) (dw/dz), and a mean V profile, whose dv/dy term comes
+    from the reference's mean removal vme(j), vme(j-1) next to U2(j+1),
+    U2(j-1): (vme(j+1) - vme(j)) / (GY(j+1) - GY(j-1)).  Planes 2..NY-1."""
+    cfg = _cfg()
+    o = _fresh(oracle_mod, cfg)
+    ny = cfg.ny
+    gy, dy, dyf = o.vec(oracle_mod.GY), o.vec(oracle_mod.DY), o.vec(oracle_mod.DYF)
+    kx, kz = o.vec(oracle_mod.KX), o.vec(oracle_mod.KZ)
+    c1 = case.fourier_view(o.field(oracle_mod.U1))
+    c2 = case.fourier_view(o.field(oracle_mod.U2))
+    c3 = case.fourier_view(o.field(oracle_mod.U3))
+    j = np.arange(ny + 2)
+    alpha = 0.02 - 0.01j
+    a = (0.3 + 0.2j) + alpha * j
+    c1[:, 0, 1] = a
+    b = -0.1 + 0.25j
+    c3[:, 1, 0] = b
+    c3[:, cfg.tnkz, 0] = np.conj(b)
+    vme = 0.05 * np.sin(0.7 * j)
+    c2[:, 0, 0] = vme
+    eps = o.tke_chan()
+    for jj in range(2, ny):
+        interp = (dyf[jj - 1] + dyf[jj]) / (2 * dy[jj])
+        want = ((dyf[jj - 1] * 2 * kx[1] ** 2 * abs(a[jj]) ** 2 + dyf[jj] * 2 * kx[1] ** 2 * abs(a[jj - 1]) ** 2)
+                / (2 * dy[jj])                                        # du/dx
+                + 2 * abs(alpha) ** 2 / dy[jj] ** 2                   # du/dy
+                + ((vme[jj + 1] - vme[jj]) / (gy[jj + 1] - gy[jj - 1])) ** 2  # dv/dy (mean part)
+                + interp * 2 * kz[1] ** 2 * abs(b) ** 2)              # dw/dz
+        assert eps[jj - 1] == pytest.approx(cfg.nu * want, rel=1e-12), jj
+
+
+@pytest.mark.parametrize("dims", [(24, 9, 18), (32, 17, 24)])
+def test_tke_epsilon_matches_parseval(oracle_mod, dims):
+    """Random stepped state: the oracle's physical-space sums equal a numpy
+    evaluation over the kept modes on planes 2..NY-1; on plane NY the dv/dy
+    block leaves du/dz's values in the scratch, and the reference adds those."""
+    cfg = _cfg(*dims)
+    o = oracle_mod.Oracle(cfg)
+    o.initialize(*case.initial_physical(cfg.replace(kick=0.3, streak=0.1)))
+    o.step()
+    eps = o.tke_chan()
+    ny = cfg.ny
+    kx = o.vec(oracle_mod.KX)[:cfg.nkx + 1][None, :]
+    kz = o.vec(oracle_mod.KZ)[:cfg.tnkz + 1][:, None]
+    gy, dy, dyf = o.vec(oracle_mod.GY), o.vec(oracle_mod.DY), o.vec(oracle_mod.DYF)
+    K = lambda f: case.kept(cfg, o.field(f)).copy()
+    u1, u2, u3 = K(oracle_mod.U1), K(oracle_mod.U2), K(oracle_mod.U3)
+    w = np.where(np.arange(cfg.nkx + 1) == 0, 1.0, 2.0)[None, :]
+    fl = np.ones((cfg.tnkz + 1, cfg.nkx + 1))
+    fl[0, 0] = 0.0
+    S = lambda x: float(np.sum(w * x))
+    a2 = lambda z: np.abs(z) ** 2
+    vme = u2[:, 0, 0].real
+    def plane(j):
+        x = lambda f, k: S(k ** 2 * a2(f[j]))
+        return x(u1, kx), x(u2, kx), x(u3, kx), x(u1, kz), x(u2, kz), x(u3, kz)
+    for jj in range(2, ny + 1):
+        A, Am = plane(jj), plane(jj - 1)
+        gyw = lambda q: (dyf[jj - 1] * A[q] + dyf[jj] * Am[q]) / (2 * dy[jj])
+        dudy = S(fl * a2((u1[jj] - u1[jj - 1]) / dy[jj]))
+        dwdy = S(fl * a2((u3[jj] - u3[jj - 1]) / dy[jj]))
+        if jj < ny:
+            dvdy = S(fl * a2((u2[jj + 1] - u2[jj - 1]) / (gy[jj + 1] - gy[jj - 1]))) \
+                + ((vme[jj + 1] - vme[jj]) / (gy[jj + 1] - gy[jj - 1])) ** 2
+        else:
+            dvdy = A[3]                       # du/dz of plane NY, left in S1
+        want = gyw(0) + A[1] + dudy + gyw(2) + gyw(3) + dvdy + dwdy + A[4] + gyw(5)
+        assert eps[jj - 1] == pytest.approx(cfg.nu * want, rel=1e-12), jj
