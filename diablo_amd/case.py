@@ -25,6 +25,7 @@ from __future__ import annotations
 
 import dataclasses
 import math
+import re
 import os
 from dataclasses import dataclass, field
 from typing import List, Optional
@@ -53,6 +54,12 @@ class ChannelConfig:
     variable_dt: bool = False
     cfl: float = 0.5
     update_dt: int = 1
+    # driver settings of input.dat (for/diablo_io.f:31-36)
+    create_new_flow: bool = True
+    time_limit: float = 1.0e6
+    reset_time: bool = False
+    save_flow_int: int = 100000
+    save_stats_int: int = 100000
     i_ro: float = 0.0
     grav_x: float = 0.0
     grav_y: float = 0.0
@@ -402,9 +409,12 @@ def _lg(b) -> str:
     return ".TRUE." if b else ".FALSE."
 
 
-def write_decks(cfg: ChannelConfig, directory: str, save_flow_int: int = 100000,
-                save_stats_int: int = 100000) -> None:
-    """Writes input.dat and input_chan.dat in the reference's fixed-line layout."""
+def write_decks(cfg: ChannelConfig, directory: str, save_flow_int: Optional[int] = None,
+                save_stats_int: Optional[int] = None) -> None:
+    """Writes input.dat and input_chan.dat in the reference's fixed-line layout
+    (and grid_def with the sizes)."""
+    save_flow_int = cfg.save_flow_int if save_flow_int is None else save_flow_int
+    save_stats_int = cfg.save_stats_int if save_stats_int is None else save_stats_int
     lines = [
         "C A data file for diablo.  This is a generic, easy method of data",
         "C input, as long as you don't change the number of comment lines.",
@@ -418,9 +428,9 @@ def write_decks(cfg: ChannelConfig, directory: str, save_flow_int: int = 100000,
         "C Vertical viscosity scale factor NU_V_SCALE (NU_V=NU*NU_V_SCALE)",
         f"   {_fl(cfg.nu_v_scale)}",
         "C Method:     NUM_PER_DIR, CREATE_NEW_FLOW",
-        "   2   .TRUE.",
+        f"   2   {_lg(cfg.create_new_flow)}",
         "C Time March:  N_TIME_STEPS, TIME_LIMIT, DELTA_T, RESET_TIME, VARIABLE_DT, CFL, UPDATE_DT",
-        f"   {cfg.n_steps}  1000000.0  {_fl(cfg.delta_t)}  .FALSE.  {'.TRUE.' if cfg.variable_dt else '.FALSE.'}"
+        f"   {cfg.n_steps}  {_fl(cfg.time_limit)}  {_fl(cfg.delta_t)}  {_lg(cfg.reset_time)}  {_lg(cfg.variable_dt)}"
         f"  {_fl(cfg.cfl)}  {cfg.update_dt}",
         "C I/O:         VERBOSITY, SAVE_FLOW_INT, SAVE_STATS_INT,  MOVIE",
         f"   1  {save_flow_int}  {save_stats_int}  .FALSE.",
@@ -475,6 +485,25 @@ def write_decks(cfg: ChannelConfig, directory: str, save_flow_int: int = 100000,
         ]
     with open(os.path.join(directory, "input_chan.dat"), "w") as f:
         f.write("\n".join(lines) + "\n")
+    with open(os.path.join(directory, "grid_def"), "w") as f:
+        f.write("!----*|--.---------.---------.---------.---------.---------.---------.-|-------|\n"
+                "! grid_def, the portion of the header that defines the grid size for diablo.\n"
+                "!----*|--.---------.---------.---------.---------.---------.---------.-|-------|\n"
+                f"      PARAMETER (NX={cfg.nx})\n      PARAMETER (NY={cfg.ny})\n"
+                f"      PARAMETER (NZ={cfg.nz})\n      PARAMETER (N_TH={cfg.n_th})\n")
+
+
+def read_grid_def(directory: str):
+    """NX, NY, NZ, N_TH from grid_def's PARAMETER statements."""
+    vals = {}
+    with open(os.path.join(directory, "grid_def")) as f:
+        for line in f:
+            if line.lstrip().startswith("!"):
+                continue
+            m = re.search(r"PARAMETER\s*\(\s*(\w+)\s*=\s*(\d+)\s*\)", line, re.IGNORECASE)
+            if m:
+                vals[m.group(1).upper()] = int(m.group(2))
+    return vals["NX"], vals["NY"], vals["NZ"], vals.get("N_TH", 0)
 
 
 class _ListDirected:
@@ -523,13 +552,13 @@ def read_decks(directory: str, nx: int, ny: int, nz: int, n_th: int) -> ChannelC
     r.skip()
     (nu_v_scale,) = map(float, r.read(1))
     r.skip()
-    num_per_dir, _create = r.read(2)
+    num_per_dir, create = r.read(2)
     if int(num_per_dir) != 2:
         raise ValueError("only NUM_PER_DIR = 2 (channel) is supported")
     r.skip()
-    nsteps, _tlim, dt, _reset, vdt, cfl, upd = r.read(7)
+    nsteps, tlim, dt, reset, vdt, cfl, upd = r.read(7)
     r.skip()
-    r.read(4)
+    _verbosity, sfi, ssi, _movie = r.read(4)
     r.skip()
     ri, pr = [], []
     for _ in range(n_th):
@@ -578,7 +607,8 @@ def read_decks(directory: str, nx: int, ny: int, nz: int, n_th: int) -> ChannelC
     return ChannelConfig(
         nx=nx, ny=ny, nz=nz, n_th=n_th, nu=nu, beta=beta, lx=lx, ly=ly, lz=lz,
         nu_v_scale=nu_v_scale, delta_t=float(dt), variable_dt=_logical(vdt), cfl=float(cfl),
-        update_dt=int(upd), i_ro=i_ro, grav_x=gx, grav_y=gyv, grav_z=gz,
+        update_dt=int(upd), create_new_flow=_logical(create), time_limit=float(tlim), reset_time=_logical(reset),
+        save_flow_int=int(sfi), save_stats_int=int(ssi), i_ro=i_ro, grav_x=gx, grav_y=gyv, grav_z=gz,
         f_type=int(f_type), ubulk0=float(ubulk0), px0=float(px0), omega0=float(omega0),
         amp_omega0=float(amp), th_bc_ymin=thmin, th_bc_ymin_c1=thmin1, th_bc_ymax=thmax,
         th_bc_ymax_c1=thmax1, ri=ri, pr=pr, drhodx=drx, drhodz=drz,
