@@ -1162,8 +1162,8 @@ static int rk_enqueue(dgpu_ctx *c, int rk_step, double time, double temp1, doubl
         // results then replace the FP32 ones in the working state.
         HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
         HIPCHK(c, hipStreamWaitEvent(c->side_stream, c->ev_fork, 0));
-        dgpu::launch_cn<double, float>(c->side_stream, As, Ps);
-        dgpu::launch_remdiv<double, float>(c->side_stream, Rs, Ps);
+        dgpu::launch_cn_side<double, float>(c->side_stream, As, Ps);
+        dgpu::launch_remdiv_side<double, float>(c->side_stream, Rs, Ps);
         if (int rc = check_launch(c, "low-mode columns")) return rc;
         HIPCHK(c, hipEventRecord(c->ev_join, c->side_stream));
         {
@@ -1624,8 +1624,10 @@ static int ens_enqueue_batched(dgpu_ensemble *e, int rk_step)
             dgpu::Geo gs = e->m[gr.idx[0]]->g;
             gs.ps = e->m[gr.idx[0]]->nside;
             const int nm = (int)gr.idx.size();
-            dgpu::launch_cn_batch<double, float>(e->side, gs, (const dgpu::ColDesc<double, float> *)gr.d_cols[k], nm);
-            dgpu::launch_remdiv_batch<double, float>(e->side, gs, (const dgpu::ColDesc<double, float> *)gr.d_cols[k],
+            dgpu::launch_cn_side_batch<double, float>(e->side, gs, (const dgpu::ColDesc<double, float> *)gr.d_cols[k],
+                                                      nm);
+            dgpu::launch_remdiv_side_batch<double, float>(e->side, gs,
+                                                          (const dgpu::ColDesc<double, float> *)gr.d_cols[k],
                                                      nm);
         }
         if (hipGetLastError() != hipSuccess) return DGPU_E_HIP;

@@ -62,7 +62,11 @@ __device__ __forceinline__ void piped_sweep(int j0, int dir, int n, Ld &&ld, Bod
 // AT / PT: the argument structs by value (a kernel's own arguments) or as
 // references into a descriptor in global memory (batched launches); either
 // way they stay out of private memory.
-template <typename T, int NTH, typename NLT, typename AT, typename PT>
+// DA, DB: planes in flight in sweeps A and B.  SIDE: the few low-mode
+// columns of the mixed path (one block): deeper prefetch, and the table rows
+// fetched with the rest, since no other wavefront hides their latency.
+template <typename T, int NTH, typename NLT, typename AT, typename PT, int DA = (sizeof(T) == 4 ? 6 : 3),
+          int DB = (sizeof(T) == 4 ? 16 : 8), bool SIDE = false>
 __device__ __forceinline__ void cn_body(AT A, PT P, int bx)
 {
     const Geo &g = A.g;
@@ -106,6 +110,7 @@ __device__ __forceinline__ void cn_body(AT A, PT P, int bx)
     if (live) {
         struct InA {
             cx<T> fp, nl, cf, aux1, aux2, cp;
+            double tr[10];  // SIDE: this row's table entries, fetched ahead
         };
         cx<T> fm = zero, f0 = fld[m];  // this system's field at j-1, j (j+1 comes in)
         cx<T> thm = zero;              // CTH at j-1 (U2 buoyancy)
@@ -116,7 +121,6 @@ __device__ __forceinline__ void cn_body(AT A, PT P, int bx)
         const bool has1 = (q != SYS_U2);
         const bool has2 = (q == SYS_TH0) || NTH;
         const bool hascp = (q != SYS_TH0);
-        constexpr int DA = sizeof(T) == 4 ? 6 : 3;  // planes in flight (register budget)
         piped_sweep<DA>(
             0, +1, NY + 2,
             [&](int j) {
@@ -129,11 +133,38 @@ __device__ __forceinline__ void cn_body(AT A, PT P, int bx)
                 in.aux1 = has1 ? a1p[o] : zero;
                 in.aux2 = has2 ? a2p[o] : zero;
                 in.cp = hascp ? p_cp[o] : zero;
+                if constexpr (SIDE) {
+                    const double *t = tab + (long)j * kRow;
+                    in.tr[0] = t[T_MULT];
+                    in.tr[1] = t[T_ISBC];
+                    in.tr[2] = t[T_CFR];
+                    in.tr[3] = t[T_B0];
+                    in.tr[4] = t[T_BM];
+                    in.tr[5] = t[T_BC00];
+                    in.tr[6] = t[T_CPW];
+                    in.tr[7] = t[T_WP];
+                    in.tr[8] = t[T_W0];
+                    in.tr[9] = t[T_WM];
+                }
                 return in;
             },
             [&](int j, const InA &in) {
                 const long o = m + (long)j * ps;
+                double trow[kRow];
                 const double *t = tab + (long)j * kRow;
+                if constexpr (SIDE) {
+                    trow[T_MULT] = in.tr[0];
+                    trow[T_ISBC] = in.tr[1];
+                    trow[T_CFR] = in.tr[2];
+                    trow[T_B0] = in.tr[3];
+                    trow[T_BM] = in.tr[4];
+                    trow[T_BC00] = in.tr[5];
+                    trow[T_CPW] = in.tr[6];
+                    trow[T_WP] = in.tr[7];
+                    trow[T_W0] = in.tr[8];
+                    trow[T_WM] = in.tr[9];
+                    t = trow;
+                }
                 const T mult = T(t[T_MULT]);
                 const bool isbc = t[T_ISBC] != 0.0;
                 const bool cfr = t[T_CFR] != 0.0;
@@ -190,19 +221,38 @@ __device__ __forceinline__ void cn_body(AT A, PT P, int bx)
     __syncthreads();
     // ---------------- sweep B ----------------
     if (live) {
+        struct InB {
+            cx<T> g;
+            double ip, u, cpy;  // SIDE: fetched ahead
+        };
         cx<T> x = zero;
-        piped_sweep<sizeof(T) == 4 ? 16 : 8>(
-            NY + 1, -1, NY + 2, [&](int j) { return gs[m + (long)j * ps]; },
-            [&](int j, const cx<T> &gj) {
+        piped_sweep<DB>(
+            NY + 1, -1, NY + 2,
+            [&](int j) {
+                InB in;
+                in.g = gs[m + (long)j * ps];
+                if constexpr (SIDE) {
+                    const double *t = tab + (long)j * kRow;
+                    in.ip = t[T_INVPIV];
+                    in.u = t[T_UP];
+                    in.cpy = t[T_CPY];
+                }
+                return in;
+            },
+            [&](int j, const InB &in) {
+                const cx<T> gj = in.g;
                 const double *t = tab + (long)j * kRow;
-                const T ip = T(t[T_INVPIV]);
+                const double tip = SIDE ? in.ip : t[T_INVPIV];
+                const double tup = SIDE ? in.u : t[T_UP];
+                const double tcpy = SIDE ? in.cpy : t[T_CPY];
+                const T ip = T(tip);
                 if (j == NY + 1) {
                     x = make_cx<T>(gj.re * ip, gj.im * ip);
                 } else {
-                    const T u = T(t[T_UP]);
+                    const T u = T(tup);
                     x = make_cx<T>((gj.re - u * x.re) * ip, (gj.im - u * x.im) * ip);
                 }
-                if (t[T_CPY] != 0.0) fld[m + (long)j * ps] = x;
+                if (tcpy != 0.0) fld[m + (long)j * ps] = x;
             });
     }
 }
@@ -213,6 +263,13 @@ __global__ void __launch_bounds__(256, 3) k_cn(CnArgs A, CnPtrs<T, NLT> P)
     cn_body<T, NTH, NLT, const CnArgs, const CnPtrs<T, NLT>>(A, P, blockIdx.x);
 }
 
+// The mixed path's low-mode columns: a block or two, so registers are free.
+template <typename T, int NTH, typename NLT>
+__global__ void __launch_bounds__(256, 1) k_cn_side(CnArgs A, CnPtrs<T, NLT> P)
+{
+    cn_body<T, NTH, NLT, const CnArgs, const CnPtrs<T, NLT>, 6, 16, true>(A, P, blockIdx.x);
+}
+
 // Ensemble members sharing NX, NZ (so ps): grid (blocks, members).
 template <typename T, int NTH, typename NLT>
 __global__ void __launch_bounds__(256, 3) k_cn_batch(const ColDesc<T, NLT> *__restrict__ d)
@@ -221,6 +278,14 @@ __global__ void __launch_bounds__(256, 3) k_cn_batch(const ColDesc<T, NLT> *__re
     // would live in scratch
     const ColDesc<T, NLT> &D = d[blockIdx.y];
     cn_body<T, NTH, NLT, const CnArgs &, const CnPtrs<T, NLT> &>(D.A, D.P, blockIdx.x);
+}
+
+// The ensemble's low-mode columns (same sweeps as k_cn_side).
+template <typename T, int NTH, typename NLT>
+__global__ void __launch_bounds__(256, 1) k_cn_side_batch(const ColDesc<T, NLT> *__restrict__ d)
+{
+    const ColDesc<T, NLT> &D = d[blockIdx.y];
+    cn_body<T, NTH, NLT, const CnArgs &, const CnPtrs<T, NLT> &, 6, 16, true>(D.A, D.P, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -263,19 +328,28 @@ __global__ void k_pois_factor(Geo g, const double *__restrict__ dy, const double
 // ---------------------------------------------------------------------------
 // k_remdiv
 // ---------------------------------------------------------------------------
-template <typename T, typename NLT>
+template <typename T, typename NLT, int DC = (sizeof(T) == 4 ? 16 : 8), int DD = (sizeof(T) == 4 ? 12 : 6)>
 __device__ __forceinline__ void remdiv_body(const RdArgs &A, const CnPtrs<T, NLT> &P, int bx)
 {
+    // Two lanes per mode: lane 2m+c takes component c (0 real, 1 imaginary)
+    // of mode m.  The Poisson factors are real, so each component's sweeps
+    // are independent; the divergence and the gradient update cross the
+    // components (i kx, i kz), which the lane reads from memory (divergence)
+    // or from its partner lane (update).  Twice the wavefronts of one lane
+    // per mode, half the work each: these sweeps are latency bound.
     const Geo &g = A.g;
-    const long m = (long)bx * blockDim.x + threadIdx.x;
-    if (m >= g.ps) return;
+    const long lane = (long)bx * blockDim.x + threadIdx.x;
+    const long m = lane >> 1;
+    const int c = (int)(lane & 1);
+    if (m >= g.ps) return;  // both lanes of a pair leave together
     const long gm = A.map ? (long)A.map[m] : m;  // mode of this lane
     const int s = (int)(gm / g.mk), i = (int)(gm - (long)s * g.mk);
     const bool is00 = (i == 0 && s == 0);
     const int NY = g.ny;
     const long ps = g.ps, fps = A.fac_ps;
     const T kx = T(A.kx[i]), kz = T(A.kz[s]);
-    // Poisson factors of this mode at plane j, at the state's precision
+    // i k x: real part -k x.im, imaginary part k x.re
+    const T sgn = c ? T(1) : T(-1);
     auto pm = [&](int j) -> T {
         if constexpr (sizeof(T) == 4) return A.pmultf[gm + (long)j * fps];
         else return T(A.pmult[gm + (long)j * fps]);
@@ -284,39 +358,45 @@ __device__ __forceinline__ void remdiv_body(const RdArgs &A, const CnPtrs<T, NLT
         if constexpr (sizeof(T) == 4) return A.pinvf[gm + (long)j * fps];
         else return T(A.pinv[gm + (long)j * fps]);
     };
-    const cx<T> zero = make_cx<T>(0, 0);
-    cx<T> *scr = P.gs[0];  // forward-eliminated right-hand side
+    // component views: element (plane offset o, component q) at 2 o + q
+    const T *u1 = reinterpret_cast<const T *>(P.cu1);
+    const T *u2 = reinterpret_cast<const T *>(P.cu2);
+    const T *u3 = reinterpret_cast<const T *>(P.cu3);
+    T *w1 = reinterpret_cast<T *>(P.cu1);
+    T *w2 = reinterpret_cast<T *>(P.cu2);
+    T *w3 = reinterpret_cast<T *>(P.cu3);
+    T *wcp = reinterpret_cast<T *>(P.cp);
+    T *scr = reinterpret_cast<T *>(P.gs[0]);  // forward-eliminated right-hand side
+    const int co = c ^ 1;
     // sweep C: rows 2..NY (row 1 is phi(1) = 0 / phi(1) - phi(2) = 0, RHS 0)
     {
         struct InC {
-            cx<T> u1, u3, c2p;
-            T mu, idf;
+            T u1o, u3o, c2p, mu, idf;
         };
-        cx<T> gprev = zero;
-        cx<T> c2 = P.cu2[m + 2 * ps];
-        scr[m + ps] = zero;
-        piped_sweep<sizeof(T) == 4 ? 16 : 8>(
+        T gprev = T(0);
+        T c2 = u2[2 * (m + 2 * ps) + c];
+        scr[2 * (m + ps) + c] = T(0);
+        piped_sweep<DC>(
             2, +1, NY - 1,
             [&](int j) {
                 const long o = m + (long)j * ps;
                 InC in;
-                in.u1 = P.cu1[o];
-                in.u3 = P.cu3[o];
-                in.c2p = P.cu2[o + ps];
+                in.u1o = u1[2 * o + co];
+                in.u3o = u3[2 * o + co];
+                in.c2p = u2[2 * (o + ps) + c];
                 in.mu = pm(j);
                 in.idf = T(A.rd[(long)j * 4 + 1]);
                 return in;
             },
             [&](int j, const InC &in) {
                 const long o = m + (long)j * ps;
-                cx<T> vec = zero;
+                T vec = T(0);
                 if (j < NY) {
                     // (ikx CU1 + (CU2(j+1)-CU2(j))/DYF(j)) + ikz CU3
-                    vec = make_cx<T>((-kx * in.u1.im + (in.c2p.re - c2.re) * in.idf) + (-kz * in.u3.im),
-                                     (kx * in.u1.re + (in.c2p.im - c2.im) * in.idf) + (kz * in.u3.re));
+                    vec = (sgn * kx * in.u1o + (in.c2p - c2) * in.idf) + (sgn * kz * in.u3o);
                 }
-                const cx<T> gj = fma_c(in.mu, gprev, vec);
-                scr[o] = gj;
+                const T gj = vec + in.mu * gprev;
+                scr[2 * o + c] = gj;
                 gprev = gj;
                 c2 = in.c2p;
             });
@@ -326,45 +406,36 @@ __device__ __forceinline__ void remdiv_body(const RdArgs &A, const CnPtrs<T, NLT
     // for that same plane, before its rewrite.
     {
         struct InD {
-            cx<T> g, u1, u3, c2, cp;
-            T ip, U, idy;
+            T g, u1, u3, c2, cp, ip, U, idy;
         };
         const T u1row = is00 ? T(0) : T(-1);
         const T itemp4 = T(A.inv_temp4);
         const bool upd = A.update_cp;
-        cx<T> phip = zero;
-        piped_sweep<sizeof(T) == 4 ? 12 : 6>(
+        T phip = T(0);
+        piped_sweep<DD>(
             NY, -1, NY,
             [&](int j) {
                 const long o = m + (long)j * ps;
                 InD in;
-                in.g = scr[o];
+                in.g = scr[2 * o + c];
                 in.ip = pv(j);
                 in.U = (j == 1) ? u1row : T(A.rd[(long)j * 4 + 0]);
-                in.u1 = P.cu1[o];
-                in.u3 = P.cu3[o];
-                in.c2 = P.cu2[o + ps];  // plane NY+1 exists (unused for j = NY)
+                in.u1 = u1[2 * o + c];
+                in.u3 = u3[2 * o + c];
+                in.c2 = u2[2 * (o + ps) + c];  // plane NY+1 exists (unused for j = NY)
                 in.idy = T(A.rd[(long)(j + 1) * 4 + 2]);
-                in.cp = upd ? P.cp[o] : zero;
+                in.cp = upd ? wcp[2 * o + c] : T(0);
                 return in;
             },
             [&](int j, const InD &in) {
                 const long o = m + (long)j * ps;
-                cx<T> phi;
-                if (j == NY) {
-                    phi = make_cx<T>(in.g.re * in.ip, in.g.im * in.ip);
-                } else {
-                    phi = make_cx<T>((in.g.re - in.U * phip.re) * in.ip, (in.g.im - in.U * phip.im) * in.ip);
-                }
-                P.cu1[o] = make_cx<T>(in.u1.re + kx * phi.im, in.u1.im - kx * phi.re);
-                P.cu3[o] = make_cx<T>(in.u3.re + kz * phi.im, in.u3.im - kz * phi.re);
-                if (j + 1 <= NY) {
-                    P.cu2[o + ps] = make_cx<T>(in.c2.re - (phip.re - phi.re) * in.idy,
-                                               in.c2.im - (phip.im - phi.im) * in.idy);
-                }
-                if (upd && j >= A.jstart && j <= A.jend) {
-                    P.cp[o] = make_cx<T>(in.cp.re + phi.re * itemp4, in.cp.im + phi.im * itemp4);
-                }
+                const T phi = (j == NY) ? in.g * in.ip : (in.g - in.U * phip) * in.ip;
+                const T phio = __shfl_xor(phi, 1);  // the partner lane's component
+                // CU1 - i kx phi: real part + kx phi.im, imaginary part - kx phi.re
+                w1[2 * o + c] = in.u1 - sgn * kx * phio;
+                w3[2 * o + c] = in.u3 - sgn * kz * phio;
+                if (j + 1 <= NY) w2[2 * (o + ps) + c] = in.c2 - (phip - phi) * in.idy;
+                if (upd && j >= A.jstart && j <= A.jend) wcp[2 * o + c] = in.cp + phi * itemp4;
                 phip = phi;
             });
     }
@@ -376,11 +447,25 @@ __global__ void __launch_bounds__(64) k_remdiv(RdArgs A, CnPtrs<T, NLT> P)
     remdiv_body<T, NLT>(A, P, blockIdx.x);
 }
 
+// The mixed path's low-mode columns: deeper prefetch (few wavefronts).
+template <typename T, typename NLT>
+__global__ void __launch_bounds__(64) k_remdiv_side(RdArgs A, CnPtrs<T, NLT> P)
+{
+    remdiv_body<T, NLT, 16, 16>(A, P, blockIdx.x);
+}
+
 template <typename T, typename NLT>
 __global__ void __launch_bounds__(64) k_remdiv_batch(const ColDesc<T, NLT> *__restrict__ d)
 {
     const ColDesc<T, NLT> &D = d[blockIdx.y];
     remdiv_body<T, NLT>(D.R, D.P, blockIdx.x);
+}
+
+template <typename T, typename NLT>
+__global__ void __launch_bounds__(64) k_remdiv_side_batch(const ColDesc<T, NLT> *__restrict__ d)
+{
+    const ColDesc<T, NLT> &D = d[blockIdx.y];
+    remdiv_body<T, NLT, 16, 16>(D.R, D.P, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -397,9 +482,26 @@ void launch_cn_batch(hipStream_t s, const Geo &g, const ColDesc<T, NLT> *d, int 
 }
 
 template <typename T, typename NLT>
-void launch_remdiv_batch(hipStream_t s, const Geo &g, const ColDesc<T, NLT> *d, int nmem)
+void launch_cn_side_batch(hipStream_t s, const Geo &g, const ColDesc<T, NLT> *d, int nmem)
 {
     const long blocks = (g.ps + 63) / 64;
+    if (g.nth == 0)
+        hipLaunchKernelGGL((k_cn_side_batch<T, 0, NLT>), dim3(blocks, nmem), dim3(256), 0, s, d);
+    else
+        hipLaunchKernelGGL((k_cn_side_batch<T, 1, NLT>), dim3(blocks, nmem), dim3(256), 0, s, d);
+}
+
+template <typename T, typename NLT>
+void launch_remdiv_side_batch(hipStream_t s, const Geo &g, const ColDesc<T, NLT> *d, int nmem)
+{
+    const long blocks = (2 * g.ps + 63) / 64;
+    hipLaunchKernelGGL((k_remdiv_side_batch<T, NLT>), dim3(blocks, nmem), dim3(64), 0, s, d);
+}
+
+template <typename T, typename NLT>
+void launch_remdiv_batch(hipStream_t s, const Geo &g, const ColDesc<T, NLT> *d, int nmem)
+{
+    const long blocks = (2 * g.ps + 63) / 64;  // two lanes per mode
     hipLaunchKernelGGL((k_remdiv_batch<T, NLT>), dim3(blocks, nmem), dim3(64), 0, s, d);
 }
 
@@ -409,6 +511,9 @@ template void launch_cn_batch<float, float>(hipStream_t, const Geo &, const ColD
 template void launch_remdiv_batch<double, double>(hipStream_t, const Geo &, const ColDesc<double, double> *, int);
 template void launch_remdiv_batch<double, float>(hipStream_t, const Geo &, const ColDesc<double, float> *, int);
 template void launch_remdiv_batch<float, float>(hipStream_t, const Geo &, const ColDesc<float, float> *, int);
+template void launch_cn_side_batch<double, float>(hipStream_t, const Geo &, const ColDesc<double, float> *, int);
+template void launch_remdiv_side_batch<double, float>(hipStream_t, const Geo &, const ColDesc<double, float> *,
+                                                       int);
 
 template <typename T, typename NLT>
 void launch_cn(hipStream_t s, const CnArgs &a, const CnPtrs<T, NLT> &p)
@@ -421,9 +526,26 @@ void launch_cn(hipStream_t s, const CnArgs &a, const CnPtrs<T, NLT> &p)
 }
 
 template <typename T, typename NLT>
-void launch_remdiv(hipStream_t s, const RdArgs &a, const CnPtrs<T, NLT> &p)
+void launch_cn_side(hipStream_t s, const CnArgs &a, const CnPtrs<T, NLT> &p)
 {
     const long blocks = (a.g.ps + 63) / 64;
+    if (a.g.nth == 0)
+        hipLaunchKernelGGL((k_cn_side<T, 0, NLT>), dim3(blocks), dim3(256), 0, s, a, p);
+    else
+        hipLaunchKernelGGL((k_cn_side<T, 1, NLT>), dim3(blocks), dim3(256), 0, s, a, p);
+}
+
+template <typename T, typename NLT>
+void launch_remdiv_side(hipStream_t s, const RdArgs &a, const CnPtrs<T, NLT> &p)
+{
+    const long blocks = (2 * a.g.ps + 63) / 64;
+    hipLaunchKernelGGL((k_remdiv_side<T, NLT>), dim3(blocks), dim3(64), 0, s, a, p);
+}
+
+template <typename T, typename NLT>
+void launch_remdiv(hipStream_t s, const RdArgs &a, const CnPtrs<T, NLT> &p)
+{
+    const long blocks = (2 * a.g.ps + 63) / 64;  // two lanes per mode
     hipLaunchKernelGGL((k_remdiv<T, NLT>), dim3(blocks), dim3(64), 0, s, a, p);
 }
 
@@ -540,5 +662,7 @@ template void launch_remdiv<float, float>(hipStream_t, const RdArgs &, const CnP
 template void launch_cn<double, float>(hipStream_t, const CnArgs &, const CnPtrs<double, float> &);
 template void launch_remdiv<double, double>(hipStream_t, const RdArgs &, const CnPtrs<double, double> &);
 template void launch_remdiv<double, float>(hipStream_t, const RdArgs &, const CnPtrs<double, float> &);
+template void launch_cn_side<double, float>(hipStream_t, const CnArgs &, const CnPtrs<double, float> &);
+template void launch_remdiv_side<double, float>(hipStream_t, const RdArgs &, const CnPtrs<double, float> &);
 
 }  // namespace dgpu

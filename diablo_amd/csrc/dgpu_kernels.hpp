@@ -97,6 +97,10 @@ template <typename T, typename NLT>
 void launch_cn_batch(hipStream_t s, const Geo &g, const ColDesc<T, NLT> *d, int nmem);
 template <typename T, typename NLT>
 void launch_remdiv_batch(hipStream_t s, const Geo &g, const ColDesc<T, NLT> *d, int nmem);
+template <typename T, typename NLT>
+void launch_cn_side_batch(hipStream_t s, const Geo &g, const ColDesc<T, NLT> *d, int nmem);
+template <typename T, typename NLT>
+void launch_remdiv_side_batch(hipStream_t s, const Geo &g, const ColDesc<T, NLT> *d, int nmem);
 
 // ---- launchers (dgpu_kernels.hip) ----
 template <typename T, typename TIN>
@@ -167,6 +171,11 @@ int xfused_ip_threads(const IpLen &fx, int nth);
 
 template <typename T, typename NLT>
 void launch_cn(hipStream_t s, const CnArgs &a, const CnPtrs<T, NLT> &p);
+// The mixed path's low-mode columns (few lanes): deeper prefetch variants.
+template <typename T, typename NLT>
+void launch_cn_side(hipStream_t s, const CnArgs &a, const CnPtrs<T, NLT> &p);
+template <typename T, typename NLT>
+void launch_remdiv_side(hipStream_t s, const RdArgs &a, const CnPtrs<T, NLT> &p);
 template <typename T, typename NLT>
 void launch_remdiv(hipStream_t s, const RdArgs &a, const CnPtrs<T, NLT> &p);
 void launch_pois_factor(hipStream_t s, const Geo &g, const double *dy, const double *dyf, const double *kx2,
