@@ -1616,3 +1616,194 @@ void dor_tke_chan(dor_state *s, double *eps)
     for (int j = 1; j <= NY; j++) eps[j - 1] = s->p.nu * e[j] / (double)(NX * NZ);
     free(U1); free(U2); free(U3); free(ume); free(vme); free(wme); free(e);
 }
+
+/* ------------------------------------------------------------------------ */
+/* les_chan (for/les.f:1-267, 274-331), LES_MODEL_TYPE = 1 (constant         */
+/* Smagorinsky; the dynamic models stop under MPI, for/les.f:269-272), at    */
+/* NPROCY = 1: APPLY_BC_VEL, APPLY_BC_LES (for/les.f:805-864),               */
+/* compute_strain_chan (for/les.f:526-571), |S| on GYF and GY, NU_T on GY    */
+/* with GHOST_LES_MPI's zero rows 1, 2, NY, NY+1 (for/mpi.f:488-503), the    */
+/* stress tau_ij = DELTA * |S| S_ij in Fourier space and its divergence      */
+/* subtracted from CF1-3, plus the explicit cross terms added to CF2.  The   */
+/* mean-statistics block (SAVE_STATS_INT) writes files only and is left out. */
+/* nu_t: NU_T (field layout), may be NULL.  The velocity state is changed    */
+/* only by the two BC routines, as in the reference; it stays in Fourier     */
+/* space here (the reference leaves U1-3 physical on return).                */
+/* ------------------------------------------------------------------------ */
+void dor_les_chan(dor_state *s, double *nu_t)
+{
+    const int NX = s->NX, NY = s->NY, NZ = s->NZ, NXP = s->NXP, TNKZ = s->TNKZ, NXM = s->NXM, NZP = s->NZP;
+    const double *DY = s->DY, *DYF = s->DYF, *KX = s->KX, *KZ = s->KZ;
+    const size_t fs = s->fsize;
+    const dor_params *p = &s->p;
+    const double C_SMAG = 0.13, alpha_sgs = 2.449, beta_sgs = 1.0;
+    (void)alpha_sgs;
+    const int J1 = s->JSTART, J2 = NY - 1;     /* RANKY = NPROCY-1 branch, for/les.f:46-50 */
+    dor_apply_bc_vel(s);
+    cplx *CU1 = (cplx *)s->fld[DOR_U1], *CU2 = (cplx *)s->fld[DOR_U2], *CU3 = (cplx *)s->fld[DOR_U3];
+    /* APPLY_BC_LES */
+    if (p->u_bc_ymax == 1)
+        for (int k = 0; k <= TNKZ; k++) for (int i = 0; i <= NXP - 1; i++) {
+            CU1[CIDX(i, k, NY)] = CU1[CIDX(i, k, NY - 1)];
+            CU1[CIDX(i, k, NY + 1)] = CU1[CIDX(i, k, NY)];
+        }
+    if (p->w_bc_ymax == 1)
+        for (int k = 0; k <= TNKZ; k++) for (int i = 0; i <= NXP - 1; i++) {
+            CU3[CIDX(i, k, NY)] = CU3[CIDX(i, k, NY - 1)];
+            CU3[CIDX(i, k, NY + 1)] = CU3[CIDX(i, k, NY)];
+        }
+    if (p->u_bc_ymin == 1)
+        for (int k = 0; k <= TNKZ; k++) for (int i = 0; i <= NXP - 1; i++) {
+            CU1[CIDX(i, k, 1)] = CU1[CIDX(i, k, 2)];
+            CU1[CIDX(i, k, 0)] = CU1[CIDX(i, k, 1)];
+        }
+    if (p->w_bc_ymin == 1)
+        for (int k = 0; k <= TNKZ; k++) for (int i = 0; i <= NXP - 1; i++) {
+            CU3[CIDX(i, k, 1)] = CU3[CIDX(i, k, 2)];
+            CU3[CIDX(i, k, 0)] = CU3[CIDX(i, k, 1)];
+        }
+    /* compute_strain_chan: S_ij in Fourier space, then physical */
+    double *Sij[7];
+    for (int q = 1; q <= 6; q++) Sij[q] = zalloc(fs);
+    cplx *CS[7];
+    for (int q = 1; q <= 6; q++) CS[q] = (cplx *)Sij[q];
+    for (int J = 1; J <= NY; J++)
+        for (int K = 0; K <= TNKZ; K++)
+            for (int i = 0; i <= NXP - 1; i++) {
+                const size_t x = CIDX(i, K, J);
+                CS[1][x] = (I * KX[i]) * CU1[x];
+                CS[2][x] = (CU2[CIDX(i, K, J + 1)] - CU2[x]) / DYF[J];
+                CS[3][x] = (I * KZ[K]) * CU3[x];
+                CS[5][x] = 0.5 * ((I * KZ[K]) * CU1[x] + (I * KX[i]) * CU3[x]);
+            }
+    for (int J = 1; J <= NY + 1; J++)
+        for (int K = 0; K <= TNKZ; K++)
+            for (int i = 0; i <= NXP - 1; i++) {
+                const size_t x = CIDX(i, K, J);
+                CS[4][x] = 0.5 * ((CU1[x] - CU1[CIDX(i, K, J - 1)]) / DY[J] + (I * KX[i]) * CU2[x]);
+                CS[6][x] = 0.5 * ((I * KZ[K]) * CU2[x] + (CU3[x] - CU3[CIDX(i, K, J - 1)]) / DY[J]);
+            }
+    for (int q = 1; q <= 6; q++) dor_fft_xz_to_physical(s, Sij[q], 0, NY + 1);
+    double *U1b = zalloc(NY + 2), *U3b = zalloc(NY + 2);
+    for (int J = 0; J <= NY + 1; J++) {
+        U1b[J] = creal(CU1[CIDX(0, 0, J)]);
+        U3b[J] = creal(CU3[CIDX(0, 0, J)]);
+    }
+    double *U1 = zalloc(fs), *U3 = zalloc(fs);
+    memcpy(U1, s->fld[DOR_U1], sizeof(double) * fs);
+    memcpy(U3, s->fld[DOR_U3], sizeof(double) * fs);
+    dor_fft_xz_to_physical(s, U1, 0, NY + 1);
+    dor_fft_xz_to_physical(s, U3, 0, NY + 1);
+    double *S1 = zalloc(fs), *TEMP = zalloc(fs), *NUT = zalloc(fs);
+#define P2(v) pow((v), 2.0)
+    for (int J = s->JSTART; J <= s->JEND; J++)
+        for (int K = 0; K <= NZP - 1; K++)
+            for (int i = 0; i <= NXM; i++) {
+                const size_t x = RIDX(i, K, J), xp = RIDX(i, K, J + 1);
+                S1[x] = sqrt(2.0 * P2(Sij[1][x])
+                             + 4.0 * P2(0.5 * (Sij[4][xp] + Sij[4][x]) - 0.5 * (U1b[J] - U1b[J - 1]) / DY[J]
+                                        - 0.5 * (U1b[J + 1] - U1b[J]) / DY[J + 1])
+                             + 4.0 * P2(Sij[5][x]) + 2.0 * P2(Sij[2][x])
+                             + 4.0 * P2(0.5 * (Sij[6][xp] + Sij[6][x]) - 0.5 * (U3b[J] - U3b[J - 1]) / DY[J]
+                                        - 0.5 * (U3b[J + 1] - U3b[J]) / DY[J + 1])
+                             + 2.0 * P2(Sij[3][x]));
+            }
+    for (int J = 2; J <= NY + 1; J++)
+        for (int K = 0; K <= NZP - 1; K++)
+            for (int i = 0; i <= NXM; i++) {
+                const size_t x = RIDX(i, K, J), xm = RIDX(i, K, J - 1);
+                TEMP[x] = sqrt(2.0 * P2((Sij[1][x] * DYF[J - 1] + Sij[1][xm] * DYF[J]) / (2.0 * DY[J]))
+                               + 4.0 * P2(Sij[4][x] - 0.5 * (U1b[J] - U1b[J - 1]) / DY[J])
+                               + 4.0 * P2((Sij[5][x] * DYF[J - 1] + Sij[5][xm] * DYF[J]) / (2.0 * DY[J]))
+                               + 2.0 * P2((Sij[2][x] * DYF[J - 1] + Sij[2][xm] * DYF[J]) / (2.0 * DY[J]))
+                               + 4.0 * P2(Sij[6][x] - 0.5 * (U3b[J] - U3b[J - 1]) / DY[J])
+                               + 2.0 * P2((Sij[3][x] * DYF[J - 1] + Sij[3][xm] * DYF[J]) / (2.0 * DY[J])));
+            }
+#undef P2
+    for (int J = s->JSTART; J <= s->JEND; J++)
+        for (int K = 0; K <= NZP - 1; K++)
+            for (int i = 0; i <= NXM; i++) {
+                const size_t x = RIDX(i, K, J);
+                Sij[1][x] = S1[x] * Sij[1][x];
+                Sij[5][x] = S1[x] * Sij[5][x];
+                Sij[2][x] = 0.0;
+                Sij[3][x] = S1[x] * Sij[3][x];
+            }
+    for (int J = 2; J <= NY + 1; J++)
+        for (int K = 0; K <= NZP - 1; K++)
+            for (int i = 0; i <= NXM; i++) {
+                const size_t x = RIDX(i, K, J), xm = RIDX(i, K, J - 1);
+                Sij[4][x] = TEMP[x] * (Sij[4][x] - 0.5 * (U1[x] - U1[xm]) / DY[J]);
+                Sij[6][x] = TEMP[x] * (Sij[6][x] - 0.5 * (U3[x] - U3[xm]) / DY[J]);
+            }
+    /* filter widths, -2 C_SMAG^2 absorbed (for/les.f:200-223) */
+    const double dx1 = p->lx / NX, dz1 = p->lz / NZ;
+    double *DELTA_YF = zalloc(NY + 2), *DELTA_Y = zalloc(NY + 2);
+    for (int J = 1; J <= NY + 1; J++) {
+        DELTA_YF[J] = -2.0 * pow(C_SMAG, 2.0) * pow(dx1 * beta_sgs * DYF[J] * 2.0 * dz1 * beta_sgs, 2.0 / 3.0);
+        DELTA_Y[J] = -2.0 * pow(C_SMAG, 2.0) * pow(dx1 * beta_sgs * DY[J] * 2.0 * dz1 * beta_sgs, 2.0 / 3.0);
+    }
+    for (int J = 2; J <= NY; J++)
+        for (int K = 0; K <= NZP - 1; K++)
+            for (int i = 0; i <= NXM; i++) NUT[RIDX(i, K, J)] = -0.5 * DELTA_Y[J] * TEMP[RIDX(i, K, J)];
+    for (int K = 0; K <= NZP - 1; K++)         /* GHOST_LES_MPI at NPROCY = 1 */
+        for (int i = 0; i <= NXM; i++) {
+            NUT[RIDX(i, K, 1)] = 0.0;
+            NUT[RIDX(i, K, 2)] = 0.0;
+            NUT[RIDX(i, K, NY)] = 0.0;
+            NUT[RIDX(i, K, NY + 1)] = 0.0;
+        }
+    const int tq[5] = {1, 3, 4, 5, 6};
+    for (int t = 0; t < 5; t++) dor_fft_xz_to_fourier(s, Sij[tq[t]], 0, NY + 1);
+    for (int J = 1; J <= NY + 1; J++)
+        for (int K = 0; K <= TNKZ; K++)
+            for (int i = 0; i <= NXP - 1; i++) {
+                const size_t x = CIDX(i, K, J);
+                CS[1][x] = DELTA_YF[J] * CS[1][x];
+                CS[5][x] = DELTA_YF[J] * CS[5][x];
+                CS[3][x] = DELTA_YF[J] * CS[3][x];
+            }
+    for (int J = 2; J <= NY + 1; J++)
+        for (int K = 0; K <= TNKZ; K++)
+            for (int i = 0; i <= NXP - 1; i++) {
+                const size_t x = CIDX(i, K, J);
+                CS[4][x] = DELTA_Y[J] * CS[4][x];
+                CS[6][x] = DELTA_Y[J] * CS[6][x];
+            }
+    /* the SGS forcing into CF1-3 (for/les.f:274-301) */
+    cplx *CF1 = (cplx *)s->fld[DOR_F1], *CF2 = (cplx *)s->fld[DOR_F2], *CF3 = (cplx *)s->fld[DOR_F3];
+    for (int J = J1; J <= J2; J++)
+        for (int K = 0; K <= TNKZ; K++)
+            for (int i = 0; i <= NXP - 1; i++) {
+                const size_t x = CIDX(i, K, J), xp = CIDX(i, K, J + 1);
+                CF1[x] = CF1[x] - (I * KX[i]) * CS[1][x] - (CS[4][xp] - CS[4][x]) / DYF[J] - (I * KZ[K]) * CS[5][x];
+                CF3[x] = CF3[x] - (I * KX[i]) * CS[5][x] - (CS[6][xp] - CS[6][x]) / DYF[J] - (I * KZ[K]) * CS[3][x];
+            }
+    for (int J = 2; J <= NY; J++)
+        for (int K = 0; K <= TNKZ; K++)
+            for (int i = 0; i <= NXP - 1; i++) {
+                const size_t x = CIDX(i, K, J);
+                CF2[x] = CF2[x] - (I * KX[i]) * CS[4][x] - (I * KZ[K]) * CS[6][x];
+            }
+    /* cross terms, explicit (for/les.f:302-331) */
+    double *T2 = zalloc(fs);
+    cplx *CT = (cplx *)T2;
+    for (int pass = 0; pass < 2; pass++) {
+        const double *UU = pass == 0 ? U1 : U3;
+        for (int K = 0; K <= NZP - 1; K++)
+            for (int i = 0; i <= NXM; i++)
+                for (int J = 2; J <= NY; J++)
+                    T2[RIDX(i, K, J)] = NUT[RIDX(i, K, J)] * (UU[RIDX(i, K, J)] - UU[RIDX(i, K, J - 1)]) / DY[J];
+        dor_fft_xz_to_fourier(s, T2, 2, NY);
+        for (int K = 0; K <= TNKZ; K++)
+            for (int i = 0; i <= NXP - 1; i++)
+                for (int J = 2; J <= NY; J++) {
+                    const size_t x = CIDX(i, K, J);
+                    CF2[x] = CF2[x] + (pass == 0 ? (I * KX[i]) : (I * KZ[K])) * CT[x];
+                }
+    }
+    if (nu_t) memcpy(nu_t, NUT, sizeof(double) * fs);
+    for (int q = 1; q <= 6; q++) free(Sij[q]);
+    free(U1b); free(U3b); free(U1); free(U3); free(S1); free(TEMP); free(NUT);
+    free(DELTA_YF); free(DELTA_Y); free(T2);
+}

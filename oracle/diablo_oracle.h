@@ -114,6 +114,11 @@ void dor_tke_chan(dor_state *s, double *eps);
 void dor_flow_to_file(dor_state *s, int id, double *out);
 void dor_flow_from_file(dor_state *s, int id, const double *in);
 
+/* les_chan, constant Smagorinsky (for/les.f:1-331): BCs, strain, NU_T on GY,
+ * the SGS stress divergence into CF1-3 and the explicit cross terms into
+ * CF2.  nu_t (field layout, may be NULL) receives NU_T. */
+void dor_les_chan(dor_state *s, double *nu_t);
+
 void dor_thomas_real(double *a, double *b, double *c, double *g, int ny, int nx);
 void dor_thomas_complex(double *a, double *b, double *c, double *g_c, int ny, int nxp);
 void dor_fft_c2c(double *data, int n, int sign);   /* unnormalised, sign -1 forward */

@@ -89,6 +89,7 @@ def lib():
         L.dor_c2r.argtypes = [dp, dp, i]
         L.dor_stats_chan.argtypes = [vp, dp, dp]
         L.dor_tke_chan.argtypes = [vp, dp]
+        L.dor_les_chan.argtypes = [vp, dp]
         _lib = L
     return _lib
 
@@ -221,6 +222,13 @@ class Oracle:
         scratch's previous contents, as in the reference)."""
         out = np.zeros(self.cfg.ny)
         self.L.dor_tke_chan(self.h, _dp(out))
+        return out
+
+    def les_chan(self) -> np.ndarray:
+        """les_chan (constant Smagorinsky): updates CF1-3 (and the velocity
+        BC rows); returns NU_T in field layout."""
+        out = np.zeros(self.cfg.field_shape())
+        self.L.dor_les_chan(self.h, _dp(out))
         return out
 
     # -- restart-file layout (WriteHDF5 / ReadHDF5) --
