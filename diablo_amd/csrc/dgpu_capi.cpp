@@ -89,6 +89,7 @@ struct dgpu_ctx {
     int courant_state = 0;
     double *d_vmax_part = nullptr, *d_vmax = nullptr;  // per-block, per-plane velocity maxima
     double *d_file = nullptr;  // one restart-file array, NX*NY*NZ
+    double *d_les = nullptr;   // les_chan: 14 Fortran-layout fields, then DELTA_YF, DELTA_Y
     // CN tables: [stage 0..2][sys][NY+2][kRow]
     double *d_cn = nullptr;
     std::vector<double> h_cn;
@@ -710,7 +711,7 @@ int32_t dgpu_destroy(dgpu_ctx *c)
     void *bufs[] = {c->d_vec, c->d_tw, c->d_state, c->d_pmult, c->d_pinv, c->d_rd, c->d_nlptr, c->d_zt, c->d_wt,
                     c->d_fa, c->d_fb, c->d_gyf, c->d_iptw, c->d_iptwf, c->d_ztf, c->d_wtf, c->d_nlf, c->d_nlptrf,
                     c->d_fort, c->d_cn, c->d_codes, c->d_c1, c->d_state32, c->d_side, c->d_map,
-                    c->d_pmultf, c->d_pinvf, c->d_gs32, c->d_stats, c->d_vmax_part, c->d_vmax, c->d_file};
+                    c->d_pmultf, c->d_pinvf, c->d_gs32, c->d_stats, c->d_vmax_part, c->d_vmax, c->d_file, c->d_les};
     for (void *b : bufs)
         if (b) hipFree(b);
     if (c->side_stream) {
@@ -890,6 +891,70 @@ int32_t dgpu_tke_chan(dgpu_ctx *c, double *epsilon)
                          P(7, j) / (DY[j] * DY[j]) + P(4, j) + gyw(5);
         epsilon[j - 1] = c->p.nu * e;
     }
+    return DGPU_OK;
+}
+
+// les_chan with LES_MODEL_TYPE = 1 (for/les.f:1-331) at NPROCY = 1: the
+// velocity BCs and APPLY_BC_LES on the state, S_ij and U1, U3 to physical
+// space, |S| and NU_T, tau_ij back to Fourier space and its divergence into
+// CF1-3, then the explicit cross terms NU_T dU/dy into CF2.
+int32_t dgpu_les_chan(dgpu_ctx *c, double *nu_t)
+{
+    if (!c) return DGPU_E_ARG;
+    if (int rc = to_master(c)) return rc;
+    c->master_dirty = true;
+    const dgpu::Geo &g = c->g;
+    const int NY = g.ny;
+    const size_t n = (size_t)(g.nx + 2) * (g.nz + 2) * (NY + 2);
+    if (!c->d_les) {
+        HIPCHK(c, hipMalloc(&c->d_les, (14 * n + 2 * (size_t)(NY + 2)) * sizeof(double)));
+        // NU_T's padding and the unset rows of the work fields stay zero
+        HIPCHK(c, hipMemsetAsync(c->d_les, 0, 14 * n * sizeof(double), c->stream));
+        // filter widths with -2 C_SMAG^2 absorbed (for/les.f:200-223):
+        // C_SMAG = 0.13, beta_sgs = 1, DELTA = (DX DY 2 DZ)^(1/3)
+        const double C_SMAG = 0.13, beta_sgs = 1.0;
+        const double dx = c->p.lx / g.nx, dz = c->p.lz / g.nz;
+        std::vector<double> dl(2 * (size_t)(NY + 2), 0.0);
+        for (int j = 1; j <= NY + 1; ++j) {
+            dl[j] = -2.0 * std::pow(C_SMAG, 2.0) * std::pow(dx * beta_sgs * c->dyf[j] * 2.0 * dz * beta_sgs, 2.0 / 3.0);
+            dl[NY + 2 + j] =
+                -2.0 * std::pow(C_SMAG, 2.0) * std::pow(dx * beta_sgs * c->dy[j] * 2.0 * dz * beta_sgs, 2.0 / 3.0);
+        }
+        HIPCHK(c, hipMemcpyAsync(c->d_les + 14 * n, dl.data(), dl.size() * sizeof(double), hipMemcpyHostToDevice,
+                                 c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    const double *delta_yf = c->d_les + 14 * n, *delta_y = delta_yf + (NY + 2);
+    dgpu::LesPtrs S, O;
+    for (int q = 0; q < 8; ++q) S.p[q] = c->d_les + (size_t)q * n;
+    for (int q = 0; q < 6; ++q) O.p[q] = c->d_les + (size_t)(8 + q) * n;
+    O.p[6] = O.p[7] = nullptr;
+    dgpu::launch_apply_bc_vel(c->stream, g, c->cu[0], c->cu[1], c->cu[2], c->d_codes, c->d_c1, c->d_dy, c->d_dyf);
+    if (int rc = check_launch(c, "apply_bc_vel")) return rc;
+    dgpu::launch_les_bc(c->stream, g, c->cu[0], c->cu[2], c->p.u_bc_ymin, c->p.u_bc_ymax, c->p.w_bc_ymin,
+                        c->p.w_bc_ymax);
+    if (int rc = check_launch(c, "les_bc")) return rc;
+    dgpu::launch_les_strain(c->stream, g, c->cu[0], c->cu[1], c->cu[2], c->d_kx, c->d_kz, c->d_dy, c->d_dyf, S);
+    if (int rc = check_launch(c, "les_strain")) return rc;
+    for (int q = 0; q < 8; ++q)
+        if (int rc = fort_to_physical(c, S.p[q], 0, NY + 1)) return rc;
+    dgpu::launch_les_nut(c->stream, g, S, O, c->cu[0], c->cu[2], c->d_dy, c->d_dyf, delta_y, c->jstart, c->jend);
+    if (int rc = check_launch(c, "les_nut")) return rc;
+    for (int q = 0; q < 5; ++q)
+        if (int rc = fort_to_fourier(c, O.p[q], 0, NY + 1)) return rc;
+    // CF1, CF3 on J1 = JSTART .. J2 = NY-1 (the RANKY = NPROCY-1 bounds, for/les.f:46-50)
+    dgpu::launch_les_cf(c->stream, g, O, c->cf[0], c->cf[1], c->cf[2], c->d_kx, c->d_kz, c->d_dyf, delta_y, delta_yf,
+                        c->jstart, NY - 1);
+    if (int rc = check_launch(c, "les_cf")) return rc;
+    for (int pass = 0; pass < 2; ++pass) {
+        dgpu::launch_les_cross_src(c->stream, g, O.p[5], S.p[6 + pass], c->d_dy, O.p[0]);
+        if (int rc = check_launch(c, "les_cross_src")) return rc;
+        if (int rc = fort_to_fourier(c, O.p[0], 2, NY)) return rc;
+        dgpu::launch_les_cross_add(c->stream, g, O.p[0], c->cf[1], c->d_kx, c->d_kz, pass);
+        if (int rc = check_launch(c, "les_cross_add")) return rc;
+    }
+    if (nu_t) HIPCHK(c, hipMemcpyAsync(nu_t, O.p[5], n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return DGPU_OK;
 }
 

@@ -227,4 +227,25 @@ void launch_side_scatter(hipStream_t s, const cx<TS> *side, long side_fs, cx<TF>
 void launch_apply_bc_vel(hipStream_t s, const Geo &g, cx<double> *cu1, cx<double> *cu2, cx<double> *cu3,
                          const int *codes, const double *c1, const double *dy, const double *dyf);
 
+// les_chan's constant Smagorinsky model (dgpu_les.hip).  Planes are
+// Fortran-layout (0:NX+1, 0:NZ+1, 0:NY+1) doubles, passed by value.
+struct LesPtrs {
+    double *p[8];
+};
+void launch_les_bc(hipStream_t s, const Geo &g, cx<double> *cu1, cx<double> *cu3, int u_min, int u_max, int w_min,
+                   int w_max);
+void launch_les_strain(hipStream_t s, const Geo &g, const cx<double> *cu1, const cx<double> *cu2,
+                       const cx<double> *cu3, const double *kx, const double *kz, const double *dy, const double *dyf,
+                       LesPtrs out);
+void launch_les_nut(hipStream_t s, const Geo &g, LesPtrs in, LesPtrs out, const cx<double> *cu1,
+                    const cx<double> *cu3, const double *dy, const double *dyf, const double *delta_y, int jstart,
+                    int jend);
+void launch_les_cf(hipStream_t s, const Geo &g, LesPtrs t, cx<double> *cf1, cx<double> *cf2, cx<double> *cf3,
+                   const double *kx, const double *kz, const double *dyf, const double *delta_y,
+                   const double *delta_yf, int j1, int j2);
+void launch_les_cross_src(hipStream_t s, const Geo &g, const double *nut, const double *u, const double *dy,
+                          double *out);
+void launch_les_cross_add(hipStream_t s, const Geo &g, const double *ct, cx<double> *cf2, const double *kx,
+                          const double *kz, int use_kz);
+
 }  // namespace dgpu

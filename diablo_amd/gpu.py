@@ -57,7 +57,7 @@ EXPORTS = (
     "dgpu_ensemble_create", "dgpu_ensemble_destroy", "dgpu_ensemble_size", "dgpu_ensemble_member",
     "dgpu_ensemble_last_error", "dgpu_ensemble_set_graphs", "dgpu_ensemble_rk_chan_1",
     "dgpu_ensemble_synchronize", "dgpu_stats_chan", "dgpu_courant_request", "dgpu_courant",
-    "dgpu_get_flow_physical", "dgpu_set_flow_physical", "dgpu_tke_chan",
+    "dgpu_get_flow_physical", "dgpu_set_flow_physical", "dgpu_tke_chan", "dgpu_les_chan",
 )
 
 # dgpu_stats_chan profiles, in the order of the header's DGPU_ST_*
@@ -121,6 +121,8 @@ def load_library(path: str = LIB_PATH):
     L.dgpu_stats_chan.argtypes = [vp, DP, DP]
     L.dgpu_tke_chan.restype = i32
     L.dgpu_tke_chan.argtypes = [vp, DP]
+    L.dgpu_les_chan.restype = i32
+    L.dgpu_les_chan.argtypes = [vp, DP]
     L.dgpu_get_flow_physical.restype = i32
     L.dgpu_get_flow_physical.argtypes = [vp, i32, DP]
     L.dgpu_set_flow_physical.restype = i32
@@ -295,6 +297,13 @@ class GpuChannel:
         """tkebudget_chan's epsilon at J = 1..NY from the device state."""
         out = np.zeros(self.cfg.ny)
         self._chk(self.L.dgpu_tke_chan(self.h, _dp(out)), "dgpu_tke_chan")
+        return out
+
+    def les_chan(self) -> np.ndarray:
+        """les_chan's constant Smagorinsky stress divergence added to CF1-3
+        (and the velocity BC rows set); returns NU_T in field layout."""
+        out = np.zeros(self.cfg.field_shape())
+        self._chk(self.L.dgpu_les_chan(self.h, _dp(out)), "dgpu_les_chan")
         return out
 
     def get_flow_physical(self, fid: int) -> np.ndarray:

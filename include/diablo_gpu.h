@@ -218,6 +218,19 @@ int32_t dgpu_stats_chan(dgpu_ctx *ctx, double *profiles, double *ubulk);
  * also adds terms from a scratch plane it never sets, which are left out. */
 int32_t dgpu_tke_chan(dgpu_ctx *ctx, double *epsilon);
 
+/* les_chan with LES_MODEL_TYPE = 1, the constant Smagorinsky model
+ * (for/les.f:1-331, called from RK_CHAN_1 at for/channel.f:303-307), in FP64
+ * on the device state at NPROCY = 1: APPLY_BC_VEL and APPLY_BC_LES
+ * (for/les.f:805-864) on CU1-CU3, then CF1, CF3 -= d tau_1j/dx_j, d tau_3j/dx_j
+ * on planes JSTART..NY-1 and CF2 -= d tau_2j/dx_j on planes 2..NY with
+ * tau_ij = -2 C^2 DELTA^2 |S| S_ij (C = 0.13, DELTA^3 = DX DY 2 DZ), the
+ * implicitly treated d<U>/dy parts left out of tau_12, tau_23, and the
+ * explicit cross terms d(NU_T dU_i/dy)/dx_i added to CF2 (for/les.f:302-331).
+ * nu_t (may be NULL): NU_T in physical space, field layout (NY+2) x (NZ+2) x
+ * (NX+2), zero on planes 0, 1, 2, NY, NY+1 (GHOST_LES_MPI).  The reference's
+ * implicit NU_T terms in the Crank-Nicolson solves are not part of this call. */
+int32_t dgpu_les_chan(dgpu_ctx *ctx, double *nu_t);
+
 /* Kernel timing: when enabled, every kernel launch is bracketed by HIP events
  * on the context's stream.  dgpu_kernel_times returns, per kernel kind
  * (0 zinv, 1 xfused, 2 zfwd_nl, 3 cn, 4 remdiv), the summed milliseconds and the

@@ -1,0 +1,118 @@
+"""les_chan's constant Smagorinsky model on the device (dgpu_les_chan)
+against the oracle: NU_T, the SGS force added to CF1-3 and the velocity BC
+rows it sets, on a wall-bounded channel, a free-slip shear layer and the
+deck with Neumann and moving-wall conditions; at 192x129x192 the model's
+homogeneity (NU_T linear, the force quadratic in the velocity)."""
+import numpy as np
+import pytest
+
+from diablo_amd import case
+from gpu_util import compare_states, kitchen_sink
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "channel": lambda: case.channel_case(48, 33, 36, kick=0.3, streak=0.1),
+    "kh": lambda: case.kh_case(32, 33, 24, kick=0.05),
+    "sink": lambda: kitchen_sink(48, 25, 36),
+}
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def gpu_mod():
+    from diablo_amd import gpu
+    return gpu
+
+
+def _cf(cfg, st):
+    return [case.kept(cfg, st[k]).copy() for k in ("CF1", "CF2", "CF3")]
+
+
+@pytest.mark.parametrize("maker", list(CASES))
+def test_les_chan_matches_oracle(oracle_mod, gpu_mod, maker):
+    cfg = CASES[maker]()
+    say("les_chan", maker, cfg.nx, cfg.ny, cfg.nz)
+    o = oracle_mod.Oracle(cfg)
+    o.initialize(*case.initial_physical(cfg))
+    o.step()
+    o.step()
+    g = gpu_mod.GpuChannel(cfg)
+    g.load_state(o.state())
+    before = _cf(cfg, o.state())
+    nut_o = o.les_chan()
+    nut_g = g.les_chan()
+    scale = np.abs(nut_o).max()
+    assert scale > 0.0
+    err = np.abs(nut_g - nut_o).max() / scale
+    say("NU_T", err)
+    assert err < 1e-12, err
+    ny = cfg.ny
+    for j in (0, 1, 2, ny, ny + 1):
+        assert np.abs(nut_g[j]).max() == 0.0, j
+    assert np.abs(nut_g[:, cfg.nz:, :]).max() == 0.0 and np.abs(nut_g[:, :, cfg.nx:]).max() == 0.0
+    so, sg = o.state(), g.state()
+    for q, (b, ao, ag) in enumerate(zip(before, _cf(cfg, so), _cf(cfg, sg))):
+        d_o, d_g = ao - b, ag - b
+        ref = np.abs(d_o).max()
+        assert ref > 0.0, q
+        e = np.abs(d_g - d_o).max() / ref
+        say(f"dCF{q + 1}", e)
+        assert e < 1e-11, (q, e)
+    errs, bad = compare_states(cfg, sg, so, 1e-13, fields=("CU1", "CU2", "CU3"))
+    say(errs)
+    assert not bad, bad
+
+
+def test_les_chan_mixed_context(oracle_mod, gpu_mod):
+    """A mixed-precision context evaluates the model on its FP64 master state:
+    the same result as an FP64 context loaded alike."""
+    cfg = CASES["channel"]()
+    o = oracle_mod.Oracle(cfg)
+    o.initialize(*case.initial_physical(cfg))
+    o.step()
+    g = gpu_mod.GpuChannel(cfg)
+    gm = gpu_mod.GpuChannel(cfg, precision="mixed")
+    g.load_state(o.state())
+    gm.load_state(o.state())
+    say("les_chan mixed context")
+    n64, nm = g.les_chan(), gm.les_chan()
+    assert np.array_equal(n64, nm)
+    a, b = g.state(), gm.state()
+    for k in ("CF1", "CF2", "CF3", "CU1", "CU3"):
+        assert np.array_equal(case.kept(cfg, a[k]), case.kept(cfg, b[k])), k
+
+
+def test_les_chan_homogeneity_large(gpu_mod):
+    """At 192x129x192 after two steps: scaling the velocity by 2 scales NU_T by
+    2 and the SGS force by 4 (the model is |S| S_ij); both calls leave every
+    other field alone."""
+    cfg = case.channel_case(192, 129, 192, kick=0.3, streak=0.1)
+    say("les_chan homogeneity", cfg.nx, cfg.ny, cfg.nz)
+    g = gpu_mod.GpuChannel(cfg)
+    g.initialize(*case.initial_physical(cfg))
+    g.step(2)
+    g.synchronize()
+    st = g.state()
+    h = gpu_mod.GpuChannel(cfg)
+    st2 = dict(st)
+    for k in ("CU1", "CU2", "CU3"):
+        st2[k] = 2.0 * st[k]
+    h.load_state(st2)
+    n1, n2 = g.les_chan(), h.les_chan()
+    e = np.abs(n2 - 2.0 * n1).max() / np.abs(n1).max()
+    say("NU_T homogeneity", e)
+    assert e < 1e-12, e
+    a, b = g.state(), h.state()
+    for k in ("CF1", "CF2", "CF3"):
+        d1 = case.kept(cfg, a[k]) - case.kept(cfg, st[k])
+        d2 = case.kept(cfg, b[k]) - case.kept(cfg, st2[k])
+        e = np.abs(d2 - 4.0 * d1).max() / np.abs(d1).max()
+        say(k, e)
+        assert e < 1e-11, (k, e)
+    for k in ("CP", "CTH0"):
+        if k in st:
+            assert np.array_equal(a[k], st[k]), k
