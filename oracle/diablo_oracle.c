@@ -164,6 +164,7 @@ struct dor_state {
     int JSTART, JEND, JSTART_TH[DOR_MAX_TH], JEND_TH[DOR_MAX_TH];
     double TIME;
     double H_BAR[4], BETA_BAR[4], ZETA_BAR[4]; /* 1-based */
+    int les, les_model;                        /* LES, and les_chan called this substep */
     int courant_pending;                       /* COURANT at the next RK_STEP 3 */
     double courant_cfl;
     double *GX, *GZ, *GY, *GYF, *DY, *DYF;     /* 0:N+1 */
@@ -359,6 +360,16 @@ double dor_courant(dor_state *s, double cfl)
     else DELTA_T = dt;
     dor_set_dt(s, DELTA_T);
     return DELTA_T;
+}
+
+void dor_set_les(dor_state *s, int les, int model)
+{
+    s->les = les != 0;
+    s->les_model = s->les && model != 0;
+    if (s->les && !s->fld[DOR_NUT]) {
+        s->fld[DOR_NUT] = zalloc(s->fsize);
+        for (int n = 0; n < s->NTH; n++) s->fld[DOR_KAPPA0 + n] = zalloc(s->fsize);
+    }
 }
 
 void dor_set_threads(int n)
@@ -691,9 +702,55 @@ static void bc_th_upper(const dor_state *s, int n, double *L, double *D, double 
 
 enum { SYS_TH, SYS_U2, SYS_U1, SYS_U3 };
 
+/* APPLY_BC_VEL_PHYS_LOWER / _UPPER (for/channel.f:2495-2640) on the physical
+ * U1..U3 arrays. */
+static void apply_bc_vel_phys(dor_state *s)
+{
+    const int NY = s->NY, NZP = s->NZP, NXM = s->NXM;
+    const dor_params *p = &s->p;
+    const double *DY = s->DY, *DYF = s->DYF;
+    double *U1 = R(DOR_U1), *U2 = R(DOR_U2), *U3 = R(DOR_U3);
+    for (int K = 0; K <= NZP - 1; K++)
+        for (int i = 0; i <= NXM; i++) {
+#define AT2(A, J) A[RIDX(i, K, (J))]
+            for (int c = 0; c < 2; c++) {
+                double *UU = c == 0 ? U1 : U3;
+                const int lo = c == 0 ? p->u_bc_ymin : p->w_bc_ymin, hi = c == 0 ? p->u_bc_ymax : p->w_bc_ymax;
+                const double clo = c == 0 ? p->u_bc_ymin_c1 : p->w_bc_ymin_c1;
+                const double chi = c == 0 ? p->u_bc_ymax_c1 : p->w_bc_ymax_c1;
+                if (lo == 0) {
+                    AT2(UU, 0) = clo;
+                    AT2(UU, 1) = clo;
+                } else {
+                    AT2(UU, 1) = AT2(UU, 2) - DY[2] * clo;
+                    AT2(UU, 0) = AT2(UU, 1) - DY[1] * clo;
+                }
+                if (hi == 0) {
+                    AT2(UU, NY) = chi;
+                    AT2(UU, NY + 1) = chi;
+                } else {
+                    AT2(UU, NY) = AT2(UU, NY - 1) + DY[NY] * chi;
+                    AT2(UU, NY + 1) = AT2(UU, NY) + DY[NY] * chi;
+                }
+            }
+            if (p->v_bc_ymin == 0) {
+                AT2(U2, 1) = 2.0 * p->v_bc_ymin_c1 - AT2(U2, 2);
+                AT2(U2, 0) = AT2(U2, 1);
+            } else {
+                AT2(U2, 1) = AT2(U2, 2) - DYF[1] * p->v_bc_ymin_c1;
+                AT2(U2, 0) = AT2(U2, 1) - DYF[1] * p->v_bc_ymin_c1;
+            }
+            if (p->v_bc_ymax == 0)
+                AT2(U2, NY + 1) = 2.0 * p->v_bc_ymax_c1 - AT2(U2, NY);
+            else
+                AT2(U2, NY + 1) = AT2(U2, NY) + DY[NY] * p->v_bc_ymax_c1;
+#undef AT2
+        }
+}
+
 /* One implicit CN solve of a physical field, looping over K planes
  * (for/channel.f:771-833, 842-901, 907-966, 969-1029). */
-static void cn_solve_phys(dor_state *s, int sys, int n, double TEMP1)
+static void cn_solve_phys(dor_state *s, int sys, int n, double TEMP1, double TEMP2)
 {
     const int NX = s->NX, NY = s->NY, NZP = s->NZP, nx = NX;
     const double NUV = s->p.nu_v_scale;
@@ -738,6 +795,25 @@ static void cn_solve_phys(dor_state *s, int sys, int n, double TEMP1)
                         MT(U, i, J) = -TEMP1 * NUV / (DY[J + 1] * DYF[J]);
                     }
                     MT(V, i, J) = rhs[RIDX(i, K, J)];
+                    if (s->les) {
+                        /* the eddy viscosity / diffusivity, implicitly (for/channel.f:793-806,
+                         * 862-874, 927-939, 991-1003); NU_T and KAPPA_T live on GY */
+                        if (sys == SYS_U2) {
+                            const double *NT = R(DOR_NUT);
+                            const double nm = NT[RIDX(i, K, J)] + NT[RIDX(i, K, J - 1)];
+                            const double np = NT[RIDX(i, K, J)] + NT[RIDX(i, K, J + 1)];
+                            MT(L, i, J) = MT(L, i, J) - TEMP2 * 0.5 * nm / (DYF[J - 1] * DY[J]);
+                            MT(D, i, J) = MT(D, i, J) + TEMP2 * 0.5 * np / (DYF[J] * DY[J])
+                                          + TEMP2 * 0.5 * nm / (DYF[J - 1] * DY[J]);
+                            MT(U, i, J) = MT(U, i, J) - TEMP2 * 0.5 * np / (DYF[J] * DY[J]);
+                        } else {
+                            const double *NT = (sys == SYS_TH) ? R(DOR_KAPPA0 + n) : R(DOR_NUT);
+                            const double n0 = NT[RIDX(i, K, J)], n1 = NT[RIDX(i, K, J + 1)];
+                            MT(L, i, J) = MT(L, i, J) - TEMP2 * n0 / (DY[J] * DYF[J]);
+                            MT(D, i, J) = MT(D, i, J) + TEMP2 * n1 / (DY[J + 1] * DYF[J]) + TEMP2 * n0 / (DY[J] * DYF[J]);
+                            MT(U, i, J) = MT(U, i, J) - TEMP2 * n1 / (DY[J + 1] * DYF[J]);
+                        }
+                    }
                 }
             if (sys == SYS_TH) {
                 bc_th_lower(s, n, L, D, U, V);
@@ -771,7 +847,6 @@ void dor_rk_chan_1(dor_state *s, int RK_STEP)
     const double TEMP3 = s->ZETA_BAR[RK_STEP] * s->H_BAR[RK_STEP];
     const double TEMP4 = s->H_BAR[RK_STEP];
     const double TEMP5 = s->BETA_BAR[RK_STEP] * s->H_BAR[RK_STEP];
-    (void)TEMP2;
     cplx *CU1 = C(DOR_U1), *CU2 = C(DOR_U2), *CU3 = C(DOR_U3), *CP = C(DOR_P);
     cplx *CR1 = C(DOR_R1), *CR2 = C(DOR_R2), *CR3 = C(DOR_R3);
     cplx *CF1 = C(DOR_F1), *CF2 = C(DOR_F2), *CF3 = C(DOR_F3), *CS1 = C(DOR_S1);
@@ -906,10 +981,40 @@ void dor_rk_chan_1(dor_state *s, int RK_STEP)
     }
     /* USER_RHS_CHAN_FOURIER is an empty hook (for/user_rhs.f). */
 
-    /* To physical space (for/channel.f:389-399). */
+    /* To physical space (for/channel.f:303-399).  With the subgrid model on,
+     * les_chan sets the velocity BCs and adds the SGS force to CF1-3 first,
+     * then the physical-space BCs are applied, KAPPA_T = NU_T, and the
+     * scalars get their horizontal eddy-diffusion terms explicitly. */
+    if (s->les_model) dor_les_chan(s, R(DOR_NUT));
     dor_fft_xz_to_physical(s, U1, 0, NY + 1);
     dor_fft_xz_to_physical(s, U2, 0, NY + 1);
     dor_fft_xz_to_physical(s, U3, 0, NY + 1);
+    if (s->les_model) {
+        apply_bc_vel_phys(s);
+        for (int n = 0; n < s->NTH; n++) {
+            double *KT = R(DOR_KAPPA0 + n);
+            const double *NT = R(DOR_NUT);
+            cplx *CTH = C(DOR_CTH0 + n), *CFTH = C(DOR_CFTH0 + n);
+            for (int J = 1; J <= NY + 1; J++)
+                for (int K = 0; K <= NZP - 1; K++)
+                    for (int i = 0; i <= NXM; i++) KT[RIDX(i, K, J)] = 1.0 * NT[RIDX(i, K, J)];
+            for (int dir = 0; dir < 2; dir++) {
+                for (int J = 1; J <= NY + 1; J++)
+                    for (int K = 0; K <= TNKZ; K++)
+                        for (int i = 0; i <= NXP - 1; i++)
+                            CS1[CIDX(i, K, J)] = (dir == 0 ? CIKX(i) : CIKZ(K)) * CTH[CIDX(i, K, J)];
+                dor_fft_xz_to_physical(s, S1, 0, NY + 1);
+                for (int J = 1; J <= NY + 1; J++)
+                    for (int K = 0; K <= NZP - 1; K++)
+                        for (int i = 0; i <= NXM; i++) S1[RIDX(i, K, J)] = KT[RIDX(i, K, J)] * S1[RIDX(i, K, J)];
+                dor_fft_xz_to_fourier(s, S1, 0, NY + 1);
+                for (int J = s->JSTART_TH[n]; J <= s->JEND_TH[n]; J++)
+                    for (int K = 0; K <= TNKZ; K++)
+                        for (int i = 0; i <= NXP - 1; i++)
+                            CFTH[CIDX(i, K, J)] = CFTH[CIDX(i, K, J)] + (dir == 0 ? CIKX(i) : CIKZ(K)) * CS1[CIDX(i, K, J)];
+            }
+        }
+    }
     for (int n = 0; n < s->NTH; n++) {
         memcpy(S1, R(DOR_CTH0 + n), sizeof(double) * s->fsize);
         dor_fft_xz_to_physical(s, S1, 0, NY + 1);
@@ -996,6 +1101,26 @@ void dor_rk_chan_1(dor_state *s, int RK_STEP)
             for (int i = 0; i <= NXM; i++)
                 AT(R2, 0) = AT(R2, 0) + TEMP1 * NUV
                     * (((AT(U2, 1) - AT(U2, 0)) / DYF[J] - (AT(U2, 0) - AT(U2, -1)) / DYF[J - 1]) / DY[J]);
+    if (s->les) {
+        /* the eddy viscosity, explicit half (for/channel.f:628-659) */
+        const double *NT = R(DOR_NUT);
+#pragma omp parallel for schedule(static)
+        for (int J = JSTART; J <= JEND; J++)
+            for (int K = 0; K <= NZP - 1; K++)
+                for (int i = 0; i <= NXM; i++) {
+                    AT(R1, 0) = AT(R1, 0) + TEMP2 * ((AT(NT, 1) * (AT(U1, 1) - AT(U1, 0)) / DY[J + 1]
+                                                      - AT(NT, 0) * (AT(U1, 0) - AT(U1, -1)) / DY[J]) / DYF[J]);
+                    AT(R3, 0) = AT(R3, 0) + TEMP2 * ((AT(NT, 1) * (AT(U3, 1) - AT(U3, 0)) / DY[J + 1]
+                                                      - AT(NT, 0) * (AT(U3, 0) - AT(U3, -1)) / DY[J]) / DYF[J]);
+                }
+#pragma omp parallel for schedule(static)
+        for (int J = 2; J <= NY; J++)
+            for (int K = 0; K <= NZP - 1; K++)
+                for (int i = 0; i <= NXM; i++)
+                    AT(R2, 0) = AT(R2, 0) + TEMP2 * ((0.5 * (AT(NT, 0) + AT(NT, 1)) * (AT(U2, 1) - AT(U2, 0)) / DYF[J]
+                                                      - 0.5 * (AT(NT, 0) + AT(NT, -1)) * (AT(U2, 0) - AT(U2, -1)) / DYF[J - 1])
+                                                     / DY[J]);
+    }
 
     /* Scalars (for/channel.f:665-836). */
     for (int n = 0; n < s->NTH; n++) {
@@ -1024,13 +1149,23 @@ void dor_rk_chan_1(dor_state *s, int RK_STEP)
                 for (int i = 0; i <= NXM; i++)
                     AT(RTH, 0) = AT(RTH, 0) + (TEMP1 / PR) * NUV
                         * (((AT(TH, 1) - AT(TH, 0)) / DY[J + 1] - (AT(TH, 0) - AT(TH, -1)) / DY[J]) / DYF[J]);
+        if (s->les) {
+            /* the eddy diffusivity, explicit half (for/channel.f:748-760) */
+            const double *KT = R(DOR_KAPPA0 + n);
+#pragma omp parallel for schedule(static)
+            for (int J = JS; J <= JE; J++)
+                for (int K = 0; K <= NZP - 1; K++)
+                    for (int i = 0; i <= NXM; i++)
+                        AT(RTH, 0) = AT(RTH, 0) + TEMP2 * ((AT(KT, 1) * (AT(TH, 1) - AT(TH, 0)) / DY[J + 1]
+                                                            - AT(KT, 0) * (AT(TH, 0) - AT(TH, -1)) / DY[J]) / DYF[J]);
+        }
         /* Solve the scalar before velocity (for/channel.f:762-833). */
-        cn_solve_phys(s, SYS_TH, n, TEMP1);
+        cn_solve_phys(s, SYS_TH, n, TEMP1, TEMP2);
     }
 
-    cn_solve_phys(s, SYS_U2, 0, TEMP1);
-    cn_solve_phys(s, SYS_U1, 0, TEMP1);
-    cn_solve_phys(s, SYS_U3, 0, TEMP1);
+    cn_solve_phys(s, SYS_U2, 0, TEMP1, TEMP2);
+    cn_solve_phys(s, SYS_U1, 0, TEMP1, TEMP2);
+    cn_solve_phys(s, SYS_U3, 0, TEMP1, TEMP2);
     /* VARIABLE_DT: COURANT on the physical velocity after the implicit solves,
      * before the projection (for/channel.f:1037-1040); the caller asks for it
      * when MOD(TIME_STEP, UPDATE_DT) = 0.  TEMP1..TEMP5 keep this step's dt. */

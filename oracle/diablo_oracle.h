@@ -42,7 +42,9 @@ typedef struct dor_params {
 enum {
     DOR_U1 = 0, DOR_U2, DOR_U3, DOR_P, DOR_R1, DOR_R2, DOR_R3,
     DOR_F1, DOR_F2, DOR_F3, DOR_S1,
+    DOR_NUT = 11,      /* NU_T (physical, LES) */
     DOR_TH0 = 16,      /* TH(:,:,:,n)   = DOR_TH0   + n (0-based n) */
+    DOR_KAPPA0 = 20,   /* KAPPA_T(:,:,:,n) (physical, LES) */
     DOR_FTH0 = 24,     /* FTH, unused by the channel path but kept   */
     DOR_RTH0 = 32,     /* RTH(:,:,:,n)  */
     DOR_CTH0 = 40,     /* CTH  (Fourier, not aliased with TH)        */
@@ -78,6 +80,14 @@ double dor_get_dt(const dor_state *s);
  * (for/channel.f:1037-1040), on the velocity before the projection. */
 double dor_courant(dor_state *s, double cfl);
 void dor_courant_request(dor_state *s, double cfl);
+
+/* LES switches of RK_CHAN_1: les = input.dat's LES (the eddy viscosity and
+ * diffusivity enter the Crank-Nicolson terms, for/channel.f:628-659, 748-760,
+ * 793-806, 862-874, 927-939, 991-1003); model = the branch condition of
+ * for/channel.f:303 (les_chan is called this substep, then KAPPA_T = NU_T and
+ * the scalars' horizontal eddy-diffusion terms, for/channel.f:303-375).
+ * NU_T and KAPPA_T start at zero. */
+void dor_set_les(dor_state *s, int les, int model);
 
 /* The reference call surface (for/mpi.f:1530-1674, for/channel.f:62-1453). */
 void dor_fft_xz_to_fourier(dor_state *s, double *v, int jmin, int jmax);

@@ -90,6 +90,7 @@ def lib():
         L.dor_stats_chan.argtypes = [vp, dp, dp]
         L.dor_tke_chan.argtypes = [vp, dp]
         L.dor_les_chan.argtypes = [vp, dp]
+        L.dor_set_les.argtypes = [vp, i, i]
         _lib = L
     return _lib
 
@@ -119,6 +120,7 @@ STATS = ("ume", "vme", "wme", "urms", "vrms", "wrms", "uv", "uw", "wv", "dudy", 
 # field ids (diablo_oracle.h)
 U1, U2, U3, P, R1, R2, R3, F1, F2, F3, S1 = range(11)
 TH0, FTH0, RTH0, CTH0, CFTH0, CRTH0 = 16, 24, 32, 40, 48, 56
+NUT, KAPPA0 = 11, 20
 GY, GYF, DY, DYF, KX, KZ, KX2, KZ2, GX, GZ = range(10)
 
 
@@ -192,6 +194,11 @@ class Oracle:
 
     def set_delta_t(self, dt: float):
         self.L.dor_set_dt(self.h, dt)
+
+    def set_les(self, les: bool, model: bool = True):
+        """RK_CHAN_1's LES switches: les = the eddy viscosity in the
+        Crank-Nicolson terms; model = les_chan called in each substep."""
+        self.L.dor_set_les(self.h, int(bool(les)), int(bool(model)))
 
     def courant(self, cfl: float) -> float:
         """COURANT on the physical U1..U3 arrays as they stand; returns DELTA_T."""
