@@ -89,7 +89,8 @@ struct dgpu_ctx {
     int courant_state = 0;
     double *d_vmax_part = nullptr, *d_vmax = nullptr;  // per-block, per-plane velocity maxima
     double *d_file = nullptr;  // one restart-file array, NX*NY*NZ
-    double *d_les = nullptr;   // les_chan: 14 Fortran-layout fields, then DELTA_YF, DELTA_Y
+    double *d_les = nullptr;   // LES work planes, NU_T, SGS force, filter widths
+    bool les = false, les_model = false;  // input.dat's LES; les_chan called in each substep
     // CN tables: [stage 0..2][sys][NY+2][kRow]
     double *d_cn = nullptr;
     std::vector<double> h_cn;
@@ -894,40 +895,65 @@ int32_t dgpu_tke_chan(dgpu_ctx *c, double *epsilon)
     return DGPU_OK;
 }
 
-// les_chan with LES_MODEL_TYPE = 1 (for/les.f:1-331) at NPROCY = 1: the
-// velocity BCs and APPLY_BC_LES on the state, S_ij and U1, U3 to physical
-// space, |S| and NU_T, tau_ij back to Fourier space and its divergence into
-// CF1-3, then the explicit cross terms NU_T dU/dy into CF2.
-int32_t dgpu_les_chan(dgpu_ctx *c, double *nu_t)
+// ---------------------------------------------------------------------------
+// LES (for/les.f, for/channel.f:303-386, 628-659, 748-1003)
+// ---------------------------------------------------------------------------
+// Work planes, Fortran layout (NX+2) x (NZ+2) x (NY+2) doubles each:
+//   0-7   S_ij and U1, U3 for the model; in the substep physical U1, U2, U3,
+//         R1, R2, R3 (0-5) and the Thomas scratch (6, 7)
+//   8-12  the stress parts to transform; in the substep CS1 (8), TH (9), RTH (10)
+//   13    NU_T (kept between substeps; KAPPA_T equals it)
+//   14-17 the SGS force for CF1, CF2, CF3, CFTH (kept modes)
+//   18    rows 1 and NY of CU1, CU3 before les_chan's BCs (kept modes)
+// then DELTA_YF, DELTA_Y.
+static const int kLesBufs = 19;
+
+static size_t les_n(const dgpu_ctx *c)
 {
-    if (!c) return DGPU_E_ARG;
-    if (int rc = to_master(c)) return rc;
-    c->master_dirty = true;
+    return (size_t)(c->g.nx + 2) * (c->g.nz + 2) * (c->g.ny + 2);
+}
+
+static double *les_buf(dgpu_ctx *c, int q) { return c->d_les + (size_t)q * les_n(c); }
+
+static int ensure_les(dgpu_ctx *c)
+{
+    if (c->d_les) return DGPU_OK;
     const dgpu::Geo &g = c->g;
     const int NY = g.ny;
-    const size_t n = (size_t)(g.nx + 2) * (g.nz + 2) * (NY + 2);
-    if (!c->d_les) {
-        HIPCHK(c, hipMalloc(&c->d_les, (14 * n + 2 * (size_t)(NY + 2)) * sizeof(double)));
-        // NU_T's padding and the unset rows of the work fields stay zero
-        HIPCHK(c, hipMemsetAsync(c->d_les, 0, 14 * n * sizeof(double), c->stream));
-        // filter widths with -2 C_SMAG^2 absorbed (for/les.f:200-223):
-        // C_SMAG = 0.13, beta_sgs = 1, DELTA = (DX DY 2 DZ)^(1/3)
-        const double C_SMAG = 0.13, beta_sgs = 1.0;
-        const double dx = c->p.lx / g.nx, dz = c->p.lz / g.nz;
-        std::vector<double> dl(2 * (size_t)(NY + 2), 0.0);
-        for (int j = 1; j <= NY + 1; ++j) {
-            dl[j] = -2.0 * std::pow(C_SMAG, 2.0) * std::pow(dx * beta_sgs * c->dyf[j] * 2.0 * dz * beta_sgs, 2.0 / 3.0);
-            dl[NY + 2 + j] =
-                -2.0 * std::pow(C_SMAG, 2.0) * std::pow(dx * beta_sgs * c->dy[j] * 2.0 * dz * beta_sgs, 2.0 / 3.0);
-        }
-        HIPCHK(c, hipMemcpyAsync(c->d_les + 14 * n, dl.data(), dl.size() * sizeof(double), hipMemcpyHostToDevice,
-                                 c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
+    const size_t n = les_n(c);
+    HIPCHK(c, hipMalloc(&c->d_les, (kLesBufs * n + 2 * (size_t)(NY + 2)) * sizeof(double)));
+    // NU_T's padding and the unset rows of the work planes stay zero
+    HIPCHK(c, hipMemsetAsync(c->d_les, 0, kLesBufs * n * sizeof(double), c->stream));
+    // filter widths with -2 C_SMAG^2 absorbed (for/les.f:200-223):
+    // C_SMAG = 0.13, beta_sgs = 1, DELTA = (DX DY 2 DZ)^(1/3)
+    const double C_SMAG = 0.13, beta_sgs = 1.0;
+    const double dx = c->p.lx / g.nx, dz = c->p.lz / g.nz;
+    std::vector<double> dl(2 * (size_t)(NY + 2), 0.0);
+    for (int j = 1; j <= NY + 1; ++j) {
+        dl[j] = -2.0 * std::pow(C_SMAG, 2.0) * std::pow(dx * beta_sgs * c->dyf[j] * 2.0 * dz * beta_sgs, 2.0 / 3.0);
+        dl[NY + 2 + j] =
+            -2.0 * std::pow(C_SMAG, 2.0) * std::pow(dx * beta_sgs * c->dy[j] * 2.0 * dz * beta_sgs, 2.0 / 3.0);
     }
-    const double *delta_yf = c->d_les + 14 * n, *delta_y = delta_yf + (NY + 2);
+    HIPCHK(c, hipMemcpyAsync(c->d_les + kLesBufs * n, dl.data(), dl.size() * sizeof(double), hipMemcpyHostToDevice,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return DGPU_OK;
+}
+
+// les_chan with LES_MODEL_TYPE = 1 (for/les.f:1-331) at NPROCY = 1: the
+// velocity BCs and APPLY_BC_LES on the state, S_ij and U1, U3 to physical
+// space, |S| and NU_T, tau_ij back to Fourier space and its divergence
+// subtracted from f1-f3 (the reference's CF1-3), then the explicit cross
+// terms NU_T dU/dy added to f2.
+static int les_model(dgpu_ctx *c, cx<double> *f1, cx<double> *f2, cx<double> *f3)
+{
+    const dgpu::Geo &g = c->g;
+    const int NY = g.ny;
+    const size_t n = les_n(c);
+    const double *delta_yf = c->d_les + kLesBufs * n, *delta_y = delta_yf + (NY + 2);
     dgpu::LesPtrs S, O;
-    for (int q = 0; q < 8; ++q) S.p[q] = c->d_les + (size_t)q * n;
-    for (int q = 0; q < 6; ++q) O.p[q] = c->d_les + (size_t)(8 + q) * n;
+    for (int q = 0; q < 8; ++q) S.p[q] = les_buf(c, q);
+    for (int q = 0; q < 6; ++q) O.p[q] = les_buf(c, 8 + q);
     O.p[6] = O.p[7] = nullptr;
     dgpu::launch_apply_bc_vel(c->stream, g, c->cu[0], c->cu[1], c->cu[2], c->d_codes, c->d_c1, c->d_dy, c->d_dyf);
     if (int rc = check_launch(c, "apply_bc_vel")) return rc;
@@ -943,18 +969,39 @@ int32_t dgpu_les_chan(dgpu_ctx *c, double *nu_t)
     for (int q = 0; q < 5; ++q)
         if (int rc = fort_to_fourier(c, O.p[q], 0, NY + 1)) return rc;
     // CF1, CF3 on J1 = JSTART .. J2 = NY-1 (the RANKY = NPROCY-1 bounds, for/les.f:46-50)
-    dgpu::launch_les_cf(c->stream, g, O, c->cf[0], c->cf[1], c->cf[2], c->d_kx, c->d_kz, c->d_dyf, delta_y, delta_yf,
-                        c->jstart, NY - 1);
+    dgpu::launch_les_cf(c->stream, g, O, f1, f2, f3, c->d_kx, c->d_kz, c->d_dyf, delta_y, delta_yf, c->jstart,
+                        NY - 1);
     if (int rc = check_launch(c, "les_cf")) return rc;
     for (int pass = 0; pass < 2; ++pass) {
         dgpu::launch_les_cross_src(c->stream, g, O.p[5], S.p[6 + pass], c->d_dy, O.p[0]);
         if (int rc = check_launch(c, "les_cross_src")) return rc;
         if (int rc = fort_to_fourier(c, O.p[0], 2, NY)) return rc;
-        dgpu::launch_les_cross_add(c->stream, g, O.p[0], c->cf[1], c->d_kx, c->d_kz, pass);
+        dgpu::launch_les_cross_add(c->stream, g, O.p[0], f2, c->d_kx, c->d_kz, pass);
         if (int rc = check_launch(c, "les_cross_add")) return rc;
     }
-    if (nu_t) HIPCHK(c, hipMemcpyAsync(nu_t, O.p[5], n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    return DGPU_OK;
+}
+
+int32_t dgpu_les_chan(dgpu_ctx *c, double *nu_t)
+{
+    if (!c) return DGPU_E_ARG;
+    if (int rc = to_master(c)) return rc;
+    c->master_dirty = true;
+    if (int rc = ensure_les(c)) return rc;
+    if (int rc = les_model(c, c->cf[0], c->cf[1], c->cf[2])) return rc;
+    if (nu_t) HIPCHK(c, hipMemcpyAsync(nu_t, les_buf(c, 13), les_n(c) * sizeof(double), hipMemcpyDeviceToHost,
+                                       c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return DGPU_OK;
+}
+
+int32_t dgpu_set_les(dgpu_ctx *c, int32_t les, int32_t model)
+{
+    if (!c) return DGPU_E_ARG;
+    if (les && c->mixed) return fail(c, DGPU_E_ARG, "LES runs in FP64 contexts only");
+    c->les = les != 0;
+    c->les_model = c->les && model != 0;
+    if (c->les) return ensure_les(c);
     return DGPU_OK;
 }
 
@@ -1283,6 +1330,160 @@ static int rk_enqueue(dgpu_ctx *c, int rk_step, double time, double temp1, doubl
     return DGPU_OK;
 }
 
+// RK_CHAN_1 with LES = .TRUE. (for/channel.f:62-1073): the explicit terms of
+// the fused path (with the SGS force when the model is called) give the
+// right-hand sides; the Crank-Nicolson solves run in physical space with the
+// eddy viscosity, then REM_DIV_CHAN and the CP update as in the DNS path.
+static int les_substep(dgpu_ctx *c, int rk_step, double time, double temp1, double temp3, double temp4,
+                       double temp5)
+{
+    const dgpu::Geo &g = c->g;
+    const int NY = g.ny;
+    const size_t n = les_n(c);
+    const int nin = 3 + g.nth;
+    const bool courant = rk_step == 3 && c->courant_state == 1;
+    const double temp2 = temp4 / 2.0;  // TEMP2 = H_BAR(RK_STEP) / 2
+    double *U[3] = {les_buf(c, 0), les_buf(c, 1), les_buf(c, 2)};
+    double *Rb[3] = {les_buf(c, 3), les_buf(c, 4), les_buf(c, 5)};
+    double *bm = les_buf(c, 6), *gw = les_buf(c, 7), *cs = les_buf(c, 8), *thp = les_buf(c, 9),
+           *rthp = les_buf(c, 10), *nut = les_buf(c, 13);
+    cx<double> *sgs[4];
+    for (int q = 0; q < 4; ++q) sgs[q] = reinterpret_cast<cx<double> *>(les_buf(c, 14 + q));
+    cx<double> *pre = reinterpret_cast<cx<double> *>(les_buf(c, 18));
+    const size_t fsb = sizeof(cx<double>) * (size_t)c->fs;
+    if (c->les_model) {
+        for (int q = 0; q < 4; ++q) HIPCHK(c, hipMemsetAsync(sgs[q], 0, fsb, c->stream));
+        for (int r = 0; r < 4; ++r)
+            HIPCHK(c, hipMemcpyAsync(pre + (size_t)r * g.ps, c->cu[r < 2 ? 0 : 2] + (size_t)(r % 2 ? NY : 1) * g.ps,
+                                     sizeof(cx<double>) * g.ps, hipMemcpyDeviceToDevice, c->stream));
+        if (int rc = les_model(c, sgs[0], sgs[1], sgs[2])) return rc;
+        // APPLY_BC_VEL_PHYS_MPI (for/channel.f:309-311)
+        dgpu::LesBc b;
+        b.u_lo = c->p.u_bc_ymin; b.u_hi = c->p.u_bc_ymax; b.w_lo = c->p.w_bc_ymin; b.w_hi = c->p.w_bc_ymax;
+        b.v_lo = c->p.v_bc_ymin; b.v_hi = c->p.v_bc_ymax;
+        b.u_lo_c = c->p.u_bc_ymin_c1; b.u_hi_c = c->p.u_bc_ymax_c1; b.w_lo_c = c->p.w_bc_ymin_c1;
+        b.w_hi_c = c->p.w_bc_ymax_c1; b.v_lo_c = c->p.v_bc_ymin_c1; b.v_hi_c = c->p.v_bc_ymax_c1;
+        dgpu::launch_les_bc_phys(c->stream, g, c->cu[0], c->cu[1], c->cu[2], b, c->d_dy, c->d_dyf);
+        if (int rc = check_launch(c, "les_bc_phys")) return rc;
+        // KAPPA_T = NU_T and the scalar's horizontal eddy diffusion, explicit
+        // (for/channel.f:313-375)
+        for (int nth = 0; nth < g.nth; ++nth)
+            for (int dir = 1; dir <= 2; ++dir) {
+                dgpu::launch_les_pack(c->stream, g, c->cth[nth], cs, c->d_kx, c->d_kz, dir, 1, NY + 1);
+                if (int rc = check_launch(c, "les_pack")) return rc;
+                if (int rc = fort_to_physical(c, cs, 0, NY + 1)) return rc;
+                dgpu::launch_les_mul(c->stream, g, cs, nut, 1, NY + 1);
+                if (int rc = check_launch(c, "les_mul")) return rc;
+                if (int rc = fort_to_fourier(c, cs, 0, NY + 1)) return rc;
+                dgpu::launch_les_acc(c->stream, g, sgs[3], cs, c->d_kx, c->d_kz, dir, c->jstart_th[nth],
+                                     c->jend_th[nth]);
+                if (int rc = check_launch(c, "les_acc")) return rc;
+            }
+    }
+    // nonlinear terms (the fused transforms of the DNS path, FP64)
+    int jlo = 2;
+    for (int q = 0; q < g.nth; ++q) jlo = std::min(jlo, c->jstart_th[q]);
+    const long zps = (long)g.nz * c->ld;
+    const cx<double> *twx = c->d_iptw, *twz = c->d_iptw + c->ipx.twn;
+    dgpu::launch_zinv_ip<double, double>(c->stream, g, c->ipz, twz, c->cu[0], c->fs, g.ps, c->d_zt, c->zfs, zps,
+                                         c->ld, 0, g.ny + 2, nin);
+    if (int rc = check_launch(c, "zinv")) return rc;
+    dgpu::launch_xfused_ip<double>(c->stream, g, c->ipx, twx, c->ld, c->d_zt, c->zfs, c->d_wt, c->zfs, c->d_dy,
+                                   c->d_dyf, c->d_kx, jlo, g.ny, c->xchunk);
+    if (int rc = check_launch(c, "xfused")) return rc;
+    dgpu::launch_zfwd_nl_ip<double, double>(c->stream, g, c->ipz, twz, c->d_wt, c->zfs, c->ld, c->d_nlptr, c->d_kz,
+                                            jlo, g.ny, nin);
+    if (int rc = check_launch(c, "zfwd_nl")) return rc;
+    // the SGS force joins the nonlinear terms (slots: x, y, z momentum, scalar)
+    if (c->les_model)
+        for (int q = 0; q < nin; ++q) {
+            dgpu::launch_les_add(c->stream, c->nl[q], sgs[q], c->fs);
+            if (int rc = check_launch(c, "les_add")) return rc;
+        }
+    // new CF and the right-hand sides (sweep A of the column kernel)
+    dgpu::CnArgs A;
+    make_cn_args(c, rk_step, time, temp3, temp4, temp5, A);
+    A.rhs_only = 1;
+    dgpu::CnPtrs<double, double> P;
+    fill_ptrs<double>(c, P, c->nl);
+    dgpu::launch_cn<double, double>(c->stream, A, P);
+    if (int rc = check_launch(c, "cn(rhs)")) return rc;
+    if (c->les_model) {
+        dgpu::LesCfFix f;
+        f.nu = c->p.nu; f.beta = c->p.beta; f.i_ro = c->p.i_ro;
+        f.drhodx = g.nth ? c->p.drhodx[0] : 0.0; f.drhodz = g.nth ? c->p.drhodz[0] : 0.0;
+        f.jstart = c->jstart; f.jend = c->jend;
+        f.jstart_th = g.nth ? c->jstart_th[0] : 1; f.jend_th = g.nth ? c->jend_th[0] : 0;
+        dgpu::launch_les_cf_rows(c->stream, g, pre, c->cu[0], c->cu[2], c->cf[0], c->cf[2],
+                                 g.nth ? c->cfth[0] : nullptr, c->d_kx2, c->d_kz2, f);
+        if (int rc = check_launch(c, "les_cf_rows")) return rc;
+    }
+    // physical right-hand sides and fields
+    for (int q = 0; q < 3; ++q) {
+        dgpu::launch_les_pack(c->stream, g, c->nl[q], Rb[q], c->d_kx, c->d_kz, 0, 0, NY + 1);
+        dgpu::launch_les_pack(c->stream, g, c->cu[q], U[q], c->d_kx, c->d_kz, 0, 0, NY + 1);
+        if (int rc = check_launch(c, "les_pack")) return rc;
+        if (int rc = fort_to_physical(c, Rb[q], 0, NY + 1)) return rc;
+        if (int rc = fort_to_physical(c, U[q], 0, NY + 1)) return rc;
+    }
+    if (g.nth) {
+        dgpu::launch_les_pack(c->stream, g, c->nl[3], rthp, c->d_kx, c->d_kz, 0, 0, NY + 1);
+        dgpu::launch_les_pack(c->stream, g, c->cth[0], thp, c->d_kx, c->d_kz, 0, 0, NY + 1);
+        if (int rc = check_launch(c, "les_pack")) return rc;
+        if (int rc = fort_to_physical(c, rthp, 0, NY + 1)) return rc;
+        if (int rc = fort_to_physical(c, thp, 0, NY + 1)) return rc;
+    }
+    // the eddy-viscosity halves, explicit (for/channel.f:628-659, 748-760)
+    dgpu::launch_les_explicit(c->stream, g, Rb[0], U[0], nut, c->d_dy, c->d_dyf, temp2, 0, c->jstart, c->jend);
+    dgpu::launch_les_explicit(c->stream, g, Rb[2], U[2], nut, c->d_dy, c->d_dyf, temp2, 0, c->jstart, c->jend);
+    dgpu::launch_les_explicit(c->stream, g, Rb[1], U[1], nut, c->d_dy, c->d_dyf, temp2, 1, 2, NY);
+    if (g.nth)
+        dgpu::launch_les_explicit(c->stream, g, rthp, thp, nut, c->d_dy, c->d_dyf, temp2, 0, c->jstart_th[0],
+                                  c->jend_th[0]);
+    if (int rc = check_launch(c, "les_explicit")) return rc;
+    // implicit solves: TH, U2, U1, U3 (for/channel.f:762-1029)
+    const double nuv = c->p.nu_v_scale;
+    auto solve = [&](int sys, int jl, int jh, int lo, int hi, double clo, double chi, double cmol, const double *rhs,
+                     double *out) {
+        dgpu::LesCn a;
+        a.sys = sys; a.ny = NY; a.jlo = jl; a.jhi = jh; a.clo = jl - 1; a.chi = jh + 1;
+        if (sys == dgpu::LES_SYS_U2) { a.clo = 1; a.chi = NY + 1; }
+        a.lo = lo; a.hi = hi; a.cmol = cmol; a.temp2 = temp2; a.clo_c = clo; a.chi_c = chi;
+        dgpu::launch_les_cn(c->stream, g, a, rhs, nut, c->d_dy, c->d_dyf, bm, gw, out);
+        return check_launch(c, "les_cn");
+    };
+    if (g.nth)
+        if (int rc = solve(dgpu::LES_SYS_TH, c->jstart_th[0], c->jend_th[0], c->p.th_bc_ymin[0], c->p.th_bc_ymax[0],
+                           c->p.th_bc_ymin_c1[0], c->p.th_bc_ymax_c1[0], temp1 / c->p.pr[0] * nuv, rthp, thp))
+            return rc;
+    if (int rc = solve(dgpu::LES_SYS_U2, 2, NY, c->p.v_bc_ymin, c->p.v_bc_ymax, c->p.v_bc_ymin_c1,
+                       c->p.v_bc_ymax_c1, temp1 * nuv, Rb[1], U[1]))
+        return rc;
+    if (int rc = solve(dgpu::LES_SYS_U13, c->jstart, c->jend, c->p.u_bc_ymin, c->p.u_bc_ymax, c->p.u_bc_ymin_c1,
+                       c->p.u_bc_ymax_c1, temp1 * nuv, Rb[0], U[0]))
+        return rc;
+    if (int rc = solve(dgpu::LES_SYS_U13, c->jstart, c->jend, c->p.w_bc_ymin, c->p.w_bc_ymax, c->p.w_bc_ymin_c1,
+                       c->p.w_bc_ymax_c1, temp1 * nuv, Rb[2], U[2]))
+        return rc;
+    // COURANT on the physical velocity after the solves (for/channel.f:1037-1040)
+    if (courant) {
+        dgpu::launch_les_vmax(c->stream, g, U[0], U[1], U[2], c->d_vmax);
+        if (int rc = check_launch(c, "les_vmax")) return rc;
+        c->courant_state = 2;
+    }
+    // back to Fourier space (for/channel.f:1043-1050)
+    for (int q = 0; q < 3 + g.nth; ++q) {
+        double *f = q < 3 ? U[q] : thp;
+        if (int rc = fort_to_fourier(c, f, 0, NY + 1)) return rc;
+        dgpu::launch_unpack_fortran(c->stream, g, f, q < 3 ? c->cu[q] : c->cth[0], 0, NY + 2);
+        if (int rc = check_launch(c, "unpack")) return rc;
+    }
+    dgpu::RdArgs R;
+    fill_rdargs(c, R, 1, temp4);
+    dgpu::launch_remdiv<double, double>(c->stream, R, P);
+    return check_launch(c, "remdiv");
+}
+
 int32_t dgpu_rk_chan_1(dgpu_ctx *c, int32_t rk_step, double time, const double *h_bar, const double *beta_bar,
                        const double *zeta_bar)
 {
@@ -1296,6 +1497,7 @@ int32_t dgpu_rk_chan_1(dgpu_ctx *c, int32_t rk_step, double time, const double *
     if (int rc = ensure_cn(c, k, temp1, temp4)) return rc;
     if (int rc = to_work(c)) return rc;
     c->work_dirty = c->f32state;
+    if (c->les) return les_substep(c, rk_step, time, temp1, temp3, temp4, temp5);
     // One captured graph per RK stage.  Not used while kernels are timed one
     // by one, or when the forcing depends on TIME (F_TYPE = 2).
     const bool graphable = c->use_graphs && !c->timing && c->p.f_type != 2 &&

@@ -103,6 +103,7 @@ __device__ __forceinline__ void cn_body(AT A, PT P, int bx)
     const cx<T> *const p_cu1 = P.cu1;
     const cx<T> *const p_cu3 = P.cu3;
     const cx<T> *const p_cth = P.cth;
+    const bool rhs_only = A.rhs_only != 0;
 
     // ---------------- sweep A ----------------
     // Inputs of plane j are loaded four planes ahead (piped_sweep); every
@@ -208,7 +209,7 @@ __device__ __forceinline__ void cn_body(AT A, PT P, int bx)
                     vec = make_cx<T>(cr.re + (wp * fp.re + w0 * f0.re + wm * fm.re),
                                      cr.im + (wp * fp.im + w0 * f0.im + wm * fm.im));
                 }
-                const cx<T> gj = (j == 0) ? vec : fma_c(mult, gprev, vec);
+                const cx<T> gj = (j == 0 || rhs_only) ? vec : fma_c(mult, gprev, vec);
                 if (cfr) cf[o] = cfn;
                 gs[o] = gj;
                 gprev = gj;
@@ -220,7 +221,7 @@ __device__ __forceinline__ void cn_body(AT A, PT P, int bx)
     }
     __syncthreads();
     // ---------------- sweep B ----------------
-    if (live) {
+    if (live && !rhs_only) {
         struct InB {
             cx<T> g;
             double ip, u, cpy;  // SIDE: fetched ahead

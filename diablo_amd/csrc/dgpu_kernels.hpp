@@ -51,6 +51,10 @@ struct CnArgs {
     // stride nl_ps.  The mixed path runs its FP64 low-|k| columns this way.
     const int *map;
     long nl_ps;
+    // 1: sweep A only, with no elimination: the solve scratch receives each
+    // row's right-hand side and the field is left as it was (the LES path
+    // solves in physical space)
+    int rhs_only;
 };
 
 template <typename T, typename NLT = T>
@@ -247,5 +251,37 @@ void launch_les_cross_src(hipStream_t s, const Geo &g, const double *nut, const 
                           double *out);
 void launch_les_cross_add(hipStream_t s, const Geo &g, const double *ct, cx<double> *cf2, const double *kx,
                           const double *kz, int use_kz);
+
+// RK_CHAN_1 with LES (dgpu_les.hip): the velocity and scalar Crank-Nicolson
+// solves in physical space with the eddy viscosity.
+struct LesBc {
+    int u_lo, u_hi, w_lo, w_hi, v_lo, v_hi;
+    double u_lo_c, u_hi_c, w_lo_c, w_hi_c, v_lo_c, v_hi_c;
+};
+enum { LES_SYS_U13 = 0, LES_SYS_U2 = 1, LES_SYS_TH = 2 };
+struct LesCn {
+    int sys, ny, jlo, jhi, clo, chi, lo, hi;  // rows, copied rows, BC codes
+    double cmol, temp2, clo_c, chi_c;         // molecular TEMP1*NU_V_SCALE (/PR), H_BAR/2, BC values
+};
+void launch_les_bc_phys(hipStream_t s, const Geo &g, cx<double> *cu1, cx<double> *cu2, cx<double> *cu3,
+                        const LesBc &b, const double *dy, const double *dyf);
+void launch_les_pack(hipStream_t s, const Geo &g, const cx<double> *src, double *fort, const double *kx,
+                     const double *kz, int kmul, int j0, int j1);
+void launch_les_mul(hipStream_t s, const Geo &g, double *a, const double *nut, int j0, int j1);
+void launch_les_acc(hipStream_t s, const Geo &g, cx<double> *dst, const double *fort, const double *kx,
+                    const double *kz, int kmul, int j0, int j1);
+void launch_les_explicit(hipStream_t s, const Geo &g, double *r, const double *u, const double *nut,
+                         const double *dy, const double *dyf, double temp2, int sys, int j0, int j1);
+void launch_les_cn(hipStream_t s, const Geo &g, const LesCn &a, const double *rhs, const double *nut,
+                   const double *dy, const double *dyf, double *bm, double *gw, double *out);
+void launch_les_vmax(hipStream_t s, const Geo &g, const double *u1, const double *u2, const double *u3, double *out);
+void launch_les_add(hipStream_t s, cx<double> *dst, const cx<double> *src, long n);
+struct LesCfFix {
+    double nu, beta, i_ro, drhodx, drhodz;
+    int jstart, jend, jstart_th, jend_th;
+};
+void launch_les_cf_rows(hipStream_t s, const Geo &g, const cx<double> *pre, const cx<double> *cu1,
+                        const cx<double> *cu3, cx<double> *cf1, cx<double> *cf3, cx<double> *cfth, const double *kx2,
+                        const double *kz2, const LesCfFix &f);
 
 }  // namespace dgpu

@@ -239,6 +239,377 @@ __global__ void k_les_cross_add(Geo g, const double *__restrict__ ct, cx<double>
     cf2[o] = f;
 }
 
+// ---------------------------------------------------------------------------
+// RK_CHAN_1 with LES: the Crank-Nicolson solves in physical space
+// ---------------------------------------------------------------------------
+
+// APPLY_BC_VEL_PHYS_LOWER / _UPPER (for/channel.f:2495-2640) stated on the
+// kept modes: a value C in physical space is C on the (0,0) mode and 0 on
+// the others, and the Neumann rows are linear in the field.
+__global__ void k_les_bc_phys(Geo g, cx<double> *__restrict__ cu1, cx<double> *__restrict__ cu2,
+                              cx<double> *__restrict__ cu3, LesBc b, const double *__restrict__ dy,
+                              const double *__restrict__ dyf)
+{
+    const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= g.ps) return;
+    const int NY = g.ny;
+    const long ps = g.ps;
+    const double on = (m == 0) ? 1.0 : 0.0;
+    auto R = [&](cx<double> *a, int j) -> cx<double> & { return a[m + (long)j * ps]; };
+    auto cst = [&](double c) { return make_cx<double>(on * c, 0.0); };
+    auto plus = [&](cx<double> a, double c) { return make_cx<double>(a.re + on * c, a.im); };
+    for (int q = 0; q < 2; ++q) {
+        cx<double> *u = q == 0 ? cu1 : cu3;
+        const int lo = q == 0 ? b.u_lo : b.w_lo, hi = q == 0 ? b.u_hi : b.w_hi;
+        const double clo = q == 0 ? b.u_lo_c : b.w_lo_c, chi = q == 0 ? b.u_hi_c : b.w_hi_c;
+        if (lo == 0) {
+            R(u, 0) = cst(clo);
+            R(u, 1) = cst(clo);
+        } else {
+            R(u, 1) = plus(R(u, 2), -dy[2] * clo);
+            R(u, 0) = plus(R(u, 1), -dy[1] * clo);
+        }
+        if (hi == 0) {
+            R(u, NY) = cst(chi);
+            R(u, NY + 1) = cst(chi);
+        } else {
+            R(u, NY) = plus(R(u, NY - 1), dy[NY] * chi);
+            R(u, NY + 1) = plus(R(u, NY), dy[NY] * chi);
+        }
+    }
+    if (b.v_lo == 0) {
+        const cx<double> u2 = R(cu2, 2);
+        R(cu2, 1) = make_cx<double>(2.0 * on * b.v_lo_c - u2.re, -u2.im);
+        R(cu2, 0) = R(cu2, 1);
+    } else {
+        R(cu2, 1) = plus(R(cu2, 2), -dyf[1] * b.v_lo_c);
+        R(cu2, 0) = plus(R(cu2, 1), -dyf[1] * b.v_lo_c);
+    }
+    if (b.v_hi == 0) {
+        const cx<double> u2 = R(cu2, NY);
+        R(cu2, NY + 1) = make_cx<double>(2.0 * on * b.v_hi_c - u2.re, -u2.im);
+    } else {
+        R(cu2, NY + 1) = plus(R(cu2, NY), dy[NY] * b.v_hi_c);
+    }
+}
+
+// Kept modes (plane stride ps) -> Fortran-layout Fourier planes, times i kx
+// (kmul 1) or i kz (kmul 2); planes outside j0..j1 and slots outside the
+// kept modes are zero.
+__global__ void k_les_pack(Geo g, const cx<double> *__restrict__ src, double *__restrict__ fort,
+                           const double *__restrict__ kxv, const double *__restrict__ kzv, int kmul, int j0, int j1)
+{
+    const int nxp = g.nx / 2, nz = g.nz;
+    const long plane = (long)(nxp + 1) * (nz + 2);
+    const int j = blockIdx.y;
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= plane) return;
+    const int K = (int)(e / (nxp + 1)), i = (int)(e - (long)K * (nxp + 1));
+    cx<double> v = make_cx<double>(0.0, 0.0);
+    if (i < g.mk && K < g.mz && j >= j0 && j <= j1) {
+        v = src[(long)K * g.mk + i + (long)j * g.ps];
+        if (kmul == 1) v = i_times(kxv[i], v);
+        else if (kmul == 2) v = i_times(kzv[K], v);
+    }
+    double *p = fort + 2 * ((long)j * plane + e);
+    p[0] = v.re;
+    p[1] = v.im;
+}
+
+// Physical planes j0..j1: a *= nut
+__global__ void k_les_mul(Geo g, double *__restrict__ a, const double *__restrict__ nut, int j0)
+{
+    const long n = (long)g.nx * g.nz;
+    const int j = j0 + blockIdx.y;
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const int i = (int)(e % g.nx), k = (int)(e / g.nx);
+    const long x = fidx3(i, k, j, g.nx, g.nz);
+    a[x] = nut[x] * a[x];
+}
+
+// dst (kept modes) += (i kx | i kz | 1) * Fortran-layout Fourier planes, j0..j1
+__global__ void k_les_acc(Geo g, cx<double> *__restrict__ dst, const double *__restrict__ fort,
+                          const double *__restrict__ kxv, const double *__restrict__ kzv, int kmul, int j0)
+{
+    const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = j0 + blockIdx.y;
+    if (m >= g.ps) return;
+    const int K = (int)(m / g.mk), i = (int)(m - (long)K * g.mk);
+    cx<double> v = reinterpret_cast<const cx<double> *>(fort)[cidx3(i, K, j, g.nx / 2, g.nz)];
+    if (kmul == 1) v = i_times(kxv[i], v);
+    else if (kmul == 2) v = i_times(kzv[K], v);
+    const long o = m + (long)j * g.ps;
+    cx<double> f = dst[o];
+    f = make_cx<double>(f.re + v.re, f.im + v.im);
+    dst[o] = f;
+}
+
+// The eddy-viscosity (diffusivity) half of the Crank-Nicolson term, explicit
+// (for/channel.f:628-659, 748-760): r += temp2 d/dy(nut du/dy) on rows
+// j0..j1 of system sys (0: on GYF rows with NU_T on the faces, U1, U3 and
+// TH; 1: U2 on GY rows, NU_T averaged onto the GYF faces).
+__global__ void k_les_explicit(Geo g, double *__restrict__ r, const double *__restrict__ u,
+                               const double *__restrict__ nut, const double *__restrict__ dy,
+                               const double *__restrict__ dyf, double temp2, int sys, int j0)
+{
+    const long n = (long)g.nx * g.nz;
+    const int j = j0 + blockIdx.y;
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const int i = (int)(e % g.nx), k = (int)(e / g.nx);
+    const long x = fidx3(i, k, j, g.nx, g.nz), xp = fidx3(i, k, j + 1, g.nx, g.nz), xm = fidx3(i, k, j - 1, g.nx, g.nz);
+    double t;
+    if (sys == 0)
+        t = (nut[xp] * (u[xp] - u[x]) / dy[j + 1] - nut[x] * (u[x] - u[xm]) / dy[j]) / dyf[j];
+    else
+        t = (0.5 * (nut[x] + nut[xp]) * (u[xp] - u[x]) / dyf[j] - 0.5 * (nut[x] + nut[xm]) * (u[x] - u[xm]) / dyf[j - 1]) /
+            dy[j];
+    r[x] = r[x] + temp2 * t;
+}
+
+// One implicit Crank-Nicolson solve in physical space per (x, z) column
+// (for/channel.f:762-1029 with the LES terms): rows jlo..jhi from the
+// molecular and eddy parts, the boundary rows of APPLY_BC_*_LOWER/UPPER
+// (for/channel.f:1660-2282), THOMAS_REAL at NPROCY = 1 (for/mpi.f:630-809),
+// and rows clo..chi written to out.  bm, gw: per-column scratch planes.
+__device__ __forceinline__ void les_row(const LesCn &a, const double *__restrict__ nut, long x, long xp, long xm,
+                                        const double *__restrict__ dy, const double *__restrict__ dyf,
+                                        const double *__restrict__ rhs, int j, double &L, double &D, double &U,
+                                        double &V)
+{
+    const int NY = a.ny;
+    L = 0.0; D = 1.0; U = 0.0; V = 0.0;
+    if (j >= a.jlo && j <= a.jhi) {
+        const double c = a.cmol;
+        if (a.sys == LES_SYS_TH) {
+            L = -c / (dy[j] * dyf[j]);
+            D = 1. + c / (dy[j + 1] * dyf[j]) + c / (dy[j] * dyf[j]);
+            U = -c / (dy[j + 1] * dyf[j]);
+        } else if (a.sys == LES_SYS_U2) {
+            L = -c / (dyf[j - 1] * dy[j]);
+            D = 1. + c / (dyf[j] * dy[j]) + c / (dyf[j - 1] * dy[j]);
+            U = -c / (dyf[j] * dy[j]);
+        } else {
+            L = -c / (dy[j] * dyf[j]);
+            D = 1. - c * (-1. / (dy[j + 1] * dyf[j]) - 1. / (dy[j] * dyf[j]));
+            U = -c / (dy[j + 1] * dyf[j]);
+        }
+        V = rhs[x];
+        const double t2 = a.temp2;
+        if (a.sys == LES_SYS_U2) {
+            const double nm = nut[x] + nut[xm], np = nut[x] + nut[xp];
+            L = L - t2 * 0.5 * nm / (dyf[j - 1] * dy[j]);
+            D = D + t2 * 0.5 * np / (dyf[j] * dy[j]) + t2 * 0.5 * nm / (dyf[j - 1] * dy[j]);
+            U = U - t2 * 0.5 * np / (dyf[j] * dy[j]);
+        } else {
+            const double n0 = nut[x], n1 = nut[xp];
+            L = L - t2 * n0 / (dy[j] * dyf[j]);
+            D = D + t2 * n1 / (dy[j + 1] * dyf[j]) + t2 * n0 / (dy[j] * dyf[j]);
+            U = U - t2 * n1 / (dy[j + 1] * dyf[j]);
+        }
+    }
+    auto set = [&](double l, double d, double u, double v) { L = l; D = d; U = u; V = v; };
+    if (a.sys == LES_SYS_U2) {
+        if (j == 0) set(0., 1., 0., 0.);
+        if (a.lo == 0) {
+            if (j == 1 || j == 2) set(0., 1., 0., a.clo_c);
+        } else if (j == 1) {
+            set(0., -1., 1., dyf[1] * a.clo_c);
+        }
+        if (a.hi == 0) {
+            if (j == NY || j == NY + 1) set(0., 1., 0., a.chi_c);
+        } else if (j == NY + 1) {
+            set(-1., 1., 0., dyf[NY] * a.chi_c);
+        }
+    } else {
+        if (j == 0) set(0., 1., 0., 0.);
+        if (j == 1) {
+            if (a.lo == 0) set(0., 1., 0., a.clo_c);
+            else set(0., -1., 1., dy[2] * a.clo_c);
+        }
+        if (j == NY + 1) set(0., 1., 0., 0.);
+        if (j == NY) {
+            if (a.hi == 0) set(0., 1., 0., a.chi_c);
+            else set(-1., 1., 0., dy[NY] * a.chi_c);
+        }
+    }
+}
+
+__global__ void k_les_cn(Geo g, LesCn a, const double *__restrict__ rhs, const double *__restrict__ nut,
+                         const double *__restrict__ dy, const double *__restrict__ dyf, double *__restrict__ bm,
+                         double *__restrict__ gw, double *__restrict__ out)
+{
+    const long n = (long)g.nx * g.nz;
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const int i = (int)(e % g.nx), k = (int)(e / g.nx);
+    const int NY = g.ny;
+    auto X = [&](int j) { return fidx3(i, k, j, g.nx, g.nz); };
+    // forward elimination, rows 1..NY+1 (THOMAS_FORWARD_REAL_MPI)
+    double L, D, U, V;
+    les_row(a, nut, X(0), X(1), X(0), dy, dyf, rhs, 0, L, D, U, V);
+    double bprev = D, cprev = U, gprev = V;
+    bm[X(0)] = D;
+    gw[X(0)] = V;
+    for (int j = 1; j <= NY + 1; ++j) {
+        const long x = X(j);
+        les_row(a, nut, x, X(j <= NY ? j + 1 : j), X(j - 1), dy, dyf, rhs, j, L, D, U, V);
+        const double m = -L / bprev;
+        D = D + m * cprev;
+        V = V + m * gprev;
+        bm[x] = D;
+        gw[x] = V;
+        bprev = D;
+        cprev = U;
+        gprev = V;
+    }
+    // back substitution with the zero-pivot guard (THOMAS_BACKWARD_REAL_MPI)
+    double xn = gprev / bprev;
+    if (NY + 1 >= a.clo && NY + 1 <= a.chi) out[X(NY + 1)] = xn;
+    for (int j = NY; j >= 0; --j) {
+        const long x = X(j);
+        les_row(a, nut, x, X(j + 1), X(j > 0 ? j - 1 : 0), dy, dyf, rhs, j, L, D, U, V);
+        const double b = bm[x];
+        xn = (b != 0.0) ? (gw[x] - U * xn) / b : 0.0;
+        if (j >= a.clo && j <= a.chi) out[x] = xn;
+    }
+}
+
+// Per-plane maxima of |U1|, |U2|, |U3| on physical planes 1..NY into
+// out[q * NY + j - 1] (COURANT's input, for/courant.f:25-45)
+__global__ void __launch_bounds__(256) k_les_vmax(Geo g, const double *__restrict__ u1, const double *__restrict__ u2,
+                                                  const double *__restrict__ u3, double *__restrict__ out)
+{
+    __shared__ double red[3][256];
+    const int j = 1 + blockIdx.x, tid = threadIdx.x;
+    const long n = (long)g.nx * g.nz;
+    double m[3] = {0.0, 0.0, 0.0};
+    for (long e = tid; e < n; e += 256) {
+        const int i = (int)(e % g.nx), k = (int)(e / g.nx);
+        const long x = fidx3(i, k, j, g.nx, g.nz);
+        m[0] = fmax(m[0], fabs(u1[x]));
+        m[1] = fmax(m[1], fabs(u2[x]));
+        m[2] = fmax(m[2], fabs(u3[x]));
+    }
+    for (int q = 0; q < 3; ++q) red[q][tid] = m[q];
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (tid < h)
+            for (int q = 0; q < 3; ++q) red[q][tid] = fmax(red[q][tid], red[q][tid + h]);
+        __syncthreads();
+    }
+    if (tid < 3) out[(long)tid * g.ny + (j - 1)] = red[tid][0];
+}
+
+// The reference forms CF from the velocity before les_chan resets its
+// boundary rows (for/channel.f:175-290 precede for/channel.f:303); the
+// column kernel saw the reset rows.  On rows 1 and NY (pre: the saved rows
+// of CU1 and CU3, plane 0 row 1, plane 1 row NY, then the same for CU3) the
+// linear terms that read them are corrected: horizontal viscosity and
+// Coriolis in CF1, CF3, the background-gradient terms in CFTH.
+__global__ void k_les_cf_rows(Geo g, const cx<double> *__restrict__ pre, const cx<double> *__restrict__ cu1,
+                              const cx<double> *__restrict__ cu3, cx<double> *__restrict__ cf1,
+                              cx<double> *__restrict__ cf3, cx<double> *__restrict__ cfth,
+                              const double *__restrict__ kx2, const double *__restrict__ kz2, LesCfFix f)
+{
+    const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= g.ps) return;
+    const int K = (int)(m / g.mk), i = (int)(m - (long)K * g.mk);
+    const long ps = g.ps;
+    const double k2 = kx2[i] + kz2[K];
+    const double visc = f.nu * ((f.beta == 1.0) ? k2 : pow(k2, f.beta));
+    for (int r = 0; r < 2; ++r) {
+        const int j = r == 0 ? 1 : g.ny;
+        const long o = m + (long)j * ps;
+        const cx<double> a = pre[m + (long)r * ps], b = pre[m + (long)(2 + r) * ps];
+        const cx<double> d1 = make_cx<double>(a.re - cu1[o].re, a.im - cu1[o].im);
+        const cx<double> d3 = make_cx<double>(b.re - cu3[o].re, b.im - cu3[o].im);
+        if (j >= f.jstart && j <= f.jend) {
+            cx<double> x = cf1[o], z = cf3[o];
+            x = make_cx<double>(x.re - visc * d1.re + f.i_ro * d3.re, x.im - visc * d1.im + f.i_ro * d3.im);
+            z = make_cx<double>(z.re - visc * d3.re - f.i_ro * d1.re, z.im - visc * d3.im - f.i_ro * d1.im);
+            cf1[o] = x;
+            cf3[o] = z;
+        }
+        if (cfth && j >= f.jstart_th && j <= f.jend_th) {
+            cx<double> t = cfth[o];
+            t = make_cx<double>(t.re - (d1.re * f.drhodx + d3.re * f.drhodz), t.im - (d1.im * f.drhodx + d3.im * f.drhodz));
+            cfth[o] = t;
+        }
+    }
+}
+
+void launch_les_cf_rows(hipStream_t s, const Geo &g, const cx<double> *pre, const cx<double> *cu1,
+                        const cx<double> *cu3, cx<double> *cf1, cx<double> *cf3, cx<double> *cfth, const double *kx2,
+                        const double *kz2, const LesCfFix &f)
+{
+    hipLaunchKernelGGL(k_les_cf_rows, dim3((unsigned)((g.ps + 255) / 256)), dim3(256), 0, s, g, pre, cu1, cu3, cf1,
+                       cf3, cfth, kx2, kz2, f);
+}
+
+__global__ void k_les_add(cx<double> *__restrict__ dst, const cx<double> *__restrict__ src, long n)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const cx<double> a = dst[e], b = src[e];
+    dst[e] = make_cx<double>(a.re + b.re, a.im + b.im);
+}
+
+void launch_les_add(hipStream_t s, cx<double> *dst, const cx<double> *src, long n)
+{
+    hipLaunchKernelGGL(k_les_add, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dst, src, n);
+}
+
+void launch_les_bc_phys(hipStream_t s, const Geo &g, cx<double> *cu1, cx<double> *cu2, cx<double> *cu3,
+                        const LesBc &b, const double *dy, const double *dyf)
+{
+    hipLaunchKernelGGL(k_les_bc_phys, dim3((unsigned)((g.ps + 255) / 256)), dim3(256), 0, s, g, cu1, cu2, cu3, b, dy,
+                       dyf);
+}
+
+void launch_les_pack(hipStream_t s, const Geo &g, const cx<double> *src, double *fort, const double *kx,
+                     const double *kz, int kmul, int j0, int j1)
+{
+    const long plane = (long)(g.nx / 2 + 1) * (g.nz + 2);
+    hipLaunchKernelGGL(k_les_pack, dim3((unsigned)((plane + 255) / 256), g.ny + 2), dim3(256), 0, s, g, src, fort,
+                       kx, kz, kmul, j0, j1);
+}
+
+void launch_les_mul(hipStream_t s, const Geo &g, double *a, const double *nut, int j0, int j1)
+{
+    const long n = (long)g.nx * g.nz;
+    hipLaunchKernelGGL(k_les_mul, dim3((unsigned)((n + 255) / 256), j1 - j0 + 1), dim3(256), 0, s, g, a, nut, j0);
+}
+
+void launch_les_acc(hipStream_t s, const Geo &g, cx<double> *dst, const double *fort, const double *kx,
+                    const double *kz, int kmul, int j0, int j1)
+{
+    hipLaunchKernelGGL(k_les_acc, dim3((unsigned)((g.ps + 255) / 256), j1 - j0 + 1), dim3(256), 0, s, g, dst, fort,
+                       kx, kz, kmul, j0);
+}
+
+void launch_les_explicit(hipStream_t s, const Geo &g, double *r, const double *u, const double *nut,
+                         const double *dy, const double *dyf, double temp2, int sys, int j0, int j1)
+{
+    const long n = (long)g.nx * g.nz;
+    hipLaunchKernelGGL(k_les_explicit, dim3((unsigned)((n + 255) / 256), j1 - j0 + 1), dim3(256), 0, s, g, r, u, nut,
+                       dy, dyf, temp2, sys, j0);
+}
+
+void launch_les_cn(hipStream_t s, const Geo &g, const LesCn &a, const double *rhs, const double *nut,
+                   const double *dy, const double *dyf, double *bm, double *gw, double *out)
+{
+    const long n = (long)g.nx * g.nz;
+    hipLaunchKernelGGL(k_les_cn, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g, a, rhs, nut, dy, dyf, bm, gw,
+                       out);
+}
+
+void launch_les_vmax(hipStream_t s, const Geo &g, const double *u1, const double *u2, const double *u3, double *out)
+{
+    hipLaunchKernelGGL(k_les_vmax, dim3(g.ny), dim3(256), 0, s, g, u1, u2, u3, out);
+}
+
 void launch_les_bc(hipStream_t s, const Geo &g, cx<double> *cu1, cx<double> *cu3, int u_min, int u_max, int w_min,
                    int w_max)
 {

@@ -231,6 +231,19 @@ int32_t dgpu_tke_chan(dgpu_ctx *ctx, double *epsilon);
  * implicit NU_T terms in the Crank-Nicolson solves are not part of this call. */
 int32_t dgpu_les_chan(dgpu_ctx *ctx, double *nu_t);
 
+/* RK_CHAN_1's LES switches (FP64 contexts only).  les: input.dat's LES =
+ * .TRUE.: dgpu_rk_chan_1 then solves the Crank-Nicolson systems in physical
+ * space with the eddy viscosity NU_T (and KAPPA_T = NU_T for the scalar) in
+ * both halves (for/channel.f:628-659, 748-760, 793-806, 862-874, 927-939,
+ * 991-1003).  model: the branch condition of for/channel.f:303
+ * (LES .AND. (.NOT.CREATE_NEW_FLOW .OR. TIME_STEP > 100)), set by the caller
+ * before each step: les_chan runs in the substep, then the physical-space
+ * velocity BCs (for/channel.f:2495-2640), KAPPA_T = NU_T and the scalar's
+ * horizontal eddy diffusion (for/channel.f:313-375).  NU_T starts at zero and
+ * keeps its last value while the model is off.  LES substeps are not
+ * captured in graphs. */
+int32_t dgpu_set_les(dgpu_ctx *ctx, int32_t les, int32_t model);
+
 /* Kernel timing: when enabled, every kernel launch is bracketed by HIP events
  * on the context's stream.  dgpu_kernel_times returns, per kernel kind
  * (0 zinv, 1 xfused, 2 zfwd_nl, 3 cn, 4 remdiv), the summed milliseconds and the
