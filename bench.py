@@ -145,6 +145,29 @@ def ensemble_run(n: int, steps: int, warmup: int, precision: str = "mixed", acc_
     return out
 
 
+def les_run(cfg, phys, steps: int, warmup: int) -> dict:
+    """RK_CHAN_1 with LES (the constant Smagorinsky model called in every
+    substep, Crank-Nicolson solves in physical space), FP64, same workload."""
+    from diablo_amd import gpu
+    g = gpu.GpuChannel(cfg, precision="fp64")
+    g.initialize(*phys)
+    g.step(1)
+    g.set_les(True, True)
+    g.step(warmup)
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.step(steps)
+    g.synchronize()
+    el = time.perf_counter() - t0
+    cu1 = g.get_field(gpu.CU1)
+    if not math.isfinite(float(abs(cu1).max())):
+        raise RuntimeError("LES state diverged")
+    g.close()
+    return {"value": round(cfg.points * 3 * steps / el / 1e6, 2), "unit": "Mpts*substeps/s",
+            "ms_per_step": round(el / steps * 1e3, 3), "steps": steps, "dtype": "f64",
+            "model": "constant Smagorinsky, les_chan every substep"}
+
+
 def cpu_baseline(cfg, phys, threads: int) -> dict:
     from oracle import oracle
     o = oracle.Oracle(cfg, threads=threads)
@@ -182,6 +205,7 @@ def main():
     ap.add_argument("--no-kernel-timing", action="store_true")
     ap.add_argument("--mixed", choices=("on", "off"), default="on")
     ap.add_argument("--accuracy-steps", type=int, default=100)
+    ap.add_argument("--les-steps", type=int, default=5, help="timed steps of the LES path (0: skip)")
     ap.add_argument("--ensemble", type=int, default=64, help="members of the batched run (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=10)
     ap.add_argument("--ensemble-precision", default="mixed", choices=("mixed", "mixed_f64state", "fp64"))
@@ -260,6 +284,13 @@ def main():
         log(f"[bench] batched aggregate {batched['value']:.1f} Mpts*substeps/s "
             f"({batched['value'] / value:.2f}x FP64 384); {batched}")
 
+    les = None
+    if args.les_steps > 0:
+        log(f"[bench] LES: timing {args.les_steps} steps")
+        les = les_run(cfg, phys, args.les_steps, 1)
+        log(f"[bench] LES {les['value']:.1f} Mpts*substeps/s ({les['ms_per_step']:.1f} ms/step, "
+            f"{les['value'] / value:.2f}x the DNS path)")
+
     cpu = None
     if args.cpu_baseline == "on":
         threads = min(16, os.cpu_count() or 1)
@@ -288,6 +319,7 @@ def main():
         "cpu_baseline": cpu,
         "mixed": mixed,
         "batched": batched,
+        "les": les,
     }
     print(json.dumps(out), flush=True)
 

@@ -56,6 +56,9 @@ class ChannelConfig:
     update_dt: int = 1
     # driver settings of input.dat (for/diablo_io.f:31-36)
     create_new_flow: bool = True
+    # USE_LES (input.dat line 2): the constant Smagorinsky model and the
+    # eddy viscosity in the Crank-Nicolson terms (LES_MODEL_TYPE = 1)
+    les: bool = False
     time_limit: float = 1.0e6
     reset_time: bool = False
     save_flow_int: int = 100000
@@ -422,7 +425,7 @@ def write_decks(cfg: ChannelConfig, directory: str, save_flow_int: Optional[int]
         "C   FLAVOR        VERSION",
         "    'Basic'        2.0",
         "C   USE_MPI       USE_LES",
-        "    .TRUE.       .FALSE.",
+        f"    .TRUE.       {_lg(cfg.les)}",
         "C Parameters:     NU,        BETA,      LX,      LY,     LZ",
         f"   {_fl(cfg.nu)} {_fl(cfg.beta)} {_fl(cfg.lx)} {_fl(cfg.ly)} {_fl(cfg.lz)}",
         "C Vertical viscosity scale factor NU_V_SCALE (NU_V=NU*NU_V_SCALE)",
@@ -545,8 +548,6 @@ def read_decks(directory: str, nx: int, ny: int, nz: int, n_th: int) -> ChannelC
         raise ValueError("Wrong input data format.")
     r.skip()
     _use_mpi, les = r.read(2)
-    if _logical(les):
-        raise ValueError("LES is not part of this time-stepper")
     r.skip()
     nu, beta, lx, ly, lz = map(float, r.read(5))
     r.skip()
@@ -577,7 +578,11 @@ def read_decks(directory: str, nx: int, ny: int, nz: int, n_th: int) -> ChannelC
     (tam,) = c.read(1)
     if int(tam) != 1:
         raise ValueError("only TIME_AD_METH = 1 (RK_CHAN_1) is supported")
-    c.skip(); c.read(1)
+    c.skip()
+    (les_model_type,) = c.read(1)
+    if _logical(les) and int(les_model_type) != 1:
+        # types 2 and 3 stop in the reference's MPI build (for/les.f:272-276)
+        raise ValueError("only LES_MODEL_TYPE = 1 (constant Smagorinsky) is supported")
     c.skip()
     ic_type, kick = c.read(2)
     c.skip()
@@ -607,7 +612,8 @@ def read_decks(directory: str, nx: int, ny: int, nz: int, n_th: int) -> ChannelC
     return ChannelConfig(
         nx=nx, ny=ny, nz=nz, n_th=n_th, nu=nu, beta=beta, lx=lx, ly=ly, lz=lz,
         nu_v_scale=nu_v_scale, delta_t=float(dt), variable_dt=_logical(vdt), cfl=float(cfl),
-        update_dt=int(upd), create_new_flow=_logical(create), time_limit=float(tlim), reset_time=_logical(reset),
+        update_dt=int(upd), create_new_flow=_logical(create), les=_logical(les), time_limit=float(tlim),
+        reset_time=_logical(reset),
         save_flow_int=int(sfi), save_stats_int=int(ssi), i_ro=i_ro, grav_x=gx, grav_y=gyv, grav_z=gz,
         f_type=int(f_type), ubulk0=float(ubulk0), px0=float(px0), omega0=float(omega0),
         amp_omega0=float(amp), th_bc_ymin=thmin, th_bc_ymin_c1=thmin1, th_bc_ymax=thmax,

@@ -1,7 +1,8 @@
 """The channel time loop of DIABLO's main program (for/diablo.f:55-138) on
 the GPU: INITIALIZE from the decks (input.dat, input_chan.dat, grid_def), a
 new flow or a restart, RK_CHAN_1 three times per step with COURANT under
-VARIABLE_DT, SAVE_STATS and SAVE_FLOW at their intervals, TIME += DELTA_T,
+VARIABLE_DT (and the LES model once past its spin-up when USE_LES is set),
+SAVE_STATS and SAVE_FLOW at their intervals, TIME += DELTA_T,
 and END_RUN's stop conditions (wall time beyond TIME_LIMIT, a file named
 stop.now; for/diablo_io.f:412-435).
 
@@ -122,6 +123,10 @@ class ChannelRun:
         first = self.time_step + 1
         for ts in range(first, first + n):
             self.time_step = ts
+            if cfg.les:
+                # the model waits out a spin-up of 100 steps on a new flow
+                # (for/channel.f:303)
+                g.set_les(True, (not cfg.create_new_flow) or ts > 100)
             for k in (1, 2, 3):
                 cfl_now = cfg.variable_dt and k == 3 and ts % cfg.update_dt == 0
                 if cfl_now:

@@ -130,6 +130,12 @@
           TYPE(C_PTR), VALUE :: CTX
           REAL(C_DOUBLE), INTENT(OUT) :: E(*)
         END FUNCTION
+        INTEGER(C_INT32_T) FUNCTION dgpu_set_les(CTX, L, M)
+     &      BIND(C, NAME='dgpu_set_les')
+          IMPORT :: C_PTR, C_INT32_T
+          TYPE(C_PTR), VALUE :: CTX
+          INTEGER(C_INT32_T), VALUE :: L, M
+        END FUNCTION
         INTEGER(C_INT32_T) FUNCTION dgpu_courant_request(CTX)
      &      BIND(C, NAME='dgpu_courant_request')
           IMPORT :: C_PTR, C_INT32_T
@@ -319,6 +325,7 @@
 ! One RK3/CN substep on the GPU (for/channel.f:62-1073).  Under VARIABLE_DT
 ! the device takes COURANT's velocity maxima inside RK_STEP 3 and DELTA_T and
 ! H_BAR are updated after it, for the next step (for/channel.f:1037-1040).
+! With LES the eddy viscosity NU_T stays on the device.
       USE DGPU_BRIDGE
       INCLUDE 'header'
       LOGICAL DO_CFL
@@ -328,6 +335,12 @@
      &     (MOD(TIME_STEP, UPDATE_DT) .EQ. 0)
       IF (DO_CFL) CALL DGPU_CHECK(dgpu_courant_request(DGPU_CTX),
      &     'dgpu_courant_request')
+! LES: the eddy viscosity in the CN terms; les_chan once past the spin-up
+! of a new flow (for/channel.f:303)
+      IF (LES) CALL DGPU_CHECK(dgpu_set_les(DGPU_CTX, 1_C_INT32_T,
+     &     MERGE(1_C_INT32_T, 0_C_INT32_T,
+     &     (.NOT.CREATE_NEW_FLOW).OR.(TIME_STEP.GT.100))),
+     &     'dgpu_set_les')
       CALL DGPU_CHECK(dgpu_rk_chan_1(DGPU_CTX, INT(RK_STEP, C_INT32_T),
      &     REAL(TIME, C_DOUBLE), H_BAR, BETA_BAR, ZETA_BAR),
      &     'dgpu_rk_chan_1')
