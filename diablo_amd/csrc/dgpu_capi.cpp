@@ -89,6 +89,10 @@ struct dgpu_ctx {
     int courant_state = 0;
     double *d_vmax_part = nullptr, *d_vmax = nullptr;  // per-block, per-plane velocity maxima
     double *d_file = nullptr;  // one restart-file array, NX*NY*NZ
+    // z transforms as split-BF16 DFT-as-GEMM (mixed precision, NZ <= 256)
+    bool zdft = false;
+    uint16_t *d_zdft = nullptr;
+    dgpu::ZdftTab zinv_tab{}, zfwd_tab{};
     double *d_les = nullptr;   // LES work planes, NU_T, SGS force, filter widths
     bool les = false, les_model = false;  // input.dat's LES; les_chan called in each substep
     // CN tables: [stage 0..2][sys][NY+2][kRow]
@@ -712,7 +716,7 @@ int32_t dgpu_destroy(dgpu_ctx *c)
     void *bufs[] = {c->d_vec, c->d_tw, c->d_state, c->d_pmult, c->d_pinv, c->d_rd, c->d_nlptr, c->d_zt, c->d_wt,
                     c->d_fa, c->d_fb, c->d_gyf, c->d_iptw, c->d_iptwf, c->d_ztf, c->d_wtf, c->d_nlf, c->d_nlptrf,
                     c->d_fort, c->d_cn, c->d_codes, c->d_c1, c->d_state32, c->d_side, c->d_map,
-                    c->d_pmultf, c->d_pinvf, c->d_gs32, c->d_stats, c->d_vmax_part, c->d_vmax, c->d_file, c->d_les};
+                    c->d_pmultf, c->d_pinvf, c->d_gs32, c->d_stats, c->d_vmax_part, c->d_vmax, c->d_file, c->d_les, c->d_zdft};
     for (void *b : bufs)
         if (b) hipFree(b);
     if (c->side_stream) {
@@ -892,6 +896,114 @@ int32_t dgpu_tke_chan(dgpu_ctx *c, double *epsilon)
                          P(7, j) / (DY[j] * DY[j]) + P(4, j) + gyw(5);
         epsilon[j - 1] = c->p.nu * e;
     }
+    return DGPU_OK;
+}
+
+// ---------------------------------------------------------------------------
+// z transforms as DFT-as-GEMM on the matrix cores (mixed precision)
+// ---------------------------------------------------------------------------
+static uint16_t bf16_rne(float f)
+{
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+static double bf16_value(uint16_t b)
+{
+    const uint32_t u = (uint32_t)b << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+// Real weights A (rows x cols, row-major) -> fragment-major split-BF16 halves:
+// [mt][ks][lane][t] = A[mt*32 + lane%32][ks*16 + (lane/32)*8 + t], zero-padded.
+static void zdft_fragments(const std::vector<double> &A, int rows, int cols, int nmt, int nks,
+                           std::vector<uint16_t> &out)
+{
+    const size_t n = (size_t)nmt * nks * 64 * 8;
+    out.assign(2 * n, 0);
+    for (int mt = 0; mt < nmt; ++mt)
+        for (int ks = 0; ks < nks; ++ks)
+            for (int l = 0; l < 64; ++l)
+                for (int t = 0; t < 8; ++t) {
+                    const int m = mt * 32 + l % 32, k = ks * 16 + (l / 32) * 8 + t;
+                    const double v = (m < rows && k < cols) ? A[(size_t)m * cols + k] : 0.0;
+                    const uint16_t h = bf16_rne((float)v);
+                    const size_t at = (((size_t)mt * nks + ks) * 64 + l) * 8 + t;
+                    out[at] = h;
+                    out[n + at] = bf16_rne((float)(v - bf16_value(h)));
+                }
+}
+
+// One complex weight w at (row q, column p) of a complex matrix as the real
+// 2x2 block [[Re w, -Im w], [Im w, Re w]].
+static void put_cx(std::vector<double> &A, int cols, int q, int p, double re, double im)
+{
+    A[(size_t)(2 * q) * cols + 2 * p] = re;
+    A[(size_t)(2 * q) * cols + 2 * p + 1] = -im;
+    A[(size_t)(2 * q + 1) * cols + 2 * p] = im;
+    A[(size_t)(2 * q + 1) * cols + 2 * p + 1] = re;
+}
+
+static int build_zdft(dgpu_ctx *c)
+{
+    const dgpu::Geo &g = c->g;
+    const int NZ = g.nz, mz = g.mz;
+    const double PI = 4.0 * std::atan(1.0);
+    auto kint = [&](int s) { return s <= g.nkz ? s : s - (g.tnkz + 1); };  // signed wavenumber of slot s
+    auto angle = [&](long kz) { return 2.0 * PI * (double)(((kz % NZ) + NZ) % NZ) / NZ; };
+    // backward: Z[z] = sum_s exp(+i 2 pi k(s) z / NZ) X[s]
+    const int ri = 2 * NZ, ci = 2 * mz;
+    const int nmt_i = (ri + 31) / 32, nks_i = (ci + 15) / 16;
+    std::vector<double> Ai((size_t)ri * ci, 0.0);
+    for (int z = 0; z < NZ; ++z)
+        for (int s = 0; s < mz; ++s) {
+            const double th = angle((long)kint(s) * z);
+            put_cx(Ai, ci, z, s, std::cos(th), std::sin(th));
+        }
+    // forward: NL[s] = sum_z exp(-i 2 pi k(s) z / NZ) / NZ (G[z] + i KZ(s) H[z])
+    const int rf = 2 * mz, cf = 4 * NZ;
+    const int nmt_f = (rf + 31) / 32, nks_f = cf / 16;
+    std::vector<double> Af((size_t)rf * cf, 0.0);
+    for (int s = 0; s < mz; ++s)
+        for (int z = 0; z < NZ; ++z) {
+            const double th = angle((long)kint(s) * z);
+            const double fr = std::cos(th) / NZ, fi = -std::sin(th) / NZ;
+            const double kz = c->kz[s];
+            put_cx(Af, cf, s, z, fr, fi);
+            put_cx(Af, cf, s, NZ + z, -kz * fi, kz * fr);
+        }
+    std::vector<uint16_t> fi_, ff_;
+    zdft_fragments(Ai, ri, ci, nmt_i, nks_i, fi_);
+    zdft_fragments(Af, rf, cf, nmt_f, nks_f, ff_);
+    HIPCHK(c, hipMalloc(&c->d_zdft, sizeof(uint16_t) * (fi_.size() + ff_.size())));
+    HIPCHK(c, hipMemcpy(c->d_zdft, fi_.data(), sizeof(uint16_t) * fi_.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_zdft + fi_.size(), ff_.data(), sizeof(uint16_t) * ff_.size(), hipMemcpyHostToDevice));
+    const uint4 *bi = reinterpret_cast<const uint4 *>(c->d_zdft);
+    const uint4 *bf = reinterpret_cast<const uint4 *>(c->d_zdft + fi_.size());
+    c->zinv_tab = {bi, bi + fi_.size() / 16, nmt_i, nks_i};
+    c->zfwd_tab = {bf, bf + ff_.size() / 16, nmt_f, nks_f};
+    return DGPU_OK;
+}
+
+int32_t dgpu_set_zdft(dgpu_ctx *c, int32_t on)
+{
+    if (!c) return DGPU_E_ARG;
+    if (on) {
+        if (!c->mixed) return fail(c, DGPU_E_ARG, "the MFMA z transforms run in mixed-precision contexts");
+        if (c->g.nz > 256 || c->g.nz % 16 != 0) return fail(c, DGPU_E_ARG, "the MFMA z transforms need NZ <= 256, a multiple of 16");
+        if (!c->d_zdft)
+            if (int rc = build_zdft(c)) return rc;
+    }
+    c->zdft = on != 0;
+    for (auto &ge : c->gexec)
+        if (ge) {
+            HIPCHK(c, hipGraphExecDestroy(ge));
+            ge = nullptr;
+        }
     return DGPU_OK;
 }
 
@@ -1210,7 +1322,13 @@ static int rk_enqueue(dgpu_ctx *c, int rk_step, double time, double temp1, doubl
         const cx<float> *twx = c->d_iptwf, *twz = c->d_iptwf + c->ipx.twn;
         {
             TimerScope t(c, 0);
-            if (c->f32state)
+            if (c->zdft && c->f32state)
+                dgpu::launch_zdft_inv<float>(c->stream, g, c->zinv_tab, c->d_state32, c->fs, g.ps, c->d_ztf, c->zfs,
+                                             zps, c->ld, 0, g.ny + 2, nin);
+            else if (c->zdft)
+                dgpu::launch_zdft_inv<double>(c->stream, g, c->zinv_tab, c->cu[0], c->fs, g.ps, c->d_ztf, c->zfs,
+                                              zps, c->ld, 0, g.ny + 2, nin);
+            else if (c->f32state)
                 dgpu::launch_zinv_ip<float, float>(c->stream, g, c->ipz, twz, c->d_state32, c->fs, g.ps, c->d_ztf,
                                                    c->zfs, zps, c->ld, 0, g.ny + 2, nin);
             else
@@ -1226,8 +1344,11 @@ static int rk_enqueue(dgpu_ctx *c, int rk_step, double time, double temp1, doubl
         if (int rc = check_launch(c, "xfused")) return rc;
         {
             TimerScope t(c, 2);
-            dgpu::launch_zfwd_nl_ip<float, float>(c->stream, g, c->ipz, twz, c->d_wtf, c->zfs, c->ld, c->d_nlptrf,
-                                                  c->d_kz, jlo, g.ny, nin);
+            if (c->zdft)
+                dgpu::launch_zdft_fwd(c->stream, g, c->zfwd_tab, c->d_wtf, c->zfs, c->ld, c->d_nlptrf, jlo, g.ny, nin);
+            else
+                dgpu::launch_zfwd_nl_ip<float, float>(c->stream, g, c->ipz, twz, c->d_wtf, c->zfs, c->ld,
+                                                      c->d_nlptrf, c->d_kz, jlo, g.ny, nin);
         }
         if (int rc = check_launch(c, "zfwd_nl")) return rc;
     } else {

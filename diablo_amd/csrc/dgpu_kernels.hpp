@@ -252,6 +252,19 @@ void launch_les_cross_src(hipStream_t s, const Geo &g, const double *nut, const 
 void launch_les_cross_add(hipStream_t s, const Geo &g, const double *ct, cx<double> *cf2, const double *kx,
                           const double *kz, int use_kz);
 
+// z transforms as split-BF16 DFT-as-GEMM on the matrix cores (dgpu_zdft.hip):
+// weights fragment-major, hi and lo halves, nmt M tiles of 32 rows, nks
+// k-steps of 16.
+struct ZdftTab {
+    const uint4 *hi, *lo;
+    int nmt, nks;
+};
+template <typename TIN>
+void launch_zdft_inv(hipStream_t s, const Geo &g, const ZdftTab &t, const cx<TIN> *in, long in_fs, long in_ps,
+                     cx<float> *out, long out_fs, long out_ps, int ld, int j0, int nplanes, int nf);
+void launch_zdft_fwd(hipStream_t s, const Geo &g, const ZdftTab &t, const cx<float> *wt, long wt_fs, int ld,
+                     cx<float> *const *nl, int jlo, int jhi, int nout);
+
 // RK_CHAN_1 with LES (dgpu_les.hip): the velocity and scalar Crank-Nicolson
 // solves in physical space with the eddy viscosity.
 struct LesBc {

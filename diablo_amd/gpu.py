@@ -57,7 +57,7 @@ EXPORTS = (
     "dgpu_ensemble_create", "dgpu_ensemble_destroy", "dgpu_ensemble_size", "dgpu_ensemble_member",
     "dgpu_ensemble_last_error", "dgpu_ensemble_set_graphs", "dgpu_ensemble_rk_chan_1",
     "dgpu_ensemble_synchronize", "dgpu_stats_chan", "dgpu_courant_request", "dgpu_courant",
-    "dgpu_get_flow_physical", "dgpu_set_flow_physical", "dgpu_tke_chan", "dgpu_les_chan", "dgpu_set_les",
+    "dgpu_get_flow_physical", "dgpu_set_flow_physical", "dgpu_tke_chan", "dgpu_les_chan", "dgpu_set_les", "dgpu_set_zdft",
 )
 
 # dgpu_stats_chan profiles, in the order of the header's DGPU_ST_*
@@ -125,6 +125,8 @@ def load_library(path: str = LIB_PATH):
     L.dgpu_les_chan.argtypes = [vp, DP]
     L.dgpu_set_les.restype = i32
     L.dgpu_set_les.argtypes = [vp, i32, i32]
+    L.dgpu_set_zdft.restype = i32
+    L.dgpu_set_zdft.argtypes = [vp, i32]
     L.dgpu_get_flow_physical.restype = i32
     L.dgpu_get_flow_physical.argtypes = [vp, i32, DP]
     L.dgpu_set_flow_physical.restype = i32
@@ -307,6 +309,11 @@ class GpuChannel:
         out = np.zeros(self.cfg.field_shape())
         self._chk(self.L.dgpu_les_chan(self.h, _dp(out)), "dgpu_les_chan")
         return out
+
+    def set_zdft(self, on: bool):
+        """Mixed precision: z transforms as split-BF16 DFT-as-GEMM on the
+        matrix cores (NZ <= 256)."""
+        self._chk(self.L.dgpu_set_zdft(self.h, int(bool(on))), "dgpu_set_zdft")
 
     def set_les(self, les: bool, model: bool = True):
         """RK_CHAN_1's LES switches: les = the eddy viscosity in the

@@ -244,6 +244,14 @@ int32_t dgpu_les_chan(dgpu_ctx *ctx, double *nu_t);
  * captured in graphs. */
 int32_t dgpu_set_les(dgpu_ctx *ctx, int32_t les, int32_t model);
 
+/* The optional DFT-as-GEMM z transforms of a DGPU_MIXED / DGPU_MIXED_F64STATE
+ * context (no counterpart in the reference, which transforms with FFTW,
+ * for/mpi.f:1530-1674): the substep's z backward and z forward (with the
+ * i kz derivative folded in) as real GEMMs on the matrix cores with split-BF16
+ * operands (three products) and FP32 accumulation.  NZ <= 256, a multiple of
+ * 16; the x transforms stay FFTs.  on = 0 returns to the FFTs. */
+int32_t dgpu_set_zdft(dgpu_ctx *ctx, int32_t on);
+
 /* Kernel timing: when enabled, every kernel launch is bracketed by HIP events
  * on the context's stream.  dgpu_kernel_times returns, per kernel kind
  * (0 zinv, 1 xfused, 2 zfwd_nl, 3 cn, 4 remdiv), the summed milliseconds and the
