@@ -27,6 +27,7 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+MFMA_BF16_PEAK_TFLOPS = 2500.0   # dense BF16 matrix-core peak of MI355X
 
 
 def log(*a):
@@ -61,11 +62,13 @@ def substep_model_bytes(cfg, s: int = 8) -> float:
     return s * (2 * (3 + cfg.n_th) * P + (23 + 7 * cfg.n_th) * F)
 
 
-def time_path(cfg, phys, precision: str, steps: int, warmup: int, kernel_timing: bool):
+def time_path(cfg, phys, precision: str, steps: int, warmup: int, kernel_timing: bool, zdft: bool = False):
     """Initial state -> warm-up -> timed steps; returns (seconds, kernel times, channel)."""
     from diablo_amd import gpu
     g = gpu.GpuChannel(cfg, precision=precision)
     g.initialize(*phys)
+    if zdft:
+        g.set_zdft(True)
     g.step(warmup)
     g.synchronize()
     if kernel_timing:
@@ -81,13 +84,15 @@ def time_path(cfg, phys, precision: str, steps: int, warmup: int, kernel_timing:
     return elapsed, kt, g
 
 
-def accuracy(cfg, phys, steps: int) -> dict:
+def accuracy(cfg, phys, steps: int, zdft: bool = False) -> dict:
     """Mixed path against the FP64 path after `steps` steps from the same state."""
     from diablo_amd import gpu, metrics
     g64 = gpu.GpuChannel(cfg, precision="fp64")
     gmx = gpu.GpuChannel(cfg, precision="mixed")
     g64.initialize(*phys)
     gmx.initialize(*phys)
+    if zdft:
+        gmx.set_zdft(True)
     g64.step(steps)
     gmx.step(steps)
     g64.synchronize()
@@ -142,6 +147,52 @@ def ensemble_run(n: int, steps: int, warmup: int, precision: str = "mixed", acc_
         emx.close()
         out.update({"rel_l2_vs_fp64_worst_member": worst["rel_l2"], "ke_drift_vs_fp64_worst_member": worst["ke_drift"],
                     "max_div_worst_member": worst["max_div"], "accuracy_steps": acc_steps})
+    return out
+
+
+def zdft_flops(cfg) -> dict:
+    """Useful (unpadded) flops per launch of the DFT-as-GEMM z kernels: real
+    GEMMs of (2 NZ x 2 MZ) and (2 MZ x 4 NZ) per column block of NKX+1 modes,
+    three split-BF16 products each."""
+    nz, mz, mk, nin = cfg.nz, 2 * cfg.nkz + 1, cfg.nkx + 1, 3 + cfg.n_th
+    jlo = min([2] + [1 if cfg.th_bc_ymin[n] == 1 else 2 for n in range(cfg.n_th)])
+    inv = 3 * 2 * (2 * nz) * (2 * mz) * mk * nin * (cfg.ny + 2)
+    fwd = 3 * 2 * (2 * mz) * (4 * nz) * mk * nin * (cfg.ny - jlo + 1)
+    return {"zinv": float(inv), "zfwd": float(fwd)}
+
+
+def zdft_run(steps: int, warmup: int, acc_steps: int) -> dict:
+    """The Kelvin-Helmholtz layer at 256 x 193 x 256: FP64, mixed with FFT z
+    transforms, mixed with the split-BF16 DFT-as-GEMM z transforms."""
+    from diablo_amd import case
+    cfg = case.WORKLOADS["kh256"]()
+    phys = case.initial_physical(cfg)
+    pts = cfg.points * 3 * steps
+    el64, _, g = time_path(cfg, phys, "fp64", steps, warmup, False)
+    g.close()
+    elm, _, g = time_path(cfg, phys, "mixed", steps, warmup, False)
+    g.close()
+    elz, ktz, g = time_path(cfg, phys, "mixed", steps, warmup, True, zdft=True)
+    g.close()
+    el_nt, _, g = time_path(cfg, phys, "mixed", steps, warmup, False, zdft=True)
+    g.close()
+    fl = zdft_flops(cfg)
+    kern = {}
+    for kind, name in (("zinv", "zinv"), ("zfwd_nl", "zfwd")):
+        ms, n = ktz.get(kind, (0.0, 0))
+        if n:
+            avg = ms / n
+            kern[name] = {"ms_per_launch": round(avg, 4), "gflop_per_launch": round(fl[name] / 1e9, 2),
+                          "achieved_tflops": round(fl[name] / (avg * 1e-3) / 1e12, 1),
+                          "frac_of_bf16_dense_peak": round(fl[name] / (avg * 1e-3) / 1e12 / MFMA_BF16_PEAK_TFLOPS, 4)}
+    out = {"workload": f"kh256: Kelvin-Helmholtz {cfg.nx}x{cfg.ny}x{cfg.nz}, Re=1000, Ri=0.1, dt={cfg.delta_t}",
+           "fp64": round(pts / el64 / 1e6, 2), "mixed_fft": round(pts / elm / 1e6, 2),
+           "mixed_zdft": round(pts / el_nt / 1e6, 2), "unit": "Mpts*substeps/s", "steps": steps,
+           "kernels": kern}
+    if acc_steps > 0:
+        acc = accuracy(cfg, phys, acc_steps, zdft=True)
+        out.update({"rel_l2_vs_fp64": acc["rel_l2"], "ke_drift_vs_fp64": acc["ke_drift"],
+                    "max_div": acc["max_div"], "accuracy_steps": acc["steps"]})
     return out
 
 
@@ -206,6 +257,9 @@ def main():
     ap.add_argument("--mixed", choices=("on", "off"), default="on")
     ap.add_argument("--accuracy-steps", type=int, default=100)
     ap.add_argument("--les-steps", type=int, default=5, help="timed steps of the LES path (0: skip)")
+    ap.add_argument("--zdft-steps", type=int, default=10,
+                    help="timed steps of the 256x193x256 KH case with the MFMA z transforms (0: skip)")
+    ap.add_argument("--zdft-accuracy-steps", type=int, default=200)
     ap.add_argument("--ensemble", type=int, default=64, help="members of the batched run (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=10)
     ap.add_argument("--ensemble-precision", default="mixed", choices=("mixed", "mixed_f64state", "fp64"))
@@ -291,6 +345,13 @@ def main():
         log(f"[bench] LES {les['value']:.1f} Mpts*substeps/s ({les['ms_per_step']:.1f} ms/step, "
             f"{les['value'] / value:.2f}x the DNS path)")
 
+    zd = None
+    if args.zdft_steps > 0:
+        log(f"[bench] kh256 with the MFMA z transforms: {args.zdft_steps} steps, accuracy over "
+            f"{args.zdft_accuracy_steps}")
+        zd = zdft_run(args.zdft_steps, 1, args.zdft_accuracy_steps)
+        log(f"[bench] kh256: {zd}")
+
     cpu = None
     if args.cpu_baseline == "on":
         threads = min(16, os.cpu_count() or 1)
@@ -320,6 +381,7 @@ def main():
         "mixed": mixed,
         "batched": batched,
         "les": les,
+        "zdft": zd,
     }
     print(json.dumps(out), flush=True)
 
