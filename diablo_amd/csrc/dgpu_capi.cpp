@@ -994,7 +994,8 @@ int32_t dgpu_set_zdft(dgpu_ctx *c, int32_t on)
     if (!c) return DGPU_E_ARG;
     if (on) {
         if (!c->mixed) return fail(c, DGPU_E_ARG, "the MFMA z transforms run in mixed-precision contexts");
-        if (c->g.nz > 256 || c->g.nz % 16 != 0) return fail(c, DGPU_E_ARG, "the MFMA z transforms need NZ <= 256, a multiple of 16");
+        if (c->g.nz > 256 || c->g.nz % 16 != 0 || c->g.nx > 256)
+            return fail(c, DGPU_E_ARG, "the MFMA z transforms need NX <= 256 and NZ <= 256, a multiple of 16");
         if (!c->d_zdft)
             if (int rc = build_zdft(c)) return rc;
     }

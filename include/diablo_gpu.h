@@ -248,8 +248,8 @@ int32_t dgpu_set_les(dgpu_ctx *ctx, int32_t les, int32_t model);
  * context (no counterpart in the reference, which transforms with FFTW,
  * for/mpi.f:1530-1674): the substep's z backward and z forward (with the
  * i kz derivative folded in) as real GEMMs on the matrix cores with split-BF16
- * operands (three products) and FP32 accumulation.  NZ <= 256, a multiple of
- * 16; the x transforms stay FFTs.  on = 0 returns to the FFTs. */
+ * operands (three products) and FP32 accumulation.  NX <= 256, NZ <= 256 and a
+ * multiple of 16; the x transforms stay FFTs.  on = 0 returns to the FFTs. */
 int32_t dgpu_set_zdft(dgpu_ctx *ctx, int32_t on);
 
 /* Kernel timing: when enabled, every kernel launch is bracketed by HIP events
