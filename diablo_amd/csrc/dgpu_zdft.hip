@@ -35,13 +35,14 @@ __device__ __forceinline__ bf16x8 as_bf16x8(uint4 v)
     return r;
 }
 
-// One block: all column tiles (NT of 32 columns) of one (field, plane);
-// wave w owns M tiles 2w, 2w+1.  The data operand is split into BF16 halves
-// once per block, a chunk of kKC k-steps at a time, into LDS in fragment
-// order; each wave reads its two weight fragments per k-step once (the next
-// k-step's are in flight while the current one's products run) and uses them
-// for all NT column tiles.  ld_b(q, i): complex input q of column i (zero
-// outside); st(q, i, v): complex output q of column i.
+// One block: all column tiles (NT of 32 columns) of one (field, plane) and
+// the M tiles from mbase on; wave w owns M tiles mbase + 2w, mbase + 2w + 1.
+// The data operand is split into BF16 halves once per block, a chunk of kKC
+// k-steps at a time, into LDS in fragment order.  Each wave reads its two
+// weight fragments per k-step once (the next k-step's are in flight while
+// the current one's products run) and uses them for all NT column tiles.
+// ld_b(q, i): complex input q of column i (zero outside); st(q, i, v):
+// complex output q of column i.
 constexpr int kKC = 8;
 
 struct AFrag {
@@ -50,11 +51,11 @@ struct AFrag {
 
 template <int NT, class LdB, class St>
 __device__ __forceinline__ void zdft_block(const uint4 *__restrict__ ahi, const uint4 *__restrict__ alo, int nmt,
-                                           int nks, int mk, LdB &&ld_b, St &&st)
+                                           int nks, int mk, int mbase, LdB &&ld_b, St &&st)
 {
     __shared__ uint4 bs[kKC * NT * 64 * 2];  // [ks][nt][lane][hi, lo]
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, nthr = blockDim.x;
-    const int mt0 = 2 * w;
+    const int mt0 = mbase + 2 * w;
     const bool has0 = mt0 < nmt, has1 = mt0 + 1 < nmt;
     const int half = lane >> 5;
     f32x16 acc[2][NT];
@@ -143,7 +144,7 @@ __device__ __forceinline__ float2 ld2(const cx<TIN> *p)
 }  // namespace
 
 // z backward: Zt[f][j][z][i] (natural z) from the kept slots of the state.
-// grid (planes, fields), block 64 * ceil(nmt / 2).
+// grid (planes, fields, M blocks), block 64 * waves (zdft_waves).
 template <typename TIN, int NT>
 __global__ void __launch_bounds__(512) k_zdft_inv(const uint4 *__restrict__ ahi, const uint4 *__restrict__ alo,
                                                    int nmt, int nks, const cx<TIN> *__restrict__ in, long in_fs,
@@ -154,7 +155,7 @@ __global__ void __launch_bounds__(512) k_zdft_inv(const uint4 *__restrict__ ahi,
     const cx<TIN> *src = in + f * in_fs + (long)j * in_ps;
     cx<float> *dst = out + f * out_fs + (long)j * out_ps;
     zdft_block<NT>(
-        ahi, alo, nmt, nks, mk,
+        ahi, alo, nmt, nks, mk, (int)blockIdx.z * 2 * (int)(blockDim.x >> 6),
         [&](int q, int i) {
             // unconditional load at a valid index, then select
             const float2 v = ld2(src + (long)min(q, mz - 1) * mk + min(i, mk - 1));
@@ -172,7 +173,7 @@ __global__ void __launch_bounds__(512) k_zdft_inv(const uint4 *__restrict__ ahi,
 
 // z forward: NL[o][j][s][i] at the kept slots s < mz from G = Wt[2o], H =
 // Wt[2o+1] (natural z); the weights hold 1/NZ and i kz.  grid (planes
-// jlo.., outputs), block 64 * ceil(nmt / 2).
+// jlo.., outputs, M blocks), block 64 * waves (zdft_waves).
 template <int NT>
 __global__ void __launch_bounds__(512) k_zdft_fwd(const uint4 *__restrict__ ahi, const uint4 *__restrict__ alo,
                                                    int nmt, int nks, const cx<float> *__restrict__ wt, long wt_fs,
@@ -183,7 +184,7 @@ __global__ void __launch_bounds__(512) k_zdft_fwd(const uint4 *__restrict__ ahi,
     const cx<float> *g = wt + (2 * o) * wt_fs + (long)j * wt_ps;
     cx<float> *dst = nl[o] + (long)j * ps;
     zdft_block<NT>(
-        ahi, alo, nmt, nks, mk,
+        ahi, alo, nmt, nks, mk, (int)blockIdx.z * 2 * (int)(blockDim.x >> 6),
         [&](int q, int i) {
             // q < NZ: G at z = q; NZ <= q < 2 NZ: H at z = q - NZ
             const int h = q >= nz ? 1 : 0;
@@ -198,6 +199,15 @@ __global__ void __launch_bounds__(512) k_zdft_fwd(const uint4 *__restrict__ ahi,
                 dst[(long)s * mk + i] = c;
             }
         });
+}
+
+// M tiles of a launch split over ceil(nmt / 8) blocks per plane: blocks of
+// at most four waves keep three blocks resident per CU, so one block's
+// staging overlaps another's products.
+static void zdft_waves(int nmt, int &waves, int &mblocks)
+{
+    mblocks = (nmt + 7) / 8;
+    waves = ((nmt + 1) / 2 + mblocks - 1) / mblocks;
 }
 
 template <class F>
@@ -216,7 +226,9 @@ void launch_zdft_inv(hipStream_t s, const Geo &g, const ZdftTab &t, const cx<TIN
 {
     with_nt(g.mk, [&](auto ntc) {
         constexpr int NT = decltype(ntc)::value;
-        hipLaunchKernelGGL((k_zdft_inv<TIN, NT>), dim3(nplanes, nf), dim3(64 * ((t.nmt + 1) / 2)), 0, s, t.hi,
+        int waves, mb;
+        zdft_waves(t.nmt, waves, mb);
+        hipLaunchKernelGGL((k_zdft_inv<TIN, NT>), dim3(nplanes, nf, mb), dim3(64 * waves), 0, s, t.hi,
                            t.lo, t.nmt, t.nks, in, in_fs, in_ps, g.mk, g.mz, out, out_fs, out_ps, ld, g.nz, j0);
     });
 }
@@ -226,7 +238,9 @@ void launch_zdft_fwd(hipStream_t s, const Geo &g, const ZdftTab &t, const cx<flo
 {
     with_nt(g.mk, [&](auto ntc) {
         constexpr int NT = decltype(ntc)::value;
-        hipLaunchKernelGGL((k_zdft_fwd<NT>), dim3(jhi - jlo + 1, nout), dim3(64 * ((t.nmt + 1) / 2)), 0, s, t.hi,
+        int waves, mb;
+        zdft_waves(t.nmt, waves, mb);
+        hipLaunchKernelGGL((k_zdft_fwd<NT>), dim3(jhi - jlo + 1, nout, mb), dim3(64 * waves), 0, s, t.hi,
                            t.lo, t.nmt, t.nks, wt, wt_fs, (long)g.nz * ld, ld, g.nz, g.mk, g.mz, nl, g.ps, jlo);
     });
 }
