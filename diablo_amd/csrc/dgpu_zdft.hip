@@ -53,7 +53,7 @@ template <int NT, class LdB, class St>
 __device__ __forceinline__ void zdft_block(const uint4 *__restrict__ ahi, const uint4 *__restrict__ alo, int nmt,
                                            int nks, int mk, int mbase, LdB &&ld_b, St &&st)
 {
-    __shared__ uint4 bs[kKC * NT * 64 * 2];  // [ks][nt][lane][hi, lo]
+    __shared__ uint4 bsh[kKC * NT * 64], bsl[kKC * NT * 64];  // [ks][nt][lane], hi and lo halves
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, nthr = blockDim.x;
     const int mt0 = mbase + 2 * w;
     const bool has0 = mt0 < nmt, has1 = mt0 + 1 < nmt;
@@ -94,8 +94,8 @@ __device__ __forceinline__ void zdft_block(const uint4 *__restrict__ ahi, const 
             uint4 h4, l4;
             __builtin_memcpy(&h4, &bh, 16);
             __builtin_memcpy(&l4, &bl, 16);
-            bs[fr * 2] = h4;
-            bs[fr * 2 + 1] = l4;
+            bsh[fr] = h4;
+            bsl[fr] = l4;
         }
         __syncthreads();
         if (has0) {
@@ -105,7 +105,7 @@ __device__ __forceinline__ void zdft_block(const uint4 *__restrict__ ahi, const 
 #pragma unroll
                 for (int n = 0; n < NT; ++n) {
                     const int fr = (ks * NT + n) * 64 + lane;
-                    const bf16x8 bh = as_bf16x8(bs[fr * 2]), bl = as_bf16x8(bs[fr * 2 + 1]);
+                    const bf16x8 bh = as_bf16x8(bsh[fr]), bl = as_bf16x8(bsl[fr]);
                     acc[0][n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(cur.h0, bh, acc[0][n], 0, 0, 0);
                     acc[0][n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(cur.h0, bl, acc[0][n], 0, 0, 0);
                     acc[0][n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(cur.l0, bh, acc[0][n], 0, 0, 0);
