@@ -62,8 +62,10 @@ def substep_model_bytes(cfg, s: int = 8) -> float:
     return s * (2 * (3 + cfg.n_th) * P + (23 + 7 * cfg.n_th) * F)
 
 
-def time_path(cfg, phys, precision: str, steps: int, warmup: int, kernel_timing: bool, zdft: bool = False):
-    """Initial state -> warm-up -> timed steps; returns (seconds, kernel times, channel)."""
+def time_path(cfg, phys, precision: str, steps: int, warmup: int, kernel_timing: bool, zdft: bool = False,
+              tune: bool = False):
+    """Initial state -> warm-up -> [autotune] -> timed steps; returns (seconds,
+    kernel times, channel); the autotuner's choice is in channel.tuned."""
     from diablo_amd import gpu
     g = gpu.GpuChannel(cfg, precision=precision)
     g.initialize(*phys)
@@ -71,6 +73,7 @@ def time_path(cfg, phys, precision: str, steps: int, warmup: int, kernel_timing:
         g.set_zdft(True)
     g.step(warmup)
     g.synchronize()
+    g.tuned = g.autotune(3) if tune else None
     if kernel_timing:
         g.set_kernel_timing(True)
         g.kernel_times()  # reset
@@ -176,6 +179,9 @@ def zdft_run(steps: int, warmup: int, acc_steps: int) -> dict:
     g.close()
     el_nt, _, g = time_path(cfg, phys, "mixed", steps, warmup, False, zdft=True)
     g.close()
+    el_t, _, g = time_path(cfg, phys, "mixed", steps, warmup, False, tune=True)
+    tuned = g.tuned
+    g.close()
     fl = zdft_flops(cfg)
     kern = {}
     for kind, name in (("zinv", "zinv"), ("zfwd_nl", "zfwd")):
@@ -187,7 +193,8 @@ def zdft_run(steps: int, warmup: int, acc_steps: int) -> dict:
                           "frac_of_bf16_dense_peak": round(fl[name] / (avg * 1e-3) / 1e12 / MFMA_BF16_PEAK_TFLOPS, 4)}
     out = {"workload": f"kh256: Kelvin-Helmholtz {cfg.nx}x{cfg.ny}x{cfg.nz}, Re=1000, Ri=0.1, dt={cfg.delta_t}",
            "fp64": round(pts / el64 / 1e6, 2), "mixed_fft": round(pts / elm / 1e6, 2),
-           "mixed_zdft": round(pts / el_nt / 1e6, 2), "unit": "Mpts*substeps/s", "steps": steps,
+           "mixed_zdft": round(pts / el_nt / 1e6, 2), "mixed_autotuned": round(pts / el_t / 1e6, 2),
+           "autotune": tuned, "unit": "Mpts*substeps/s", "steps": steps,
            "kernels": kern}
     if acc_steps > 0:
         acc = accuracy(cfg, phys, acc_steps, zdft=True)
@@ -256,6 +263,8 @@ def main():
     ap.add_argument("--no-kernel-timing", action="store_true")
     ap.add_argument("--mixed", choices=("on", "off"), default="on")
     ap.add_argument("--accuracy-steps", type=int, default=100)
+    ap.add_argument("--autotune", choices=("on", "off"), default="on",
+                    help="pick the launch shapes with dgpu_autotune before timing (FP64 and mixed legs)")
     ap.add_argument("--les-steps", type=int, default=5, help="timed steps of the LES path (0: skip)")
     ap.add_argument("--zdft-steps", type=int, default=10,
                     help="timed steps of the 256x193x256 KH case with the MFMA z transforms (0: skip)")
@@ -281,7 +290,9 @@ def main():
     log(f"[bench] workload {args.workload}: {cfg.nx}x{cfg.ny}x{cfg.nz}, building initial state")
     phys = case.initial_physical(cfg)
     log(f"[bench] FP64: warmup {args.warmup}, timing {args.steps} steps")
-    elapsed, kt, g = time_path(cfg, phys, "fp64", args.steps, args.warmup, not args.no_kernel_timing)
+    elapsed, kt, g = time_path(cfg, phys, "fp64", args.steps, args.warmup, not args.no_kernel_timing,
+                               tune=args.autotune == "on")
+    tuned = {"fp64": g.tuned}
     substeps = 3 * args.steps
     value = cfg.points * substeps / elapsed / 1e6
     # sanity: the state must stay finite
@@ -315,7 +326,9 @@ def main():
     mixed = None
     if args.mixed == "on":
         log(f"[bench] mixed: warmup {args.warmup}, timing {args.steps} steps")
-        el_m, kt_m, gm = time_path(cfg, phys, "mixed", args.steps, args.warmup, not args.no_kernel_timing)
+        el_m, kt_m, gm = time_path(cfg, phys, "mixed", args.steps, args.warmup, not args.no_kernel_timing,
+                                   tune=args.autotune == "on")
+        tuned["mixed"] = gm.tuned
         gm.close()
         vm = cfg.points * substeps / el_m / 1e6
         for k, (ms, n) in kt_m.items():
@@ -382,6 +395,7 @@ def main():
         "batched": batched,
         "les": les,
         "zdft": zd,
+        "autotune": tuned if args.autotune == "on" else None,
     }
     print(json.dumps(out), flush=True)
 

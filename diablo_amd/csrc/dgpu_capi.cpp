@@ -1649,6 +1649,88 @@ int32_t dgpu_rk_chan_1(dgpu_ctx *c, int32_t rk_step, double time, const double *
     return DGPU_OK;
 }
 
+// Launch-shape autotuner: times `steps` RK3 steps (after one untimed step)
+// of this context's own state (with time step delta_t) for each candidate -- planes per block of the
+// fused x kernel, and for mixed contexts with NX, NZ <= 256 the FFT or the
+// MFMA z transforms -- and keeps the fastest.  The state is copied aside
+// before and put back after, so tuning does not advance the run.
+int32_t dgpu_autotune(dgpu_ctx *c, int32_t steps, double delta_t, int32_t *xchunk_out, int32_t *zdft_out,
+                      double *ms_per_step)
+{
+    if (!c || steps < 1 || !(delta_t > 0.0)) return DGPU_E_ARG;
+    if (c->courant_state != 0) return fail(c, DGPU_E_ARG, "a COURANT request is pending");
+    if (int rc = to_master(c)) return rc;
+    const size_t bytes = sizeof(cx<double>) * c->fs * c->nslot;
+    void *backup = nullptr;
+    HIPCHK(c, hipMalloc(&backup, bytes));
+    auto restore = [&]() -> int {
+        HIPCHK(c, hipMemcpyAsync(c->d_state, backup, bytes, hipMemcpyDeviceToDevice, c->stream));
+        c->master_dirty = true;
+        c->work_dirty = false;
+        return DGPU_OK;
+    };
+    int rc = DGPU_OK;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipMemcpyAsync(backup, c->d_state, bytes, hipMemcpyDeviceToDevice, c->stream) != hipSuccess ||
+        hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        rc = fail(c, DGPU_E_HIP, "autotune set-up");
+    const double dt = delta_t;
+    const double h_bar[3] = {dt * (8.0 / 15.0), dt * (2.0 / 15.0), dt * (5.0 / 15.0)};
+    const double beta_bar[3] = {1.0, 25.0 / 8.0, 9.0 / 4.0}, zeta_bar[3] = {0.0, -17.0 / 8.0, -5.0 / 4.0};
+    const int xchunk0 = c->xchunk;
+    const bool zdft0 = c->zdft;
+    const bool try_zdft = c->mixed && !c->les && c->g.nz <= 256 && c->g.nz % 16 == 0 && c->g.nx <= 256;
+    const int chunks[] = {4, 8, 16, 32, 64};
+    int best_x = xchunk0;
+    bool best_z = zdft0;
+    double best = 1e300;
+    for (int z = 0; z <= (try_zdft ? 1 : 0) && rc == DGPU_OK; ++z)
+        for (int xc : chunks) {
+            if (rc != DGPU_OK) break;
+            if ((rc = dgpu_set_zdft(c, z))) break;
+            c->xchunk = xc;
+            for (auto &ge : c->gexec)
+                if (ge) {
+                    hipGraphExecDestroy(ge);
+                    ge = nullptr;
+                }
+            if ((rc = restore())) break;
+            for (int k = 1; k <= 3 && rc == DGPU_OK; ++k) rc = dgpu_rk_chan_1(c, k, 0.0, h_bar, beta_bar, zeta_bar);
+            if (rc) break;
+            if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = fail(c, DGPU_E_HIP, "autotune event"); break; }
+            for (int n = 0; n < steps && rc == DGPU_OK; ++n)
+                for (int k = 1; k <= 3 && rc == DGPU_OK; ++k)
+                    rc = dgpu_rk_chan_1(c, k, 0.0, h_bar, beta_bar, zeta_bar);
+            if (rc) break;
+            float ms = 0.f;
+            if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                rc = fail(c, DGPU_E_HIP, "autotune timing");
+                break;
+            }
+            if (ms / steps < best) {
+                best = ms / steps;
+                best_x = xc;
+                best_z = z != 0;
+            }
+        }
+    // the chosen shape, and the state as it was
+    c->xchunk = rc == DGPU_OK ? best_x : xchunk0;
+    const int rz = dgpu_set_zdft(c, rc == DGPU_OK ? best_z : zdft0);
+    if (rc == DGPU_OK) rc = rz;
+    const int rr = restore();
+    if (rc == DGPU_OK) rc = rr;
+    if (hipStreamSynchronize(c->stream) != hipSuccess && rc == DGPU_OK) rc = fail(c, DGPU_E_HIP, "autotune sync");
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    hipFree(backup);
+    if (rc) return rc;
+    if (xchunk_out) *xchunk_out = c->xchunk;
+    if (zdft_out) *zdft_out = c->zdft ? 1 : 0;
+    if (ms_per_step) *ms_per_step = best;
+    return DGPU_OK;
+}
+
 int32_t dgpu_set_graphs(dgpu_ctx *c, int32_t enable)
 {
     if (!c) return DGPU_E_ARG;

@@ -57,7 +57,7 @@ EXPORTS = (
     "dgpu_ensemble_create", "dgpu_ensemble_destroy", "dgpu_ensemble_size", "dgpu_ensemble_member",
     "dgpu_ensemble_last_error", "dgpu_ensemble_set_graphs", "dgpu_ensemble_rk_chan_1",
     "dgpu_ensemble_synchronize", "dgpu_stats_chan", "dgpu_courant_request", "dgpu_courant",
-    "dgpu_get_flow_physical", "dgpu_set_flow_physical", "dgpu_tke_chan", "dgpu_les_chan", "dgpu_set_les", "dgpu_set_zdft",
+    "dgpu_get_flow_physical", "dgpu_set_flow_physical", "dgpu_tke_chan", "dgpu_les_chan", "dgpu_set_les", "dgpu_set_zdft", "dgpu_autotune",
 )
 
 # dgpu_stats_chan profiles, in the order of the header's DGPU_ST_*
@@ -127,6 +127,8 @@ def load_library(path: str = LIB_PATH):
     L.dgpu_set_les.argtypes = [vp, i32, i32]
     L.dgpu_set_zdft.restype = i32
     L.dgpu_set_zdft.argtypes = [vp, i32]
+    L.dgpu_autotune.restype = i32
+    L.dgpu_autotune.argtypes = [vp, i32, C.c_double, C.POINTER(i32), C.POINTER(i32), DP]
     L.dgpu_get_flow_physical.restype = i32
     L.dgpu_get_flow_physical.argtypes = [vp, i32, DP]
     L.dgpu_set_flow_physical.restype = i32
@@ -314,6 +316,14 @@ class GpuChannel:
         """Mixed precision: z transforms as split-BF16 DFT-as-GEMM on the
         matrix cores (NZ <= 256)."""
         self._chk(self.L.dgpu_set_zdft(self.h, int(bool(on))), "dgpu_set_zdft")
+
+    def autotune(self, steps: int = 3) -> dict:
+        """Times the launch-shape candidates on this context's state, keeps
+        the fastest; the state is left as it was."""
+        xc, zd, ms = C.c_int32(0), C.c_int32(0), C.c_double(0.0)
+        self._chk(self.L.dgpu_autotune(self.h, steps, self.delta_t, C.byref(xc), C.byref(zd), C.byref(ms)),
+                  "dgpu_autotune")
+        return {"xchunk": xc.value, "zdft": bool(zd.value), "ms_per_step": ms.value}
 
     def set_les(self, les: bool, model: bool = True):
         """RK_CHAN_1's LES switches: les = the eddy viscosity in the

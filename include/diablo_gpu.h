@@ -252,6 +252,18 @@ int32_t dgpu_set_les(dgpu_ctx *ctx, int32_t les, int32_t model);
  * multiple of 16; the x transforms stay FFTs.  on = 0 returns to the FFTs. */
 int32_t dgpu_set_zdft(dgpu_ctx *ctx, int32_t on);
 
+/* Launch-shape autotuner (no counterpart in the reference): times `steps` RK3
+ * steps of size delta_t (after one untimed step) of the context's own state
+ * for each candidate -- planes per block of the fused x kernel (4..64), and
+ * for mixed contexts with NX, NZ <= 256 the FFT or the MFMA z transforms --
+ * keeps the fastest and puts the state back as it was (the run does not
+ * advance).
+ * Outputs (may be NULL): the chosen planes per block, whether the MFMA z
+ * transforms are on, and the best ms per step.  Not with a COURANT request
+ * pending.  Allocates a copy of the state while it runs. */
+int32_t dgpu_autotune(dgpu_ctx *ctx, int32_t steps, double delta_t, int32_t *xchunk, int32_t *zdft,
+                      double *ms_per_step);
+
 /* Kernel timing: when enabled, every kernel launch is bracketed by HIP events
  * on the context's stream.  dgpu_kernel_times returns, per kernel kind
  * (0 zinv, 1 xfused, 2 zfwd_nl, 3 cn, 4 remdiv), the summed milliseconds and the
