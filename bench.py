@@ -262,17 +262,18 @@ def main():
     ap.add_argument("--cpu-baseline", choices=("on", "off"), default="on")
     ap.add_argument("--no-kernel-timing", action="store_true")
     ap.add_argument("--mixed", choices=("on", "off"), default="on")
-    ap.add_argument("--accuracy-steps", type=int, default=100)
+    ap.add_argument("--accuracy-steps", type=int, default=1000,
+                    help="steps of the mixed-vs-FP64 accuracy run (the 1000-step criterion)")
     ap.add_argument("--autotune", choices=("on", "off"), default="on",
                     help="pick the launch shapes with dgpu_autotune before timing (FP64 and mixed legs)")
     ap.add_argument("--les-steps", type=int, default=5, help="timed steps of the LES path (0: skip)")
     ap.add_argument("--zdft-steps", type=int, default=10,
                     help="timed steps of the 256x193x256 KH case with the MFMA z transforms (0: skip)")
-    ap.add_argument("--zdft-accuracy-steps", type=int, default=200)
+    ap.add_argument("--zdft-accuracy-steps", type=int, default=2000)
     ap.add_argument("--ensemble", type=int, default=64, help="members of the batched run (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=10)
     ap.add_argument("--ensemble-precision", default="mixed", choices=("mixed", "mixed_f64state", "fp64"))
-    ap.add_argument("--ensemble-accuracy-steps", type=int, default=100)
+    ap.add_argument("--ensemble-accuracy-steps", type=int, default=500)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB five times (reference point for HBM counters)")
     args = ap.parse_args()
