@@ -1108,6 +1108,27 @@ int32_t dgpu_les_chan(dgpu_ctx *c, double *nu_t)
     return DGPU_OK;
 }
 
+int32_t dgpu_tke_les(dgpu_ctx *c, double *eps_sgs)
+{
+    if (!c || !eps_sgs) return DGPU_E_ARG;
+    if (!c->les) return fail(c, DGPU_E_ARG, "tkebudget_chan_les needs LES on");
+    if (int rc = to_master(c)) return rc;
+    const dgpu::Geo &g = c->g;
+    const int NY = g.ny;
+    double *U[3] = {les_buf(c, 0), les_buf(c, 1), les_buf(c, 2)};
+    for (int q = 0; q < 3; ++q) {
+        dgpu::launch_les_pack(c->stream, g, c->cu[q], U[q], c->d_kx, c->d_kz, 0, 0, NY + 1);
+        if (int rc = check_launch(c, "les_pack")) return rc;
+        if (int rc = fort_to_physical(c, U[q], 0, NY + 1)) return rc;
+    }
+    double *out = les_buf(c, 6);
+    dgpu::launch_les_eps(c->stream, g, U[0], U[1], U[2], les_buf(c, 13), c->d_dy, c->d_dyf, out);
+    if (int rc = check_launch(c, "les_eps")) return rc;
+    HIPCHK(c, hipMemcpyAsync(eps_sgs, out, sizeof(double) * NY, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return DGPU_OK;
+}
+
 int32_t dgpu_set_les(dgpu_ctx *c, int32_t les, int32_t model)
 {
     if (!c) return DGPU_E_ARG;

@@ -289,6 +289,8 @@ void launch_les_cn(hipStream_t s, const Geo &g, const LesCn &a, const double *rh
                    const double *dy, const double *dyf, double *bm, double *gw, double *out);
 void launch_les_vmax(hipStream_t s, const Geo &g, const double *u1, const double *u2, const double *u3, double *out);
 void launch_les_add(hipStream_t s, cx<double> *dst, const cx<double> *src, long n);
+void launch_les_eps(hipStream_t s, const Geo &g, const double *u1, const double *u2, const double *u3,
+                    const double *nut, const double *dy, const double *dyf, double *out);
 struct LesCfFix {
     double nu, beta, i_ro, drhodx, drhodz;
     int jstart, jend, jstart_th, jend_th;

@@ -548,6 +548,44 @@ void launch_les_cf_rows(hipStream_t s, const Geo &g, const cx<double> *pre, cons
                        cf3, cfth, kx2, kz2, f);
 }
 
+// tkebudget_chan_les (for/save_stats.f:887-960): per plane j = 1..NY the sum
+// over the physical points of u_i times its implicit eddy-viscosity term;
+// out[j - 1].  Fixed-order reduction.
+__global__ void __launch_bounds__(256) k_les_eps(Geo g, const double *__restrict__ u1, const double *__restrict__ u2,
+                                                 const double *__restrict__ u3, const double *__restrict__ nut,
+                                                 const double *__restrict__ dy, const double *__restrict__ dyf,
+                                                 double *__restrict__ out)
+{
+    __shared__ double red[256];
+    const int j = 1 + blockIdx.x, tid = threadIdx.x;
+    const long n = (long)g.nx * g.nz;
+    double acc = 0.0;
+    for (long e = tid; e < n; e += 256) {
+        const int i = (int)(e % g.nx), k = (int)(e / g.nx);
+        const long x = fidx3(i, k, j, g.nx, g.nz), xp = fidx3(i, k, j + 1, g.nx, g.nz),
+                   xm = fidx3(i, k, j - 1, g.nx, g.nz);
+        const double t = u1[x] * ((nut[xp] * (u1[xp] - u1[x]) / dy[j + 1] - nut[x] * (u1[x] - u1[xm]) / dy[j]) / dyf[j]) +
+                         u3[x] * ((nut[xp] * (u3[xp] - u3[x]) / dy[j + 1] - nut[x] * (u3[x] - u3[xm]) / dy[j]) / dyf[j]) +
+                         u2[x] * ((0.5 * (nut[x] + nut[xp]) * (u2[xp] - u2[x]) / dyf[j] -
+                                   0.5 * (nut[x] + nut[xm]) * (u2[x] - u2[xm]) / dyf[j - 1]) /
+                                  dy[j]);
+        acc += t;
+    }
+    red[tid] = acc;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (tid < h) red[tid] += red[tid + h];
+        __syncthreads();
+    }
+    if (tid == 0) out[j - 1] = red[0];
+}
+
+void launch_les_eps(hipStream_t s, const Geo &g, const double *u1, const double *u2, const double *u3,
+                    const double *nut, const double *dy, const double *dyf, double *out)
+{
+    hipLaunchKernelGGL(k_les_eps, dim3(g.ny), dim3(256), 0, s, g, u1, u2, u3, nut, dy, dyf, out);
+}
+
 __global__ void k_les_add(cx<double> *__restrict__ dst, const cx<double> *__restrict__ src, long n)
 {
     const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;

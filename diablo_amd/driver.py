@@ -7,8 +7,9 @@ and END_RUN's stop conditions (wall time beyond TIME_LIMIT, a file named
 stop.now; for/diablo_io.f:412-435).
 
 Output files follow the reference's text layout where it has one: mean.txt,
-mean_th.txt (for/save_stats.f:430-447, 670-686) and tke.txt
-(for/save_stats.f:1222-1235).  Restart files are the .npz archives of
+mean_th.txt (for/save_stats.f:430-447, 670-686), tke.txt
+(for/save_stats.f:1222-1235) and, with LES, tke_les.txt
+(for/save_stats.f:940-958).  Restart files are the .npz archives of
 diablo_amd.restart (out.npz, end.npz with the pressure; start.npz is read
 when CREATE_NEW_FLOW is false).
 
@@ -69,6 +70,8 @@ class ChannelRun:
     # -- INITIALIZE (for/diablo_io.f:140-176) --
     def initialize(self):
         cfg, g = self.cfg, self.chan
+        if cfg.les:
+            g.set_les(True, model=False)      # NU_T = 0 until les_chan first runs
         if cfg.create_new_flow:
             g.initialize(*case.initial_physical(cfg, self.gy))
             self.log("A new flowfield has been created")
@@ -107,6 +110,13 @@ class ChannelRun:
             f.write(f" {self.time_step} {g.time!r} {g.delta_t!r}\n")
             for j in range(2, ny):
                 f.write(f"{j:3d} {self.gyf[j - 1]:20.9f} {eps[j - 1]:20.9f} \n")
+        if cfg.les:
+            # tkebudget_chan_les's text output (for/save_stats.f:940-958)
+            eps_sgs = g.tke_les()
+            with open(os.path.join(self.dir, "tke_les.txt"), "a") as f:
+                f.write(f" {self.time_step} {g.time!r} {g.delta_t!r}\n")
+                for j in range(1, ny + 1):
+                    f.write(f"{j:3d} {_fmt(self.gyf[j - 1])} {_fmt(eps_sgs[j - 1])} \n")
 
     # -- SAVE_FLOW (for/diablo_io.f:305-341) --
     def save_flow(self, final: bool):

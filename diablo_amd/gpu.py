@@ -57,7 +57,7 @@ EXPORTS = (
     "dgpu_ensemble_create", "dgpu_ensemble_destroy", "dgpu_ensemble_size", "dgpu_ensemble_member",
     "dgpu_ensemble_last_error", "dgpu_ensemble_set_graphs", "dgpu_ensemble_rk_chan_1",
     "dgpu_ensemble_synchronize", "dgpu_stats_chan", "dgpu_courant_request", "dgpu_courant",
-    "dgpu_get_flow_physical", "dgpu_set_flow_physical", "dgpu_tke_chan", "dgpu_les_chan", "dgpu_set_les", "dgpu_set_zdft", "dgpu_autotune",
+    "dgpu_get_flow_physical", "dgpu_set_flow_physical", "dgpu_tke_chan", "dgpu_les_chan", "dgpu_set_les", "dgpu_set_zdft", "dgpu_autotune", "dgpu_tke_les",
 )
 
 # dgpu_stats_chan profiles, in the order of the header's DGPU_ST_*
@@ -125,6 +125,8 @@ def load_library(path: str = LIB_PATH):
     L.dgpu_les_chan.argtypes = [vp, DP]
     L.dgpu_set_les.restype = i32
     L.dgpu_set_les.argtypes = [vp, i32, i32]
+    L.dgpu_tke_les.restype = i32
+    L.dgpu_tke_les.argtypes = [vp, DP]
     L.dgpu_set_zdft.restype = i32
     L.dgpu_set_zdft.argtypes = [vp, i32]
     L.dgpu_autotune.restype = i32
@@ -324,6 +326,13 @@ class GpuChannel:
         self._chk(self.L.dgpu_autotune(self.h, steps, self.delta_t, C.byref(xc), C.byref(zd), C.byref(ms)),
                   "dgpu_autotune")
         return {"xchunk": xc.value, "zdft": bool(zd.value), "ms_per_step": ms.value}
+
+    def tke_les(self) -> np.ndarray:
+        """tkebudget_chan_les's epsilon_sgs at J = 1..NY (LES on; J = 1 is NaN
+        as in the reference)."""
+        out = np.zeros(self.cfg.ny)
+        self._chk(self.L.dgpu_tke_les(self.h, _dp(out)), "dgpu_tke_les")
+        return out
 
     def set_les(self, les: bool, model: bool = True):
         """RK_CHAN_1's LES switches: les = the eddy viscosity in the

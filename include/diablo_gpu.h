@@ -244,6 +244,14 @@ int32_t dgpu_les_chan(dgpu_ctx *ctx, double *nu_t);
  * captured in graphs. */
 int32_t dgpu_set_les(dgpu_ctx *ctx, int32_t les, int32_t model);
 
+/* tkebudget_chan_les (for/save_stats.f:887-960), as SAVE_STATS_CHAN calls it
+ * under LES: eps_sgs[J - 1], J = 1..NY, the sum over the physical points of
+ * plane J of u_i times its implicitly treated eddy-viscosity term, from the
+ * device state and NU_T, in FP64 (tke.h5's 'epsilon_sgs').  eps_sgs[0] is
+ * NaN, as in the reference: its U2 term divides 0 by DYF(0), which is never
+ * set.  LES must be on (dgpu_set_les). */
+int32_t dgpu_tke_les(dgpu_ctx *ctx, double *eps_sgs);
+
 /* The optional DFT-as-GEMM z transforms of a DGPU_MIXED / DGPU_MIXED_F64STATE
  * context (no counterpart in the reference, which transforms with FFTW,
  * for/mpi.f:1530-1674): the substep's z backward and z forward (with the

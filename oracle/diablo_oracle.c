@@ -1942,3 +1942,43 @@ void dor_les_chan(dor_state *s, double *nu_t)
     free(U1b); free(U3b); free(U1); free(U3); free(S1); free(TEMP); free(NUT);
     free(DELTA_YF); free(DELTA_Y); free(T2);
 }
+
+/* tkebudget_chan_les (for/save_stats.f:887-960): epsilon_sgs(J), J = 1..NY,
+ * the sum over the physical points of plane J of u_i times the implicitly
+ * treated eddy-viscosity term of u_i, from the velocity as SAVE_STATS_CHAN
+ * takes it to physical space (for/save_stats.f:163-170) and the current
+ * NU_T.  Needs LES on (NU_T allocated).  Plane 1 is NaN, as in the
+ * reference: its U2 term divides 0 by DYF(0), which is never set. */
+void dor_tke_les(dor_state *s, double *eps)
+{
+    const int NY = s->NY, NZP = s->NZP, NXM = s->NXM;
+    const double *DY = s->DY, *DYF = s->DYF;
+    const size_t fs = s->fsize;
+    const double *NT = s->fld[DOR_NUT];
+    double *U1 = zalloc(fs), *U2 = zalloc(fs), *U3 = zalloc(fs);
+    memcpy(U1, s->fld[DOR_U1], sizeof(double) * fs);
+    memcpy(U2, s->fld[DOR_U2], sizeof(double) * fs);
+    memcpy(U3, s->fld[DOR_U3], sizeof(double) * fs);
+    dor_fft_xz_to_physical(s, U1, 0, NY + 1);
+    dor_fft_xz_to_physical(s, U2, 0, NY + 1);
+    dor_fft_xz_to_physical(s, U3, 0, NY + 1);
+#define AT(A, dj) A[RIDX(i, K, J + (dj))]
+    for (int J = 1; J <= NY; J++) {
+        double e = 0.0;
+        for (int i = 0; i <= NXM; i++)
+            for (int K = 0; K <= NZP - 1; K++) {
+                const double t =
+                    AT(U1, 0) * ((AT(NT, 1) * (AT(U1, 1) - AT(U1, 0)) / DY[J + 1]
+                                  - AT(NT, 0) * (AT(U1, 0) - AT(U1, -1)) / DY[J]) / DYF[J])
+                    + AT(U3, 0) * ((AT(NT, 1) * (AT(U3, 1) - AT(U3, 0)) / DY[J + 1]
+                                    - AT(NT, 0) * (AT(U3, 0) - AT(U3, -1)) / DY[J]) / DYF[J])
+                    + AT(U2, 0) * ((0.5 * (AT(NT, 0) + AT(NT, 1)) * (AT(U2, 1) - AT(U2, 0)) / DYF[J]
+                                    - 0.5 * (AT(NT, 0) + AT(NT, -1)) * (AT(U2, 0) - AT(U2, -1)) / DYF[J - 1])
+                                   / DY[J]);
+                e = e + t;
+            }
+        eps[J - 1] = e;
+    }
+#undef AT
+    free(U1); free(U2); free(U3);
+}

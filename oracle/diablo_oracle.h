@@ -116,6 +116,7 @@ void dor_stats_chan(dor_state *s, double *out, double *ubulk);
  * reference uses CS1/S1 (its plane 0 is read unset, so eps[0] depends on
  * what S1 held); the velocity state is not modified. */
 void dor_tke_chan(dor_state *s, double *eps);
+void dor_tke_les(dor_state *s, double *eps);
 
 /* WriteHDF5 / ReadHDF5's field layout (for/hdf5.f:172-268, 470-534): id is
  * DOR_U1, DOR_U2, DOR_U3, DOR_P or DOR_CTH0 + n; the file array is

@@ -89,6 +89,7 @@ def lib():
         L.dor_c2r.argtypes = [dp, dp, i]
         L.dor_stats_chan.argtypes = [vp, dp, dp]
         L.dor_tke_chan.argtypes = [vp, dp]
+        L.dor_tke_les.argtypes = [vp, dp]
         L.dor_les_chan.argtypes = [vp, dp]
         L.dor_set_les.argtypes = [vp, i, i]
         _lib = L
@@ -229,6 +230,12 @@ class Oracle:
         scratch's previous contents, as in the reference)."""
         out = np.zeros(self.cfg.ny)
         self.L.dor_tke_chan(self.h, _dp(out))
+        return out
+
+    def tke_les(self) -> np.ndarray:
+        """tkebudget_chan_les's epsilon_sgs at J = 1..NY (LES on)."""
+        out = np.zeros(self.cfg.ny)
+        self.L.dor_tke_les(self.h, _dp(out))
         return out
 
     def les_chan(self) -> np.ndarray:

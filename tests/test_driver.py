@@ -100,3 +100,10 @@ def test_driver_les_spinup(tmp_path):
     a, b = r.chan.state(), ref.state()
     for k in ("CU1", "CU2", "CU3", "CP", "CTH0"):
         assert np.array_equal(a[k], b[k]), k
+    # tke_les.txt: the initial block (NU_T = 0) and the final one
+    lines = open(os.path.join(d, "tke_les.txt")).read().splitlines()
+    assert len(lines) == 2 * (1 + cfg.ny)
+    first = [float(x.split()[2]) for x in lines[2:1 + cfg.ny]]
+    assert all(v == 0.0 for v in first)
+    last = np.array([float(x.split()[2]) for x in lines[cfg.ny + 3:]])
+    assert np.all(np.isfinite(last)) and np.abs(last).max() > 0.0

@@ -237,3 +237,20 @@ def test_les_path_at_full_size(gpu_mod):
     scale = np.abs(cu2[j + 1] - cu2[j]).max() / dyf[1:cfg.ny].min() + np.abs(kxk * cu1[j]).max()
     say("div", np.abs(div).max() / scale)
     assert np.abs(div).max() < 1e-10 * scale
+
+
+@pytest.mark.parametrize("maker", ["channel", "sink"])
+def test_tke_les_matches_oracle(oracle_mod, gpu_mod, maker):
+    """tkebudget_chan_les's epsilon_sgs after two LES steps, on planes 2..NY
+    (plane 1 is NaN in both, as in the reference)."""
+    cfg = CASES[maker]()
+    say("tke_les", maker)
+    o, g = _les_pair(oracle_mod, gpu_mod, cfg)
+    for _ in range(2):
+        o.step()
+        g.step(1)
+    eo, eg = o.tke_les(), g.tke_les()
+    assert np.isnan(eo[0]) and np.isnan(eg[0])
+    err = np.abs(eg[1:] - eo[1:]).max() / np.abs(eo[1:]).max()
+    say("epsilon_sgs", err, np.abs(eo[1:]).max())
+    assert np.abs(eo[1:]).max() > 0.0 and err < 1e-11, err

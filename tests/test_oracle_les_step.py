@@ -115,3 +115,40 @@ def test_laminar_profile_fixed_point_with_model(oracle_mod):
     k = case.kept(cfg, o.field(oracle_mod.U1)).copy()
     k[:, 0, 0] = 0.0
     assert np.abs(k).max() < 1e-15
+
+
+def test_tke_les_summation_by_parts(oracle_mod):
+    """tkebudget_chan_les on mean profiles with a y-dependent NU_T (zero on
+    GHOST_LES's rows): summed over the column with the staggered weights, the
+    SGS work of each velocity component equals minus the eddy dissipation
+    of its discrete gradient (summation by parts), which fixes the face
+    placement of NU_T in both stencils."""
+    cfg = case.channel_case(16, 25, 16, kick=0.0)
+    ny, n = cfg.ny, cfg.nx * cfg.nz
+    o = oracle_mod.Oracle(cfg)
+    _zero_state(o, oracle_mod, cfg)
+    o.set_les(True, model=False)
+    gy = np.concatenate([[0.0], case.grid_gy(cfg)])                    # GY(0..NY+1), GY(0) unused
+    nt = np.zeros(ny + 2)
+    nt[3:ny] = 0.02 * (1.5 + np.sin(2.0 * gy[3:ny]))
+    o.field(oracle_mod.NUT)[...] = nt[:, None, None]
+    dy, dyf = o.vec(oracle_mod.DY), o.vec(oracle_mod.DYF)
+    rng = np.random.default_rng(3)
+    u = rng.standard_normal(ny + 2)
+    case.fourier_view(o.field(oracle_mod.U1))[:, 0, 0] = u
+    eps = o.tke_les()
+    # plane 1: the reference's U2 term divides 0 by DYF(0), which it never
+    # sets, so epsilon_sgs(1) is NaN there too; its exact value is 0 (NU_T
+    # vanishes on the faces of plane 1), so the sums start at plane 2
+    assert np.isnan(eps[0])
+    lhs = float(np.sum(dyf[2:ny + 1] * eps[1:]))
+    rhs = -n * float(np.sum(nt[2:ny + 1] * (u[2:ny + 1] - u[1:ny]) ** 2 / dy[2:ny + 1]))
+    assert rhs < 0 and abs(lhs - rhs) < 1e-12 * abs(rhs), (lhs, rhs)
+    # the wall-normal component on GY rows, NU_T averaged onto the GYF faces
+    case.fourier_view(o.field(oracle_mod.U1))[...] = 0.0
+    case.fourier_view(o.field(oracle_mod.U2))[:, 0, 0] = u
+    eps = o.tke_les()
+    lhs = float(np.sum(dy[2:ny + 1] * eps[1:]))
+    nf = 0.5 * (nt[1:ny] + nt[2:ny + 1])                                 # faces GYF(1..NY-1)
+    rhs = -n * float(np.sum(nf * (u[2:ny + 1] - u[1:ny]) ** 2 / dyf[1:ny]))
+    assert rhs < 0 and abs(lhs - rhs) < 1e-12 * abs(rhs), (lhs, rhs)
