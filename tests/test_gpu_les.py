@@ -15,6 +15,7 @@ CASES = {
     "channel": lambda: case.channel_case(48, 33, 36, kick=0.3, streak=0.1),
     "kh": lambda: case.kh_case(32, 33, 24, kick=0.05),
     "sink": lambda: kitchen_sink(48, 25, 36),
+    "no_scalar": lambda: case.channel_case(48, 33, 36, kick=0.3, streak=0.1, n_th=0),
 }
 
 

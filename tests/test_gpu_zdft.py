@@ -80,6 +80,27 @@ def test_zdft_against_fp64(gpu_mod):
     assert np.isfinite(e) and e < 1e-5, e
 
 
+def test_zdft_without_scalar(gpu_mod):
+    """N_TH = 0: three fields through the GEMM z transforms."""
+    cfg = case.channel_case(96, 33, 96, kick=0.3, streak=0.1, n_th=0)
+    say("zdft n_th=0")
+    a = gpu_mod.GpuChannel(cfg, precision="mixed")
+    a.initialize(*case.initial_physical(cfg))
+    a.step(1)
+    st = a.state()
+    b = gpu_mod.GpuChannel(cfg, precision="mixed")
+    b.load_state(st)
+    b.time = a.time
+    b.set_zdft(True)
+    a.step(1)
+    b.step(1)
+    sa, sb = a.state(), b.state()
+    for k in ("CU1", "CU2", "CU3"):
+        e = _rel_l2(cfg, sb[k], sa[k])
+        say(k, e)
+        assert e < 1e-5, (k, e)
+
+
 def test_zdft_graph_replays_match_eager(gpu_mod):
     cfg = case.kh_case(64, 33, 64, kick=0.05, dt=2e-3)
     say("zdft graphs")
