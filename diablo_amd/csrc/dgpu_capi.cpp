@@ -1013,8 +1013,10 @@ int32_t dgpu_set_zdft(dgpu_ctx *c, int32_t on)
 // ---------------------------------------------------------------------------
 // Work planes, Fortran layout (NX+2) x (NZ+2) x (NY+2) doubles each:
 //   0-7   S_ij and U1, U3 for the model; in the substep physical U1, U2, U3,
-//         R1, R2, R3 (0-5) and the Thomas scratch (6, 7)
-//   8-12  the stress parts to transform; in the substep CS1 (8), TH (9), RTH (10)
+//         R1, R2, R3 (0-5) and Thomas scratch (6, 7)
+//   8-12  the stress parts to transform; in the substep CS1 (8), TH (9), RTH
+//         (10), Thomas scratch (8, 11, 12, with 14-16 once the SGS force is
+//         spent)
 //   13    NU_T (kept between substeps; KAPPA_T equals it)
 //   14-17 the SGS force for CF1, CF2, CF3, CFTH (kept modes)
 //   18    rows 1 and NY of CU1, CU3 before les_chan's BCs (kept modes)
@@ -1584,30 +1586,36 @@ static int les_substep(dgpu_ctx *c, int rk_step, double time, double temp1, doub
         dgpu::launch_les_explicit(c->stream, g, rthp, thp, nut, c->d_dy, c->d_dyf, temp2, 0, c->jstart_th[0],
                                   c->jend_th[0]);
     if (int rc = check_launch(c, "les_explicit")) return rc;
-    // implicit solves: TH, U2, U1, U3 (for/channel.f:762-1029)
+    // implicit solves: TH, U2, U1, U3 (for/channel.f:762-1029), independent,
+    // in one launch; scratch planes: 6, 7 (work planes of the model), 8, 11,
+    // 12, and 14-16 (the SGS force, already added to the nonlinear terms)
     const double nuv = c->p.nu_v_scale;
-    auto solve = [&](int sys, int jl, int jh, int lo, int hi, double clo, double chi, double cmol, const double *rhs,
-                     double *out) {
-        dgpu::LesCn a;
+    double *scratch[8] = {bm, gw, les_buf(c, 8), les_buf(c, 11), les_buf(c, 12), les_buf(c, 14), les_buf(c, 15),
+                          les_buf(c, 16)};
+    dgpu::LesCnSet set{};
+    auto add = [&](int sys, int jl, int jh, int lo, int hi, double clo, double chi, double cmol, const double *rhs,
+                   double *out) {
+        dgpu::LesCn &a = set.a[set.n];
         a.sys = sys; a.ny = NY; a.jlo = jl; a.jhi = jh; a.clo = jl - 1; a.chi = jh + 1;
         if (sys == dgpu::LES_SYS_U2) { a.clo = 1; a.chi = NY + 1; }
         a.lo = lo; a.hi = hi; a.cmol = cmol; a.temp2 = temp2; a.clo_c = clo; a.chi_c = chi;
-        dgpu::launch_les_cn(c->stream, g, a, rhs, nut, c->d_dy, c->d_dyf, bm, gw, out);
-        return check_launch(c, "les_cn");
+        set.rhs[set.n] = rhs;
+        set.out[set.n] = out;
+        set.bm[set.n] = scratch[2 * set.n];
+        set.gw[set.n] = scratch[2 * set.n + 1];
+        ++set.n;
     };
     if (g.nth)
-        if (int rc = solve(dgpu::LES_SYS_TH, c->jstart_th[0], c->jend_th[0], c->p.th_bc_ymin[0], c->p.th_bc_ymax[0],
-                           c->p.th_bc_ymin_c1[0], c->p.th_bc_ymax_c1[0], temp1 / c->p.pr[0] * nuv, rthp, thp))
-            return rc;
-    if (int rc = solve(dgpu::LES_SYS_U2, 2, NY, c->p.v_bc_ymin, c->p.v_bc_ymax, c->p.v_bc_ymin_c1,
-                       c->p.v_bc_ymax_c1, temp1 * nuv, Rb[1], U[1]))
-        return rc;
-    if (int rc = solve(dgpu::LES_SYS_U13, c->jstart, c->jend, c->p.u_bc_ymin, c->p.u_bc_ymax, c->p.u_bc_ymin_c1,
-                       c->p.u_bc_ymax_c1, temp1 * nuv, Rb[0], U[0]))
-        return rc;
-    if (int rc = solve(dgpu::LES_SYS_U13, c->jstart, c->jend, c->p.w_bc_ymin, c->p.w_bc_ymax, c->p.w_bc_ymin_c1,
-                       c->p.w_bc_ymax_c1, temp1 * nuv, Rb[2], U[2]))
-        return rc;
+        add(dgpu::LES_SYS_TH, c->jstart_th[0], c->jend_th[0], c->p.th_bc_ymin[0], c->p.th_bc_ymax[0],
+            c->p.th_bc_ymin_c1[0], c->p.th_bc_ymax_c1[0], temp1 / c->p.pr[0] * nuv, rthp, thp);
+    add(dgpu::LES_SYS_U2, 2, NY, c->p.v_bc_ymin, c->p.v_bc_ymax, c->p.v_bc_ymin_c1, c->p.v_bc_ymax_c1, temp1 * nuv,
+        Rb[1], U[1]);
+    add(dgpu::LES_SYS_U13, c->jstart, c->jend, c->p.u_bc_ymin, c->p.u_bc_ymax, c->p.u_bc_ymin_c1, c->p.u_bc_ymax_c1,
+        temp1 * nuv, Rb[0], U[0]);
+    add(dgpu::LES_SYS_U13, c->jstart, c->jend, c->p.w_bc_ymin, c->p.w_bc_ymax, c->p.w_bc_ymin_c1, c->p.w_bc_ymax_c1,
+        temp1 * nuv, Rb[2], U[2]);
+    dgpu::launch_les_cn(c->stream, g, set, nut, c->d_dy, c->d_dyf);
+    if (int rc = check_launch(c, "les_cn")) return rc;
     // COURANT on the physical velocity after the solves (for/channel.f:1037-1040)
     if (courant) {
         dgpu::launch_les_vmax(c->stream, g, U[0], U[1], U[2], c->d_vmax);

@@ -285,8 +285,16 @@ void launch_les_acc(hipStream_t s, const Geo &g, cx<double> *dst, const double *
                     const double *kz, int kmul, int j0, int j1);
 void launch_les_explicit(hipStream_t s, const Geo &g, double *r, const double *u, const double *nut,
                          const double *dy, const double *dyf, double temp2, int sys, int j0, int j1);
-void launch_les_cn(hipStream_t s, const Geo &g, const LesCn &a, const double *rhs, const double *nut,
-                   const double *dy, const double *dyf, double *bm, double *gw, double *out);
+// Up to four independent solves (TH, U2, U1, U3), each with its right-hand
+// side, output and two scratch planes.
+struct LesCnSet {
+    LesCn a[4];
+    const double *rhs[4];
+    double *out[4], *bm[4], *gw[4];
+    int n;
+};
+void launch_les_cn(hipStream_t s, const Geo &g, const LesCnSet &set, const double *nut, const double *dy,
+                   const double *dyf);
 void launch_les_vmax(hipStream_t s, const Geo &g, const double *u1, const double *u2, const double *u3, double *out);
 void launch_les_add(hipStream_t s, cx<double> *dst, const cx<double> *src, long n);
 void launch_les_eps(hipStream_t s, const Geo &g, const double *u1, const double *u2, const double *u3,

@@ -436,13 +436,20 @@ __device__ __forceinline__ void les_row(const LesCn &a, const double *__restrict
     }
 }
 
-__global__ void k_les_cn(Geo g, LesCn a, const double *__restrict__ rhs, const double *__restrict__ nut,
-                         const double *__restrict__ dy, const double *__restrict__ dyf, double *__restrict__ bm,
-                         double *__restrict__ gw, double *__restrict__ out)
+// grid (column blocks, systems): every system of the substep in one launch
+// (they are independent), so the sweeps of all four fill the machine.
+__global__ void k_les_cn(Geo g, LesCnSet set, const double *__restrict__ nut, const double *__restrict__ dy,
+                         const double *__restrict__ dyf)
 {
     const long n = (long)g.nx * g.nz;
     const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
+    const int sy = blockIdx.y;
+    const LesCn &a = set.a[sy];
+    const double *__restrict__ rhs = set.rhs[sy];
+    double *__restrict__ bm = set.bm[sy];
+    double *__restrict__ gw = set.gw[sy];
+    double *__restrict__ out = set.out[sy];
     const int i = (int)(e % g.nx), k = (int)(e / g.nx);
     const int NY = g.ny;
     auto X = [&](int j) { return fidx3(i, k, j, g.nx, g.nz); };
@@ -635,12 +642,11 @@ void launch_les_explicit(hipStream_t s, const Geo &g, double *r, const double *u
                        dy, dyf, temp2, sys, j0);
 }
 
-void launch_les_cn(hipStream_t s, const Geo &g, const LesCn &a, const double *rhs, const double *nut,
-                   const double *dy, const double *dyf, double *bm, double *gw, double *out)
+void launch_les_cn(hipStream_t s, const Geo &g, const LesCnSet &set, const double *nut, const double *dy,
+                   const double *dyf)
 {
     const long n = (long)g.nx * g.nz;
-    hipLaunchKernelGGL(k_les_cn, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g, a, rhs, nut, dy, dyf, bm, gw,
-                       out);
+    hipLaunchKernelGGL(k_les_cn, dim3((unsigned)((n + 255) / 256), set.n), dim3(256), 0, s, g, set, nut, dy, dyf);
 }
 
 void launch_les_vmax(hipStream_t s, const Geo &g, const double *u1, const double *u2, const double *u3, double *out)
