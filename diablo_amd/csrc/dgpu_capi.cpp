@@ -1012,8 +1012,8 @@ int32_t dgpu_set_zdft(dgpu_ctx *c, int32_t on)
 // LES (for/les.f, for/channel.f:303-386, 628-659, 748-1003)
 // ---------------------------------------------------------------------------
 // Work planes, Fortran layout (NX+2) x (NZ+2) x (NY+2) doubles each:
-//   0-7   S_ij and U1, U3 for the model; in the substep physical U1, U2, U3,
-//         R1, R2, R3 (0-5) and Thomas scratch (6, 7)
+//   0-7   S_ij (3-7, 1) and U1, U3 (0, 2) for the model; in the substep
+//         physical U1, U2, U3, R1, R2, R3 (0-5) and Thomas scratch (6, 7)
 //   8-12  the stress parts to transform; in the substep CS1 (8), TH (9), RTH
 //         (10), Thomas scratch (8, 11, 12, with 14-16 once the SGS force is
 //         spent)
@@ -1067,7 +1067,10 @@ static int les_model(dgpu_ctx *c, cx<double> *f1, cx<double> *f2, cx<double> *f3
     const size_t n = les_n(c);
     const double *delta_yf = c->d_les + kLesBufs * n, *delta_y = delta_yf + (NY + 2);
     dgpu::LesPtrs S, O;
-    for (int q = 0; q < 8; ++q) S.p[q] = les_buf(c, q);
+    // S_ij in planes 3-7 and 1, U1 and U3 in planes 0 and 2 (where the
+    // substep keeps its physical U1, U3)
+    const int splane[8] = {3, 4, 5, 6, 7, 1, 0, 2};
+    for (int q = 0; q < 8; ++q) S.p[q] = les_buf(c, splane[q]);
     for (int q = 0; q < 6; ++q) O.p[q] = les_buf(c, 8 + q);
     O.p[6] = O.p[7] = nullptr;
     dgpu::launch_apply_bc_vel(c->stream, g, c->cu[0], c->cu[1], c->cu[2], c->d_codes, c->d_c1, c->d_dy, c->d_dyf);
@@ -1510,6 +1513,10 @@ static int les_substep(dgpu_ctx *c, int rk_step, double time, double temp1, doub
         b.w_hi_c = c->p.w_bc_ymax_c1; b.v_lo_c = c->p.v_bc_ymin_c1; b.v_hi_c = c->p.v_bc_ymax_c1;
         dgpu::launch_les_bc_phys(c->stream, g, c->cu[0], c->cu[1], c->cu[2], b, c->d_dy, c->d_dyf);
         if (int rc = check_launch(c, "les_bc_phys")) return rc;
+        // and on the model's physical U1, U3 (planes 0, 2), as the reference
+        // applies it to the arrays les_chan left in physical space
+        dgpu::launch_les_bc_phys_planes(c->stream, g, U[0], U[2], b, c->d_dy);
+        if (int rc = check_launch(c, "les_bc_phys_planes")) return rc;
         // KAPPA_T = NU_T and the scalar's horizontal eddy diffusion, explicit
         // (for/channel.f:313-375)
         for (int nth = 0; nth < g.nth; ++nth)
@@ -1566,9 +1573,11 @@ static int les_substep(dgpu_ctx *c, int rk_step, double time, double temp1, doub
     // physical right-hand sides and fields
     for (int q = 0; q < 3; ++q) {
         dgpu::launch_les_pack(c->stream, g, c->nl[q], Rb[q], c->d_kx, c->d_kz, 0, 0, NY + 1);
-        dgpu::launch_les_pack(c->stream, g, c->cu[q], U[q], c->d_kx, c->d_kz, 0, 0, NY + 1);
         if (int rc = check_launch(c, "les_pack")) return rc;
         if (int rc = fort_to_physical(c, Rb[q], 0, NY + 1)) return rc;
+        if (c->les_model && q != 1) continue;  // the model's U1, U3 with their BCs
+        dgpu::launch_les_pack(c->stream, g, c->cu[q], U[q], c->d_kx, c->d_kz, 0, 0, NY + 1);
+        if (int rc = check_launch(c, "les_pack")) return rc;
         if (int rc = fort_to_physical(c, U[q], 0, NY + 1)) return rc;
     }
     if (g.nth) {

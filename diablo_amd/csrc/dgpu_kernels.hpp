@@ -278,6 +278,7 @@ struct LesCn {
 };
 void launch_les_bc_phys(hipStream_t s, const Geo &g, cx<double> *cu1, cx<double> *cu2, cx<double> *cu3,
                         const LesBc &b, const double *dy, const double *dyf);
+void launch_les_bc_phys_planes(hipStream_t s, const Geo &g, double *u1, double *u3, const LesBc &b, const double *dy);
 void launch_les_pack(hipStream_t s, const Geo &g, const cx<double> *src, double *fort, const double *kx,
                      const double *kz, int kmul, int j0, int j1);
 void launch_les_mul(hipStream_t s, const Geo &g, double *a, const double *nut, int j0, int j1);

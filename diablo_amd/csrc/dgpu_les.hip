@@ -293,6 +293,43 @@ __global__ void k_les_bc_phys(Geo g, cx<double> *__restrict__ cu1, cx<double> *_
     }
 }
 
+// APPLY_BC_VEL_PHYS_LOWER / _UPPER's U1 and U3 rows (for/channel.f:2495-2640)
+// on physical Fortran-layout planes.
+__global__ void k_les_bc_phys_planes(Geo g, double *__restrict__ u1, double *__restrict__ u3, LesBc b,
+                                     const double *__restrict__ dy)
+{
+    const long n = (long)g.nx * g.nz;
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const int i = (int)(e % g.nx), k = (int)(e / g.nx), NY = g.ny;
+    auto X = [&](int j) { return fidx3(i, k, j, g.nx, g.nz); };
+    for (int q = 0; q < 2; ++q) {
+        double *u = q == 0 ? u1 : u3;
+        const int lo = q == 0 ? b.u_lo : b.w_lo, hi = q == 0 ? b.u_hi : b.w_hi;
+        const double clo = q == 0 ? b.u_lo_c : b.w_lo_c, chi = q == 0 ? b.u_hi_c : b.w_hi_c;
+        if (lo == 0) {
+            u[X(0)] = clo;
+            u[X(1)] = clo;
+        } else {
+            u[X(1)] = u[X(2)] - dy[2] * clo;
+            u[X(0)] = u[X(1)] - dy[1] * clo;
+        }
+        if (hi == 0) {
+            u[X(NY)] = chi;
+            u[X(NY + 1)] = chi;
+        } else {
+            u[X(NY)] = u[X(NY - 1)] + dy[NY] * chi;
+            u[X(NY + 1)] = u[X(NY)] + dy[NY] * chi;
+        }
+    }
+}
+
+void launch_les_bc_phys_planes(hipStream_t s, const Geo &g, double *u1, double *u3, const LesBc &b, const double *dy)
+{
+    const long n = (long)g.nx * g.nz;
+    hipLaunchKernelGGL(k_les_bc_phys_planes, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g, u1, u3, b, dy);
+}
+
 // Kept modes (plane stride ps) -> Fortran-layout Fourier planes, times i kx
 // (kmul 1) or i kz (kmul 2); planes outside j0..j1 and slots outside the
 // kept modes are zero.
