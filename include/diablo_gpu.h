@@ -241,7 +241,9 @@ int32_t dgpu_les_chan(dgpu_ctx *ctx, double *nu_t);
  * velocity BCs (for/channel.f:2495-2640), KAPPA_T = NU_T and the scalar's
  * horizontal eddy diffusion (for/channel.f:313-375).  NU_T starts at zero and
  * keeps its last value while the model is off.  LES substeps are not
- * captured in graphs. */
+ * captured in graphs.  The first call with les = 1 allocates 19
+ * Fortran-layout work planes, (NX+2)(NZ+2)(NY+2) doubles each (5.9 GB at
+ * 384 x 257 x 384), held until dgpu_destroy. */
 int32_t dgpu_set_les(dgpu_ctx *ctx, int32_t les, int32_t model);
 
 /* tkebudget_chan_les (for/save_stats.f:887-960), as SAVE_STATS_CHAN calls it
