@@ -71,14 +71,22 @@ __device__ __forceinline__ void zinv_body(const IpLen &fz, const cx<T> *__restri
     });
 }
 
+// grid (8 * nchunks * ceil(nplanes / 8), 1, fields): workgroups go to the
+// eight XCDs in turn, so block L runs on XCD L % 8; all column chunks of a
+// plane are given to one XCD, one after another, because neighbouring chunks
+// share the cache lines of every input row (a row of NKX+1 modes does not
+// start on a line boundary) and then find them in that XCD's L2.
 template <typename T, int E, typename TIN, int PID>
 __global__ void __launch_bounds__(256) k_zinv_ip(IpLen fz, const cx<T> *__restrict__ tw,
                                                   const cx<TIN> *__restrict__ in, long in_fs, long in_ps, int mk,
                                                   cx<T> *__restrict__ out, long out_fs, long out_ps, int ld, int j0,
-                                                  int nkz, int tnkz, int lw)
+                                                  int nkz, int tnkz, int lw, int nchunks, int nplanes)
 {
+    const int L = blockIdx.x, r = L >> 3;
+    const int chunk = r % nchunks, plane = 8 * (r / nchunks) + (L & 7);
+    if (plane >= nplanes) return;
     zinv_body<T, E, TIN, PID != 0>(plan_ref<PID>(fz), tw, in, in_fs, in_ps, mk, out, out_fs, out_ps, ld, j0, nkz,
-                                   tnkz, lw, blockIdx.x, blockIdx.y, blockIdx.z);
+                                   tnkz, lw, chunk, plane, blockIdx.z);
 }
 
 // Members of one transform-length group: grid.z = member * nfields + field,
@@ -227,10 +235,11 @@ static void zinv_ip_e(hipStream_t s, const Geo &g, const IpLen &fz, const cx<T> 
                       long in_ps, cx<T> *out, long out_fs, long out_ps, int ld, int j0, int nplanes, int nfields)
 {
     const int lw = z_lw(fz.t), W = 1 << lw;
-    dim3 grid((g.mk + W - 1) / W, nplanes, nfields);
+    const int nchunks = (g.mk + W - 1) / W;
+    dim3 grid((unsigned)(8 * nchunks * ((nplanes + 7) / 8)), 1, nfields);
     const size_t lds = sizeof(cx<T>) * W * fz.n;
     hipLaunchKernelGGL((k_zinv_ip<T, E, TIN, PID>), grid, dim3(W * fz.t), lds, s, fz, tw, in, in_fs, in_ps, g.mk, out,
-                       out_fs, out_ps, ld, j0, g.nkz, g.tnkz, lw);
+                       out_fs, out_ps, ld, j0, g.nkz, g.tnkz, lw, nchunks, nplanes);
 }
 
 template <typename T, typename TIN>
