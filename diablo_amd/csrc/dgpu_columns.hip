@@ -27,6 +27,13 @@ __device__ __forceinline__ cx<T> neg_scale(T a, cx<T> x) { return make_cx<T>(-(a
 
 __device__ __forceinline__ int wave_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Workgroups go to the eight XCDs in turn (block L to XCD L % 8).  For a grid
+// padded to a multiple of 8, this gives each XCD one contiguous range of
+// column blocks, in order: neighbouring blocks then share, in one L2, the
+// cache line their rows straddle (planes of ps modes do not start on line
+// boundaries).
+__device__ __forceinline__ int xcd_contiguous(int L, int nb8) { return (L & 7) * (nb8 >> 3) + (L >> 3); }
+
 // A sweep over planes j = j0, j0+dir, ... (n planes) whose per-plane inputs
 // do not depend on the recurrence: the inputs of plane j are loaded D planes
 // ahead into a ring of registers (ld(j) must be safe for any j in range, the
@@ -261,7 +268,7 @@ __device__ __forceinline__ void cn_body(AT A, PT P, int bx)
 template <typename T, int NTH, typename NLT>
 __global__ void __launch_bounds__(256, 3) k_cn(CnArgs A, CnPtrs<T, NLT> P)
 {
-    cn_body<T, NTH, NLT, const CnArgs, const CnPtrs<T, NLT>>(A, P, blockIdx.x);
+    cn_body<T, NTH, NLT, const CnArgs, const CnPtrs<T, NLT>>(A, P, xcd_contiguous(blockIdx.x, gridDim.x));
 }
 
 // The mixed path's low-mode columns: a block or two, so registers are free.
@@ -445,7 +452,7 @@ __device__ __forceinline__ void remdiv_body(const RdArgs &A, const CnPtrs<T, NLT
 template <typename T, typename NLT>
 __global__ void __launch_bounds__(64) k_remdiv(RdArgs A, CnPtrs<T, NLT> P)
 {
-    remdiv_body<T, NLT>(A, P, blockIdx.x);
+    remdiv_body<T, NLT>(A, P, xcd_contiguous(blockIdx.x, gridDim.x));
 }
 
 // The mixed path's low-mode columns: deeper prefetch (few wavefronts).
@@ -519,7 +526,7 @@ template void launch_remdiv_side_batch<double, float>(hipStream_t, const Geo &, 
 template <typename T, typename NLT>
 void launch_cn(hipStream_t s, const CnArgs &a, const CnPtrs<T, NLT> &p)
 {
-    const long blocks = (a.g.ps + 63) / 64;
+    const long blocks = ((a.g.ps + 63) / 64 + 7) / 8 * 8;  // a multiple of 8 (xcd_contiguous)
     if (a.g.nth == 0)
         hipLaunchKernelGGL((k_cn<T, 0, NLT>), dim3(blocks), dim3(256), 0, s, a, p);
     else
@@ -546,7 +553,7 @@ void launch_remdiv_side(hipStream_t s, const RdArgs &a, const CnPtrs<T, NLT> &p)
 template <typename T, typename NLT>
 void launch_remdiv(hipStream_t s, const RdArgs &a, const CnPtrs<T, NLT> &p)
 {
-    const long blocks = (2 * a.g.ps + 63) / 64;  // two lanes per mode
+    const long blocks = ((2 * a.g.ps + 63) / 64 + 7) / 8 * 8;  // two lanes per mode; a multiple of 8
     hipLaunchKernelGGL((k_remdiv<T, NLT>), dim3(blocks), dim3(64), 0, s, a, p);
 }
 
