@@ -187,14 +187,21 @@ __device__ __forceinline__ void zfwd_body(const IpLen &fz, const cx<T> *__restri
     });
 }
 
+// grid (8 * nchunks * ceil(nplanes / 8), 1, outputs), the XCD order of
+// k_zinv_ip: the kept-slot rows it writes straddle cache lines shared by
+// neighbouring column chunks.
 template <typename T, int E, typename TOUT, int PID>
 __global__ void __launch_bounds__(256) k_zfwd_nl_ip(IpLen fz, const cx<T> *__restrict__ tw,
                                                      const cx<T> *__restrict__ wt, long wt_fs, long wt_ps, int ld,
                                                      int mk, cx<TOUT> *const *__restrict__ nl, long ps,
-                                                     const double *__restrict__ kzv, int jlo, int nkz, int tnkz, int lw)
+                                                     const double *__restrict__ kzv, int jlo, int nkz, int tnkz, int lw,
+                                                     int nchunks, int nplanes)
 {
+    const int L = blockIdx.x, r = L >> 3;
+    const int chunk = r % nchunks, plane = 8 * (r / nchunks) + (L & 7);
+    if (plane >= nplanes) return;
     zfwd_body<T, E, TOUT, PID != 0>(plan_ref<PID>(fz), tw, wt, wt_fs, wt_ps, ld, mk, nl, ps, kzv, jlo, nkz, tnkz, lw,
-                                    blockIdx.x, blockIdx.y, blockIdx.z);
+                                    chunk, plane, blockIdx.z);
 }
 
 // grid (column chunks, planes of the tallest member, members * nout)
@@ -261,11 +268,12 @@ static void zfwd_nl_ip_e(hipStream_t s, const Geo &g, const IpLen &fz, const cx<
                          int ld, cx<TOUT> *const *nl, const double *kz, int jlo, int jhi, int nout)
 {
     const int lw = z_lw(fz.t), W = 1 << lw;
-    dim3 grid((g.mk + W - 1) / W, jhi - jlo + 1, nout);
+    const int nchunks = (g.mk + W - 1) / W, nplanes = jhi - jlo + 1;
+    dim3 grid((unsigned)(8 * nchunks * ((nplanes + 7) / 8)), 1, nout);
     const size_t lds = sizeof(cx<T>) * W * fz.n;
     const long wt_ps = (long)g.nz * ld;
     hipLaunchKernelGGL((k_zfwd_nl_ip<T, E, TOUT, PID>), grid, dim3(W * fz.t), lds, s, fz, tw, wt, wt_fs, wt_ps, ld,
-                       g.mk, nl, g.ps, kz, jlo, g.nkz, g.tnkz, lw);
+                       g.mk, nl, g.ps, kz, jlo, g.nkz, g.tnkz, lw, nchunks, nplanes);
 }
 
 template <typename T, typename TOUT>
