@@ -65,7 +65,12 @@ def substep_model_bytes(cfg, s: int = 8) -> float:
 def time_path(cfg, phys, precision: str, steps: int, warmup: int, kernel_timing: bool, zdft: bool = False,
               tune: bool = False):
     """Initial state -> warm-up -> [autotune] -> timed steps; returns (seconds,
-    kernel times, channel); the autotuner's choice is in channel.tuned."""
+    kernel times, channel); the autotuner's choice is in channel.tuned.
+
+    With kernel timing the timed steps run eagerly (an event pair around each
+    hot kernel on the context's stream, so no captured graphs); the same
+    number of steps is then timed again with the per-stage graphs and no
+    events, the production path, into channel.graph_elapsed."""
     from diablo_amd import gpu
     g = gpu.GpuChannel(cfg, precision=precision)
     g.initialize(*phys)
@@ -84,6 +89,14 @@ def time_path(cfg, phys, precision: str, steps: int, warmup: int, kernel_timing:
     elapsed = time.perf_counter() - t0
     kt = g.kernel_times() if kernel_timing else {}
     g.set_kernel_timing(False)
+    g.graph_elapsed = None
+    if kernel_timing:
+        g.step(1)  # captures the stage graphs outside the timed steps
+        g.synchronize()
+        t0 = time.perf_counter()
+        g.step(steps)
+        g.synchronize()
+        g.graph_elapsed = time.perf_counter() - t0
     return elapsed, kt, g
 
 
@@ -296,6 +309,13 @@ def main():
     tuned = {"fp64": g.tuned}
     substeps = 3 * args.steps
     value = cfg.points * substeps / elapsed / 1e6
+    graphs = None
+    if g.graph_elapsed:
+        graphs = {"note": "the same number of steps again, per-stage graphs, no kernel events",
+                  "fp64": {"value": round(cfg.points * substeps / g.graph_elapsed / 1e6, 2),
+                           "ms_per_step": round(g.graph_elapsed / args.steps * 1e3, 3)}}
+        log(f"[bench] FP64 with graphs {graphs['fp64']['value']:.1f} Mpts*substeps/s, "
+            f"{graphs['fp64']['ms_per_step']:.2f} ms/step")
     # sanity: the state must stay finite
     cu1 = g.get_field(gpu.CU1)
     if not math.isfinite(float(abs(cu1).max())):
@@ -330,6 +350,12 @@ def main():
         el_m, kt_m, gm = time_path(cfg, phys, "mixed", args.steps, args.warmup, not args.no_kernel_timing,
                                    tune=args.autotune == "on")
         tuned["mixed"] = gm.tuned
+        if graphs and gm.graph_elapsed:
+            vg = cfg.points * substeps / gm.graph_elapsed / 1e6
+            graphs["mixed"] = {"value": round(vg, 2), "ms_per_step": round(gm.graph_elapsed / args.steps * 1e3, 3),
+                               "speedup_vs_fp64": round(vg / graphs["fp64"]["value"], 3)}
+            log(f"[bench] mixed with graphs {vg:.1f} Mpts*substeps/s "
+                f"({graphs['mixed']['speedup_vs_fp64']:.2f}x FP64 with graphs)")
         gm.close()
         vm = cfg.points * substeps / el_m / 1e6
         for k, (ms, n) in kt_m.items():
@@ -397,6 +423,7 @@ def main():
         "les": les,
         "zdft": zd,
         "autotune": tuned if args.autotune == "on" else None,
+        "graphs": graphs,
     }
     print(json.dumps(out), flush=True)
 
