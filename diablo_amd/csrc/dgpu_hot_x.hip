@@ -128,7 +128,10 @@ __device__ __forceinline__ void xfused_body(const IpLen &fx, const cx<T> *__rest
     // The eleven products of plane j at every position, from planes j-1, j, j+1.
     auto products = [&](int j) {
         const cx<T> *wm = slot_row(j - 1, 0), *w0 = slot_row(j, 0), *wp = slot_row(j + 1, 0);
-        const T dyj = T(dy[j]), dyfj = T(dyf[j]), dyfm = T(dyf[j - 1]);
+        const T dyfj = T(dyf[j]), dyfm = T(dyf[j - 1]);
+        // the plane's divisors as reciprocals, once per plane (an IEEE
+        // division is a dozen instructions; within an ulp of dividing)
+        const T r2dy = T(1) / (T(2) * T(dy[j])), r2dyf = T(1) / (T(2) * dyfj), rdy = T(1) / T(dy[j]);
         for (int x = tid; x < nx; x += nthr) {
             const cx<T> a0 = w0[x], am = wm[x], ap = wp[x];
             const cx<T> b0 = w0[nx + x], bm = wm[nx + x], bp = wp[nx + x];
@@ -138,12 +141,12 @@ __device__ __forceinline__ void xfused_body(const IpLen &fx, const cx<T> *__rest
             const T p0 = u3 * u1;                                                      // U1*U3
             const T p1 = u1 * u1;                                                      // U1*U1
             const T p2 = u3 * u3;                                                      // U3*U3
-            const T p3 = ((dyfj * u1 + dyfm * u1m) / (T(2) * dyj)) * u2;               // U1*U2 at GY
-            const T p4 = ((dyfj * u3 + dyfm * u3m) / (T(2) * dyj)) * u2;               // U3*U2 at GY
-            const T p5 = (u1p * u2p + u1 * u2p - u1 * u2 - u1m * u2) / (T(2) * dyfj);  // d/dy(U1 U2)
-            const T p6 = (u3p * u2p + u3 * u2p - u3 * u2 - u3m * u2) / (T(2) * dyfj);  // d/dy(U3 U2)
+            const T p3 = ((dyfj * u1 + dyfm * u1m) * r2dy) * u2;                 // U1*U2 at GY
+            const T p4 = ((dyfj * u3 + dyfm * u3m) * r2dy) * u2;                 // U3*U2 at GY
+            const T p5 = (u1p * u2p + u1 * u2p - u1 * u2 - u1m * u2) * r2dyf;    // d/dy(U1 U2)
+            const T p6 = (u3p * u2p + u3 * u2p - u3 * u2 - u3m * u2) * r2dyf;    // d/dy(U3 U2)
             const T sp = u2 + u2p, sm = u2 + u2m;
-            const T p7 = (T(0.25) * (sp * sp) - T(0.25) * (sm * sm)) / dyj;            // d/dy(U2 U2)
+            const T p7 = (T(0.25) * (sp * sp) - T(0.25) * (sm * sm)) * rdy;      // d/dy(U2 U2)
             fb[0 * nx + x] = make_cx<T>(p1, p5);
             fb[1 * nx + x] = make_cx<T>(p3, p7);
             fb[2 * nx + x] = make_cx<T>(p0, p6);
@@ -152,7 +155,7 @@ __device__ __forceinline__ void xfused_body(const IpLen &fx, const cx<T> *__rest
                 const T t0 = b0.im, tm = bm.im, tp = bp.im;
                 const T p8 = t0 * u1;
                 const T p9 = t0 * u3;
-                const T p10 = (tp * u2p + t0 * u2p - t0 * u2 - tm * u2) / (T(2) * dyfj);
+                const T p10 = (tp * u2p + t0 * u2p - t0 * u2 - tm * u2) * r2dyf;
                 fb[4 * nx + x] = make_cx<T>(p8, p10);
                 fb[5 * nx + x] = make_cx<T>(p9, T(0));
             }
