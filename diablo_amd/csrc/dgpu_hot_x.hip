@@ -18,9 +18,9 @@ namespace dgpu {
 //   row 1 (U1U2@GY, d/dy U2U2) -> G2        row 4 (TH U1, d/dy TH U2) -> G4
 //   row 2 (U1U3, d/dy U3U2) -> G3, H1       row 5 (TH U3, 0) -> H4
 // LDS: window of three physical planes [3][npi][NX], products [npo][NX],
-// twiddles.  Transform positions sit at swizzled slots (ip_swz) of their rows;
-// the products read and write slot by slot, which is position by position in
-// the same permutation for every row.
+// twiddles, the kept modes' KX.  Transform positions sit at swizzled slots
+// (ip_swz) of their rows; the products read and write slot by slot, which is
+// position by position in the same permutation for every row.
 // -------------------------------------------------------------------------
 template <typename T, int E, int NTH, bool CP>
 __device__ __forceinline__ void xfused_body(const IpLen &fx, const cx<T> *__restrict__ twg, int mk, int ld,
@@ -38,8 +38,10 @@ __device__ __forceinline__ void xfused_body(const IpLen &fx, const cx<T> *__rest
     cx<T> *win = reinterpret_cast<cx<T> *>(smem_raw);
     cx<T> *fb = win + 3 * npi * nx;
     cx<T> *tw = fb + npo * nx;
+    T *kxs = reinterpret_cast<T *>(tw + fx.twn);  // KX of the kept modes, read by every plane's output
     const int tid = threadIdx.x, nthr = blockDim.x;
     for (int e = tid; e < fx.twn; e += nthr) tw[e] = twg[e];
+    for (int e = tid; e < mk; e += nthr) kxs[e] = T(kxv[e]);
     const int TT = fx.t, S = fx.nstages, L0 = fx.span[0];
     const int p = tid / TT;
     int th = tid - p * TT;  // re-laundered every plane (see the loop)
@@ -103,7 +105,7 @@ __device__ __forceinline__ void xfused_body(const IpLen &fx, const cx<T> *__rest
         auto emit = [&](int kp, cx<T> Zk, cx<T> Zn) {
             const cx<T> Xa = make_cx<T>((Zk.re + Zn.re) * h, (Zk.im - Zn.im) * h);
             const cx<T> Xb = make_cx<T>((Zk.im + Zn.im) * h, -(Zk.re - Zn.re) * h);
-            const T kx = T(kxv[kp]);
+            const T kx = kxs[kp];
             // -(i kx Xa + Xb)
             const cx<T> G = make_cx<T>(kx * Xa.im - Xb.re, -kx * Xa.re - Xb.im);
             const cx<T> nXa = make_cx<T>(-Xa.re, -Xa.im), nXb = make_cx<T>(-Xb.re, -Xb.im);
@@ -169,7 +171,7 @@ __device__ __forceinline__ void xfused_body(const IpLen &fx, const cx<T> *__rest
     const bool pro = p < 2 * npi;
     const int pro_plane = ja - 1 + (p >= npi ? 1 : 0), pro_pair = p < npi ? p : p - npi;
     if (pro) fetch(pro_plane, pro_pair);
-    __syncthreads();  // twiddles
+    __syncthreads();  // twiddles, wavenumbers
     if (pro) dif0(slot_row(pro_plane, pro_pair));
     __syncthreads();
     stage_loop<CP>(1, S, [&](int s) {
@@ -375,7 +377,7 @@ static void xfused_ip_e(hipStream_t s, const Geo &g, const IpLen &fx, const cx<T
     using C = XCount<NTH>;
     const int nchunks = (jhi - jlo + 1 + chunk - 1) / chunk;
     const int threads = fx.t * (C::npi + C::npo);
-    const size_t lds = sizeof(cx<T>) * ((size_t)fx.n * (3 * C::npi + C::npo) + fx.twn);
+    const size_t lds = sizeof(cx<T>) * ((size_t)fx.n * (3 * C::npi + C::npo) + fx.twn) + sizeof(T) * g.mk;
     static bool attr = false;
     if (!attr) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_xfused_ip<T, E, NTH, PID>),
@@ -387,10 +389,10 @@ static void xfused_ip_e(hipStream_t s, const Geo &g, const IpLen &fx, const cx<T
                        zt, zt_fs, ps, wt, wt_fs, ps, dy, dyf, kx, jlo, jhi, chunk);
 }
 
-size_t xfused_ip_lds_bytes(const IpLen &fx, int nth, size_t elem)
+size_t xfused_ip_lds_bytes(const IpLen &fx, int nth, size_t elem, int mk)
 {
     const int npi = (3 + nth + 1) / 2, npo = (8 + 3 * nth + 1) / 2;
-    return elem * ((size_t)fx.n * (3 * npi + npo) + fx.twn);
+    return elem * ((size_t)fx.n * (3 * npi + npo) + fx.twn) + elem / 2 * (size_t)mk;
 }
 
 int xfused_ip_threads(const IpLen &fx, int nth) { return fx.t * ((3 + nth + 1) / 2 + (8 + 3 * nth + 1) / 2); }
@@ -432,7 +434,7 @@ static void xfused_ip_batch_e(hipStream_t s, const Geo &g, const IpLen &fx, cons
 {
     using C = XCount<NTH>;
     const int threads = fx.t * (C::npi + C::npo);
-    const size_t lds = sizeof(cx<T>) * ((size_t)fx.n * (3 * C::npi + C::npo) + fx.twn);
+    const size_t lds = sizeof(cx<T>) * ((size_t)fx.n * (3 * C::npi + C::npo) + fx.twn) + sizeof(T) * g.mk;
     static bool attr = false;
     if (!attr) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_xfused_ip_batch<T, E, NTH, PID>),
