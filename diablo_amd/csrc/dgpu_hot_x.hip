@@ -202,10 +202,30 @@ __device__ __forceinline__ void xfused_body(const IpLen &fx, const cx<T> *__rest
         dif0(slot_row(ja + 1, p));
     }
     __syncthreads();
+    // With T dividing 64 every transform row lies inside one wavefront, and a
+    // row's stages exchange data only among its own T threads: between its
+    // stages the wavefront's LDS operations need only stay in order (a
+    // wavefront executes its LDS operations in order; the fences keep the
+    // compiler from moving them across).  Cross-wavefront hand-overs (the
+    // product rows, the window slots read by every thread) keep the block
+    // barriers.
+    const bool wave_rows = (64 % TT) == 0;
+    auto row_sync = [&]() {
+        if (wave_rows) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        } else {
+            __syncthreads();
+        }
+    };
     // Iteration j: stages 1..S-1 of plane j+1's backward and the forward of
     // plane j-1's products (S phases), the products of plane j, then stage 0
     // of plane j+2's backward from registers fetched during the products.
-    // Slot (j+2)%3 held plane j-1, free once the products of plane j are done.
+    // Slot (j+2)%3 held plane j-1, free once the products of plane j are done
+    // (the block barrier after them); plane j+2's backward then runs on the
+    // wavefront that wrote its stage 0, and every other reader of the slot
+    // (the products of plane j+1) comes after the next block barrier.
     for (int j = ja; j <= jb + 1; ++j) {
         // Keeps the compiler from hoisting the per-thread index arithmetic of
         // every stage out of the sweep, which would hold it in registers
@@ -218,7 +238,7 @@ __device__ __forceinline__ void xfused_body(const IpLen &fx, const cx<T> *__rest
         stage_loop<CP>(0, S - 1, [&](int s) {
             if (inv) ip_stage_row<T, E, +1, true>(drow, fx, s + 1, tw, th);
             if (fwd) ip_stage_row<T, E, -1, false>(frow, fx, S - 1 - s, tw, th);
-            __syncthreads();
+            row_sync();
         });
         if (fwd) dit_out(frow, p - npi, j - 1);
         __syncthreads();
@@ -228,7 +248,7 @@ __device__ __forceinline__ void xfused_body(const IpLen &fx, const cx<T> *__rest
             products(j);
             __syncthreads();
             if (nxt) dif0(slot_row(j + 2, p));
-            __syncthreads();
+            row_sync();
         }
     }
 }
