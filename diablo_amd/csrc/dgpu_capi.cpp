@@ -45,7 +45,7 @@ struct dgpu_ctx {
     cx<double> *d_iptw = nullptr;
     cx<float> *d_iptwf = nullptr;
     int ld = 0;      // row length of the intermediates (>= mk, multiple of 8)
-    int xchunk = 16; // planes per block of the fused x kernel
+    int xchunk = 64; // planes per block of the fused x kernel (set at creation)
     // mixed paths: FP32 transforms, intermediates and nonlinear terms
     int mixed = 0;
     // DGPU_MIXED: FP32 working state [slot][j][K][I] (same slots as d_state's
@@ -519,6 +519,10 @@ int32_t dgpu_create(const dgpu_params *p, dgpu_ctx **out)
             delete c;
             return DGPU_E_ARG;
         }
+        // planes per block: what dgpu_autotune measures best at 384x257x384
+        // (64 in FP64, 32 with the FP32 transforms: three full rounds of
+        // resident blocks either way, at two and four blocks per CU)
+        c->xchunk = c->mixed ? 32 : 64;
         if (xc && std::atoi(xc) > 0) c->xchunk = std::atoi(xc);
         const char *sw = std::getenv("DGPU_XSWZ");  // 0 turns the row swizzle off
         c->ipx.swz = (sw && std::atoi(sw) == 0) ? 0u : dgpu::ip_pick_swizzle(c->ipx);
