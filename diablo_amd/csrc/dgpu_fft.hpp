@@ -398,7 +398,20 @@ DGPU_HD void ip_stage_row(cx<T> *x, const IpLen &f, int i, const cx<T> *__restri
 
 // One Stockham stage of radix R for the transform whose element e sits at
 // lds[e * stride].  th is this thread's index inside the transform.
-template <typename T, int E, int R, int SIGN>
+// Orders a wavefront's own LDS accesses across this point: a wavefront's LDS
+// operations execute in order, and the fences keep the compiler from moving
+// them across.  Enough in place of a block barrier when the data exchanged
+// stays inside the wavefront (a transform whose threads all lie in it).
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// WS: every transform's threads lie inside one wavefront (T divides 64), so
+// the stage synchronises with wave_lds_sync instead of block barriers.
+template <typename T, int E, int R, int SIGN, bool WS = false>
 __device__ __forceinline__ void stage(cx<T> *lds, int stride, int n, int ns, unsigned magic, int th, int nthr,
                                       const cx<T> *__restrict__ tw, bool active)
 {
@@ -420,7 +433,8 @@ __device__ __forceinline__ void stage(cx<T> *lds, int stride, int n, int ns, uns
             for (int r = 1; r < R; ++r) w[u * R + r] = tw[r * k * tws];
         }
     }
-    __syncthreads();
+    if constexpr (WS) wave_lds_sync();
+    else __syncthreads();
     if (active) {
 #pragma unroll
         for (int u = 0; u < NB; ++u) {
@@ -439,12 +453,15 @@ __device__ __forceinline__ void stage(cx<T> *lds, int stride, int n, int ns, uns
             for (int r = 0; r < R; ++r) lds[(d + r * ns) * stride] = v[u * R + r];
         }
     }
-    __syncthreads();
+    if constexpr (WS) wave_lds_sync();
+    else __syncthreads();
 }
 
 // Full transform.  Every thread of the block must call this (it contains
 // barriers); threads with active == false only take part in the barriers.
-template <typename T, int E, int SIGN>
+// With WS the transform synchronises within its wavefront only, and the
+// caller places a block barrier before other wavefronts read the result.
+template <typename T, int E, int SIGN, bool WS = false>
 __device__ void fft_lds(cx<T> *lds, int stride, const FftLen &f, int th, bool active,
                         const cx<T> *__restrict__ tw)
 {
@@ -453,16 +470,16 @@ __device__ void fft_lds(cx<T> *lds, int stride, const FftLen &f, int th, bool ac
         const int R = f.radix[s];
         const unsigned mg = f.magic[s];
         if constexpr (E % 4 == 0) {
-            if (R == 4) { stage<T, E, 4, SIGN>(lds, stride, f.n, ns, mg, th, f.t, tw, active); ns *= 4; continue; }
+            if (R == 4) { stage<T, E, 4, SIGN, WS>(lds, stride, f.n, ns, mg, th, f.t, tw, active); ns *= 4; continue; }
         }
         if constexpr (E % 2 == 0) {
-            if (R == 2) { stage<T, E, 2, SIGN>(lds, stride, f.n, ns, mg, th, f.t, tw, active); ns *= 2; continue; }
+            if (R == 2) { stage<T, E, 2, SIGN, WS>(lds, stride, f.n, ns, mg, th, f.t, tw, active); ns *= 2; continue; }
         }
         if constexpr (E % 3 == 0) {
-            if (R == 3) { stage<T, E, 3, SIGN>(lds, stride, f.n, ns, mg, th, f.t, tw, active); ns *= 3; continue; }
+            if (R == 3) { stage<T, E, 3, SIGN, WS>(lds, stride, f.n, ns, mg, th, f.t, tw, active); ns *= 3; continue; }
         }
         if constexpr (E % 5 == 0) {
-            if (R == 5) { stage<T, E, 5, SIGN>(lds, stride, f.n, ns, mg, th, f.t, tw, active); ns *= 5; continue; }
+            if (R == 5) { stage<T, E, 5, SIGN, WS>(lds, stride, f.n, ns, mg, th, f.t, tw, active); ns *= 5; continue; }
         }
     }
 }

@@ -211,13 +211,8 @@ __device__ __forceinline__ void xfused_body(const IpLen &fx, const cx<T> *__rest
     // barriers.
     const bool wave_rows = (64 % TT) == 0;
     auto row_sync = [&]() {
-        if (wave_rows) {
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        } else {
-            __syncthreads();
-        }
+        if (wave_rows) wave_lds_sync();
+        else __syncthreads();
     };
     // Iteration j: stages 1..S-1 of plane j+1's backward and the forward of
     // plane j-1's products (S phases), the products of plane j, then stage 0

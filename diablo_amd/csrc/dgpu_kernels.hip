@@ -91,7 +91,10 @@ __global__ void k_x_r2c(FftLen fx, const cx<T> *__restrict__ tw, const double *_
     }
     __syncthreads();
     const int p = tid / fx.t, th = tid - p * fx.t;
-    fft_lds<T, E, -1>(lds + p * nx, 1, fx, th, p < npair, tw);
+    // rows of T consecutive threads: inside one wavefront when T divides 64
+    if (64 % fx.t == 0) fft_lds<T, E, -1, true>(lds + p * nx, 1, fx, th, p < npair, tw);
+    else fft_lds<T, E, -1>(lds + p * nx, 1, fx, th, p < npair, tw);
+    __syncthreads();  // the outputs below read every row
     const T h = T(0.5) / T(nx);
     for (int idx = tid; idx < npair * mk; idx += nthr) {
         const int pp = idx / mk, k = idx - pp * mk;
@@ -174,7 +177,9 @@ __global__ void k_x_c2r(FftLen fx, const cx<T> *__restrict__ tw, const cx<T> *__
     }
     __syncthreads();
     const int p = tid / fx.t, th = tid - p * fx.t;
-    fft_lds<T, E, +1>(lds + p * nx, 1, fx, th, p < npair, tw);
+    if (64 % fx.t == 0) fft_lds<T, E, +1, true>(lds + p * nx, 1, fx, th, p < npair, tw);
+    else fft_lds<T, E, +1>(lds + p * nx, 1, fx, th, p < npair, tw);
+    __syncthreads();  // the outputs below read every row
     const long rowlen = nx + 2;
     double *pl = fort + (long)j * (nz + 2) * rowlen;
     for (int idx = tid; idx < npair * (nx + 2); idx += nthr) {
