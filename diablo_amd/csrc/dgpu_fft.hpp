@@ -308,6 +308,8 @@ constexpr ConstPlanSpec kConstPlanSpecs[] = {
     {96, 12, 0, 0u, 0},            // 9: (6,4,4)
     {128, 8, 0, 0u, 0},            // 10: (8,8,2)
     {192, 24, 0, 0u, 0},           // 11: (8,8,3), FP32
+    // x plan appended after the z plans (their ids stay)
+    {256, 8, 4, 0x76543210u, 0},   // 12: (4,8,8)
 };
 constexpr int kNumConstPlans = (int)(sizeof(kConstPlanSpecs) / sizeof(kConstPlanSpecs[0]));
 

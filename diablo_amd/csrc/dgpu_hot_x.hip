@@ -248,11 +248,11 @@ __device__ __forceinline__ void xfused_body(const IpLen &fx, const cx<T> *__rest
     }
 }
 
-// FP32 with a compile-time plan: four resident blocks per CU (LDS holds four
-// FP32 windows at NX = 384, 40 KB each) at the cost of a few spilled
-// registers; FP64 stays at the two its LDS allows.
+// FP32 with the 384 plan: four resident blocks per CU (LDS holds four FP32
+// windows at NX = 384, 40 KB each) at the cost of a few spilled registers;
+// FP64 stays at the two its LDS allows.
 template <typename T, int E, int NTH, int PID>
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && PID != 0 ? 4 : 2)))
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && PID == 1 ? 4 : 2)))
 k_xfused_ip(IpLen fx, const cx<T> *__restrict__ twg, int mk, int ld,
                                                     const cx<T> *__restrict__ zt, long zt_fs, long zt_ps,
                                                     cx<T> *__restrict__ wt, long wt_fs, long wt_ps,
@@ -431,17 +431,19 @@ size_t xfused_ip_lds_bytes(const IpLen &fx, int nth, size_t elem, int mk)
 
 int xfused_ip_threads(const IpLen &fx, int nth) { return fx.t * ((3 + nth + 1) / 2 + (8 + 3 * nth + 1) / 2); }
 
-// The single-run x kernel is built for the 384 plan only (the ensembles use
-// the batched kernel); one scalar.
+// The single-run x kernel is built for the 384 and 256 plans (the ensembles
+// use the batched kernel) with one scalar; other shapes run on runtime plans.
 template <typename T>
 void launch_xfused_ip(hipStream_t s, const Geo &g, const IpLen &fx, const cx<T> *tw, int ld, const cx<T> *zt,
                       long zt_fs, cx<T> *wt, long wt_fs, const double *dy, const double *dyf, const double *kx,
                       int jlo, int jhi, int chunk)
 {
-    if (g.nth == 1 && const_plan_id(fx) == 1) {
-        xfused_ip_e<T, 12, 1, 1>(s, g, fx, tw, ld, zt, zt_fs, wt, wt_fs, dy, dyf, kx, jlo, jhi, chunk);
+    if (g.nth == 1 && with_pid(const_plan_id(fx), XPidsSingle{}, [&](auto pc) {
+            constexpr int P = decltype(pc)::value;
+            xfused_ip_e<T, kConstPlanSpecs[P].e, 1, P>(s, g, fx, tw, ld, zt, zt_fs, wt, wt_fs, dy, dyf, kx, jlo, jhi,
+                                                       chunk);
+        }))
         return;
-    }
 #define DGPU_XF(EE_)                                                                                             \
     do {                                                                                                         \
         if (g.nth == 0)                                                                                          \

@@ -160,6 +160,33 @@ def test_multi_step(oracle_mod, gpu_mod, maker, nsteps):
     assert not bad, bad
 
 
+@pytest.mark.parametrize("nx", [256, 384])
+def test_compile_time_x_plans(oracle_mod, gpu_mod, nx):
+    """The x kernel's compile-time plans (NX = 384: radices (6,4,4,4);
+    NX = 256: (4,8,8)) against the oracle over two steps, NY and NZ small so
+    the oracle finishes in seconds; the mixed path on the same plans within
+    1e-5 of the FP64 one."""
+    cfg = case.channel_case(nx, 17, 16, kick=0.05)
+    say("compile-time x plan", nx)
+    o, g = make_pair(oracle_mod, gpu_mod, cfg)
+    gm = gpu_mod.GpuChannel(cfg, precision="mixed")
+    gm.load_state(o.state())
+    gm.time = o.time
+    for _ in range(2):
+        o.step()
+        g.step()
+        gm.step()
+    g.synchronize()
+    errs, bad = compare_states(cfg, g.state(), o.state(), 1e-10)
+    say(errs)
+    assert not bad, bad
+    sg, sm = g.state(), gm.state()
+    for k in ("CU1", "CU2", "CU3", "CTH0"):
+        e = rel_err(cfg, sm[k], sg[k])
+        say(k, "mixed", e)
+        assert e < 1e-5, (k, e)
+
+
 def test_divergence_free_after_steps(gpu_mod):
     """Property at a larger size: the discrete divergence of REM_DIV_CHAN's
     stencil vanishes on the interior rows after every step."""
