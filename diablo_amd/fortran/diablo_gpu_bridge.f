@@ -14,7 +14,9 @@
 !
 ! Compile with the reference's own include files on the include path
 ! ('header' and what it includes) and link against libdiablo_gpu.so; the five
-! reference definitions are then left out of the link (INTEGRATION.md).
+! reference definitions are then left out of the link: channel.f's
+! RK_CHAN_1, REM_DIV_CHAN and POISSON_P_CHAN and mpi.f's FFT_XZ_* go behind
+! '#ifndef DIABLO_GPU' guards in the reference build.
 !
 ! State coherence.  The Fourier state lives on the GPU during a time step.
 ! By default RK_CHAN_1 uploads CU1-3, CP, CTH and CF at RK_STEP = 1 and writes

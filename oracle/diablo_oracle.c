@@ -36,6 +36,16 @@
 
 typedef double complex cplx;
 
+/* Planted mistakes.  tests/test_oracle_rk_terms.py builds this file with
+ * -DDOR_PLANT=n (n >= 1) into a throw-away library and checks that the pins
+ * of RK_CHAN_1's explicit terms reject it; each PLANT(n, wrong, right) site
+ * below marks the term a pin must guard.  DOR_PLANT = 0, the default build,
+ * is the restatement itself: every site takes its right-hand operand. */
+#ifndef DOR_PLANT
+#define DOR_PLANT 0
+#endif
+#define PLANT(n, wrong, right) (DOR_PLANT == (n) ? (wrong) : (right))
+
 /* ------------------------------------------------------------------------- */
 /* FFT: mixed-radix recursive decimation in time                            */
 /* ------------------------------------------------------------------------- */
@@ -890,14 +900,16 @@ void dor_rk_chan_1(dor_state *s, int RK_STEP)
         for (int K = 0; K <= TNKZ; K++)
             for (int i = 0; i <= NXP - 1; i++) {
                 cplx d = CP[CIDX(i, K, J)] - CP[CIDX(i, K, J - 1)];
-                CS1[CIDX(i, K, J)] = creal(d) / DY[J] + I * (cimag(d) / DY[J]);
+                const int JD = PLANT(2, J + 1, J);
+                CS1[CIDX(i, K, J)] = creal(d) / DY[JD] + I * (cimag(d) / DY[JD]);
             }
 #pragma omp parallel for schedule(static)
     for (int J = JSTART; J <= JEND; J++)
         for (int K = 0; K <= TNKZ; K++)
             for (int i = 0; i <= NXP - 1; i++) {
-                CR1[CIDX(i, K, J)] = CR1[CIDX(i, K, J)] - TEMP4 * (CIKX(i) * CP[CIDX(i, K, J)]);
-                CR3[CIDX(i, K, J)] = CR3[CIDX(i, K, J)] - TEMP4 * (CIKZ(K) * CP[CIDX(i, K, J)]);
+                CR1[CIDX(i, K, J)] = PLANT(1, CR1[CIDX(i, K, J)] + TEMP4 * (CIKX(i) * CP[CIDX(i, K, J)]),
+                                           CR1[CIDX(i, K, J)] - TEMP4 * (CIKX(i) * CP[CIDX(i, K, J)]));
+                CR3[CIDX(i, K, J)] = CR3[CIDX(i, K, J)] - TEMP4 * (PLANT(3, CIKX(i), CIKZ(K)) * CP[CIDX(i, K, J)]);
             }
 #pragma omp parallel for schedule(static)
     for (int J = 2; J <= NY; J++)
@@ -907,10 +919,10 @@ void dor_rk_chan_1(dor_state *s, int RK_STEP)
 
     /* Mean forcing (for/channel.f:155-167), RANKZ = 0. */
     if (s->p.f_type == 1) {
-        for (int J = JSTART; J <= JEND; J++) CR1[CIDX(0, 0, J)] = CR1[CIDX(0, 0, J)] - TEMP4 * s->p.px0;
+        for (int J = JSTART; J <= JEND; J++) CR1[CIDX(0, 0, J)] = CR1[CIDX(0, 0, J)] - TEMP4 * PLANT(29, -s->p.px0, s->p.px0);
     } else if (s->p.f_type == 2) {
         for (int J = JSTART; J <= JEND; J++)
-            CR1[CIDX(0, 0, J)] = CR1[CIDX(0, 0, J)] - TEMP4 * (s->p.px0 + s->p.amp_omega0 * cos(s->p.omega0 * s->TIME));
+            CR1[CIDX(0, 0, J)] = CR1[CIDX(0, 0, J)] - TEMP4 * (s->p.px0 + s->p.amp_omega0 * PLANT(4, sin(s->p.omega0 * s->TIME), cos(s->p.omega0 * s->TIME)));
     }
 
     /* Horizontal viscous terms (for/channel.f:175-189). */
@@ -918,7 +930,7 @@ void dor_rk_chan_1(dor_state *s, int RK_STEP)
     for (int J = JSTART; J <= JEND; J++)
         for (int K = 0; K <= TNKZ; K++)
             for (int i = 0; i <= NXP - 1; i++) {
-                const double f = NU * pow(KX2[i] + KZ2[K], BETA);
+                const double f = NU * PLANT(5, (KX2[i] + KZ2[K]) * BETA, pow(KX2[i] + KZ2[K], BETA));
                 CF1[CIDX(i, K, J)] = -(f * CU1[CIDX(i, K, J)]);
                 CF3[CIDX(i, K, J)] = -(f * CU3[CIDX(i, K, J)]);
             }
@@ -935,8 +947,8 @@ void dor_rk_chan_1(dor_state *s, int RK_STEP)
     for (int K = 0; K <= TNKZ; K++)
         for (int i = 0; i <= NXP - 1; i++)
             for (int J = JSTART; J <= JEND; J++) {
-                CF1[CIDX(i, K, J)] = CF1[CIDX(i, K, J)] + s->p.i_ro * CU3[CIDX(i, K, J)];
-                CF3[CIDX(i, K, J)] = CF3[CIDX(i, K, J)] - s->p.i_ro * CU1[CIDX(i, K, J)];
+                CF1[CIDX(i, K, J)] = CF1[CIDX(i, K, J)] + PLANT(6, -s->p.i_ro, s->p.i_ro) * CU3[CIDX(i, K, J)];
+                CF3[CIDX(i, K, J)] = CF3[CIDX(i, K, J)] - PLANT(7, -s->p.i_ro, s->p.i_ro) * CU1[CIDX(i, K, J)];
             }
 
     for (int n = 0; n < s->NTH; n++) {
@@ -948,13 +960,13 @@ void dor_rk_chan_1(dor_state *s, int RK_STEP)
         for (int J = 2; J <= NY; J++) {
             for (int K = 1; K <= TNKZ; K++)
                 for (int i = 0; i <= NXP - 1; i++) {
-                    cplx t = CTH[CIDX(i, K, J)] * DYF[J - 1] + CTH[CIDX(i, K, J - 1)] * DYF[J];
+                    cplx t = CTH[CIDX(i, K, J)] * DYF[PLANT(8, J, J - 1)] + CTH[CIDX(i, K, J - 1)] * DYF[PLANT(8, J - 1, J)];
                     cplx b = (RI * s->p.grav_y) * t;
                     CF2[CIDX(i, K, J)] = CF2[CIDX(i, K, J)] + (creal(b) / (2.0 * DY[J]) + I * (cimag(b) / (2.0 * DY[J])));
                 }
-            for (int i = 1; i <= NXP - 1; i++) {
+            for (int i = PLANT(9, 0, 1); i <= NXP - 1; i++) {
                 const int K = 0;
-                cplx t = CTH[CIDX(i, K, J)] * DYF[J - 1] + CTH[CIDX(i, K, J - 1)] * DYF[J];
+                cplx t = CTH[CIDX(i, K, J)] * DYF[PLANT(8, J, J - 1)] + CTH[CIDX(i, K, J - 1)] * DYF[PLANT(8, J - 1, J)];
                 cplx b = (RI * s->p.grav_y) * t;
                 CF2[CIDX(i, K, J)] = CF2[CIDX(i, K, J)] + (creal(b) / (2.0 * DY[J]) + I * (cimag(b) / (2.0 * DY[J])));
             }
@@ -963,7 +975,7 @@ void dor_rk_chan_1(dor_state *s, int RK_STEP)
         for (int J = JSTART; J <= JEND; J++)
             for (int K = 0; K <= TNKZ; K++)
                 for (int i = 0; i <= NXP - 1; i++) {
-                    CF1[CIDX(i, K, J)] = CF1[CIDX(i, K, J)] + (RI * s->p.grav_x) * CTH[CIDX(i, K, J)];
+                    CF1[CIDX(i, K, J)] = CF1[CIDX(i, K, J)] + (RI * PLANT(10, s->p.grav_z, s->p.grav_x)) * CTH[CIDX(i, K, J)];
                     CF3[CIDX(i, K, J)] = CF3[CIDX(i, K, J)] + (RI * s->p.grav_z) * CTH[CIDX(i, K, J)];
                 }
         /* Scalar RHS (for/channel.f:254-290). */
@@ -973,10 +985,10 @@ void dor_rk_chan_1(dor_state *s, int RK_STEP)
                 for (int i = 0; i <= NXP - 1; i++) {
                     CRTH[CIDX(i, K, J)] = CTH[CIDX(i, K, J)];
                     if (RK_STEP > 1) CRTH[CIDX(i, K, J)] = CRTH[CIDX(i, K, J)] + TEMP3 * CFTH[CIDX(i, K, J)];
-                    const double f = (NU / PR) * pow(KX2[i] + KZ2[K], BETA);
+                    const double f = PLANT(11, NU * PR, NU / PR) * pow(KX2[i] + KZ2[K], BETA);
                     CFTH[CIDX(i, K, J)] = -(f * CTH[CIDX(i, K, J)]);
-                    CFTH[CIDX(i, K, J)] = CFTH[CIDX(i, K, J)] - CU1[CIDX(i, K, J)] * s->p.drhodx[n]
-                                          - CU3[CIDX(i, K, J)] * s->p.drhodz[n];
+                    CFTH[CIDX(i, K, J)] = CFTH[CIDX(i, K, J)] - CU1[CIDX(i, K, J)] * PLANT(12, -s->p.drhodx[n], s->p.drhodx[n])
+                                          - PLANT(13, CU1[CIDX(i, K, J)], CU3[CIDX(i, K, J)]) * s->p.drhodz[n];
                 }
     }
     /* USER_RHS_CHAN_FOURIER is an empty hook (for/user_rhs.f). */
@@ -1038,39 +1050,42 @@ void dor_rk_chan_1(dor_state *s, int RK_STEP)
     /* U1*U3 */
     PROD_LOOP(JSTART, JEND, AT(U3, 0) * AT(U1, 0));
     dor_fft_xz_to_fourier(s, S1, 0, NY + 1);
-    ACC_LOOP(JSTART, JEND, CA(CF1) = CA(CF1) - CIKZ(K) * CA(CS1); CA(CF3) = CA(CF3) - CIKX(i) * CA(CS1));
+    ACC_LOOP(JSTART, JEND, CA(CF1) = CA(CF1) - PLANT(14, CIKX(i), CIKZ(K)) * CA(CS1); CA(CF3) = CA(CF3) - CIKX(i) * CA(CS1));
     /* U1*U1 */
     PROD_LOOP(JSTART, JEND, AT(U1, 0) * AT(U1, 0));
     dor_fft_xz_to_fourier(s, S1, 0, NY + 1);
-    ACC_LOOP(JSTART, JEND, CA(CF1) = CA(CF1) - CIKX(i) * CA(CS1));
+    ACC_LOOP(JSTART, JEND, CA(CF1) = CA(CF1) - PLANT(30, CIKZ(K), CIKX(i)) * CA(CS1));
     /* U3*U3 */
     PROD_LOOP(JSTART, JEND, AT(U3, 0) * AT(U3, 0));
     dor_fft_xz_to_fourier(s, S1, 0, NY + 1);
-    ACC_LOOP(JSTART, JEND, CA(CF3) = CA(CF3) - CIKZ(K) * CA(CS1));
+    ACC_LOOP(JSTART, JEND, CA(CF3) = PLANT(15, CA(CF3) + CIKZ(K) * CA(CS1), CA(CF3) - CIKZ(K) * CA(CS1)));
     /* U1*U2 at GY */
-    PROD_LOOP(2, NY, ((DYF[J] * AT(U1, 0) + DYF[J - 1] * AT(U1, -1)) / (2. * DY[J])) * AT(U2, 0));
+    PROD_LOOP(2, NY, ((DYF[PLANT(16, J - 1, J)] * AT(U1, 0) + DYF[PLANT(16, J, J - 1)] * AT(U1, -1)) / (2. * DY[J]))
+                         * AT(U2, 0));
     dor_fft_xz_to_fourier(s, S1, 0, NY + 1);
     ACC_LOOP(2, NY, CA(CF2) = CA(CF2) - CIKX(i) * CA(CS1));
     /* U3*U2 at GY */
-    PROD_LOOP(2, NY, ((DYF[J] * AT(U3, 0) + DYF[J - 1] * AT(U3, -1)) / (2. * DY[J])) * AT(U2, 0));
+    PROD_LOOP(2, NY, ((DYF[PLANT(17, J - 1, J)] * AT(U3, 0) + DYF[PLANT(17, J, J - 1)] * AT(U3, -1)) / (2. * DY[J]))
+                         * AT(U2, 0));
     dor_fft_xz_to_fourier(s, S1, 0, NY + 1);
     ACC_LOOP(2, NY, CA(CF2) = CA(CF2) - CIKZ(K) * CA(CS1));
     /* d/dy(U1*U2) */
     PROD_LOOP(JSTART, JEND,
-              (AT(U1, 1) * AT(U2, 1) + AT(U1, 0) * AT(U2, 1) - AT(U1, 0) * AT(U2, 0) - AT(U1, -1) * AT(U2, 0))
+              (AT(U1, 1) * AT(U2, 1) + PLANT(18, AT(U1, 1) * AT(U2, 0), AT(U1, 0) * AT(U2, 1)) - AT(U1, 0) * AT(U2, 0)
+               - AT(U1, -1) * AT(U2, 0))
                   / (2.0 * DYF[J]));
     dor_fft_xz_to_fourier(s, S1, 0, NY + 1);
     ACC_LOOP(JSTART, JEND, CA(CF1) = CA(CF1) - CA(CS1));
     /* d/dy(U3*U2) */
     PROD_LOOP(JSTART, JEND,
               (AT(U3, 1) * AT(U2, 1) + AT(U3, 0) * AT(U2, 1) - AT(U3, 0) * AT(U2, 0) - AT(U3, -1) * AT(U2, 0))
-                  / (2.0 * DYF[J]));
+                  / (2.0 * PLANT(19, DY[J], DYF[J])));
     dor_fft_xz_to_fourier(s, S1, 0, NY + 1);
     ACC_LOOP(JSTART, JEND, CA(CF3) = CA(CF3) - CA(CS1));
     /* d/dy(U2*U2) */
     PROD_LOOP(2, NY,
               (0.25 * ((AT(U2, 0) + AT(U2, 1)) * (AT(U2, 0) + AT(U2, 1)))
-               - 0.25 * ((AT(U2, 0) + AT(U2, -1)) * (AT(U2, 0) + AT(U2, -1)))) / DY[J]);
+               - 0.25 * ((AT(U2, 0) + AT(U2, -1)) * (AT(U2, 0) + AT(U2, -1)))) / PLANT(20, DYF[J], DY[J]));
     dor_fft_xz_to_fourier(s, S1, 0, NY + 1);
     ACC_LOOP(2, NY, CA(CF2) = CA(CF2) - CA(CS1));
 
@@ -1090,8 +1105,8 @@ void dor_rk_chan_1(dor_state *s, int RK_STEP)
     for (int J = JSTART; J <= JEND; J++)
         for (int K = 0; K <= NZP - 1; K++)
             for (int i = 0; i <= NXM; i++) {
-                AT(R1, 0) = AT(R1, 0) + TEMP1 * NUV
-                    * (((AT(U1, 1) - AT(U1, 0)) / DY[J + 1] - (AT(U1, 0) - AT(U1, -1)) / DY[J]) / DYF[J]);
+                AT(R1, 0) = AT(R1, 0) + TEMP1 * PLANT(27, 1.0, NUV)
+                    * (((AT(U1, 1) - AT(U1, 0)) / DY[PLANT(24, J, J + 1)] - (AT(U1, 0) - AT(U1, -1)) / DY[J]) / DYF[J]);
                 AT(R3, 0) = AT(R3, 0) + TEMP1 * NUV
                     * (((AT(U3, 1) - AT(U3, 0)) / DY[J + 1] - (AT(U3, 0) - AT(U3, -1)) / DY[J]) / DYF[J]);
             }
@@ -1100,7 +1115,7 @@ void dor_rk_chan_1(dor_state *s, int RK_STEP)
         for (int K = 0; K <= NZP - 1; K++)
             for (int i = 0; i <= NXM; i++)
                 AT(R2, 0) = AT(R2, 0) + TEMP1 * NUV
-                    * (((AT(U2, 1) - AT(U2, 0)) / DYF[J] - (AT(U2, 0) - AT(U2, -1)) / DYF[J - 1]) / DY[J]);
+                    * (((AT(U2, 1) - AT(U2, 0)) / DYF[J] - (AT(U2, 0) - AT(U2, -1)) / DYF[PLANT(25, J, J - 1)]) / DY[J]);
     if (s->les) {
         /* the eddy viscosity, explicit half (for/channel.f:628-659) */
         const double *NT = R(DOR_NUT);
@@ -1129,12 +1144,13 @@ void dor_rk_chan_1(dor_state *s, int RK_STEP)
         const int JS = s->JSTART_TH[n], JE = s->JEND_TH[n];
         PROD_LOOP(JS, JE, AT(TH, 0) * AT(U1, 0));
         dor_fft_xz_to_fourier(s, S1, 0, NY + 1);
-        ACC_LOOP(JS, JE, CA(CFTH) = CA(CFTH) - CIKX(i) * CA(CS1));
+        ACC_LOOP(JS, JE, CA(CFTH) = CA(CFTH) - PLANT(21, CIKZ(K), CIKX(i)) * CA(CS1));
         PROD_LOOP(JS, JE, AT(TH, 0) * AT(U3, 0));
         dor_fft_xz_to_fourier(s, S1, 0, NY + 1);
-        ACC_LOOP(JS, JE, CA(CFTH) = CA(CFTH) - CIKZ(K) * CA(CS1));
+        ACC_LOOP(JS, JE, CA(CFTH) = PLANT(22, CA(CFTH) + CIKZ(K) * CA(CS1), CA(CFTH) - CIKZ(K) * CA(CS1)));
         PROD_LOOP(JS, JE,
-                  (AT(TH, 1) * AT(U2, 1) + AT(TH, 0) * AT(U2, 1) - AT(TH, 0) * AT(U2, 0) - AT(TH, -1) * AT(U2, 0))
+                  (AT(TH, 1) * AT(U2, 1) + AT(TH, 0) * AT(U2, 1) - AT(TH, 0) * AT(U2, 0)
+                   - AT(TH, -1) * AT(U2, PLANT(23, -1, 0)))
                       / (2.0 * DYF[J]));
         dor_fft_xz_to_fourier(s, S1, 0, NY + 1);
         ACC_LOOP(JS, JE, CA(CFTH) = CA(CFTH) - CA(CS1));
@@ -1147,7 +1163,7 @@ void dor_rk_chan_1(dor_state *s, int RK_STEP)
         for (int J = JS; J <= JE; J++)
             for (int K = 0; K <= NZP - 1; K++)
                 for (int i = 0; i <= NXM; i++)
-                    AT(RTH, 0) = AT(RTH, 0) + (TEMP1 / PR) * NUV
+                    AT(RTH, 0) = AT(RTH, 0) + PLANT(26, TEMP1 * PR, TEMP1 / PR) * NUV
                         * (((AT(TH, 1) - AT(TH, 0)) / DY[J + 1] - (AT(TH, 0) - AT(TH, -1)) / DY[J]) / DYF[J]);
         if (s->les) {
             /* the eddy diffusivity, explicit half (for/channel.f:748-760) */
@@ -1189,7 +1205,8 @@ void dor_rk_chan_1(dor_state *s, int RK_STEP)
     /* CP update (for/channel.f:1059-1065); phi is in CR1. */
     ACC_LOOP(JSTART, JEND, {
         cplx t = CA(CR1);
-        CA(CP) = CA(CP) + (creal(t) / TEMP4 + I * (cimag(t) / TEMP4));
+        const double T4 = PLANT(28, TEMP5, TEMP4);
+        CA(CP) = CA(CP) + (creal(t) / T4 + I * (cimag(t) / T4));
     });
     /* GHOST_CHAN_MPI is a no-op at NPROCY = 1. */
 #undef AT

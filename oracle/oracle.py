@@ -39,23 +39,33 @@ class _Params(C.Structure):
     ]
 
 
-def build(quiet: bool = True) -> str:
-    """Compiles oracle/libdiablo_oracle.so with the committed Makefile."""
-    out = subprocess.run(["make", "-C", _HERE], capture_output=True, text=True)
-    if out.returncode != 0:
-        raise RuntimeError("oracle build failed:\n" + out.stdout + out.stderr)
-    return _LIB
+def build(quiet: bool = True, plant: int = 0, out: str | None = None) -> str:
+    """Compiles oracle/libdiablo_oracle.so with the committed Makefile; with
+    plant = n >= 1, a copy at `out` carrying planted mistake n (the tests of
+    the pins use these copies; nothing else loads them)."""
+    cmd = ["make", "-C", _HERE]
+    if plant:
+        if not out:
+            raise ValueError("a planted build needs its own output path")
+        cmd += [f"LIB={out}", f"PLANT={int(plant)}"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + res.stdout + res.stderr)
+    return out if plant else _LIB
 
 
-_lib = None
+_libs: dict = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(_LIB):
+def lib(path: str | None = None):
+    """The oracle library (default: oracle/libdiablo_oracle.so, built if
+    missing), loaded once per path with its signatures declared."""
+    path = path or _LIB
+    L = _libs.get(path)
+    if L is None:
+        if path == _LIB and not os.path.exists(_LIB):
             build()
-        L = C.CDLL(_LIB)
+        L = C.CDLL(path)
         vp, i, d = C.c_void_p, C.c_int, C.c_double
         dp = C.POINTER(C.c_double)
         L.dor_create.restype = vp
@@ -92,8 +102,8 @@ def lib():
         L.dor_tke_les.argtypes = [vp, dp]
         L.dor_les_chan.argtypes = [vp, dp]
         L.dor_set_les.argtypes = [vp, i, i]
-        _lib = L
-    return _lib
+        _libs[path] = L
+    return L
 
 
 def _dp(a: np.ndarray):
@@ -128,10 +138,11 @@ GY, GYF, DY, DYF, KX, KZ, KX2, KZ2, GX, GZ = range(10)
 class Oracle:
     """The reference's channel state and call surface on the CPU."""
 
-    def __init__(self, cfg: ChannelConfig, gy_1: np.ndarray | None = None, threads: int = 0):
+    def __init__(self, cfg: ChannelConfig, gy_1: np.ndarray | None = None, threads: int = 0,
+                 library: str | None = None):
         cfg.validate()
         self.cfg = cfg
-        self.L = lib()
+        self.L = lib(library)
         if threads:
             self.L.dor_set_threads(threads)
         gy = np.ascontiguousarray(grid_gy(cfg) if gy_1 is None else gy_1, dtype=np.float64)
