@@ -239,18 +239,47 @@ def les_run(cfg, phys, steps: int, warmup: int) -> dict:
             "model": "constant Smagorinsky, les_chan every substep"}
 
 
-def cpu_baseline(cfg, phys, threads: int) -> dict:
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(cfg, phys, threads: int, steps: int = 3, single_core_substeps: int = 1) -> dict:
+    """The oracle (C restatement of the reference path, physical-space CN as
+    in the reference) on the host: `steps` RK3 steps with `threads` OpenMP
+    threads from the same initial state as the GPU legs, then
+    `single_core_substeps` substeps on one core (the reference's one-rank
+    configuration) from the state reached."""
     from oracle import oracle
     o = oracle.Oracle(cfg, threads=threads)
     o.initialize(*phys)
     t0 = time.perf_counter()
-    o.step()
+    for _ in range(steps):
+        o.step()
     dt = time.perf_counter() - t0
-    return {"value": cfg.points * 3 / dt / 1e6, "unit": "Mpts*substeps/s", "cores": threads,
-            "kind": "port",
-            "sample": f"1 RK3 step (3 substeps) of the same {cfg.nx}x{cfg.ny}x{cfg.nz} case from the same "
-                      f"initial state, oracle (C restatement of RK_CHAN_1, physical-space CN as in the "
-                      f"reference) with {threads} OpenMP threads, {dt:.1f} s"}
+    out = {"value": round(cfg.points * 3 * steps / dt / 1e6, 3), "unit": "Mpts*substeps/s", "cores": threads,
+           "kind": "port",
+           "sample": f"{steps} RK3 steps ({3 * steps} substeps) of the same {cfg.nx}x{cfg.ny}x{cfg.nz} case "
+                     f"from the same initial state: the oracle (C restatement of RK_CHAN_1, physical-space CN "
+                     f"as in the reference) with {threads} OpenMP threads, {dt:.1f} s",
+           "cpu_model": cpu_model(), "nproc": os.cpu_count()}
+    if single_core_substeps > 0:
+        oracle.lib().dor_set_threads(1)
+        t0 = time.perf_counter()
+        for k in range(single_core_substeps):
+            o.rk_chan_1(1 + k % 3)
+        d1 = time.perf_counter() - t0
+        oracle.lib().dor_set_threads(threads)
+        out["single_core"] = {"value": round(cfg.points * single_core_substeps / d1 / 1e6, 3),
+                              "unit": "Mpts*substeps/s", "cores": 1,
+                              "sample": f"{single_core_substeps} substep(s) on one thread, {d1:.1f} s"}
+    return out
 
 
 def copy_calibration(mib: int) -> None:
@@ -273,6 +302,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="channel384")
     ap.add_argument("--cpu-baseline", choices=("on", "off"), default="on")
+    ap.add_argument("--cpu-steps", type=int, default=3, help="RK3 steps of the all-core CPU baseline")
+    ap.add_argument("--cpu-single-substeps", type=int, default=1,
+                    help="substeps of the one-core CPU baseline (0: skip)")
     ap.add_argument("--no-kernel-timing", action="store_true")
     ap.add_argument("--mixed", choices=("on", "off"), default="on")
     ap.add_argument("--accuracy-steps", type=int, default=1000,
@@ -395,9 +427,10 @@ def main():
     cpu = None
     if args.cpu_baseline == "on":
         threads = min(16, os.cpu_count() or 1)
-        log(f"[bench] cpu baseline: oracle, 1 step, {threads} threads")
-        cpu = cpu_baseline(cfg, phys, threads)
-        log(f"[bench] cpu baseline {cpu['value']:.3f} Mpts*substeps/s")
+        log(f"[bench] cpu baseline: oracle, {args.cpu_steps} steps on {threads} threads, "
+            f"{args.cpu_single_substeps} substep(s) on 1")
+        cpu = cpu_baseline(cfg, phys, threads, args.cpu_steps, args.cpu_single_substeps)
+        log(f"[bench] cpu baseline {cpu}")
 
     out = {
         "metric": "Mpts·RK-substeps/s FP64 vs mixed+batched at rel-L2≤1e-5, 1 MI355X",

@@ -14,7 +14,7 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libdiablo_gpu.so")
-SOURCES = ["dgpu_kernels.hip", "dgpu_hot_x.hip", "dgpu_hot_z.hip", "dgpu_columns.hip", "dgpu_stats.hip", "dgpu_les.hip", "dgpu_zdft.hip", "dgpu_poisson.hip",
+SOURCES = ["dgpu_kernels.hip", "dgpu_hot_x.hip", "dgpu_hot_z.hip", "dgpu_columns.hip", "dgpu_colseg.hip", "dgpu_stats.hip", "dgpu_les.hip", "dgpu_zdft.hip", "dgpu_poisson.hip",
            "dgpu_capi.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("DGPU_ARCH", "gfx950")

@@ -83,7 +83,7 @@ struct dgpu_ctx {
     long zfs = 0;  // complex elements per intermediate field: (NY+2)*NZ*ld
     double *d_fort = nullptr;  // Fortran-layout staging for host entry points
     double *d_fa = nullptr, *d_fb = nullptr;  // POISSON_P_CHAN work planes
-    double *d_gyf = nullptr;
+    double *d_gyf = nullptr;    // GYF (0:NY+1), part of d_vec
     double *d_stats = nullptr;  // dgpu_stats_chan output
     // COURANT: 0 idle, 1 requested for the next RK_STEP 3, 2 maxima recorded
     int courant_state = 0;
@@ -105,6 +105,7 @@ struct dgpu_ctx {
     double *d_c1 = nullptr;
     hipStream_t stream = nullptr;
     bool own_stream = true;
+    hipStream_t ens_stream = nullptr;  // an ensemble member: the ensemble's stream also reads d_cn
     bool use_graphs = true;
     hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
     double gkey[3][4] = {};
@@ -340,7 +341,12 @@ int ensure_cn(dgpu_ctx *c, int stage, double temp1, double temp4)
     const long blk = cn_block(c);
     double *base = c->h_cn.data() + (long)stage * nsys * blk;
     for (int q = 0; q < nsys; ++q) build_cn(c, q, temp1, temp4, base + q * blk);
+    // every launch that may still read this stage's table must finish first:
+    // the context's streams and, for an ensemble member, the ensemble's stream
+    // (its batched launches and graphs run there)
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->side_stream) HIPCHK(c, hipStreamSynchronize(c->side_stream));
+    if (c->ens_stream) HIPCHK(c, hipStreamSynchronize(c->ens_stream));
     HIPCHK(c, hipMemcpy(c->d_cn + (long)stage * nsys * blk, base, sizeof(double) * nsys * blk, hipMemcpyHostToDevice));
     c->cn_key[stage][0] = key[0];
     c->cn_key[stage][1] = key[1];
@@ -536,8 +542,8 @@ int32_t dgpu_create(const dgpu_params *p, dgpu_ctx **out)
     int rc = 0;
     auto bail = [&](int code) { rc = code; dgpu_destroy(c); return code; };
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return bail(DGPU_E_HIP);
-    // device vectors: dy, dyf (NY+2), kx, kx2 (NXP+1), kz, kz2 (TNKZ+1)
-    const long nv = 2L * (NY + 2) + 2L * (NXP + 1) + 2L * (g.tnkz + 1);
+    // device vectors: dy, dyf (NY+2), kx, kx2 (NXP+1), kz, kz2 (TNKZ+1), gyf (NY+2)
+    const long nv = 3L * (NY + 2) + 2L * (NXP + 1) + 2L * (g.tnkz + 1);
     if (hipMalloc(&c->d_vec, sizeof(double) * nv) != hipSuccess) return bail(DGPU_E_NOMEM);
     {
         std::vector<double> h;
@@ -547,6 +553,7 @@ int32_t dgpu_create(const dgpu_params *p, dgpu_ctx **out)
         h.insert(h.end(), c->kx2.begin(), c->kx2.end());
         h.insert(h.end(), c->kz.begin(), c->kz.end());
         h.insert(h.end(), c->kz2.begin(), c->kz2.end());
+        h.insert(h.end(), c->gyf.begin(), c->gyf.end());
         if (hipMemcpy(c->d_vec, h.data(), sizeof(double) * nv, hipMemcpyHostToDevice) != hipSuccess)
             return bail(DGPU_E_HIP);
         c->d_dy = c->d_vec;
@@ -555,6 +562,7 @@ int32_t dgpu_create(const dgpu_params *p, dgpu_ctx **out)
         c->d_kx2 = c->d_kx + (NXP + 1);
         c->d_kz = c->d_kx2 + (NXP + 1);
         c->d_kz2 = c->d_kz + (g.tnkz + 1);
+        c->d_gyf = c->d_kz2 + (g.tnkz + 1);
     }
     // twiddles
     {
@@ -718,7 +726,7 @@ int32_t dgpu_destroy(dgpu_ctx *c)
         if (ge) hipGraphExecDestroy(ge);
     for (auto e : c->event_pool) hipEventDestroy(e);
     void *bufs[] = {c->d_vec, c->d_tw, c->d_state, c->d_pmult, c->d_pinv, c->d_rd, c->d_nlptr, c->d_zt, c->d_wt,
-                    c->d_fa, c->d_fb, c->d_gyf, c->d_iptw, c->d_iptwf, c->d_ztf, c->d_wtf, c->d_nlf, c->d_nlptrf,
+                    c->d_fa, c->d_fb, c->d_iptw, c->d_iptwf, c->d_ztf, c->d_wtf, c->d_nlf, c->d_nlptrf,
                     c->d_fort, c->d_cn, c->d_codes, c->d_c1, c->d_state32, c->d_side, c->d_map,
                     c->d_pmultf, c->d_pinvf, c->d_gs32, c->d_stats, c->d_vmax_part, c->d_vmax, c->d_file, c->d_les, c->d_zdft};
     for (void *b : bufs)
@@ -739,10 +747,6 @@ int32_t dgpu_stats_chan(dgpu_ctx *c, double *profiles, double *ubulk)
     if (!c || !profiles || !ubulk) return DGPU_E_ARG;
     if (int rc = to_master(c)) return rc;
     const dgpu::Geo &g = c->g;
-    if (!c->d_gyf) {
-        HIPCHK(c, hipMalloc(&c->d_gyf, sizeof(double) * (g.ny + 2)));
-        HIPCHK(c, hipMemcpy(c->d_gyf, c->gyf.data(), sizeof(double) * (g.ny + 2), hipMemcpyHostToDevice));
-    }
     if (!c->d_stats) HIPCHK(c, hipMalloc(&c->d_stats, sizeof(double) * DGPU_NSTAT * g.ny));
     dgpu::launch_stats(c->stream, g, c->cu[0], c->cu[1], c->cu[2], c->cp, g.nth ? c->cth[0] : nullptr, c->d_dy,
                        c->d_dyf, c->d_gyf, c->d_kx, c->d_kz, c->d_stats);
@@ -1232,7 +1236,7 @@ static void fill_ptrs32(dgpu_ctx *c, dgpu::CnPtrs<float, float> &P)
     const int map[4] = {0, 2, 1, 3};
     for (int q = 0; q < nin; ++q) {
         P.nl[q] = c->nlf[map[q]];
-        P.gs[q] = c->d_gs32 + q * fs;
+        P.gs[q] = c->d_gs32 + map[q] * fs;  // field order: U1, U2, U3, TH
     }
 }
 
@@ -1315,28 +1319,42 @@ static void make_cn_args(dgpu_ctx *c, int rk_step, double time, double temp3, do
 // COURANT's velocity maxima (for/channel.f:1037-1040): the velocity after the
 // implicit solves and before the projection, taken to physical space in the
 // precision of the run's transforms; planes 1..NY of CU1..CU3.
-static int enqueue_vmax(dgpu_ctx *c)
+// vel: the velocity after the implicit solves, three consecutive fields
+// (U1, U2, U3) with the state's field stride; null: the state's own CU1..3.
+static int enqueue_vmax(dgpu_ctx *c, const void *vel = nullptr)
 {
     const dgpu::Geo &g = c->g;
     const long zps = (long)g.nz * c->ld;
     if (c->mixed) {
         const cx<float> *twx = c->d_iptwf, *twz = c->d_iptwf + c->ipx.twn;
         if (c->f32state)
-            dgpu::launch_zinv_ip<float, float>(c->stream, g, c->ipz, twz, c->d_state32, c->fs, g.ps, c->d_ztf,
+            dgpu::launch_zinv_ip<float, float>(c->stream, g, c->ipz, twz,
+                                               vel ? (const cx<float> *)vel : c->d_state32, c->fs, g.ps, c->d_ztf,
                                                c->zfs, zps, c->ld, 1, g.ny, 3);
         else
-            dgpu::launch_zinv_ip<float, double>(c->stream, g, c->ipz, twz, c->cu[0], c->fs, g.ps, c->d_ztf, c->zfs,
-                                                zps, c->ld, 1, g.ny, 3);
+            dgpu::launch_zinv_ip<float, double>(c->stream, g, c->ipz, twz, vel ? (const cx<double> *)vel : c->cu[0],
+                                                c->fs, g.ps, c->d_ztf, c->zfs, zps, c->ld, 1, g.ny, 3);
         dgpu::launch_vmax_ip<float>(c->stream, g, c->ipx, twx, c->ld, c->d_ztf, c->zfs, c->d_vmax_part, c->d_vmax);
     } else {
         const cx<double> *twx = c->d_iptw, *twz = c->d_iptw + c->ipx.twn;
-        dgpu::launch_zinv_ip<double, double>(c->stream, g, c->ipz, twz, c->cu[0], c->fs, g.ps, c->d_zt, c->zfs, zps,
-                                             c->ld, 1, g.ny, 3);
+        dgpu::launch_zinv_ip<double, double>(c->stream, g, c->ipz, twz, vel ? (const cx<double> *)vel : c->cu[0],
+                                             c->fs, g.ps, c->d_zt, c->zfs, zps, c->ld, 1, g.ny, 3);
         dgpu::launch_vmax_ip<double>(c->stream, g, c->ipx, twx, c->ld, c->d_zt, c->zfs, c->d_vmax_part, c->d_vmax);
     }
     if (int rc = check_launch(c, "courant maxima")) return rc;
     c->courant_state = 2;
     return DGPU_OK;
+}
+
+// The column stage runs the segmented kernels (dgpu_colseg.hip) unless
+// DGPU_COLSEG=0 selects the whole-column sweeps (dgpu_columns.hip).
+static bool use_colseg()
+{
+    static const bool on = [] {
+        const char *e = std::getenv("DGPU_COLSEG");
+        return !(e && std::atoi(e) == 0);
+    }();
+    return on;
 }
 
 static int rk_enqueue(dgpu_ctx *c, int rk_step, double time, double temp1, double temp3, double temp4, double temp5)
@@ -1407,6 +1425,9 @@ static int rk_enqueue(dgpu_ctx *c, int rk_step, double time, double temp1, doubl
     make_cn_args(c, rk_step, time, temp3, temp4, temp5, A);
     dgpu::RdArgs R;
     fill_rdargs(c, R, 1, temp4);
+    // the segmented column kernels where the column fits them
+    const bool seg64 = use_colseg() && dgpu::colseg_supported<double>(g.ny);
+    const bool seg32 = use_colseg() && dgpu::colseg_supported<float>(g.ny);
     if (c->f32state) {
         // FP32 solves on every mode, then the low modes again in FP64 on their
         // side columns, copied into the working state for the transforms.
@@ -1426,21 +1447,28 @@ static int rk_enqueue(dgpu_ctx *c, int rk_step, double time, double temp1, doubl
         // results then replace the FP32 ones in the working state.
         HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
         HIPCHK(c, hipStreamWaitEvent(c->side_stream, c->ev_fork, 0));
-        dgpu::launch_cn_side<double, float>(c->side_stream, As, Ps);
-        dgpu::launch_remdiv_side<double, float>(c->side_stream, Rs, Ps);
+        if (seg64) {
+            dgpu::launch_cn_seg<double, float>(c->side_stream, As, Ps);
+            dgpu::launch_remdiv_seg<double, float>(c->side_stream, Rs, Ps);
+        } else {
+            dgpu::launch_cn_side<double, float>(c->side_stream, As, Ps);
+            dgpu::launch_remdiv_side<double, float>(c->side_stream, Rs, Ps);
+        }
         if (int rc = check_launch(c, "low-mode columns")) return rc;
         HIPCHK(c, hipEventRecord(c->ev_join, c->side_stream));
         {
             TimerScope t(c, 3);
-            dgpu::launch_cn<float, float>(c->stream, A, P);
+            if (seg32) dgpu::launch_cn_seg<float, float>(c->stream, A, P);
+            else dgpu::launch_cn<float, float>(c->stream, A, P);
         }
         if (int rc = check_launch(c, "cn")) return rc;
         if (courant) {
-            if (int rc = enqueue_vmax(c)) return rc;
+            if (int rc = enqueue_vmax(c, seg32 ? (const void *)P.gs[dgpu::SYS_U1] : nullptr)) return rc;
         }
         {
             TimerScope t(c, 4);
-            dgpu::launch_remdiv<float, float>(c->stream, R, P);
+            if (seg32) dgpu::launch_remdiv_seg<float, float>(c->stream, R, P);
+            else dgpu::launch_remdiv<float, float>(c->stream, R, P);
         }
         if (int rc = check_launch(c, "remdiv")) return rc;
         HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
@@ -1452,30 +1480,34 @@ static int rk_enqueue(dgpu_ctx *c, int rk_step, double time, double temp1, doubl
         fill_ptrs<float>(c, P, c->nlf);
         {
             TimerScope t(c, 3);
-            dgpu::launch_cn<double, float>(c->stream, A, P);
+            if (seg64) dgpu::launch_cn_seg<double, float>(c->stream, A, P);
+            else dgpu::launch_cn<double, float>(c->stream, A, P);
         }
         if (int rc = check_launch(c, "cn")) return rc;
         if (courant) {
-            if (int rc = enqueue_vmax(c)) return rc;
+            if (int rc = enqueue_vmax(c, seg64 ? (const void *)P.gs[dgpu::SYS_U1] : nullptr)) return rc;
         }
         {
             TimerScope t(c, 4);
-            dgpu::launch_remdiv<double, float>(c->stream, R, P);
+            if (seg64) dgpu::launch_remdiv_seg<double, float>(c->stream, R, P);
+            else dgpu::launch_remdiv<double, float>(c->stream, R, P);
         }
     } else {
         dgpu::CnPtrs<double, double> P;
         fill_ptrs<double>(c, P, c->nl);
         {
             TimerScope t(c, 3);
-            dgpu::launch_cn<double, double>(c->stream, A, P);
+            if (seg64) dgpu::launch_cn_seg<double, double>(c->stream, A, P);
+            else dgpu::launch_cn<double, double>(c->stream, A, P);
         }
         if (int rc = check_launch(c, "cn")) return rc;
         if (courant) {
-            if (int rc = enqueue_vmax(c)) return rc;
+            if (int rc = enqueue_vmax(c, seg64 ? (const void *)P.gs[dgpu::SYS_U1] : nullptr)) return rc;
         }
         {
             TimerScope t(c, 4);
-            dgpu::launch_remdiv<double, double>(c->stream, R, P);
+            if (seg64) dgpu::launch_remdiv_seg<double, double>(c->stream, R, P);
+            else dgpu::launch_remdiv<double, double>(c->stream, R, P);
         }
     }
     if (int rc = check_launch(c, "remdiv")) return rc;
@@ -1881,10 +1913,6 @@ int32_t dgpu_poisson_p_chan(dgpu_ctx *c)
     const size_t n = (size_t)(g.nx + 2) * (g.nz + 2) * (g.ny + 2);
     if (!c->d_fa) HIPCHK(c, hipMalloc(&c->d_fa, n * sizeof(double)));
     if (!c->d_fb) HIPCHK(c, hipMalloc(&c->d_fb, n * sizeof(double)));
-    if (!c->d_gyf) {
-        HIPCHK(c, hipMalloc(&c->d_gyf, sizeof(double) * (g.ny + 2)));
-        HIPCHK(c, hipMemcpy(c->d_gyf, c->gyf.data(), sizeof(double) * (g.ny + 2), hipMemcpyHostToDevice));
-    }
     double *fa = c->d_fa, *fb = c->d_fb, *fc = c->d_fort;
     cx<double> *cs = c->nl[1], *scr = c->nl[0];
     const int j0 = 2, j1 = g.ny;
@@ -2137,11 +2165,15 @@ static int ens_enqueue_batched(dgpu_ensemble *e, int rk_step)
             dgpu::Geo gs = e->m[gr.idx[0]]->g;
             gs.ps = e->m[gr.idx[0]]->nside;
             const int nm = (int)gr.idx.size();
-            dgpu::launch_cn_side_batch<double, float>(e->side, gs, (const dgpu::ColDesc<double, float> *)gr.d_cols[k],
-                                                      nm);
-            dgpu::launch_remdiv_side_batch<double, float>(e->side, gs,
-                                                          (const dgpu::ColDesc<double, float> *)gr.d_cols[k],
-                                                     nm);
+            const int max_ny = gr.max_planes - 2;
+            const auto *d = (const dgpu::ColDesc<double, float> *)gr.d_cols[k];
+            if (use_colseg() && dgpu::colseg_supported<double>(max_ny)) {
+                dgpu::launch_cn_seg_batch<double, float>(e->side, gs, max_ny, d, nm);
+                dgpu::launch_remdiv_seg_batch<double, float>(e->side, gs, max_ny, d, nm);
+            } else {
+                dgpu::launch_cn_side_batch<double, float>(e->side, gs, d, nm);
+                dgpu::launch_remdiv_side_batch<double, float>(e->side, gs, d, nm);
+            }
         }
         if (hipGetLastError() != hipSuccess) return DGPU_E_HIP;
         if (hipEventRecord(e->ev_j, e->side) != hipSuccess) return DGPU_E_HIP;
@@ -2149,15 +2181,34 @@ static int ens_enqueue_batched(dgpu_ensemble *e, int rk_step)
     for (auto &gr : e->groups) {
         const dgpu::Geo &g = e->m[gr.idx[0]]->g;
         const int nm = (int)gr.idx.size();
+        const int max_ny = gr.max_planes - 2;
         if (!mixed) {
-            dgpu::launch_cn_batch<double, double>(s, g, (const dgpu::ColDesc<double, double> *)gr.d_col[k], nm);
-            dgpu::launch_remdiv_batch<double, double>(s, g, (const dgpu::ColDesc<double, double> *)gr.d_col[k], nm);
+            const auto *d = (const dgpu::ColDesc<double, double> *)gr.d_col[k];
+            if (use_colseg() && dgpu::colseg_supported<double>(max_ny)) {
+                dgpu::launch_cn_seg_batch<double, double>(s, g, max_ny, d, nm);
+                dgpu::launch_remdiv_seg_batch<double, double>(s, g, max_ny, d, nm);
+            } else {
+                dgpu::launch_cn_batch<double, double>(s, g, d, nm);
+                dgpu::launch_remdiv_batch<double, double>(s, g, d, nm);
+            }
         } else if (!f32) {
-            dgpu::launch_cn_batch<double, float>(s, g, (const dgpu::ColDesc<double, float> *)gr.d_col[k], nm);
-            dgpu::launch_remdiv_batch<double, float>(s, g, (const dgpu::ColDesc<double, float> *)gr.d_col[k], nm);
+            const auto *d = (const dgpu::ColDesc<double, float> *)gr.d_col[k];
+            if (use_colseg() && dgpu::colseg_supported<double>(max_ny)) {
+                dgpu::launch_cn_seg_batch<double, float>(s, g, max_ny, d, nm);
+                dgpu::launch_remdiv_seg_batch<double, float>(s, g, max_ny, d, nm);
+            } else {
+                dgpu::launch_cn_batch<double, float>(s, g, d, nm);
+                dgpu::launch_remdiv_batch<double, float>(s, g, d, nm);
+            }
         } else {
-            dgpu::launch_cn_batch<float, float>(s, g, (const dgpu::ColDesc<float, float> *)gr.d_col[k], nm);
-            dgpu::launch_remdiv_batch<float, float>(s, g, (const dgpu::ColDesc<float, float> *)gr.d_col[k], nm);
+            const auto *d = (const dgpu::ColDesc<float, float> *)gr.d_col[k];
+            if (use_colseg() && dgpu::colseg_supported<float>(max_ny)) {
+                dgpu::launch_cn_seg_batch<float, float>(s, g, max_ny, d, nm);
+                dgpu::launch_remdiv_seg_batch<float, float>(s, g, max_ny, d, nm);
+            } else {
+                dgpu::launch_cn_batch<float, float>(s, g, d, nm);
+                dgpu::launch_remdiv_batch<float, float>(s, g, d, nm);
+            }
         }
         if (hipGetLastError() != hipSuccess) return DGPU_E_HIP;
     }
@@ -2223,6 +2274,7 @@ int32_t dgpu_ensemble_create(const dgpu_params *members, int32_t n, dgpu_ensembl
         dgpu_ensemble_destroy(e);
         return DGPU_E_HIP;
     }
+    for (dgpu_ctx *c : e->m) c->ens_stream = e->main;
     e->join.assign(n, nullptr);
     for (int i = 0; i < n; ++i)
         if (hipEventCreateWithFlags(&e->join[i], hipEventDisableTiming) != hipSuccess) {
@@ -2262,6 +2314,13 @@ int32_t dgpu_ensemble_rk_chan_1(dgpu_ensemble *e, int32_t rk_step, const double 
     const int n = (int)e->m.size();
     std::vector<double> key;
     bool graphable = e->use_graphs;
+    // a member's COURANT request (dgpu_courant_request) is served at RK
+    // stage 3 by the per-member launches, which record the velocity maxima;
+    // that call runs without graphs and without the batched launches
+    bool courant = false;
+    for (int i = 0; i < n; ++i) courant = courant || (rk_step == 3 && e->m[i]->courant_state == 1);
+    if (courant) graphable = false;
+    const bool batched = e->batched && !courant;
     for (int i = 0; i < n; ++i) {
         dgpu_ctx *c = e->m[i];
         const double h = h_bar[3 * i + k];
@@ -2283,14 +2342,18 @@ int32_t dgpu_ensemble_rk_chan_1(dgpu_ensemble *e, int32_t rk_step, const double 
         }
         c->work_dirty = c->f32state;
     }
-    if (e->batched) {
+    if (batched) {
         if (int rc = ens_refresh_columns(e, rk_step, key)) {
             e->err = "batched column descriptors";
             return rc;
         }
+    } else if (e->batched) {
+        // earlier batched launches on the ensemble stream finish before the
+        // members' own streams take over
+        if (hipStreamSynchronize(e->main) != hipSuccess) return DGPU_E_HIP;
     }
     auto enqueue_all = [&](bool joined) -> int {
-        if (e->batched) {
+        if (batched) {
             // one launch per kernel kind and member group on the ensemble stream
             if (!joined) {
                 for (int i = 0; i < n; ++i)

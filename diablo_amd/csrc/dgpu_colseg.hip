@@ -1,0 +1,578 @@
+// dgpu_colseg.hip -- the per-mode column stage of RK_CHAN_1 with each
+// y-column split into segments (for/channel.f:88-290, 581-1029 for the
+// Crank-Nicolson part, for/channel.f:1103-1214 and 1059-1065 for REM_DIV_CHAN
+// and the CP update).
+//
+// A tridiagonal solve by elimination is two linear recurrences over the rows:
+// g(j) = r(j) + m(j) g(j-1) forward, x(j) = (g(j) - U(j) x(j+1)) / B(j)
+// backward.  A recurrence over a segment [a, b] of rows started from zero
+// differs from the true one by its carry times a product of its
+// coefficients:
+//   g(j) = gl(j) + C(j) g(a-1),   C(j) = m(a) ... m(j),
+//   x(j) = xl(j) + E(j) x(b+1),   E(j) = (-U(j)/B(j)) ... (-U(b)/B(b)).
+// So every segment runs its rows locally (a lane holds its segment's rows in
+// registers), the carries pass from segment to segment through LDS (one
+// product and one addition per segment), and each lane corrects its rows.
+// The solve's intermediate values never leave the chip, and the sequential
+// chain of a lane is one segment (9 to 17 rows) long instead of a column.
+//
+// k_cn_seg: one block per (64 or 32 modes, system): the explicit right-hand
+//   side, the new CF, the CN solve; the solution of every row goes to the
+//   system's solve scratch, which holds the complete new field afterwards
+//   (rows outside the solve's copy range keep the old value).  The state's
+//   own fields are only read: the other systems' blocks read them as
+//   explicit terms (Coriolis, buoyancy, background gradients).
+// k_remdiv_seg: one block per mode group: the divergence of the new
+//   velocity (from the scratch), the Poisson solve with the per-mode
+//   factorisation, the projection into CU1..3, the CP update, and the copy
+//   of the new scalar into CTH.
+#include "dgpu_kernels.hpp"
+
+namespace dgpu {
+
+namespace {
+
+template <typename T>
+__device__ __forceinline__ cx<T> cfma(T a, cx<T> x, cx<T> y) { return make_cx<T>(y.re + a * x.re, y.im + a * x.im); }
+
+// table entries kept in LDS, per row
+enum { L_MULT = 0, L_INVPIV, L_UP, L_BC00, L_ISBC, L_WP, L_W0, L_WM, L_CPW, L_B0, L_BM, L_CFR, L_CPY, kLRow };
+constexpr int kTabMap[kLRow] = {T_MULT, T_INVPIV, T_UP, T_BC00, T_ISBC, T_WP, T_W0, T_WM,
+                                T_CPW,  T_B0,     T_BM, T_CFR,  T_CPY};
+
+__device__ __forceinline__ int xcd_block(int L, int nb8) { return (L & 7) * (nb8 >> 3) + (L >> 3); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// k_cn_seg
+// ---------------------------------------------------------------------------
+template <typename T, int NTH, typename NLT, int MPW, int RMAX, typename AT, typename PT>
+__device__ __forceinline__ void cn_seg_body(AT A, PT P, int bx, int q)
+{
+    constexpr int SPW = 64 / MPW;
+    constexpr int NSEG = 16 * SPW;
+    extern __shared__ unsigned char smem_raw[];
+    const Geo &g = A.g;
+    const int NY = g.ny, N = NY + 2;
+    const long ps = g.ps;
+    const int R = (N + NSEG - 1) / NSEG;  // rows per segment, <= RMAX (host check)
+    const int nseg = (N + R - 1) / R;
+    T *tl = reinterpret_cast<T *>(smem_raw);                               // [N][kLRow]
+    cx<T> *GL = reinterpret_cast<cx<T> *>(tl + ((N * kLRow + 1) & ~1));   // [NSEG][MPW]
+    cx<T> *XL = GL + NSEG * MPW;                                           // [NSEG][MPW]
+    T *Cb = reinterpret_cast<T *>(XL + NSEG * MPW);                        // [NSEG]
+    T *Ea = Cb + NSEG;                                                     // [NSEG]
+
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int seg = (tid >> 6) * SPW + lane / MPW, ml = lane % MPW;
+    const long m = (long)bx * MPW + ml;
+    const bool live = m < ps;
+    const int a = seg * R, b = min(a + R, N) - 1;
+    const bool rows = live && a <= b;
+
+    const double *tab = A.tab + (long)q * N * kRow;
+    for (int e = tid; e < N * kLRow; e += blockDim.x) {
+        const int j = e / kLRow, k = e - j * kLRow;
+        tl[e] = T(tab[(long)j * kRow + kTabMap[k]]);
+    }
+
+    int s = 0, i = 0;
+    const long gm = (live && A.map) ? (long)A.map[m] : m;
+    if (live) { s = (int)(gm / g.mk); i = (int)(gm - (long)s * g.mk); }
+    const bool is00 = live && i == 0 && s == 0;
+    const long nps = A.nl_ps;
+    const T kx = live ? T(A.kx[i]) : T(0), kz = live ? T(A.kz[s]) : T(0);
+    const T k2 = live ? T(A.kx2[i]) + T(A.kz2[s]) : T(0);
+    const T pw = (A.beta == 1.0) ? k2 : T(pow(double(k2), A.beta));
+    const T temp3 = T(A.temp3), temp4 = T(A.temp4), temp5 = T(A.temp5);
+    const bool rk_gt1 = A.rk_gt1 != 0;
+    const int f_type = A.f_type;
+    const T i_ro = T(A.i_ro), rgx = T(A.rgx), rgz = T(A.rgz), drhodx = T(A.drhodx), drhodz = T(A.drhodz);
+    const T force00 = T(A.force00);
+    const T visc = (q == SYS_TH0) ? T(A.nupr) * pw : T(A.nu) * pw;
+    cx<T> *fld = (q == SYS_U1) ? P.cu1 : (q == SYS_U3) ? P.cu3 : (q == SYS_U2) ? P.cu2 : P.cth;
+    cx<T> *cf = (q == SYS_U1) ? P.cf1 : (q == SYS_U3) ? P.cf3 : (q == SYS_U2) ? P.cf2 : P.cfth;
+    const cx<NLT> *nl = P.nl[q];
+    cx<T> *out = P.gs[q];
+    const cx<T> *a1p = (q == SYS_U1) ? P.cu3 : P.cu1;   // Coriolis partner / TH advection
+    const cx<T> *a2p = (q == SYS_TH0) ? P.cu3 : P.cth;  // buoyancy / TH advection
+    const bool has1 = (q != SYS_U2);
+    const bool has2 = (q == SYS_TH0) || NTH;
+    const bool hascp = (q != SYS_TH0);
+    const cx<T> zero = make_cx<T>(0, 0);
+    __syncthreads();  // table
+
+    // ---- forward, local: right-hand side, CF, elimination from zero ----
+    cx<T> yv[RMAX];
+    if (rows) {
+        struct In {
+            cx<T> fp, nl, cf, aux1, aux2, cp;
+        };
+        auto load = [&](int j) {
+            const long o = m + (long)j * ps;
+            In in;
+            in.fp = fld[o + (j <= NY ? ps : 0)];
+            const cx<NLT> nv = nl[gm + (long)j * nps];
+            in.nl = make_cx<T>(T(nv.re), T(nv.im));
+            in.cf = rk_gt1 ? cf[o] : zero;
+            in.aux1 = has1 ? a1p[o] : zero;
+            in.aux2 = has2 ? a2p[o] : zero;
+            in.cp = hascp ? P.cp[o] : zero;
+            return in;
+        };
+        const long oa = m + (long)a * ps;
+        cx<T> fm = a >= 1 ? fld[oa - ps] : zero, f0 = fld[oa];
+        cx<T> thm = (a >= 1 && has2) ? a2p[oa - ps] : zero;
+        cx<T> cpm = (a >= 1 && hascp) ? P.cp[oa - ps] : zero;
+        cx<T> gl = zero;
+        T cprod = T(1);
+        In cur = load(a);
+#pragma unroll
+        for (int r = 0; r < RMAX; ++r) {
+            const int j = a + r;
+            if (j <= b) {
+                const In nxt = load(min(j + 1, b));
+                const T *t = tl + j * kLRow;
+                const T mult = t[L_MULT];
+                const bool isbc = t[L_ISBC] != T(0);
+                const bool cfr = t[L_CFR] != T(0);
+                const cx<T> fp = (j <= NY) ? cur.fp : zero;
+                const bool use = cfr || !isbc;
+                const cx<T> nlv = use ? cur.nl : zero;
+                const cx<T> cfo = use ? cur.cf : zero;
+                const cx<T> aux1 = cur.aux1, aux2 = cur.aux2, cp0 = cur.cp;
+                // new explicit term CF (for/channel.f:175-290 + nonlinear terms)
+                cx<T> cfn = make_cx<T>(-(visc * f0.re), -(visc * f0.im));
+                if (q == SYS_U1) {
+                    cfn = cfma(i_ro, aux1, cfn);
+                    if (NTH) cfn = cfma(rgx, aux2, cfn);
+                } else if (q == SYS_U3) {
+                    cfn = cfma(-i_ro, aux1, cfn);
+                    if (NTH) cfn = cfma(rgz, aux2, cfn);
+                } else if (q == SYS_U2) {
+                    if (NTH && !is00)
+                        cfn = cfn + make_cx<T>(t[L_B0] * aux2.re + t[L_BM] * thm.re, t[L_B0] * aux2.im + t[L_BM] * thm.im);
+                } else {
+                    cfn = cfn - make_cx<T>(aux1.re * drhodx + aux2.re * drhodz, aux1.im * drhodx + aux2.im * drhodz);
+                }
+                cfn = cfn + nlv;
+                cx<T> vec;
+                if (isbc) {
+                    vec = is00 ? make_cx<T>(t[L_BC00], 0) : zero;
+                } else {
+                    cx<T> cr = f0;
+                    if (rk_gt1) cr = cfma(temp3, cfo, cr);
+                    if (q == SYS_U1) {
+                        cr = cr - make_cx<T>(-temp4 * (kx * cp0.im), temp4 * (kx * cp0.re));
+                        if (is00 && (f_type == 1 || f_type == 2)) cr.re = cr.re - temp4 * force00;
+                    } else if (q == SYS_U3) {
+                        cr = cr - make_cx<T>(-temp4 * (kz * cp0.im), temp4 * (kz * cp0.re));
+                    } else if (q == SYS_U2) {
+                        const T w = t[L_CPW];
+                        cr = cr - make_cx<T>(w * (cp0.re - cpm.re), w * (cp0.im - cpm.im));
+                    }
+                    cr = cfma(temp5, cfn, cr);
+                    const T wp = t[L_WP], w0 = t[L_W0], wm = t[L_WM];
+                    vec = make_cx<T>(cr.re + (wp * fp.re + w0 * f0.re + wm * fm.re),
+                                     cr.im + (wp * fp.im + w0 * f0.im + wm * fm.im));
+                }
+                gl = (r == 0) ? vec : cfma(mult, gl, vec);
+                cprod = (r == 0) ? mult : cprod * mult;
+                yv[r] = gl;
+                if (cfr) cf[m + (long)j * ps] = cfn;
+                fm = f0;
+                f0 = fp;
+                thm = aux2;
+                cpm = cp0;
+                cur = nxt;
+            }
+        }
+        GL[seg * MPW + ml] = gl;
+        if (ml == 0) Cb[seg] = cprod;
+    }
+    __syncthreads();
+    // ---- carries of the forward recurrence, back substitution, local ----
+    if (rows) {
+        cx<T> gin = zero;
+        for (int t = 0; t < seg; ++t) gin = cfma(Cb[t], gin, GL[t * MPW + ml]);
+        T c = T(1);
+#pragma unroll
+        for (int r = 0; r < RMAX; ++r) {
+            const int j = a + r;
+            if (j <= b) {
+                c *= tl[j * kLRow + L_MULT];
+                yv[r] = cfma(c, gin, yv[r]);
+            }
+        }
+        cx<T> xl = zero;
+        T e = T(1);
+        bool first = true;
+#pragma unroll
+        for (int r = RMAX - 1; r >= 0; --r) {
+            const int j = a + r;
+            if (j <= b) {
+                const T ip = tl[j * kLRow + L_INVPIV], u = tl[j * kLRow + L_UP];
+                if (first) {
+                    xl = make_cx<T>(yv[r].re * ip, yv[r].im * ip);
+                    e = -u * ip;
+                    first = false;
+                } else {
+                    xl = make_cx<T>((yv[r].re - u * xl.re) * ip, (yv[r].im - u * xl.im) * ip);
+                    e = (-u * ip) * e;
+                }
+                yv[r] = xl;
+            }
+        }
+        XL[seg * MPW + ml] = xl;
+        if (ml == 0) Ea[seg] = e;
+    }
+    __syncthreads();
+    // ---- carries of the backward recurrence, the solution of every row ----
+    if (rows) {
+        cx<T> xin = zero;
+        for (int t = nseg - 1; t > seg; --t) xin = cfma(Ea[t], xin, XL[t * MPW + ml]);
+        T e = T(1);
+        bool first = true;
+#pragma unroll
+        for (int r = RMAX - 1; r >= 0; --r) {
+            const int j = a + r;
+            if (j <= b) {
+                const T *t = tl + j * kLRow;
+                const T f = -t[L_UP] * t[L_INVPIV];
+                e = first ? f : f * e;
+                first = false;
+                const long o = m + (long)j * ps;
+                out[o] = (t[L_CPY] != T(0)) ? cfma(e, xin, yv[r]) : fld[o];
+            }
+        }
+    }
+}
+
+template <typename T, int NTH, typename NLT, int MPW, int RMAX>
+__global__ void __launch_bounds__(1024) k_cn_seg(CnArgs A, CnPtrs<T, NLT> P)
+{
+    cn_seg_body<T, NTH, NLT, MPW, RMAX, const CnArgs &, const CnPtrs<T, NLT> &>(
+        A, P, xcd_block(blockIdx.x, gridDim.x), blockIdx.y);
+}
+
+template <typename T, int NTH, typename NLT, int MPW, int RMAX>
+__global__ void __launch_bounds__(1024) k_cn_seg_batch(const ColDesc<T, NLT> *__restrict__ d)
+{
+    const ColDesc<T, NLT> &D = d[blockIdx.z];
+    if ((long)blockIdx.x * MPW >= D.A.g.ps) return;
+    cn_seg_body<T, NTH, NLT, MPW, RMAX, const CnArgs &, const CnPtrs<T, NLT> &>(D.A, D.P, blockIdx.x, blockIdx.y);
+}
+
+// ---------------------------------------------------------------------------
+// k_remdiv_seg
+// ---------------------------------------------------------------------------
+// Rows of the Poisson solve: 1..NY (row 1 is the gauge / Neumann row with a
+// zero right-hand side, row NY the top Neumann row).  The new velocity U*
+// and scalar come from the solve scratch (P.gs, slot per system); with
+// P.gs[q] equal to the state's own field the call works in place
+// (REM_DIV_CHAN on the state).
+template <typename T, typename NLT, int MPW, int RMAX, int NTH>
+__device__ __forceinline__ void remdiv_seg_body(const RdArgs &A, const CnPtrs<T, NLT> &P, int bx)
+{
+    constexpr int SPW = 64 / MPW;
+    constexpr int NSEG = 16 * SPW;
+    extern __shared__ unsigned char smem_raw[];
+    const Geo &g = A.g;
+    const int NY = g.ny;
+    const long ps = g.ps, fps = A.fac_ps;
+    const int R = (NY + NSEG - 1) / NSEG;
+    const int nseg = (NY + R - 1) / R;
+    cx<T> *GL = reinterpret_cast<cx<T> *>(smem_raw);  // [NSEG][MPW]
+    cx<T> *XL = GL + NSEG * MPW;
+    T *Cb = reinterpret_cast<T *>(XL + NSEG * MPW);   // [NSEG][MPW]
+    T *Ea = Cb + NSEG * MPW;
+
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int seg = (tid >> 6) * SPW + lane / MPW, ml = lane % MPW;
+    const long m = (long)bx * MPW + ml;
+    const bool live = m < ps;
+    const int a = 1 + seg * R, b = min(a + R - 1, NY);
+    const bool rows = live && a <= b;
+    int s = 0, i = 0;
+    const long gm = (live && A.map) ? (long)A.map[m] : m;
+    if (live) { s = (int)(gm / g.mk); i = (int)(gm - (long)s * g.mk); }
+    const bool is00 = live && i == 0 && s == 0;
+    const T kx = live ? T(A.kx[i]) : T(0), kz = live ? T(A.kz[s]) : T(0);
+    const cx<T> *u1 = P.gs[SYS_U1], *u2 = P.gs[SYS_U2], *u3 = P.gs[SYS_U3];
+    const cx<T> zero = make_cx<T>(0, 0);
+    auto pm = [&](int j) -> T {
+        if constexpr (sizeof(T) == 4) return A.pmultf[gm + (long)j * fps];
+        else return T(A.pmult[gm + (long)j * fps]);
+    };
+    auto pv = [&](int j) -> T {
+        if constexpr (sizeof(T) == 4) return A.pinvf[gm + (long)j * fps];
+        else return T(A.pinv[gm + (long)j * fps]);
+    };
+    // i k x
+    auto ik = [](T k, cx<T> x) { return make_cx<T>(-k * x.im, k * x.re); };
+
+    // ---- forward, local: divergence of U* and the Poisson elimination ----
+    cx<T> yv[RMAX];
+    T mr[RMAX], fr[RMAX];  // per row: the multiplier, -U / pivot
+    if (rows) {
+        cx<T> hl = zero;
+        T cprod = T(1);
+        cx<T> c2 = u2[m + (long)a * ps];
+#pragma unroll
+        for (int r = 0; r < RMAX; ++r) {
+            const int j = a + r;
+            if (j <= b) {
+                const long o = m + (long)j * ps;
+                const cx<T> c2p = u2[o + ps];
+                cx<T> vec = zero;
+                if (j >= 2 && j < NY) {
+                    // (i kx CU1 + (CU2(j+1) - CU2(j))/DYF(j)) + i kz CU3 (for/channel.f:1139-1141)
+                    const T idf = T(A.rd[(long)j * 4 + 1]);
+                    const cx<T> d1 = ik(kx, u1[o]), d3 = ik(kz, u3[o]);
+                    vec = make_cx<T>((d1.re + (c2p.re - c2.re) * idf) + d3.re, (d1.im + (c2p.im - c2.im) * idf) + d3.im);
+                }
+                const T mu = j >= 2 ? pm(j) : T(0);
+                hl = (r == 0) ? vec : cfma(mu, hl, vec);
+                cprod = (r == 0) ? mu : cprod * mu;
+                yv[r] = hl;
+                mr[r] = mu;
+                c2 = c2p;
+            }
+        }
+        GL[seg * MPW + ml] = hl;
+        Cb[seg * MPW + ml] = cprod;
+    }
+    __syncthreads();
+    const T u1row = is00 ? T(0) : T(-1);
+    if (rows) {
+        cx<T> hin = zero;
+        for (int t = 0; t < seg; ++t) hin = cfma(Cb[t * MPW + ml], hin, GL[t * MPW + ml]);
+        T c = T(1);
+#pragma unroll
+        for (int r = 0; r < RMAX; ++r) {
+            const int j = a + r;
+            if (j <= b) {
+                c *= mr[r];
+                yv[r] = cfma(c, hin, yv[r]);
+            }
+        }
+        cx<T> xl = zero;
+        T e = T(1);
+        bool first = true;
+#pragma unroll
+        for (int r = RMAX - 1; r >= 0; --r) {
+            const int j = a + r;
+            if (j <= b) {
+                const T ip = pv(j);
+                const T u = (j == 1) ? u1row : T(A.rd[(long)j * 4 + 0]);
+                fr[r] = -u * ip;
+                if (first) {
+                    xl = make_cx<T>((yv[r].re) * ip, (yv[r].im) * ip);
+                    e = fr[r];
+                    first = false;
+                } else {
+                    xl = make_cx<T>((yv[r].re - u * xl.re) * ip, (yv[r].im - u * xl.im) * ip);
+                    e = fr[r] * e;
+                }
+                yv[r] = xl;
+            }
+        }
+        XL[seg * MPW + ml] = xl;
+        Ea[seg * MPW + ml] = e;
+    }
+    __syncthreads();
+    // ---- phi of every row, projection, CP update, copies ----
+    if (rows) {
+        cx<T> xin = zero;
+        for (int t = nseg - 1; t > seg; --t) xin = cfma(Ea[t * MPW + ml], xin, XL[t * MPW + ml]);
+        const T itemp4 = T(A.inv_temp4);
+        const bool upd = A.update_cp;
+        const bool copy = P.gs[SYS_U1] != P.cu1;
+        cx<T> phip = xin;  // phi(b+1)
+        T e = T(1);
+        bool first = true;
+#pragma unroll
+        for (int r = RMAX - 1; r >= 0; --r) {
+            const int j = a + r;
+            if (j <= b) {
+                const long o = m + (long)j * ps;
+                e = first ? fr[r] : fr[r] * e;
+                first = false;
+                const cx<T> phi = cfma(e, xin, yv[r]);
+                // CU1 - i kx phi, CU3 - i kz phi (rows 1..NY), CU2(j+1) - (phi(j+1) - phi(j))/DY(j+1)
+                P.cu1[o] = u1[o] - ik(kx, phi);
+                P.cu3[o] = u3[o] - ik(kz, phi);
+                if (j + 1 <= NY) {
+                    const T idy = T(A.rd[(long)(j + 1) * 4 + 2]);
+                    const cx<T> v = u2[o + ps];
+                    P.cu2[o + ps] = make_cx<T>(v.re - (phip.re - phi.re) * idy, v.im - (phip.im - phi.im) * idy);
+                }
+                if (upd && j >= A.jstart && j <= A.jend) {
+                    const cx<T> p0 = P.cp[o];
+                    P.cp[o] = make_cx<T>(p0.re + phi.re * itemp4, p0.im + phi.im * itemp4);
+                }
+                if (NTH && copy) P.cth[o] = P.gs[SYS_TH0][o];
+                phip = phi;
+            }
+        }
+        if (copy) {
+            // rows the projection does not touch: CU1, CU3 at 0 and NY+1, CU2 at
+            // 0, 1 and NY+1, the scalar at 0 and NY+1
+            if (seg == 0) {
+                const long o0 = m, o1 = m + ps;
+                P.cu1[o0] = u1[o0];
+                P.cu3[o0] = u3[o0];
+                P.cu2[o0] = u2[o0];
+                P.cu2[o1] = u2[o1];
+                if (NTH) P.cth[o0] = P.gs[SYS_TH0][o0];
+            }
+            if (b == NY) {
+                const long o = m + (long)(NY + 1) * ps;
+                P.cu1[o] = u1[o];
+                P.cu3[o] = u3[o];
+                P.cu2[o] = u2[o];
+                if (NTH) P.cth[o] = P.gs[SYS_TH0][o];
+            }
+        }
+    }
+}
+
+template <typename T, typename NLT, int MPW, int RMAX, int NTH>
+__global__ void __launch_bounds__(1024) k_remdiv_seg(RdArgs A, CnPtrs<T, NLT> P)
+{
+    remdiv_seg_body<T, NLT, MPW, RMAX, NTH>(A, P, xcd_block(blockIdx.x, gridDim.x));
+}
+
+template <typename T, typename NLT, int MPW, int RMAX, int NTH>
+__global__ void __launch_bounds__(1024) k_remdiv_seg_batch(const ColDesc<T, NLT> *__restrict__ d)
+{
+    const ColDesc<T, NLT> &D = d[blockIdx.z];
+    if ((long)blockIdx.x * MPW >= D.R.g.ps) return;
+    remdiv_seg_body<T, NLT, MPW, RMAX, NTH>(D.R, D.P, blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+namespace {
+
+template <typename T>
+constexpr int seg_mpw() { return sizeof(T) == 4 ? 64 : 32; }
+template <typename T>
+constexpr int seg_nseg() { return 16 * (64 / seg_mpw<T>()); }
+
+template <typename T>
+size_t cn_seg_lds(int ny)
+{
+    constexpr int MPW = seg_mpw<T>(), NSEG = seg_nseg<T>();
+    const int n = ny + 2;
+    return sizeof(T) * (size_t)((n * kLRow + 1) & ~1) + 2 * sizeof(cx<T>) * NSEG * MPW + 2 * sizeof(T) * NSEG;
+}
+
+template <typename T>
+size_t remdiv_seg_lds()
+{
+    constexpr int MPW = seg_mpw<T>(), NSEG = seg_nseg<T>();
+    return 2 * sizeof(cx<T>) * NSEG * MPW + 2 * sizeof(T) * NSEG * MPW;
+}
+
+// rows per segment of the tallest column -> the compile-time bound
+template <typename F>
+bool with_rmax(int rows, F &&f)
+{
+    if (rows <= 9) { f(std::integral_constant<int, 9>{}); return true; }
+    if (rows <= 17) { f(std::integral_constant<int, 17>{}); return true; }
+    return false;
+}
+
+}  // namespace
+
+template <typename T>
+bool colseg_supported(int ny)
+{
+    constexpr int NSEG = seg_nseg<T>();
+    const int rows = (ny + 2 + NSEG - 1) / NSEG;
+    return ny >= 4 && rows <= 17 && cn_seg_lds<T>(ny) <= 64 * 1024;
+}
+
+template <typename T, typename NLT>
+void launch_cn_seg(hipStream_t s, const CnArgs &a, const CnPtrs<T, NLT> &p)
+{
+    constexpr int MPW = seg_mpw<T>(), NSEG = seg_nseg<T>();
+    const long blocks = ((a.g.ps + MPW - 1) / MPW + 7) / 8 * 8;  // a multiple of 8 (xcd_block)
+    const dim3 grid(blocks, 3 + a.g.nth);
+    const size_t lds = cn_seg_lds<T>(a.g.ny);
+    with_rmax((a.g.ny + 2 + NSEG - 1) / NSEG, [&](auto rc) {
+        constexpr int RM = decltype(rc)::value;
+        if (a.g.nth == 0)
+            hipLaunchKernelGGL((k_cn_seg<T, 0, NLT, MPW, RM>), grid, dim3(1024), lds, s, a, p);
+        else
+            hipLaunchKernelGGL((k_cn_seg<T, 1, NLT, MPW, RM>), grid, dim3(1024), lds, s, a, p);
+    });
+}
+
+template <typename T, typename NLT>
+void launch_remdiv_seg(hipStream_t s, const RdArgs &a, const CnPtrs<T, NLT> &p)
+{
+    constexpr int MPW = seg_mpw<T>(), NSEG = seg_nseg<T>();
+    const long blocks = ((a.g.ps + MPW - 1) / MPW + 7) / 8 * 8;
+    with_rmax((a.g.ny + NSEG - 1) / NSEG, [&](auto rc) {
+        constexpr int RM = decltype(rc)::value;
+        if (a.g.nth == 0)
+            hipLaunchKernelGGL((k_remdiv_seg<T, NLT, MPW, RM, 0>), dim3(blocks), dim3(1024), remdiv_seg_lds<T>(), s,
+                               a, p);
+        else
+            hipLaunchKernelGGL((k_remdiv_seg<T, NLT, MPW, RM, 1>), dim3(blocks), dim3(1024), remdiv_seg_lds<T>(), s,
+                               a, p);
+    });
+}
+
+template <typename T, typename NLT>
+void launch_cn_seg_batch(hipStream_t s, const Geo &g, int max_ny, const ColDesc<T, NLT> *d, int nmem)
+{
+    constexpr int MPW = seg_mpw<T>(), NSEG = seg_nseg<T>();
+    const dim3 grid((g.ps + MPW - 1) / MPW, 3 + g.nth, nmem);
+    const size_t lds = cn_seg_lds<T>(max_ny);
+    with_rmax((max_ny + 2 + NSEG - 1) / NSEG, [&](auto rc) {
+        constexpr int RM = decltype(rc)::value;
+        if (g.nth == 0)
+            hipLaunchKernelGGL((k_cn_seg_batch<T, 0, NLT, MPW, RM>), grid, dim3(1024), lds, s, d);
+        else
+            hipLaunchKernelGGL((k_cn_seg_batch<T, 1, NLT, MPW, RM>), grid, dim3(1024), lds, s, d);
+    });
+}
+
+template <typename T, typename NLT>
+void launch_remdiv_seg_batch(hipStream_t s, const Geo &g, int max_ny, const ColDesc<T, NLT> *d, int nmem)
+{
+    constexpr int MPW = seg_mpw<T>(), NSEG = seg_nseg<T>();
+    const dim3 grid((g.ps + MPW - 1) / MPW, 1, nmem);
+    with_rmax((max_ny + NSEG - 1) / NSEG, [&](auto rc) {
+        constexpr int RM = decltype(rc)::value;
+        if (g.nth == 0)
+            hipLaunchKernelGGL((k_remdiv_seg_batch<T, NLT, MPW, RM, 0>), grid, dim3(1024), remdiv_seg_lds<T>(), s, d);
+        else
+            hipLaunchKernelGGL((k_remdiv_seg_batch<T, NLT, MPW, RM, 1>), grid, dim3(1024), remdiv_seg_lds<T>(), s, d);
+    });
+}
+
+template bool colseg_supported<double>(int);
+template bool colseg_supported<float>(int);
+template void launch_cn_seg<double, double>(hipStream_t, const CnArgs &, const CnPtrs<double, double> &);
+template void launch_cn_seg<double, float>(hipStream_t, const CnArgs &, const CnPtrs<double, float> &);
+template void launch_cn_seg<float, float>(hipStream_t, const CnArgs &, const CnPtrs<float, float> &);
+template void launch_remdiv_seg<double, double>(hipStream_t, const RdArgs &, const CnPtrs<double, double> &);
+template void launch_remdiv_seg<double, float>(hipStream_t, const RdArgs &, const CnPtrs<double, float> &);
+template void launch_remdiv_seg<float, float>(hipStream_t, const RdArgs &, const CnPtrs<float, float> &);
+template void launch_cn_seg_batch<double, double>(hipStream_t, const Geo &, int, const ColDesc<double, double> *, int);
+template void launch_cn_seg_batch<double, float>(hipStream_t, const Geo &, int, const ColDesc<double, float> *, int);
+template void launch_cn_seg_batch<float, float>(hipStream_t, const Geo &, int, const ColDesc<float, float> *, int);
+template void launch_remdiv_seg_batch<double, double>(hipStream_t, const Geo &, int, const ColDesc<double, double> *,
+                                                      int);
+template void launch_remdiv_seg_batch<double, float>(hipStream_t, const Geo &, int, const ColDesc<double, float> *,
+                                                     int);
+template void launch_remdiv_seg_batch<float, float>(hipStream_t, const Geo &, int, const ColDesc<float, float> *,
+                                                    int);
+
+}  // namespace dgpu

@@ -70,10 +70,11 @@ __device__ __forceinline__ void piped_sweep(int j0, int dir, int n, Ld &&ld, Bod
 // references into a descriptor in global memory (batched launches); either
 // way they stay out of private memory.
 // DA, DB: planes in flight in sweeps A and B.  SIDE: the few low-mode
-// columns of the mixed path (one block): deeper prefetch, and the table rows
-// fetched with the rest, since no other wavefront hides their latency.
+// columns of the mixed path (one block): deeper prefetch.  TPF: the table
+// rows are fetched with the rest of a plane's inputs, DA/DB planes ahead,
+// instead of read (scalar loads) when the plane is processed.
 template <typename T, int NTH, typename NLT, typename AT, typename PT, int DA = (sizeof(T) == 4 ? 6 : 3),
-          int DB = (sizeof(T) == 4 ? 16 : 8), bool SIDE = false>
+          int DB = (sizeof(T) == 4 ? 16 : 8), bool SIDE = false, bool TPF = SIDE>
 __device__ __forceinline__ void cn_body(AT A, PT P, int bx)
 {
     const Geo &g = A.g;
@@ -118,7 +119,7 @@ __device__ __forceinline__ void cn_body(AT A, PT P, int bx)
     if (live) {
         struct InA {
             cx<T> fp, nl, cf, aux1, aux2, cp;
-            double tr[10];  // SIDE: this row's table entries, fetched ahead
+            double tr[10];  // TPF: this row's table entries, fetched ahead
         };
         cx<T> fm = zero, f0 = fld[m];  // this system's field at j-1, j (j+1 comes in)
         cx<T> thm = zero;              // CTH at j-1 (U2 buoyancy)
@@ -141,7 +142,7 @@ __device__ __forceinline__ void cn_body(AT A, PT P, int bx)
                 in.aux1 = has1 ? a1p[o] : zero;
                 in.aux2 = has2 ? a2p[o] : zero;
                 in.cp = hascp ? p_cp[o] : zero;
-                if constexpr (SIDE) {
+                if constexpr (TPF) {
                     const double *t = tab + (long)j * kRow;
                     in.tr[0] = t[T_MULT];
                     in.tr[1] = t[T_ISBC];
@@ -160,7 +161,7 @@ __device__ __forceinline__ void cn_body(AT A, PT P, int bx)
                 const long o = m + (long)j * ps;
                 double trow[kRow];
                 const double *t = tab + (long)j * kRow;
-                if constexpr (SIDE) {
+                if constexpr (TPF) {
                     trow[T_MULT] = in.tr[0];
                     trow[T_ISBC] = in.tr[1];
                     trow[T_CFR] = in.tr[2];
@@ -231,7 +232,7 @@ __device__ __forceinline__ void cn_body(AT A, PT P, int bx)
     if (live && !rhs_only) {
         struct InB {
             cx<T> g;
-            double ip, u, cpy;  // SIDE: fetched ahead
+            double ip, u, cpy;  // TPF: fetched ahead
         };
         cx<T> x = zero;
         piped_sweep<DB>(
@@ -239,7 +240,7 @@ __device__ __forceinline__ void cn_body(AT A, PT P, int bx)
             [&](int j) {
                 InB in;
                 in.g = gs[m + (long)j * ps];
-                if constexpr (SIDE) {
+                if constexpr (TPF) {
                     const double *t = tab + (long)j * kRow;
                     in.ip = t[T_INVPIV];
                     in.u = t[T_UP];
@@ -250,9 +251,9 @@ __device__ __forceinline__ void cn_body(AT A, PT P, int bx)
             [&](int j, const InB &in) {
                 const cx<T> gj = in.g;
                 const double *t = tab + (long)j * kRow;
-                const double tip = SIDE ? in.ip : t[T_INVPIV];
-                const double tup = SIDE ? in.u : t[T_UP];
-                const double tcpy = SIDE ? in.cpy : t[T_CPY];
+                const double tip = TPF ? in.ip : t[T_INVPIV];
+                const double tup = TPF ? in.u : t[T_UP];
+                const double tcpy = TPF ? in.cpy : t[T_CPY];
                 const T ip = T(tip);
                 if (j == NY + 1) {
                     x = make_cx<T>(gj.re * ip, gj.im * ip);
@@ -265,10 +266,13 @@ __device__ __forceinline__ void cn_body(AT A, PT P, int bx)
     }
 }
 
+// The table rows ride in the prefetch ring in FP32 (0.66 -> 0.60 ms at
+// 384x257x384); in FP64 the ring's extra registers spill (1.2 -> 2.5 ms).
 template <typename T, int NTH, typename NLT>
 __global__ void __launch_bounds__(256, 3) k_cn(CnArgs A, CnPtrs<T, NLT> P)
 {
-    cn_body<T, NTH, NLT, const CnArgs, const CnPtrs<T, NLT>>(A, P, xcd_contiguous(blockIdx.x, gridDim.x));
+    cn_body<T, NTH, NLT, const CnArgs, const CnPtrs<T, NLT>, (sizeof(T) == 4 ? 6 : 3), (sizeof(T) == 4 ? 16 : 8),
+            false, sizeof(T) == 4>(A, P, xcd_contiguous(blockIdx.x, gridDim.x));
 }
 
 // The mixed path's low-mode columns: a block or two, so registers are free.
@@ -285,7 +289,8 @@ __global__ void __launch_bounds__(256, 3) k_cn_batch(const ColDesc<T, NLT> *__re
     // references into the descriptor (uniform loads), not local copies that
     // would live in scratch
     const ColDesc<T, NLT> &D = d[blockIdx.y];
-    cn_body<T, NTH, NLT, const CnArgs &, const CnPtrs<T, NLT> &>(D.A, D.P, blockIdx.x);
+    cn_body<T, NTH, NLT, const CnArgs &, const CnPtrs<T, NLT> &, (sizeof(T) == 4 ? 6 : 3), (sizeof(T) == 4 ? 16 : 8),
+            false, sizeof(T) == 4>(D.A, D.P, blockIdx.x);
 }
 
 // The ensemble's low-mode columns (same sweeps as k_cn_side).

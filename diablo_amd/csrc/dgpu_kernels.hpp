@@ -99,6 +99,18 @@ void launch_side_scatter_batch(hipStream_t s, const ScatDesc<TS, TF> *d, int nme
                                int max_planes, int nf);
 template <typename T, typename NLT>
 void launch_cn_batch(hipStream_t s, const Geo &g, const ColDesc<T, NLT> *d, int nmem);
+// Segmented column kernels (dgpu_colseg.hip): the CN solutions go to P.gs[q]
+// (complete new fields), k_remdiv_seg projects them into the state.
+template <typename T>
+bool colseg_supported(int ny);
+template <typename T, typename NLT>
+void launch_cn_seg(hipStream_t s, const CnArgs &a, const CnPtrs<T, NLT> &p);
+template <typename T, typename NLT>
+void launch_remdiv_seg(hipStream_t s, const RdArgs &a, const CnPtrs<T, NLT> &p);
+template <typename T, typename NLT>
+void launch_cn_seg_batch(hipStream_t s, const Geo &g, int max_ny, const ColDesc<T, NLT> *d, int nmem);
+template <typename T, typename NLT>
+void launch_remdiv_seg_batch(hipStream_t s, const Geo &g, int max_ny, const ColDesc<T, NLT> *d, int nmem);
 template <typename T, typename NLT>
 void launch_remdiv_batch(hipStream_t s, const Geo &g, const ColDesc<T, NLT> *d, int nmem);
 template <typename T, typename NLT>

@@ -81,6 +81,9 @@ class ChannelRun:
         if cfg.reset_time or cfg.create_new_flow:
             self.time_step = 0
             g.time = 0.0
+        # both branches write a step-0 record: CREATE_FLOW_CHAN ends with
+        # SAVE_STATS_CHAN(.FALSE.) (for/set_ics.f:181-182), READ_FLOW's branch
+        # calls SAVE_STATS(.FALSE.) (for/diablo_io.f:173)
         self.save_stats(final=False)
 
     # -- SAVE_STATS: the profiles in the reference's text files --
@@ -156,6 +159,12 @@ class ChannelRun:
                 break
         g.synchronize()
         elapsed = _time.time() - start
+        # TIME_STEP = TIME_STEP - 1 before the final files (for/diablo.f:130):
+        # after a completed DO loop that is the last step; after END_RUN's
+        # EXIT the loop variable still holds the last step, so the final
+        # records carry the step before it, as the reference's do
+        if stop:
+            self.time_step -= 1
         self.save_flow(final=True)
         self.save_stats(final=True)
         return {"steps": done, "time_step": self.time_step, "time": g.time, "delta_t": g.delta_t,

@@ -162,10 +162,11 @@
       SUBROUTINE DGPU_CHECK(RC, WHAT)
       INTEGER(C_INT32_T), INTENT(IN) :: RC
       CHARACTER(*), INTENT(IN) :: WHAT
-! The reference reports unrecoverable errors with STOP (e.g. for/channel.f:2322)
+! The reference reports unrecoverable errors with STOP (e.g. for/channel.f:2322);
+! ERROR STOP also gives the shell a non-zero exit status
       IF (RC .NE. 0) THEN
         WRITE(*,*) 'libdiablo_gpu: ', WHAT, ' failed, code ', RC
-        STOP 'DIABLO GPU path error'
+        ERROR STOP 'DIABLO GPU path error'
       END IF
       END SUBROUTINE DGPU_CHECK
 

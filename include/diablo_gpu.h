@@ -289,6 +289,15 @@ int32_t dgpu_kernel_times(dgpu_ctx *ctx, double *ms, int64_t *count, int32_t n);
  * branches are the members.  h_bar, beta_bar and zeta_bar hold 3 values per
  * member (member-major).  Members are ordinary contexts for every other call
  * (dgpu_ensemble_member): set/get fields, initial transforms, REM_DIV.
+ * Ordering: the ensemble's launches run on a stream of its own.  Call
+ * dgpu_ensemble_synchronize before any call on a member after
+ * dgpu_ensemble_rk_chan_1, and finish work enqueued on a member
+ * (dgpu_synchronize on it) before the next dgpu_ensemble_rk_chan_1.  A new
+ * h_bar for a stage waits for the ensemble's earlier launches before the
+ * stage's Crank-Nicolson tables are rebuilt.  A member's COURANT request
+ * (dgpu_courant_request) is served as for a single context: the rk_step = 3
+ * call then runs every member through its own launches (no graphs, no
+ * batched launches) and records the maxima for dgpu_courant.
  */
 typedef struct dgpu_ensemble dgpu_ensemble;
 int32_t dgpu_ensemble_create(const dgpu_params *members, int32_t n, dgpu_ensemble **out);

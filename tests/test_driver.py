@@ -67,8 +67,11 @@ def test_driver_run(tmp_path):
         assert np.array_equal(a[k], b[k]), k
     assert os.path.exists(os.path.join(d, "out.npz")) and os.path.exists(os.path.join(d, "end.npz"))
     lines = open(os.path.join(d, "mean.txt")).read().splitlines()
-    # initial, steps 2 and 4, final: 4 blocks of 2 header lines + NY rows
+    # the new flow's record (CREATE_FLOW_CHAN's SAVE_STATS_CHAN, step 0),
+    # steps 2 and 4, final (step 4): 4 blocks of 2 header lines + NY rows
     assert len(lines) == 4 * (2 + cfg.ny)
+    steps = [int(lines[b * (2 + cfg.ny)].split()[0]) for b in range(4)]
+    assert steps == [0, 2, 4, 4]
     row = lines[2].split()
     assert int(row[0]) == 1 and len(row) == 18
     tke = open(os.path.join(d, "tke.txt")).read().splitlines()
@@ -76,7 +79,10 @@ def test_driver_run(tmp_path):
     open(os.path.join(d, "stop.now"), "w").close()
     r2 = driver.ChannelRun(d, log=lambda *a: None)
     r2.initialize()
-    assert r2.run()["steps"] == 1
+    out2 = r2.run()
+    # END_RUN stops after step 1; the final records carry TIME_STEP - 1
+    # (for/diablo.f:130 after the loop's EXIT)
+    assert out2["steps"] == 1 and out2["time_step"] == 0
 
 
 @pytest.mark.gpu
