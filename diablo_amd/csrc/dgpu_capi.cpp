@@ -85,6 +85,7 @@ struct dgpu_ctx {
     double *d_fa = nullptr, *d_fb = nullptr;  // POISSON_P_CHAN work planes
     double *d_gyf = nullptr;    // GYF (0:NY+1), part of d_vec
     double *d_stats = nullptr;  // dgpu_stats_chan output
+    double *d_noalloc = nullptr;  // this GPU's fixed array of dgpu_stats_chan_noalloc
     // COURANT: 0 idle, 1 requested for the next RK_STEP 3, 2 maxima recorded
     int courant_state = 0;
     double *d_vmax_part = nullptr, *d_vmax = nullptr;  // per-block, per-plane velocity maxima
@@ -564,6 +565,12 @@ int32_t dgpu_create(const dgpu_params *p, dgpu_ctx **out)
         c->d_kz2 = c->d_kz + (g.tnkz + 1);
         c->d_gyf = c->d_kz2 + (g.tnkz + 1);
     }
+    // the fixed array of dgpu_stats_chan_noalloc on this (the current) device;
+    // on failure only that entry point is unavailable
+    if (dgpu::noalloc_stats_scratch(&c->d_noalloc) != hipSuccess) {
+        c->d_noalloc = nullptr;
+        (void)hipGetLastError();
+    }
     // twiddles
     {
         std::vector<cx<double>> twx, twz;
@@ -742,16 +749,15 @@ int32_t dgpu_destroy(dgpu_ctx *c)
     return DGPU_OK;
 }
 
-int32_t dgpu_stats_chan(dgpu_ctx *c, double *profiles, double *ubulk)
+// The statistics kernel into the device buffer buf (DGPU_NSTAT * NY doubles),
+// copied to the caller's profiles; UBULK on the host.  Allocates nothing.
+static int32_t stats_through(dgpu_ctx *c, double *buf, double *profiles, double *ubulk)
 {
-    if (!c || !profiles || !ubulk) return DGPU_E_ARG;
-    if (int rc = to_master(c)) return rc;
     const dgpu::Geo &g = c->g;
-    if (!c->d_stats) HIPCHK(c, hipMalloc(&c->d_stats, sizeof(double) * DGPU_NSTAT * g.ny));
     dgpu::launch_stats(c->stream, g, c->cu[0], c->cu[1], c->cu[2], c->cp, g.nth ? c->cth[0] : nullptr, c->d_dy,
-                       c->d_dyf, c->d_gyf, c->d_kx, c->d_kz, c->d_stats);
+                       c->d_dyf, c->d_gyf, c->d_kx, c->d_kz, buf);
     if (int rc = check_launch(c, "stats")) return rc;
-    HIPCHK(c, hipMemcpyAsync(profiles, c->d_stats, sizeof(double) * DGPU_NSTAT * g.ny, hipMemcpyDeviceToHost,
+    HIPCHK(c, hipMemcpyAsync(profiles, buf, sizeof(double) * DGPU_NSTAT * g.ny, hipMemcpyDeviceToHost,
                              c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     // INTEGRATE_Y_VAR (for/mpi.f:1676-1698) of ume
@@ -761,6 +767,28 @@ int32_t dgpu_stats_chan(dgpu_ctx *c, double *profiles, double *ubulk)
     *ubulk = res / c->p.ly;
     return DGPU_OK;
 }
+
+int32_t dgpu_stats_chan(dgpu_ctx *c, double *profiles, double *ubulk)
+{
+    if (!c || !profiles || !ubulk) return DGPU_E_ARG;
+    if (int rc = to_master(c)) return rc;
+    if (!c->d_stats) HIPCHK(c, hipMalloc(&c->d_stats, sizeof(double) * DGPU_NSTAT * c->g.ny));
+    return stats_through(c, c->d_stats, profiles, ubulk);
+}
+
+int32_t dgpu_stats_chan_noalloc(dgpu_ctx *c, double *profiles, double *ubulk)
+{
+    if (!c || !profiles || !ubulk) return DGPU_E_ARG;
+    if (c->g.ny > DGPU_NOALLOC_MAX_NY)
+        return fail(c, DGPU_E_ARG, "dgpu_stats_chan_noalloc: NY = " + std::to_string(c->g.ny) +
+                                       " exceeds DGPU_NOALLOC_MAX_NY = " + std::to_string(DGPU_NOALLOC_MAX_NY) +
+                                       " (use dgpu_stats_chan)");
+    if (!c->d_noalloc)
+        return fail(c, DGPU_E_STATE, "dgpu_stats_chan_noalloc: the fixed device array has no address on this GPU");
+    if (int rc = to_master(c)) return rc;
+    return stats_through(c, c->d_noalloc, profiles, ubulk);
+}
+
 
 static int fort_to_physical(dgpu_ctx *c, double *buf, int j0, int j1);
 static int fort_to_fourier(dgpu_ctx *c, double *buf, int j0, int j1);

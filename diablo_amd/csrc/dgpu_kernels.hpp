@@ -224,6 +224,10 @@ void launch_tke(hipStream_t s, const Geo &g, const cx<double> *cu1, const cx<dou
 void launch_stats(hipStream_t s, const Geo &g, const cx<double> *cu1, const cx<double> *cu2, const cx<double> *cu3,
                   const cx<double> *cp, const cx<double> *cth, const double *dy, const double *dyf,
                   const double *gyf, const double *kx, const double *kz, double *out);
+// The fixed statistics array of dgpu_stats_chan_noalloc (dgpu_stats.hip):
+// device address of this GPU's copy, resolved once per device with the
+// device current (hipGetSymbolAddress).  Returns hipSuccess or the error.
+hipError_t noalloc_stats_scratch(double **out);
 void launch_unpack_fortran(hipStream_t s, const Geo &g, const double *fort, cx<double> *kept, int j0, int nplanes);
 // Restart-file layout: out(1:NX, 1:NY, 1:NZ) from the physical Fortran-layout
 // field (G2GF first for V), and back (GF2G for V).

@@ -12,6 +12,8 @@
 #include "../../include/diablo_gpu.h"
 #include "dgpu_kernels.hpp"
 
+#include <mutex>
+
 namespace dgpu {
 
 namespace {
@@ -196,6 +198,38 @@ void launch_stats(hipStream_t s, const Geo &g, const cx<double> *cu1, const cx<d
                   const double *gyf, const double *kx, const double *kz, double *out)
 {
     hipLaunchKernelGGL(k_stats, dim3(g.ny), dim3(256), 0, s, g, cu1, cu2, cu3, cp, cth, dy, dyf, gyf, kx, kz, out);
+}
+
+// The per-plane statistics of dgpu_stats_chan_noalloc: DGPU_NSTAT rows of
+// up to DGPU_NOALLOC_MAX_NY planes.  k_stats writes every entry of the NY
+// planes it covers once per call, so nothing has to be cleared between
+// calls.  Host code must never pass the array's name to a kernel or a copy:
+// on the host it is only a stand-in; the device address of each GPU's copy
+// comes from hipGetSymbolAddress below.
+static __device__ __align__(256) double scratch[(long)DGPU_NSTAT * DGPU_NOALLOC_MAX_NY];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mtx;
+double *g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+hipError_t noalloc_stats_scratch(double **out)
+{
+    *out = nullptr;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mtx);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = static_cast<double *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
 }
 
 }  // namespace dgpu

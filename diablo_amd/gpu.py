@@ -58,6 +58,7 @@ EXPORTS = (
     "dgpu_ensemble_last_error", "dgpu_ensemble_set_graphs", "dgpu_ensemble_rk_chan_1",
     "dgpu_ensemble_synchronize", "dgpu_stats_chan", "dgpu_courant_request", "dgpu_courant",
     "dgpu_get_flow_physical", "dgpu_set_flow_physical", "dgpu_tke_chan", "dgpu_les_chan", "dgpu_set_les", "dgpu_set_zdft", "dgpu_autotune", "dgpu_tke_les",
+    "dgpu_stats_chan_noalloc",
 )
 
 # dgpu_stats_chan profiles, in the order of the header's DGPU_ST_*
@@ -117,8 +118,9 @@ def load_library(path: str = LIB_PATH):
     L.dgpu_set_kernel_timing.argtypes = [vp, i32]
     L.dgpu_kernel_times.restype = i32
     L.dgpu_kernel_times.argtypes = [vp, DP, C.POINTER(C.c_int64), i32]
-    L.dgpu_stats_chan.restype = i32
-    L.dgpu_stats_chan.argtypes = [vp, DP, DP]
+    for name in ("dgpu_stats_chan", "dgpu_stats_chan_noalloc"):
+        getattr(L, name).restype = i32
+        getattr(L, name).argtypes = [vp, DP, DP]
     L.dgpu_tke_chan.restype = i32
     L.dgpu_tke_chan.argtypes = [vp, DP]
     L.dgpu_les_chan.restype = i32
@@ -361,6 +363,15 @@ class GpuChannel:
         out = np.zeros(len(STATS) * ny)
         ub = np.zeros(1)
         self._chk(self.L.dgpu_stats_chan(self.h, _dp(out), _dp(ub)), "dgpu_stats_chan")
+        return {k: out[q * ny:(q + 1) * ny].copy() for q, k in enumerate(STATS)}, float(ub[0])
+
+    def stats_chan_noalloc(self) -> tuple[dict, float]:
+        """stats_chan through dgpu_stats_chan_noalloc: the same statistics with
+        the library's fixed device array instead of an allocated buffer."""
+        ny = self.cfg.ny
+        out = np.zeros(len(STATS) * ny)
+        ub = np.zeros(1)
+        self._chk(self.L.dgpu_stats_chan_noalloc(self.h, _dp(out), _dp(ub)), "dgpu_stats_chan_noalloc")
         return {k: out[q * ny:(q + 1) * ny].copy() for q, k in enumerate(STATS)}, float(ub[0])
 
     def fft_xz_to_fourier(self, a: np.ndarray, jmin: int, jmax: int):

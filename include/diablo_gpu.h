@@ -210,6 +210,20 @@ int32_t dgpu_synchronize(dgpu_ctx *ctx);
  * name, for/save_stats.f:404-406). */
 int32_t dgpu_stats_chan(dgpu_ctx *ctx, double *profiles, double *ubulk);
 
+/* dgpu_stats_chan without any allocation (no counterpart in the reference).
+ * For callers that cannot allocate device memory -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc: the same
+ * kernel and the same results as dgpu_stats_chan, but the per-plane sums go
+ * through a fixed device array of the library (DGPU_NSTAT *
+ * DGPU_NOALLOC_MAX_NY doubles, 688 KiB, one copy per GPU, held until the
+ * process ends) instead of a buffer allocated on the first call.  NY above
+ * DGPU_NOALLOC_MAX_NY is refused with DGPU_E_ARG.  Every context on one GPU
+ * shares that array: calls that overlap on one GPU (contexts on different
+ * streams, or host threads) race and must be serialised by the caller.
+ * Synchronises the context's stream before returning, like dgpu_stats_chan. */
+#define DGPU_NOALLOC_MAX_NY 4096
+int32_t dgpu_stats_chan_noalloc(dgpu_ctx *ctx, double *profiles, double *ubulk);
+
 /* tkebudget_chan's pseudo-dissipation epsilon = NU <du_i/dx_j du_i/dx_j>
  * (for/save_stats.f:965-1159), as SAVE_STATS_CHAN calls it, from the device
  * state in FP64: epsilon[J - 1], J = 1..NY (tke.h5's 'epsilon').  Planes
