@@ -46,6 +46,8 @@ struct dgpu_ctx {
     cx<float> *d_iptwf = nullptr;
     int ld = 0;      // row length of the intermediates (>= mk, multiple of 8)
     int xchunk = 64; // planes per block of the fused x kernel (set at creation)
+    int colnw = 8;   // waves per block of the segmented column kernels; 0: whole-column sweeps
+    int colfuse = -1;     // FP32 state, fused column kernel: -1 where NY >= 160, 0 never, 1 where it fits
     // mixed paths: FP32 transforms, intermediates and nonlinear terms
     int mixed = 0;
     // DGPU_MIXED: FP32 working state [slot][j][K][I] (same slots as d_state's
@@ -136,6 +138,7 @@ struct dgpu_ensemble {
     std::vector<hipEvent_t> join;
     hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
     std::vector<double> gkey[3];
+    std::vector<int> gshape[3];  // the members' launch shapes each graph was captured with
     bool use_graphs = true;
     // batched launches (all members at one precision)
     bool batched = false;
@@ -531,6 +534,13 @@ int32_t dgpu_create(const dgpu_params *p, dgpu_ctx **out)
         // resident blocks either way, at two and four blocks per CU)
         c->xchunk = c->mixed ? 32 : 64;
         if (xc && std::atoi(xc) > 0) c->xchunk = std::atoi(xc);
+        // column stage: DGPU_COLSEG=0 selects the whole-column sweeps,
+        // DGPU_COLSEG_NW the waves per block of the segmented kernels
+        const char *cs = std::getenv("DGPU_COLSEG"), *cw = std::getenv("DGPU_COLSEG_NW");
+        if (cw && (std::atoi(cw) == 4 || std::atoi(cw) == 8 || std::atoi(cw) == 16)) c->colnw = std::atoi(cw);
+        if (cs && std::atoi(cs) == 0) c->colnw = 0;
+        const char *cfz = std::getenv("DGPU_COLFUSE");
+        if (cfz && (std::atoi(cfz) == 0 || std::atoi(cfz) == 1)) c->colfuse = std::atoi(cfz);
         const char *sw = std::getenv("DGPU_XSWZ");  // 0 turns the row swizzle off
         c->ipx.swz = (sw && std::atoi(sw) == 0) ? 0u : dgpu::ip_pick_swizzle(c->ipx);
         // the z kernels' column batches: W = the largest power of two with W*T <= 256
@@ -1044,6 +1054,47 @@ int32_t dgpu_set_zdft(dgpu_ctx *c, int32_t on)
     return DGPU_OK;
 }
 
+static int fused_waves(const dgpu_ctx *c, int ny);
+
+// a shape some segmented kernel of the context's precision runs with
+static bool column_shape_fits(const dgpu_ctx *c, int waves)
+{
+    if (waves == 0) return true;
+    return c->f32state ? (dgpu::colseg_supported<float>(c->g.ny, waves) || dgpu::colfused_supported(c->g.ny, waves))
+                       : dgpu::colseg_supported<double>(c->g.ny, waves);
+}
+
+int32_t dgpu_set_column_shape(dgpu_ctx *c, int32_t waves, int32_t fused)
+{
+    if (!c) return DGPU_E_ARG;
+    if (waves != 0 && waves != 4 && waves != 8 && waves != 16)
+        return fail(c, DGPU_E_ARG, "column shape: waves per block must be 0, 4, 8 or 16");
+    if (fused < -1 || fused > 1) return fail(c, DGPU_E_ARG, "column shape: fused must be -1, 0 or 1");
+    if (!column_shape_fits(c, waves))
+        return fail(c, DGPU_E_ARG, "column shape: the segmented column kernels do not fit NY = " +
+                                       std::to_string(c->g.ny) + " with " + std::to_string(waves) + " waves");
+    if (fused == 1 && !(c->f32state && waves && dgpu::colfused_supported(c->g.ny, waves)))
+        return fail(c, DGPU_E_ARG, "column shape: the fused column kernel needs the FP32 state (DGPU_MIXED), "
+                                   "4, 8 or 16 waves and NY <= 286");
+    if (c->colnw == waves && c->colfuse == fused) return DGPU_OK;
+    c->colnw = waves;
+    c->colfuse = fused;
+    for (auto &ge : c->gexec)
+        if (ge) {
+            HIPCHK(c, hipGraphExecDestroy(ge));
+            ge = nullptr;
+        }
+    return DGPU_OK;
+}
+
+int32_t dgpu_get_column_shape(const dgpu_ctx *c, int32_t *waves, int32_t *fused)
+{
+    if (!c || !waves) return DGPU_E_ARG;
+    *waves = column_shape_fits(c, c->colnw) ? c->colnw : 0;
+    if (fused) *fused = fused_waves(c, c->g.ny) ? 1 : 0;
+    return DGPU_OK;
+}
+
 // ---------------------------------------------------------------------------
 // LES (for/les.f, for/channel.f:303-386, 628-659, 748-1003)
 // ---------------------------------------------------------------------------
@@ -1374,15 +1425,30 @@ static int enqueue_vmax(dgpu_ctx *c, const void *vel = nullptr)
     return DGPU_OK;
 }
 
-// The column stage runs the segmented kernels (dgpu_colseg.hip) unless
-// DGPU_COLSEG=0 selects the whole-column sweeps (dgpu_columns.hip).
-static bool use_colseg()
+// Waves per block of the segmented column kernels (dgpu_colseg.hip) for a
+// column of NY rows in FP32 or FP64, or 0 for the whole-column sweeps
+// (dgpu_columns.hip): the context's choice where the column fits it.
+static int seg_waves(const dgpu_ctx *c, int ny, bool fp32)
 {
-    static const bool on = [] {
-        const char *e = std::getenv("DGPU_COLSEG");
-        return !(e && std::atoi(e) == 0);
-    }();
-    return on;
+    if (!c->colnw) return 0;
+    const bool fits = fp32 ? dgpu::colseg_supported<float>(ny, c->colnw) : dgpu::colseg_supported<double>(ny, c->colnw);
+    return fits ? c->colnw : 0;
+}
+// The fused column kernel of the FP32 state (dgpu_colseg.hip): the
+// context's shape where the column fits it, else 0.
+// Left to itself (colfuse -1) the fused kernel runs from NY = 160 up (six or
+// more rows per segment): on shorter columns its four solves in a row are
+// mostly barriers, and the separate kernels are faster.
+static int fused_waves(const dgpu_ctx *c, int ny)
+{
+    if (!c->colnw || !c->f32state) return 0;
+    const bool want = c->colfuse == 1 || (c->colfuse < 0 && ny >= 160);
+    return (want && dgpu::colfused_supported(ny, c->colnw)) ? c->colnw : 0;
+}
+// the low-mode side columns (a few modes per system): the smallest blocks
+static int side_waves(const dgpu_ctx *c, int ny)
+{
+    return (c->colnw && dgpu::colseg_supported<double>(ny, 4)) ? 4 : 0;
 }
 
 static int rk_enqueue(dgpu_ctx *c, int rk_step, double time, double temp1, double temp3, double temp4, double temp5)
@@ -1454,8 +1520,8 @@ static int rk_enqueue(dgpu_ctx *c, int rk_step, double time, double temp1, doubl
     dgpu::RdArgs R;
     fill_rdargs(c, R, 1, temp4);
     // the segmented column kernels where the column fits them
-    const bool seg64 = use_colseg() && dgpu::colseg_supported<double>(g.ny);
-    const bool seg32 = use_colseg() && dgpu::colseg_supported<float>(g.ny);
+    const int seg64 = seg_waves(c, g.ny, false), seg32 = seg_waves(c, g.ny, true), segs = side_waves(c, g.ny);
+    const int fused32 = fused_waves(c, g.ny);
     if (c->f32state) {
         // FP32 solves on every mode, then the low modes again in FP64 on their
         // side columns, copied into the working state for the transforms.
@@ -1475,28 +1541,34 @@ static int rk_enqueue(dgpu_ctx *c, int rk_step, double time, double temp1, doubl
         // results then replace the FP32 ones in the working state.
         HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
         HIPCHK(c, hipStreamWaitEvent(c->side_stream, c->ev_fork, 0));
-        if (seg64) {
-            dgpu::launch_cn_seg<double, float>(c->side_stream, As, Ps);
-            dgpu::launch_remdiv_seg<double, float>(c->side_stream, Rs, Ps);
+        if (segs) {
+            dgpu::launch_cn_seg<double, float>(c->side_stream, segs, As, Ps);
+            dgpu::launch_remdiv_seg<double, float>(c->side_stream, segs, Rs, Ps);
         } else {
             dgpu::launch_cn_side<double, float>(c->side_stream, As, Ps);
             dgpu::launch_remdiv_side<double, float>(c->side_stream, Rs, Ps);
         }
         if (int rc = check_launch(c, "low-mode columns")) return rc;
         HIPCHK(c, hipEventRecord(c->ev_join, c->side_stream));
-        {
+        if (fused32 && !courant) {
+            // the whole column stage in one kernel (timed as the cn kind)
             TimerScope t(c, 3);
-            if (seg32) dgpu::launch_cn_seg<float, float>(c->stream, A, P);
-            else dgpu::launch_cn<float, float>(c->stream, A, P);
-        }
-        if (int rc = check_launch(c, "cn")) return rc;
-        if (courant) {
-            if (int rc = enqueue_vmax(c, seg32 ? (const void *)P.gs[dgpu::SYS_U1] : nullptr)) return rc;
-        }
-        {
-            TimerScope t(c, 4);
-            if (seg32) dgpu::launch_remdiv_seg<float, float>(c->stream, R, P);
-            else dgpu::launch_remdiv<float, float>(c->stream, R, P);
+            dgpu::launch_colfused<float, float>(c->stream, fused32, A, R, P);
+        } else {
+            {
+                TimerScope t(c, 3);
+                if (seg32) dgpu::launch_cn_seg<float, float>(c->stream, seg32, A, P);
+                else dgpu::launch_cn<float, float>(c->stream, A, P);
+            }
+            if (int rc = check_launch(c, "cn")) return rc;
+            if (courant) {
+                if (int rc = enqueue_vmax(c, seg32 ? (const void *)P.gs[dgpu::SYS_U1] : nullptr)) return rc;
+            }
+            {
+                TimerScope t(c, 4);
+                if (seg32) dgpu::launch_remdiv_seg<float, float>(c->stream, seg32, R, P);
+                else dgpu::launch_remdiv<float, float>(c->stream, R, P);
+            }
         }
         if (int rc = check_launch(c, "remdiv")) return rc;
         HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
@@ -1508,7 +1580,7 @@ static int rk_enqueue(dgpu_ctx *c, int rk_step, double time, double temp1, doubl
         fill_ptrs<float>(c, P, c->nlf);
         {
             TimerScope t(c, 3);
-            if (seg64) dgpu::launch_cn_seg<double, float>(c->stream, A, P);
+            if (seg64) dgpu::launch_cn_seg<double, float>(c->stream, seg64, A, P);
             else dgpu::launch_cn<double, float>(c->stream, A, P);
         }
         if (int rc = check_launch(c, "cn")) return rc;
@@ -1517,7 +1589,7 @@ static int rk_enqueue(dgpu_ctx *c, int rk_step, double time, double temp1, doubl
         }
         {
             TimerScope t(c, 4);
-            if (seg64) dgpu::launch_remdiv_seg<double, float>(c->stream, R, P);
+            if (seg64) dgpu::launch_remdiv_seg<double, float>(c->stream, seg64, R, P);
             else dgpu::launch_remdiv<double, float>(c->stream, R, P);
         }
     } else {
@@ -1525,7 +1597,7 @@ static int rk_enqueue(dgpu_ctx *c, int rk_step, double time, double temp1, doubl
         fill_ptrs<double>(c, P, c->nl);
         {
             TimerScope t(c, 3);
-            if (seg64) dgpu::launch_cn_seg<double, double>(c->stream, A, P);
+            if (seg64) dgpu::launch_cn_seg<double, double>(c->stream, seg64, A, P);
             else dgpu::launch_cn<double, double>(c->stream, A, P);
         }
         if (int rc = check_launch(c, "cn")) return rc;
@@ -1534,7 +1606,7 @@ static int rk_enqueue(dgpu_ctx *c, int rk_step, double time, double temp1, doubl
         }
         {
             TimerScope t(c, 4);
-            if (seg64) dgpu::launch_remdiv_seg<double, double>(c->stream, R, P);
+            if (seg64) dgpu::launch_remdiv_seg<double, double>(c->stream, seg64, R, P);
             else dgpu::launch_remdiv<double, double>(c->stream, R, P);
         }
     }
@@ -1779,11 +1851,34 @@ int32_t dgpu_autotune(dgpu_ctx *c, int32_t steps, double delta_t, int32_t *xchun
     const double dt = delta_t;
     const double h_bar[3] = {dt * (8.0 / 15.0), dt * (2.0 / 15.0), dt * (5.0 / 15.0)};
     const double beta_bar[3] = {1.0, 25.0 / 8.0, 9.0 / 4.0}, zeta_bar[3] = {0.0, -17.0 / 8.0, -5.0 / 4.0};
-    const int xchunk0 = c->xchunk;
+    const int xchunk0 = c->xchunk, colnw0 = c->colnw, colfuse0 = c->colfuse;
     const bool zdft0 = c->zdft;
     const bool try_zdft = c->mixed && !c->les && c->g.nz <= 256 && c->g.nz % 16 == 0 && c->g.nx <= 256;
+    // ms per step of the current shape, from the saved state
+    auto time_shape = [&](double &ms_out) -> int {
+        for (auto &ge : c->gexec)
+            if (ge) {
+                hipGraphExecDestroy(ge);
+                ge = nullptr;
+            }
+        if (int r = restore()) return r;
+        for (int k = 1; k <= 3; ++k)
+            if (int r = dgpu_rk_chan_1(c, k, 0.0, h_bar, beta_bar, zeta_bar)) return r;
+        if (hipEventRecord(e0, c->stream) != hipSuccess) return fail(c, DGPU_E_HIP, "autotune event");
+        for (int n = 0; n < steps; ++n)
+            for (int k = 1; k <= 3; ++k)
+                if (int r = dgpu_rk_chan_1(c, k, 0.0, h_bar, beta_bar, zeta_bar)) return r;
+        float ms = 0.f;
+        if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+            return fail(c, DGPU_E_HIP, "autotune timing");
+        ms_out = ms / steps;
+        return DGPU_OK;
+    };
+    // x-kernel planes per block and the z transforms, then the column shape
+    // (LES runs keep the column stage of the LES substep)
     const int chunks[] = {4, 8, 16, 32, 64};
-    int best_x = xchunk0;
+    int best_x = xchunk0, best_c = colnw0, best_f = colfuse0;
     bool best_z = zdft0;
     double best = 1e300;
     for (int z = 0; z <= (try_zdft ? 1 : 0) && rc == DGPU_OK; ++z)
@@ -1791,33 +1886,40 @@ int32_t dgpu_autotune(dgpu_ctx *c, int32_t steps, double delta_t, int32_t *xchun
             if (rc != DGPU_OK) break;
             if ((rc = dgpu_set_zdft(c, z))) break;
             c->xchunk = xc;
-            for (auto &ge : c->gexec)
-                if (ge) {
-                    hipGraphExecDestroy(ge);
-                    ge = nullptr;
-                }
-            if ((rc = restore())) break;
-            for (int k = 1; k <= 3 && rc == DGPU_OK; ++k) rc = dgpu_rk_chan_1(c, k, 0.0, h_bar, beta_bar, zeta_bar);
-            if (rc) break;
-            if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = fail(c, DGPU_E_HIP, "autotune event"); break; }
-            for (int n = 0; n < steps && rc == DGPU_OK; ++n)
-                for (int k = 1; k <= 3 && rc == DGPU_OK; ++k)
-                    rc = dgpu_rk_chan_1(c, k, 0.0, h_bar, beta_bar, zeta_bar);
-            if (rc) break;
-            float ms = 0.f;
-            if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
-                rc = fail(c, DGPU_E_HIP, "autotune timing");
-                break;
-            }
-            if (ms / steps < best) {
-                best = ms / steps;
+            double ms = 0.0;
+            if ((rc = time_shape(ms))) break;
+            if (ms < best) {
+                best = ms;
                 best_x = xc;
                 best_z = z != 0;
             }
         }
+    if (rc == DGPU_OK && !c->les && colnw0 != 0) {
+        c->xchunk = best_x;
+        rc = dgpu_set_zdft(c, best_z);
+        // with the FP32 state also the fused column kernel on and off
+        const bool f0 = fused_waves(c, c->g.ny) != 0;
+        for (int fz = 0; fz <= (c->f32state ? 1 : 0) && rc == DGPU_OK; ++fz)
+            for (int nw : {4, 8, 16}) {
+                const bool fz_on = fz ? !f0 : f0;
+                if (rc != DGPU_OK || !column_shape_fits(c, nw)) continue;
+                if (fz_on && !dgpu::colfused_supported(c->g.ny, nw)) continue;
+                if (nw == colnw0 && fz_on == f0) continue;  // timed above
+                c->colnw = nw;
+                c->colfuse = fz_on ? 1 : 0;
+                double ms = 0.0;
+                if ((rc = time_shape(ms))) break;
+                if (ms < best) {
+                    best = ms;
+                    best_c = nw;
+                    best_f = c->colfuse;
+                }
+            }
+    }
     // the chosen shape, and the state as it was
     c->xchunk = rc == DGPU_OK ? best_x : xchunk0;
+    c->colnw = rc == DGPU_OK ? best_c : colnw0;
+    c->colfuse = rc == DGPU_OK ? best_f : colfuse0;
     const int rz = dgpu_set_zdft(c, rc == DGPU_OK ? best_z : zdft0);
     if (rc == DGPU_OK) rc = rz;
     const int rr = restore();
@@ -2195,9 +2297,9 @@ static int ens_enqueue_batched(dgpu_ensemble *e, int rk_step)
             const int nm = (int)gr.idx.size();
             const int max_ny = gr.max_planes - 2;
             const auto *d = (const dgpu::ColDesc<double, float> *)gr.d_cols[k];
-            if (use_colseg() && dgpu::colseg_supported<double>(max_ny)) {
-                dgpu::launch_cn_seg_batch<double, float>(e->side, gs, max_ny, d, nm);
-                dgpu::launch_remdiv_seg_batch<double, float>(e->side, gs, max_ny, d, nm);
+            if (const int nw = side_waves(e->m[gr.idx[0]], max_ny)) {
+                dgpu::launch_cn_seg_batch<double, float>(e->side, nw, gs, max_ny, d, nm);
+                dgpu::launch_remdiv_seg_batch<double, float>(e->side, nw, gs, max_ny, d, nm);
             } else {
                 dgpu::launch_cn_side_batch<double, float>(e->side, gs, d, nm);
                 dgpu::launch_remdiv_side_batch<double, float>(e->side, gs, d, nm);
@@ -2212,27 +2314,29 @@ static int ens_enqueue_batched(dgpu_ensemble *e, int rk_step)
         const int max_ny = gr.max_planes - 2;
         if (!mixed) {
             const auto *d = (const dgpu::ColDesc<double, double> *)gr.d_col[k];
-            if (use_colseg() && dgpu::colseg_supported<double>(max_ny)) {
-                dgpu::launch_cn_seg_batch<double, double>(s, g, max_ny, d, nm);
-                dgpu::launch_remdiv_seg_batch<double, double>(s, g, max_ny, d, nm);
+            if (const int nw = seg_waves(e->m[gr.idx[0]], max_ny, false)) {
+                dgpu::launch_cn_seg_batch<double, double>(s, nw, g, max_ny, d, nm);
+                dgpu::launch_remdiv_seg_batch<double, double>(s, nw, g, max_ny, d, nm);
             } else {
                 dgpu::launch_cn_batch<double, double>(s, g, d, nm);
                 dgpu::launch_remdiv_batch<double, double>(s, g, d, nm);
             }
         } else if (!f32) {
             const auto *d = (const dgpu::ColDesc<double, float> *)gr.d_col[k];
-            if (use_colseg() && dgpu::colseg_supported<double>(max_ny)) {
-                dgpu::launch_cn_seg_batch<double, float>(s, g, max_ny, d, nm);
-                dgpu::launch_remdiv_seg_batch<double, float>(s, g, max_ny, d, nm);
+            if (const int nw = seg_waves(e->m[gr.idx[0]], max_ny, false)) {
+                dgpu::launch_cn_seg_batch<double, float>(s, nw, g, max_ny, d, nm);
+                dgpu::launch_remdiv_seg_batch<double, float>(s, nw, g, max_ny, d, nm);
             } else {
                 dgpu::launch_cn_batch<double, float>(s, g, d, nm);
                 dgpu::launch_remdiv_batch<double, float>(s, g, d, nm);
             }
         } else {
             const auto *d = (const dgpu::ColDesc<float, float> *)gr.d_col[k];
-            if (use_colseg() && dgpu::colseg_supported<float>(max_ny)) {
-                dgpu::launch_cn_seg_batch<float, float>(s, g, max_ny, d, nm);
-                dgpu::launch_remdiv_seg_batch<float, float>(s, g, max_ny, d, nm);
+            if (const int nf = fused_waves(e->m[gr.idx[0]], max_ny)) {
+                dgpu::launch_colfused_batch<float, float>(s, nf, g, max_ny, d, nm);
+            } else if (const int nw = seg_waves(e->m[gr.idx[0]], max_ny, true)) {
+                dgpu::launch_cn_seg_batch<float, float>(s, nw, g, max_ny, d, nm);
+                dgpu::launch_remdiv_seg_batch<float, float>(s, nw, g, max_ny, d, nm);
             } else {
                 dgpu::launch_cn_batch<float, float>(s, g, d, nm);
                 dgpu::launch_remdiv_batch<float, float>(s, g, d, nm);
@@ -2408,7 +2512,9 @@ int32_t dgpu_ensemble_rk_chan_1(dgpu_ensemble *e, int32_t rk_step, const double 
         return DGPU_OK;
     };
     if (!graphable) return enqueue_all(false);
-    if (!(e->gexec[k] && e->gkey[k] == key)) {
+    std::vector<int> shape;
+    for (const dgpu_ctx *c : e->m) shape.insert(shape.end(), {c->xchunk, c->colnw, c->zdft ? 1 : 0});
+    if (!(e->gexec[k] && e->gkey[k] == key && e->gshape[k] == shape)) {
         if (e->gexec[k]) {
             hipGraphExecDestroy(e->gexec[k]);
             e->gexec[k] = nullptr;
@@ -2431,6 +2537,7 @@ int32_t dgpu_ensemble_rk_chan_1(dgpu_ensemble *e, int32_t rk_step, const double 
             return DGPU_E_HIP;
         }
         e->gkey[k] = key;
+        e->gshape[k] = shape;
     }
     if (hipGraphLaunch(e->gexec[k], e->main) != hipSuccess) return DGPU_E_HIP;
     return DGPU_OK;

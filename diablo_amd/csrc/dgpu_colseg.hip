@@ -16,7 +16,15 @@
 // The solve's intermediate values never leave the chip, and the sequential
 // chain of a lane is one segment (9 to 17 rows) long instead of a column.
 //
-// k_cn_seg: one block per (64 or 32 modes, system): the explicit right-hand
+// Block shape: NW waves (4, 8 or 16; a runtime choice, dgpu_set_column_shape)
+// of 64 lanes; a lane is (mode, segment), MPW modes side by side so that a
+// wave's loads of one row are MPW consecutive modes.  The segment count is
+// fixed per precision (32 in FP64: rows per segment <= 9 up to NY = 286;
+// 16 in FP32), so MPW = 64 NW / NSEG.  The kernels ask for four waves per
+// SIMD (<= 128 VGPRs): one 16-wave block per CU, or two / four smaller
+// blocks whose phases (loads, carries, stores) overlap.
+//
+// k_cn_seg: one block per (MPW modes, system): the explicit right-hand
 //   side, the new CF, the CN solve; the solution of every row goes to the
 //   system's solve scratch, which holds the complete new field afterwards
 //   (rows outside the solve's copy range keep the old value).  The state's
@@ -47,11 +55,16 @@ __device__ __forceinline__ int xcd_block(int L, int nb8) { return (L & 7) * (nb8
 // ---------------------------------------------------------------------------
 // k_cn_seg
 // ---------------------------------------------------------------------------
-template <typename T, int NTH, typename NLT, int MPW, int RMAX, typename AT, typename PT>
-__device__ __forceinline__ void cn_seg_body(AT A, PT P, int bx, int q)
+// The CN solve of system q for the block's modes: sol[r] = the new value of
+// row a + r of the lane's segment (the rows outside the solve's copy range
+// keep the old value).  The LDS holds the system's table and the carries
+// (cn_seg_lds); a caller that solves several systems in a row puts a block
+// barrier between them.
+template <typename T, int NTH, typename NLT, int MPW, int NW, int RMAX, typename AT, typename PT>
+__device__ __forceinline__ void cn_seg_solve(AT A, PT P, int bx, int q, cx<T> (&sol)[RMAX])
 {
     constexpr int SPW = 64 / MPW;
-    constexpr int NSEG = 16 * SPW;
+    constexpr int NSEG = NW * SPW;
     extern __shared__ unsigned char smem_raw[];
     const Geo &g = A.g;
     const int NY = g.ny, N = NY + 2;
@@ -94,7 +107,6 @@ __device__ __forceinline__ void cn_seg_body(AT A, PT P, int bx, int q)
     cx<T> *fld = (q == SYS_U1) ? P.cu1 : (q == SYS_U3) ? P.cu3 : (q == SYS_U2) ? P.cu2 : P.cth;
     cx<T> *cf = (q == SYS_U1) ? P.cf1 : (q == SYS_U3) ? P.cf3 : (q == SYS_U2) ? P.cf2 : P.cfth;
     const cx<NLT> *nl = P.nl[q];
-    cx<T> *out = P.gs[q];
     const cx<T> *a1p = (q == SYS_U1) ? P.cu3 : P.cu1;   // Coriolis partner / TH advection
     const cx<T> *a2p = (q == SYS_TH0) ? P.cu3 : P.cth;  // buoyancy / TH advection
     const bool has1 = (q != SYS_U2);
@@ -243,25 +255,56 @@ __device__ __forceinline__ void cn_seg_body(AT A, PT P, int bx, int q)
                 e = first ? f : f * e;
                 first = false;
                 const long o = m + (long)j * ps;
-                out[o] = (t[L_CPY] != T(0)) ? cfma(e, xin, yv[r]) : fld[o];
+                sol[r] = (t[L_CPY] != T(0)) ? cfma(e, xin, yv[r]) : fld[o];
             }
         }
     }
 }
 
-template <typename T, int NTH, typename NLT, int MPW, int RMAX>
-__global__ void __launch_bounds__(1024) k_cn_seg(CnArgs A, CnPtrs<T, NLT> P)
+// k_cn_seg's block: system q's new field into its solve scratch P.gs[q]
+template <typename T, int NTH, typename NLT, int MPW, int NW, int RMAX, typename AT, typename PT>
+__device__ __forceinline__ void cn_seg_body(AT A, PT P, int bx, int q)
 {
-    cn_seg_body<T, NTH, NLT, MPW, RMAX, const CnArgs &, const CnPtrs<T, NLT> &>(
-        A, P, xcd_block(blockIdx.x, gridDim.x), blockIdx.y);
+    constexpr int NSEG = NW * (64 / MPW);
+    const int N = A.g.ny + 2;
+    const int R = (N + NSEG - 1) / NSEG;
+    const int lane = threadIdx.x & 63;
+    const int seg = (threadIdx.x >> 6) * (64 / MPW) + lane / MPW;
+    const long m = (long)bx * MPW + lane % MPW;
+    const int a = seg * R, b = min(a + R, N) - 1;
+    cx<T> sol[RMAX];
+    cn_seg_solve<T, NTH, NLT, MPW, NW, RMAX, AT, PT>(A, P, bx, q, sol);
+    if (m < A.g.ps && a <= b) {
+        cx<T> *out = P.gs[q];
+#pragma unroll
+        for (int r = 0; r < RMAX; ++r)
+            if (a + r <= b) out[m + (long)(a + r) * A.g.ps] = sol[r];
+    }
 }
 
-template <typename T, int NTH, typename NLT, int MPW, int RMAX>
-__global__ void __launch_bounds__(1024) k_cn_seg_batch(const ColDesc<T, NLT> *__restrict__ d)
+// One flat grid over (mode group, system), the systems of a mode group
+// adjacent on one XCD: the fields every system reads as explicit terms
+// (CU1, CU3, CTH, CP) come from HBM once per mode group and from that XCD's
+// L2 for the other systems' blocks.
+template <typename T, int NTH, typename NLT, int MPW, int NW, int RMAX>
+__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4)))
+k_cn_seg(CnArgs A, CnPtrs<T, NLT> P)
+{
+    constexpr int NSYS = 3 + NTH;
+    const int L = xcd_block(blockIdx.x, gridDim.x);
+    const int q = L % NSYS, bx = L / NSYS;
+    if ((long)bx * MPW >= A.g.ps) return;  // padding blocks of the flat grid (whole block)
+    cn_seg_body<T, NTH, NLT, MPW, NW, RMAX, const CnArgs &, const CnPtrs<T, NLT> &>(A, P, bx, q);
+}
+
+template <typename T, int NTH, typename NLT, int MPW, int NW, int RMAX>
+__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4)))
+k_cn_seg_batch(const ColDesc<T, NLT> *__restrict__ d)
 {
     const ColDesc<T, NLT> &D = d[blockIdx.z];
     if ((long)blockIdx.x * MPW >= D.A.g.ps) return;
-    cn_seg_body<T, NTH, NLT, MPW, RMAX, const CnArgs &, const CnPtrs<T, NLT> &>(D.A, D.P, blockIdx.x, blockIdx.y);
+    cn_seg_body<T, NTH, NLT, MPW, NW, RMAX, const CnArgs &, const CnPtrs<T, NLT> &>(D.A, D.P, blockIdx.x,
+                                                                                   blockIdx.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -272,11 +315,11 @@ __global__ void __launch_bounds__(1024) k_cn_seg_batch(const ColDesc<T, NLT> *__
 // and scalar come from the solve scratch (P.gs, slot per system); with
 // P.gs[q] equal to the state's own field the call works in place
 // (REM_DIV_CHAN on the state).
-template <typename T, typename NLT, int MPW, int RMAX, int NTH>
+template <typename T, typename NLT, int MPW, int NW, int RMAX, int NTH>
 __device__ __forceinline__ void remdiv_seg_body(const RdArgs &A, const CnPtrs<T, NLT> &P, int bx)
 {
     constexpr int SPW = 64 / MPW;
-    constexpr int NSEG = 16 * SPW;
+    constexpr int NSEG = NW * SPW;
     extern __shared__ unsigned char smem_raw[];
     const Geo &g = A.g;
     const int NY = g.ny;
@@ -438,18 +481,224 @@ __device__ __forceinline__ void remdiv_seg_body(const RdArgs &A, const CnPtrs<T,
     }
 }
 
-template <typename T, typename NLT, int MPW, int RMAX, int NTH>
-__global__ void __launch_bounds__(1024) k_remdiv_seg(RdArgs A, CnPtrs<T, NLT> P)
+template <typename T, typename NLT, int MPW, int NW, int RMAX, int NTH>
+__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4)))
+k_remdiv_seg(RdArgs A, CnPtrs<T, NLT> P)
 {
-    remdiv_seg_body<T, NLT, MPW, RMAX, NTH>(A, P, xcd_block(blockIdx.x, gridDim.x));
+    remdiv_seg_body<T, NLT, MPW, NW, RMAX, NTH>(A, P, xcd_block(blockIdx.x, gridDim.x));
 }
 
-template <typename T, typename NLT, int MPW, int RMAX, int NTH>
-__global__ void __launch_bounds__(1024) k_remdiv_seg_batch(const ColDesc<T, NLT> *__restrict__ d)
+template <typename T, typename NLT, int MPW, int NW, int RMAX, int NTH>
+__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4)))
+k_remdiv_seg_batch(const ColDesc<T, NLT> *__restrict__ d)
 {
     const ColDesc<T, NLT> &D = d[blockIdx.z];
     if ((long)blockIdx.x * MPW >= D.R.g.ps) return;
-    remdiv_seg_body<T, NLT, MPW, RMAX, NTH>(D.R, D.P, blockIdx.x);
+    remdiv_seg_body<T, NLT, MPW, NW, RMAX, NTH>(D.R, D.P, blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// k_colfused: the whole column stage of a substep for a group of modes with
+// the FP32 state.  The CN solves of U1, U3, U2 stay in registers; TH's goes
+// straight into CTH (every system of these modes has read the old CTH by
+// then, and no other block reads these modes); then the divergence of U*,
+// the Poisson solve, the projection into CU1..3 and the CP update.  The
+// state is read once and written once, with no solve scratch.  The Poisson
+// rows 1..NY run on the lanes that hold them in the CN part (32 segments of
+// rows 0..NY+1); a segment without Poisson rows passes the carries through.
+// ---------------------------------------------------------------------------
+template <typename T, typename NLT, int MPW, int NW, int RMAX, int NTH>
+__device__ __forceinline__ void colfused_body(const CnArgs &A, const RdArgs &Rd, const CnPtrs<T, NLT> &P, int bx)
+{
+    constexpr int SPW = 64 / MPW;
+    constexpr int NSEG = NW * SPW;
+    extern __shared__ unsigned char smem_raw[];
+    const Geo &g = A.g;
+    const int NY = g.ny, N = NY + 2;
+    const long ps = g.ps, fps = Rd.fac_ps;
+    const int R = (N + NSEG - 1) / NSEG;
+    const int nseg = (N + R - 1) / R;
+    T *tl = reinterpret_cast<T *>(smem_raw);                               // [N][kLRow] (cn_seg_solve)
+    cx<T> *GL = reinterpret_cast<cx<T> *>(tl + ((N * kLRow + 1) & ~1));   // [NSEG][MPW]
+    cx<T> *XL = GL + NSEG * MPW;                                           // [NSEG][MPW]
+    T *Cm = reinterpret_cast<T *>(XL + NSEG * MPW) + 2 * NSEG;             // [NSEG][MPW] (after CN's Cb, Ea)
+    T *Em = Cm + NSEG * MPW;                                               // [NSEG][MPW]
+    cx<T> *X2 = reinterpret_cast<cx<T> *>(Em + NSEG * MPW);                // [NSEG][MPW]: U2*(a)
+
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int seg = (tid >> 6) * SPW + lane / MPW, ml = lane % MPW;
+    const long m = (long)bx * MPW + ml;
+    const bool live = m < ps;
+    const int a = seg * R, b = min(a + R, N) - 1;
+    const bool rows = live && a <= b;
+    const cx<T> zero = make_cx<T>(0, 0);
+
+    // ---- the CN solves ----
+    cx<T> u1s[RMAX], u3s[RMAX], u2s[RMAX];
+    cn_seg_solve<T, NTH, NLT, MPW, NW, RMAX, const CnArgs &, const CnPtrs<T, NLT> &>(A, P, bx, SYS_U1, u1s);
+    __syncthreads();
+    cn_seg_solve<T, NTH, NLT, MPW, NW, RMAX, const CnArgs &, const CnPtrs<T, NLT> &>(A, P, bx, SYS_U3, u3s);
+    __syncthreads();
+    cn_seg_solve<T, NTH, NLT, MPW, NW, RMAX, const CnArgs &, const CnPtrs<T, NLT> &>(A, P, bx, SYS_U2, u2s);
+    if (NTH) {
+        __syncthreads();
+        cx<T> ts[RMAX];
+        cn_seg_solve<T, NTH, NLT, MPW, NW, RMAX, const CnArgs &, const CnPtrs<T, NLT> &>(A, P, bx, SYS_TH0, ts);
+        if (rows) {
+#pragma unroll
+            for (int r = 0; r < RMAX; ++r) {
+                const int j = a + r;
+                if (j <= b && tl[j * kLRow + L_CPY] != T(0)) P.cth[m + (long)j * ps] = ts[r];
+            }
+        }
+    }
+    __syncthreads();  // the carry arrays are reused below
+    if (rows) X2[seg * MPW + ml] = u2s[0];
+    __syncthreads();
+    const cx<T> u2nx = (rows && seg + 1 < nseg) ? X2[(seg + 1) * MPW + ml] : zero;  // U2*(b+1)
+
+    // ---- divergence of U* and the Poisson elimination, rows max(a,1)..min(b,NY) ----
+    int s = 0, i = 0;
+    const long gm = (live && Rd.map) ? (long)Rd.map[m] : m;
+    if (live) { s = (int)(gm / g.mk); i = (int)(gm - (long)s * g.mk); }
+    const bool is00 = live && i == 0 && s == 0;
+    const T kx = live ? T(Rd.kx[i]) : T(0), kz = live ? T(Rd.kz[s]) : T(0);
+    auto pm = [&](int j) -> T {
+        if constexpr (sizeof(T) == 4) return Rd.pmultf[gm + (long)j * fps];
+        else return T(Rd.pmult[gm + (long)j * fps]);
+    };
+    auto pv = [&](int j) -> T {
+        if constexpr (sizeof(T) == 4) return Rd.pinvf[gm + (long)j * fps];
+        else return T(Rd.pinv[gm + (long)j * fps]);
+    };
+    auto ik = [](T k, cx<T> x) { return make_cx<T>(-k * x.im, k * x.re); };
+    // U2* of the row after a + r: the next register, or the next segment's first row
+    auto u2up = [&](int r) -> cx<T> { return (a + r < b) ? u2s[r + 1 < RMAX ? r + 1 : r] : u2nx; };
+    cx<T> yv[RMAX];
+    T mr[RMAX], fr[RMAX];
+    if (rows) {
+        cx<T> hl = zero;
+        T cprod = T(1);
+        bool first = true;
+#pragma unroll
+        for (int r = 0; r < RMAX; ++r) {
+            const int j = a + r;
+            if (j <= b && j >= 1 && j <= NY) {
+                cx<T> vec = zero;
+                if (j >= 2 && j < NY) {
+                    // (i kx CU1 + (CU2(j+1) - CU2(j))/DYF(j)) + i kz CU3 (for/channel.f:1139-1141)
+                    const T idf = T(Rd.rd[(long)j * 4 + 1]);
+                    const cx<T> d1 = ik(kx, u1s[r]), d3 = ik(kz, u3s[r]), c2p = u2up(r), c2 = u2s[r];
+                    vec = make_cx<T>((d1.re + (c2p.re - c2.re) * idf) + d3.re,
+                                     (d1.im + (c2p.im - c2.im) * idf) + d3.im);
+                }
+                const T mu = j >= 2 ? pm(j) : T(0);
+                hl = first ? vec : cfma(mu, hl, vec);
+                cprod = first ? mu : cprod * mu;
+                first = false;
+                yv[r] = hl;
+                mr[r] = mu;
+            }
+        }
+        GL[seg * MPW + ml] = hl;
+        Cm[seg * MPW + ml] = cprod;
+    }
+    __syncthreads();
+    const T u1row = is00 ? T(0) : T(-1);
+    if (rows) {
+        cx<T> hin = zero;
+        for (int t = 0; t < seg; ++t) hin = cfma(Cm[t * MPW + ml], hin, GL[t * MPW + ml]);
+        T c = T(1);
+#pragma unroll
+        for (int r = 0; r < RMAX; ++r) {
+            const int j = a + r;
+            if (j <= b && j >= 1 && j <= NY) {
+                c *= mr[r];
+                yv[r] = cfma(c, hin, yv[r]);
+            }
+        }
+        cx<T> xl = zero;
+        T e = T(1);
+        bool first = true;
+#pragma unroll
+        for (int r = RMAX - 1; r >= 0; --r) {
+            const int j = a + r;
+            if (j <= b && j >= 1 && j <= NY) {
+                const T ip = pv(j);
+                const T u = (j == 1) ? u1row : T(Rd.rd[(long)j * 4 + 0]);
+                fr[r] = -u * ip;
+                if (first) {
+                    xl = make_cx<T>(yv[r].re * ip, yv[r].im * ip);
+                    e = fr[r];
+                    first = false;
+                } else {
+                    xl = make_cx<T>((yv[r].re - u * xl.re) * ip, (yv[r].im - u * xl.im) * ip);
+                    e = fr[r] * e;
+                }
+                yv[r] = xl;
+            }
+        }
+        XL[seg * MPW + ml] = xl;
+        Em[seg * MPW + ml] = e;
+    }
+    __syncthreads();
+    // ---- phi, the projection into CU1..3, the CP update ----
+    if (rows) {
+        cx<T> xin = zero;
+        for (int t = nseg - 1; t > seg; --t) xin = cfma(Em[t * MPW + ml], xin, XL[t * MPW + ml]);
+        const T itemp4 = T(Rd.inv_temp4);
+        const bool upd = Rd.update_cp;
+        cx<T> phip = xin;  // phi of the row after the lane's last Poisson row
+        T e = T(1);
+        bool first = true;
+#pragma unroll
+        for (int r = RMAX - 1; r >= 0; --r) {
+            const int j = a + r;
+            if (j <= b) {
+                const long o = m + (long)j * ps;
+                if (j >= 1 && j <= NY) {
+                    e = first ? fr[r] : fr[r] * e;
+                    first = false;
+                    const cx<T> phi = cfma(e, xin, yv[r]);
+                    // CU1 - i kx phi, CU3 - i kz phi (rows 1..NY), CU2(j+1) - (phi(j+1) - phi(j))/DY(j+1)
+                    P.cu1[o] = u1s[r] - ik(kx, phi);
+                    P.cu3[o] = u3s[r] - ik(kz, phi);
+                    if (j + 1 <= NY) {
+                        const T idy = T(Rd.rd[(long)(j + 1) * 4 + 2]);
+                        const cx<T> v = u2up(r);
+                        P.cu2[o + ps] = make_cx<T>(v.re - (phip.re - phi.re) * idy, v.im - (phip.im - phi.im) * idy);
+                    }
+                    if (upd && j >= Rd.jstart && j <= Rd.jend) {
+                        const cx<T> p0 = P.cp[o];
+                        P.cp[o] = make_cx<T>(p0.re + phi.re * itemp4, p0.im + phi.im * itemp4);
+                    }
+                    if (j == 1) P.cu2[o] = u2s[r];  // CU2(1) is outside the projection
+                    phip = phi;
+                } else {
+                    // rows 0 and NY+1: no projection
+                    P.cu1[o] = u1s[r];
+                    P.cu3[o] = u3s[r];
+                    P.cu2[o] = u2s[r];
+                }
+            }
+        }
+    }
+}
+
+template <typename T, typename NLT, int MPW, int NW, int RMAX, int NTH>
+__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4)))
+k_colfused(CnArgs A, RdArgs Rd, CnPtrs<T, NLT> P)
+{
+    colfused_body<T, NLT, MPW, NW, RMAX, NTH>(A, Rd, P, xcd_block(blockIdx.x, gridDim.x));
+}
+
+template <typename T, typename NLT, int MPW, int NW, int RMAX, int NTH>
+__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4)))
+k_colfused_batch(const ColDesc<T, NLT> *__restrict__ d)
+{
+    const ColDesc<T, NLT> &D = d[blockIdx.z];
+    if ((long)blockIdx.x * MPW >= D.A.g.ps) return;
+    colfused_body<T, NLT, MPW, NW, RMAX, NTH>(D.A, D.R, D.P, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -457,24 +706,26 @@ __global__ void __launch_bounds__(1024) k_remdiv_seg_batch(const ColDesc<T, NLT>
 // ---------------------------------------------------------------------------
 namespace {
 
+// segments per column: fixed per precision
 template <typename T>
-constexpr int seg_mpw() { return sizeof(T) == 4 ? 64 : 32; }
-template <typename T>
-constexpr int seg_nseg() { return 16 * (64 / seg_mpw<T>()); }
+constexpr int seg_nseg() { return sizeof(T) == 4 ? 16 : 32; }
+// modes per segment group for NW waves per block
+template <typename T, int NW>
+constexpr int seg_mpw() { return 64 * NW / seg_nseg<T>(); }
 
 template <typename T>
-size_t cn_seg_lds(int ny)
+size_t cn_seg_lds(int ny, int nw)
 {
-    constexpr int MPW = seg_mpw<T>(), NSEG = seg_nseg<T>();
+    const int nseg = seg_nseg<T>(), mpw = 64 * nw / nseg;
     const int n = ny + 2;
-    return sizeof(T) * (size_t)((n * kLRow + 1) & ~1) + 2 * sizeof(cx<T>) * NSEG * MPW + 2 * sizeof(T) * NSEG;
+    return sizeof(T) * (size_t)((n * kLRow + 1) & ~1) + 2 * sizeof(cx<T>) * nseg * mpw + 2 * sizeof(T) * nseg;
 }
 
 template <typename T>
-size_t remdiv_seg_lds()
+size_t remdiv_seg_lds(int nw)
 {
-    constexpr int MPW = seg_mpw<T>(), NSEG = seg_nseg<T>();
-    return 2 * sizeof(cx<T>) * NSEG * MPW + 2 * sizeof(T) * NSEG * MPW;
+    const int nseg = seg_nseg<T>(), mpw = 64 * nw / nseg;
+    return 2 * sizeof(cx<T>) * nseg * mpw + 2 * sizeof(T) * nseg * mpw;
 }
 
 // rows per segment of the tallest column -> the compile-time bound
@@ -486,93 +737,167 @@ bool with_rmax(int rows, F &&f)
     return false;
 }
 
+// waves per block -> the compile-time shape
+template <typename F>
+bool with_nw(int nw, F &&f)
+{
+    switch (nw) {
+    case 4: f(std::integral_constant<int, 4>{}); return true;
+    case 8: f(std::integral_constant<int, 8>{}); return true;
+    case 16: f(std::integral_constant<int, 16>{}); return true;
+    default: return false;
+    }
+}
+
 }  // namespace
 
 template <typename T>
-bool colseg_supported(int ny)
+bool colseg_supported(int ny, int nw)
+{
+    const int rows = (ny + 2 + seg_nseg<T>() - 1) / seg_nseg<T>();
+    return (nw == 4 || nw == 8 || nw == 16) && ny >= 4 && rows <= 17 && cn_seg_lds<T>(ny, nw) <= 64 * 1024;
+}
+
+template <typename T, typename NLT>
+void launch_cn_seg(hipStream_t s, int nw, const CnArgs &a, const CnPtrs<T, NLT> &p)
 {
     constexpr int NSEG = seg_nseg<T>();
-    const int rows = (ny + 2 + NSEG - 1) / NSEG;
-    return ny >= 4 && rows <= 17 && cn_seg_lds<T>(ny) <= 64 * 1024;
+    const size_t lds = cn_seg_lds<T>(a.g.ny, nw);
+    with_nw(nw, [&](auto wc) {
+        constexpr int NW = decltype(wc)::value, MPW = seg_mpw<T, NW>();
+        const long groups = (a.g.ps + MPW - 1) / MPW;
+        const long blocks = (groups * (3 + a.g.nth) + 7) / 8 * 8;  // a multiple of 8 (xcd_block)
+        const dim3 grid(blocks);
+        with_rmax((a.g.ny + 2 + NSEG - 1) / NSEG, [&](auto rc) {
+            constexpr int RM = decltype(rc)::value;
+            if (a.g.nth == 0)
+                hipLaunchKernelGGL((k_cn_seg<T, 0, NLT, MPW, NW, RM>), grid, dim3(64 * NW), lds, s, a, p);
+            else
+                hipLaunchKernelGGL((k_cn_seg<T, 1, NLT, MPW, NW, RM>), grid, dim3(64 * NW), lds, s, a, p);
+        });
+    });
 }
 
 template <typename T, typename NLT>
-void launch_cn_seg(hipStream_t s, const CnArgs &a, const CnPtrs<T, NLT> &p)
+void launch_remdiv_seg(hipStream_t s, int nw, const RdArgs &a, const CnPtrs<T, NLT> &p)
 {
-    constexpr int MPW = seg_mpw<T>(), NSEG = seg_nseg<T>();
-    const long blocks = ((a.g.ps + MPW - 1) / MPW + 7) / 8 * 8;  // a multiple of 8 (xcd_block)
-    const dim3 grid(blocks, 3 + a.g.nth);
-    const size_t lds = cn_seg_lds<T>(a.g.ny);
-    with_rmax((a.g.ny + 2 + NSEG - 1) / NSEG, [&](auto rc) {
-        constexpr int RM = decltype(rc)::value;
+    constexpr int NSEG = seg_nseg<T>();
+    const size_t lds = remdiv_seg_lds<T>(nw);
+    with_nw(nw, [&](auto wc) {
+        constexpr int NW = decltype(wc)::value, MPW = seg_mpw<T, NW>();
+        const long blocks = ((a.g.ps + MPW - 1) / MPW + 7) / 8 * 8;
+        with_rmax((a.g.ny + NSEG - 1) / NSEG, [&](auto rc) {
+            constexpr int RM = decltype(rc)::value;
+            if (a.g.nth == 0)
+                hipLaunchKernelGGL((k_remdiv_seg<T, NLT, MPW, NW, RM, 0>), dim3(blocks), dim3(64 * NW), lds, s, a, p);
+            else
+                hipLaunchKernelGGL((k_remdiv_seg<T, NLT, MPW, NW, RM, 1>), dim3(blocks), dim3(64 * NW), lds, s, a, p);
+        });
+    });
+}
+
+template <typename T, typename NLT>
+void launch_cn_seg_batch(hipStream_t s, int nw, const Geo &g, int max_ny, const ColDesc<T, NLT> *d, int nmem)
+{
+    constexpr int NSEG = seg_nseg<T>();
+    const size_t lds = cn_seg_lds<T>(max_ny, nw);
+    with_nw(nw, [&](auto wc) {
+        constexpr int NW = decltype(wc)::value, MPW = seg_mpw<T, NW>();
+        const dim3 grid((g.ps + MPW - 1) / MPW, 3 + g.nth, nmem);
+        with_rmax((max_ny + 2 + NSEG - 1) / NSEG, [&](auto rc) {
+            constexpr int RM = decltype(rc)::value;
+            if (g.nth == 0)
+                hipLaunchKernelGGL((k_cn_seg_batch<T, 0, NLT, MPW, NW, RM>), grid, dim3(64 * NW), lds, s, d);
+            else
+                hipLaunchKernelGGL((k_cn_seg_batch<T, 1, NLT, MPW, NW, RM>), grid, dim3(64 * NW), lds, s, d);
+        });
+    });
+}
+
+template <typename T, typename NLT>
+void launch_remdiv_seg_batch(hipStream_t s, int nw, const Geo &g, int max_ny, const ColDesc<T, NLT> *d, int nmem)
+{
+    constexpr int NSEG = seg_nseg<T>();
+    const size_t lds = remdiv_seg_lds<T>(nw);
+    with_nw(nw, [&](auto wc) {
+        constexpr int NW = decltype(wc)::value, MPW = seg_mpw<T, NW>();
+        const dim3 grid((g.ps + MPW - 1) / MPW, 1, nmem);
+        with_rmax((max_ny + NSEG - 1) / NSEG, [&](auto rc) {
+            constexpr int RM = decltype(rc)::value;
+            if (g.nth == 0)
+                hipLaunchKernelGGL((k_remdiv_seg_batch<T, NLT, MPW, NW, RM, 0>), grid, dim3(64 * NW), lds, s, d);
+            else
+                hipLaunchKernelGGL((k_remdiv_seg_batch<T, NLT, MPW, NW, RM, 1>), grid, dim3(64 * NW), lds, s, d);
+        });
+    });
+}
+
+// The fused column kernel: 32 segments per column, at most 9 rows each
+// (NY <= 286), FP32 state.
+constexpr int kFusedSeg = 32;
+
+static size_t colfused_lds(int ny, int nw)
+{
+    const int mpw = 64 * nw / kFusedSeg, n = ny + 2;
+    using T = float;
+    return sizeof(T) * (size_t)((n * kLRow + 1) & ~1) + 2 * sizeof(cx<T>) * kFusedSeg * mpw +
+           2 * sizeof(T) * kFusedSeg + 2 * sizeof(T) * kFusedSeg * mpw + sizeof(cx<T>) * kFusedSeg * mpw;
+}
+
+bool colfused_supported(int ny, int nw)
+{
+    return (nw == 4 || nw == 8 || nw == 16) && ny >= 4 && (ny + 2 + kFusedSeg - 1) / kFusedSeg <= 9 &&
+           colfused_lds(ny, nw) <= 64 * 1024;
+}
+
+template <typename T, typename NLT>
+void launch_colfused(hipStream_t s, int nw, const CnArgs &a, const RdArgs &r, const CnPtrs<T, NLT> &p)
+{
+    const size_t lds = colfused_lds(a.g.ny, nw);
+    with_nw(nw, [&](auto wc) {
+        constexpr int NW = decltype(wc)::value, MPW = 64 * NW / kFusedSeg;
+        const long blocks = ((a.g.ps + MPW - 1) / MPW + 7) / 8 * 8;  // a multiple of 8 (xcd_block)
         if (a.g.nth == 0)
-            hipLaunchKernelGGL((k_cn_seg<T, 0, NLT, MPW, RM>), grid, dim3(1024), lds, s, a, p);
+            hipLaunchKernelGGL((k_colfused<T, NLT, MPW, NW, 9, 0>), dim3(blocks), dim3(64 * NW), lds, s, a, r, p);
         else
-            hipLaunchKernelGGL((k_cn_seg<T, 1, NLT, MPW, RM>), grid, dim3(1024), lds, s, a, p);
+            hipLaunchKernelGGL((k_colfused<T, NLT, MPW, NW, 9, 1>), dim3(blocks), dim3(64 * NW), lds, s, a, r, p);
     });
 }
 
 template <typename T, typename NLT>
-void launch_remdiv_seg(hipStream_t s, const RdArgs &a, const CnPtrs<T, NLT> &p)
+void launch_colfused_batch(hipStream_t s, int nw, const Geo &g, int max_ny, const ColDesc<T, NLT> *d, int nmem)
 {
-    constexpr int MPW = seg_mpw<T>(), NSEG = seg_nseg<T>();
-    const long blocks = ((a.g.ps + MPW - 1) / MPW + 7) / 8 * 8;
-    with_rmax((a.g.ny + NSEG - 1) / NSEG, [&](auto rc) {
-        constexpr int RM = decltype(rc)::value;
-        if (a.g.nth == 0)
-            hipLaunchKernelGGL((k_remdiv_seg<T, NLT, MPW, RM, 0>), dim3(blocks), dim3(1024), remdiv_seg_lds<T>(), s,
-                               a, p);
-        else
-            hipLaunchKernelGGL((k_remdiv_seg<T, NLT, MPW, RM, 1>), dim3(blocks), dim3(1024), remdiv_seg_lds<T>(), s,
-                               a, p);
-    });
-}
-
-template <typename T, typename NLT>
-void launch_cn_seg_batch(hipStream_t s, const Geo &g, int max_ny, const ColDesc<T, NLT> *d, int nmem)
-{
-    constexpr int MPW = seg_mpw<T>(), NSEG = seg_nseg<T>();
-    const dim3 grid((g.ps + MPW - 1) / MPW, 3 + g.nth, nmem);
-    const size_t lds = cn_seg_lds<T>(max_ny);
-    with_rmax((max_ny + 2 + NSEG - 1) / NSEG, [&](auto rc) {
-        constexpr int RM = decltype(rc)::value;
+    const size_t lds = colfused_lds(max_ny, nw);
+    with_nw(nw, [&](auto wc) {
+        constexpr int NW = decltype(wc)::value, MPW = 64 * NW / kFusedSeg;
+        const dim3 grid((g.ps + MPW - 1) / MPW, 1, nmem);
         if (g.nth == 0)
-            hipLaunchKernelGGL((k_cn_seg_batch<T, 0, NLT, MPW, RM>), grid, dim3(1024), lds, s, d);
+            hipLaunchKernelGGL((k_colfused_batch<T, NLT, MPW, NW, 9, 0>), grid, dim3(64 * NW), lds, s, d);
         else
-            hipLaunchKernelGGL((k_cn_seg_batch<T, 1, NLT, MPW, RM>), grid, dim3(1024), lds, s, d);
+            hipLaunchKernelGGL((k_colfused_batch<T, NLT, MPW, NW, 9, 1>), grid, dim3(64 * NW), lds, s, d);
     });
 }
 
-template <typename T, typename NLT>
-void launch_remdiv_seg_batch(hipStream_t s, const Geo &g, int max_ny, const ColDesc<T, NLT> *d, int nmem)
-{
-    constexpr int MPW = seg_mpw<T>(), NSEG = seg_nseg<T>();
-    const dim3 grid((g.ps + MPW - 1) / MPW, 1, nmem);
-    with_rmax((max_ny + NSEG - 1) / NSEG, [&](auto rc) {
-        constexpr int RM = decltype(rc)::value;
-        if (g.nth == 0)
-            hipLaunchKernelGGL((k_remdiv_seg_batch<T, NLT, MPW, RM, 0>), grid, dim3(1024), remdiv_seg_lds<T>(), s, d);
-        else
-            hipLaunchKernelGGL((k_remdiv_seg_batch<T, NLT, MPW, RM, 1>), grid, dim3(1024), remdiv_seg_lds<T>(), s, d);
-    });
-}
-
-template bool colseg_supported<double>(int);
-template bool colseg_supported<float>(int);
-template void launch_cn_seg<double, double>(hipStream_t, const CnArgs &, const CnPtrs<double, double> &);
-template void launch_cn_seg<double, float>(hipStream_t, const CnArgs &, const CnPtrs<double, float> &);
-template void launch_cn_seg<float, float>(hipStream_t, const CnArgs &, const CnPtrs<float, float> &);
-template void launch_remdiv_seg<double, double>(hipStream_t, const RdArgs &, const CnPtrs<double, double> &);
-template void launch_remdiv_seg<double, float>(hipStream_t, const RdArgs &, const CnPtrs<double, float> &);
-template void launch_remdiv_seg<float, float>(hipStream_t, const RdArgs &, const CnPtrs<float, float> &);
-template void launch_cn_seg_batch<double, double>(hipStream_t, const Geo &, int, const ColDesc<double, double> *, int);
-template void launch_cn_seg_batch<double, float>(hipStream_t, const Geo &, int, const ColDesc<double, float> *, int);
-template void launch_cn_seg_batch<float, float>(hipStream_t, const Geo &, int, const ColDesc<float, float> *, int);
-template void launch_remdiv_seg_batch<double, double>(hipStream_t, const Geo &, int, const ColDesc<double, double> *,
+template void launch_colfused<float, float>(hipStream_t, int, const CnArgs &, const RdArgs &,
+                                            const CnPtrs<float, float> &);
+template void launch_colfused_batch<float, float>(hipStream_t, int, const Geo &, int, const ColDesc<float, float> *,
+                                                  int);
+template bool colseg_supported<double>(int, int);
+template bool colseg_supported<float>(int, int);
+template void launch_cn_seg<double, double>(hipStream_t, int, const CnArgs &, const CnPtrs<double, double> &);
+template void launch_cn_seg<double, float>(hipStream_t, int, const CnArgs &, const CnPtrs<double, float> &);
+template void launch_cn_seg<float, float>(hipStream_t, int, const CnArgs &, const CnPtrs<float, float> &);
+template void launch_remdiv_seg<double, double>(hipStream_t, int, const RdArgs &, const CnPtrs<double, double> &);
+template void launch_remdiv_seg<double, float>(hipStream_t, int, const RdArgs &, const CnPtrs<double, float> &);
+template void launch_remdiv_seg<float, float>(hipStream_t, int, const RdArgs &, const CnPtrs<float, float> &);
+template void launch_cn_seg_batch<double, double>(hipStream_t, int, const Geo &, int, const ColDesc<double, double> *, int);
+template void launch_cn_seg_batch<double, float>(hipStream_t, int, const Geo &, int, const ColDesc<double, float> *, int);
+template void launch_cn_seg_batch<float, float>(hipStream_t, int, const Geo &, int, const ColDesc<float, float> *, int);
+template void launch_remdiv_seg_batch<double, double>(hipStream_t, int, const Geo &, int, const ColDesc<double, double> *,
                                                       int);
-template void launch_remdiv_seg_batch<double, float>(hipStream_t, const Geo &, int, const ColDesc<double, float> *,
+template void launch_remdiv_seg_batch<double, float>(hipStream_t, int, const Geo &, int, const ColDesc<double, float> *,
                                                      int);
-template void launch_remdiv_seg_batch<float, float>(hipStream_t, const Geo &, int, const ColDesc<float, float> *,
+template void launch_remdiv_seg_batch<float, float>(hipStream_t, int, const Geo &, int, const ColDesc<float, float> *,
                                                     int);
 
 }  // namespace dgpu

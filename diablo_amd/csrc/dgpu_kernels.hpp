@@ -100,17 +100,25 @@ void launch_side_scatter_batch(hipStream_t s, const ScatDesc<TS, TF> *d, int nme
 template <typename T, typename NLT>
 void launch_cn_batch(hipStream_t s, const Geo &g, const ColDesc<T, NLT> *d, int nmem);
 // Segmented column kernels (dgpu_colseg.hip): the CN solutions go to P.gs[q]
-// (complete new fields), k_remdiv_seg projects them into the state.
+// (complete new fields), k_remdiv_seg projects them into the state.  nw:
+// waves per block (4, 8 or 16).
 template <typename T>
-bool colseg_supported(int ny);
+bool colseg_supported(int ny, int nw);
 template <typename T, typename NLT>
-void launch_cn_seg(hipStream_t s, const CnArgs &a, const CnPtrs<T, NLT> &p);
+void launch_cn_seg(hipStream_t s, int nw, const CnArgs &a, const CnPtrs<T, NLT> &p);
 template <typename T, typename NLT>
-void launch_remdiv_seg(hipStream_t s, const RdArgs &a, const CnPtrs<T, NLT> &p);
+void launch_remdiv_seg(hipStream_t s, int nw, const RdArgs &a, const CnPtrs<T, NLT> &p);
 template <typename T, typename NLT>
-void launch_cn_seg_batch(hipStream_t s, const Geo &g, int max_ny, const ColDesc<T, NLT> *d, int nmem);
+void launch_cn_seg_batch(hipStream_t s, int nw, const Geo &g, int max_ny, const ColDesc<T, NLT> *d, int nmem);
 template <typename T, typename NLT>
-void launch_remdiv_seg_batch(hipStream_t s, const Geo &g, int max_ny, const ColDesc<T, NLT> *d, int nmem);
+void launch_remdiv_seg_batch(hipStream_t s, int nw, const Geo &g, int max_ny, const ColDesc<T, NLT> *d, int nmem);
+// The fused column stage (FP32 state): the CN solves, the divergence, the
+// Poisson solve, the projection and the CP update in one kernel, no scratch.
+bool colfused_supported(int ny, int nw);
+template <typename T, typename NLT>
+void launch_colfused(hipStream_t s, int nw, const CnArgs &a, const RdArgs &r, const CnPtrs<T, NLT> &p);
+template <typename T, typename NLT>
+void launch_colfused_batch(hipStream_t s, int nw, const Geo &g, int max_ny, const ColDesc<T, NLT> *d, int nmem);
 template <typename T, typename NLT>
 void launch_remdiv_batch(hipStream_t s, const Geo &g, const ColDesc<T, NLT> *d, int nmem);
 template <typename T, typename NLT>

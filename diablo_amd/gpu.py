@@ -58,7 +58,7 @@ EXPORTS = (
     "dgpu_ensemble_last_error", "dgpu_ensemble_set_graphs", "dgpu_ensemble_rk_chan_1",
     "dgpu_ensemble_synchronize", "dgpu_stats_chan", "dgpu_courant_request", "dgpu_courant",
     "dgpu_get_flow_physical", "dgpu_set_flow_physical", "dgpu_tke_chan", "dgpu_les_chan", "dgpu_set_les", "dgpu_set_zdft", "dgpu_autotune", "dgpu_tke_les",
-    "dgpu_stats_chan_noalloc",
+    "dgpu_stats_chan_noalloc", "dgpu_set_column_shape", "dgpu_get_column_shape",
 )
 
 # dgpu_stats_chan profiles, in the order of the header's DGPU_ST_*
@@ -131,6 +131,10 @@ def load_library(path: str = LIB_PATH):
     L.dgpu_tke_les.argtypes = [vp, DP]
     L.dgpu_set_zdft.restype = i32
     L.dgpu_set_zdft.argtypes = [vp, i32]
+    L.dgpu_set_column_shape.restype = i32
+    L.dgpu_set_column_shape.argtypes = [vp, i32, i32]
+    L.dgpu_get_column_shape.restype = i32
+    L.dgpu_get_column_shape.argtypes = [vp, C.POINTER(i32), C.POINTER(i32)]
     L.dgpu_autotune.restype = i32
     L.dgpu_autotune.argtypes = [vp, i32, C.c_double, C.POINTER(i32), C.POINTER(i32), DP]
     L.dgpu_get_flow_physical.restype = i32
@@ -327,7 +331,25 @@ class GpuChannel:
         xc, zd, ms = C.c_int32(0), C.c_int32(0), C.c_double(0.0)
         self._chk(self.L.dgpu_autotune(self.h, steps, self.delta_t, C.byref(xc), C.byref(zd), C.byref(ms)),
                   "dgpu_autotune")
-        return {"xchunk": xc.value, "zdft": bool(zd.value), "ms_per_step": ms.value}
+        return {"xchunk": xc.value, "zdft": bool(zd.value), "column_waves": self.column_shape(),
+                "column_fused": self.column_fused(), "ms_per_step": ms.value}
+
+    def set_column_shape(self, waves: int, fused: int = -1):
+        """Waves per block of the segmented column kernels (4, 8, 16), or 0
+        for the whole-column sweeps; fused (FP32 state): 1 the one-kernel
+        column stage, 0 two kernels, -1 the library's choice by NY."""
+        self._chk(self.L.dgpu_set_column_shape(self.h, int(waves), int(fused)), "dgpu_set_column_shape")
+
+    def _column(self):
+        w, f = C.c_int32(0), C.c_int32(0)
+        self._chk(self.L.dgpu_get_column_shape(self.h, C.byref(w), C.byref(f)), "dgpu_get_column_shape")
+        return w.value, bool(f.value)
+
+    def column_shape(self) -> int:
+        return self._column()[0]
+
+    def column_fused(self) -> bool:
+        return self._column()[1]
 
     def tke_les(self) -> np.ndarray:
         """tkebudget_chan_les's epsilon_sgs at J = 1..NY (LES on; J = 1 is NaN

@@ -276,12 +276,40 @@ int32_t dgpu_tke_les(dgpu_ctx *ctx, double *eps_sgs);
  * multiple of 16; the x transforms stay FFTs.  on = 0 returns to the FFTs. */
 int32_t dgpu_set_zdft(dgpu_ctx *ctx, int32_t on);
 
+/* Column-stage launch shape (no counterpart in the reference): the
+ * Crank-Nicolson and REM_DIV column solves of RK_CHAN_1 run as segmented
+ * solves (each y column cut into 32 segments in FP64, 16 in FP32, whose
+ * carries pass through LDS) in blocks of `waves` wavefronts: 4, 8 or 16
+ * (modes per block 2 * waves in FP64, 4 * waves in FP32); 0 selects the
+ * whole-column sweeps.  The default is 8, what dgpu_autotune measures best
+ * at 384 x 257 x 384 (environment: DGPU_COLSEG_NW, DGPU_COLSEG=0).  A shape
+ * the column does not fit (more than 17 rows per segment, NY > 542 in
+ * FP64) is refused with DGPU_E_ARG; a context whose NY is outside the
+ * segmented kernels' range runs the sweeps whatever the setting.  The
+ * results are the same bits for 4, 8 and 16 (the same operations in the
+ * same order for each mode); the sweeps round differently (a different
+ * elimination order), within round-off.
+ * fused (contexts with the FP32 state, DGPU_MIXED): 1 runs the whole column
+ * stage -- the four CN solves, the divergence, the Poisson solve, the
+ * projection and the CP update -- as one kernel that reads and writes the
+ * state once (32 segments, NY <= 286); 0 keeps the two kernels; -1 (the
+ * default; environment DGPU_COLFUSE) fuses from NY = 160 up.  The fused
+ * kernel rounds differently from the two kernels (32 segments, not 16),
+ * within FP32 round-off.  An RK stage 3 with a COURANT request runs the two
+ * kernels (the maxima are taken between them).
+ * dgpu_get_column_shape reports the waves in use (0 for the sweeps) and
+ * whether the fused kernel runs (fused may be NULL). */
+int32_t dgpu_set_column_shape(dgpu_ctx *ctx, int32_t waves, int32_t fused);
+int32_t dgpu_get_column_shape(const dgpu_ctx *ctx, int32_t *waves, int32_t *fused);
+
 /* Launch-shape autotuner (no counterpart in the reference): times `steps` RK3
  * steps of size delta_t (after one untimed step) of the context's own state
  * for each candidate -- planes per block of the fused x kernel (4..64), and
- * for mixed contexts with NX, NZ <= 256 the FFT or the MFMA z transforms --
- * keeps the fastest and puts the state back as it was (the run does not
- * advance).
+ * for mixed contexts with NX, NZ <= 256 the FFT or the MFMA z transforms;
+ * then, with those fixed, the column shape (dgpu_set_column_shape: 4, 8 or
+ * 16 waves per block, unless the sweeps are selected; with the FP32 state
+ * also the fused column kernel on and off) -- keeps the fastest and puts
+ * the state back as it was (the run does not advance).
  * Outputs (may be NULL): the chosen planes per block, whether the MFMA z
  * transforms are on, and the best ms per step.  Not with a COURANT request
  * pending.  Allocates a copy of the state while it runs. */
