@@ -452,7 +452,7 @@ int32_t dgpu_create(const dgpu_params *p, dgpu_ctx **out)
     if (!p || !out) return DGPU_E_ARG;
     *out = nullptr;
     if (p->nx < 4 || p->nz < 4 || p->ny < 4 || (p->nx & 1) || (p->nz & 1)) return DGPU_E_ARG;
-    if (p->n_th < 0 || p->n_th > 1) return DGPU_E_ARG;  // one active scalar supported by the fused kernels
+    if (p->n_th < 0 || p->n_th > DGPU_MAX_TH) return DGPU_E_ARG;  // the fused kernels carry one scalar
     if (p->precision != DGPU_FP64 && p->precision != DGPU_MIXED && p->precision != DGPU_MIXED_F64STATE)
         return DGPU_E_ARG;
     if (!p->gy) return DGPU_E_ARG;

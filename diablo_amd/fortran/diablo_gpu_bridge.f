@@ -32,7 +32,7 @@
       USE ISO_C_BINDING
       IMPLICIT NONE
 
-      INTEGER, PARAMETER :: DGPU_MAX_TH = 4
+      INTEGER, PARAMETER :: DGPU_MAX_TH = 1
       INTEGER, PARAMETER :: DGPU_CU1 = 0, DGPU_CU2 = 1, DGPU_CU3 = 2
       INTEGER, PARAMETER :: DGPU_CP = 3, DGPU_CF1 = 4, DGPU_CF2 = 5
       INTEGER, PARAMETER :: DGPU_CF3 = 6, DGPU_CTH0 = 16
@@ -188,6 +188,8 @@
       INTEGER ENVLEN, ENVSTAT
 
       IF (C_ASSOCIATED(DGPU_CTX)) RETURN
+      IF (N_TH .GT. DGPU_MAX_TH) ERROR STOP
+     &  'DIABLO GPU path: N_TH > 1 is not supported'
       PRM%NX = NX
       PRM%NY = NY
       PRM%NZ = NZ

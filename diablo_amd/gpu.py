@@ -13,13 +13,16 @@ import os
 
 import numpy as np
 
-from .case import MAX_TH, ChannelConfig, grid_gy, rkw3_table
+from .case import ChannelConfig, grid_gy, rkw3_table
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libdiablo_gpu.so")
 
-I32x = C.c_int32 * MAX_TH
-F64x = C.c_double * MAX_TH
+# include/diablo_gpu.h: DGPU_ABI_VERSION, DGPU_MAX_TH (active scalars the device path carries)
+ABI_VERSION = 2
+DGPU_MAX_TH = 1
+I32x = C.c_int32 * DGPU_MAX_TH
+F64x = C.c_double * DGPU_MAX_TH
 DP = C.POINTER(C.c_double)
 
 # field ids (include/diablo_gpu.h)
@@ -78,6 +81,8 @@ def load_library(path: str = LIB_PATH):
     L = C.CDLL(path)
     vp, i32 = C.c_void_p, C.c_int32
     L.dgpu_abi_version.restype = i32
+    if L.dgpu_abi_version() != ABI_VERSION:
+        raise RuntimeError(f"{path}: ABI version {L.dgpu_abi_version()}, expected {ABI_VERSION} (rebuild it)")
     L.dgpu_build_info.restype = C.c_char_p
     L.dgpu_create.restype = i32
     L.dgpu_create.argtypes = [C.POINTER(Params), C.POINTER(vp)]
@@ -159,6 +164,8 @@ PRECISIONS = {"fp64": 0, "mixed": 1, "mixed_f64state": 2}
 
 
 def params_of(cfg: ChannelConfig, gy_1: np.ndarray, precision: str = "fp64") -> Params:
+    if cfg.n_th > DGPU_MAX_TH:
+        raise DgpuError(f"N_TH = {cfg.n_th}: the device path carries at most {DGPU_MAX_TH} scalar")
     p = Params()
     for name, _ in Params._fields_:
         if name in ("gy", "gyf", "dy", "dyf", "kx", "kz", "kx2", "kz2", "precision"):
@@ -166,7 +173,7 @@ def params_of(cfg: ChannelConfig, gy_1: np.ndarray, precision: str = "fp64") -> 
         v = getattr(cfg, name)
         if isinstance(v, list):
             arr = getattr(p, name)
-            for n, x in enumerate(v[:MAX_TH]):
+            for n, x in enumerate(v[:DGPU_MAX_TH]):
                 arr[n] = x
         else:
             setattr(p, name, v)

@@ -34,8 +34,12 @@
 extern "C" {
 #endif
 
-#define DGPU_ABI_VERSION 1
-#define DGPU_MAX_TH 4
+#define DGPU_ABI_VERSION 2
+/* Active scalars the device path carries: N_TH = 0 or 1.  The reference
+ * loops over N_TH scalars (DO N = 1, N_TH, e.g. for/channel.f:203, 314);
+ * the fused kernels here carry one (products, transforms and column systems
+ * are laid out for U1, U2, U3, TH), and dgpu_create refuses n_th > 1. */
+#define DGPU_MAX_TH 1
 
 enum dgpu_status {
     DGPU_OK = 0,

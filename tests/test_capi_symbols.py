@@ -40,7 +40,8 @@ def test_python_mirror_lists_all_exports():
 
 
 def test_abi_version(lib):
-    assert lib.dgpu_abi_version() == 1
+    from diablo_amd import gpu
+    assert lib.dgpu_abi_version() == gpu.ABI_VERSION == 2
     assert b"gfx950" in lib.dgpu_build_info()
 
 
