@@ -120,7 +120,8 @@ def accuracy(cfg, phys, steps: int, zdft: bool = False) -> dict:
     return r
 
 
-def ensemble_run(n: int, steps: int, warmup: int, precision: str = "mixed", acc_steps: int = 0) -> dict:
+def ensemble_run(n: int, steps: int, warmup: int, precision: str = "mixed", acc_steps: int = 0,
+                 tune: bool = False) -> dict:
     """Aggregate throughput of n independent KH runs on uneven grids advanced
     together (members with the same NX, NZ share one launch per kernel kind,
     one graph per RK stage); with acc_steps > 0 also the worst member's error
@@ -133,6 +134,7 @@ def ensemble_run(n: int, steps: int, warmup: int, precision: str = "mixed", acc_
         m.initialize(*p)
     ens.step(warmup)
     ens.synchronize()
+    tuned = ens.autotune(2) if tune else None
     t0 = time.perf_counter()
     ens.step(steps)
     ens.synchronize()
@@ -142,7 +144,8 @@ def ensemble_run(n: int, steps: int, warmup: int, precision: str = "mixed", acc_
     out = {"value": round(pts * 3 * steps / el / 1e6, 2), "unit": "Mpts*substeps/s", "members": n,
            "steps": steps, "precision": precision, "ms_per_step": round(el / steps * 1e3, 3),
            "points_total": pts, "per_member_min": round(min(per), 2), "per_member_max": round(max(per), 2),
-           "grids": "NX=NZ in {64,96,128,192} x NY in {33,65,97,129}, Ri in [0.05,0.25], Re in {500,1000}"}
+           "grids": "NX=NZ in {64,96,128,192} x NY in {33,65,97,129}, Ri in [0.05,0.25], Re in {500,1000}",
+           "autotune": tuned}
     ens.close()
     if acc_steps > 0 and precision != "fp64":
         e64 = gpu.GpuEnsemble(cfgs, precision="fp64")
@@ -418,7 +421,7 @@ def main():
     if args.ensemble > 0:
         log(f"[bench] batched: {args.ensemble} KH members, {args.ensemble_precision}, {args.ensemble_steps} steps")
         batched = ensemble_run(args.ensemble, args.ensemble_steps, 1, args.ensemble_precision,
-                               args.ensemble_accuracy_steps)
+                               args.ensemble_accuracy_steps, tune=args.autotune == "on")
         log(f"[bench] batched aggregate {batched['value']:.1f} Mpts*substeps/s "
             f"({batched['value'] / value:.2f}x FP64 384); {batched}")
 

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -2540,6 +2541,108 @@ int32_t dgpu_ensemble_rk_chan_1(dgpu_ensemble *e, int32_t rk_step, const double 
         e->gshape[k] = shape;
     }
     if (hipGraphLaunch(e->gexec[k], e->main) != hipSuccess) return DGPU_E_HIP;
+    return DGPU_OK;
+}
+
+// Ensemble launch shapes: every member's x-kernel planes per block (the
+// batched x launch of a group takes its first member's) and column shape,
+// timed over the ensemble's own states, which are put back afterwards.
+int32_t dgpu_ensemble_autotune(dgpu_ensemble *e, int32_t steps, const double *h_bar, const double *beta_bar,
+                               const double *zeta_bar, int32_t *xchunk_out, int32_t *waves_out, double *ms_per_step)
+{
+    if (!e || steps < 1 || !h_bar || !beta_bar || !zeta_bar) return DGPU_E_ARG;
+    const int n = (int)e->m.size();
+    for (const dgpu_ctx *c : e->m)
+        if (c->courant_state != 0) {
+            e->err = "ensemble autotune: a member has a COURANT request pending";
+            return DGPU_E_ARG;
+        }
+    if (int rc = dgpu_ensemble_synchronize(e)) return rc;
+    std::vector<void *> backup(n, nullptr);
+    auto free_all = [&]() {
+        for (void *b : backup)
+            if (b) hipFree(b);
+    };
+    for (int i = 0; i < n; ++i) {
+        dgpu_ctx *c = e->m[i];
+        const size_t bytes = sizeof(cx<double>) * c->fs * c->nslot;
+        int rc = to_master(c);
+        if (!rc && hipMalloc(&backup[i], bytes) != hipSuccess) rc = DGPU_E_NOMEM;
+        if (!rc && (hipMemcpyAsync(backup[i], c->d_state, bytes, hipMemcpyDeviceToDevice, c->stream) != hipSuccess ||
+                    hipStreamSynchronize(c->stream) != hipSuccess))
+            rc = DGPU_E_HIP;
+        if (rc) {
+            e->err = "ensemble autotune: state backup";
+            free_all();
+            return rc;
+        }
+    }
+    auto restore = [&]() -> int {
+        for (int i = 0; i < n; ++i) {
+            dgpu_ctx *c = e->m[i];
+            const size_t bytes = sizeof(cx<double>) * c->fs * c->nslot;
+            if (hipMemcpyAsync(c->d_state, backup[i], bytes, hipMemcpyDeviceToDevice, c->stream) != hipSuccess ||
+                hipStreamSynchronize(c->stream) != hipSuccess)
+                return DGPU_E_HIP;
+            c->master_dirty = true;
+            c->work_dirty = false;
+        }
+        return DGPU_OK;
+    };
+    auto set_shape = [&](int xc, int nw) {
+        for (dgpu_ctx *c : e->m) {
+            c->xchunk = xc;
+            c->colnw = nw;
+        }
+    };
+    // wall time per step of the current shapes, from the saved states
+    auto time_shape = [&](double &ms) -> int {
+        if (int rc = restore()) return rc;
+        for (int k = 1; k <= 3; ++k)
+            if (int rc = dgpu_ensemble_rk_chan_1(e, k, h_bar, beta_bar, zeta_bar)) return rc;
+        if (int rc = dgpu_ensemble_synchronize(e)) return rc;
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int s = 0; s < steps; ++s)
+            for (int k = 1; k <= 3; ++k)
+                if (int rc = dgpu_ensemble_rk_chan_1(e, k, h_bar, beta_bar, zeta_bar)) return rc;
+        if (int rc = dgpu_ensemble_synchronize(e)) return rc;
+        ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / steps;
+        return DGPU_OK;
+    };
+    const int xc0 = e->m[0]->xchunk, nw0 = e->m[0]->colnw;
+    int best_x = xc0, best_w = nw0;
+    double best = 1e300;
+    int rc = DGPU_OK;
+    const int chunks[] = {8, 16, 32, 64};
+    const int waves[] = {4, 8, 16};
+    for (int xc : chunks) {
+        for (int nw : waves) {
+            if (nw0 == 0 && nw != waves[0]) continue;  // the sweeps were chosen: x chunks only
+            bool fits = true;
+            for (const dgpu_ctx *c : e->m) fits = fits && column_shape_fits(c, nw);
+            const int w = (nw0 == 0 || !fits) ? nw0 : nw;
+            set_shape(xc, w);
+            double ms = 0.0;
+            if ((rc = time_shape(ms))) break;
+            if (ms < best) {
+                best = ms;
+                best_x = xc;
+                best_w = w;
+            }
+        }
+        if (rc) break;
+    }
+    set_shape(rc == DGPU_OK ? best_x : xc0, rc == DGPU_OK ? best_w : nw0);
+    const int rr = restore();
+    if (rc == DGPU_OK) rc = rr;
+    free_all();
+    if (rc) {
+        if (e->err.empty()) e->err = "ensemble autotune";
+        return rc;
+    }
+    if (xchunk_out) *xchunk_out = best_x;
+    if (waves_out) *waves_out = best_w;
+    if (ms_per_step) *ms_per_step = best;
     return DGPU_OK;
 }
 

@@ -61,7 +61,7 @@ EXPORTS = (
     "dgpu_ensemble_last_error", "dgpu_ensemble_set_graphs", "dgpu_ensemble_rk_chan_1",
     "dgpu_ensemble_synchronize", "dgpu_stats_chan", "dgpu_courant_request", "dgpu_courant",
     "dgpu_get_flow_physical", "dgpu_set_flow_physical", "dgpu_tke_chan", "dgpu_les_chan", "dgpu_set_les", "dgpu_set_zdft", "dgpu_autotune", "dgpu_tke_les",
-    "dgpu_stats_chan_noalloc", "dgpu_set_column_shape", "dgpu_get_column_shape",
+    "dgpu_stats_chan_noalloc", "dgpu_set_column_shape", "dgpu_get_column_shape", "dgpu_ensemble_autotune",
 )
 
 # dgpu_stats_chan profiles, in the order of the header's DGPU_ST_*
@@ -140,6 +140,8 @@ def load_library(path: str = LIB_PATH):
     L.dgpu_set_column_shape.argtypes = [vp, i32, i32]
     L.dgpu_get_column_shape.restype = i32
     L.dgpu_get_column_shape.argtypes = [vp, C.POINTER(i32), C.POINTER(i32)]
+    L.dgpu_ensemble_autotune.restype = i32
+    L.dgpu_ensemble_autotune.argtypes = [vp, i32, DP, DP, DP, C.POINTER(i32), C.POINTER(i32), DP]
     L.dgpu_autotune.restype = i32
     L.dgpu_autotune.argtypes = [vp, i32, C.c_double, C.POINTER(i32), C.POINTER(i32), DP]
     L.dgpu_get_flow_physical.restype = i32
@@ -488,6 +490,16 @@ class GpuEnsemble:
                 self.rk_chan_1(k)
             for m in self.members:
                 m.time += m.cfg.delta_t
+
+    def autotune(self, steps: int = 2) -> dict:
+        """Times the batched launch shapes on the members' states (every
+        member's x-kernel planes per block and column waves), keeps the
+        fastest; the states are left as they were."""
+        xc, w, ms = C.c_int32(0), C.c_int32(0), C.c_double(0.0)
+        self._chk(self.L.dgpu_ensemble_autotune(self.h, steps, _dp(self.h_bar), _dp(self.beta_bar),
+                                                _dp(self.zeta_bar), C.byref(xc), C.byref(w), C.byref(ms)),
+                  "dgpu_ensemble_autotune")
+        return {"xchunk": xc.value, "column_waves": w.value, "ms_per_step": ms.value}
 
     def synchronize(self):
         self._chk(self.L.dgpu_ensemble_synchronize(self.h), "dgpu_ensemble_synchronize")

@@ -356,6 +356,20 @@ int32_t dgpu_ensemble_rk_chan_1(dgpu_ensemble *ens, int32_t rk_step, const doubl
                                 const double *beta_bar, const double *zeta_bar);
 int32_t dgpu_ensemble_synchronize(dgpu_ensemble *ens);
 
+/* Ensemble launch-shape autotuner (no counterpart in the reference): times
+ * `steps` RK3 steps (after one untimed step, wall clock between ensemble
+ * synchronisations) of the members' own states for each candidate -- planes
+ * per block of the batched x kernel (8..64, every member) and the column
+ * shape (dgpu_set_column_shape's waves: 4, 8, 16, every member; the fused
+ * setting of each member stays) -- keeps the fastest and puts every
+ * member's state back (the run does not advance).  h_bar, beta_bar and
+ * zeta_bar as for dgpu_ensemble_rk_chan_1.  Outputs (may be NULL): the
+ * planes per block, the waves, the best ms per step.  Not with a member's
+ * COURANT request pending.  Allocates a copy of every member's state while
+ * it runs. */
+int32_t dgpu_ensemble_autotune(dgpu_ensemble *ens, int32_t steps, const double *h_bar, const double *beta_bar,
+                               const double *zeta_bar, int32_t *xchunk, int32_t *waves, double *ms_per_step);
+
 #ifdef __cplusplus
 }
 #endif

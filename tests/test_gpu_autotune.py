@@ -55,3 +55,46 @@ def test_autotune_mixed_keeps_state(gpu_mod):
         assert np.array_equal(after[k], st[k]), k
     g.step(2)
     assert np.all(np.isfinite(g.state()["CU1"]))
+
+
+def test_ensemble_autotune_keeps_states(gpu_mod):
+    """dgpu_ensemble_autotune leaves every member's state as it was, picks a
+    shape from its candidates, and the ensemble then still advances each
+    member bit for bit as its single run (with that member's shape)."""
+    cfgs = [case.kh_case(24, 17, 24, kick=0.05, dt=0.01, ri=0.1, seed=1),
+            case.kh_case(48, 33, 36, kick=0.05, dt=0.01, ri=0.2, re=500.0, seed=2),
+            case.kh_case(48, 65, 36, kick=0.05, dt=0.01, ri=0.15, seed=3)]
+    for precision in ("fp64", "mixed"):
+        ens = gpu_mod.GpuEnsemble(cfgs, precision=precision)
+        singles = []
+        for cfg, m in zip(cfgs, ens.members):
+            phys = case.initial_physical(cfg)
+            m.initialize(*phys)
+            s = gpu_mod.GpuChannel(cfg, precision=precision)
+            s.initialize(*phys)
+            singles.append(s)
+        ens.step(1)
+        ens.synchronize()
+        before = [m.state() for m in ens.members]
+        r = ens.autotune(1)
+        say(precision, "ensemble autotune", r)
+        assert r["xchunk"] in (8, 16, 32, 64) and r["column_waves"] in (4, 8, 16) and r["ms_per_step"] > 0
+        for b, m in zip(before, ens.members):
+            a = m.state()
+            for k in ("CU1", "CU2", "CU3", "CP", "CTH0", "CF1", "CF2", "CF3", "CFTH0"):
+                assert np.array_equal(a[k], b[k]), (precision, k)
+        for s, m in zip(singles, ens.members):
+            s.step(1)
+            s.set_column_shape(m.column_shape(), -1)
+        ens.step(2)
+        ens.synchronize()
+        for s in singles:
+            s.step(2)
+            s.synchronize()
+        for cfg, s, m in zip(cfgs, singles, ens.members):
+            a, b = m.state(), s.state()
+            for k in ("CU1", "CU2", "CU3", "CP", "CTH0"):
+                assert np.array_equal(a[k], b[k]), (precision, cfg.nx, cfg.ny, k)
+        ens.close()
+        for s in singles:
+            s.close()
