@@ -543,7 +543,13 @@ int32_t dgpu_create(const dgpu_params *p, dgpu_ctx **out)
         const char *cfz = std::getenv("DGPU_COLFUSE");
         if (cfz && (std::atoi(cfz) == 0 || std::atoi(cfz) == 1)) c->colfuse = std::atoi(cfz);
         const char *sw = std::getenv("DGPU_XSWZ");  // 0 turns the row swizzle off
-        c->ipx.swz = (sw && std::atoi(sw) == 0) ? 0u : dgpu::ip_pick_swizzle(c->ipx);
+        // FP32 transforms (mixed contexts) hold 8-byte values in the rows
+        if (sw && std::atoi(sw) == 0) {
+            c->ipx.swz = 0;
+            c->ipx.swzs = 3;
+        } else {
+            dgpu::ip_pick_swizzle(c->ipx, c->mixed ? 2 : 4);
+        }
         // the z kernels' column batches: W = the largest power of two with W*T <= 256
         int zw = 1;
         while (2 * zw * c->ipz.t <= 256) zw *= 2;

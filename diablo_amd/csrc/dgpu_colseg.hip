@@ -34,6 +34,8 @@
 //   velocity (from the scratch), the Poisson solve with the per-mode
 //   factorisation, the projection into CU1..3, the CP update, and the copy
 //   of the new scalar into CTH.
+#include <algorithm>
+
 #include "dgpu_kernels.hpp"
 
 namespace dgpu {
@@ -55,6 +57,89 @@ __device__ __forceinline__ int xcd_block(int L, int nb8) { return (L & 7) * (nb8
 // ---------------------------------------------------------------------------
 // k_cn_seg
 // ---------------------------------------------------------------------------
+// Constants of one mode's CN right-hand side (for/channel.f:88-290, 581-1029)
+template <typename T>
+struct CnMode {
+    T kx, kz, pw, temp3, temp4, temp5, i_ro, rgx, rgz, drhodx, drhodz, force00, nu, nupr;
+    bool is00, rk_gt1;
+    int f_type;
+};
+
+template <typename T>
+__device__ __forceinline__ CnMode<T> cn_mode(const CnArgs &A, bool live, int i, int s)
+{
+    CnMode<T> K;
+    K.is00 = live && i == 0 && s == 0;
+    K.kx = live ? T(A.kx[i]) : T(0);
+    K.kz = live ? T(A.kz[s]) : T(0);
+    const T k2 = live ? T(A.kx2[i]) + T(A.kz2[s]) : T(0);
+    K.pw = (A.beta == 1.0) ? k2 : T(pow(double(k2), A.beta));
+    K.temp3 = T(A.temp3);
+    K.temp4 = T(A.temp4);
+    K.temp5 = T(A.temp5);
+    K.rk_gt1 = A.rk_gt1 != 0;
+    K.f_type = A.f_type;
+    K.i_ro = T(A.i_ro);
+    K.rgx = T(A.rgx);
+    K.rgz = T(A.rgz);
+    K.drhodx = T(A.drhodx);
+    K.drhodz = T(A.drhodz);
+    K.force00 = T(A.force00);
+    K.nu = T(A.nu);
+    K.nupr = T(A.nupr);
+    return K;
+}
+
+// One row of system q: the new explicit term CF (cfn) and the right-hand
+// side of the CN solve (vec).  tb(k) reads the row's table entry L_k; f0, fm,
+// fp: the system's field at j, j-1, j+1 (fp zero above NY); aux1, aux2: the
+// partner fields at j (Coriolis / TH advection, buoyancy / TH advection);
+// thm: CTH at j-1 (U2's buoyancy); cp0, cpm: CP at j, j-1; nlv, cfo: the
+// nonlinear term and the old CF (zero on BC rows outside the CF range).
+template <typename T, int NTH, typename TB>
+__device__ __forceinline__ void cn_rhs(int q, const CnMode<T> &K, TB &&tb, cx<T> f0, cx<T> fm, cx<T> fp, cx<T> aux1,
+                                       cx<T> aux2, cx<T> thm, cx<T> cp0, cx<T> cpm, cx<T> nlv, cx<T> cfo,
+                                       cx<T> &cfn, cx<T> &vec)
+{
+    const cx<T> zero = make_cx<T>(0, 0);
+    const T visc = (q == SYS_TH0) ? K.nupr * K.pw : K.nu * K.pw;
+    cfn = make_cx<T>(-(visc * f0.re), -(visc * f0.im));
+    if (q == SYS_U1) {
+        cfn = cfma(K.i_ro, aux1, cfn);
+        if (NTH) cfn = cfma(K.rgx, aux2, cfn);
+    } else if (q == SYS_U3) {
+        cfn = cfma(-K.i_ro, aux1, cfn);
+        if (NTH) cfn = cfma(K.rgz, aux2, cfn);
+    } else if (q == SYS_U2) {
+        if (NTH && !K.is00) {
+            const T b0 = tb(L_B0), bm = tb(L_BM);
+            cfn = cfn + make_cx<T>(b0 * aux2.re + bm * thm.re, b0 * aux2.im + bm * thm.im);
+        }
+    } else {
+        cfn = cfn - make_cx<T>(aux1.re * K.drhodx + aux2.re * K.drhodz, aux1.im * K.drhodx + aux2.im * K.drhodz);
+    }
+    cfn = cfn + nlv;
+    if (tb(L_ISBC) != T(0)) {
+        vec = K.is00 ? make_cx<T>(tb(L_BC00), 0) : zero;
+    } else {
+        cx<T> cr = f0;
+        if (K.rk_gt1) cr = cfma(K.temp3, cfo, cr);
+        if (q == SYS_U1) {
+            cr = cr - make_cx<T>(-K.temp4 * (K.kx * cp0.im), K.temp4 * (K.kx * cp0.re));
+            if (K.is00 && (K.f_type == 1 || K.f_type == 2)) cr.re = cr.re - K.temp4 * K.force00;
+        } else if (q == SYS_U3) {
+            cr = cr - make_cx<T>(-K.temp4 * (K.kz * cp0.im), K.temp4 * (K.kz * cp0.re));
+        } else if (q == SYS_U2) {
+            const T w = tb(L_CPW);
+            cr = cr - make_cx<T>(w * (cp0.re - cpm.re), w * (cp0.im - cpm.im));
+        }
+        cr = cfma(K.temp5, cfn, cr);
+        const T wp = tb(L_WP), w0 = tb(L_W0), wm = tb(L_WM);
+        vec = make_cx<T>(cr.re + (wp * fp.re + w0 * f0.re + wm * fm.re),
+                         cr.im + (wp * fp.im + w0 * f0.im + wm * fm.im));
+    }
+}
+
 // The CN solve of system q for the block's modes: sol[r] = the new value of
 // row a + r of the lane's segment (the rows outside the solve's copy range
 // keep the old value).  The LDS holds the system's table and the carries
@@ -93,17 +178,9 @@ __device__ __forceinline__ void cn_seg_solve(AT A, PT P, int bx, int q, cx<T> (&
     int s = 0, i = 0;
     const long gm = (live && A.map) ? (long)A.map[m] : m;
     if (live) { s = (int)(gm / g.mk); i = (int)(gm - (long)s * g.mk); }
-    const bool is00 = live && i == 0 && s == 0;
     const long nps = A.nl_ps;
-    const T kx = live ? T(A.kx[i]) : T(0), kz = live ? T(A.kz[s]) : T(0);
-    const T k2 = live ? T(A.kx2[i]) + T(A.kz2[s]) : T(0);
-    const T pw = (A.beta == 1.0) ? k2 : T(pow(double(k2), A.beta));
-    const T temp3 = T(A.temp3), temp4 = T(A.temp4), temp5 = T(A.temp5);
-    const bool rk_gt1 = A.rk_gt1 != 0;
-    const int f_type = A.f_type;
-    const T i_ro = T(A.i_ro), rgx = T(A.rgx), rgz = T(A.rgz), drhodx = T(A.drhodx), drhodz = T(A.drhodz);
-    const T force00 = T(A.force00);
-    const T visc = (q == SYS_TH0) ? T(A.nupr) * pw : T(A.nu) * pw;
+    const CnMode<T> K = cn_mode<T>(A, live, i, s);
+    const bool rk_gt1 = K.rk_gt1;
     cx<T> *fld = (q == SYS_U1) ? P.cu1 : (q == SYS_U3) ? P.cu3 : (q == SYS_U2) ? P.cu2 : P.cth;
     cx<T> *cf = (q == SYS_U1) ? P.cf1 : (q == SYS_U3) ? P.cf3 : (q == SYS_U2) ? P.cf2 : P.cfth;
     const cx<NLT> *nl = P.nl[q];
@@ -153,42 +230,10 @@ __device__ __forceinline__ void cn_seg_solve(AT A, PT P, int bx, int q, cx<T> (&
                 const bool use = cfr || !isbc;
                 const cx<T> nlv = use ? cur.nl : zero;
                 const cx<T> cfo = use ? cur.cf : zero;
-                const cx<T> aux1 = cur.aux1, aux2 = cur.aux2, cp0 = cur.cp;
-                // new explicit term CF (for/channel.f:175-290 + nonlinear terms)
-                cx<T> cfn = make_cx<T>(-(visc * f0.re), -(visc * f0.im));
-                if (q == SYS_U1) {
-                    cfn = cfma(i_ro, aux1, cfn);
-                    if (NTH) cfn = cfma(rgx, aux2, cfn);
-                } else if (q == SYS_U3) {
-                    cfn = cfma(-i_ro, aux1, cfn);
-                    if (NTH) cfn = cfma(rgz, aux2, cfn);
-                } else if (q == SYS_U2) {
-                    if (NTH && !is00)
-                        cfn = cfn + make_cx<T>(t[L_B0] * aux2.re + t[L_BM] * thm.re, t[L_B0] * aux2.im + t[L_BM] * thm.im);
-                } else {
-                    cfn = cfn - make_cx<T>(aux1.re * drhodx + aux2.re * drhodz, aux1.im * drhodx + aux2.im * drhodz);
-                }
-                cfn = cfn + nlv;
-                cx<T> vec;
-                if (isbc) {
-                    vec = is00 ? make_cx<T>(t[L_BC00], 0) : zero;
-                } else {
-                    cx<T> cr = f0;
-                    if (rk_gt1) cr = cfma(temp3, cfo, cr);
-                    if (q == SYS_U1) {
-                        cr = cr - make_cx<T>(-temp4 * (kx * cp0.im), temp4 * (kx * cp0.re));
-                        if (is00 && (f_type == 1 || f_type == 2)) cr.re = cr.re - temp4 * force00;
-                    } else if (q == SYS_U3) {
-                        cr = cr - make_cx<T>(-temp4 * (kz * cp0.im), temp4 * (kz * cp0.re));
-                    } else if (q == SYS_U2) {
-                        const T w = t[L_CPW];
-                        cr = cr - make_cx<T>(w * (cp0.re - cpm.re), w * (cp0.im - cpm.im));
-                    }
-                    cr = cfma(temp5, cfn, cr);
-                    const T wp = t[L_WP], w0 = t[L_W0], wm = t[L_WM];
-                    vec = make_cx<T>(cr.re + (wp * fp.re + w0 * f0.re + wm * fm.re),
-                                     cr.im + (wp * fp.im + w0 * f0.im + wm * fm.im));
-                }
+                const cx<T> aux2 = cur.aux2, cp0 = cur.cp;
+                cx<T> cfn, vec;
+                cn_rhs<T, NTH>(q, K, [&](int k) { return t[k]; }, f0, fm, fp, cur.aux1, aux2, thm, cp0, cpm, nlv, cfo,
+                               cfn, vec);
                 gl = (r == 0) ? vec : cfma(mult, gl, vec);
                 cprod = (r == 0) ? mult : cprod * mult;
                 yv[r] = gl;
@@ -497,11 +542,200 @@ k_remdiv_seg_batch(const ColDesc<T, NLT> *__restrict__ d)
     remdiv_seg_body<T, NLT, MPW, NW, RMAX, NTH>(D.R, D.P, blockIdx.x);
 }
 
+// The CN solves of U1, U3 and U2 together, row by row (the fused column
+// kernel's first pass): each row's CU1..3, CTH, CP, NL and CF are loaded
+// once for the three systems (the partner fields of the explicit terms are
+// the other systems' own rows), the three tables sit in LDS, and the three
+// recurrences run side by side.  Carries through
+// LDS as in cn_seg_solve, three systems per barrier.  u1s, u3s, u2s: the new
+// values of the lane's rows (the rows outside a solve's copy range keep the
+// old value).
+template <typename T, int NTH, typename NLT, int MPW, int NW, int RMAX>
+__device__ __forceinline__ void cn3_solve(const CnArgs &A, const CnPtrs<T, NLT> &P, int bx, cx<T> (&u1s)[RMAX],
+                                          cx<T> (&u3s)[RMAX], cx<T> (&u2s)[RMAX])
+{
+    constexpr int SPW = 64 / MPW;
+    constexpr int NSEG = NW * SPW;
+    constexpr int NM = NSEG * MPW;
+    extern __shared__ unsigned char smem_raw[];
+    const Geo &g = A.g;
+    const int NY = g.ny, N = NY + 2;
+    const long ps = g.ps;
+    const int R = (N + NSEG - 1) / NSEG;
+    const int nseg = (N + R - 1) / R;
+    cx<T> *GL3 = reinterpret_cast<cx<T> *>(smem_raw);  // [3][NSEG][MPW]
+    cx<T> *XL3 = GL3 + 3 * NM;                          // [3][NSEG][MPW]
+    T *Cb3 = reinterpret_cast<T *>(XL3 + 3 * NM);       // [3][NSEG]
+    T *Ea3 = Cb3 + 3 * NSEG;                            // [3][NSEG]
+    T *tl3 = Ea3 + 3 * NSEG;                            // [3][N][kLRow]
+    for (int e = threadIdx.x; e < 3 * N * kLRow; e += blockDim.x) {
+        const int qj = e / kLRow, k = e - qj * kLRow;  // qj = q * N + j
+        tl3[e] = T(A.tab[(long)qj * kRow + kTabMap[k]]);
+    }
+    __syncthreads();
+
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int seg = (tid >> 6) * SPW + lane / MPW, ml = lane % MPW;
+    const long m = (long)bx * MPW + ml;
+    const bool live = m < ps;
+    const int a = seg * R, b = min(a + R, N) - 1;
+    const bool rows = live && a <= b;
+    int s = 0, i = 0;
+    const long gm = (live && A.map) ? (long)A.map[m] : m;
+    if (live) { s = (int)(gm / g.mk); i = (int)(gm - (long)s * g.mk); }
+    const long nps = A.nl_ps;
+    const CnMode<T> K = cn_mode<T>(A, live, i, s);
+    const cx<T> zero = make_cx<T>(0, 0);
+    // systems 0, 1, 2 = SYS_U1, SYS_U3, SYS_U2
+    cx<T> *const fldp[3] = {P.cu1, P.cu3, P.cu2};
+    cx<T> *const cfp[3] = {P.cf1, P.cf3, P.cf2};
+    auto tab = [&](int q, int j, int k) -> T { return tl3[(q * N + j) * kLRow + k]; };
+    auto put = [&](int q, int r, cx<T> v) {
+        if (q == 0) u1s[r] = v;
+        else if (q == 1) u3s[r] = v;
+        else u2s[r] = v;
+    };
+    auto get = [&](int q, int r) -> cx<T> { return q == 0 ? u1s[r] : q == 1 ? u3s[r] : u2s[r]; };
+
+    // ---- forward, local ----
+    if (rows) {
+        struct In3 {
+            cx<T> fp[3], nl[3], cf[3], th, cp;
+        };
+        auto load = [&](int j) {
+            const long o = m + (long)j * ps;
+            In3 in;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                in.fp[q] = fldp[q][o + (j <= NY ? ps : 0)];
+                const cx<NLT> nv = P.nl[q][gm + (long)j * nps];
+                in.nl[q] = make_cx<T>(T(nv.re), T(nv.im));
+                in.cf[q] = K.rk_gt1 ? cfp[q][o] : zero;
+            }
+            in.th = NTH ? P.cth[o] : zero;
+            in.cp = P.cp[o];
+            return in;
+        };
+        const long oa = m + (long)a * ps;
+        cx<T> f0[3], fm[3], gl[3];
+        T cprod[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            fm[q] = a >= 1 ? fldp[q][oa - ps] : zero;
+            f0[q] = fldp[q][oa];
+            gl[q] = zero;
+            cprod[q] = T(1);
+        }
+        cx<T> thm = (a >= 1 && NTH) ? P.cth[oa - ps] : zero;
+        cx<T> cpm = a >= 1 ? P.cp[oa - ps] : zero;
+#pragma unroll
+        for (int r = 0; r < RMAX; ++r) {
+            const int j = a + r;
+            if (j <= b) {
+                const In3 cur = load(j);
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    auto tb = [&](int k) { return tab(q, j, k); };
+                    const T mult = tb(L_MULT);
+                    const bool isbc = tb(L_ISBC) != T(0);
+                    const bool cfr = tb(L_CFR) != T(0);
+                    const cx<T> fp = (j <= NY) ? cur.fp[q] : zero;
+                    const bool use = cfr || !isbc;
+                    const cx<T> nlv = use ? cur.nl[q] : zero;
+                    const cx<T> cfo = use ? cur.cf[q] : zero;
+                    // U1's partner is CU3, U3's CU1 (their own rows here)
+                    const cx<T> aux1 = q == 0 ? f0[1] : q == 1 ? f0[0] : zero;
+                    cx<T> cfn, vec;
+                    cn_rhs<T, NTH>(q, K, tb, f0[q], fm[q], fp, aux1, cur.th, thm, cur.cp, cpm, nlv, cfo, cfn, vec);
+                    gl[q] = (r == 0) ? vec : cfma(mult, gl[q], vec);
+                    cprod[q] = (r == 0) ? mult : cprod[q] * mult;
+                    put(q, r, gl[q]);
+                    if (cfr) cfp[q][m + (long)j * ps] = cfn;
+                }
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    fm[q] = f0[q];
+                    f0[q] = (j <= NY) ? cur.fp[q] : zero;
+                }
+                thm = cur.th;
+                cpm = cur.cp;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            GL3[q * NM + seg * MPW + ml] = gl[q];
+            if (ml == 0) Cb3[q * NSEG + seg] = cprod[q];
+        }
+    }
+    __syncthreads();
+    // ---- carries of the forward recurrences, back substitution, local ----
+    if (rows) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            cx<T> gin = zero;
+            for (int t = 0; t < seg; ++t) gin = cfma(Cb3[q * NSEG + t], gin, GL3[q * NM + t * MPW + ml]);
+            T c = T(1);
+#pragma unroll
+            for (int r = 0; r < RMAX; ++r) {
+                const int j = a + r;
+                if (j <= b) {
+                    c *= tab(q, j, L_MULT);
+                    put(q, r, cfma(c, gin, get(q, r)));
+                }
+            }
+            cx<T> xl = zero;
+            T e = T(1);
+            bool first = true;
+#pragma unroll
+            for (int r = RMAX - 1; r >= 0; --r) {
+                const int j = a + r;
+                if (j <= b) {
+                    const T ip = tab(q, j, L_INVPIV), u = tab(q, j, L_UP);
+                    const cx<T> y = get(q, r);
+                    if (first) {
+                        xl = make_cx<T>(y.re * ip, y.im * ip);
+                        e = -u * ip;
+                        first = false;
+                    } else {
+                        xl = make_cx<T>((y.re - u * xl.re) * ip, (y.im - u * xl.im) * ip);
+                        e = (-u * ip) * e;
+                    }
+                    put(q, r, xl);
+                }
+            }
+            XL3[q * NM + seg * MPW + ml] = xl;
+            if (ml == 0) Ea3[q * NSEG + seg] = e;
+        }
+    }
+    __syncthreads();
+    // ---- carries of the backward recurrences, the solution of every row ----
+    if (rows) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            cx<T> xin = zero;
+            for (int t = nseg - 1; t > seg; --t) xin = cfma(Ea3[q * NSEG + t], xin, XL3[q * NM + t * MPW + ml]);
+            T e = T(1);
+            bool first = true;
+#pragma unroll
+            for (int r = RMAX - 1; r >= 0; --r) {
+                const int j = a + r;
+                if (j <= b) {
+                    const T f = -tab(q, j, L_UP) * tab(q, j, L_INVPIV);
+                    e = first ? f : f * e;
+                    first = false;
+                    put(q, r, (tab(q, j, L_CPY) != T(0)) ? cfma(e, xin, get(q, r)) : fldp[q][m + (long)j * ps]);
+                }
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // k_colfused: the whole column stage of a substep for a group of modes with
-// the FP32 state.  The CN solves of U1, U3, U2 stay in registers; TH's goes
-// straight into CTH (every system of these modes has read the old CTH by
-// then, and no other block reads these modes); then the divergence of U*,
+// the FP32 state.  The CN solves of U1, U3, U2 (cn3_solve: one pass over the
+// rows for the three) stay in registers; TH's goes straight into CTH (every
+// system of these modes has read the old CTH by then, and no other block
+// reads these modes); then the divergence of U*,
 // the Poisson solve, the projection into CU1..3 and the CP update.  The
 // state is read once and written once, with no solve scratch.  The Poisson
 // rows 1..NY run on the lanes that hold them in the CN part (32 segments of
@@ -533,15 +767,11 @@ __device__ __forceinline__ void colfused_body(const CnArgs &A, const RdArgs &Rd,
     const bool rows = live && a <= b;
     const cx<T> zero = make_cx<T>(0, 0);
 
-    // ---- the CN solves ----
+    // ---- the CN solves: U1, U3, U2 together, then TH ----
     cx<T> u1s[RMAX], u3s[RMAX], u2s[RMAX];
-    cn_seg_solve<T, NTH, NLT, MPW, NW, RMAX, const CnArgs &, const CnPtrs<T, NLT> &>(A, P, bx, SYS_U1, u1s);
-    __syncthreads();
-    cn_seg_solve<T, NTH, NLT, MPW, NW, RMAX, const CnArgs &, const CnPtrs<T, NLT> &>(A, P, bx, SYS_U3, u3s);
-    __syncthreads();
-    cn_seg_solve<T, NTH, NLT, MPW, NW, RMAX, const CnArgs &, const CnPtrs<T, NLT> &>(A, P, bx, SYS_U2, u2s);
+    cn3_solve<T, NTH, NLT, MPW, NW, RMAX>(A, P, bx, u1s, u3s, u2s);
     if (NTH) {
-        __syncthreads();
+        __syncthreads();  // the TH solve's table goes where the three systems' carries were
         cx<T> ts[RMAX];
         cn_seg_solve<T, NTH, NLT, MPW, NW, RMAX, const CnArgs &, const CnPtrs<T, NLT> &>(A, P, bx, SYS_TH0, ts);
         if (rows) {
@@ -840,14 +1070,17 @@ static size_t colfused_lds(int ny, int nw)
 {
     const int mpw = 64 * nw / kFusedSeg, n = ny + 2;
     using T = float;
-    return sizeof(T) * (size_t)((n * kLRow + 1) & ~1) + 2 * sizeof(cx<T>) * kFusedSeg * mpw +
-           2 * sizeof(T) * kFusedSeg + 2 * sizeof(T) * kFusedSeg * mpw + sizeof(cx<T>) * kFusedSeg * mpw;
+    const size_t one = sizeof(T) * (size_t)((n * kLRow + 1) & ~1) + 2 * sizeof(cx<T>) * kFusedSeg * mpw +
+                       2 * sizeof(T) * kFusedSeg + 2 * sizeof(T) * kFusedSeg * mpw + sizeof(cx<T>) * kFusedSeg * mpw;
+    const size_t three = 6 * sizeof(cx<T>) * kFusedSeg * mpw + 6 * sizeof(T) * kFusedSeg +
+                         3 * sizeof(T) * (size_t)n * kLRow;  // cn3_solve: carries, tables
+    return std::max(one, three);
 }
 
 bool colfused_supported(int ny, int nw)
 {
     return (nw == 4 || nw == 8 || nw == 16) && ny >= 4 && (ny + 2 + kFusedSeg - 1) / kFusedSeg <= 9 &&
-           colfused_lds(ny, nw) <= 64 * 1024;
+           colfused_lds(ny, nw) <= 150 * 1024;
 }
 
 template <typename T, typename NLT>
@@ -857,6 +1090,14 @@ void launch_colfused(hipStream_t s, int nw, const CnArgs &a, const RdArgs &r, co
     with_nw(nw, [&](auto wc) {
         constexpr int NW = decltype(wc)::value, MPW = 64 * NW / kFusedSeg;
         const long blocks = ((a.g.ps + MPW - 1) / MPW + 7) / 8 * 8;  // a multiple of 8 (xcd_block)
+        static bool attr = false;
+        if (!attr) {
+            (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_colfused<T, NLT, MPW, NW, 9, 0>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+            (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_colfused<T, NLT, MPW, NW, 9, 1>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+            attr = true;
+        }
         if (a.g.nth == 0)
             hipLaunchKernelGGL((k_colfused<T, NLT, MPW, NW, 9, 0>), dim3(blocks), dim3(64 * NW), lds, s, a, r, p);
         else
@@ -871,6 +1112,14 @@ void launch_colfused_batch(hipStream_t s, int nw, const Geo &g, int max_ny, cons
     with_nw(nw, [&](auto wc) {
         constexpr int NW = decltype(wc)::value, MPW = 64 * NW / kFusedSeg;
         const dim3 grid((g.ps + MPW - 1) / MPW, 1, nmem);
+        static bool attr = false;
+        if (!attr) {
+            (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_colfused_batch<T, NLT, MPW, NW, 9, 0>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+            (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_colfused_batch<T, NLT, MPW, NW, 9, 1>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+            attr = true;
+        }
         if (g.nth == 0)
             hipLaunchKernelGGL((k_colfused_batch<T, NLT, MPW, NW, 9, 0>), grid, dim3(64 * NW), lds, s, d);
         else

@@ -181,8 +181,9 @@ struct IpLen {
     int twn;                    // length of the whole twiddle table
     int e;                      // values per thread (multiple of every R_i)
     int t;                      // threads per transform = n / e
-    unsigned swz;               // LDS swizzle: 4-bit XOR mask of bits 0..3, per value of bits 3..5
+    unsigned swz;               // LDS swizzle: 4-bit XOR mask of bits 0..3, per value of bits swzs..swzs+2
     int cswz;                   // column-batch swizzle: bit 0 of a position XORed with bit cswz (0: none)
+    int swzs;                   // the swizzle's selector shift: 3 for 16-byte slots, 3 or 4 for 8-byte ones
 };
 
 // Slot of position pos in a column batch (lds[slot * W + column]): swapping
@@ -194,11 +195,12 @@ DGPU_HD int ip_cswz(const IpLen &f, int pos)
 }
 
 // LDS slot of position pos of a row: the low four bits are XORed with a mask
-// chosen by bits 3..5, which spreads the strided accesses of the late stages
-// over the banks (the host checks that the map is a bijection of 0..n-1).
+// chosen by the three bits from swzs up, which spreads the strided accesses
+// of the late stages over the banks (the host checks that the map is a
+// bijection of 0..n-1).
 DGPU_HD int ip_swz(const IpLen &f, int pos)
 {
-    return pos ^ (int)((f.swz >> (((pos >> 3) & 7) << 2)) & 15u);
+    return pos ^ (int)((f.swz >> (((pos >> f.swzs) & 7) << 2)) & 15u);
 }
 
 // Twiddle + butterfly of one group (values v[0..R-1] at positions
@@ -292,6 +294,7 @@ struct ConstPlanSpec {
     int n, e, r0;  // r0 > 0: outermost radix fixed (x plans), 0: greedy (z plans)
     unsigned swz;
     int cswz;
+    int swzs = 3;
 };
 constexpr ConstPlanSpec kConstPlanSpecs[] = {
     {0, 0, 0, 0u, 0},
@@ -308,8 +311,11 @@ constexpr ConstPlanSpec kConstPlanSpecs[] = {
     {96, 12, 0, 0u, 0},            // 9: (6,4,4)
     {128, 8, 0, 0u, 0},            // 10: (8,8,2)
     {192, 24, 0, 0u, 0},           // 11: (8,8,3), FP32
-    // x plan appended after the z plans (their ids stay)
+    // x plans appended after the z plans (their ids stay)
     {256, 8, 4, 0x76543210u, 0},   // 12: (4,8,8)
+    // FP32 x plans with the swizzle picked for 8-byte slots
+    {256, 8, 4, 0x7654ba98u, 0, 4},  // 13: (4,8,8), FP32
+    {128, 8, 4, 0xfda87520u, 0, 4},  // 14: (4,8,4), FP32
 };
 constexpr int kNumConstPlans = (int)(sizeof(kConstPlanSpecs) / sizeof(kConstPlanSpecs[0]));
 
@@ -350,6 +356,7 @@ constexpr IpLen const_plan(int id)
     f.twn = off;
     f.swz = c.swz;
     f.cswz = c.cswz;
+    f.swzs = c.swzs;
     return f;
 }
 

@@ -35,10 +35,10 @@ struct XCount {
 }  // namespace
 
 // The plans the kernels are also built for at compile time (PID > 0).
-__device__ constexpr IpLen kDevConstPlans[] = {const_plan(0), const_plan(1), const_plan(2),  const_plan(3),
-                                               const_plan(4), const_plan(5), const_plan(6),  const_plan(7),
-                                               const_plan(8), const_plan(9), const_plan(10), const_plan(11),
-                                               const_plan(12)};
+__device__ constexpr IpLen kDevConstPlans[] = {const_plan(0),  const_plan(1),  const_plan(2),  const_plan(3),
+                                               const_plan(4),  const_plan(5),  const_plan(6),  const_plan(7),
+                                               const_plan(8),  const_plan(9),  const_plan(10), const_plan(11),
+                                               const_plan(12), const_plan(13), const_plan(14)};
 static_assert(sizeof(kDevConstPlans) / sizeof(IpLen) == (size_t)kNumConstPlans, "one entry per compile-time plan");
 
 template <int PID>
@@ -74,9 +74,10 @@ static inline int z_lw(int t)
 template <int... P>
 struct PidList {};
 template <typename T>
-using XPids = std::conditional_t<sizeof(T) == 4, PidList<1, 2, 3, 4, 5, 12>, PidList<1, 12>>;
+using XPids = std::conditional_t<sizeof(T) == 4, PidList<1, 2, 3, 4, 5, 12, 13, 14>, PidList<1, 12>>;
 // the single-context x kernel: the 384 and 256 plans
-using XPidsSingle = PidList<1, 12>;
+template <typename T>
+using XPidsSingle = std::conditional_t<sizeof(T) == 4, PidList<1, 12, 13>, PidList<1, 12>>;
 template <typename T>
 using ZPids = std::conditional_t<sizeof(T) == 4, PidList<7, 8, 9, 10, 11>, PidList<6>>;
 

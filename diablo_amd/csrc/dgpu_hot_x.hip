@@ -438,7 +438,7 @@ void launch_xfused_ip(hipStream_t s, const Geo &g, const IpLen &fx, const cx<T> 
                       long zt_fs, cx<T> *wt, long wt_fs, const double *dy, const double *dyf, const double *kx,
                       int jlo, int jhi, int chunk)
 {
-    if (g.nth == 1 && with_pid(const_plan_id(fx), XPidsSingle{}, [&](auto pc) {
+    if (g.nth == 1 && with_pid(const_plan_id(fx), XPidsSingle<T>{}, [&](auto pc) {
             constexpr int P = decltype(pc)::value;
             xfused_ip_e<T, kConstPlanSpecs[P].e, 1, P>(s, g, fx, tw, ld, zt, zt_fs, wt, wt_fs, dy, dyf, kx, jlo, jhi,
                                                        chunk);

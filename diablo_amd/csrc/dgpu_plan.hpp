@@ -41,6 +41,7 @@ inline bool ip_fill(int n, int e, const std::vector<int> &radix, IpLen &f)
 {
     if ((int)radix.size() > kMaxStages || radix.empty() || n % e) return false;
     f = IpLen{};
+    f.swzs = 3;
     f.n = n;
     f.nstages = (int)radix.size();
     f.e = e;
@@ -79,34 +80,35 @@ inline void ip_twiddles(const IpLen &f, std::vector<cx<double>> &tw)
 }
 
 // LDS cost of a plan's stage accesses for one row layout (positions of a row
-// at x[swz(pos)], rows of n 16-byte values back to back, 64 lanes a
-// wavefront, T lanes a row): the sum over the two bank models that bound
-// what the hardware does with 16-byte accesses (32 four-byte banks serving 8
-// lanes a cycle, 64 banks serving 16), relative to conflict-free.
-inline double ip_bank_cost(const IpLen &f, unsigned swz)
+// at x[swz(pos)], rows of n values of wpe 4-byte words back to back: 4 for
+// complex FP64, 2 for complex FP32; 64 lanes a wavefront, T lanes a row):
+// the sum over the two bank models that bound what the hardware does (32 or
+// 64 four-byte banks, as many lanes a cycle as fill them), relative to
+// conflict-free.
+inline double ip_bank_cost(const IpLen &f, unsigned swz, int swzs = 3, int wpe = 4)
 {
     IpLen g = f;
     g.swz = swz;
+    g.swzs = swzs;
     const int T = f.t;
     double tot = 0, ideal = 0;
-    const int models[2][2] = {{32, 8}, {64, 16}};
+    const int bank_models[2] = {32, 64};
     for (int i = 0; i < f.nstages; ++i) {
         const int R = f.radix[i], L = f.span[i];
         for (int u = 0; u < f.e / R; ++u)
             for (int q = 0; q < R; ++q)
-                for (const auto &md : models) {
-                    const int banks = md[0], lpc = md[1];
+                for (const int banks : bank_models) {
+                    const int lpc = banks / wpe;
                     for (int g0 = 0; g0 < 64; g0 += lpc) {
                         int cnt[64] = {0};
                         int mx = 0;
                         for (int lane = g0; lane < g0 + lpc; ++lane) {
                             const int row = lane / T, th = lane % T;
                             const int t = th + T * u, b = t / L, k = t - b * L;
-                            const long a = ((long)row * f.n + ip_swz(g, b * R * L + k + q * L)) * 4;  // 4-byte words
-                            for (int w = 0; w < 4; ++w) mx = std::max(mx, ++cnt[(a + w) % banks]);
+                            const long a = ((long)row * f.n + ip_swz(g, b * R * L + k + q * L)) * wpe;
+                            for (int w = 0; w < wpe; ++w) mx = std::max(mx, ++cnt[(a + w) % banks]);
                         }
-                        const int per = lpc * 4 / banks;
-                        tot += (double)std::max(1, (mx + per - 1) / per);
+                        tot += (double)mx;
                         ideal += 1.0;
                     }
                 }
@@ -114,9 +116,10 @@ inline double ip_bank_cost(const IpLen &f, unsigned swz)
     return ideal > 0 ? tot / ideal : 1.0;
 }
 
-// Picks the cheapest bijective swizzle for the rows of an x-direction plan
-// from a few masks (identity first, so it wins ties).
-inline unsigned ip_pick_swizzle(const IpLen &f)
+// Picks the cheapest bijective swizzle (mask table and selector shift) for
+// the rows of an x-direction plan whose values are wpe 4-byte words, from a
+// few candidates (identity first, so it wins ties), into f.swz / f.swzs.
+inline void ip_pick_swizzle(IpLen &f, int wpe = 4)
 {
     auto table = [](int kind) {
         unsigned w = 0;
@@ -124,18 +127,28 @@ inline unsigned ip_pick_swizzle(const IpLen &f)
             const int b3 = idx & 1, b4 = (idx >> 1) & 1, b5 = (idx >> 2) & 1;
             unsigned m = 0;
             if (kind == 1) m = b4 | (b3 << 1) | (b4 << 2) | (b5 << 3);  // per-stage radix-4 spread
-            if (kind == 2) m = (unsigned)idx;                            // bits 3..5 into bits 0..2
+            if (kind == 2) m = (unsigned)idx;                            // selector bits into bits 0..2
+            if (kind == 3) m = (unsigned)(idx < 4 ? idx | 8 : idx);      // as 2, bit 3 set for the lower half
             w |= m << (4 * idx);
         }
         return w;
     };
-    unsigned best = 0;
-    double best_c = ip_bank_cost(f, 0);
-    if (f.n % 16) return 0;
-    for (int kind = 1; kind <= 2; ++kind) {
-        const unsigned w = table(kind);
+    f.swz = 0;
+    f.swzs = 3;
+    if (f.n % 16) return;
+    double best_c = ip_bank_cost(f, 0, 3, wpe);
+    struct Cand {
+        unsigned w;
+        int sh;
+    };
+    // 16-byte slots: selector bits 3..5; 8-byte slots also try bits 4..6
+    const Cand cands[] = {{table(1), 3}, {table(2), 3}, {table(1), 4}, {table(2), 4}, {table(3), 4}};
+    const int ncand = wpe == 4 ? 2 : 5;
+    for (int k = 0; k < ncand; ++k) {
+        const Cand cd = cands[k];
         IpLen g = f;
-        g.swz = w;
+        g.swz = cd.w;
+        g.swzs = cd.sh;
         std::vector<char> seen(f.n, 0);
         bool ok = true;
         for (int p = 0; p < f.n && ok; ++p) {
@@ -144,13 +157,13 @@ inline unsigned ip_pick_swizzle(const IpLen &f)
             else seen[q] = 1;
         }
         if (!ok) continue;
-        const double c = ip_bank_cost(f, w);
+        const double c = ip_bank_cost(f, cd.w, cd.sh, wpe);
         if (c < best_c - 1e-9) {
             best_c = c;
-            best = w;
+            f.swz = cd.w;
+            f.swzs = cd.sh;
         }
     }
-    return best;
 }
 
 // Same model for a column batch (lds[slot * W + b], lanes b = lane % W
