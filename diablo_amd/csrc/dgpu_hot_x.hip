@@ -6,7 +6,9 @@
 namespace dgpu {
 
 // -------------------------------------------------------------------------
-// xfused: one z row per block, planes ja..jb (a chunk of the sweep).
+// xfused: zr z rows per block (one, unless the rows are short: then several
+// rows fill the block, each with its own window and product rows), planes
+// ja..jb (a chunk of the sweep).
 //
 // Rows of the transform passes (T = fx.t threads each): rows 0..npi-1 run the
 // x backward of the next plane (pairs (u1,u2), (u3,th)), rows npi..npi+npo-1
@@ -22,30 +24,39 @@ namespace dgpu {
 // (ip_swz) of their rows; the products read and write slot by slot, which is
 // position by position in the same permutation for every row.
 // -------------------------------------------------------------------------
-template <typename T, int E, int NTH, bool CP>
+template <typename T, int E, int NTH, bool CP, bool MZ>
 __device__ __forceinline__ void xfused_body(const IpLen &fx, const cx<T> *__restrict__ twg, int mk, int ld,
                                             const cx<T> *__restrict__ zt, long zt_fs, long zt_ps,
                                             cx<T> *__restrict__ wt, long wt_fs, long wt_ps,
                                             const double *__restrict__ dy, const double *__restrict__ dyf,
                                             const double *__restrict__ kxv, int jlo, int jhi, int chunk, int bx,
-                                            int by)
+                                            int by, int zr_rt, int nz)
 {
+    // MZ: several z rows per block (zr_rt); the single-run kernels of the
+    // long-row plans keep one, known at compile time
+    const int zr = MZ ? zr_rt : 1;
     using C = XCount<NTH>;
     constexpr int npi = C::npi, npo = C::npo;
+    constexpr int nrow = npi + npo;  // transform rows per z row
     constexpr int R0 = E / 2;
     extern __shared__ unsigned char smem_raw[];
     const int nx = fx.n;
-    cx<T> *win = reinterpret_cast<cx<T> *>(smem_raw);
-    cx<T> *fb = win + 3 * npi * nx;
-    cx<T> *tw = fb + npo * nx;
-    T *kxs = reinterpret_cast<T *>(tw + fx.twn);  // KX of the kept modes, read by every plane's output
+    const int zsz = (3 * npi + npo) * nx;  // a z row's window and product rows
+    const int TT = fx.t, S = fx.nstages, L0 = fx.span[0];
     const int tid = threadIdx.x, nthr = blockDim.x;
+    const int pg = tid / TT, zs = MZ ? pg / nrow : 0;
+    const int p = pg - zs * nrow;
+    int th = tid - pg * TT;  // re-laundered every plane (see the loop)
+    cx<T> *win = reinterpret_cast<cx<T> *>(smem_raw) + zs * zsz;
+    cx<T> *fb = win + 3 * npi * nx;
+    cx<T> *tw = reinterpret_cast<cx<T> *>(smem_raw) + zr * zsz;
+    T *kxs = reinterpret_cast<T *>(tw + fx.twn);  // KX of the kept modes, read by every plane's output
     for (int e = tid; e < fx.twn; e += nthr) tw[e] = twg[e];
     for (int e = tid; e < mk; e += nthr) kxs[e] = T(kxv[e]);
-    const int TT = fx.t, S = fx.nstages, L0 = fx.span[0];
-    const int p = tid / TT;
-    int th = tid - p * TT;  // re-laundered every plane (see the loop)
-    const long zrow = (long)bx * ld;
+    // a block's last rows beyond NZ run on row NZ-1's data and store nothing
+    const int zi = bx * zr + zs;
+    const bool zvalid = zi < nz;
+    const long zrow = (long)min(zi, nz - 1) * ld;
     const int ja = jlo + by * chunk;
     const int jb = min(jhi, ja + chunk - 1);
 
@@ -112,6 +123,7 @@ __device__ __forceinline__ void xfused_body(const IpLen &fx, const cx<T> *__rest
         const long off2 = (long)(r == 2 ? 1 : 5) * wt_fs;
         const bool two = r == 2 || r == 3, xa1 = r == 3 || r == 5;
         auto emit = [&](int kp, cx<T> Zk, cx<T> Zn) {
+            if (!zvalid) return;
             const cx<T> Xa = make_cx<T>((Zk.re + Zn.re) * h, (Zk.im - Zn.im) * h);
             const cx<T> Xb = make_cx<T>((Zk.im + Zn.im) * h, -(Zk.re - Zn.re) * h);
             const T kx = kxs[kp];
@@ -143,7 +155,7 @@ __device__ __forceinline__ void xfused_body(const IpLen &fx, const cx<T> *__rest
     auto slot_row = [&](int plane, int pair) { return win + (plane % 3) * npi * nx + pair * nx; };
     // The eleven products of plane j at every position, from planes j-1, j, j+1.
     auto products = [&](int j) {
-        const cx<T> *wm = slot_row(j - 1, 0), *w0 = slot_row(j, 0), *wp = slot_row(j + 1, 0);
+        cx<T> *const base = reinterpret_cast<cx<T> *>(smem_raw);
         const T dyfj = T(dyf[j]), dyfm = T(dyf[j - 1]);
         // the plane's divisors as reciprocals, once per plane (an IEEE
         // division is a dozen instructions; within an ulp of dividing)
@@ -151,9 +163,15 @@ __device__ __forceinline__ void xfused_body(const IpLen &fx, const cx<T> *__rest
         // The positions beyond the last full pass go to the highest
         // wavefronts, not to wavefront 0, which also fetches the next plane
         // and runs its backward stage 0 alone after this phase.
-        const int extra = nx % nthr;
-        const int x0 = extra ? (tid + nthr - extra) % nthr : tid;
-        for (int x = x0; x < nx; x += nthr) {
+        const int npos = zr * nx;  // positions of all the block's z rows
+        const int extra = npos % nthr;
+        const int e0 = extra ? (tid + nthr - extra) % nthr : tid;
+        for (int e = e0; e < npos; e += nthr) {
+            const int zq = e / nx, x = e - zq * nx;
+            cx<T> *const zw = base + zq * zsz;
+            const cx<T> *wm = zw + ((j - 1) % 3) * npi * nx, *w0 = zw + (j % 3) * npi * nx,
+                        *wp = zw + ((j + 1) % 3) * npi * nx;
+            cx<T> *const fq = zw + 3 * npi * nx;
             const cx<T> a0 = w0[x], am = wm[x], ap = wp[x];
             const cx<T> b0 = w0[nx + x], bm = wm[nx + x], bp = wp[nx + x];
             const T u1 = a0.re, u2 = a0.im, u3 = b0.re;
@@ -168,17 +186,17 @@ __device__ __forceinline__ void xfused_body(const IpLen &fx, const cx<T> *__rest
             const T p6 = (u3p * u2p + u3 * u2p - u3 * u2 - u3m * u2) * r2dyf;    // d/dy(U3 U2)
             const T sp = u2 + u2p, sm = u2 + u2m;
             const T p7 = (T(0.25) * (sp * sp) - T(0.25) * (sm * sm)) * rdy;      // d/dy(U2 U2)
-            fb[0 * nx + x] = make_cx<T>(p1, p5);
-            fb[1 * nx + x] = make_cx<T>(p3, p7);
-            fb[2 * nx + x] = make_cx<T>(p0, p6);
-            fb[3 * nx + x] = make_cx<T>(p4, p2);
+            fq[0 * nx + x] = make_cx<T>(p1, p5);
+            fq[1 * nx + x] = make_cx<T>(p3, p7);
+            fq[2 * nx + x] = make_cx<T>(p0, p6);
+            fq[3 * nx + x] = make_cx<T>(p4, p2);
             if constexpr (NTH >= 1) {
                 const T t0 = b0.im, tm = bm.im, tp = bp.im;
                 const T p8 = t0 * u1;
                 const T p9 = t0 * u3;
                 const T p10 = (tp * u2p + t0 * u2p - t0 * u2 - tm * u2) * r2dyf;
-                fb[4 * nx + x] = make_cx<T>(p8, p10);
-                fb[5 * nx + x] = make_cx<T>(p9, T(0));
+                fq[4 * nx + x] = make_cx<T>(p8, p10);
+                fq[5 * nx + x] = make_cx<T>(p9, T(0));
             }
         }
     };
@@ -248,6 +266,17 @@ __device__ __forceinline__ void xfused_body(const IpLen &fx, const cx<T> *__rest
     }
 }
 
+// Which kernels take the z-row count at run time (MZ).  The long-row plans
+// (T = 32: the 384 and 256 ones) always run one row per block; their FP32
+// kernels are compiled for exactly one (the run-time form cost them 9% at
+// 384), their FP64 ones keep the run-time form (6% faster at 384, 32 planes
+// per block: the same work, scheduled differently by the compiler).
+template <typename T>
+constexpr bool xf_multi(int pid)
+{
+    return sizeof(T) == 8 || !(pid == 1 || pid == 12 || pid == 13);
+}
+
 // FP32 with the 384 plan: four resident blocks per CU (LDS holds four FP32
 // windows at NX = 384, 40 KB each) at the cost of a few spilled registers;
 // FP64 stays at the two its LDS allows.
@@ -257,22 +286,24 @@ k_xfused_ip(IpLen fx, const cx<T> *__restrict__ twg, int mk, int ld,
                                                     const cx<T> *__restrict__ zt, long zt_fs, long zt_ps,
                                                     cx<T> *__restrict__ wt, long wt_fs, long wt_ps,
                                                     const double *__restrict__ dy, const double *__restrict__ dyf,
-                                                    const double *__restrict__ kxv, int jlo, int jhi, int chunk)
+                                                    const double *__restrict__ kxv, int jlo, int jhi, int chunk,
+                                                    int zr, int nz)
 {
-    xfused_body<T, E, NTH, PID != 0>(plan_ref<PID>(fx), twg, mk, ld, zt, zt_fs, zt_ps, wt, wt_fs, wt_ps, dy, dyf,
-                                      kxv, jlo, jhi, chunk, blockIdx.x, blockIdx.y);
+    xfused_body<T, E, NTH, PID != 0, xf_multi<T>(PID)>(plan_ref<PID>(fx), twg, mk, ld, zt, zt_fs, zt_ps, wt, wt_fs, wt_ps,
+                                                     dy, dyf, kxv, jlo, jhi, chunk, blockIdx.x, blockIdx.y, zr, nz);
 }
 
-// grid (NZ, chunks of the tallest member, members)
+// grid (z-row groups, chunks of the tallest member, members)
 template <typename T, int E, int NTH, int PID>
 __global__ void __launch_bounds__(512) k_xfused_ip_batch(const XfDesc<T> *__restrict__ d, IpLen fx,
                                                           const cx<T> *__restrict__ twg, int mk, int ld, long ps,
-                                                          int chunk)
+                                                          int chunk, int zr, int nz)
 {
     const XfDesc<T> D = d[blockIdx.z];
     if ((int)blockIdx.y * chunk > D.jhi - D.jlo) return;
-    xfused_body<T, E, NTH, PID != 0>(plan_ref<PID>(fx), twg, mk, ld, D.zt, D.zt_fs, ps, D.wt, D.wt_fs, ps, D.dy,
-                                     D.dyf, D.kx, D.jlo, D.jhi, chunk, blockIdx.x, blockIdx.y);
+    xfused_body<T, E, NTH, PID != 0, xf_multi<T>(PID)>(plan_ref<PID>(fx), twg, mk, ld, D.zt, D.zt_fs, ps, D.wt, D.wt_fs,
+                                                    ps, D.dy, D.dyf, D.kx, D.jlo, D.jhi, chunk, blockIdx.x, blockIdx.y, zr,
+                                                    nz);
 }
 
 // -------------------------------------------------------------------------
@@ -403,6 +434,25 @@ __global__ void __launch_bounds__(256) k_vmax_planes(const double *__restrict__ 
 // launchers
 // -------------------------------------------------------------------------
 
+// z rows per block: short rows (few threads per transform) are packed until
+// a block has 256 threads, or until 64 KB of LDS
+template <typename T, int NTH>
+static int xfused_zrows(const IpLen &fx, int nz, int mk)
+{
+    using C = XCount<NTH>;
+    const int per = fx.t * (C::npi + C::npo);
+    const size_t row = sizeof(cx<T>) * (size_t)fx.n * (3 * C::npi + C::npo);
+    const size_t fixed = sizeof(cx<T>) * fx.twn + sizeof(T) * mk;
+    static const bool one = [] {
+        const char *e = std::getenv("DGPU_XZR");
+        return e && std::atoi(e) == 1;
+    }();
+    int zr = 1;
+    if (one) return zr;
+    while (2 * zr * per <= 256 && 2 * zr <= nz && fixed + 2 * zr * row <= 64 * 1024) zr *= 2;
+    return zr;
+}
+
 template <typename T, int E, int NTH, int PID>
 static void xfused_ip_e(hipStream_t s, const Geo &g, const IpLen &fx, const cx<T> *tw, int ld, const cx<T> *zt,
                         long zt_fs, cx<T> *wt, long wt_fs, const double *dy, const double *dyf, const double *kx,
@@ -410,8 +460,10 @@ static void xfused_ip_e(hipStream_t s, const Geo &g, const IpLen &fx, const cx<T
 {
     using C = XCount<NTH>;
     const int nchunks = (jhi - jlo + 1 + chunk - 1) / chunk;
-    const int threads = fx.t * (C::npi + C::npo);
-    const size_t lds = sizeof(cx<T>) * ((size_t)fx.n * (3 * C::npi + C::npo) + fx.twn) + sizeof(T) * g.mk;
+    const int zr = xfused_zrows<T, NTH>(fx, g.nz, g.mk);
+    const int threads = zr * fx.t * (C::npi + C::npo);
+    const size_t lds =
+        sizeof(cx<T>) * ((size_t)zr * fx.n * (3 * C::npi + C::npo) + fx.twn) + sizeof(T) * g.mk;
     static bool attr = false;
     if (!attr) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_xfused_ip<T, E, NTH, PID>),
@@ -419,17 +471,10 @@ static void xfused_ip_e(hipStream_t s, const Geo &g, const IpLen &fx, const cx<T
         attr = true;
     }
     const long ps = (long)g.nz * ld;
-    hipLaunchKernelGGL((k_xfused_ip<T, E, NTH, PID>), dim3(g.nz, nchunks), dim3(threads), lds, s, fx, tw, g.mk, ld,
-                       zt, zt_fs, ps, wt, wt_fs, ps, dy, dyf, kx, jlo, jhi, chunk);
+    hipLaunchKernelGGL((k_xfused_ip<T, E, NTH, PID>), dim3((g.nz + zr - 1) / zr, nchunks), dim3(threads), lds, s, fx,
+                       tw, g.mk, ld, zt, zt_fs, ps, wt, wt_fs, ps, dy, dyf, kx, jlo, jhi, chunk, zr, g.nz);
 }
 
-size_t xfused_ip_lds_bytes(const IpLen &fx, int nth, size_t elem, int mk)
-{
-    const int npi = (3 + nth + 1) / 2, npo = (8 + 3 * nth + 1) / 2;
-    return elem * ((size_t)fx.n * (3 * npi + npo) + fx.twn) + elem / 2 * (size_t)mk;
-}
-
-int xfused_ip_threads(const IpLen &fx, int nth) { return fx.t * ((3 + nth + 1) / 2 + (8 + 3 * nth + 1) / 2); }
 
 // The single-run x kernel is built for the 384 and 256 plans (the ensembles
 // use the batched kernel) with one scalar; other shapes run on runtime plans.
@@ -469,17 +514,19 @@ static void xfused_ip_batch_e(hipStream_t s, const Geo &g, const IpLen &fx, cons
                               int nmem, int max_span, int ld, int chunk)
 {
     using C = XCount<NTH>;
-    const int threads = fx.t * (C::npi + C::npo);
-    const size_t lds = sizeof(cx<T>) * ((size_t)fx.n * (3 * C::npi + C::npo) + fx.twn) + sizeof(T) * g.mk;
+    const int zr = xfused_zrows<T, NTH>(fx, g.nz, g.mk);
+    const int threads = zr * fx.t * (C::npi + C::npo);
+    const size_t lds =
+        sizeof(cx<T>) * ((size_t)zr * fx.n * (3 * C::npi + C::npo) + fx.twn) + sizeof(T) * g.mk;
     static bool attr = false;
     if (!attr) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_xfused_ip_batch<T, E, NTH, PID>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         attr = true;
     }
-    dim3 grid(g.nz, (max_span + chunk - 1) / chunk, nmem);
+    dim3 grid((g.nz + zr - 1) / zr, (max_span + chunk - 1) / chunk, nmem);
     hipLaunchKernelGGL((k_xfused_ip_batch<T, E, NTH, PID>), grid, dim3(threads), lds, s, d, fx, tw, g.mk, ld,
-                       (long)g.nz * ld, chunk);
+                       (long)g.nz * ld, chunk, zr, g.nz);
 }
 
 template <typename T>

@@ -190,8 +190,6 @@ void launch_vmax_ip(hipStream_t s, const Geo &g, const IpLen &fx, const cx<T> *t
 int vmax_rows_per_block(const IpLen &fx);
 size_t vmax_part_count(const Geo &g, const IpLen &fx);
 size_t zip_lds_bytes(const IpLen &fz, size_t elem);
-size_t xfused_ip_lds_bytes(const IpLen &fx, int nth, size_t elem, int mk);
-int xfused_ip_threads(const IpLen &fx, int nth);
 
 template <typename T, typename NLT>
 void launch_cn(hipStream_t s, const CnArgs &a, const CnPtrs<T, NLT> &p);
