@@ -523,8 +523,10 @@ int32_t dgpu_create(const dgpu_params *p, dgpu_ctx **out)
         // per thread, DGPU_XCHUNK for the planes per block of the x kernel
         const char *ex = std::getenv("DGPU_FFT_EX"), *ez = std::getenv("DGPU_FFT_EZ");
         const char *xc = std::getenv("DGPU_XCHUNK");
-        // FP64 kernels spill above 12 values per thread
-        const int emax = c->mixed ? 24 : 12;
+        // FP64 kernels spill above 12 values per thread; the FP32 z kernels
+        // are faster at 12 too (NZ = 384: four radix stages at twice the
+        // resident columns beat three stages of 24 values)
+        const int emax = 12;
         if (!dgpu::ip_plan_x(g.nx, ex ? std::atoi(ex) : 0, c->mixed ? 16 : 12, 64, c->ipx) ||
             !dgpu::ip_plan_z(g.nz, ez ? std::atoi(ez) : 0, emax, c->ipz)) {
             delete c;

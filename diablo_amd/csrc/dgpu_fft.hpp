@@ -306,16 +306,19 @@ constexpr ConstPlanSpec kConstPlanSpecs[] = {
     {64, 16, 8, 0xfda87520u, 0},   // 5: (8,8), FP32
     // z plans (column batches)
     {384, 12, 0, 0u, 2},           // 6: (6,4,4,4), FP64
-    {384, 24, 0, 0u, 0},           // 7: (8,8,6), FP32
+    {384, 24, 0, 0u, 0},           // 7: (8,8,6), FP32 (DGPU_FFT_EZ=24)
     {64, 8, 0, 0u, 0},             // 8: (8,8)
     {96, 12, 0, 0u, 0},            // 9: (6,4,4)
     {128, 8, 0, 0u, 0},            // 10: (8,8,2)
-    {192, 24, 0, 0u, 0},           // 11: (8,8,3), FP32
+    {192, 24, 0, 0u, 0},           // 11: (8,8,3), FP32 (DGPU_FFT_EZ=24)
     // x plans appended after the z plans (their ids stay)
     {256, 8, 4, 0x76543210u, 0},   // 12: (4,8,8)
     // FP32 x plans with the swizzle picked for 8-byte slots
     {256, 8, 4, 0x7654ba98u, 0, 4},  // 13: (4,8,8), FP32
     {128, 8, 4, 0xfda87520u, 0, 4},  // 14: (4,8,4), FP32
+    // z plans at up to 12 values per thread
+    {192, 12, 0, 0u, 0},           // 15: (6,4,4,2)
+    {256, 8, 0, 0u, 2},            // 16: (8,8,4)
 };
 constexpr int kNumConstPlans = (int)(sizeof(kConstPlanSpecs) / sizeof(kConstPlanSpecs[0]));
 

@@ -69,8 +69,9 @@ static inline int z_lw(int t)
 }
 
 // Compile-time plans each kernel family is built for, per precision (ids of
-// kConstPlanSpecs): the 384 case in both precisions and the ensemble lengths
-// of the FP32 path.
+// kConstPlanSpecs): x, the 384 and 256 cases in both precisions and the
+// ensemble lengths of the FP32 path; z, every length with a plan, both
+// precisions.
 template <int... P>
 struct PidList {};
 template <typename T>
@@ -79,7 +80,7 @@ using XPids = std::conditional_t<sizeof(T) == 4, PidList<1, 2, 3, 4, 5, 12, 13, 
 template <typename T>
 using XPidsSingle = std::conditional_t<sizeof(T) == 4, PidList<1, 12, 13>, PidList<1, 12>>;
 template <typename T>
-using ZPids = std::conditional_t<sizeof(T) == 4, PidList<7, 8, 9, 10, 11>, PidList<6>>;
+using ZPids = PidList<6, 8, 9, 10, 15, 16>;
 
 // Calls fn(std::integral_constant<int, P>) for the P of the list equal to
 // pid; false when pid is not in the list.
