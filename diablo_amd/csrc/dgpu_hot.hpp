@@ -38,7 +38,8 @@ struct XCount {
 __device__ constexpr IpLen kDevConstPlans[] = {const_plan(0),  const_plan(1),  const_plan(2),  const_plan(3),
                                                const_plan(4),  const_plan(5),  const_plan(6),  const_plan(7),
                                                const_plan(8),  const_plan(9),  const_plan(10), const_plan(11),
-                                               const_plan(12), const_plan(13), const_plan(14)};
+                                               const_plan(12), const_plan(13), const_plan(14), const_plan(15),
+                                               const_plan(16)};
 static_assert(sizeof(kDevConstPlans) / sizeof(IpLen) == (size_t)kNumConstPlans, "one entry per compile-time plan");
 
 template <int PID>
