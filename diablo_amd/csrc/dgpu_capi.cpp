@@ -48,7 +48,7 @@ struct dgpu_ctx {
     int ld = 0;      // row length of the intermediates (>= mk, multiple of 8)
     int xchunk = 64; // planes per block of the fused x kernel (set at creation)
     int colnw = 8;   // waves per block of the segmented column kernels; 0: whole-column sweeps
-    int colfuse = -1;     // FP32 state, fused column kernel: -1 where NY >= 160, 0 never, 1 where it fits
+    int colfuse = -1;     // fused column kernel: -1 where NY >= 160 (FP64: at 8 waves), 0 never, 1 where it fits
     // mixed paths: FP32 transforms, intermediates and nonlinear terms
     int mixed = 0;
     // DGPU_MIXED: FP32 working state [slot][j][K][I] (same slots as d_state's
@@ -1064,13 +1064,21 @@ int32_t dgpu_set_zdft(dgpu_ctx *c, int32_t on)
 }
 
 static int fused_waves(const dgpu_ctx *c, int ny);
+// the fused column kernel runs with the FP32 state or in FP64 (not with the
+// FP64 state under FP32 transforms)
+static bool fused_fits(const dgpu_ctx *c, int ny, int waves)
+{
+    if (c->f32state) return dgpu::colfused_supported<float>(ny, waves);
+    return !c->mixed && dgpu::colfused_supported<double>(ny, waves);
+}
 
 // a shape some segmented kernel of the context's precision runs with
 static bool column_shape_fits(const dgpu_ctx *c, int waves)
 {
     if (waves == 0) return true;
-    return c->f32state ? (dgpu::colseg_supported<float>(c->g.ny, waves) || dgpu::colfused_supported(c->g.ny, waves))
-                       : dgpu::colseg_supported<double>(c->g.ny, waves);
+    return c->f32state
+               ? (dgpu::colseg_supported<float>(c->g.ny, waves) || dgpu::colfused_supported<float>(c->g.ny, waves))
+               : dgpu::colseg_supported<double>(c->g.ny, waves);
 }
 
 int32_t dgpu_set_column_shape(dgpu_ctx *c, int32_t waves, int32_t fused)
@@ -1082,9 +1090,9 @@ int32_t dgpu_set_column_shape(dgpu_ctx *c, int32_t waves, int32_t fused)
     if (!column_shape_fits(c, waves))
         return fail(c, DGPU_E_ARG, "column shape: the segmented column kernels do not fit NY = " +
                                        std::to_string(c->g.ny) + " with " + std::to_string(waves) + " waves");
-    if (fused == 1 && !(c->f32state && waves && dgpu::colfused_supported(c->g.ny, waves)))
-        return fail(c, DGPU_E_ARG, "column shape: the fused column kernel needs the FP32 state (DGPU_MIXED), "
-                                   "4, 8 or 16 waves and NY <= 286");
+    if (fused == 1 && !(waves && fused_fits(c, c->g.ny, waves)))
+        return fail(c, DGPU_E_ARG, "column shape: the fused column kernel needs the FP32 state (DGPU_MIXED) or "
+                                   "the FP64 path, 4, 8 or 16 waves (4 or 8 in FP64) and NY <= 286");
     if (c->colnw == waves && c->colfuse == fused) return DGPU_OK;
     c->colnw = waves;
     c->colfuse = fused;
@@ -1443,16 +1451,20 @@ static int seg_waves(const dgpu_ctx *c, int ny, bool fp32)
     const bool fits = fp32 ? dgpu::colseg_supported<float>(ny, c->colnw) : dgpu::colseg_supported<double>(ny, c->colnw);
     return fits ? c->colnw : 0;
 }
-// The fused column kernel of the FP32 state (dgpu_colseg.hip): the
-// context's shape where the column fits it, else 0.
+// The fused column kernel (dgpu_colseg.hip): the context's shape where the
+// column fits it, else 0.
 // Left to itself (colfuse -1) the fused kernel runs from NY = 160 up (six or
 // more rows per segment): on shorter columns its four solves in a row are
-// mostly barriers, and the separate kernels are faster.
+// mostly barriers, and the separate kernels are faster.  In FP64 only at 8
+// waves: its 252 VGPRs and 131 KB of LDS (NY = 257) leave one block per CU,
+// and at 4 waves that is too few wavefronts (1.73 ms against 1.22 at 8 and
+// 1.24 for the two kernels, 384 x 257 x 384).
 static int fused_waves(const dgpu_ctx *c, int ny)
 {
-    if (!c->colnw || !c->f32state) return 0;
-    const bool want = c->colfuse == 1 || (c->colfuse < 0 && ny >= 160);
-    return (want && dgpu::colfused_supported(ny, c->colnw)) ? c->colnw : 0;
+    if (!c->colnw) return 0;
+    const bool auto_on = ny >= 160 && (c->f32state || c->colnw == 8);
+    const bool want = c->colfuse == 1 || (c->colfuse < 0 && auto_on);
+    return (want && fused_fits(c, ny, c->colnw)) ? c->colnw : 0;
 }
 // the low-mode side columns (a few modes per system): the smallest blocks
 static int side_waves(const dgpu_ctx *c, int ny)
@@ -1604,6 +1616,11 @@ static int rk_enqueue(dgpu_ctx *c, int rk_step, double time, double temp1, doubl
     } else {
         dgpu::CnPtrs<double, double> P;
         fill_ptrs<double>(c, P, c->nl);
+        if (const int nf = fused_waves(c, g.ny); nf && !courant) {
+            TimerScope t(c, 3);
+            dgpu::launch_colfused<double, double>(c->stream, nf, A, R, P);
+            return check_launch(c, "colfused");
+        }
         {
             TimerScope t(c, 3);
             if (seg64) dgpu::launch_cn_seg<double, double>(c->stream, seg64, A, P);
@@ -1906,13 +1923,13 @@ int32_t dgpu_autotune(dgpu_ctx *c, int32_t steps, double delta_t, int32_t *xchun
     if (rc == DGPU_OK && !c->les && colnw0 != 0) {
         c->xchunk = best_x;
         rc = dgpu_set_zdft(c, best_z);
-        // with the FP32 state also the fused column kernel on and off
+        // with the FP32 state and in FP64 also the fused column kernel on and off
         const bool f0 = fused_waves(c, c->g.ny) != 0;
-        for (int fz = 0; fz <= (c->f32state ? 1 : 0) && rc == DGPU_OK; ++fz)
+        for (int fz = 0; fz <= ((c->f32state || !c->mixed) ? 1 : 0) && rc == DGPU_OK; ++fz)
             for (int nw : {4, 8, 16}) {
                 const bool fz_on = fz ? !f0 : f0;
                 if (rc != DGPU_OK || !column_shape_fits(c, nw)) continue;
-                if (fz_on && !dgpu::colfused_supported(c->g.ny, nw)) continue;
+                if (fz_on && !fused_fits(c, c->g.ny, nw)) continue;
                 if (nw == colnw0 && fz_on == f0) continue;  // timed above
                 c->colnw = nw;
                 c->colfuse = fz_on ? 1 : 0;
@@ -2323,7 +2340,9 @@ static int ens_enqueue_batched(dgpu_ensemble *e, int rk_step)
         const int max_ny = gr.max_planes - 2;
         if (!mixed) {
             const auto *d = (const dgpu::ColDesc<double, double> *)gr.d_col[k];
-            if (const int nw = seg_waves(e->m[gr.idx[0]], max_ny, false)) {
+            if (const int nf = fused_waves(e->m[gr.idx[0]], max_ny)) {
+                dgpu::launch_colfused_batch<double, double>(s, nf, g, max_ny, d, nm);
+            } else if (const int nw = seg_waves(e->m[gr.idx[0]], max_ny, false)) {
                 dgpu::launch_cn_seg_batch<double, double>(s, nw, g, max_ny, d, nm);
                 dgpu::launch_remdiv_seg_batch<double, double>(s, nw, g, max_ny, d, nm);
             } else {
@@ -2522,7 +2541,7 @@ int32_t dgpu_ensemble_rk_chan_1(dgpu_ensemble *e, int32_t rk_step, const double 
     };
     if (!graphable) return enqueue_all(false);
     std::vector<int> shape;
-    for (const dgpu_ctx *c : e->m) shape.insert(shape.end(), {c->xchunk, c->colnw, c->zdft ? 1 : 0});
+    for (const dgpu_ctx *c : e->m) shape.insert(shape.end(), {c->xchunk, c->colnw, c->colfuse, c->zdft ? 1 : 0});
     if (!(e->gexec[k] && e->gkey[k] == key && e->gshape[k] == shape)) {
         if (e->gexec[k]) {
             hipGraphExecDestroy(e->gexec[k]);

@@ -731,8 +731,8 @@ __device__ __forceinline__ void cn3_solve(const CnArgs &A, const CnPtrs<T, NLT> 
 }
 
 // ---------------------------------------------------------------------------
-// k_colfused: the whole column stage of a substep for a group of modes with
-// the FP32 state.  The CN solves of U1, U3, U2 (cn3_solve: one pass over the
+// k_colfused: the whole column stage of a substep for a group of modes, with
+// the FP32 state or in FP64.  The CN solves of U1, U3, U2 (cn3_solve: one pass over the
 // rows for the three) stay in registers; TH's goes straight into CTH (every
 // system of these modes has read the old CTH by then, and no other block
 // reads these modes); then the divergence of U*,
@@ -916,14 +916,14 @@ __device__ __forceinline__ void colfused_body(const CnArgs &A, const RdArgs &Rd,
 }
 
 template <typename T, typename NLT, int MPW, int NW, int RMAX, int NTH>
-__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 2)))
 k_colfused(CnArgs A, RdArgs Rd, CnPtrs<T, NLT> P)
 {
     colfused_body<T, NLT, MPW, NW, RMAX, NTH>(A, Rd, P, xcd_block(blockIdx.x, gridDim.x));
 }
 
 template <typename T, typename NLT, int MPW, int NW, int RMAX, int NTH>
-__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 2)))
 k_colfused_batch(const ColDesc<T, NLT> *__restrict__ d)
 {
     const ColDesc<T, NLT> &D = d[blockIdx.z];
@@ -1063,13 +1063,15 @@ void launch_remdiv_seg_batch(hipStream_t s, int nw, const Geo &g, int max_ny, co
 }
 
 // The fused column kernel: 32 segments per column, at most 9 rows each
-// (NY <= 286), FP32 state.
+// (NY <= 286), FP32 state or FP64 (there at two waves per SIMD: 252 VGPRs,
+// and the three systems' tables take 81 KB of LDS at NY = 257, so 4 or 8
+// waves per block).
 constexpr int kFusedSeg = 32;
 
+template <typename T>
 static size_t colfused_lds(int ny, int nw)
 {
     const int mpw = 64 * nw / kFusedSeg, n = ny + 2;
-    using T = float;
     const size_t one = sizeof(T) * (size_t)((n * kLRow + 1) & ~1) + 2 * sizeof(cx<T>) * kFusedSeg * mpw +
                        2 * sizeof(T) * kFusedSeg + 2 * sizeof(T) * kFusedSeg * mpw + sizeof(cx<T>) * kFusedSeg * mpw;
     const size_t three = 6 * sizeof(cx<T>) * kFusedSeg * mpw + 6 * sizeof(T) * kFusedSeg +
@@ -1077,60 +1079,76 @@ static size_t colfused_lds(int ny, int nw)
     return std::max(one, three);
 }
 
+template <typename T>
 bool colfused_supported(int ny, int nw)
 {
-    return (nw == 4 || nw == 8 || nw == 16) && ny >= 4 && (ny + 2 + kFusedSeg - 1) / kFusedSeg <= 9 &&
-           colfused_lds(ny, nw) <= 150 * 1024;
+    // FP64 blocks of 16 waves would hold its 252 VGPRs in 128 (spills)
+    return (nw == 4 || nw == 8 || (nw == 16 && sizeof(T) == 4)) && ny >= 4 &&
+           (ny + 2 + kFusedSeg - 1) / kFusedSeg <= 9 && colfused_lds<T>(ny, nw) <= 150 * 1024;
 }
 
 template <typename T, typename NLT>
 void launch_colfused(hipStream_t s, int nw, const CnArgs &a, const RdArgs &r, const CnPtrs<T, NLT> &p)
 {
-    const size_t lds = colfused_lds(a.g.ny, nw);
+    const size_t lds = colfused_lds<T>(a.g.ny, nw);
     with_nw(nw, [&](auto wc) {
         constexpr int NW = decltype(wc)::value, MPW = 64 * NW / kFusedSeg;
-        const long blocks = ((a.g.ps + MPW - 1) / MPW + 7) / 8 * 8;  // a multiple of 8 (xcd_block)
-        static bool attr = false;
-        if (!attr) {
-            (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_colfused<T, NLT, MPW, NW, 9, 0>),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-            (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_colfused<T, NLT, MPW, NW, 9, 1>),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-            attr = true;
+        if constexpr (sizeof(T) == 8 && NW == 16) {
+            return;  // not built: colfused_supported refuses the shape
+        } else {
+            const long blocks = ((a.g.ps + MPW - 1) / MPW + 7) / 8 * 8;  // a multiple of 8 (xcd_block)
+            static bool attr = false;
+            if (!attr) {
+                (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_colfused<T, NLT, MPW, NW, 9, 0>),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+                (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_colfused<T, NLT, MPW, NW, 9, 1>),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+                attr = true;
+            }
+            if (a.g.nth == 0)
+                hipLaunchKernelGGL((k_colfused<T, NLT, MPW, NW, 9, 0>), dim3(blocks), dim3(64 * NW), lds, s, a, r, p);
+            else
+                hipLaunchKernelGGL((k_colfused<T, NLT, MPW, NW, 9, 1>), dim3(blocks), dim3(64 * NW), lds, s, a, r, p);
         }
-        if (a.g.nth == 0)
-            hipLaunchKernelGGL((k_colfused<T, NLT, MPW, NW, 9, 0>), dim3(blocks), dim3(64 * NW), lds, s, a, r, p);
-        else
-            hipLaunchKernelGGL((k_colfused<T, NLT, MPW, NW, 9, 1>), dim3(blocks), dim3(64 * NW), lds, s, a, r, p);
     });
 }
 
 template <typename T, typename NLT>
 void launch_colfused_batch(hipStream_t s, int nw, const Geo &g, int max_ny, const ColDesc<T, NLT> *d, int nmem)
 {
-    const size_t lds = colfused_lds(max_ny, nw);
+    const size_t lds = colfused_lds<T>(max_ny, nw);
     with_nw(nw, [&](auto wc) {
         constexpr int NW = decltype(wc)::value, MPW = 64 * NW / kFusedSeg;
-        const dim3 grid((g.ps + MPW - 1) / MPW, 1, nmem);
-        static bool attr = false;
-        if (!attr) {
-            (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_colfused_batch<T, NLT, MPW, NW, 9, 0>),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-            (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_colfused_batch<T, NLT, MPW, NW, 9, 1>),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-            attr = true;
+        if constexpr (sizeof(T) == 8 && NW == 16) {
+            return;  // not built: colfused_supported refuses the shape
+        } else {
+            const dim3 grid((g.ps + MPW - 1) / MPW, 1, nmem);
+            static bool attr = false;
+            if (!attr) {
+                (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_colfused_batch<T, NLT, MPW, NW, 9, 0>),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+                (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_colfused_batch<T, NLT, MPW, NW, 9, 1>),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+                attr = true;
+            }
+            if (g.nth == 0)
+                hipLaunchKernelGGL((k_colfused_batch<T, NLT, MPW, NW, 9, 0>), grid, dim3(64 * NW), lds, s, d);
+            else
+                hipLaunchKernelGGL((k_colfused_batch<T, NLT, MPW, NW, 9, 1>), grid, dim3(64 * NW), lds, s, d);
         }
-        if (g.nth == 0)
-            hipLaunchKernelGGL((k_colfused_batch<T, NLT, MPW, NW, 9, 0>), grid, dim3(64 * NW), lds, s, d);
-        else
-            hipLaunchKernelGGL((k_colfused_batch<T, NLT, MPW, NW, 9, 1>), grid, dim3(64 * NW), lds, s, d);
     });
 }
 
+template bool colfused_supported<float>(int, int);
+template bool colfused_supported<double>(int, int);
 template void launch_colfused<float, float>(hipStream_t, int, const CnArgs &, const RdArgs &,
                                             const CnPtrs<float, float> &);
+template void launch_colfused<double, double>(hipStream_t, int, const CnArgs &, const RdArgs &,
+                                              const CnPtrs<double, double> &);
 template void launch_colfused_batch<float, float>(hipStream_t, int, const Geo &, int, const ColDesc<float, float> *,
                                                   int);
+template void launch_colfused_batch<double, double>(hipStream_t, int, const Geo &, int,
+                                                    const ColDesc<double, double> *, int);
 template bool colseg_supported<double>(int, int);
 template bool colseg_supported<float>(int, int);
 template void launch_cn_seg<double, double>(hipStream_t, int, const CnArgs &, const CnPtrs<double, double> &);

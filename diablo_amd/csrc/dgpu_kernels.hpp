@@ -112,8 +112,10 @@ template <typename T, typename NLT>
 void launch_cn_seg_batch(hipStream_t s, int nw, const Geo &g, int max_ny, const ColDesc<T, NLT> *d, int nmem);
 template <typename T, typename NLT>
 void launch_remdiv_seg_batch(hipStream_t s, int nw, const Geo &g, int max_ny, const ColDesc<T, NLT> *d, int nmem);
-// The fused column stage (FP32 state): the CN solves, the divergence, the
-// Poisson solve, the projection and the CP update in one kernel, no scratch.
+// The fused column stage (FP32 or FP64 state): the CN solves, the
+// divergence, the Poisson solve, the projection and the CP update in one
+// kernel, no scratch.
+template <typename T>
 bool colfused_supported(int ny, int nw);
 template <typename T, typename NLT>
 void launch_colfused(hipStream_t s, int nw, const CnArgs &a, const RdArgs &r, const CnPtrs<T, NLT> &p);

@@ -345,8 +345,9 @@ class GpuChannel:
 
     def set_column_shape(self, waves: int, fused: int = -1):
         """Waves per block of the segmented column kernels (4, 8, 16), or 0
-        for the whole-column sweeps; fused (FP32 state): 1 the one-kernel
-        column stage, 0 two kernels, -1 the library's choice by NY."""
+        for the whole-column sweeps; fused (FP32 state or FP64): 1 the
+        one-kernel column stage, 0 two kernels, -1 the library's choice by
+        NY (and, in FP64, the waves)."""
         self._chk(self.L.dgpu_set_column_shape(self.h, int(waves), int(fused)), "dgpu_set_column_shape")
 
     def _column(self):

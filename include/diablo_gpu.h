@@ -293,14 +293,16 @@ int32_t dgpu_set_zdft(dgpu_ctx *ctx, int32_t on);
  * results are the same bits for 4, 8 and 16 (the same operations in the
  * same order for each mode); the sweeps round differently (a different
  * elimination order), within round-off.
- * fused (contexts with the FP32 state, DGPU_MIXED): 1 runs the whole column
- * stage -- the four CN solves, the divergence, the Poisson solve, the
- * projection and the CP update -- as one kernel that reads and writes the
- * state once (32 segments, NY <= 286); 0 keeps the two kernels; -1 (the
- * default; environment DGPU_COLFUSE) fuses from NY = 160 up.  The fused
- * kernel rounds differently from the two kernels (32 segments, not 16),
- * within FP32 round-off.  An RK stage 3 with a COURANT request runs the two
- * kernels (the maxima are taken between them).
+ * fused (contexts with the FP32 state, DGPU_MIXED, and FP64 contexts): 1
+ * runs the whole column stage -- the four CN solves, the divergence, the
+ * Poisson solve, the projection and the CP update -- as one kernel that
+ * reads and writes the state once (32 segments, NY <= 286; 4 or 8 waves in
+ * FP64); 0 keeps the two kernels; -1 (the default; environment
+ * DGPU_COLFUSE) fuses from NY = 160 up (in FP64 only at 8 waves).  With the
+ * FP32 state the fused kernel rounds differently from the two kernels (32
+ * segments, not 16), within FP32 round-off; in FP64 both use 32 segments
+ * and agree to round-off.  An RK stage 3 with a COURANT request runs the
+ * two kernels (the maxima are taken between them).
  * dgpu_get_column_shape reports the waves in use (0 for the sweeps) and
  * whether the fused kernel runs (fused may be NULL). */
 int32_t dgpu_set_column_shape(dgpu_ctx *ctx, int32_t waves, int32_t fused);
@@ -312,7 +314,7 @@ int32_t dgpu_get_column_shape(const dgpu_ctx *ctx, int32_t *waves, int32_t *fuse
  * for mixed contexts with NX, NZ <= 256 the FFT or the MFMA z transforms;
  * then, with those fixed, the column shape (dgpu_set_column_shape: 4, 8 or
  * 16 waves per block, unless the sweeps are selected; with the FP32 state
- * also the fused column kernel on and off) -- keeps the fastest and puts
+ * and in FP64 also the fused column kernel on and off) -- keeps the fastest and puts
  * the state back as it was (the run does not advance).
  * Outputs (may be NULL): the chosen planes per block, whether the MFMA z
  * transforms are on, and the best ms per step.  Not with a COURANT request

@@ -1,9 +1,10 @@
 """The column stage's launch shapes (dgpu_set_column_shape): the segmented
 column solves in blocks of 4, 8 and 16 waves give the same bits; the
-whole-column sweeps agree to round-off; the fused one-kernel column stage of
-the FP32 state is as accurate as the two kernels and agrees with the oracle;
-the oracle agrees with the shape the context runs; shapes a column does not
-fit are refused.  All through the C-ABI."""
+whole-column sweeps agree to round-off; the fused one-kernel column stage
+agrees with the two kernels to round-off in FP64 and is as accurate as they
+are with the FP32 state, and agrees with the oracle; the oracle agrees with
+the shape the context runs; shapes a column does not fit are refused.  All
+through the C-ABI."""
 import numpy as np
 import pytest
 
@@ -78,14 +79,16 @@ def test_column_shapes_same_bits(gpu_mod, name, precision):
     assert not bad, bad
 
 
-@pytest.mark.parametrize("waves", [4, 8])
-def test_column_shape_vs_oracle(gpu_mod, oracle_mod, waves):
+@pytest.mark.parametrize("waves,fused", [(4, 0), (8, 0), (8, 1)])
+def test_column_shape_vs_oracle(gpu_mod, oracle_mod, waves, fused):
     """The kitchen-sink deck (every term of RK_CHAN_1 on) through the
-    smaller blocks against the oracle, 1e-11 after 2 steps."""
+    smaller blocks and the FP64 fused column kernel against the oracle,
+    1e-11 after 2 steps."""
     cfg = kitchen_sink()
     phys = case.initial_physical(cfg)
     g = gpu_mod.GpuChannel(cfg)
-    g.set_column_shape(waves)
+    g.set_column_shape(waves, fused)
+    assert g.column_fused() == (fused == 1)
     g.initialize(*phys)
     g.step(2)
     o = oracle_mod.Oracle(cfg)
@@ -113,8 +116,9 @@ def test_column_shape_refusals(gpu_mod):
         t.set_column_shape(8)
     t.set_column_shape(0)
     t.close()
-    # the fused kernel: FP32 state only, NY <= 286
-    f = gpu_mod.GpuChannel(cfg)
+    # the fused kernel: the FP32 state or the FP64 path (not the FP64 state
+    # under FP32 transforms), NY <= 286
+    f = gpu_mod.GpuChannel(cfg, precision="mixed_f64state")
     with pytest.raises(gpu_mod.DgpuError, match="FP32 state"):
         f.set_column_shape(8, 1)
     f.close()
@@ -127,22 +131,43 @@ def test_column_shape_refusals(gpu_mod):
         a = gpu_mod.GpuChannel(case.channel_case(24, ny, 18, kick=0.0, streak=0.0), precision="mixed")
         assert a.column_fused() == want, ny
         a.close()
+    # FP64: fused by default from NY = 160 at 8 waves only; not at 16 waves
+    # (its registers do not fit a block of 16)
+    d = gpu_mod.GpuChannel(case.channel_case(24, 161, 18, kick=0.0, streak=0.0))
+    assert d.column_shape() == 8 and d.column_fused()
+    d.set_column_shape(4)
+    assert not d.column_fused()
+    d.set_column_shape(4, 1)
+    assert d.column_fused()
+    with pytest.raises(gpu_mod.DgpuError, match="column shape"):
+        d.set_column_shape(16, 1)
+    d.close()
 
 
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
 @pytest.mark.parametrize("name", list(CASES))
-def test_fused_column_stage(gpu_mod, name):
-    """The fused column kernel (FP32 state): 4, 8 and 16 waves give the same
-    bits; against the FP64 run it is no less accurate than the two kernels
-    (within 1.5x of their error, field by field, plus 1e-7)."""
+def test_fused_column_stage(gpu_mod, name, precision):
+    """The fused column kernel: 4, 8 and (FP32 state) 16 waves give the same
+    bits.  In FP64 it agrees with the two kernels within 1e-11 (the parity
+    bar of the substep tests; both solve on 32 segments, the Poisson
+    segments differ).  With the FP32 state, against the FP64 run it is no
+    less accurate than the two kernels (within 1.5x of their error, field by
+    field, plus 1e-7)."""
     cfg = CASES[name]()
     runs = {}
-    for w in (8, 4, 16):
-        say(f"{name}: fused column stage, {w} waves")
-        runs[w] = _run(gpu_mod, cfg, "mixed", w, fused=1)
-    for w in (4, 16):
+    shapes = (8, 4, 16) if precision == "mixed" else (8, 4)
+    for w in shapes:
+        say(f"{name} {precision}: fused column stage, {w} waves")
+        runs[w] = _run(gpu_mod, cfg, precision, w, fused=1)
+    for w in shapes[1:]:
         for k in FIELDS:
             assert np.array_equal(runs[w][k], runs[8][k]), (w, k)
-    two = _run(gpu_mod, cfg, "mixed", 8, fused=0)
+    two = _run(gpu_mod, cfg, precision, 8, fused=0)
+    if precision == "fp64":
+        errs, bad = compare_states(cfg, runs[8], two, 1e-11, FIELDS)
+        say("fused vs two kernels", errs)
+        assert not bad, bad
+        return
     ref = _run(gpu_mod, cfg, "fp64", 8)
     e_f, _ = compare_states(cfg, runs[8], ref, 1.0, FIELDS)
     e_t, _ = compare_states(cfg, two, ref, 1.0, FIELDS)
@@ -177,7 +202,8 @@ def test_fused_column_stage_vs_oracle(gpu_mod, oracle_mod):
         assert rel < 1e-5, (k, rel)
 
 
-def test_fused_column_stage_with_courant(gpu_mod):
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_fused_column_stage_with_courant(gpu_mod, precision):
     """COURANT steps run the two kernels for their stage 3 (the maxima are
     taken between them); the run with the fused kernel follows the run with
     the two kernels: DELTA_T within 1e-6, the state within 1e-5."""
@@ -185,7 +211,7 @@ def test_fused_column_stage_with_courant(gpu_mod):
     phys = case.initial_physical(cfg)
     out = []
     for fz in (1, 0):
-        g = gpu_mod.GpuChannel(cfg, precision="mixed")
+        g = gpu_mod.GpuChannel(cfg, precision=precision)
         g.set_column_shape(8, fz)
         g.initialize(*phys)
         dts = []
@@ -207,8 +233,8 @@ def test_fused_column_stage_with_courant(gpu_mod):
 def test_ensemble_follows_member_shapes(gpu_mod, precision):
     """Ensemble graphs are captured per launch shape: after the members'
     column shape changes between steps, the ensemble re-captures and still
-    matches each member's single run bit for bit (with the FP32 state also
-    through the fused column kernel, batched and single)."""
+    matches each member's single run bit for bit (also through the fused
+    column kernel, batched and single)."""
     cfgs = [case.kh_case(24, 17, 24, kick=0.05, dt=0.01, ri=0.1, seed=1),
             case.kh_case(48, 33, 36, kick=0.05, dt=0.01, ri=0.2, re=500.0, seed=2),
             case.kh_case(48, 65, 36, kick=0.05, dt=0.01, ri=0.15, seed=3)]
@@ -221,7 +247,8 @@ def test_ensemble_follows_member_shapes(gpu_mod, precision):
         s.initialize(*phys)
         singles.append(s)
     ens.synchronize()
-    shapes = ((16, 0), (4, 0), (8, 0)) if precision == "fp64" else ((16, 0), (8, 1), (4, 1), (8, 0))
+    # a step that changes only the fused setting must re-capture too
+    shapes = ((16, 0), (8, 0), (8, 1), (4, 0)) if precision == "fp64" else ((16, 0), (8, 1), (4, 1), (4, 0))
     for step, (w, fz) in enumerate(shapes):
         say(f"ensemble {precision} step {step}: members at {w} waves, fused {fz}")
         for m, s in zip(ens.members, singles):
