@@ -66,14 +66,16 @@ def substep_model_bytes(cfg, s: int = 8) -> float:
 
 
 def time_path(cfg, phys, precision: str, steps: int, warmup: int, kernel_timing: bool, zdft: bool = False,
-              tune: bool = False):
+              tune: bool = False, graph_pass: bool = True):
     """Initial state -> warm-up -> [autotune] -> timed steps; returns (seconds,
     kernel times, channel); the autotuner's choice is in channel.tuned.
 
     With kernel timing the timed steps run eagerly (an event pair around each
     hot kernel on the context's stream, so no captured graphs); the same
     number of steps is then timed again with the per-stage graphs and no
-    events, the production path, into channel.graph_elapsed."""
+    events, the production path, into channel.graph_elapsed (graph_pass=False
+    skips it: a profiler's trace then holds only the event-bracketed
+    launches)."""
     from diablo_amd import gpu
     g = gpu.GpuChannel(cfg, precision=precision)
     g.initialize(*phys)
@@ -93,7 +95,7 @@ def time_path(cfg, phys, precision: str, steps: int, warmup: int, kernel_timing:
     kt = g.kernel_times() if kernel_timing else {}
     g.set_kernel_timing(False)
     g.graph_elapsed = None
-    if kernel_timing:
+    if kernel_timing and graph_pass:
         g.step(1)  # captures the stage graphs outside the timed steps
         g.synchronize()
         t0 = time.perf_counter()
@@ -324,6 +326,8 @@ def main():
     ap.add_argument("--cpu-single-substeps", type=int, default=1,
                     help="substeps of the one-core CPU baseline (0: skip)")
     ap.add_argument("--no-kernel-timing", action="store_true")
+    ap.add_argument("--graph-pass", choices=("on", "off"), default="on",
+                    help="time the steps again with the per-stage graphs (off: only the event-timed steps run)")
     ap.add_argument("--mixed", choices=("on", "off"), default="on")
     ap.add_argument("--accuracy-steps", type=int, default=1000,
                     help="steps of the mixed-vs-FP64 accuracy run (the 1000-step criterion)")
@@ -355,7 +359,7 @@ def main():
     phys = case.initial_physical(cfg)
     log(f"[bench] FP64: warmup {args.warmup}, timing {args.steps} steps")
     elapsed, kt, g = time_path(cfg, phys, "fp64", args.steps, args.warmup, not args.no_kernel_timing,
-                               tune=args.autotune == "on")
+                               tune=args.autotune == "on", graph_pass=args.graph_pass == "on")
     tuned = {"fp64": g.tuned}
     substeps = 3 * args.steps
     value = cfg.points * substeps / elapsed / 1e6
@@ -402,7 +406,7 @@ def main():
     if args.mixed == "on":
         log(f"[bench] mixed: warmup {args.warmup}, timing {args.steps} steps")
         el_m, kt_m, gm = time_path(cfg, phys, "mixed", args.steps, args.warmup, not args.no_kernel_timing,
-                                   tune=args.autotune == "on")
+                                   tune=args.autotune == "on", graph_pass=args.graph_pass == "on")
         tuned["mixed"] = gm.tuned
         if graphs and gm.graph_elapsed:
             vg = cfg.points * substeps / gm.graph_elapsed / 1e6
