@@ -130,6 +130,8 @@ struct dgpu_ens_group {
     void *d_col[3] = {nullptr, nullptr, nullptr};   // column descriptors per RK stage
     void *d_cols[3] = {nullptr, nullptr, nullptr};  // low-mode columns (FP32 state)
     std::vector<char> col_host[3], cols_host[3];
+    hipStream_t stream = nullptr;  // the group's own stream (several groups)
+    hipEvent_t done = nullptr;
 };
 
 struct dgpu_ensemble {
@@ -146,6 +148,10 @@ struct dgpu_ensemble {
     std::vector<dgpu_ens_group> groups;
     hipStream_t side = nullptr;
     hipEvent_t ev_f = nullptr, ev_j = nullptr;
+    // several groups: each runs its launches on its own stream, forked from
+    // and joined back into the ensemble stream (DGPU_ENS_STREAMS=0: one stream)
+    bool group_streams = false;
+    hipEvent_t gfork = nullptr;
     std::string err;
 };
 }
@@ -2196,6 +2202,15 @@ static int ens_build_groups(dgpu_ensemble *e)
             hipEventCreateWithFlags(&e->ev_j, hipEventDisableTiming) != hipSuccess)
             return DGPU_E_HIP;
     }
+    const char *es = std::getenv("DGPU_ENS_STREAMS");
+    if (e->groups.size() > 1 && !(es && std::atoi(es) == 0)) {
+        if (hipEventCreateWithFlags(&e->gfork, hipEventDisableTiming) != hipSuccess) return DGPU_E_HIP;
+        for (auto &gr : e->groups)
+            if (hipStreamCreateWithFlags(&gr.stream, hipStreamNonBlocking) != hipSuccess ||
+                hipEventCreateWithFlags(&gr.done, hipEventDisableTiming) != hipSuccess)
+                return DGPU_E_HIP;
+        e->group_streams = true;
+    }
     e->batched = true;
     return DGPU_OK;
 }
@@ -2280,11 +2295,21 @@ static int ens_enqueue_batched(dgpu_ensemble *e, int rk_step)
 {
     const int k = rk_step - 1;
     const bool mixed = e->m[0]->mixed, f32 = e->m[0]->f32state;
-    hipStream_t s = e->main;
+    const bool par = e->group_streams;
+    // with several groups each group's chain runs on its own stream (the
+    // groups share nothing), so the small groups' launches fill the GPU
+    // beside the large ones'
+    if (par) {
+        if (hipEventRecord(e->gfork, e->main) != hipSuccess) return DGPU_E_HIP;
+        for (auto &gr : e->groups)
+            if (hipStreamWaitEvent(gr.stream, e->gfork, 0) != hipSuccess) return DGPU_E_HIP;
+    }
+    auto gs = [&](const dgpu_ens_group &gr) { return par ? gr.stream : e->main; };
     for (auto &gr : e->groups) {
         dgpu_ctx *c0 = e->m[gr.idx[0]];
         const dgpu::Geo &g = c0->g;
         const int nm = (int)gr.idx.size(), nin = 3 + g.nth;
+        hipStream_t s = gs(gr);
         if (!mixed) {
             const cx<double> *twx = c0->d_iptw, *twz = c0->d_iptw + c0->ipx.twn;
             dgpu::launch_zinv_ip_batch<double, double>(s, g, c0->ipz, twz,
@@ -2314,30 +2339,34 @@ static int ens_enqueue_batched(dgpu_ensemble *e, int rk_step)
         if (hipGetLastError() != hipSuccess) return DGPU_E_HIP;
     }
     if (f32) {
-        // the side stream waits for every group's nonlinear terms
-        if (hipEventRecord(e->ev_f, s) != hipSuccess || hipStreamWaitEvent(e->side, e->ev_f, 0) != hipSuccess)
+        // the low-mode FP64 columns after the nonlinear terms: on each
+        // group's stream, or on the side stream once every group has them
+        if (!par && (hipEventRecord(e->ev_f, e->main) != hipSuccess ||
+                     hipStreamWaitEvent(e->side, e->ev_f, 0) != hipSuccess))
             return DGPU_E_HIP;
         for (auto &gr : e->groups) {
-            dgpu::Geo gs = e->m[gr.idx[0]]->g;
-            gs.ps = e->m[gr.idx[0]]->nside;
+            hipStream_t ss = par ? gr.stream : e->side;
+            dgpu::Geo gsd = e->m[gr.idx[0]]->g;
+            gsd.ps = e->m[gr.idx[0]]->nside;
             const int nm = (int)gr.idx.size();
             const int max_ny = gr.max_planes - 2;
             const auto *d = (const dgpu::ColDesc<double, float> *)gr.d_cols[k];
             if (const int nw = side_waves(e->m[gr.idx[0]], max_ny)) {
-                dgpu::launch_cn_seg_batch<double, float>(e->side, nw, gs, max_ny, d, nm);
-                dgpu::launch_remdiv_seg_batch<double, float>(e->side, nw, gs, max_ny, d, nm);
+                dgpu::launch_cn_seg_batch<double, float>(ss, nw, gsd, max_ny, d, nm);
+                dgpu::launch_remdiv_seg_batch<double, float>(ss, nw, gsd, max_ny, d, nm);
             } else {
-                dgpu::launch_cn_side_batch<double, float>(e->side, gs, d, nm);
-                dgpu::launch_remdiv_side_batch<double, float>(e->side, gs, d, nm);
+                dgpu::launch_cn_side_batch<double, float>(ss, gsd, d, nm);
+                dgpu::launch_remdiv_side_batch<double, float>(ss, gsd, d, nm);
             }
         }
         if (hipGetLastError() != hipSuccess) return DGPU_E_HIP;
-        if (hipEventRecord(e->ev_j, e->side) != hipSuccess) return DGPU_E_HIP;
+        if (!par && hipEventRecord(e->ev_j, e->side) != hipSuccess) return DGPU_E_HIP;
     }
     for (auto &gr : e->groups) {
         const dgpu::Geo &g = e->m[gr.idx[0]]->g;
         const int nm = (int)gr.idx.size();
         const int max_ny = gr.max_planes - 2;
+        hipStream_t s = gs(gr);
         if (!mixed) {
             const auto *d = (const dgpu::ColDesc<double, double> *)gr.d_col[k];
             if (const int nf = fused_waves(e->m[gr.idx[0]], max_ny)) {
@@ -2373,14 +2402,21 @@ static int ens_enqueue_batched(dgpu_ensemble *e, int rk_step)
         if (hipGetLastError() != hipSuccess) return DGPU_E_HIP;
     }
     if (f32) {
-        if (hipStreamWaitEvent(s, e->ev_j, 0) != hipSuccess) return DGPU_E_HIP;
+        // CU1-3, CTH and CP of the low modes into the working state, after
+        // both column solves
+        if (!par && hipStreamWaitEvent(e->main, e->ev_j, 0) != hipSuccess) return DGPU_E_HIP;
         for (auto &gr : e->groups) {
             const dgpu_ctx *c0 = e->m[gr.idx[0]];
-            dgpu::launch_side_scatter_batch<double, float>(s, (const dgpu::ScatDesc<double, float> *)gr.d_scat,
+            dgpu::launch_side_scatter_batch<double, float>(gs(gr), (const dgpu::ScatDesc<double, float> *)gr.d_scat,
                                                            (int)gr.idx.size(), c0->nside, c0->g.ps, gr.max_planes,
                                                            4 + c0->g.nth);
         }
         if (hipGetLastError() != hipSuccess) return DGPU_E_HIP;
+    }
+    if (par) {
+        for (auto &gr : e->groups)
+            if (hipEventRecord(gr.done, gr.stream) != hipSuccess || hipStreamWaitEvent(e->main, gr.done, 0) != hipSuccess)
+                return DGPU_E_HIP;
     }
     return DGPU_OK;
 }
@@ -2398,6 +2434,11 @@ int32_t dgpu_ensemble_destroy(dgpu_ensemble *e)
     if (e->ev_f) hipEventDestroy(e->ev_f);
     if (e->ev_j) hipEventDestroy(e->ev_j);
     for (auto &gr : e->groups) {
+        if (gr.stream) {
+            hipStreamSynchronize(gr.stream);
+            hipStreamDestroy(gr.stream);
+        }
+        if (gr.done) hipEventDestroy(gr.done);
         void *bufs[] = {gr.d_zinv, gr.d_xf, gr.d_zf, gr.d_scat, gr.d_col[0], gr.d_col[1], gr.d_col[2],
                         gr.d_cols[0], gr.d_cols[1], gr.d_cols[2]};
         for (void *b : bufs)
@@ -2409,6 +2450,7 @@ int32_t dgpu_ensemble_destroy(dgpu_ensemble *e)
     for (auto ev : e->join)
         if (ev) hipEventDestroy(ev);
     if (e->fork) hipEventDestroy(e->fork);
+    if (e->gfork) hipEventDestroy(e->gfork);
     if (e->main) hipStreamDestroy(e->main);
     delete e;
     return DGPU_OK;
