@@ -105,3 +105,36 @@ def test_ensemble_member_courant(precision):
     ens.synchronize()
     _same(cfgs, ens, singles, "step after COURANT")
     ens.close()
+
+
+@pytest.mark.parametrize("graphs", [False, True])
+def test_ensemble_group_streams(monkeypatch, graphs):
+    """The member groups (here two: NX = 24 and 48) run on streams of their
+    own, forked from and joined into the ensemble stream; three steps back
+    to back with no synchronisation between them, eagerly and through the
+    stage graphs, give each member's single run bit for bit, and the same
+    bits as the ensemble on one stream (DGPU_ENS_STREAMS=0)."""
+    from diablo_amd import gpu
+    cfgs = _cfgs()
+    runs = []
+    for one in (False, True):
+        if one:
+            monkeypatch.setenv("DGPU_ENS_STREAMS", "0")
+        else:
+            monkeypatch.delenv("DGPU_ENS_STREAMS", raising=False)
+        ens, singles = _pair(gpu, cfgs, "mixed")
+        ens.set_graphs(graphs)
+        say(f"ensemble, graphs {graphs}, one stream {one}")
+        ens.step(3)
+        ens.synchronize()
+        for g in singles:
+            g.step(3)
+            g.synchronize()
+        _same(cfgs, ens, singles, f"graphs {graphs} one stream {one}")
+        runs.append([m.state() for m in ens.members])
+        ens.close()
+        for g in singles:
+            g.close()
+    for a, b in zip(*runs):
+        for k in FIELDS:
+            assert np.array_equal(a[k], b[k]), k
