@@ -337,7 +337,12 @@ int32_t dgpu_kernel_times(dgpu_ctx *ctx, double *ms, int64_t *count, int32_t n);
  * branches are the members.  h_bar, beta_bar and zeta_bar hold 3 values per
  * member (member-major).  Members are ordinary contexts for every other call
  * (dgpu_ensemble_member): set/get fields, initial transforms, REM_DIV.
- * Ordering: the ensemble's launches run on a stream of its own.  Call
+ * Members of one (NX, NZ, N_TH) group share batched launches (one per
+ * kernel kind); with several groups each group's launches run on a stream
+ * of its own, forked from and joined into the ensemble's stream
+ * (environment DGPU_ENS_STREAMS=0 at dgpu_ensemble_create: one stream; the
+ * same bits either way).
+ * Ordering: the ensemble's launches run on streams of its own.  Call
  * dgpu_ensemble_synchronize before any call on a member after
  * dgpu_ensemble_rk_chan_1, and finish work enqueued on a member
  * (dgpu_synchronize on it) before the next dgpu_ensemble_rk_chan_1.  A new
