@@ -181,9 +181,9 @@ struct IpLen {
     int twn;                    // length of the whole twiddle table
     int e;                      // values per thread (multiple of every R_i)
     int t;                      // threads per transform = n / e
-    unsigned swz;               // LDS swizzle: 4-bit XOR mask of bits 0..3, per value of bits swzs..swzs+2
+    unsigned swz;               // row swizzle: the mask of pos >> swzs XORed into pos (0: none)
     int cswz;                   // column-batch swizzle: bit 0 of a position XORed with bit cswz (0: none)
-    int swzs;                   // the swizzle's selector shift: 3 for 16-byte slots, 3 or 4 for 8-byte ones
+    int swzs;                   // the row swizzle's shift
 };
 
 // Slot of position pos in a column batch (lds[slot * W + column]): swapping
@@ -194,13 +194,15 @@ DGPU_HD int ip_cswz(const IpLen &f, int pos)
     return f.cswz ? (pos ^ ((pos >> f.cswz) & 1)) : pos;
 }
 
-// LDS slot of position pos of a row: the low four bits are XORed with a mask
-// chosen by the three bits from swzs up, which spreads the strided accesses
-// of the late stages over the banks (the host checks that the map is a
-// bijection of 0..n-1).
+// LDS slot of position pos of a row: pos XORed with its own higher bits
+// (pos >> swzs, masked), which spreads the strided accesses of the late
+// stages over the banks.  A map x -> x ^ ((x >> s) & m) is invertible (the
+// bits it changes are recovered from the higher ones, top down), and with
+// m < 32 it stays inside each 32-aligned block of positions (the host
+// checks that it is a bijection of 0..n-1).  Two instructions per access.
 DGPU_HD int ip_swz(const IpLen &f, int pos)
 {
-    return pos ^ (int)((f.swz >> (((pos >> f.swzs) & 7) << 2)) & 15u);
+    return pos ^ ((pos >> f.swzs) & (int)f.swz);
 }
 
 // Twiddle + butterfly of one group (values v[0..R-1] at positions
@@ -298,12 +300,12 @@ struct ConstPlanSpec {
 };
 constexpr ConstPlanSpec kConstPlanSpecs[] = {
     {0, 0, 0, 0u, 0},
-    // x plans (rows of the fused nonlinear kernel)
-    {384, 12, 6, 0xfda87520u, 0},  // 1: (6,4,4,4)
-    {192, 12, 6, 0xfda87520u, 0},  // 2: (6,4,4,2)
-    {96, 12, 6, 0xfda87520u, 0},   // 3: (6,4,4)
-    {128, 8, 4, 0x76543210u, 0},   // 4: (4,8,4)
-    {64, 16, 8, 0xfda87520u, 0},   // 5: (8,8), FP32
+    // x plans (rows of the fused nonlinear kernel); row swizzle mask, shift
+    {384, 12, 6, 15u, 0, 2},  // 1: (6,4,4,4), FP64
+    {192, 12, 6, 15u, 0, 2},  // 2: (6,4,4,2), FP32
+    {96, 12, 6, 14u, 0, 1},   // 3: (6,4,4), FP32
+    {128, 8, 4, 15u, 0, 3},   // 4: (4,8,4), FP64
+    {64, 16, 8, 12u, 0, 1},   // 5: (8,8), FP32
     // z plans (column batches)
     {384, 12, 0, 0u, 2},           // 6: (6,4,4,4), FP64
     {384, 24, 0, 0u, 0},           // 7: (8,8,6), FP32 (DGPU_FFT_EZ=24)
@@ -312,13 +314,14 @@ constexpr ConstPlanSpec kConstPlanSpecs[] = {
     {128, 8, 0, 0u, 0},            // 10: (8,8,2)
     {192, 24, 0, 0u, 0},           // 11: (8,8,3), FP32 (DGPU_FFT_EZ=24)
     // x plans appended after the z plans (their ids stay)
-    {256, 8, 4, 0x76543210u, 0},   // 12: (4,8,8)
-    // FP32 x plans with the swizzle picked for 8-byte slots
-    {256, 8, 4, 0x7654ba98u, 0, 4},  // 13: (4,8,8), FP32
-    {128, 8, 4, 0xfda87520u, 0, 4},  // 14: (4,8,4), FP32
+    {256, 8, 4, 15u, 0, 3},   // 12: (4,8,8), FP64
+    // FP32 x plans (8-byte slots)
+    {256, 8, 4, 31u, 0, 3},   // 13: (4,8,8), FP32
+    {128, 8, 4, 31u, 0, 2},   // 14: (4,8,4), FP32
     // z plans at up to 12 values per thread
     {192, 12, 0, 0u, 0},           // 15: (6,4,4,2)
     {256, 8, 0, 0u, 2},            // 16: (8,8,4)
+    {384, 12, 6, 31u, 0, 2},       // 17: (6,4,4,4), FP32 x plan
 };
 constexpr int kNumConstPlans = (int)(sizeof(kConstPlanSpecs) / sizeof(kConstPlanSpecs[0]));
 

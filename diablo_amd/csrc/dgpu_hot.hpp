@@ -39,7 +39,7 @@ __device__ constexpr IpLen kDevConstPlans[] = {const_plan(0),  const_plan(1),  c
                                                const_plan(4),  const_plan(5),  const_plan(6),  const_plan(7),
                                                const_plan(8),  const_plan(9),  const_plan(10), const_plan(11),
                                                const_plan(12), const_plan(13), const_plan(14), const_plan(15),
-                                               const_plan(16)};
+                                               const_plan(16), const_plan(17)};
 static_assert(sizeof(kDevConstPlans) / sizeof(IpLen) == (size_t)kNumConstPlans, "one entry per compile-time plan");
 
 template <int PID>
@@ -76,10 +76,10 @@ static inline int z_lw(int t)
 template <int... P>
 struct PidList {};
 template <typename T>
-using XPids = std::conditional_t<sizeof(T) == 4, PidList<1, 2, 3, 4, 5, 12, 13, 14>, PidList<1, 12>>;
+using XPids = std::conditional_t<sizeof(T) == 4, PidList<2, 3, 5, 13, 14, 17>, PidList<1, 12>>;
 // the single-context x kernel: the 384 and 256 plans
 template <typename T>
-using XPidsSingle = std::conditional_t<sizeof(T) == 4, PidList<1, 12, 13>, PidList<1, 12>>;
+using XPidsSingle = std::conditional_t<sizeof(T) == 4, PidList<13, 17>, PidList<1, 12>>;
 template <typename T>
 using ZPids = PidList<6, 8, 9, 10, 15, 16>;
 

@@ -274,14 +274,14 @@ __device__ __forceinline__ void xfused_body(const IpLen &fx, const cx<T> *__rest
 template <typename T>
 constexpr bool xf_multi(int pid)
 {
-    return sizeof(T) == 8 || !(pid == 1 || pid == 12 || pid == 13);
+    return sizeof(T) == 8 || !(pid == 13 || pid == 17);
 }
 
 // FP32 with the 384 plan: four resident blocks per CU (LDS holds four FP32
 // windows at NX = 384, 40 KB each) at the cost of a few spilled registers;
 // FP64 stays at the two its LDS allows.
 template <typename T, int E, int NTH, int PID>
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && PID == 1 ? 4 : 2)))
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && PID == 17 ? 4 : 2)))
 k_xfused_ip(IpLen fx, const cx<T> *__restrict__ twg, int mk, int ld,
                                                     const cx<T> *__restrict__ zt, long zt_fs, long zt_ps,
                                                     cx<T> *__restrict__ wt, long wt_fs, long wt_ps,

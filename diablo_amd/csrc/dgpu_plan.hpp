@@ -81,89 +81,71 @@ inline void ip_twiddles(const IpLen &f, std::vector<cx<double>> &tw)
 
 // LDS cost of a plan's stage accesses for one row layout (positions of a row
 // at x[swz(pos)], rows of n values of wpe 4-byte words back to back: 4 for
-// complex FP64, 2 for complex FP32; 64 lanes a wavefront, T lanes a row):
-// the sum over the two bank models that bound what the hardware does (32 or
-// 64 four-byte banks, as many lanes a cycle as fill them), relative to
-// conflict-free.
-inline double ip_bank_cost(const IpLen &f, unsigned swz, int swzs = 3, int wpe = 4)
+// complex FP64, 2 for complex FP32; 64 lanes a wavefront, T lanes a row),
+// relative to conflict-free, for `banks` four-byte banks serving as many
+// lanes a cycle as fill them.  64 banks is what MI355X shows: the FP32 x
+// kernel's LDS bank-conflict cycles (34% of its LDS cycles) are what this
+// model gives its former swizzle (1.75; conflict-free with 32 banks).
+inline double ip_bank_cost(const IpLen &f, unsigned swz, int swzs, int wpe, int banks)
 {
     IpLen g = f;
     g.swz = swz;
     g.swzs = swzs;
     const int T = f.t;
+    const int lpc = banks / wpe;
     double tot = 0, ideal = 0;
-    const int bank_models[2] = {32, 64};
     for (int i = 0; i < f.nstages; ++i) {
         const int R = f.radix[i], L = f.span[i];
         for (int u = 0; u < f.e / R; ++u)
             for (int q = 0; q < R; ++q)
-                for (const int banks : bank_models) {
-                    const int lpc = banks / wpe;
-                    for (int g0 = 0; g0 < 64; g0 += lpc) {
-                        int cnt[64] = {0};
-                        int mx = 0;
-                        for (int lane = g0; lane < g0 + lpc; ++lane) {
-                            const int row = lane / T, th = lane % T;
-                            const int t = th + T * u, b = t / L, k = t - b * L;
-                            const long a = ((long)row * f.n + ip_swz(g, b * R * L + k + q * L)) * wpe;
-                            for (int w = 0; w < wpe; ++w) mx = std::max(mx, ++cnt[(a + w) % banks]);
-                        }
-                        tot += (double)mx;
-                        ideal += 1.0;
+                for (int g0 = 0; g0 < 64; g0 += lpc) {
+                    int cnt[64] = {0};
+                    int mx = 0;
+                    for (int lane = g0; lane < g0 + lpc; ++lane) {
+                        const int row = lane / T, th = lane % T;
+                        const int t = th + T * u, b = t / L, k = t - b * L;
+                        const long a = ((long)row * f.n + ip_swz(g, b * R * L + k + q * L)) * wpe;
+                        for (int w = 0; w < wpe; ++w) mx = std::max(mx, ++cnt[(a + w) % banks]);
                     }
+                    tot += (double)mx;
+                    ideal += 1.0;
                 }
     }
     return ideal > 0 ? tot / ideal : 1.0;
 }
 
-// Picks the cheapest bijective swizzle (mask table and selector shift) for
-// the rows of an x-direction plan whose values are wpe 4-byte words, from a
-// few candidates (identity first, so it wins ties), into f.swz / f.swzs.
+// Picks the row swizzle x -> x ^ ((x >> s) & m) of an x-direction plan
+// whose values are wpe 4-byte words: the fewest modelled conflicts with 64
+// banks, then with 32, over s = 1..7 and m = 1..31 (the map must be a
+// bijection of 0..n-1; none when nothing beats the plain layout).
 inline void ip_pick_swizzle(IpLen &f, int wpe = 4)
 {
-    auto table = [](int kind) {
-        unsigned w = 0;
-        for (int idx = 0; idx < 8; ++idx) {
-            const int b3 = idx & 1, b4 = (idx >> 1) & 1, b5 = (idx >> 2) & 1;
-            unsigned m = 0;
-            if (kind == 1) m = b4 | (b3 << 1) | (b4 << 2) | (b5 << 3);  // per-stage radix-4 spread
-            if (kind == 2) m = (unsigned)idx;                            // selector bits into bits 0..2
-            if (kind == 3) m = (unsigned)(idx < 4 ? idx | 8 : idx);      // as 2, bit 3 set for the lower half
-            w |= m << (4 * idx);
-        }
-        return w;
-    };
     f.swz = 0;
     f.swzs = 3;
-    if (f.n % 16) return;
-    double best_c = ip_bank_cost(f, 0, 3, wpe);
-    struct Cand {
-        unsigned w;
-        int sh;
-    };
-    // 16-byte slots: selector bits 3..5; 8-byte slots also try bits 4..6
-    const Cand cands[] = {{table(1), 3}, {table(2), 3}, {table(1), 4}, {table(2), 4}, {table(3), 4}};
-    const int ncand = wpe == 4 ? 2 : 5;
-    for (int k = 0; k < ncand; ++k) {
-        const Cand cd = cands[k];
-        IpLen g = f;
-        g.swz = cd.w;
-        g.swzs = cd.sh;
-        std::vector<char> seen(f.n, 0);
-        bool ok = true;
-        for (int p = 0; p < f.n && ok; ++p) {
-            const int q = ip_swz(g, p);
-            if (q < 0 || q >= f.n || seen[q]) ok = false;
-            else seen[q] = 1;
+    double best64 = ip_bank_cost(f, 0, 3, wpe, 64), best32 = ip_bank_cost(f, 0, 3, wpe, 32);
+    for (int sh = 1; sh <= 7; ++sh)
+        for (unsigned m = 1; m < 32; ++m) {
+            IpLen g = f;
+            g.swz = m;
+            g.swzs = sh;
+            std::vector<char> seen(f.n, 0);
+            bool ok = true;
+            for (int p = 0; p < f.n && ok; ++p) {
+                const int q = ip_swz(g, p);
+                if (q < 0 || q >= f.n || seen[q]) ok = false;
+                else seen[q] = 1;
+            }
+            if (!ok) continue;
+            const double c64 = ip_bank_cost(f, m, sh, wpe, 64);
+            if (c64 > best64 + 1e-9) continue;
+            const double c32 = ip_bank_cost(f, m, sh, wpe, 32);
+            if (c64 < best64 - 1e-9 || c32 < best32 - 1e-9) {
+                best64 = c64;
+                best32 = c32;
+                f.swz = m;
+                f.swzs = sh;
+            }
         }
-        if (!ok) continue;
-        const double c = ip_bank_cost(f, cd.w, cd.sh, wpe);
-        if (c < best_c - 1e-9) {
-            best_c = c;
-            f.swz = cd.w;
-            f.swzs = cd.sh;
-        }
-    }
 }
 
 // Same model for a column batch (lds[slot * W + b], lanes b = lane % W
