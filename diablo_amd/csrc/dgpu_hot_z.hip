@@ -118,7 +118,8 @@ __device__ __forceinline__ void zfwd_body(const IpLen &fz, const cx<T> *__restri
     cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
     const int W = 1 << lw;
     const int tid = threadIdx.x;
-    const int b = tid & (W - 1), th = tid >> lw;
+    const int b = tid & (W - 1);
+    int th = tid >> lw;  // re-laundered every stage (see the stage loops)
     const int i = bx * W + b;
     const bool col = i < mk;
     const int j = jlo + by;
@@ -155,6 +156,9 @@ __device__ __forceinline__ void zfwd_body(const IpLen &fz, const cx<T> *__restri
     first(gin);
     __syncthreads();
     stage_loop<CP>(1, S - 1, [&](int s) {
+        // keeps the compiler from hoisting every stage's index arithmetic
+        // out of the unrolled stages (both transforms' values are live)
+        asm volatile("" : "+v"(th));
         ip_stage_mem<T, E, -1, false>(x, W, fz, S - 1 - s, tw, th);
         __syncthreads();
     });
@@ -167,6 +171,9 @@ __device__ __forceinline__ void zfwd_body(const IpLen &fz, const cx<T> *__restri
     first(hin);
     __syncthreads();
     stage_loop<CP>(1, S - 1, [&](int s) {
+        // keeps the compiler from hoisting every stage's index arithmetic
+        // out of the unrolled stages (both transforms' values are live)
+        asm volatile("" : "+v"(th));
         ip_stage_mem<T, E, -1, false>(x, W, fz, S - 1 - s, tw, th);
         __syncthreads();
     });
@@ -280,8 +287,11 @@ template <typename T, typename TOUT>
 void launch_zfwd_nl_ip(hipStream_t s, const Geo &g, const IpLen &fz, const cx<T> *tw, const cx<T> *wt, long wt_fs,
                        int ld, cx<TOUT> *const *nl, const double *kz, int jlo, int jhi, int nout)
 {
-    // runtime plans only: with the stages unrolled the compiler keeps more of
-    // the two transforms in registers and the kernel loses occupancy
+    if (with_pid(const_plan_id(fz), ZPids<T>{}, [&](auto pc) {
+            constexpr int P = decltype(pc)::value;
+            zfwd_nl_ip_e<T, kConstPlanSpecs[P].e, TOUT, P>(s, g, fz, tw, wt, wt_fs, ld, nl, kz, jlo, jhi, nout);
+        }))
+        return;
     DGPU_DISPATCH_IPE(fz.e, (zfwd_nl_ip_e<T, EE, TOUT, 0>(s, g, fz, tw, wt, wt_fs, ld, nl, kz, jlo, jhi, nout)));
 }
 
@@ -324,6 +334,11 @@ template <typename T, typename TOUT>
 void launch_zfwd_nl_ip_batch(hipStream_t s, const Geo &g, const IpLen &fz, const cx<T> *tw, const ZfDesc<T, TOUT> *d,
                              int nmem, int max_planes, int ld, int nout)
 {
+    if (with_pid(const_plan_id(fz), ZPids<T>{}, [&](auto pc) {
+            constexpr int P = decltype(pc)::value;
+            zfwd_nl_ip_batch_e<T, kConstPlanSpecs[P].e, TOUT, P>(s, g, fz, tw, d, nmem, max_planes, ld, nout);
+        }))
+        return;
     DGPU_DISPATCH_IPE(fz.e, (zfwd_nl_ip_batch_e<T, EE, TOUT, 0>(s, g, fz, tw, d, nmem, max_planes, ld, nout)));
 }
 
