@@ -173,33 +173,8 @@ int fail(dgpu_ctx *c, int code, const std::string &msg)
 
 bool make_len(int n, dgpu::FftLen &f)
 {
-    f.n = n;
-    f.nstages = 0;
-    int m = n;
-    bool has2 = false, has3 = false, has4 = false;
-    while (m % 4 == 0 && f.nstages < dgpu::kMaxStages) { f.radix[f.nstages++] = 4; m /= 4; has4 = true; }
-    while (m % 2 == 0 && f.nstages < dgpu::kMaxStages) { f.radix[f.nstages++] = 2; m /= 2; has2 = true; }
-    while (m % 3 == 0 && f.nstages < dgpu::kMaxStages) { f.radix[f.nstages++] = 3; m /= 3; has3 = true; }
-    if (m != 1 || n < 4) return false;
-    int l = has4 ? 4 : (has2 ? 2 : 1);
-    if (has3) l = (l == 4) ? 12 : (l == 2 ? 6 : 3);
-    const int cand[5] = {4, 6, 8, 12, 16};
-    int best = -1;
-    for (int e : cand) {
-        if (e % l || n % e) continue;
-        if (best < 0) best = e;
-        if (n / e <= 64) { best = e; break; }
-        best = e;
-    }
-    if (best < 0) return false;
-    f.e = best;
-    f.t = n / best;
-    unsigned long long ns = 1;
-    for (int s = 0; s < f.nstages; ++s) {
-        f.magic[s] = (ns == 1) ? 0u : (unsigned)(((1ULL << 32) + ns - 1) / ns);
-        ns *= (unsigned long long)f.radix[s];
-    }
-    return true;
+    f = dgpu::fft_len_of(n);
+    return f.nstages > 0;
 }
 
 void twiddles(int n, std::vector<cx<double>> &out)

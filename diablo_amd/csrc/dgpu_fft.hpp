@@ -472,15 +472,59 @@ __device__ __forceinline__ void stage(cx<T> *lds, int stride, int n, int ns, uns
     else __syncthreads();
 }
 
+// Stockham plan of length n: radix-4 stages, then 2s, then 3s.  E (values
+// per thread) is the first of 4, 6, 8, 12, 16 that divides n, is a multiple
+// of every radix and leaves at most 64 threads a transform, else the largest
+// such.  nstages == 0: no plan (n < 4 or n has another prime factor).
+// The host plans with it; the kernels also take it at compile time.
+constexpr FftLen fft_len_of(int n)
+{
+    FftLen f{};
+    f.n = n;
+    if (n < 4) return f;
+    int m = n, ns = 0;
+    bool has2 = false, has3 = false, has4 = false;
+    while (m % 4 == 0 && ns < kMaxStages) { f.radix[ns++] = 4; m /= 4; has4 = true; }
+    while (m % 2 == 0 && ns < kMaxStages) { f.radix[ns++] = 2; m /= 2; has2 = true; }
+    while (m % 3 == 0 && ns < kMaxStages) { f.radix[ns++] = 3; m /= 3; has3 = true; }
+    if (m != 1) return FftLen{};
+    int l = has4 ? 4 : (has2 ? 2 : 1);
+    if (has3) l = (l == 4) ? 12 : (l == 2 ? 6 : 3);
+    const int cand[5] = {4, 6, 8, 12, 16};
+    int best = -1;
+    for (int e : cand) {
+        if (e % l || n % e) continue;
+        best = e;
+        if (n / e <= 64) break;
+    }
+    if (best < 0) return FftLen{};
+    f.nstages = ns;
+    f.e = best;
+    f.t = n / best;
+    unsigned long long p = 1;
+    for (int s = 0; s < ns; ++s) {
+        f.magic[s] = (p == 1) ? 0u : (unsigned)(((1ULL << 32) + p - 1) / p);
+        p *= (unsigned long long)f.radix[s];
+    }
+    return f;
+}
+
+// Lengths the standalone transforms are also built for (FID > 0).
+constexpr int kConstFftN[] = {0, 384, 256};
+constexpr int kNumConstFft = (int)(sizeof(kConstFftN) / sizeof(kConstFftN[0]));
+
 // Full transform.  Every thread of the block must call this (it contains
 // barriers); threads with active == false only take part in the barriers.
 // With WS the transform synchronises within its wavefront only, and the
 // caller places a block barrier before other wavefronts read the result.
+// With a plan known at compile time the stage loop unrolls and every
+// radix, span and reciprocal is a constant.
 template <typename T, int E, int SIGN, bool WS = false>
-__device__ void fft_lds(cx<T> *lds, int stride, const FftLen &f, int th, bool active,
-                        const cx<T> *__restrict__ tw)
+__device__ __forceinline__ void fft_lds(cx<T> *lds, int stride, const FftLen &f, int th, bool active,
+                                        const cx<T> *__restrict__ tw)
 {
     int ns = 1;
+#pragma unroll
     for (int s = 0; s < f.nstages; ++s) {
         const int R = f.radix[s];
         const unsigned mg = f.magic[s];

@@ -4,10 +4,27 @@
 // side of these transforms is in natural order (Fortran-layout planes).  The
 // substep's own transforms are in dgpu_hot_z.hip and dgpu_hot_x.hip.
 #include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <type_traits>
 
 #include "dgpu_kernels.hpp"
 
 namespace dgpu {
+
+// Stockham plans the standalone transforms are also built for (FID > 0).
+__device__ constexpr FftLen kDevConstFft[] = {fft_len_of(kConstFftN[0]), fft_len_of(kConstFftN[1]),
+                                              fft_len_of(kConstFftN[2])};
+static_assert(sizeof(kDevConstFft) / sizeof(FftLen) == (size_t)kNumConstFft, "one entry per compile-time plan");
+
+template <int FID>
+__device__ __forceinline__ const FftLen &fft_ref(const FftLen &rt)
+{
+    if constexpr (FID == 0)
+        return rt;
+    else
+        return kDevConstFft[FID];
+}
 
 // -------------------------------------------------------------------------
 // helpers
@@ -28,11 +45,12 @@ __device__ __forceinline__ cx<TO> cvt(cx<FROM> a) { return make_cx<TO>(TO(a.re),
 // for/mpi.f:1627-1641): unpack, zero the gap, unnormalised backward C2C.
 // grid (col-chunks, planes, fields); block W*T threads; LDS W*NZ complex.
 // -------------------------------------------------------------------------
-template <typename T, int E, typename TIN>
-__global__ void k_zinv(FftLen fz, const cx<T> *__restrict__ tw, const cx<TIN> *__restrict__ in, long in_fs,
+template <typename T, int E, typename TIN, int FID>
+__global__ void k_zinv(FftLen fz_rt, const cx<T> *__restrict__ tw, const cx<TIN> *__restrict__ in, long in_fs,
                        long in_ps, int in_mk, int ncols, cx<T> *__restrict__ out, long out_fs, long out_ps,
                        int out_mk, int j0, int nkz, int tnkz, int lw)
 {
+    const FftLen &fz = fft_ref<FID>(fz_rt);
     extern __shared__ unsigned char smem_raw[];
     cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
     const int W = 1 << lw;
@@ -69,10 +87,11 @@ __global__ void k_zinv(FftLen fz, const cx<T> *__restrict__ tw, const cx<TIN> *_
 
 // R2C along x of the physical rows of Fortran-layout planes; output
 // xt[j][z][0..NKX] scaled 1/NX (for/mpi.f:1547-1561). Pairs of rows per FFT.
-template <typename T, int E>
-__global__ void k_x_r2c(FftLen fx, const cx<T> *__restrict__ tw, const double *__restrict__ phys, int nzp2,
+template <typename T, int E, int FID>
+__global__ void k_x_r2c(FftLen fx_rt, const cx<T> *__restrict__ tw, const double *__restrict__ phys, int nzp2,
                         cx<T> *__restrict__ xt, int mk, int nz, int j0)
 {
+    const FftLen &fx = fft_ref<FID>(fx_rt);
     extern __shared__ unsigned char smem_raw[];
     cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
     const int nx = fx.n;
@@ -109,10 +128,11 @@ __global__ void k_x_r2c(FftLen fx, const cx<T> *__restrict__ tw, const double *_
 
 // z-forward of xt columns, scale 1/NZ, pack kept slots into Fortran layout;
 // every other slot of the plane is set to zero.
-template <typename T, int E>
-__global__ void k_zfwd_pack(FftLen fz, const cx<T> *__restrict__ tw, const cx<T> *__restrict__ xt, int mk,
+template <typename T, int E, int FID>
+__global__ void k_zfwd_pack(FftLen fz_rt, const cx<T> *__restrict__ tw, const cx<T> *__restrict__ xt, int mk,
                             double *__restrict__ fort, int nxp, int nkz, int tnkz, int j0, int lw)
 {
+    const FftLen &fz = fft_ref<FID>(fz_rt);
     extern __shared__ unsigned char smem_raw[];
     cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
     const int W = 1 << lw;
@@ -147,10 +167,11 @@ __global__ void k_zfwd_pack(FftLen fz, const cx<T> *__restrict__ tw, const cx<T>
 // C2R along x: zt[j][z][0..mkin-1] (Nyquist zeroed, Im of DC ignored) into the
 // physical rows of Fortran-layout planes (for/mpi.f:1643-1672). The x pads of
 // rows 0..NZ-1 are set to zero; rows NZ, NZ+1 are not touched.
-template <typename T, int E>
-__global__ void k_x_c2r(FftLen fx, const cx<T> *__restrict__ tw, const cx<T> *__restrict__ zt, int mkin,
+template <typename T, int E, int FID>
+__global__ void k_x_c2r(FftLen fx_rt, const cx<T> *__restrict__ tw, const cx<T> *__restrict__ zt, int mkin,
                         double *__restrict__ fort, int nz, int j0)
 {
+    const FftLen &fx = fft_ref<FID>(fx_rt);
     extern __shared__ unsigned char smem_raw[];
     cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
     const int nx = fx.n;
@@ -281,7 +302,33 @@ static inline int chunk_lw(int t, int threads)
     return lw;
 }
 
-template <typename T, int E, typename TIN>
+// Id of the compile-time Stockham plan equal to f, or 0
+// (DGPU_NO_CONST_PLANS=1: always 0).
+static int const_fft_id(const FftLen &f)
+{
+    static const bool off = std::getenv("DGPU_NO_CONST_PLANS") != nullptr;
+    if (off) return 0;
+    for (int id = 1; id < kNumConstFft; ++id) {
+        const FftLen c = fft_len_of(kConstFftN[id]);
+        if (std::memcmp(&c, &f, sizeof(FftLen)) == 0) return id;
+    }
+    return 0;
+}
+
+// Calls fn(std::integral_constant<int, FID>) for a compile-time plan id;
+// false for 0.
+template <class F>
+static bool with_fid(int fid, F &&fn)
+{
+    static_assert(kNumConstFft == 3, "one case per compile-time plan");
+    switch (fid) {
+    case 1: fn(std::integral_constant<int, 1>{}); return true;
+    case 2: fn(std::integral_constant<int, 2>{}); return true;
+    default: return false;
+    }
+}
+
+template <typename T, int E, typename TIN, int FID>
 static void zinv_e(hipStream_t s, const Geo &g, const Fft<T> &fz, const cx<TIN> *in, long in_fs, long in_ps,
                    int in_mk, int ncols, cx<T> *out, long out_fs, long out_ps, int out_mk, int j0, int nplanes,
                    int nfields)
@@ -289,7 +336,7 @@ static void zinv_e(hipStream_t s, const Geo &g, const Fft<T> &fz, const cx<TIN> 
     const int lw = chunk_lw(fz.len.t, 256), W = 1 << lw;
     dim3 grid((ncols + W - 1) / W, nplanes, nfields);
     const size_t lds = sizeof(cx<T>) * W * fz.len.n;
-    hipLaunchKernelGGL((k_zinv<T, E, TIN>), grid, dim3(W * fz.len.t), lds, s, fz.len, fz.tw, in, in_fs, in_ps, in_mk,
+    hipLaunchKernelGGL((k_zinv<T, E, TIN, FID>), grid, dim3(W * fz.len.t), lds, s, fz.len, fz.tw, in, in_fs, in_ps, in_mk,
                        ncols, out, out_fs, out_ps, out_mk, j0, g.nkz, g.tnkz, lw);
 }
 
@@ -307,11 +354,17 @@ template <typename T, typename TIN>
 void launch_zinv(hipStream_t s, const Geo &g, const Fft<T> &fz, const cx<TIN> *in, long in_fs, long in_ps, int in_mk,
                  int ncols, cx<T> *out, long out_fs, long out_ps, int out_mk, int j0, int nplanes, int nfields)
 {
-    DGPU_DISPATCH_E(fz.len.e, (zinv_e<T, EE, TIN>(s, g, fz, in, in_fs, in_ps, in_mk, ncols, out, out_fs, out_ps, out_mk,
-                                             j0, nplanes, nfields)));
+    if (with_fid(const_fft_id(fz.len), [&](auto fc) {
+            constexpr int FI = decltype(fc)::value;
+            zinv_e<T, fft_len_of(kConstFftN[FI]).e, TIN, FI>(s, g, fz, in, in_fs, in_ps, in_mk, ncols, out, out_fs,
+                                                              out_ps, out_mk, j0, nplanes, nfields);
+        }))
+        return;
+    DGPU_DISPATCH_E(fz.len.e, (zinv_e<T, EE, TIN, 0>(s, g, fz, in, in_fs, in_ps, in_mk, ncols, out, out_fs, out_ps,
+                                                out_mk, j0, nplanes, nfields)));
 }
 
-template <typename T, int E>
+template <typename T, int E, int FID>
 static void x_r2c_e(hipStream_t s, const Geo &g, const Fft<T> &fx, const double *phys, cx<T> *xt, int j0,
                     int nplanes)
 {
@@ -319,24 +372,29 @@ static void x_r2c_e(hipStream_t s, const Geo &g, const Fft<T> &fx, const double 
     if (npair < 1) npair = 1;
     const int zper = 2 * npair;
     dim3 grid((g.nz + zper - 1) / zper, nplanes);
-    hipLaunchKernelGGL((k_x_r2c<T, E>), grid, dim3(npair * fx.len.t), sizeof(cx<T>) * npair * fx.len.n, s, fx.len,
+    hipLaunchKernelGGL((k_x_r2c<T, E, FID>), grid, dim3(npair * fx.len.t), sizeof(cx<T>) * npair * fx.len.n, s, fx.len,
                        fx.tw, phys, g.nz + 2, xt, g.mk, g.nz, j0);
 }
 
 template <typename T>
 void launch_x_r2c(hipStream_t s, const Geo &g, const Fft<T> &fx, const double *phys, cx<T> *xt, int j0, int nplanes)
 {
-    DGPU_DISPATCH_E(fx.len.e, (x_r2c_e<T, EE>(s, g, fx, phys, xt, j0, nplanes)));
+    if (with_fid(const_fft_id(fx.len), [&](auto fc) {
+            constexpr int FI = decltype(fc)::value;
+            x_r2c_e<T, fft_len_of(kConstFftN[FI]).e, FI>(s, g, fx, phys, xt, j0, nplanes);
+        }))
+        return;
+    DGPU_DISPATCH_E(fx.len.e, (x_r2c_e<T, EE, 0>(s, g, fx, phys, xt, j0, nplanes)));
 }
 
-template <typename T, int E>
+template <typename T, int E, int FID>
 static void zfwd_pack_e(hipStream_t s, const Geo &g, const Fft<T> &fz, const cx<T> *xt, double *fort, int j0,
                         int nplanes)
 {
     const int lw = chunk_lw(fz.len.t, 256), W = 1 << lw;
     const int ncols = g.nx / 2 + 1;  // 0..NXP, zero beyond NKX
     dim3 grid((ncols + W - 1) / W, nplanes);
-    hipLaunchKernelGGL((k_zfwd_pack<T, E>), grid, dim3(W * fz.len.t), sizeof(cx<T>) * W * fz.len.n, s, fz.len, fz.tw,
+    hipLaunchKernelGGL((k_zfwd_pack<T, E, FID>), grid, dim3(W * fz.len.t), sizeof(cx<T>) * W * fz.len.n, s, fz.len, fz.tw,
                        xt, g.mk, fort, g.nx / 2, g.nkz, g.tnkz, j0, lw);
 }
 
@@ -344,10 +402,15 @@ template <typename T>
 void launch_zfwd_pack(hipStream_t s, const Geo &g, const Fft<T> &fz, const cx<T> *xt, double *fort, int j0,
                       int nplanes)
 {
-    DGPU_DISPATCH_E(fz.len.e, (zfwd_pack_e<T, EE>(s, g, fz, xt, fort, j0, nplanes)));
+    if (with_fid(const_fft_id(fz.len), [&](auto fc) {
+            constexpr int FI = decltype(fc)::value;
+            zfwd_pack_e<T, fft_len_of(kConstFftN[FI]).e, FI>(s, g, fz, xt, fort, j0, nplanes);
+        }))
+        return;
+    DGPU_DISPATCH_E(fz.len.e, (zfwd_pack_e<T, EE, 0>(s, g, fz, xt, fort, j0, nplanes)));
 }
 
-template <typename T, int E>
+template <typename T, int E, int FID>
 static void x_c2r_e(hipStream_t s, const Geo &g, const Fft<T> &fx, const cx<T> *zt, int mkin, double *fort, int j0,
                     int nplanes)
 {
@@ -355,7 +418,7 @@ static void x_c2r_e(hipStream_t s, const Geo &g, const Fft<T> &fx, const cx<T> *
     if (npair < 1) npair = 1;
     const int zper = 2 * npair;
     dim3 grid((g.nz + zper - 1) / zper, nplanes);
-    hipLaunchKernelGGL((k_x_c2r<T, E>), grid, dim3(npair * fx.len.t), sizeof(cx<T>) * npair * fx.len.n, s, fx.len,
+    hipLaunchKernelGGL((k_x_c2r<T, E, FID>), grid, dim3(npair * fx.len.t), sizeof(cx<T>) * npair * fx.len.n, s, fx.len,
                        fx.tw, zt, mkin, fort, g.nz, j0);
 }
 
@@ -363,7 +426,12 @@ template <typename T>
 void launch_x_c2r(hipStream_t s, const Geo &g, const Fft<T> &fx, const cx<T> *zt, int mkin, double *fort, int j0,
                   int nplanes)
 {
-    DGPU_DISPATCH_E(fx.len.e, (x_c2r_e<T, EE>(s, g, fx, zt, mkin, fort, j0, nplanes)));
+    if (with_fid(const_fft_id(fx.len), [&](auto fc) {
+            constexpr int FI = decltype(fc)::value;
+            x_c2r_e<T, fft_len_of(kConstFftN[FI]).e, FI>(s, g, fx, zt, mkin, fort, j0, nplanes);
+        }))
+        return;
+    DGPU_DISPATCH_E(fx.len.e, (x_c2r_e<T, EE, 0>(s, g, fx, zt, mkin, fort, j0, nplanes)));
 }
 
 void launch_unpack_fortran(hipStream_t s, const Geo &g, const double *fort, cx<double> *kept, int j0, int nplanes)
