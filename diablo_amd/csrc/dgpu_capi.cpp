@@ -1730,16 +1730,9 @@ static int les_substep(dgpu_ctx *c, int rk_step, double time, double temp1, doub
         if (int rc = fort_to_physical(c, rthp, 0, NY + 1)) return rc;
         if (int rc = fort_to_physical(c, thp, 0, NY + 1)) return rc;
     }
-    // the eddy-viscosity halves, explicit (for/channel.f:628-659, 748-760)
-    dgpu::launch_les_explicit(c->stream, g, Rb[0], U[0], nut, c->d_dy, c->d_dyf, temp2, 0, c->jstart, c->jend);
-    dgpu::launch_les_explicit(c->stream, g, Rb[2], U[2], nut, c->d_dy, c->d_dyf, temp2, 0, c->jstart, c->jend);
-    dgpu::launch_les_explicit(c->stream, g, Rb[1], U[1], nut, c->d_dy, c->d_dyf, temp2, 1, 2, NY);
-    if (g.nth)
-        dgpu::launch_les_explicit(c->stream, g, rthp, thp, nut, c->d_dy, c->d_dyf, temp2, 0, c->jstart_th[0],
-                                  c->jend_th[0]);
-    if (int rc = check_launch(c, "les_explicit")) return rc;
     // implicit solves: TH, U2, U1, U3 (for/channel.f:762-1029), independent,
-    // in one launch; scratch planes: 6, 7 (work planes of the model), 8, 11,
+    // in one launch, each right-hand side with its explicit eddy-viscosity
+    // half (for/channel.f:628-659, 748-760) added on the fly; scratch planes: 6, 7 (work planes of the model), 8, 11,
     // 12, and 14-16 (the SGS force, already added to the nonlinear terms)
     const double nuv = c->p.nu_v_scale;
     double *scratch[8] = {bm, gw, les_buf(c, 8), les_buf(c, 11), les_buf(c, 12), les_buf(c, 14), les_buf(c, 15),

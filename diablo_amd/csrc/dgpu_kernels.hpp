@@ -308,8 +308,6 @@ void launch_les_pack(hipStream_t s, const Geo &g, const cx<double> *src, double 
 void launch_les_mul(hipStream_t s, const Geo &g, double *a, const double *nut, int j0, int j1);
 void launch_les_acc(hipStream_t s, const Geo &g, cx<double> *dst, const double *fort, const double *kx,
                     const double *kz, int kmul, int j0, int j1);
-void launch_les_explicit(hipStream_t s, const Geo &g, double *r, const double *u, const double *nut,
-                         const double *dy, const double *dyf, double temp2, int sys, int j0, int j1);
 // Up to four independent solves (TH, U2, U1, U3), each with its right-hand
 // side, output and two scratch planes.
 struct LesCnSet {

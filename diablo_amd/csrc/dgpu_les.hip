@@ -382,38 +382,21 @@ __global__ void k_les_acc(Geo g, cx<double> *__restrict__ dst, const double *__r
     dst[o] = f;
 }
 
-// The eddy-viscosity (diffusivity) half of the Crank-Nicolson term, explicit
-// (for/channel.f:628-659, 748-760): r += temp2 d/dy(nut du/dy) on rows
-// j0..j1 of system sys (0: on GYF rows with NU_T on the faces, U1, U3 and
-// TH; 1: U2 on GY rows, NU_T averaged onto the GYF faces).
-__global__ void k_les_explicit(Geo g, double *__restrict__ r, const double *__restrict__ u,
-                               const double *__restrict__ nut, const double *__restrict__ dy,
-                               const double *__restrict__ dyf, double temp2, int sys, int j0)
-{
-    const long n = (long)g.nx * g.nz;
-    const int j = j0 + blockIdx.y;
-    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n) return;
-    const int i = (int)(e % g.nx), k = (int)(e / g.nx);
-    const long x = fidx3(i, k, j, g.nx, g.nz), xp = fidx3(i, k, j + 1, g.nx, g.nz), xm = fidx3(i, k, j - 1, g.nx, g.nz);
-    double t;
-    if (sys == 0)
-        t = (nut[xp] * (u[xp] - u[x]) / dy[j + 1] - nut[x] * (u[x] - u[xm]) / dy[j]) / dyf[j];
-    else
-        t = (0.5 * (nut[x] + nut[xp]) * (u[xp] - u[x]) / dyf[j] - 0.5 * (nut[x] + nut[xm]) * (u[x] - u[xm]) / dyf[j - 1]) /
-            dy[j];
-    r[x] = r[x] + temp2 * t;
-}
-
 // One implicit Crank-Nicolson solve in physical space per (x, z) column
 // (for/channel.f:762-1029 with the LES terms): rows jlo..jhi from the
 // molecular and eddy parts, the boundary rows of APPLY_BC_*_LOWER/UPPER
 // (for/channel.f:1660-2282), THOMAS_REAL at NPROCY = 1 (for/mpi.f:630-809),
 // and rows clo..chi written to out.  bm, gw: per-column scratch planes.
-__device__ __forceinline__ void les_row(const LesCn &a, const double *__restrict__ nut, long x, long xp, long xm,
-                                        const double *__restrict__ dy, const double *__restrict__ dyf,
-                                        const double *__restrict__ rhs, int j, double &L, double &D, double &U,
-                                        double &V)
+// With EXPL the right-hand side of rows jlo..jhi also takes the explicit
+// eddy-viscosity (diffusivity) half, temp2 d/dy(nut du/dy)
+// (for/channel.f:628-659, 748-760), from the field u at rows j-1..j+1: on
+// GYF rows with NU_T on the faces (U1, U3, TH), or on GY rows with NU_T
+// averaged onto the GYF faces (U2).
+template <bool EXPL>
+__device__ __forceinline__ void les_row(const LesCn &a, const double *__restrict__ nut, const double *u, long x,
+                                        long xp, long xm, const double *__restrict__ dy,
+                                        const double *__restrict__ dyf, const double *__restrict__ rhs, int j,
+                                        double &L, double &D, double &U, double &V)
 {
     const int NY = a.ny;
     L = 0.0; D = 1.0; U = 0.0; V = 0.0;
@@ -434,6 +417,16 @@ __device__ __forceinline__ void les_row(const LesCn &a, const double *__restrict
         }
         V = rhs[x];
         const double t2 = a.temp2;
+        if constexpr (EXPL) {
+            double t;
+            if (a.sys == LES_SYS_U2)
+                t = (0.5 * (nut[x] + nut[xp]) * (u[xp] - u[x]) / dyf[j] -
+                     0.5 * (nut[x] + nut[xm]) * (u[x] - u[xm]) / dyf[j - 1]) /
+                    dy[j];
+            else
+                t = (nut[xp] * (u[xp] - u[x]) / dy[j + 1] - nut[x] * (u[x] - u[xm]) / dy[j]) / dyf[j];
+            V = V + t2 * t;
+        }
         if (a.sys == LES_SYS_U2) {
             const double nm = nut[x] + nut[xm], np = nut[x] + nut[xp];
             L = L - t2 * 0.5 * nm / (dyf[j - 1] * dy[j]);
@@ -474,7 +467,10 @@ __device__ __forceinline__ void les_row(const LesCn &a, const double *__restrict
 }
 
 // grid (column blocks, systems): every system of the substep in one launch
-// (they are independent), so the sweeps of all four fill the machine.
+// (they are independent), so the sweeps of all four fill the machine.  The
+// system's field is read (the explicit half, forward sweep) and overwritten
+// (back substitution) through out: a column is read and written by its own
+// thread only, every read before the first write.
 __global__ void k_les_cn(Geo g, LesCnSet set, const double *__restrict__ nut, const double *__restrict__ dy,
                          const double *__restrict__ dyf)
 {
@@ -492,13 +488,13 @@ __global__ void k_les_cn(Geo g, LesCnSet set, const double *__restrict__ nut, co
     auto X = [&](int j) { return fidx3(i, k, j, g.nx, g.nz); };
     // forward elimination, rows 1..NY+1 (THOMAS_FORWARD_REAL_MPI)
     double L, D, U, V;
-    les_row(a, nut, X(0), X(1), X(0), dy, dyf, rhs, 0, L, D, U, V);
+    les_row<true>(a, nut, out, X(0), X(1), X(0), dy, dyf, rhs, 0, L, D, U, V);
     double bprev = D, cprev = U, gprev = V;
     bm[X(0)] = D;
     gw[X(0)] = V;
     for (int j = 1; j <= NY + 1; ++j) {
         const long x = X(j);
-        les_row(a, nut, x, X(j <= NY ? j + 1 : j), X(j - 1), dy, dyf, rhs, j, L, D, U, V);
+        les_row<true>(a, nut, out, x, X(j <= NY ? j + 1 : j), X(j - 1), dy, dyf, rhs, j, L, D, U, V);
         const double m = -L / bprev;
         D = D + m * cprev;
         V = V + m * gprev;
@@ -513,7 +509,7 @@ __global__ void k_les_cn(Geo g, LesCnSet set, const double *__restrict__ nut, co
     if (NY + 1 >= a.clo && NY + 1 <= a.chi) out[X(NY + 1)] = xn;
     for (int j = NY; j >= 0; --j) {
         const long x = X(j);
-        les_row(a, nut, x, X(j + 1), X(j > 0 ? j - 1 : 0), dy, dyf, rhs, j, L, D, U, V);
+        les_row<false>(a, nut, out, x, X(j + 1), X(j > 0 ? j - 1 : 0), dy, dyf, rhs, j, L, D, U, V);
         const double b = bm[x];
         xn = (b != 0.0) ? (gw[x] - U * xn) / b : 0.0;
         if (j >= a.clo && j <= a.chi) out[x] = xn;
@@ -669,14 +665,6 @@ void launch_les_acc(hipStream_t s, const Geo &g, cx<double> *dst, const double *
 {
     hipLaunchKernelGGL(k_les_acc, dim3((unsigned)((g.ps + 255) / 256), j1 - j0 + 1), dim3(256), 0, s, g, dst, fort,
                        kx, kz, kmul, j0);
-}
-
-void launch_les_explicit(hipStream_t s, const Geo &g, double *r, const double *u, const double *nut,
-                         const double *dy, const double *dyf, double temp2, int sys, int j0, int j1)
-{
-    const long n = (long)g.nx * g.nz;
-    hipLaunchKernelGGL(k_les_explicit, dim3((unsigned)((n + 255) / 256), j1 - j0 + 1), dim3(256), 0, s, g, r, u, nut,
-                       dy, dyf, temp2, sys, j0);
 }
 
 void launch_les_cn(hipStream_t s, const Geo &g, const LesCnSet &set, const double *nut, const double *dy,
