@@ -1073,7 +1073,7 @@ int32_t dgpu_set_column_shape(dgpu_ctx *c, int32_t waves, int32_t fused)
                                        std::to_string(c->g.ny) + " with " + std::to_string(waves) + " waves");
     if (fused == 1 && !(waves && fused_fits(c, c->g.ny, waves)))
         return fail(c, DGPU_E_ARG, "column shape: the fused column kernel needs the FP32 state (DGPU_MIXED) or "
-                                   "the FP64 path, 4, 8 or 16 waves (4 or 8 in FP64) and NY <= 286");
+                                   "the FP64 path, 4, 8 or 16 waves (4 or 8 in FP64) and NY <= 268");
     if (c->colnw == waves && c->colfuse == fused) return DGPU_OK;
     c->colnw = waves;
     c->colfuse = fused;

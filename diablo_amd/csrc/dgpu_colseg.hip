@@ -18,9 +18,10 @@
 //
 // Block shape: NW waves (4, 8 or 16; a runtime choice, dgpu_set_column_shape)
 // of 64 lanes; a lane is (mode, segment), MPW modes side by side so that a
-// wave's loads of one row are MPW consecutive modes.  The segment count is
-// fixed per precision (32 in FP64: rows per segment <= 9 up to NY = 286;
-// 16 in FP32), so MPW = 64 NW / NSEG.  The kernels ask for four waves per
+// wave's loads of one row are MPW consecutive modes.  The segment count of
+// k_cn_seg / k_remdiv_seg is fixed per precision (32 in FP64: rows per
+// segment <= 9 up to NY = 286; 16 in FP32), so MPW = 64 NW / NSEG; the fused
+// kernel has 30 segments at every shape.  The kernels ask for four waves per
 // SIMD (<= 128 VGPRs): one 16-wave block per CU, or two / four smaller
 // blocks whose phases (loads, carries, stores) overlap.
 //
@@ -39,6 +40,27 @@
 #include "dgpu_kernels.hpp"
 
 namespace dgpu {
+
+// Lane -> (segment, mode) in a block of NW waves with MPW modes side by side
+// (a segment's lanes read MPW consecutive modes of a row): NSEG = 64 NW / MPW
+// segments.  When MPW does not divide 64 NW, the lanes past NSEG * MPW repeat
+// lanes of the last segment: the same inputs, the same work, the same values
+// stored to the same places.
+constexpr int seg_count(int mpw, int nw) { return 64 * nw / mpw; }
+template <int MPW, int NSEG>
+__device__ __forceinline__ void seg_lane(int tid, int &seg, int &ml)
+{
+    const int s = tid / MPW;
+    ml = tid - s * MPW;
+    seg = min(s, NSEG - 1);
+}
+
+// The fused column kernel's segments per column, at every block shape (the
+// same segments, so the same bits, at 4, 8 and 16 waves): at most 9 rows
+// each (NY <= 268).  Modes per block: 8 at 4 waves, 17 at 8 and 34 at 16
+// (240 of 256, 510 of 512, 1020 of 1024 lanes distinct).
+constexpr int kFusedSeg = 30;
+constexpr int fused_mpw(int nw) { return 64 * nw / kFusedSeg; }
 
 namespace {
 
@@ -145,11 +167,10 @@ __device__ __forceinline__ void cn_rhs(int q, const CnMode<T> &K, TB &&tb, cx<T>
 // keep the old value).  The LDS holds the system's table and the carries
 // (cn_seg_lds); a caller that solves several systems in a row puts a block
 // barrier between them.
-template <typename T, int NTH, typename NLT, int MPW, int NW, int RMAX, typename AT, typename PT>
+template <typename T, int NTH, typename NLT, int MPW, int NW, int RMAX, typename AT, typename PT,
+          int NSEG = seg_count(MPW, NW)>
 __device__ __forceinline__ void cn_seg_solve(AT A, PT P, int bx, int q, cx<T> (&sol)[RMAX])
 {
-    constexpr int SPW = 64 / MPW;
-    constexpr int NSEG = NW * SPW;
     extern __shared__ unsigned char smem_raw[];
     const Geo &g = A.g;
     const int NY = g.ny, N = NY + 2;
@@ -162,8 +183,9 @@ __device__ __forceinline__ void cn_seg_solve(AT A, PT P, int bx, int q, cx<T> (&
     T *Cb = reinterpret_cast<T *>(XL + NSEG * MPW);                        // [NSEG]
     T *Ea = Cb + NSEG;                                                     // [NSEG]
 
-    const int tid = threadIdx.x, lane = tid & 63;
-    const int seg = (tid >> 6) * SPW + lane / MPW, ml = lane % MPW;
+    const int tid = threadIdx.x;
+    int seg, ml;
+    seg_lane<MPW, NSEG>(tid, seg, ml);
     const long m = (long)bx * MPW + ml;
     const bool live = m < ps;
     const int a = seg * R, b = min(a + R, N) - 1;
@@ -310,12 +332,12 @@ __device__ __forceinline__ void cn_seg_solve(AT A, PT P, int bx, int q, cx<T> (&
 template <typename T, int NTH, typename NLT, int MPW, int NW, int RMAX, typename AT, typename PT>
 __device__ __forceinline__ void cn_seg_body(AT A, PT P, int bx, int q)
 {
-    constexpr int NSEG = NW * (64 / MPW);
+    constexpr int NSEG = seg_count(MPW, NW);
     const int N = A.g.ny + 2;
     const int R = (N + NSEG - 1) / NSEG;
-    const int lane = threadIdx.x & 63;
-    const int seg = (threadIdx.x >> 6) * (64 / MPW) + lane / MPW;
-    const long m = (long)bx * MPW + lane % MPW;
+    int seg, ml;
+    seg_lane<MPW, NSEG>(threadIdx.x, seg, ml);
+    const long m = (long)bx * MPW + ml;
     const int a = seg * R, b = min(a + R, N) - 1;
     cx<T> sol[RMAX];
     cn_seg_solve<T, NTH, NLT, MPW, NW, RMAX, AT, PT>(A, P, bx, q, sol);
@@ -363,8 +385,7 @@ k_cn_seg_batch(const ColDesc<T, NLT> *__restrict__ d)
 template <typename T, typename NLT, int MPW, int NW, int RMAX, int NTH>
 __device__ __forceinline__ void remdiv_seg_body(const RdArgs &A, const CnPtrs<T, NLT> &P, int bx)
 {
-    constexpr int SPW = 64 / MPW;
-    constexpr int NSEG = NW * SPW;
+    constexpr int NSEG = seg_count(MPW, NW);
     extern __shared__ unsigned char smem_raw[];
     const Geo &g = A.g;
     const int NY = g.ny;
@@ -376,8 +397,9 @@ __device__ __forceinline__ void remdiv_seg_body(const RdArgs &A, const CnPtrs<T,
     T *Cb = reinterpret_cast<T *>(XL + NSEG * MPW);   // [NSEG][MPW]
     T *Ea = Cb + NSEG * MPW;
 
-    const int tid = threadIdx.x, lane = tid & 63;
-    const int seg = (tid >> 6) * SPW + lane / MPW, ml = lane % MPW;
+    const int tid = threadIdx.x;
+    int seg, ml;
+    seg_lane<MPW, NSEG>(tid, seg, ml);
     const long m = (long)bx * MPW + ml;
     const bool live = m < ps;
     const int a = 1 + seg * R, b = min(a + R - 1, NY);
@@ -554,8 +576,7 @@ template <typename T, int NTH, typename NLT, int MPW, int NW, int RMAX>
 __device__ __forceinline__ void cn3_solve(const CnArgs &A, const CnPtrs<T, NLT> &P, int bx, cx<T> (&u1s)[RMAX],
                                           cx<T> (&u3s)[RMAX], cx<T> (&u2s)[RMAX])
 {
-    constexpr int SPW = 64 / MPW;
-    constexpr int NSEG = NW * SPW;
+    constexpr int NSEG = kFusedSeg;
     constexpr int NM = NSEG * MPW;
     extern __shared__ unsigned char smem_raw[];
     const Geo &g = A.g;
@@ -574,8 +595,9 @@ __device__ __forceinline__ void cn3_solve(const CnArgs &A, const CnPtrs<T, NLT> 
     }
     __syncthreads();
 
-    const int tid = threadIdx.x, lane = tid & 63;
-    const int seg = (tid >> 6) * SPW + lane / MPW, ml = lane % MPW;
+    const int tid = threadIdx.x;
+    int seg, ml;
+    seg_lane<MPW, NSEG>(tid, seg, ml);
     const long m = (long)bx * MPW + ml;
     const bool live = m < ps;
     const int a = seg * R, b = min(a + R, N) - 1;
@@ -738,14 +760,13 @@ __device__ __forceinline__ void cn3_solve(const CnArgs &A, const CnPtrs<T, NLT> 
 // reads these modes); then the divergence of U*,
 // the Poisson solve, the projection into CU1..3 and the CP update.  The
 // state is read once and written once, with no solve scratch.  The Poisson
-// rows 1..NY run on the lanes that hold them in the CN part (32 segments of
+// rows 1..NY run on the lanes that hold them in the CN part (the segments of
 // rows 0..NY+1); a segment without Poisson rows passes the carries through.
 // ---------------------------------------------------------------------------
 template <typename T, typename NLT, int MPW, int NW, int RMAX, int NTH>
 __device__ __forceinline__ void colfused_body(const CnArgs &A, const RdArgs &Rd, const CnPtrs<T, NLT> &P, int bx)
 {
-    constexpr int SPW = 64 / MPW;
-    constexpr int NSEG = NW * SPW;
+    constexpr int NSEG = kFusedSeg;
     extern __shared__ unsigned char smem_raw[];
     const Geo &g = A.g;
     const int NY = g.ny, N = NY + 2;
@@ -759,8 +780,9 @@ __device__ __forceinline__ void colfused_body(const CnArgs &A, const RdArgs &Rd,
     T *Em = Cm + NSEG * MPW;                                               // [NSEG][MPW]
     cx<T> *X2 = reinterpret_cast<cx<T> *>(Em + NSEG * MPW);                // [NSEG][MPW]: U2*(a)
 
-    const int tid = threadIdx.x, lane = tid & 63;
-    const int seg = (tid >> 6) * SPW + lane / MPW, ml = lane % MPW;
+    const int tid = threadIdx.x;
+    int seg, ml;
+    seg_lane<MPW, NSEG>(tid, seg, ml);
     const long m = (long)bx * MPW + ml;
     const bool live = m < ps;
     const int a = seg * R, b = min(a + R, N) - 1;
@@ -773,7 +795,8 @@ __device__ __forceinline__ void colfused_body(const CnArgs &A, const RdArgs &Rd,
     if (NTH) {
         __syncthreads();  // the TH solve's table goes where the three systems' carries were
         cx<T> ts[RMAX];
-        cn_seg_solve<T, NTH, NLT, MPW, NW, RMAX, const CnArgs &, const CnPtrs<T, NLT> &>(A, P, bx, SYS_TH0, ts);
+        cn_seg_solve<T, NTH, NLT, MPW, NW, RMAX, const CnArgs &, const CnPtrs<T, NLT> &, kFusedSeg>(A, P, bx,
+                                                                                                   SYS_TH0, ts);
         if (rows) {
 #pragma unroll
             for (int r = 0; r < RMAX; ++r) {
@@ -1062,19 +1085,20 @@ void launch_remdiv_seg_batch(hipStream_t s, int nw, const Geo &g, int max_ny, co
     });
 }
 
-// The fused column kernel: 32 segments per column, at most 9 rows each
-// (NY <= 286), FP32 state or FP64 (there at two waves per SIMD: 252 VGPRs,
-// and the three systems' tables take 81 KB of LDS at NY = 257, so 4 or 8
-// waves per block).
-constexpr int kFusedSeg = 32;
-
+// The fused column kernel (kFusedSeg segments of at most 9 rows), FP32 state
+// or FP64 (there at two waves per SIMD: 252 VGPRs, and the three systems'
+// tables take 81 KB of LDS at NY = 257, so 4 or 8 waves per block).  A round
+// of resident blocks is latency-bound, so the mode count per block sets the
+// number of rounds: at 384x257x384 (33153 modes, one FP64 block of 8 waves
+// per CU) 16 modes per block took 8 full rounds and a ninth of 32 blocks,
+// 17 take 7.6.
 template <typename T>
 static size_t colfused_lds(int ny, int nw)
 {
-    const int mpw = 64 * nw / kFusedSeg, n = ny + 2;
-    const size_t one = sizeof(T) * (size_t)((n * kLRow + 1) & ~1) + 2 * sizeof(cx<T>) * kFusedSeg * mpw +
-                       2 * sizeof(T) * kFusedSeg + 2 * sizeof(T) * kFusedSeg * mpw + sizeof(cx<T>) * kFusedSeg * mpw;
-    const size_t three = 6 * sizeof(cx<T>) * kFusedSeg * mpw + 6 * sizeof(T) * kFusedSeg +
+    const int mpw = fused_mpw(nw), nseg = kFusedSeg, n = ny + 2;
+    const size_t one = sizeof(T) * (size_t)((n * kLRow + 1) & ~1) + 2 * sizeof(cx<T>) * nseg * mpw +
+                       2 * sizeof(T) * nseg + 2 * sizeof(T) * nseg * mpw + sizeof(cx<T>) * nseg * mpw;
+    const size_t three = 6 * sizeof(cx<T>) * nseg * mpw + 6 * sizeof(T) * nseg +
                          3 * sizeof(T) * (size_t)n * kLRow;  // cn3_solve: carries, tables
     return std::max(one, three);
 }
@@ -1092,7 +1116,7 @@ void launch_colfused(hipStream_t s, int nw, const CnArgs &a, const RdArgs &r, co
 {
     const size_t lds = colfused_lds<T>(a.g.ny, nw);
     with_nw(nw, [&](auto wc) {
-        constexpr int NW = decltype(wc)::value, MPW = 64 * NW / kFusedSeg;
+        constexpr int NW = decltype(wc)::value, MPW = fused_mpw(NW);
         if constexpr (sizeof(T) == 8 && NW == 16) {
             return;  // not built: colfused_supported refuses the shape
         } else {
@@ -1118,7 +1142,7 @@ void launch_colfused_batch(hipStream_t s, int nw, const Geo &g, int max_ny, cons
 {
     const size_t lds = colfused_lds<T>(max_ny, nw);
     with_nw(nw, [&](auto wc) {
-        constexpr int NW = decltype(wc)::value, MPW = 64 * NW / kFusedSeg;
+        constexpr int NW = decltype(wc)::value, MPW = fused_mpw(NW);
         if constexpr (sizeof(T) == 8 && NW == 16) {
             return;  // not built: colfused_supported refuses the shape
         } else {

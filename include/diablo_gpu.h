@@ -296,12 +296,12 @@ int32_t dgpu_set_zdft(dgpu_ctx *ctx, int32_t on);
  * fused (contexts with the FP32 state, DGPU_MIXED, and FP64 contexts): 1
  * runs the whole column stage -- the four CN solves, the divergence, the
  * Poisson solve, the projection and the CP update -- as one kernel that
- * reads and writes the state once (32 segments, NY <= 286; 4 or 8 waves in
+ * reads and writes the state once (30 segments, NY <= 268; 4 or 8 waves in
  * FP64); 0 keeps the two kernels; -1 (the default; environment
  * DGPU_COLFUSE) fuses from NY = 160 up (in FP64 only at 8 waves).  With the
- * FP32 state the fused kernel rounds differently from the two kernels (32
- * segments, not 16), within FP32 round-off; in FP64 both use 32 segments
- * and agree to round-off.  An RK stage 3 with a COURANT request runs the
+ * FP32 state the fused kernel rounds differently from the two kernels (30
+ * segments, not 16), within FP32 round-off; in FP64 (30 against 32
+ * segments) they agree to round-off.  An RK stage 3 with a COURANT request runs the
  * two kernels (the maxima are taken between them).
  * dgpu_get_column_shape reports the waves in use (0 for the sweeps) and
  * whether the fused kernel runs (fused may be NULL). */

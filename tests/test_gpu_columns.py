@@ -117,7 +117,7 @@ def test_column_shape_refusals(gpu_mod):
     t.set_column_shape(0)
     t.close()
     # the fused kernel: the FP32 state or the FP64 path (not the FP64 state
-    # under FP32 transforms), NY <= 286
+    # under FP32 transforms), NY <= 268
     f = gpu_mod.GpuChannel(cfg, precision="mixed_f64state")
     with pytest.raises(gpu_mod.DgpuError, match="FP32 state"):
         f.set_column_shape(8, 1)
@@ -148,9 +148,9 @@ def test_column_shape_refusals(gpu_mod):
 @pytest.mark.parametrize("name", list(CASES))
 def test_fused_column_stage(gpu_mod, name, precision):
     """The fused column kernel: 4, 8 and (FP32 state) 16 waves give the same
-    bits.  In FP64 it agrees with the two kernels within 1e-11 (the parity
-    bar of the substep tests; both solve on 32 segments, the Poisson
-    segments differ).  With the FP32 state, against the FP64 run it is no
+    bits (30 segments at every shape).  In FP64 it agrees with the two
+    kernels within 1e-11 (the parity bar of the substep tests; the two
+    kernels solve on 32 segments).  With the FP32 state, against the FP64 run it is no
     less accurate than the two kernels (within 1.5x of their error, field by
     field, plus 1e-7)."""
     cfg = CASES[name]()
