@@ -1144,8 +1144,9 @@ static int ensure_les(dgpu_ctx *c)
 // velocity BCs and APPLY_BC_LES on the state, S_ij and U1, U3 to physical
 // space, |S| and NU_T, tau_ij back to Fourier space and its divergence
 // subtracted from f1-f3 (the reference's CF1-3), then the explicit cross
-// terms NU_T dU/dy added to f2.
-static int les_model(dgpu_ctx *c, cx<double> *f1, cx<double> *f2, cx<double> *f3)
+// terms NU_T dU/dy added to f2.  fresh: f1-f3 start from zero (the SGS
+// force alone) and need no clear beforehand.
+static int les_model(dgpu_ctx *c, cx<double> *f1, cx<double> *f2, cx<double> *f3, bool fresh)
 {
     const dgpu::Geo &g = c->g;
     const int NY = g.ny;
@@ -1173,7 +1174,7 @@ static int les_model(dgpu_ctx *c, cx<double> *f1, cx<double> *f2, cx<double> *f3
         if (int rc = fort_to_fourier(c, O.p[q], 0, NY + 1)) return rc;
     // CF1, CF3 on J1 = JSTART .. J2 = NY-1 (the RANKY = NPROCY-1 bounds, for/les.f:46-50)
     dgpu::launch_les_cf(c->stream, g, O, f1, f2, f3, c->d_kx, c->d_kz, c->d_dyf, delta_y, delta_yf, c->jstart,
-                        NY - 1);
+                        NY - 1, fresh);
     if (int rc = check_launch(c, "les_cf")) return rc;
     for (int pass = 0; pass < 2; ++pass) {
         dgpu::launch_les_cross_src(c->stream, g, O.p[5], S.p[6 + pass], c->d_dy, O.p[0]);
@@ -1191,7 +1192,7 @@ int32_t dgpu_les_chan(dgpu_ctx *c, double *nu_t)
     if (int rc = to_master(c)) return rc;
     c->master_dirty = true;
     if (int rc = ensure_les(c)) return rc;
-    if (int rc = les_model(c, c->cf[0], c->cf[1], c->cf[2])) return rc;
+    if (int rc = les_model(c, c->cf[0], c->cf[1], c->cf[2], false)) return rc;
     if (nu_t) HIPCHK(c, hipMemcpyAsync(nu_t, les_buf(c, 13), les_n(c) * sizeof(double), hipMemcpyDeviceToHost,
                                        c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1643,11 +1644,13 @@ static int les_substep(dgpu_ctx *c, int rk_step, double time, double temp1, doub
     cx<double> *pre = reinterpret_cast<cx<double> *>(les_buf(c, 18));
     const size_t fsb = sizeof(cx<double>) * (size_t)c->fs;
     if (c->les_model) {
-        for (int q = 0; q < 4; ++q) HIPCHK(c, hipMemsetAsync(sgs[q], 0, fsb, c->stream));
+        // the model writes the whole of sgs[0..2]; the scalar's part is
+        // accumulated over its own rows
+        if (g.nth) HIPCHK(c, hipMemsetAsync(sgs[3], 0, fsb, c->stream));
         for (int r = 0; r < 4; ++r)
             HIPCHK(c, hipMemcpyAsync(pre + (size_t)r * g.ps, c->cu[r < 2 ? 0 : 2] + (size_t)(r % 2 ? NY : 1) * g.ps,
                                      sizeof(cx<double>) * g.ps, hipMemcpyDeviceToDevice, c->stream));
-        if (int rc = les_model(c, sgs[0], sgs[1], sgs[2])) return rc;
+        if (int rc = les_model(c, sgs[0], sgs[1], sgs[2], true)) return rc;
         // APPLY_BC_VEL_PHYS_MPI (for/channel.f:309-311)
         dgpu::LesBc b;
         b.u_lo = c->p.u_bc_ymin; b.u_hi = c->p.u_bc_ymax; b.w_lo = c->p.w_bc_ymin; b.w_hi = c->p.w_bc_ymax;

@@ -270,7 +270,7 @@ void launch_les_nut(hipStream_t s, const Geo &g, LesPtrs in, LesPtrs out, const 
                     int jend);
 void launch_les_cf(hipStream_t s, const Geo &g, LesPtrs t, cx<double> *cf1, cx<double> *cf2, cx<double> *cf3,
                    const double *kx, const double *kz, const double *dyf, const double *delta_y,
-                   const double *delta_yf, int j1, int j2);
+                   const double *delta_yf, int j1, int j2, bool fresh);
 void launch_les_cross_src(hipStream_t s, const Geo &g, const double *nut, const double *u, const double *dy,
                           double *out);
 void launch_les_cross_add(hipStream_t s, const Geo &g, const double *ct, cx<double> *cf2, const double *kx,

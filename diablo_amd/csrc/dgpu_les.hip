@@ -164,10 +164,13 @@ __global__ void k_les_nut(Geo g, LesPtrs in, LesPtrs out,
 
 // CF -= div tau (for/les.f:236-301): t[0..4] the transformed parts of
 // k_les_nut (Fortran-layout Fourier), scaled here by delta_yf / delta_y.
+// fresh: cf1..cf3 start from zero and every row 0..NY+1 is written (the
+// rows outside the force's ranges with zero), so the buffers need no clear.
 __global__ void k_les_cf(Geo g, LesPtrs t, cx<double> *__restrict__ cf1,
                          cx<double> *__restrict__ cf2, cx<double> *__restrict__ cf3, const double *__restrict__ kxv,
                          const double *__restrict__ kzv, const double *__restrict__ dyf,
-                         const double *__restrict__ delta_y, const double *__restrict__ delta_yf, int j1, int j2)
+                         const double *__restrict__ delta_y, const double *__restrict__ delta_yf, int j1, int j2,
+                         int fresh)
 {
     const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
@@ -182,28 +185,34 @@ __global__ void k_les_cf(Geo g, LesPtrs t, cx<double> *__restrict__ cf1,
         return make_cx<double>(scale * v.re, scale * v.im);
     };
     const long o = m + (long)j * ps;
+    const cx<double> zero = make_cx<double>(0.0, 0.0);
     if (j >= j1 && j <= j2) {
         const cx<double> c11 = T(0, j, delta_yf[j]), c13 = T(3, j, delta_yf[j]), c33 = T(1, j, delta_yf[j]);
         const double dyj = j >= 2 ? delta_y[j] : 1.0;  // rows from 2 on are scaled (for/les.f:262-270)
         const cx<double> c12 = T(2, j, dyj), c12p = T(2, j + 1, delta_y[j + 1]);
         const cx<double> c23 = T(4, j, dyj), c23p = T(4, j + 1, delta_y[j + 1]);
-        cx<double> f = cf1[o];
+        cx<double> f = fresh ? zero : cf1[o];
         const cx<double> a = i_times(kx, c11), b = i_times(kz, c13);
         f = make_cx<double>(f.re - a.re - (c12p.re - c12.re) / dyf[j] - b.re,
                             f.im - a.im - (c12p.im - c12.im) / dyf[j] - b.im);
         cf1[o] = f;
-        cx<double> h = cf3[o];
+        cx<double> h = fresh ? zero : cf3[o];
         const cx<double> c = i_times(kx, c13), d = i_times(kz, c33);
         h = make_cx<double>(h.re - c.re - (c23p.re - c23.re) / dyf[j] - d.re,
                             h.im - c.im - (c23p.im - c23.im) / dyf[j] - d.im);
         cf3[o] = h;
+    } else if (fresh) {
+        cf1[o] = zero;
+        cf3[o] = zero;
     }
     if (j >= 2 && j <= NY) {
         const cx<double> c12 = T(2, j, delta_y[j]), c23 = T(4, j, delta_y[j]);
         const cx<double> a = i_times(kx, c12), b = i_times(kz, c23);
-        cx<double> f = cf2[o];
+        cx<double> f = fresh ? zero : cf2[o];
         f = make_cx<double>(f.re - a.re - b.re, f.im - a.im - b.im);
         cf2[o] = f;
+    } else if (fresh) {
+        cf2[o] = zero;
     }
 }
 
@@ -706,10 +715,10 @@ void launch_les_nut(hipStream_t s, const Geo &g, LesPtrs in, LesPtrs out,
 
 void launch_les_cf(hipStream_t s, const Geo &g, LesPtrs t, cx<double> *cf1, cx<double> *cf2,
                    cx<double> *cf3, const double *kx, const double *kz, const double *dyf, const double *delta_y,
-                   const double *delta_yf, int j1, int j2)
+                   const double *delta_yf, int j1, int j2, bool fresh)
 {
-    hipLaunchKernelGGL(k_les_cf, dim3((unsigned)((g.ps + 255) / 256), g.ny + 1), dim3(256), 0, s, g, t, cf1, cf2,
-                       cf3, kx, kz, dyf, delta_y, delta_yf, j1, j2);
+    hipLaunchKernelGGL(k_les_cf, dim3((unsigned)((g.ps + 255) / 256), g.ny + (fresh ? 2 : 1)), dim3(256), 0, s, g,
+                       t, cf1, cf2, cf3, kx, kz, dyf, delta_y, delta_yf, j1, j2, fresh ? 1 : 0);
 }
 
 void launch_les_cross_src(hipStream_t s, const Geo &g, const double *nut, const double *u, const double *dy,
