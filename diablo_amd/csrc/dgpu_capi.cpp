@@ -1444,7 +1444,10 @@ static int seg_waves(const dgpu_ctx *c, int ny, bool fp32)
 static int fused_waves(const dgpu_ctx *c, int ny)
 {
     if (!c->colnw) return 0;
-    const bool auto_on = ny >= 160 && (c->f32state || c->colnw == 8);
+    // tall columns; with the FP32 state also the short ones the fused kernel
+    // cuts into 16 segments (NY <= 142): the 64-member ensemble 17.13k ->
+    // 17.65k Mpts*substeps/s against the two kernels
+    const bool auto_on = (ny >= 160 && (c->f32state || c->colnw == 8)) || (c->f32state && ny + 2 <= 144);
     const bool want = c->colfuse == 1 || (c->colfuse < 0 && auto_on);
     return (want && fused_fits(c, ny, c->colnw)) ? c->colnw : 0;
 }

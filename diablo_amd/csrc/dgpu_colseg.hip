@@ -55,12 +55,17 @@ __device__ __forceinline__ void seg_lane(int tid, int &seg, int &ml)
     seg = min(s, NSEG - 1);
 }
 
-// The fused column kernel's segments per column, at every block shape (the
-// same segments, so the same bits, at 4, 8 and 16 waves): at most 9 rows
-// each (NY <= 268).  Modes per block: 8 at 4 waves, 17 at 8 and 34 at 16
-// (240 of 256, 510 of 512, 1020 of 1024 lanes distinct).
-constexpr int kFusedSeg = 30;
-constexpr int fused_mpw(int nw) { return 64 * nw / kFusedSeg; }
+// The fused column kernel's segments per column, set by the column height
+// and the same at every block shape (so 4, 8 and 16 waves give the same
+// bits), at most 9 rows each: 16 up to NY = 142 (modes per block 16, 32, 64
+// at 4, 8, 16 waves), else 30 (NY <= 268; modes per block 8, 17, 34: 240 of
+// 256, 510 of 512, 1020 of 1024 lanes distinct).  Short columns on 30
+// segments would leave up to 40% of the lanes without rows.
+constexpr int kFusedSeg = 30, kFusedSegShort = 16;
+constexpr int fused_nseg(int ny) { return ny + 2 <= 9 * kFusedSegShort ? kFusedSegShort : kFusedSeg; }
+constexpr int fused_mpw(int nw, int nseg) { return 64 * nw / nseg; }
+// the segment count of a fused kernel built for (MPW, NW)
+constexpr int fused_nseg_of(int mpw, int nw) { return mpw == fused_mpw(nw, kFusedSeg) ? kFusedSeg : kFusedSegShort; }
 
 namespace {
 
@@ -576,7 +581,7 @@ template <typename T, int NTH, typename NLT, int MPW, int NW, int RMAX>
 __device__ __forceinline__ void cn3_solve(const CnArgs &A, const CnPtrs<T, NLT> &P, int bx, cx<T> (&u1s)[RMAX],
                                           cx<T> (&u3s)[RMAX], cx<T> (&u2s)[RMAX])
 {
-    constexpr int NSEG = kFusedSeg;
+    constexpr int NSEG = fused_nseg_of(MPW, NW);
     constexpr int NM = NSEG * MPW;
     extern __shared__ unsigned char smem_raw[];
     const Geo &g = A.g;
@@ -766,7 +771,7 @@ __device__ __forceinline__ void cn3_solve(const CnArgs &A, const CnPtrs<T, NLT> 
 template <typename T, typename NLT, int MPW, int NW, int RMAX, int NTH>
 __device__ __forceinline__ void colfused_body(const CnArgs &A, const RdArgs &Rd, const CnPtrs<T, NLT> &P, int bx)
 {
-    constexpr int NSEG = kFusedSeg;
+    constexpr int NSEG = fused_nseg_of(MPW, NW);
     extern __shared__ unsigned char smem_raw[];
     const Geo &g = A.g;
     const int NY = g.ny, N = NY + 2;
@@ -795,8 +800,8 @@ __device__ __forceinline__ void colfused_body(const CnArgs &A, const RdArgs &Rd,
     if (NTH) {
         __syncthreads();  // the TH solve's table goes where the three systems' carries were
         cx<T> ts[RMAX];
-        cn_seg_solve<T, NTH, NLT, MPW, NW, RMAX, const CnArgs &, const CnPtrs<T, NLT> &, kFusedSeg>(A, P, bx,
-                                                                                                   SYS_TH0, ts);
+        cn_seg_solve<T, NTH, NLT, MPW, NW, RMAX, const CnArgs &, const CnPtrs<T, NLT> &, NSEG>(A, P, bx, SYS_TH0,
+                                                                                              ts);
         if (rows) {
 #pragma unroll
             for (int r = 0; r < RMAX; ++r) {
@@ -1085,17 +1090,17 @@ void launch_remdiv_seg_batch(hipStream_t s, int nw, const Geo &g, int max_ny, co
     });
 }
 
-// The fused column kernel (kFusedSeg segments of at most 9 rows), FP32 state
-// or FP64 (there at two waves per SIMD: 252 VGPRs, and the three systems'
-// tables take 81 KB of LDS at NY = 257, so 4 or 8 waves per block).  A round
-// of resident blocks is latency-bound, so the mode count per block sets the
-// number of rounds: at 384x257x384 (33153 modes, one FP64 block of 8 waves
-// per CU) 16 modes per block took 8 full rounds and a ninth of 32 blocks,
-// 17 take 7.6.
+// The fused column kernel (fused_nseg(NY) segments of at most 9 rows), FP32
+// state or FP64 (there at two waves per SIMD: 252 VGPRs, and the three
+// systems' tables take 81 KB of LDS at NY = 257, so 4 or 8 waves per block).
+// A round of resident blocks is latency-bound, so the mode count per block
+// sets the number of rounds: at 384x257x384 (33153 modes, one FP64 block of
+// 8 waves per CU) 16 modes per block took 8 full rounds and a ninth of 32
+// blocks, 17 take 7.6.
 template <typename T>
 static size_t colfused_lds(int ny, int nw)
 {
-    const int mpw = fused_mpw(nw), nseg = kFusedSeg, n = ny + 2;
+    const int nseg = fused_nseg(ny), mpw = fused_mpw(nw, nseg), n = ny + 2;
     const size_t one = sizeof(T) * (size_t)((n * kLRow + 1) & ~1) + 2 * sizeof(cx<T>) * nseg * mpw +
                        2 * sizeof(T) * nseg + 2 * sizeof(T) * nseg * mpw + sizeof(cx<T>) * nseg * mpw;
     const size_t three = 6 * sizeof(cx<T>) * nseg * mpw + 6 * sizeof(T) * nseg +
@@ -1107,58 +1112,83 @@ template <typename T>
 bool colfused_supported(int ny, int nw)
 {
     // FP64 blocks of 16 waves would hold its 252 VGPRs in 128 (spills)
-    return (nw == 4 || nw == 8 || (nw == 16 && sizeof(T) == 4)) && ny >= 4 &&
-           (ny + 2 + kFusedSeg - 1) / kFusedSeg <= 9 && colfused_lds<T>(ny, nw) <= 150 * 1024;
+    const int nseg = fused_nseg(ny);
+    return (nw == 4 || nw == 8 || (nw == 16 && sizeof(T) == 4)) && ny >= 4 && (ny + 2 + nseg - 1) / nseg <= 9 &&
+           colfused_lds<T>(ny, nw) <= 150 * 1024;
 }
+
+namespace {
+
+// Calls fn(std::integral_constant<int, MPW>) with the fused kernel's modes
+// per block for NW waves and the segment count of columns of ny rows.
+template <int NW, class F>
+void with_fused_mpw(int ny, F &&fn)
+{
+    if (fused_nseg(ny) == kFusedSegShort) fn(std::integral_constant<int, fused_mpw(NW, kFusedSegShort)>{});
+    else fn(std::integral_constant<int, fused_mpw(NW, kFusedSeg)>{});
+}
+
+template <typename T, typename NLT, int MPW, int NW>
+void colfused_attr()
+{
+    static bool attr = false;
+    if (attr) return;
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_colfused<T, NLT, MPW, NW, 9, 0>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_colfused<T, NLT, MPW, NW, 9, 1>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_colfused_batch<T, NLT, MPW, NW, 9, 0>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_colfused_batch<T, NLT, MPW, NW, 9, 1>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    attr = true;
+}
+
+}  // namespace
 
 template <typename T, typename NLT>
 void launch_colfused(hipStream_t s, int nw, const CnArgs &a, const RdArgs &r, const CnPtrs<T, NLT> &p)
 {
     const size_t lds = colfused_lds<T>(a.g.ny, nw);
     with_nw(nw, [&](auto wc) {
-        constexpr int NW = decltype(wc)::value, MPW = fused_mpw(NW);
+        constexpr int NW = decltype(wc)::value;
         if constexpr (sizeof(T) == 8 && NW == 16) {
             return;  // not built: colfused_supported refuses the shape
         } else {
-            const long blocks = ((a.g.ps + MPW - 1) / MPW + 7) / 8 * 8;  // a multiple of 8 (xcd_block)
-            static bool attr = false;
-            if (!attr) {
-                (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_colfused<T, NLT, MPW, NW, 9, 0>),
-                                          hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-                (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_colfused<T, NLT, MPW, NW, 9, 1>),
-                                          hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-                attr = true;
-            }
-            if (a.g.nth == 0)
-                hipLaunchKernelGGL((k_colfused<T, NLT, MPW, NW, 9, 0>), dim3(blocks), dim3(64 * NW), lds, s, a, r, p);
-            else
-                hipLaunchKernelGGL((k_colfused<T, NLT, MPW, NW, 9, 1>), dim3(blocks), dim3(64 * NW), lds, s, a, r, p);
+            with_fused_mpw<NW>(a.g.ny, [&](auto mc) {
+                constexpr int MPW = decltype(mc)::value;
+                const long blocks = ((a.g.ps + MPW - 1) / MPW + 7) / 8 * 8;  // a multiple of 8 (xcd_block)
+                colfused_attr<T, NLT, MPW, NW>();
+                if (a.g.nth == 0)
+                    hipLaunchKernelGGL((k_colfused<T, NLT, MPW, NW, 9, 0>), dim3(blocks), dim3(64 * NW), lds, s, a, r,
+                                       p);
+                else
+                    hipLaunchKernelGGL((k_colfused<T, NLT, MPW, NW, 9, 1>), dim3(blocks), dim3(64 * NW), lds, s, a, r,
+                                       p);
+            });
         }
     });
 }
 
+// Every member of the batch runs the kernel built for the tallest column.
 template <typename T, typename NLT>
 void launch_colfused_batch(hipStream_t s, int nw, const Geo &g, int max_ny, const ColDesc<T, NLT> *d, int nmem)
 {
     const size_t lds = colfused_lds<T>(max_ny, nw);
     with_nw(nw, [&](auto wc) {
-        constexpr int NW = decltype(wc)::value, MPW = fused_mpw(NW);
+        constexpr int NW = decltype(wc)::value;
         if constexpr (sizeof(T) == 8 && NW == 16) {
             return;  // not built: colfused_supported refuses the shape
         } else {
-            const dim3 grid((g.ps + MPW - 1) / MPW, 1, nmem);
-            static bool attr = false;
-            if (!attr) {
-                (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_colfused_batch<T, NLT, MPW, NW, 9, 0>),
-                                          hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-                (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_colfused_batch<T, NLT, MPW, NW, 9, 1>),
-                                          hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-                attr = true;
-            }
-            if (g.nth == 0)
-                hipLaunchKernelGGL((k_colfused_batch<T, NLT, MPW, NW, 9, 0>), grid, dim3(64 * NW), lds, s, d);
-            else
-                hipLaunchKernelGGL((k_colfused_batch<T, NLT, MPW, NW, 9, 1>), grid, dim3(64 * NW), lds, s, d);
+            with_fused_mpw<NW>(max_ny, [&](auto mc) {
+                constexpr int MPW = decltype(mc)::value;
+                const dim3 grid((g.ps + MPW - 1) / MPW, 1, nmem);
+                colfused_attr<T, NLT, MPW, NW>();
+                if (g.nth == 0)
+                    hipLaunchKernelGGL((k_colfused_batch<T, NLT, MPW, NW, 9, 0>), grid, dim3(64 * NW), lds, s, d);
+                else
+                    hipLaunchKernelGGL((k_colfused_batch<T, NLT, MPW, NW, 9, 1>), grid, dim3(64 * NW), lds, s, d);
+            });
         }
     });
 }

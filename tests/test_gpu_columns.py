@@ -127,7 +127,9 @@ def test_column_shape_refusals(gpu_mod):
     with pytest.raises(gpu_mod.DgpuError, match="column shape"):
         m.set_column_shape(8, 1)
     m.close()
-    for ny, want in ((129, False), (161, True)):
+    # by default from NY = 160 up, and with the FP32 state on the short
+    # columns it cuts into 16 segments (NY <= 142)
+    for ny, want in ((129, True), (150, False), (161, True)):
         a = gpu_mod.GpuChannel(case.channel_case(24, ny, 18, kick=0.0, streak=0.0), precision="mixed")
         assert a.column_fused() == want, ny
         a.close()
