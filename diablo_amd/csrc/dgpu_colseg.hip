@@ -58,14 +58,29 @@ __device__ __forceinline__ void seg_lane(int tid, int &seg, int &ml)
 // The fused column kernel's segments per column, set by the column height
 // and the same at every block shape (so 4, 8 and 16 waves give the same
 // bits), at most 9 rows each: 16 up to NY = 142 (modes per block 16, 32, 64
-// at 4, 8, 16 waves), else 30 (NY <= 268; modes per block 8, 17, 34: 240 of
-// 256, 510 of 512, 1020 of 1024 lanes distinct).  Short columns on 30
-// segments would leave up to 40% of the lanes without rows.
-constexpr int kFusedSeg = 30, kFusedSegShort = 16;
-constexpr int fused_nseg(int ny) { return ny + 2 <= 9 * kFusedSegShort ? kFusedSegShort : kFusedSeg; }
+// at 4, 8, 16 waves), 22 up to NY = 196 (11, 23, 46), else 30 (NY <= 268;
+// 8, 17, 34).  The lanes past nseg * mpw repeat lanes of the last segment.
+// The fewest segments a column fits in means the most modes per block:
+// short columns on 30 segments would leave up to 40% of the lanes without
+// rows, and fewer, fuller blocks take fewer rounds of resident blocks.
+constexpr int kFusedSeg = 30, kFusedSegMid = 22, kFusedSegShort = 16;
+constexpr int fused_nseg(int ny)
+{
+    return ny + 2 <= 9 * kFusedSegShort ? kFusedSegShort : ny + 2 <= 9 * kFusedSegMid ? kFusedSegMid : kFusedSeg;
+}
 constexpr int fused_mpw(int nw, int nseg) { return 64 * nw / nseg; }
 // the segment count of a fused kernel built for (MPW, NW)
-constexpr int fused_nseg_of(int mpw, int nw) { return mpw == fused_mpw(nw, kFusedSeg) ? kFusedSeg : kFusedSegShort; }
+constexpr int fused_nseg_of(int mpw, int nw)
+{
+    return mpw == fused_mpw(nw, kFusedSeg) ? kFusedSeg : mpw == fused_mpw(nw, kFusedSegMid) ? kFusedSegMid
+                                                                                            : kFusedSegShort;
+}
+static_assert(fused_mpw(4, kFusedSeg) != fused_mpw(4, kFusedSegMid) && fused_mpw(8, kFusedSeg) != fused_mpw(8, kFusedSegMid) &&
+                  fused_mpw(16, kFusedSeg) != fused_mpw(16, kFusedSegMid) &&
+                  fused_mpw(4, kFusedSegShort) != fused_mpw(4, kFusedSegMid) &&
+                  fused_mpw(8, kFusedSegShort) != fused_mpw(8, kFusedSegMid) &&
+                  fused_mpw(16, kFusedSegShort) != fused_mpw(16, kFusedSegMid),
+              "each segment count has its own modes per block");
 
 namespace {
 
@@ -1124,7 +1139,9 @@ namespace {
 template <int NW, class F>
 void with_fused_mpw(int ny, F &&fn)
 {
-    if (fused_nseg(ny) == kFusedSegShort) fn(std::integral_constant<int, fused_mpw(NW, kFusedSegShort)>{});
+    const int nseg = fused_nseg(ny);
+    if (nseg == kFusedSegShort) fn(std::integral_constant<int, fused_mpw(NW, kFusedSegShort)>{});
+    else if (nseg == kFusedSegMid) fn(std::integral_constant<int, fused_mpw(NW, kFusedSegMid)>{});
     else fn(std::integral_constant<int, fused_mpw(NW, kFusedSeg)>{});
 }
 

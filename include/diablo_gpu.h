@@ -299,7 +299,8 @@ int32_t dgpu_set_zdft(dgpu_ctx *ctx, int32_t on);
  * reads and writes the state once (30 segments, NY <= 268; 4 or 8 waves in
  * FP64); 0 keeps the two kernels; -1 (the default; environment
  * DGPU_COLFUSE) fuses from NY = 160 up (in FP64 only at 8 waves) and, with
- * the FP32 state, up to NY = 142 (16 segments there, else 30).  With the
+ * the FP32 state, up to NY = 142 (16 segments there, 22 up to NY = 196,
+ * else 30).  With the
  * FP32 state the fused kernel rounds differently from the two kernels (30
  * segments, not 16), within FP32 round-off; in FP64 (30 against 32
  * segments) they agree to round-off.  An RK stage 3 with a COURANT request runs the
