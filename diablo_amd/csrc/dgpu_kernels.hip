@@ -43,12 +43,27 @@ __device__ __forceinline__ cx<TO> cvt(cx<FROM> a) { return make_cx<TO>(TO(a.re),
 // -------------------------------------------------------------------------
 // z-inverse: kept slots -> length-NZ columns (FFT_XZ_TO_PHYSICAL z part,
 // for/mpi.f:1627-1641): unpack, zero the gap, unnormalised backward C2C.
-// grid (col-chunks, planes, fields); block W*T threads; LDS W*NZ complex.
+// grid (8 * col-chunks * plane groups of 8, 1, fields), in plane_chunk's
+// order; block W*T threads; LDS W*NZ complex.
 // -------------------------------------------------------------------------
+// Block L of a plane-major grid -> (column chunk, plane): the hardware deals
+// consecutive blocks to the 8 XCDs in turn, so L & 7 picks the plane within
+// a group of 8 and every chunk of a plane runs on one XCD.  A chunk's W
+// columns are W * 16 bytes of a row that need not start on a cache line;
+// the line two chunks share is then fetched (and written back) once, from
+// that XCD's L2.  false: a block past the last plane.
+__device__ __forceinline__ bool plane_chunk(int L, int nchunks, int nplanes, int &chunk, int &plane)
+{
+    const int r = L >> 3;
+    chunk = r % nchunks;
+    plane = 8 * (r / nchunks) + (L & 7);
+    return plane < nplanes;
+}
+
 template <typename T, int E, typename TIN, int FID>
 __global__ void k_zinv(FftLen fz_rt, const cx<T> *__restrict__ tw, const cx<TIN> *__restrict__ in, long in_fs,
                        long in_ps, int in_mk, int ncols, cx<T> *__restrict__ out, long out_fs, long out_ps,
-                       int out_mk, int j0, int nkz, int tnkz, int lw)
+                       int out_mk, int j0, int nkz, int tnkz, int lw, int nchunks, int nplanes)
 {
     const FftLen &fz = fft_ref<FID>(fz_rt);
     extern __shared__ unsigned char smem_raw[];
@@ -56,8 +71,10 @@ __global__ void k_zinv(FftLen fz_rt, const cx<T> *__restrict__ tw, const cx<TIN>
     const int W = 1 << lw;
     const int nz = fz.n;
     const int tid = threadIdx.x, nthr = blockDim.x;
-    const int i0 = blockIdx.x * W;
-    const int j = j0 + blockIdx.y;
+    int chunk, jp;
+    if (!plane_chunk(blockIdx.x, nchunks, nplanes, chunk, jp)) return;
+    const int i0 = chunk * W;
+    const int j = j0 + jp;
     const int f = blockIdx.z;
     const cx<TIN> *src = in + f * in_fs + (long)j * in_ps;
     cx<T> *dst = out + f * out_fs + (long)j * out_ps;
@@ -130,7 +147,8 @@ __global__ void k_x_r2c(FftLen fx_rt, const cx<T> *__restrict__ tw, const double
 // every other slot of the plane is set to zero.
 template <typename T, int E, int FID>
 __global__ void k_zfwd_pack(FftLen fz_rt, const cx<T> *__restrict__ tw, const cx<T> *__restrict__ xt, int mk,
-                            double *__restrict__ fort, int nxp, int nkz, int tnkz, int j0, int lw)
+                            double *__restrict__ fort, int nxp, int nkz, int tnkz, int j0, int lw, int nchunks,
+                            int nplanes)
 {
     const FftLen &fz = fft_ref<FID>(fz_rt);
     extern __shared__ unsigned char smem_raw[];
@@ -138,8 +156,10 @@ __global__ void k_zfwd_pack(FftLen fz_rt, const cx<T> *__restrict__ tw, const cx
     const int W = 1 << lw;
     const int nz = fz.n;
     const int tid = threadIdx.x, nthr = blockDim.x;
-    const int i0 = blockIdx.x * W;
-    const int j = j0 + blockIdx.y;
+    int chunk, jp;
+    if (!plane_chunk(blockIdx.x, nchunks, nplanes, chunk, jp)) return;
+    const int i0 = chunk * W;
+    const int j = j0 + jp;
     const cx<T> *src = xt + (long)j * nz * mk;
     for (int idx = tid; idx < W * nz; idx += nthr) {
         const int z = idx >> lw, b = idx & (W - 1);
@@ -334,10 +354,11 @@ static void zinv_e(hipStream_t s, const Geo &g, const Fft<T> &fz, const cx<TIN> 
                    int nfields)
 {
     const int lw = chunk_lw(fz.len.t, 256), W = 1 << lw;
-    dim3 grid((ncols + W - 1) / W, nplanes, nfields);
+    const int nchunks = (ncols + W - 1) / W;
+    dim3 grid((unsigned)(8 * nchunks * ((nplanes + 7) / 8)), 1, nfields);
     const size_t lds = sizeof(cx<T>) * W * fz.len.n;
     hipLaunchKernelGGL((k_zinv<T, E, TIN, FID>), grid, dim3(W * fz.len.t), lds, s, fz.len, fz.tw, in, in_fs, in_ps, in_mk,
-                       ncols, out, out_fs, out_ps, out_mk, j0, g.nkz, g.tnkz, lw);
+                       ncols, out, out_fs, out_ps, out_mk, j0, g.nkz, g.tnkz, lw, nchunks, nplanes);
 }
 
 #define DGPU_DISPATCH_E(E_, CALL)                                  \
@@ -393,9 +414,10 @@ static void zfwd_pack_e(hipStream_t s, const Geo &g, const Fft<T> &fz, const cx<
 {
     const int lw = chunk_lw(fz.len.t, 256), W = 1 << lw;
     const int ncols = g.nx / 2 + 1;  // 0..NXP, zero beyond NKX
-    dim3 grid((ncols + W - 1) / W, nplanes);
+    const int nchunks = (ncols + W - 1) / W;
+    dim3 grid((unsigned)(8 * nchunks * ((nplanes + 7) / 8)), 1);
     hipLaunchKernelGGL((k_zfwd_pack<T, E, FID>), grid, dim3(W * fz.len.t), sizeof(cx<T>) * W * fz.len.n, s, fz.len, fz.tw,
-                       xt, g.mk, fort, g.nx / 2, g.nkz, g.tnkz, j0, lw);
+                       xt, g.mk, fort, g.nx / 2, g.nkz, g.tnkz, j0, lw, nchunks, nplanes);
 }
 
 template <typename T>
