@@ -970,8 +970,9 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(si
 k_colfused_batch(const ColDesc<T, NLT> *__restrict__ d)
 {
     const ColDesc<T, NLT> &D = d[blockIdx.z];
-    if ((long)blockIdx.x * MPW >= D.A.g.ps) return;
-    colfused_body<T, NLT, MPW, NW, RMAX, NTH>(D.A, D.R, D.P, blockIdx.x);
+    const int bx = xcd_block(blockIdx.x, gridDim.x);  // a member's mode groups in 8 runs, one per XCD
+    if ((long)bx * MPW >= D.A.g.ps) return;
+    colfused_body<T, NLT, MPW, NW, RMAX, NTH>(D.A, D.R, D.P, bx);
 }
 
 // ---------------------------------------------------------------------------
@@ -1199,7 +1200,7 @@ void launch_colfused_batch(hipStream_t s, int nw, const Geo &g, int max_ny, cons
         } else {
             with_fused_mpw<NW>(max_ny, [&](auto mc) {
                 constexpr int MPW = decltype(mc)::value;
-                const dim3 grid((g.ps + MPW - 1) / MPW, 1, nmem);
+                const dim3 grid((unsigned)(((g.ps + MPW - 1) / MPW + 7) / 8 * 8), 1, nmem);  // a multiple of 8 (xcd_block)
                 colfused_attr<T, NLT, MPW, NW>();
                 if (g.nth == 0)
                     hipLaunchKernelGGL((k_colfused_batch<T, NLT, MPW, NW, 9, 0>), grid, dim3(64 * NW), lds, s, d);

@@ -89,8 +89,9 @@ __global__ void __launch_bounds__(256) k_zinv_ip(IpLen fz, const cx<T> *__restri
                                    tnkz, lw, chunk, plane, blockIdx.z);
 }
 
-// Members of one transform-length group: grid.z = member * nfields + field,
-// planes beyond a member's own NY+2 exit at once.
+// Members of one transform-length group: grid.x over (column chunks, planes
+// of the tallest member) in plane_chunk's order, grid.z = member * nfields +
+// field; planes beyond a member's own NY+2 exit at once.
 template <typename T, int E, typename TIN, int PID>
 __global__ void __launch_bounds__(256) k_zinv_ip_batch(const ZinvDesc<T, TIN> *__restrict__ d, IpLen fz,
                                                         const cx<T> *__restrict__ tw, long in_ps, int mk,
@@ -98,9 +99,10 @@ __global__ void __launch_bounds__(256) k_zinv_ip_batch(const ZinvDesc<T, TIN> *_
 {
     const int mem = blockIdx.z / nfields, f = blockIdx.z - mem * nfields;
     const ZinvDesc<T, TIN> D = d[mem];
-    if ((int)blockIdx.y >= D.nplanes) return;
+    int chunk, plane;
+    if (!plane_chunk(blockIdx.x, (mk + (1 << lw) - 1) >> lw, D.nplanes, chunk, plane)) return;
     zinv_body<T, E, TIN, PID != 0>(plan_ref<PID>(fz), tw, D.in, D.in_fs, in_ps, mk, D.out, D.out_fs, out_ps, ld, 0,
-                                   nkz, tnkz, lw, blockIdx.x, blockIdx.y, f);
+                                   nkz, tnkz, lw, chunk, plane, f);
 }
 
 // -------------------------------------------------------------------------
@@ -211,7 +213,8 @@ __global__ void __launch_bounds__(256) k_zfwd_nl_ip(IpLen fz, const cx<T> *__res
                                     chunk, plane, blockIdx.z);
 }
 
-// grid (column chunks, planes of the tallest member, members * nout)
+// grid (column chunks x planes of the tallest member in plane_chunk's order,
+// 1, members * nout)
 template <typename T, int E, typename TOUT, int PID>
 __global__ void __launch_bounds__(256) k_zfwd_nl_ip_batch(const ZfDesc<T, TOUT> *__restrict__ d, IpLen fz,
                                                            const cx<T> *__restrict__ tw, long wt_ps, int ld, int mk,
@@ -219,9 +222,10 @@ __global__ void __launch_bounds__(256) k_zfwd_nl_ip_batch(const ZfDesc<T, TOUT> 
 {
     const int mem = blockIdx.z / nout, o = blockIdx.z - mem * nout;
     const ZfDesc<T, TOUT> D = d[mem];
-    if ((int)blockIdx.y >= D.nplanes) return;
+    int chunk, plane;
+    if (!plane_chunk(blockIdx.x, (mk + (1 << lw) - 1) >> lw, D.nplanes, chunk, plane)) return;
     zfwd_body<T, E, TOUT, PID != 0>(plan_ref<PID>(fz), tw, D.wt, D.wt_fs, wt_ps, ld, mk, D.nl, ps, D.kz, D.jlo, nkz,
-                                    tnkz, lw, blockIdx.x, blockIdx.y, o);
+                                    tnkz, lw, chunk, plane, o);
 }
 
 // -------------------------------------------------------------------------
@@ -301,7 +305,8 @@ static void zinv_ip_batch_e(hipStream_t s, const Geo &g, const IpLen &fz, const 
                             const ZinvDesc<T, TIN> *d, int nmem, int max_planes, int ld, int nfields)
 {
     const int lw = z_lw(fz.t), W = 1 << lw;
-    dim3 grid((g.mk + W - 1) / W, max_planes, nfields * nmem);
+    // plane_chunk's order over (column chunks, planes of the tallest member)
+    dim3 grid((unsigned)(8 * ((g.mk + W - 1) / W) * ((max_planes + 7) / 8)), 1, nfields * nmem);
     const size_t lds = sizeof(cx<T>) * W * fz.n;
     hipLaunchKernelGGL((k_zinv_ip_batch<T, E, TIN, PID>), grid, dim3(W * fz.t), lds, s, d, fz, tw, g.ps, g.mk,
                        (long)g.nz * ld, ld, g.nkz, g.tnkz, lw, nfields);
@@ -324,7 +329,8 @@ static void zfwd_nl_ip_batch_e(hipStream_t s, const Geo &g, const IpLen &fz, con
                                const ZfDesc<T, TOUT> *d, int nmem, int max_planes, int ld, int nout)
 {
     const int lw = z_lw(fz.t), W = 1 << lw;
-    dim3 grid((g.mk + W - 1) / W, max_planes, nout * nmem);
+    // plane_chunk's order over (column chunks, planes of the tallest member)
+    dim3 grid((unsigned)(8 * ((g.mk + W - 1) / W) * ((max_planes + 7) / 8)), 1, nout * nmem);
     const size_t lds = sizeof(cx<T>) * W * fz.n;
     hipLaunchKernelGGL((k_zfwd_nl_ip_batch<T, E, TOUT, PID>), grid, dim3(W * fz.t), lds, s, d, fz, tw,
                        (long)g.nz * ld, ld, g.mk, g.ps, g.nkz, g.tnkz, lw, nout);

@@ -46,20 +46,6 @@ __device__ __forceinline__ cx<TO> cvt(cx<FROM> a) { return make_cx<TO>(TO(a.re),
 // grid (8 * col-chunks * plane groups of 8, 1, fields), in plane_chunk's
 // order; block W*T threads; LDS W*NZ complex.
 // -------------------------------------------------------------------------
-// Block L of a plane-major grid -> (column chunk, plane): the hardware deals
-// consecutive blocks to the 8 XCDs in turn, so L & 7 picks the plane within
-// a group of 8 and every chunk of a plane runs on one XCD.  A chunk's W
-// columns are W * 16 bytes of a row that need not start on a cache line;
-// the line two chunks share is then fetched (and written back) once, from
-// that XCD's L2.  false: a block past the last plane.
-__device__ __forceinline__ bool plane_chunk(int L, int nchunks, int nplanes, int &chunk, int &plane)
-{
-    const int r = L >> 3;
-    chunk = r % nchunks;
-    plane = 8 * (r / nchunks) + (L & 7);
-    return plane < nplanes;
-}
-
 template <typename T, int E, typename TIN, int FID>
 __global__ void k_zinv(FftLen fz_rt, const cx<T> *__restrict__ tw, const cx<TIN> *__restrict__ in, long in_fs,
                        long in_ps, int in_mk, int ncols, cx<T> *__restrict__ out, long out_fs, long out_ps,

@@ -330,4 +330,19 @@ void launch_les_cf_rows(hipStream_t s, const Geo &g, const cx<double> *pre, cons
                         const cx<double> *cu3, cx<double> *cf1, cx<double> *cf3, cx<double> *cfth, const double *kx2,
                         const double *kz2, const LesCfFix &f);
 
+// Block L of a plane-major grid -> (column chunk, plane): the hardware deals
+// consecutive blocks to the 8 XCDs in turn, so L & 7 picks the plane within
+// a group of 8 and every chunk of a plane runs on one XCD.  A chunk's W
+// columns are W * 16 bytes of a row that need not start on a cache line;
+// the line two chunks share is then fetched (and written back) once, from
+// that XCD's L2.  false: a block past the last plane.
+__device__ __forceinline__ bool plane_chunk(int L, int nchunks, int nplanes, int &chunk, int &plane)
+{
+    const int r = L >> 3;
+    chunk = r % nchunks;
+    plane = 8 * (r / nchunks) + (L & 7);
+    return plane < nplanes;
+}
+
+
 }  // namespace dgpu
