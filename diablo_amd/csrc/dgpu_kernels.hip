@@ -37,6 +37,13 @@ __device__ __forceinline__ int kzslot_to_z(int s, int nkz, int tnkz, int nz)
     return s <= nkz ? s : nz - (tnkz + 1 - s);
 }
 
+// The inverse: the packed slot of transform index z, or -1 in the gap.
+__device__ __forceinline__ int z_to_kzslot(int z, int nkz, int tnkz, int nz)
+{
+    if (z <= nkz) return z;
+    return z >= nz - tnkz + nkz ? z - nz + tnkz + 1 : -1;
+}
+
 template <typename TO, typename FROM>
 __device__ __forceinline__ cx<TO> cvt(cx<FROM> a) { return make_cx<TO>(TO(a.re), TO(a.im)); }
 
@@ -65,13 +72,11 @@ __global__ void k_zinv(FftLen fz_rt, const cx<T> *__restrict__ tw, const cx<TIN>
     const cx<TIN> *src = in + f * in_fs + (long)j * in_ps;
     cx<T> *dst = out + f * out_fs + (long)j * out_ps;
     const cx<T> zero = make_cx<T>(0, 0);
-    for (int idx = tid; idx < W * nz; idx += nthr) lds[idx] = zero;
-    __syncthreads();
-    const int mz = tnkz + 1;
-    for (int idx = tid; idx < W * mz; idx += nthr) {
-        const int s = idx >> lw, b = idx & (W - 1);
-        const int i = i0 + b;
-        if (i < ncols) lds[kzslot_to_z(s, nkz, tnkz, nz) * W + b] = cvt<T>(src[(long)s * in_mk + i]);
+    // every transform index once: the kept slots, zero in the gap
+    for (int idx = tid; idx < W * nz; idx += nthr) {
+        const int z = idx >> lw, b = idx & (W - 1);
+        const int i = i0 + b, s = z_to_kzslot(z, nkz, tnkz, nz);
+        lds[idx] = (i < ncols && s >= 0) ? cvt<T>(src[(long)s * in_mk + i]) : zero;
     }
     __syncthreads();
     // batch index fastest across lanes: LDS accesses of a wavefront are contiguous
