@@ -359,7 +359,7 @@ struct TimerScope {
     {
         if (c->timing) {
             a = get_event(c);
-            if (a) hipEventRecord(a, c->stream);
+            if (a) (void)hipEventRecord(a, c->stream);
         }
     }
     ~TimerScope()
@@ -367,7 +367,7 @@ struct TimerScope {
         if (c->timing && a) {
             hipEvent_t b = get_event(c);
             if (b) {
-                hipEventRecord(b, c->stream);
+                (void)hipEventRecord(b, c->stream);
                 c->pending.push_back({kind, a, b});
             }
         }
@@ -727,24 +727,24 @@ static int to_master(dgpu_ctx *c)
 int32_t dgpu_destroy(dgpu_ctx *c)
 {
     if (!c) return DGPU_OK;
-    if (c->stream) hipStreamSynchronize(c->stream);
-    for (auto &t : c->pending) { hipEventDestroy(t.a); hipEventDestroy(t.b); }
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto &t : c->pending) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
     for (auto &ge : c->gexec)
-        if (ge) hipGraphExecDestroy(ge);
-    for (auto e : c->event_pool) hipEventDestroy(e);
+        if (ge) (void)hipGraphExecDestroy(ge);
+    for (auto e : c->event_pool) (void)hipEventDestroy(e);
     void *bufs[] = {c->d_vec, c->d_tw, c->d_state, c->d_pmult, c->d_pinv, c->d_rd, c->d_nlptr, c->d_zt, c->d_wt,
                     c->d_fa, c->d_fb, c->d_iptw, c->d_iptwf, c->d_ztf, c->d_wtf, c->d_nlf, c->d_nlptrf,
                     c->d_fort, c->d_cn, c->d_codes, c->d_c1, c->d_state32, c->d_side, c->d_map,
                     c->d_pmultf, c->d_pinvf, c->d_gs32, c->d_stats, c->d_vmax_part, c->d_vmax, c->d_file, c->d_les, c->d_zdft};
     for (void *b : bufs)
-        if (b) hipFree(b);
+        if (b) (void)hipFree(b);
     if (c->side_stream) {
-        hipStreamSynchronize(c->side_stream);
-        hipStreamDestroy(c->side_stream);
+        (void)hipStreamSynchronize(c->side_stream);
+        (void)hipStreamDestroy(c->side_stream);
     }
-    if (c->ev_fork) hipEventDestroy(c->ev_fork);
-    if (c->ev_join) hipEventDestroy(c->ev_join);
-    if (c->stream && c->own_stream) hipStreamDestroy(c->stream);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return DGPU_OK;
 }
@@ -1459,7 +1459,6 @@ static int side_waves(const dgpu_ctx *c, int ny)
 
 static int rk_enqueue(dgpu_ctx *c, int rk_step, double time, double temp1, double temp3, double temp4, double temp5)
 {
-    const int k = rk_step - 1;
     (void)temp1;
     const dgpu::Geo &g = c->g;
     const bool courant = rk_step == 3 && c->courant_state == 1;
@@ -1634,7 +1633,6 @@ static int les_substep(dgpu_ctx *c, int rk_step, double time, double temp1, doub
 {
     const dgpu::Geo &g = c->g;
     const int NY = g.ny;
-    const size_t n = les_n(c);
     const int nin = 3 + g.nth;
     const bool courant = rk_step == 3 && c->courant_state == 1;
     const double temp2 = temp4 / 2.0;  // TEMP2 = H_BAR(RK_STEP) / 2
@@ -1816,12 +1814,12 @@ int32_t dgpu_rk_chan_1(dgpu_ctx *c, int32_t rk_step, double time, const double *
         const int rc = rk_enqueue(c, rk_step, time, temp1, temp3, temp4, temp5);
         const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
         if (rc) {
-            if (graph) hipGraphDestroy(graph);
+            if (graph) (void)hipGraphDestroy(graph);
             return rc;
         }
         HIPCHK(c, ec);
         const hipError_t ei = hipGraphInstantiate(&c->gexec[k], graph, nullptr, nullptr, 0);
-        hipGraphDestroy(graph);
+        (void)hipGraphDestroy(graph);
         HIPCHK(c, ei);
         std::memcpy(c->gkey[k], key, sizeof(key));
     }
@@ -1864,7 +1862,7 @@ int32_t dgpu_autotune(dgpu_ctx *c, int32_t steps, double delta_t, int32_t *xchun
     auto time_shape = [&](double &ms_out) -> int {
         for (auto &ge : c->gexec)
             if (ge) {
-                hipGraphExecDestroy(ge);
+                (void)hipGraphExecDestroy(ge);
                 ge = nullptr;
             }
         if (int r = restore()) return r;
@@ -1931,9 +1929,9 @@ int32_t dgpu_autotune(dgpu_ctx *c, int32_t steps, double delta_t, int32_t *xchun
     const int rr = restore();
     if (rc == DGPU_OK) rc = rr;
     if (hipStreamSynchronize(c->stream) != hipSuccess && rc == DGPU_OK) rc = fail(c, DGPU_E_HIP, "autotune sync");
-    if (e0) hipEventDestroy(e0);
-    if (e1) hipEventDestroy(e1);
-    hipFree(backup);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(backup);
     if (rc) return rc;
     if (xchunk_out) *xchunk_out = c->xchunk;
     if (zdft_out) *zdft_out = c->zdft ? 1 : 0;
@@ -2400,32 +2398,32 @@ static int ens_enqueue_batched(dgpu_ensemble *e, int rk_step)
 int32_t dgpu_ensemble_destroy(dgpu_ensemble *e)
 {
     if (!e) return DGPU_OK;
-    if (e->main) hipStreamSynchronize(e->main);
+    if (e->main) (void)hipStreamSynchronize(e->main);
     if (e->side) {
-        hipStreamSynchronize(e->side);
-        hipStreamDestroy(e->side);
+        (void)hipStreamSynchronize(e->side);
+        (void)hipStreamDestroy(e->side);
     }
-    if (e->ev_f) hipEventDestroy(e->ev_f);
-    if (e->ev_j) hipEventDestroy(e->ev_j);
+    if (e->ev_f) (void)hipEventDestroy(e->ev_f);
+    if (e->ev_j) (void)hipEventDestroy(e->ev_j);
     for (auto &gr : e->groups) {
         if (gr.stream) {
-            hipStreamSynchronize(gr.stream);
-            hipStreamDestroy(gr.stream);
+            (void)hipStreamSynchronize(gr.stream);
+            (void)hipStreamDestroy(gr.stream);
         }
-        if (gr.done) hipEventDestroy(gr.done);
+        if (gr.done) (void)hipEventDestroy(gr.done);
         void *bufs[] = {gr.d_zinv, gr.d_xf, gr.d_zf, gr.d_scat, gr.d_col[0], gr.d_col[1], gr.d_col[2],
                         gr.d_cols[0], gr.d_cols[1], gr.d_cols[2]};
         for (void *b : bufs)
-            if (b) hipFree(b);
+            if (b) (void)hipFree(b);
     }
     for (auto &ge : e->gexec)
-        if (ge) hipGraphExecDestroy(ge);
+        if (ge) (void)hipGraphExecDestroy(ge);
     for (auto *c : e->m) dgpu_destroy(c);
     for (auto ev : e->join)
-        if (ev) hipEventDestroy(ev);
-    if (e->fork) hipEventDestroy(e->fork);
-    if (e->gfork) hipEventDestroy(e->gfork);
-    if (e->main) hipStreamDestroy(e->main);
+        if (ev) (void)hipEventDestroy(ev);
+    if (e->fork) (void)hipEventDestroy(e->fork);
+    if (e->gfork) (void)hipEventDestroy(e->gfork);
+    if (e->main) (void)hipStreamDestroy(e->main);
     delete e;
     return DGPU_OK;
 }
@@ -2560,7 +2558,7 @@ int32_t dgpu_ensemble_rk_chan_1(dgpu_ensemble *e, int32_t rk_step, const double 
     for (const dgpu_ctx *c : e->m) shape.insert(shape.end(), {c->xchunk, c->colnw, c->colfuse, c->zdft ? 1 : 0});
     if (!(e->gexec[k] && e->gkey[k] == key && e->gshape[k] == shape)) {
         if (e->gexec[k]) {
-            hipGraphExecDestroy(e->gexec[k]);
+            (void)hipGraphExecDestroy(e->gexec[k]);
             e->gexec[k] = nullptr;
         }
         if (hipStreamBeginCapture(e->main, hipStreamCaptureModeThreadLocal) != hipSuccess) return DGPU_E_HIP;
@@ -2570,12 +2568,12 @@ int32_t dgpu_ensemble_rk_chan_1(dgpu_ensemble *e, int32_t rk_step, const double 
         if (!rc) rc = enqueue_all(true);
         const hipError_t ec = hipStreamEndCapture(e->main, &graph);
         if (rc || ec != hipSuccess) {
-            if (graph) hipGraphDestroy(graph);
+            if (graph) (void)hipGraphDestroy(graph);
             if (!rc) e->err = std::string("graph capture: ") + hipGetErrorString(ec);
             return rc ? rc : DGPU_E_HIP;
         }
         const hipError_t ei = hipGraphInstantiate(&e->gexec[k], graph, nullptr, nullptr, 0);
-        hipGraphDestroy(graph);
+        (void)hipGraphDestroy(graph);
         if (ei != hipSuccess) {
             e->err = std::string("graph instantiate: ") + hipGetErrorString(ei);
             return DGPU_E_HIP;
@@ -2604,7 +2602,7 @@ int32_t dgpu_ensemble_autotune(dgpu_ensemble *e, int32_t steps, const double *h_
     std::vector<void *> backup(n, nullptr);
     auto free_all = [&]() {
         for (void *b : backup)
-            if (b) hipFree(b);
+            if (b) (void)hipFree(b);
     };
     for (int i = 0; i < n; ++i) {
         dgpu_ctx *c = e->m[i];
